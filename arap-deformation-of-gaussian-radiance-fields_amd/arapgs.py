@@ -1,0 +1,265 @@
+"""ctypes binding of libarapgs.so (include/arapgs.h) for Python callers.
+
+Device buffers are passed as raw pointers (``tensor.data_ptr()``); PyTorch is
+used only to own device memory and streams. There is no CPU fallback: if the
+library is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libarapgs.so")
+
+P = C.c_void_p
+_lib = None
+
+
+class ArapgsError(RuntimeError):
+    pass
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [("m", C.c_int), ("k", C.c_int), ("nbr_h", P), ("block_ptr_h", P), ("block_nodes_h", P),
+                ("block_type_h", P), ("nblocks", C.c_int), ("src_idx_h", P), ("src_w_h", P), ("node_pos_h", P),
+                ("w_rot", C.c_float), ("w_reg", C.c_float), ("w_con", C.c_float), ("weighting_factor", C.c_float),
+                ("constraints_on_center", C.c_int), ("n_aggregates", C.c_int), ("cg_max_iters", C.c_int),
+                ("cg_tol", C.c_float)]
+
+
+EXPORTS = {
+    "arapgs_create": ([C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_destroy": ([P], None),
+    "arapgs_last_error": ([P], C.c_char_p),
+    "arapgs_set_stream": ([P, P], C.c_int),
+    "arapgs_synchronize": ([P], C.c_int),
+    "arapgs_version": ([], C.c_char_p),
+    "arapgs_fps": ([P, P, C.c_int64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
+    "arapgs_knn": ([P, P, C.c_int, P, C.c_int64, C.c_int, P, P], C.c_int),
+    "arapgs_graph_edges": ([P, P, C.c_int, C.c_int, P], C.c_int),
+    "arapgs_end_points": ([P, P, P, P, C.c_int64, C.c_float, C.c_float, P], C.c_int),
+    "arapgs_solver_create": ([P, C.POINTER(SolverDesc), C.POINTER(P)], C.c_int),
+    "arapgs_solver_destroy": ([P], None),
+    "arapgs_solver_dims": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "arapgs_gn_solve": ([P, P, P, P, C.c_int, C.c_int, P, P, P], C.c_int),
+    "arapgs_gn_energy": ([P, P, P, P, P, P], C.c_int),
+    "arapgs_put_free_inputs": ([P, P, P, P, P], C.c_int),
+    "arapgs_blend_points": ([P, P, C.c_int64, P, P, C.c_int, P, C.c_int, P, P], C.c_int),
+    "arapgs_six_point_update": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float, P, P, P, P],
+                                C.c_int),
+    "arapgs_node_quats": ([P, P, C.c_int, P], C.c_int),
+    "arapgs_sample_sh": ([P, C.c_int64, C.c_int, P, P, P, P, P], C.c_int),
+    "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
+    "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
+}
+
+
+def build(force: bool = False) -> None:
+    """Compile libarapgs.so for gfx950 with hipcc (make, in place)."""
+    cmd = ["make", "-s", "-C", HERE, "-j8"]
+    if force:
+        subprocess.check_call(["make", "-s", "-C", HERE, "clean"])
+    subprocess.check_call(cmd)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ArapgsError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C {HERE}")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in EXPORTS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ArapgsError("arapgs calls take device tensors")
+    if not t.is_contiguous():
+        raise ArapgsError("arapgs calls take contiguous tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One arapgs context on a device, bound to torch's current stream."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self._torch = torch
+        h = P()
+        rc = lib().arapgs_create(device, C.byref(h))
+        if rc != 0:
+            raise ArapgsError(f"arapgs_create failed ({rc})")
+        self.h = h
+        self.device = device
+        self.set_stream(stream if stream is not None else torch.cuda.current_stream(device))
+
+    def set_stream(self, stream):
+        self.stream = stream
+        self._check(lib().arapgs_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().arapgs_last_error(self.h)
+            raise ArapgsError(f"arapgs error {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            lib().arapgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self._check(lib().arapgs_synchronize(self.h))
+
+    # ------------------------------------------------------------ graph build
+    def fps(self, pos, m):
+        t = self._torch
+        out = t.zeros(m + 1, dtype=t.int32, device=pos.device)
+        cnt = C.c_int(0)
+        self._check(lib().arapgs_fps(self.h, ptr(pos), pos.shape[0], m, ptr(out), C.byref(cnt)))
+        return out[: cnt.value]
+
+    def knn(self, nodes, queries, k, with_weights=True):
+        t = self._torch
+        Q = queries.shape[0]
+        idx = t.empty((Q, k + 1), dtype=t.int32, device=queries.device)
+        w = t.empty((Q, k), dtype=t.float32, device=queries.device) if with_weights else None
+        self._check(lib().arapgs_knn(self.h, ptr(nodes), nodes.shape[0], ptr(queries), Q, k, ptr(idx), ptr(w)))
+        return idx, w
+
+    def graph_edges(self, nodes, k):
+        t = self._torch
+        nbr = t.empty((nodes.shape[0], k), dtype=t.int32, device=nodes.device)
+        self._check(lib().arapgs_graph_edges(self.h, ptr(nodes), nodes.shape[0], k, ptr(nbr)))
+        return nbr
+
+    def end_points(self, pos, rot, scale, pad=1e-3, coeff=2.0):
+        t = self._torch
+        ends = t.empty((pos.shape[0], 6, 3), dtype=t.float32, device=pos.device)
+        self._check(lib().arapgs_end_points(self.h, ptr(pos), ptr(rot), ptr(scale), pos.shape[0], pad, coeff,
+                                            ptr(ends)))
+        return ends
+
+    # ------------------------------------------------------------ deformation
+    def blend_points(self, pts, idx, w, table, skip=None, out=None):
+        t = self._torch
+        out = t.empty_like(pts) if out is None else out
+        self._check(lib().arapgs_blend_points(self.h, ptr(pts), pts.shape[0], ptr(idx), ptr(w), idx.shape[-1],
+                                              ptr(table), table.shape[0], ptr(skip), ptr(out)))
+        return out
+
+    def six_point_update(self, table, ends_idx, ends_w, ends, scale0, is_static, pos, rot, scale, sh, pad=1e-3,
+                         coeff=2.0):
+        k = ends_idx.shape[-1]
+        self._check(lib().arapgs_six_point_update(self.h, pos.shape[0], k, ptr(table), table.shape[0], ptr(ends_idx),
+                                                  ptr(ends_w), ptr(ends), ptr(scale0), ptr(is_static), pad, coeff,
+                                                  ptr(pos), ptr(rot), ptr(scale), ptr(sh)))
+
+    def node_quats(self, table):
+        t = self._torch
+        q = t.empty((table.shape[0], 4), dtype=t.float32, device=table.device)
+        self._check(lib().arapgs_node_quats(self.h, ptr(table), table.shape[0], ptr(q)))
+        return q
+
+    def sample_sh(self, w, idx, node_q, is_static, feat):
+        self._check(lib().arapgs_sample_sh(self.h, feat.shape[0], idx.shape[-1], ptr(w), ptr(idx), ptr(node_q),
+                                           ptr(is_static), ptr(feat)))
+
+    def sh_rotate(self, R9, sh):
+        self._check(lib().arapgs_sh_rotate(self.h, sh.shape[0], ptr(R9), ptr(sh)))
+
+    def polar(self, M9):
+        t = self._torch
+        R = t.empty_like(M9)
+        S = t.empty((M9.shape[0], 3), dtype=t.float64, device=M9.device)
+        self._check(lib().arapgs_polar(self.h, M9.shape[0], ptr(M9), ptr(R), ptr(S)))
+        return R, S
+
+
+def node_table(n, device):
+    """Empty arapgs_node_xf table (n x 16 float32)."""
+    import torch
+    return torch.zeros((n, 16), dtype=torch.float32, device=device)
+
+
+class Solver:
+    """arapgs_solver for one block layout (Deform::Deform, Deform.hpp:414-455)."""
+
+    def __init__(self, ctx: Context, nbr, blocks, block_types, src_idx, src_w, node_pos, w_rot=1.0, w_reg=10.0,
+                 w_con=100.0, weighting_factor=1.0, n_aggregates=0, cg_max_iters=0, cg_tol=0.0):
+        import numpy as np
+        self.ctx = ctx
+        nbr = np.ascontiguousarray(nbr, np.uint32)
+        M, K = nbr.shape
+        ptrs = [0]
+        for b in blocks:
+            ptrs.append(ptrs[-1] + len(b))
+        self._keep = dict(
+            nbr=nbr,
+            bptr=np.ascontiguousarray(ptrs, np.int32),
+            bnodes=np.ascontiguousarray(np.concatenate([np.asarray(b, np.int32) for b in blocks]) if blocks
+                                        else np.zeros(1, np.int32), np.int32),
+            btype=np.ascontiguousarray(block_types if len(blocks) else np.zeros(1), np.int32),
+            src_idx=np.ascontiguousarray(src_idx, np.uint32),
+            src_w=np.ascontiguousarray(src_w, np.float32),
+            pos=np.ascontiguousarray(node_pos, np.float32),
+        )
+        kp = self._keep
+        d = SolverDesc(M, K, kp["nbr"].ctypes.data_as(P), kp["bptr"].ctypes.data_as(P),
+                       kp["bnodes"].ctypes.data_as(P), kp["btype"].ctypes.data_as(P), len(blocks),
+                       kp["src_idx"].ctypes.data_as(P), kp["src_w"].ctypes.data_as(P), kp["pos"].ctypes.data_as(P),
+                       w_rot, w_reg, w_con, weighting_factor, 0, n_aggregates, cg_max_iters, cg_tol)
+        h = P()
+        ctx._check(lib().arapgs_solver_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        nf, nu = C.c_int(0), C.c_int(0)
+        lib().arapgs_solver_dims(h, C.byref(nf), C.byref(nu))
+        self.n_free, self.n_unknowns = nf.value, nu.value
+        self.M, self.K = M, K
+
+    def close(self):
+        if self.h:
+            lib().arapgs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, node_pos, aims, iters, stop_rule=True, x_out=None, energies=None, iters_done=None):
+        t = self.ctx._torch
+        dev = node_pos.device
+        x_out = t.empty(self.n_unknowns, dtype=t.float32, device=dev) if x_out is None else x_out
+        energies = t.zeros(iters + 1, dtype=t.float64, device=dev) if energies is None else energies
+        iters_done = t.zeros(1, dtype=t.int32, device=dev) if iters_done is None else iters_done
+        self.ctx._check(lib().arapgs_gn_solve(self.ctx.h, self.h, ptr(node_pos), ptr(aims), iters, int(stop_rule),
+                                              ptr(x_out), ptr(energies), ptr(iters_done)))
+        return x_out, energies, iters_done
+
+    def energy(self, node_pos, aims, x):
+        t = self.ctx._torch
+        e = t.zeros(1, dtype=t.float64, device=node_pos.device)
+        self.ctx._check(lib().arapgs_gn_energy(self.ctx.h, self.h, ptr(node_pos), ptr(aims), ptr(x), ptr(e)))
+        return e
+
+    def put_free_inputs(self, node_pos, x, table=None):
+        table = node_table(self.M, node_pos.device) if table is None else table
+        self.ctx._check(lib().arapgs_put_free_inputs(self.ctx.h, self.h, ptr(node_pos), ptr(x), ptr(table)))
+        return table

@@ -1,0 +1,360 @@
+// C-ABI of the arapgs library (include/arapgs.h): context, workspaces, argument
+// checks and dispatch to the gfx950 kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/arapgs.h"
+#include "common.hpp"
+#include "solver.hpp"
+
+using namespace arapgs;
+
+struct arapgs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    ShRotTables shrot;
+    // growable workspaces
+    void* knn_list = nullptr;
+    size_t knn_list_cap = 0;
+    uint32_t* knn_count = nullptr;
+    void* fps_dist = nullptr;
+    size_t fps_dist_cap = 0;
+    unsigned long long* fps_partial = nullptr;
+    unsigned int* fps_counter = nullptr;
+    unsigned long long* fps_best = nullptr;
+    uint32_t* edge_tmp = nullptr;
+    size_t edge_tmp_cap = 0;
+};
+
+struct arapgs_solver {
+    arapgs_ctx* ctx = nullptr;
+    SolverWork w;
+};
+
+namespace {
+
+constexpr int FPS_BLOCKS = 1024;
+
+int fail(arapgs_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code < 0 ? code : -1;
+}
+
+int hip_fail(arapgs_ctx* c, hipError_t e, const char* what) {
+    return fail(c, -1000 - (int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_launch(arapgs_ctx* c, int rc, const char* what) {
+    if (rc == 0) return 0;
+    if (rc <= -1000) return fail(c, rc, "%s: HIP error %d (%s)", what, -rc - 1000, hipGetErrorString((hipError_t)(-rc - 1000)));
+    return fail(c, rc, "%s: unsupported arguments", what);
+}
+
+int grow(arapgs_ctx* c, void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return 0;
+    if (*p) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "workspace");
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) { *cap = 0; return hip_fail(c, e, "workspace allocation"); }
+    *cap = bytes;
+    return 0;
+}
+
+#define CHECK_CTX(c) \
+    if (!(c)) return -1;
+
+}  // namespace
+
+extern "C" {
+
+const char* arapgs_version(void) { return "arapgs 0.1 (gfx950)"; }
+
+int arapgs_create(int device, arapgs_ctx** out) {
+    if (!out) return -1;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -2;
+    if (device < 0 || device >= n) return -3;
+    if (hipSetDevice(device) != hipSuccess) return -4;
+    arapgs_ctx* c = new arapgs_ctx();
+    c->device = device;
+    build_shrot_tables(&c->shrot);
+    hipError_t e;
+    if ((e = hipMalloc(&c->knn_count, sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&c->fps_partial, sizeof(unsigned long long) * FPS_BLOCKS)) != hipSuccess ||
+        (e = hipMalloc(&c->fps_counter, sizeof(unsigned int))) != hipSuccess ||
+        (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess) {
+        delete c;
+        return -5;
+    }
+    (void)hipMemset(c->fps_counter, 0, sizeof(unsigned int));
+    *out = c;
+    return 0;
+}
+
+void arapgs_destroy(arapgs_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
+                    (void*)c->fps_best, (void*)c->edge_tmp})
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+const char* arapgs_last_error(const arapgs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int arapgs_set_stream(arapgs_ctx* c, void* s) {
+    CHECK_CTX(c);
+    c->stream = (hipStream_t)s;
+    return 0;
+}
+
+int arapgs_synchronize(arapgs_ctx* c) {
+    CHECK_CTX(c);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "synchronize");
+}
+
+// ------------------------------------------------------------ graph build
+int arapgs_fps(arapgs_ctx* c, const float* pos, int64_t n, int m, int* out_idx, int* count_h) {
+    CHECK_CTX(c);
+    if (!pos || !out_idx || !count_h || n <= 0 || m <= 0) return fail(c, -1, "fps: bad arguments");
+    (void)hipSetDevice(c->device);
+    const int mm = (int)std::min<int64_t>(m, n);
+    int rc = grow(c, &c->fps_dist, &c->fps_dist_cap, sizeof(float) * (size_t)n);
+    if (rc) return rc;
+    // Number of argmax steps of the reference loop (helper.cpp:157-174): m-1,
+    // except m == 1 with a first node != 0, which takes one step.
+    int first_h = 0;
+    int steps = mm - 1;
+    if (mm == 1) {
+        rc = check_launch(c, launch_fps(pos, n, 0, out_idx, (float*)c->fps_dist, c->fps_partial, FPS_BLOCKS,
+                                        c->fps_counter, c->fps_best, c->stream), "fps");
+        if (rc) return rc;
+        hipError_t e = hipMemcpyAsync(&first_h, out_idx, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "fps");
+        steps = first_h != 0 ? 1 : 0;
+        if (steps == 0) { *count_h = 1; return 0; }
+    }
+    rc = check_launch(c, launch_fps(pos, n, steps, out_idx, (float*)c->fps_dist, c->fps_partial, FPS_BLOCKS,
+                                    c->fps_counter, c->fps_best, c->stream), "fps");
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "fps");
+    *count_h = steps + 1;
+    return 0;
+}
+
+int arapgs_knn(arapgs_ctx* c, const float* nodes, int m, const float* queries, int64_t q, int k, uint32_t* out_idx,
+               float* out_w) {
+    CHECK_CTX(c);
+    if (!nodes || !queries || !out_idx || q < 0) return fail(c, -1, "knn: bad arguments");
+    if (k < 1 || k > KNN_MAX_T) return fail(c, -1, "knn: k must be in 1..12");
+    if (m < k + 1) return fail(c, -1, "knn: need at least k+1 nodes (m=%d, k=%d)", m, k);
+    if ((size_t)m * 3 * sizeof(float) > 64 * 1024) return fail(c, -1, "knn: m=%d nodes exceed the LDS stage", m);
+    if (q > 0xffffffffLL) return fail(c, -1, "knn: too many queries");
+    (void)hipSetDevice(c->device);
+    int rc = grow(c, &c->knn_list, &c->knn_list_cap, sizeof(uint32_t) * (size_t)std::max<int64_t>(q, 1));
+    if (rc) return rc;
+    return check_launch(c, launch_knn(nodes, m, queries, q, k, out_idx, out_w, (uint32_t*)c->knn_list, c->knn_count,
+                                      c->stream), "knn");
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void drop_first_kernel(const uint32_t* __restrict__ idx, int m, int k, uint32_t* __restrict__ nbr) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)m * k) return;
+    const int64_t i = e / k, j = e % k;
+    nbr[e] = idx[i * (k + 1) + j + 1];
+}
+}  // namespace
+
+extern "C" {
+
+int arapgs_graph_edges(arapgs_ctx* c, const float* nodes, int m, int k, uint32_t* out_nbr) {
+    CHECK_CTX(c);
+    if (!out_nbr) return fail(c, -1, "graph_edges: bad arguments");
+    int rc = grow(c, (void**)&c->edge_tmp, &c->edge_tmp_cap, sizeof(uint32_t) * (size_t)m * (k + 1));
+    if (rc) return rc;
+    rc = arapgs_knn(c, nodes, m, nodes, m, k, c->edge_tmp, nullptr);
+    if (rc) return rc;
+    const int64_t tot = (int64_t)m * k;
+    hipLaunchKernelGGL(drop_first_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->edge_tmp,
+                       m, k, out_nbr);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(c, e, "graph_edges");
+}
+
+int arapgs_end_points(arapgs_ctx* c, const float* pos, const float* rot, const float* scale, int64_t n, float pad,
+                      float coeff, float* ends) {
+    CHECK_CTX(c);
+    if (!pos || !rot || !scale || !ends || n < 0) return fail(c, -1, "end_points: bad arguments");
+    return check_launch(c, launch_end_points(pos, rot, scale, n, pad, coeff, ends, c->stream), "end_points");
+}
+
+// ------------------------------------------------------------ solver
+int arapgs_solver_create(arapgs_ctx* c, const arapgs_solver_desc* d, arapgs_solver** out) {
+    CHECK_CTX(c);
+    if (!d || !out || !d->nbr_h || !d->src_idx_h || !d->src_w_h || !d->node_pos_h || (d->nblocks > 0 && (!d->block_ptr_h || !d->block_nodes_h || !d->block_type_h)))
+        return fail(c, -1, "solver_create: bad arguments");
+    *out = nullptr;
+    (void)hipSetDevice(c->device);
+    arapgs_solver* s = new arapgs_solver();
+    s->ctx = c;
+    SolverDesc sd;
+    sd.M = d->m; sd.K = d->k; sd.nbr = d->nbr_h; sd.block_ptr = d->block_ptr_h; sd.block_nodes = d->block_nodes_h;
+    sd.block_type = d->block_type_h; sd.nblocks = d->nblocks; sd.src_idx = d->src_idx_h; sd.src_w = d->src_w_h;
+    sd.node_pos = d->node_pos_h; sd.w_rot = d->w_rot; sd.w_reg = d->w_reg; sd.w_con = d->w_con;
+    sd.wf = d->weighting_factor; sd.center = d->constraints_on_center; sd.n_agg = d->n_aggregates;
+    std::string err;
+    int rc = solver_build(sd, s->w, err);
+    if (rc) {
+        delete s;
+        return fail(c, rc, "%s", err.c_str());
+    }
+    if (d->cg_max_iters > 0) s->w.cg_max_iters = d->cg_max_iters;
+    if (d->cg_tol > 0.0f) s->w.cg_tol = d->cg_tol;
+    *out = s;
+    return 0;
+}
+
+void arapgs_solver_destroy(arapgs_solver* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipDeviceSynchronize();
+    solver_free(s->w);
+    delete s;
+}
+
+int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h) {
+    if (!s) return -1;
+    if (n_free_h) *n_free_h = s->w.dev.nF;
+    if (n_unknowns_h) *n_unknowns_h = s->w.n;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void copy_f32_kernel(float* __restrict__ dst, const float* __restrict__ src, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+__global__ void copy_done_kernel(const arapgs::GnScalars* __restrict__ gs, int* __restrict__ out) { *out = gs->iters_done; }
+}  // namespace
+
+extern "C" {
+
+int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* aims, int iters,
+                    int stop_rule, float* x_out, double* energies, int* iters_done) {
+    CHECK_CTX(c);
+    if (!s || !node_pos || !aims || !energies || iters < 0) return fail(c, -1, "gn_solve: bad arguments");
+    SolverWork& W = s->w;
+    W.dev.g = node_pos;
+    W.dev.aim = aims;
+    int rc = check_launch(c, solver_frame_setup(W, c->stream), "gn_solve setup");
+    if (rc) return rc;
+    for (int it = 0; it < iters; it++) {
+        rc = check_launch(c, solver_gn_begin(W, c->stream), "gn_solve rhs");
+        if (rc) return rc;
+        for (int k = 0; k < W.cg_max_iters; k++) {
+            rc = check_launch(c, solver_cg_iteration(W, c->stream), "gn_solve cg");
+            if (rc) return rc;
+        }
+        rc = check_launch(c, solver_gn_end(W, it, stop_rule, energies, c->stream), "gn_solve line search");
+        if (rc) return rc;
+    }
+    if (iters == 0) {
+        hipError_t e = hipMemcpyAsync(energies, W.cand_e, sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
+    }
+    if (x_out)
+        hipLaunchKernelGGL(copy_f32_kernel, dim3(64), dim3(256), 0, c->stream, x_out, (const float*)W.x, W.n);
+    if (iters_done) hipLaunchKernelGGL(copy_done_kernel, dim3(1), dim3(1), 0, c->stream, W.gs, iters_done);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
+}
+
+int arapgs_gn_energy(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
+                     double* energy) {
+    CHECK_CTX(c);
+    if (!s || !node_pos || !aims || !x || !energy) return fail(c, -1, "gn_energy: bad arguments");
+    s->w.dev.g = node_pos;
+    s->w.dev.aim = aims;
+    return check_launch(c, solver_energy(s->w, x, energy, c->stream), "gn_energy");
+}
+
+int arapgs_put_free_inputs(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* x,
+                           arapgs_node_xf* table) {
+    CHECK_CTX(c);
+    if (!s || !node_pos || !x || !table) return fail(c, -1, "put_free_inputs: bad arguments");
+    return check_launch(c, launch_build_node_table(s->w.dev.M, node_pos, x, s->w.dev.free_id, (NodeXf*)table,
+                                                   c->stream), "put_free_inputs");
+}
+
+// ------------------------------------------------------------ deformation
+int arapgs_blend_points(arapgs_ctx* c, const float* pts, int64_t p, const uint32_t* idx, const float* w, int k,
+                        const arapgs_node_xf* table, int m, const uint8_t* skip, float* out) {
+    CHECK_CTX(c);
+    if (!pts || !idx || !w || !table || !out || p < 0 || k < 1 || k > KNN_MAX_T)
+        return fail(c, -1, "blend_points: bad arguments");
+    return check_launch(c, launch_blend_points(pts, p, idx, w, k, (const NodeXf*)table, m, skip, out, c->stream),
+                        "blend_points");
+}
+
+int arapgs_six_point_update(arapgs_ctx* c, int64_t n, int k, const arapgs_node_xf* table, int m,
+                            const uint32_t* ends_idx, const float* ends_w, float* ends, const float* scale0,
+                            const uint8_t* is_static, float pad, float coeff, float* pos, float* rot, float* scale,
+                            float* sh) {
+    CHECK_CTX(c);
+    if (!table || !ends_idx || !ends_w || !ends || !scale0 || !pos || !rot || !scale || !sh || n < 0 || k < 1 ||
+        k > KNN_MAX_T)
+        return fail(c, -1, "six_point_update: bad arguments");
+    return check_launch(c, launch_six_point(n, k, (const NodeXf*)table, m, ends_idx, ends_w, ends, scale0, is_static,
+                                            pad, coeff, pos, rot, scale, sh, c->shrot, c->stream),
+                        "six_point_update");
+}
+
+int arapgs_node_quats(arapgs_ctx* c, const arapgs_node_xf* table, int m, float* q) {
+    CHECK_CTX(c);
+    if (!table || !q || m < 0) return fail(c, -1, "node_quats: bad arguments");
+    return check_launch(c, launch_node_quats(m, (const NodeXf*)table, q, c->stream), "node_quats");
+}
+
+int arapgs_sample_sh(arapgs_ctx* c, int64_t s, int k, const float* w, const uint32_t* idx, const float* node_q,
+                     const uint8_t* is_static, float* feat) {
+    CHECK_CTX(c);
+    if (!w || !idx || !node_q || !feat || s < 0 || k < 1 || k > KNN_MAX_T) return fail(c, -1, "sample_sh: bad arguments");
+    return check_launch(c, launch_sample_sh(s, k, w, idx, node_q, is_static, feat, c->shrot, c->stream), "sample_sh");
+}
+
+int arapgs_sh_rotate(arapgs_ctx* c, int64_t n, const float* r9, float* sh) {
+    CHECK_CTX(c);
+    if (!r9 || !sh || n < 0) return fail(c, -1, "sh_rotate: bad arguments");
+    return check_launch(c, launch_sh_rotate(n, r9, sh, c->shrot, c->stream), "sh_rotate");
+}
+
+int arapgs_polar(arapgs_ctx* c, int64_t n, const double* m9, double* r9, double* sdiag) {
+    CHECK_CTX(c);
+    if (!m9 || !r9 || !sdiag || n < 0) return fail(c, -1, "polar: bad arguments");
+    return check_launch(c, launch_polar(n, m9, r9, sdiag, c->stream), "polar");
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Shared host/device declarations of the arapgs HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_math.hpp"
+
+#define KNN_MAX_T 12  // KNN_MAX of helper.hpp:48
+
+#define HIP_RET(expr)                              \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return -(int)_e - 1000; \
+    } while (0)
+
+namespace arapgs {
+
+// ----------------------------------------------------------------- knn.hip
+int launch_knn(const float* nodes, int M, const float* queries, int64_t Q, int k, uint32_t* out_idx, float* out_w,
+               uint32_t* scratch_list, uint32_t* scratch_count, hipStream_t st);
+int launch_fps(const float* pos, int64_t N, int m_steps, int* out, float* dist, unsigned long long* partial,
+               int n_partial, unsigned int* counter, unsigned long long* best, hipStream_t st);
+
+// ----------------------------------------------------------------- deform.hip
+// Per-node deformation table: g (current position), R (row-major), t.
+struct NodeXf {
+    float g[3];
+    float R[9];
+    float t[3];
+    float pad;
+};
+static_assert(sizeof(NodeXf) == 64, "NodeXf is one 64-byte record");
+
+int launch_end_points(const float* pos, const float* rot, const float* scale, int64_t N, float pad, float coeff,
+                      float* ends, hipStream_t st);
+int launch_build_node_table(int M, const float* node_pos, const float* x_free, const int* free_id, NodeXf* table,
+                            hipStream_t st);
+int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const float* w, int k, const NodeXf* table,
+                        int M, const uint8_t* skip, float* out, hipStream_t st);
+int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
+                     float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
+                     float* rot, float* scale, float* sh, const ShRotTables& T, hipStream_t st);
+int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st);
+int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
+                     const uint8_t* is_static, float* feat, const ShRotTables& T, hipStream_t st);
+int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables& T, hipStream_t st);
+int launch_polar(int64_t n, const double* M9, double* R9, double* Sdiag, hipStream_t st);
+
+void build_shrot_tables(ShRotTables* T);
+
+}  // namespace arapgs

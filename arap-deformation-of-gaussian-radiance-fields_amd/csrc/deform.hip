@@ -1,0 +1,417 @@
+// Per-frame deformation of Gaussians, end points, nodes and samples (gfx950).
+//
+//   * end points (GaussianView.cpp:4643-4668)
+//   * node transform table from the solved unknowns (DeformGraph::putFreeInputs,
+//     Deform.hpp:140-151)
+//   * K-node blend p' = sum_j w_j (R_j (p - g_j) + t_j + g_j) (predict_mesh /
+//     predict_samples, Deform.hpp:230-268) for samples and nodes
+//   * the fused per-Gaussian update: blend of the 6 end points, FP64 polar of
+//     M = [(e0+ - e0-)/2 | (e1+ - e1-)/2 | (e2+ - e2-)/2], quaternion, scales,
+//     centre and SH rotation (UpdatePosition + UpdateAsSixPointsWithdrawBad,
+//     GaussianView.cpp:2986-3166)
+//   * node quaternions and the per-sample slerp-blend SH rotation
+//     (FastUpdateSamplesSH + RotateSHs, GaussianView.cpp:3169-3186,
+//     cudakdtree.cu:201-222)
+//
+// Node tables are 64-byte records gathered through L2 (a frame's table is at
+// most a few hundred KB); all per-point streams are read once and written once.
+#include "common.hpp"
+
+namespace arapgs {
+
+namespace {
+
+// One blended point with the arithmetic of Deform.hpp:238-246: p - g_j in
+// float, the term w_j (R_j(p - g_j) + t_j + g_j) evaluated in double left to
+// right, and a float accumulator updated once per neighbour.
+__device__ __forceinline__ void blend_point(const NodeXf* __restrict__ table, const uint32_t* __restrict__ idx,
+                                            const float* __restrict__ w, int k, const float p[3], float o[3]) {
+#pragma clang fp contract(off)
+    o[0] = o[1] = o[2] = 0.0f;
+    for (int j = 0; j < k; j++) {
+        const NodeXf& n = table[idx[j]];
+        const double wj = (double)w[j];
+        const float tx = p[0] - n.g[0], ty = p[1] - n.g[1], tz = p[2] - n.g[2];
+        o[0] = (float)((double)o[0] + wj * ((double)n.R[0] * tx + (double)n.R[1] * ty + (double)n.R[2] * tz + (double)n.t[0] + (double)n.g[0]));
+        o[1] = (float)((double)o[1] + wj * ((double)n.R[3] * tx + (double)n.R[4] * ty + (double)n.R[5] * tz + (double)n.t[1] + (double)n.g[1]));
+        o[2] = (float)((double)o[2] + wj * ((double)n.R[6] * tx + (double)n.R[7] * ty + (double)n.R[8] * tz + (double)n.t[2] + (double)n.g[2]));
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void blend_point_k(const NodeXf* __restrict__ table, const uint32_t* __restrict__ idx,
+                                              const float* __restrict__ w, const float p[3], float o[3]) {
+#pragma clang fp contract(off)
+    uint32_t id[K];
+    float wt[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) { id[j] = idx[j]; wt[j] = w[j]; }
+    o[0] = o[1] = o[2] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const float4* r = reinterpret_cast<const float4*>(table + id[j]);
+        const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+        // a = g0 g1 g2 R0 | b = R1..R4 | c = R5..R8 | d = t0 t1 t2 pad
+        const float tx = p[0] - a.x, ty = p[1] - a.y, tz = p[2] - a.z;
+        const double wj = (double)wt[j];
+        o[0] = (float)((double)o[0] + wj * ((double)a.w * tx + (double)b.x * ty + (double)b.y * tz + (double)d.x + (double)a.x));
+        o[1] = (float)((double)o[1] + wj * ((double)b.z * tx + (double)b.w * ty + (double)c.x * tz + (double)d.y + (double)a.y));
+        o[2] = (float)((double)o[2] + wj * ((double)c.y * tx + (double)c.z * ty + (double)c.w * tz + (double)d.z + (double)a.z));
+    }
+}
+
+__global__ void end_points_kernel(const float* __restrict__ pos, const float* __restrict__ rot,
+                                  const float* __restrict__ scale, int64_t N, float pad, float coeff,
+                                  float* __restrict__ ends) {
+#pragma clang fp contract(off)
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float q[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
+    quat_normalize(q);
+    float R[3][3];
+    quat_to_mat(q[0], q[1], q[2], q[3], R);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float len = (scale[3 * g + i] + pad) * coeff;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float off = R[c][i] * len;
+            ends[g * 18 + i * 6 + c] = pos[3 * g + c] + off;
+            ends[g * 18 + i * 6 + 3 + c] = pos[3 * g + c] - off;
+        }
+    }
+}
+
+__global__ void node_table_kernel(int M, const float* __restrict__ node_pos, const float* __restrict__ x,
+                                  const int* __restrict__ free_id, NodeXf* __restrict__ table) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= M) return;
+    NodeXf r;
+    r.g[0] = node_pos[3 * n]; r.g[1] = node_pos[3 * n + 1]; r.g[2] = node_pos[3 * n + 2];
+    const int f = free_id ? free_id[n] : -1;
+    if (f >= 0) {
+        const float* xf = x + 12 * (int64_t)f;
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) r.R[rr * 3 + c] = xf[3 * c + rr];  // x is column-major
+        r.t[0] = xf[9]; r.t[1] = xf[10]; r.t[2] = xf[11];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) r.R[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+        r.t[0] = r.t[1] = r.t[2] = 0.0f;
+    }
+    r.pad = 0.0f;
+    table[n] = r;
+}
+
+template <int K>
+__global__ void __launch_bounds__(256) blend_points_kernel(const float* __restrict__ pts, int64_t P,
+                                                           const uint32_t* __restrict__ idx,
+                                                           const float* __restrict__ w,
+                                                           const NodeXf* __restrict__ table,
+                                                           const uint8_t* __restrict__ skip,
+                                                           float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    if (skip && skip[i]) {
+        out[3 * i] = p[0]; out[3 * i + 1] = p[1]; out[3 * i + 2] = p[2];
+        return;
+    }
+    float o[3];
+    if (K > 0) blend_point_k<K>(table, idx + i * K, w + i * K, p, o);
+    out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
+}
+
+__global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* __restrict__ pts, int64_t P,
+                                                                   const uint32_t* __restrict__ idx,
+                                                                   const float* __restrict__ w, int k,
+                                                                   const NodeXf* __restrict__ table,
+                                                                   const uint8_t* __restrict__ skip,
+                                                                   float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    if (skip && skip[i]) {
+        out[3 * i] = p[0]; out[3 * i + 1] = p[1]; out[3 * i + 2] = p[2];
+        return;
+    }
+    float o[3];
+    blend_point(table, idx + i * k, w + i * k, k, p, o);
+    out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
+}
+
+// The fused per-Gaussian update. Static Gaussians (every end-point neighbour a
+// static node) still blend their end points, as the reference does, but keep
+// their attributes.
+template <int K>
+__global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf* __restrict__ table,
+                                                        const uint32_t* __restrict__ ends_idx,
+                                                        const float* __restrict__ ends_w, float* __restrict__ ends,
+                                                        const float* __restrict__ scale0,
+                                                        const uint8_t* __restrict__ is_static, float pad, float coeff,
+                                                        float* __restrict__ pos, float* __restrict__ rot,
+                                                        float* __restrict__ scale, float* __restrict__ sh,
+                                                        const ShRotTables T) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float e[6][3];
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        float in[3] = {ends[g * 18 + p * 3], ends[g * 18 + p * 3 + 1], ends[g * 18 + p * 3 + 2]};
+        blend_point_k<K>(table, ends_idx + (g * 6 + p) * K, ends_w + (g * 6 + p) * K, in, e[p]);
+    }
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) ends[g * 18 + p * 3 + c] = e[p][c];
+    if (is_static && is_static[g]) return;
+
+    float center[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int p = 0; p < 6; p++) sum += e[p][c];
+        center[c] = sum / 6.0f;
+    }
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) M[c][i] = 0.5 * ((double)e[2 * i][c] - (double)e[2 * i + 1][c]);
+    double Rd[3][3], Sd[3];
+    polar3_fp64(M, Rd, Sd);
+    float Rf[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Rf[i][j] = (float)Rd[i][j];
+    float q[4];
+    mat_to_quat(Rf, q);
+    quat_normalize(q);
+    const float4 qo = *reinterpret_cast<const float4*>(rot + 4 * g);
+    *reinterpret_cast<float4*>(rot + 4 * g) = make_float4(q[0], q[1], q[2], q[3]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float s0 = scale0[3 * g + i];
+        scale[3 * g + i] = (float)Sd[i] / ((s0 + pad) * coeff) * s0;
+        pos[3 * g + i] = center[i];
+    }
+    // relative rotation q * q_old^-1, normalised
+    const float n2 = qo.x * qo.x + qo.y * qo.y + qo.z * qo.z + qo.w * qo.w;
+    const float qi[4] = {qo.x / n2, -qo.y / n2, -qo.z / n2, -qo.w / n2};
+    float qr[4];
+    quat_mul(q, qi, qr);
+    quat_normalize(qr);
+    float Rr[3][3];
+    quat_to_mat(qr[0], qr[1], qr[2], qr[3], Rr);
+    float c[48];
+    const float4* s4 = reinterpret_cast<const float4*>(sh + g * 48);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 v = s4[i];
+        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+    }
+    sh_rotate_resample(T, Rr, c);
+    float4* o4 = reinterpret_cast<float4*>(sh + g * 48);
+#pragma unroll
+    for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
+}
+
+// Node rotation -> orthogonal polar factor -> unit quaternion (w,x,y,z).
+__global__ void node_quats_kernel(int M, const NodeXf* __restrict__ table, float* __restrict__ q) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= M) return;
+    double A[3][3];
+#pragma unroll
+    for (int i = 0; i < 9; i++) A[i / 3][i % 3] = (double)table[n].R[i];
+    double Rd[3][3], Sd[3];
+    polar3_fp64(A, Rd, Sd);
+    float Rf[3][3];
+#pragma unroll
+    for (int i = 0; i < 9; i++) Rf[i / 3][i % 3] = (float)Rd[i / 3][i % 3];
+    float qq[4];
+    mat_to_quat(Rf, qq);
+    quat_normalize(qq);
+    *reinterpret_cast<float4*>(q + 4 * n) = make_float4(qq[0], qq[1], qq[2], qq[3]);
+}
+
+// Running slerp from identity towards each neighbour node's quaternion with
+// t = w_j / (w_j + sum of previous weights) (Q_SlerpCUDA, cudakdtree.cu:113-148),
+// then SH rotation of the sample's 48-float feature.
+template <int K>
+__global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* __restrict__ w,
+                                                        const uint32_t* __restrict__ idx,
+                                                        const float* __restrict__ node_q,
+                                                        const uint8_t* __restrict__ is_static,
+                                                        float* __restrict__ feat, const ShRotTables T) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    if (is_static && is_static[s]) return;
+    float acc[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    float last = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const float cw = w[s * K + j];
+        const float t = cw / (cw + last);
+        const float4 e4 = *reinterpret_cast<const float4*>(node_q + 4 * (int64_t)idx[s * K + j]);
+        float e[4] = {e4.x, e4.y, e4.z, e4.w};
+        float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
+        if (cs < 0.0f) {
+            e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3];
+            cs = -cs;
+        }
+        float ra, rb;
+        if (cs > 0.99995f) {
+            ra = 1.0f - t;
+            rb = t;
+        } else {
+            const float sn = __builtin_sqrtf(1.0f - cs * cs);
+            const float ang = atan2f(sn, cs);
+            ra = sinf((1.0f - t) * ang) / sn;
+            rb = sinf(t * ang) / sn;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] = ra * acc[c] + rb * e[c];
+        quat_normalize(acc);
+        last += cw;
+    }
+    quat_normalize(acc);
+    float R[3][3];
+    quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
+    float c[48];
+    const float4* s4 = reinterpret_cast<const float4*>(feat + s * 48);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 v = s4[i];
+        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+    }
+    sh_rotate_resample(T, R, c);
+    float4* o4 = reinterpret_cast<float4*>(feat + s * 48);
+#pragma unroll
+    for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
+}
+
+__global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh, const ShRotTables T) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float R[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) R[k / 3][k % 3] = R9[9 * i + k];
+    float c[48];
+#pragma unroll
+    for (int k = 0; k < 48; k++) c[k] = sh[48 * i + k];
+    sh_rotate_resample(T, R, c);
+#pragma unroll
+    for (int k = 0; k < 48; k++) sh[48 * i + k] = c[k];
+}
+
+__global__ void polar_kernel(int64_t n, const double* __restrict__ M9, double* __restrict__ R9,
+                             double* __restrict__ Sd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double M[3][3], R[3][3], S[3];
+#pragma unroll
+    for (int k = 0; k < 9; k++) M[k / 3][k % 3] = M9[9 * i + k];
+    polar3_fp64(M, R, S);
+#pragma unroll
+    for (int k = 0; k < 9; k++) R9[9 * i + k] = R[k / 3][k % 3];
+    Sd[3 * i] = S[0]; Sd[3 * i + 1] = S[1]; Sd[3 * i + 2] = S[2];
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+int launch_end_points(const float* pos, const float* rot, const float* scale, int64_t N, float pad, float coeff,
+                      float* ends, hipStream_t st) {
+    if (N == 0) return 0;
+    hipLaunchKernelGGL(end_points_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pos, rot, scale, N, pad, coeff, ends);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_build_node_table(int M, const float* node_pos, const float* x_free, const int* free_id, NodeXf* table,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(node_table_kernel, dim3(nblk(M, 256)), dim3(256), 0, st, M, node_pos, x_free, free_id, table);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const float* w, int k, const NodeXf* table,
+                        int M, const uint8_t* skip, float* out, hipStream_t st) {
+    (void)M;
+    if (P == 0) return 0;
+    if (k == 8)
+        hipLaunchKernelGGL(blend_points_kernel<8>, dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, table, skip,
+                           out);
+    else
+        hipLaunchKernelGGL(blend_points_generic_kernel, dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, k,
+                           table, skip, out);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
+                     float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
+                     float* rot, float* scale, float* sh, const ShRotTables& T, hipStream_t st) {
+    (void)M;
+    if (N == 0) return 0;
+    switch (k) {
+#define ARAPGS_SP_CASE(KV)                                                                                         \
+    case KV:                                                                                                       \
+        hipLaunchKernelGGL(six_point_kernel<KV>, dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx, ends_w, \
+                           ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);                          \
+        break;
+        ARAPGS_SP_CASE(1) ARAPGS_SP_CASE(2) ARAPGS_SP_CASE(3) ARAPGS_SP_CASE(4) ARAPGS_SP_CASE(5) ARAPGS_SP_CASE(6)
+        ARAPGS_SP_CASE(7) ARAPGS_SP_CASE(8) ARAPGS_SP_CASE(9) ARAPGS_SP_CASE(10) ARAPGS_SP_CASE(11) ARAPGS_SP_CASE(12)
+#undef ARAPGS_SP_CASE
+        default:
+            return -1;
+    }
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st) {
+    hipLaunchKernelGGL(node_quats_kernel, dim3(nblk(M, 128)), dim3(128), 0, st, M, table, q);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
+                     const uint8_t* is_static, float* feat, const ShRotTables& T, hipStream_t st) {
+    if (S == 0) return 0;
+    switch (k) {
+#define ARAPGS_SS_CASE(KV)                                                                                        \
+    case KV:                                                                                                      \
+        hipLaunchKernelGGL(sample_sh_kernel<KV>, dim3(nblk(S, 256)), dim3(256), 0, st, S, w, idx, node_q, is_static, \
+                           feat, T);                                                                              \
+        break;
+        ARAPGS_SS_CASE(1) ARAPGS_SS_CASE(2) ARAPGS_SS_CASE(3) ARAPGS_SS_CASE(4) ARAPGS_SS_CASE(5) ARAPGS_SS_CASE(6)
+        ARAPGS_SS_CASE(7) ARAPGS_SS_CASE(8) ARAPGS_SS_CASE(9) ARAPGS_SS_CASE(10) ARAPGS_SS_CASE(11) ARAPGS_SS_CASE(12)
+#undef ARAPGS_SS_CASE
+        default:
+            return -1;
+    }
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables& T, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(sh_rotate_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, n, R9, sh, T);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_polar(int64_t n, const double* M9, double* R9, double* Sd, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(polar_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, n, M9, R9, Sd);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+}  // namespace arapgs

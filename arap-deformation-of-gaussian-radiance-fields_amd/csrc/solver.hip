@@ -1,0 +1,1026 @@
+// Embedded-deformation ("ARAP") Gauss-Newton on gfx950 (Deform.cpp:95-581).
+//
+// Unknowns: 12 per free node, x[12f + 3c + r] = R(r,c), x[12f + 9 + r] = t_r.
+// Residual groups (each a few rows, sqrt(w)-weighted):
+//   rot(i)      6 rows  {c0.c1, c0.c2, c1.c2, |c0|^2-1, |c1|^2-1, |c2|^2-1}
+//   reg(i,t)    3 rows  s_ik (R_i d + g_i + t_i - g_k - t_k), d = g_k - g_i
+//   static(i,t) 3 rows  -t_k   (static i, free k)
+//   con(c)      3 rows  sum_j w_j (free j ? R_j (v - g_j) + g_j + t_j : v) - aim_c
+// The Jacobian keeps the reference's asymmetry: d reg / d t_i carries no s_ik
+// (Deform.cpp:257 vs :261).
+//
+// J^T J is never assembled. Products are applied matrix-free in the adjoint
+// form: for every free node ("owner") a 16-lane group walks the residual groups
+// that touch it — rot, its out-edges, its in-edges and the constraints whose
+// source vertex uses it — recomputes J_g p and accumulates J_{g,i}^T (J_g p).
+// The linear system is solved by PCG (FP32 vectors, FP64 dots) with a two-level
+// additive preconditioner: exact 12x12 block-Jacobi plus a coarse space of one
+// affine map per node aggregate (R_j = A_a, t_j = A_a (g_j - c_a) + b_a),
+// whose 12n_a x 12n_a Galerkin matrix is assembled and inverted once per frame.
+// The step-halving line search evaluates all 50 halvings of the reference
+// (alpha = 1 .. 2^-49) in one pass and takes the first with E <= E_old.
+#include "common.hpp"
+#include "solver.hpp"
+
+namespace arapgs {
+
+namespace {
+
+constexpr int LANES = 16;  // lanes per owner node
+
+__device__ __forceinline__ float xR(const float* x, int r, int c) { return x[3 * c + r]; }
+
+// ---------------------------------------------------------------- residuals
+// Residual of one group at unknowns x (+ alpha*h when h != nullptr). Returns
+// the number of rows written to f (<= 6).
+__device__ int group_residual(const SolverDev& S, const float* __restrict__ x, const float* __restrict__ h,
+                              float alpha, int64_t g, float f[6]) {
+    auto X = [&](int node_free, int comp) -> float {
+        const int64_t o = 12 * (int64_t)node_free + comp;
+        return h ? x[o] + alpha * h[o] : x[o];
+    };
+    if (g < S.nF) {
+        const int fi = (int)g;
+        float c[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) c[j][r] = X(fi, 3 * j + r);
+        auto dot = [&](int a, int b) { return c[a][0] * c[b][0] + c[a][1] * c[b][1] + c[a][2] * c[b][2]; };
+        f[0] = S.sw_rot * dot(0, 1);
+        f[1] = S.sw_rot * dot(0, 2);
+        f[2] = S.sw_rot * dot(1, 2);
+        f[3] = S.sw_rot * (dot(0, 0) - 1.0f);
+        f[4] = S.sw_rot * (dot(1, 1) - 1.0f);
+        f[5] = S.sw_rot * (dot(2, 2) - 1.0f);
+        return 6;
+    }
+    g -= S.nF;
+    if (g < (int64_t)S.nF * S.K) {
+        const int fi = (int)(g / S.K), t = (int)(g % S.K);
+        const int i = S.free_nodes[fi];
+        const int k = (int)S.nbr[(int64_t)i * S.K + t];
+        const int fk = S.free_id[k];
+        const float s = S.edge_scale[(int64_t)i * S.K + t];
+        const float* gi = S.g + 3 * i;
+        const float* gk = S.g + 3 * k;
+        const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const float tk = fk >= 0 ? X(fk, 9 + r) : 0.0f;
+            const float v = X(fi, r) * d[0] + X(fi, 3 + r) * d[1] + X(fi, 6 + r) * d[2] + gi[r] + X(fi, 9 + r) -
+                            gk[r] - tk;
+            f[r] = S.sw_reg * v * s;
+        }
+        return 3;
+    }
+    g -= (int64_t)S.nF * S.K;
+    if (g < (int64_t)S.nS * S.K) {
+        const int si = (int)(g / S.K), t = (int)(g % S.K);
+        const int i = S.static_nodes[si];
+        const int k = (int)S.nbr[(int64_t)i * S.K + t];
+        const int fk = S.free_id[k];
+#pragma unroll
+        for (int r = 0; r < 3; r++) f[r] = S.sw_reg * -(fk >= 0 ? X(fk, 9 + r) : 0.0f);
+        return 3;
+    }
+    g -= (int64_t)S.nS * S.K;
+    // constraint
+    const int cid = (int)g;
+    const int c = S.con_node[cid];
+    const float v[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
+    float nv[3] = {0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < S.K; j++) {
+        const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+        const float w = S.src_w[(int64_t)c * S.K + j];
+        const int fj = S.free_id[nj];
+        if (fj < 0) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) nv[r] += w * v[r];
+        } else {
+            const float* gj = S.g + 3 * nj;
+            const float dd[3] = {v[0] - gj[0], v[1] - gj[1], v[2] - gj[2]};
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                nv[r] += w * (X(fj, r) * dd[0] + X(fj, 3 + r) * dd[1] + X(fj, 6 + r) * dd[2] + gj[r] + X(fj, 9 + r));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) f[r] = S.sw_con * (nv[r] - S.aim[3 * c + r]);
+    return 3;
+}
+
+__device__ __forceinline__ double block_reduce_d(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < nw; w++) r += sh[w];
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+// Energies of x + 2^-c h for c in [0, NC) (h == nullptr: the energy of x).
+// blockIdx.y = c. Per-block FP64 partials, reduced by reduce_partials_kernel.
+__global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* __restrict__ x,
+                                                     const float* __restrict__ h, double* __restrict__ partial,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[8];
+    const int c = blockIdx.y;
+    const float alpha = ldexpf(1.0f, -c);
+    double acc = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        float f[6];
+        const int nr = group_residual(S, x, h, alpha, g, f);
+        for (int r = 0; r < nr; r++) acc += (double)f[r] * (double)f[r];
+    }
+    const double t = block_reduce_d(acc, sh);
+    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int ncand,
+                                       double* __restrict__ out, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[8];
+    for (int c = 0; c < ncand; c++) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partial[(int64_t)c * nblocks + b];
+        const double t = block_reduce_d(acc, sh);
+        if (threadIdx.x == 0) out[c] = t;
+    }
+}
+
+// ---------------------------------------------------------------- owner walk
+// Visit every residual group touching free node fi with its Jacobian block
+// wrt fi and the group's J*v for a given vector v (12 per free node):
+//   fn(rows, Ji[rows][12], Jv[rows]).
+// Items: 0 = rot, 1..K = out-edges, then in-edges, then constraint slots.
+template <typename Fn>
+__device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item, const float* __restrict__ x,
+                                           const float* __restrict__ v, Fn&& fn) {
+    const int i = S.free_nodes[fi];
+    float Ji[6][12];
+    float Jv[6];
+#pragma unroll
+    for (int a = 0; a < 6; a++) {
+        Jv[a] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < 12; b++) Ji[a][b] = 0.0f;
+    }
+    if (item == 0) {
+        // rot(i): d(c_a . c_b)/d c_a = c_b, d|c_j|^2 / d c_j = 2 c_j
+        const float* xi = x + 12 * (int64_t)fi;
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                Ji[p][3 * pa[p] + c] = S.sw_rot * xi[3 * pb[p] + c];
+                Ji[p][3 * pb[p] + c] += S.sw_rot * xi[3 * pa[p] + c];
+            }
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) Ji[3 + j][3 * j + c] = 2.0f * S.sw_rot * xi[3 * j + c];
+        const float* vi = v + 12 * (int64_t)fi;
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 9; b++) s += Ji[a][b] * vi[b];
+            Jv[a] = s;
+        }
+        fn(6, Ji, Jv);
+        return;
+    }
+    item -= 1;
+    if (item < S.K) {
+        const int t = item;
+        const int k = (int)S.nbr[(int64_t)i * S.K + t];
+        const int fk = S.free_id[k];
+        const float s = S.edge_scale[(int64_t)i * S.K + t];
+        const float* gi = S.g + 3 * i;
+        const float* gk = S.g + 3 * k;
+        const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
+        const float* vi = v + 12 * (int64_t)fi;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) Ji[r][3 * c + r] = S.sw_reg * s * d[c];
+            Ji[r][9 + r] = S.sw_reg;
+            float jv = S.sw_reg * (s * (vi[r] * d[0] + vi[3 + r] * d[1] + vi[6 + r] * d[2]) + vi[9 + r]);
+            if (fk >= 0) jv -= S.sw_reg * s * v[12 * (int64_t)fk + 9 + r];
+            Jv[r] = jv;
+        }
+        fn(3, Ji, Jv);
+        return;
+    }
+    item -= S.K;
+    const int n_in = S.in_ptr[fi + 1] - S.in_ptr[fi];
+    if (item < n_in) {
+        const int e = S.in_edge[S.in_ptr[fi] + item];  // src*K + t
+        const int src = e / S.K;
+        const int fs = S.free_id[src];
+        if (fs < 0) {
+            // static(src, t): rows -sw_reg t_i
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                Ji[r][9 + r] = -S.sw_reg;
+                Jv[r] = -S.sw_reg * v[12 * (int64_t)fi + 9 + r];
+            }
+        } else {
+            const float s = S.edge_scale[e];
+            const float* gs = S.g + 3 * src;
+            const float* gi = S.g + 3 * i;
+            const float d[3] = {gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2]};
+            const float* vs = v + 12 * (int64_t)fs;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                Ji[r][9 + r] = -S.sw_reg * s;
+                Jv[r] = S.sw_reg * (s * (vs[r] * d[0] + vs[3 + r] * d[1] + vs[6 + r] * d[2]) + vs[9 + r]) -
+                        S.sw_reg * s * v[12 * (int64_t)fi + 9 + r];
+            }
+        }
+        fn(3, Ji, Jv);
+        return;
+    }
+    item -= n_in;
+    const int ent = S.cm_ptr[fi] + item;
+    const int cid = S.cm_con[ent];
+    const int slot = S.cm_slot[ent];
+    const int c = S.con_node[cid];
+    const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
+    {
+        const float wi = S.src_w[(int64_t)c * S.K + slot];
+        const float* gi = S.g + 3 * i;
+        const float dd[3] = {vv[0] - gi[0], vv[1] - gi[1], vv[2] - gi[2]};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) Ji[r][3 * cc + r] = S.sw_con * wi * dd[cc];
+            Ji[r][9 + r] = S.sw_con * wi;
+        }
+    }
+    for (int j = 0; j < S.K; j++) {
+        const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+        const int fj = S.free_id[nj];
+        if (fj < 0) continue;
+        const float w = S.src_w[(int64_t)c * S.K + j];
+        const float* gj = S.g + 3 * nj;
+        const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
+        const float* vj = v + 12 * (int64_t)fj;
+#pragma unroll
+        for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (vj[r] * dd[0] + vj[3 + r] * dd[1] + vj[6 + r] * dd[2] + vj[9 + r]);
+    }
+    fn(3, Ji, Jv);
+}
+
+__device__ __forceinline__ int owner_items(const SolverDev& S, int fi) {
+    return 1 + S.K + (S.in_ptr[fi + 1] - S.in_ptr[fi]) + (S.cm_ptr[fi + 1] - S.cm_ptr[fi]);
+}
+
+__device__ __forceinline__ float group_reduce16(float v) {
+    v += __shfl_xor(v, 8, 16);
+    v += __shfl_xor(v, 4, 16);
+    v += __shfl_xor(v, 2, 16);
+    v += __shfl_xor(v, 1, 16);
+    return v;
+}
+
+// Cholesky-based inverse of a 12x12 SPD block (row-major), in place.
+__device__ void invert12(float A[144]) {
+    float L[12][12];
+    for (int j = 0; j < 12; j++) {
+        float d = A[j * 12 + j];
+        for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
+        d = d > 1e-30f ? sqrtf(d) : 1e-15f;
+        L[j][j] = d;
+        for (int i = j + 1; i < 12; i++) {
+            float s = A[i * 12 + j];
+            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+            L[i][j] = s / d;
+        }
+        for (int i = 0; i < j; i++) L[i][j] = 0.0f;
+    }
+    // inv = L^-T L^-1, column by column
+    for (int col = 0; col < 12; col++) {
+        float y[12];
+        for (int i = 0; i < 12; i++) {
+            float s = (i == col) ? 1.0f : 0.0f;
+            for (int k = 0; k < i; k++) s -= L[i][k] * y[k];
+            y[i] = s / L[i][i];
+        }
+        for (int i = 11; i >= 0; i--) {
+            float s = y[i];
+            for (int k = i + 1; k < 12; k++) s -= L[k][i] * y[k];
+            y[i] = s / L[i][i];
+        }
+        for (int i = 0; i < 12; i++) A[i * 12 + col] = y[i];
+    }
+}
+
+// b = -J^T f, diagonal block D_i = sum J_i^T J_i and its inverse; |b|^2 partials.
+__global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float* __restrict__ x,
+                                                       float* __restrict__ b, float* __restrict__ binv,
+                                                       double* __restrict__ partial, const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ float sD[256 / LANES][144];
+    __shared__ double shd[8];
+    const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int fi = blockIdx.x * (256 / LANES) + grp;
+    float bl[12], D[78];
+#pragma unroll
+    for (int a = 0; a < 12; a++) bl[a] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 78; a++) D[a] = 0.0f;
+    if (fi < S.nF) {
+        const int ni = owner_items(S, fi);
+        for (int it = lane; it < ni; it += LANES) {
+            // residual value of the group for the gradient: evaluate f via J*x
+            // identities is not possible for the affine parts, so recompute f.
+            float f[6];
+            int64_t gid;
+            // map item -> global group id for the residual
+            {
+                const int i = S.free_nodes[fi];
+                int item = it;
+                if (item == 0) gid = fi;
+                else if ((item -= 1) < S.K) gid = S.nF + (int64_t)fi * S.K + item;
+                else {
+                    item -= S.K;
+                    const int n_in = S.in_ptr[fi + 1] - S.in_ptr[fi];
+                    if (item < n_in) {
+                        const int e = S.in_edge[S.in_ptr[fi] + item];
+                        const int src = e / S.K, t = e % S.K;
+                        const int fs = S.free_id[src];
+                        gid = fs >= 0 ? S.nF + (int64_t)fs * S.K + t
+                                      : S.nF + (int64_t)S.nF * S.K + (int64_t)S.static_id[src] * S.K + t;
+                    } else {
+                        item -= n_in;
+                        gid = S.nF + (int64_t)(S.nF + S.nS) * S.K + S.cm_con[S.cm_ptr[fi] + item];
+                    }
+                }
+                (void)i;
+            }
+            group_residual(S, x, nullptr, 0.0f, gid, f);
+            owner_item(S, fi, it, x, x, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
+                (void)Jv;
+                for (int r = 0; r < rows; r++) {
+#pragma unroll
+                    for (int a = 0; a < 12; a++) bl[a] -= Ji[r][a] * f[r];
+                    int o = 0;
+#pragma unroll
+                    for (int a = 0; a < 12; a++)
+#pragma unroll
+                        for (int c = a; c < 12; c++) D[o++] += Ji[r][a] * Ji[r][c];
+                }
+            });
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 12; a++) bl[a] = group_reduce16(bl[a]);
+#pragma unroll
+    for (int a = 0; a < 78; a++) D[a] = group_reduce16(D[a]);
+    double nb = 0.0;
+    if (fi < S.nF && lane == 0) {
+        int o = 0;
+        for (int a = 0; a < 12; a++)
+            for (int c = a; c < 12; c++) {
+                sD[grp][a * 12 + c] = D[o];
+                sD[grp][c * 12 + a] = D[o];
+                o++;
+            }
+        for (int a = 0; a < 12; a++) {
+            b[12 * (int64_t)fi + a] = bl[a];
+            nb += (double)bl[a] * bl[a];
+        }
+        invert12(sD[grp]);
+        for (int a = 0; a < 144; a++) binv[144 * (int64_t)fi + a] = sD[grp][a];
+    }
+    const double t = block_reduce_d(nb, shd);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// q = A p for all free nodes; partial p.q
+__global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
+                                                    const float* __restrict__ p, float* __restrict__ q,
+                                                    double* __restrict__ partial, const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ double shd[8];
+    const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int fi = blockIdx.x * (256 / LANES) + grp;
+    float acc[12];
+#pragma unroll
+    for (int a = 0; a < 12; a++) acc[a] = 0.0f;
+    if (fi < S.nF) {
+        const int ni = owner_items(S, fi);
+        for (int it = lane; it < ni; it += LANES) {
+            owner_item(S, fi, it, x, p, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
+                for (int r = 0; r < rows; r++)
+#pragma unroll
+                    for (int a = 0; a < 12; a++) acc[a] += Ji[r][a] * Jv[r];
+            });
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 12; a++) acc[a] = group_reduce16(acc[a]);
+    double pq = 0.0;
+    if (fi < S.nF && lane == 0) {
+        for (int a = 0; a < 12; a++) {
+            q[12 * (int64_t)fi + a] = acc[a];
+            pq += (double)acc[a] * p[12 * (int64_t)fi + a];
+        }
+    }
+    const double t = block_reduce_d(pq, shd);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Scalar bookkeeping of PCG in one thread (all FP64):
+//   mode 0: init   rz = r.z (from partial), p = z handled elsewhere
+//   mode 1: alpha  alpha = rz / pq
+//   mode 2: beta   rz_new, beta = rz_new / rz, convergence test on |r|^2
+__global__ void scalar_kernel(int mode, const double* __restrict__ partial, int np, CgScalars* __restrict__ s,
+                              const double* __restrict__ partial2, int np2, int* __restrict__ skip) {
+    if (*skip) return;
+    if (threadIdx.x != 0) return;
+    double a = 0.0;
+    for (int i = 0; i < np; i++) a += partial[i];
+    if (mode == 0) {
+        s->rz = a;
+        s->beta = 0.0;
+        double rr = 0.0;
+        for (int i = 0; i < np2; i++) rr += partial2[i];
+        s->rr = rr;
+        s->bb = rr;
+        s->iters = 0;
+        if (!(rr > 0.0)) *skip = 1;
+    } else if (mode == 1) {
+        s->alpha = (a != 0.0) ? s->rz / a : 0.0;
+    } else {
+        double rr = 0.0;
+        for (int i = 0; i < np2; i++) rr += partial2[i];
+        s->rr = rr;
+        s->beta = (s->rz != 0.0) ? a / s->rz : 0.0;
+        s->rz = a;
+        s->iters += 1;
+        if (rr <= s->tol2 * s->bb || s->iters >= s->max_iters) *skip = 1;
+    }
+}
+
+// One workgroup per aggregate: h += alpha p, r -= alpha q (mode 1), then
+// z1 = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
+__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const CgScalars* __restrict__ s,
+                                                              float* __restrict__ hsol, float* __restrict__ r,
+                                                              const float* __restrict__ p, const float* __restrict__ q,
+                                                              const float* __restrict__ binv, float* __restrict__ z,
+                                                              float* __restrict__ rc, double* __restrict__ partial,
+                                                              const int* __restrict__ skip) {
+    if (*skip) return;
+    const int a = blockIdx.x;
+    const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
+    const float alpha = mode == 1 ? (float)s->alpha : 0.0f;
+    __shared__ float sred[256 / LANES][12];
+    __shared__ double shd[8];
+    const float* ca = S.agg_center + 3 * a;
+    float rcl[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) rcl[c] = 0.0f;
+    double rr = 0.0;
+    // 16 lanes per node: lane c < 12 handles component c
+    for (int m = beg + (int)(threadIdx.x / LANES); m < end; m += 256 / LANES) {
+        const int fi = S.agg_member[m];
+        const int c = threadIdx.x % LANES;
+        float rv = 0.0f;
+        if (c < 12) {
+            const int64_t o = 12 * (int64_t)fi + c;
+            if (mode == 1) {
+                hsol[o] += alpha * p[o];
+                r[o] -= alpha * q[o];
+            }
+            rv = r[o];
+            rr += (double)rv * rv;
+        }
+        // z1 = Binv r (row c), needs all 12 r of the node: gather within 16 lanes
+        float rn[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) rn[j] = __shfl(rv, j, 16);
+        if (c < 12) {
+            float zz = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 12; j++) zz += binv[144 * (int64_t)fi + c * 12 + j] * rn[j];
+            z[12 * (int64_t)fi + c] = zz;
+        }
+        // restriction P^T r: R part (9) passes through; t part r_t contributes to
+        // b (identity) and to A(r,c') with weight (g_j - c_a)_c'
+        const int node = S.free_nodes[fi];
+        const float dj[3] = {S.g[3 * node] - ca[0], S.g[3 * node + 1] - ca[1], S.g[3 * node + 2] - ca[2]};
+        if (c == 0) {
+#pragma unroll
+            for (int q9 = 0; q9 < 9; q9++) rcl[q9] += rn[q9];
+#pragma unroll
+            for (int rr3 = 0; rr3 < 3; rr3++) {
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) rcl[3 * cc + rr3] += dj[cc] * rn[9 + rr3];
+                rcl[9 + rr3] += rn[9 + rr3];
+            }
+        }
+    }
+    if (threadIdx.x % LANES == 0)
+#pragma unroll
+        for (int c = 0; c < 12; c++) sred[threadIdx.x / LANES][c] = rcl[c];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        float t = 0.0f;
+        for (int g = 0; g < 256 / LANES; g++) t += sred[g][threadIdx.x];
+        rc[12 * a + threadIdx.x] = t;
+    }
+    const double tr = block_reduce_d(rr, shd);
+    if (threadIdx.x == 0) partial[a] = tr;
+}
+
+// One workgroup per aggregate: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z.
+__global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const float* __restrict__ acinv,
+                                                             const float* __restrict__ rc, const float* __restrict__ r,
+                                                             float* __restrict__ z, double* __restrict__ partial,
+                                                             const int* __restrict__ skip) {
+    if (*skip) return;
+    const int a = blockIdx.x;
+    const int nc = 12 * S.n_agg;
+    const int ld = S.nc_pad;
+    __shared__ float zc[12];
+    __shared__ double shd[8];
+    // 12 rows of Ac^-1 times rc: 16 lanes per row
+    {
+        const int row = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+        float acc = 0.0f;
+        if (row < 12)
+            for (int j = lane; j < nc; j += LANES) acc += acinv[(int64_t)(12 * a + row) * ld + j] * rc[j];
+        acc = group_reduce16(acc);
+        if (row < 12 && lane == 0) zc[row] = acc;
+    }
+    __syncthreads();
+    const float* ca = S.agg_center + 3 * a;
+    double rz = 0.0;
+    const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
+    for (int m = beg + (int)(threadIdx.x / LANES); m < end; m += 256 / LANES) {
+        const int fi = S.agg_member[m];
+        const int c = threadIdx.x % LANES;
+        if (c >= 12) continue;
+        const int node = S.free_nodes[fi];
+        float add;
+        if (c < 9) add = zc[c];
+        else {
+            const int rr3 = c - 9;
+            const float dj[3] = {S.g[3 * node] - ca[0], S.g[3 * node + 1] - ca[1], S.g[3 * node + 2] - ca[2]};
+            add = zc[9 + rr3] + zc[rr3] * dj[0] + zc[3 + rr3] * dj[1] + zc[6 + rr3] * dj[2];
+        }
+        const int64_t o = 12 * (int64_t)fi + c;
+        const float zz = z[o] + add;
+        z[o] = zz;
+        rz += (double)zz * r[o];
+    }
+    const double t = block_reduce_d(rz, shd);
+    if (threadIdx.x == 0) partial[a] = t;
+}
+
+// p = z + beta p
+__global__ void pupdate_kernel(int n, const CgScalars* __restrict__ s, const float* __restrict__ z,
+                               float* __restrict__ p, const int* __restrict__ skip) {
+    if (*skip) return;
+    const float beta = (float)s->beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void zero_kernel(float* __restrict__ v, int64_t n, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = 0.0f;
+}
+
+__global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n,
+                            const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------- coarse space
+__global__ void agg_center_kernel(SolverDev S) {
+    const int a = blockIdx.x;
+    __shared__ float sx[256], sy[256], sz[256];
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
+    for (int m = beg + threadIdx.x; m < end; m += blockDim.x) {
+        const int node = S.free_nodes[S.agg_member[m]];
+        x += S.g[3 * node]; y += S.g[3 * node + 1]; z += S.g[3 * node + 2];
+    }
+    sx[threadIdx.x] = x; sy[threadIdx.x] = y; sz[threadIdx.x] = z;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sx[threadIdx.x] += sx[threadIdx.x + s];
+            sy[threadIdx.x] += sy[threadIdx.x + s];
+            sz[threadIdx.x] += sz[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float n = (float)(end - beg > 0 ? end - beg : 1);
+        S.agg_center[3 * a] = sx[0] / n; S.agg_center[3 * a + 1] = sy[0] / n; S.agg_center[3 * a + 2] = sz[0] / n;
+    }
+}
+
+// Galerkin coarse matrix Ac = P^T (J^T J) P accumulated per residual group:
+// each group's J projected onto the coarse dofs of the aggregates it touches.
+__global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
+                                                              float* __restrict__ Ac) {
+    const int nc = S.nc_pad;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        // collect (free node, Jblock) pairs of the group
+        int nodes[KNN_MAX_T + 1];
+        float J[KNN_MAX_T + 1][3][12];
+        int rows = 3, nn = 0;
+        float Jrot[6][9];
+        int64_t gg = g;
+        if (gg < S.nF) {
+            const int fi = (int)gg;
+            const float* xi = x + 12 * (int64_t)fi;
+            for (int a = 0; a < 6; a++)
+                for (int b = 0; b < 9; b++) Jrot[a][b] = 0.0f;
+            const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+            for (int p = 0; p < 3; p++)
+                for (int c = 0; c < 3; c++) {
+                    Jrot[p][3 * pa[p] + c] = S.sw_rot * xi[3 * pb[p] + c];
+                    Jrot[p][3 * pb[p] + c] += S.sw_rot * xi[3 * pa[p] + c];
+                }
+            for (int j = 0; j < 3; j++)
+                for (int c = 0; c < 3; c++) Jrot[3 + j][3 * j + c] = 2.0f * S.sw_rot * xi[3 * j + c];
+            // R part maps identically to the aggregate's A: rows x 9 block
+            const int a = S.agg_of[fi];
+            for (int u = 0; u < 9; u++)
+                for (int w = 0; w < 9; w++) {
+                    float s = 0.0f;
+                    for (int rr = 0; rr < 6; rr++) s += Jrot[rr][u] * Jrot[rr][w];
+                    if (s != 0.0f) atomicAdd(&Ac[(int64_t)(12 * a + u) * nc + 12 * a + w], s);
+                }
+            continue;
+        }
+        gg -= S.nF;
+        for (int q = 0; q < KNN_MAX_T + 1; q++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 12; c++) J[q][r][c] = 0.0f;
+        if (gg < (int64_t)S.nF * S.K) {
+            const int fi = (int)(gg / S.K), t = (int)(gg % S.K);
+            const int i = S.free_nodes[fi];
+            const int k = (int)S.nbr[(int64_t)i * S.K + t];
+            const int fk = S.free_id[k];
+            const float s = S.edge_scale[(int64_t)i * S.K + t];
+            const float d[3] = {S.g[3 * k] - S.g[3 * i], S.g[3 * k + 1] - S.g[3 * i + 1], S.g[3 * k + 2] - S.g[3 * i + 2]};
+            nodes[nn] = fi;
+            for (int r = 0; r < 3; r++) {
+                for (int c = 0; c < 3; c++) J[nn][r][3 * c + r] = S.sw_reg * s * d[c];
+                J[nn][r][9 + r] = S.sw_reg;
+            }
+            nn++;
+            if (fk >= 0) {
+                nodes[nn] = fk;
+                for (int r = 0; r < 3; r++) J[nn][r][9 + r] = -S.sw_reg * s;
+                nn++;
+            }
+        } else {
+            gg -= (int64_t)S.nF * S.K;
+            if (gg < (int64_t)S.nS * S.K) {
+                const int si = (int)(gg / S.K), t = (int)(gg % S.K);
+                const int i = S.static_nodes[si];
+                const int k = (int)S.nbr[(int64_t)i * S.K + t];
+                const int fk = S.free_id[k];
+                if (fk < 0) continue;
+                nodes[nn] = fk;
+                for (int r = 0; r < 3; r++) J[nn][r][9 + r] = -S.sw_reg;
+                nn++;
+            } else {
+                gg -= (int64_t)S.nS * S.K;
+                const int c = S.con_node[(int)gg];
+                const float v[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
+                for (int j = 0; j < S.K; j++) {
+                    const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+                    const int fj = S.free_id[nj];
+                    if (fj < 0) continue;
+                    const float w = S.src_w[(int64_t)c * S.K + j];
+                    const float dd[3] = {v[0] - S.g[3 * nj], v[1] - S.g[3 * nj + 1], v[2] - S.g[3 * nj + 2]};
+                    nodes[nn] = fj;
+                    for (int r = 0; r < 3; r++) {
+                        for (int cc = 0; cc < 3; cc++) J[nn][r][3 * cc + r] = S.sw_con * w * dd[cc];
+                        J[nn][r][9 + r] = S.sw_con * w;
+                    }
+                    nn++;
+                }
+            }
+        }
+        // project each node's block onto its aggregate: JP[q][r][12]
+        float JP[KNN_MAX_T + 1][3][12];
+        int aggs[KNN_MAX_T + 1];
+        for (int q = 0; q < nn; q++) {
+            const int fi = nodes[q];
+            const int a = S.agg_of[fi];
+            aggs[q] = a;
+            const int node = S.free_nodes[fi];
+            const float dj[3] = {S.g[3 * node] - S.agg_center[3 * a], S.g[3 * node + 1] - S.agg_center[3 * a + 1],
+                                 S.g[3 * node + 2] - S.agg_center[3 * a + 2]};
+            for (int r = 0; r < rows; r++) {
+                for (int u = 0; u < 9; u++) JP[q][r][u] = J[q][r][u];
+                for (int rr3 = 0; rr3 < 3; rr3++) {
+                    const float jt = J[q][r][9 + rr3];
+                    for (int cc = 0; cc < 3; cc++) JP[q][r][3 * cc + rr3] += jt * dj[cc];
+                    JP[q][r][9 + rr3] = jt;
+                }
+            }
+        }
+        // merge nodes sharing an aggregate
+        for (int q = 0; q < nn; q++) {
+            if (aggs[q] < 0) continue;
+            for (int q2 = q + 1; q2 < nn; q2++)
+                if (aggs[q2] == aggs[q]) {
+                    for (int r = 0; r < rows; r++)
+                        for (int u = 0; u < 12; u++) JP[q][r][u] += JP[q2][r][u];
+                    aggs[q2] = -1;
+                }
+        }
+        for (int q = 0; q < nn; q++) {
+            if (aggs[q] < 0) continue;
+            for (int q2 = 0; q2 < nn; q2++) {
+                if (aggs[q2] < 0) continue;
+                for (int u = 0; u < 12; u++)
+                    for (int w = 0; w < 12; w++) {
+                        float s = 0.0f;
+                        for (int r = 0; r < rows; r++) s += JP[q][r][u] * JP[q2][r][w];
+                        if (s != 0.0f) atomicAdd(&Ac[(int64_t)(12 * aggs[q] + u) * nc + 12 * aggs[q2] + w], s);
+                    }
+            }
+        }
+    }
+}
+
+// Blocked in-place Gauss-Jordan inverse of the SPD coarse matrix (no pivoting;
+// n a multiple of 64, padding rows/cols hold the identity). Per 64-wide pivot
+// block K: (1) invert A_KK in LDS, (2) scale the row panel by A_KK^-1 and save
+// the old column panel, (3) rank-64 update of every other tile and the new
+// column panel -A_iK A_KK^-1.
+constexpr int GJ_T = 64;
+
+__global__ void __launch_bounds__(256) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
+    __shared__ float P[GJ_T][GJ_T + 1];
+    const int k0 = kb * GJ_T;
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) P[e / GJ_T][e % GJ_T] = A[(int64_t)(k0 + e / GJ_T) * n + k0 + e % GJ_T];
+    __syncthreads();
+    __shared__ float inv;
+    for (int k = 0; k < GJ_T; k++) {
+        if (threadIdx.x == 0) {
+            const float d = P[k][k];
+            inv = (fabsf(d) > 1e-30f) ? 1.0f / d : 0.0f;
+        }
+        __syncthreads();
+        const float iv = inv;
+        for (int j = threadIdx.x; j < GJ_T; j += blockDim.x)
+            if (j != k) P[k][j] *= iv;
+        __syncthreads();
+        for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
+            const int i = e / GJ_T, j = e % GJ_T;
+            if (i == k || j == k) continue;
+            P[i][j] -= P[i][k] * P[k][j];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < GJ_T; i += blockDim.x)
+            if (i != k) P[i][k] *= -iv;
+        __syncthreads();
+        if (threadIdx.x == 0) P[k][k] = iv;
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) piv[e] = P[e / GJ_T][e % GJ_T];
+}
+
+// blockIdx.x = tile column jb != kb: row panel A_Kj <- piv * A_Kj; also copies
+// the old column panel tile A_{jb,K} into colbuf.
+__global__ void __launch_bounds__(256) gj_panel_kernel(float* __restrict__ A, int n, int kb, const float* __restrict__ piv,
+                                                       float* __restrict__ colbuf) {
+    const int jb = blockIdx.x;
+    if (jb == kb) return;
+    __shared__ float Pv[GJ_T][GJ_T + 1];
+    __shared__ float R[GJ_T][GJ_T + 1];
+    const int k0 = kb * GJ_T, j0 = jb * GJ_T;
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
+        const int r = e / GJ_T, c = e % GJ_T;
+        Pv[r][c] = piv[e];
+        R[r][c] = A[(int64_t)(k0 + r) * n + j0 + c];
+        colbuf[(int64_t)(j0 + r) * GJ_T + c] = A[(int64_t)(j0 + r) * n + k0 + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
+        const int r = e / GJ_T, c = e % GJ_T;
+        float acc = 0.0f;
+        for (int t = 0; t < GJ_T; t++) acc += Pv[r][t] * R[t][c];
+        A[(int64_t)(k0 + r) * n + j0 + c] = acc;
+    }
+}
+
+// blockIdx = (ib, jb): A_ij -= C_iK * A_Kj (new row panel) for ib != kb;
+// for jb == kb the tile becomes -C_iK * piv; tile (kb, kb) becomes piv.
+__global__ void __launch_bounds__(256) gj_update_kernel(float* __restrict__ A, int n, int kb, const float* __restrict__ piv,
+                                                        const float* __restrict__ colbuf) {
+    const int ib = blockIdx.y, jb = blockIdx.x;
+    const int k0 = kb * GJ_T, i0 = ib * GJ_T, j0 = jb * GJ_T;
+    if (ib == kb) {
+        if (jb == kb)
+            for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x)
+                A[(int64_t)(k0 + e / GJ_T) * n + k0 + e % GJ_T] = piv[e];
+        return;
+    }
+    __shared__ float Cs[GJ_T][GJ_T + 1];
+    __shared__ float Bs[GJ_T][GJ_T + 1];
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
+        const int r = e / GJ_T, c = e % GJ_T;
+        Cs[r][c] = colbuf[(int64_t)(i0 + r) * GJ_T + c];
+        Bs[r][c] = (jb == kb) ? piv[e] : A[(int64_t)(k0 + r) * n + j0 + c];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
+        const int r = e / GJ_T, c = e % GJ_T;
+        float acc = 0.0f;
+        for (int t = 0; t < GJ_T; t++) acc += Cs[r][t] * Bs[t][c];
+        const int64_t o = (int64_t)(i0 + r) * n + j0 + c;
+        if (jb == kb) A[o] = -acc;
+        else A[o] -= acc;
+    }
+}
+
+__global__ void pad_identity_kernel(float* __restrict__ A, int n, int nc) {
+    for (int i = nc + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) A[(int64_t)i * n + i] = 1.0f;
+}
+
+// ---------------------------------------------------------------- GN step
+// Choose the first candidate with E <= E_old, update x, record energies and the
+// stop rule |h_final| < (|x_old| + 1e-6) 1e-6 (Deform.cpp:142-166).
+__global__ void accept_kernel(int n, float* __restrict__ x, const float* __restrict__ hsol,
+                              const double* __restrict__ cand_e, int ncand, GnScalars* __restrict__ gs,
+                              double* __restrict__ energies, int iter, int stop_rule, int* __restrict__ gn_done) {
+    if (*gn_done) return;
+    __shared__ int chosen;
+    __shared__ double red[256];
+    if (threadIdx.x == 0) {
+        chosen = -1;
+        for (int c = 0; c < ncand; c++)
+            if (!(cand_e[c] > gs->e_old)) { chosen = c; break; }
+        energies[iter] = gs->e_old;
+    }
+    __syncthreads();
+    double nx = 0.0, nh = 0.0;
+    const float a = chosen >= 0 ? ldexpf(1.0f, -chosen) : 0.0f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        nx += (double)x[i] * x[i];
+        nh += (double)hsol[i] * hsol[i];
+        if (chosen >= 0) x[i] += a * hsol[i];
+    }
+    red[threadIdx.x] = nx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double normv = sqrt(red[0]);
+    __syncthreads();
+    red[threadIdx.x] = nh;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double scale = chosen >= 0 ? ldexp(1.0, -chosen) : ldexp(1.0, -ncand);
+        const double normh = sqrt(red[0]) * scale;
+        if (chosen >= 0) gs->e_old = cand_e[chosen];
+        gs->iters_done = iter + 1;
+        energies[iter + 1] = gs->e_old;
+        if (stop_rule && normh < (normv + 1e-6) * 1e-6) *gn_done = 1;
+    }
+}
+
+__global__ void init_x_kernel(int n, float* __restrict__ x, GnScalars* __restrict__ gs, int* __restrict__ gn_done,
+                              const double* __restrict__ e0) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = i % 12;
+        x[i] = (c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *gn_done = 0;
+        gs->iters_done = 0;
+        (void)e0;
+    }
+}
+
+__global__ void set_eold_kernel(GnScalars* __restrict__ gs, const double* __restrict__ e) { gs->e_old = e[0]; }
+
+__global__ void cg_begin_kernel(const int* __restrict__ gn_done, int* __restrict__ cg_done, CgScalars* __restrict__ s,
+                                int max_iters, double tol2) {
+    *cg_done = *gn_done;
+    s->max_iters = max_iters;
+    s->tol2 = tol2;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host driver
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int solver_frame_setup(SolverWork& W, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    hipLaunchKernelGGL(agg_center_kernel, dim3(S.n_agg), dim3(256), 0, st, S);
+    const int nc = 12 * S.n_agg;
+    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.acinv, (int64_t)S.nc_pad * S.nc_pad,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(init_x_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.n, W.x, W.gs, W.gn_done,
+                       (const double*)nullptr);
+    hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.acinv);
+    const int np = S.nc_pad;
+    hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
+    const int nt = np / GJ_T;
+    for (int kb = 0; kb < nt; kb++) {
+        hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
+        hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+        hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+    }
+    // E(x0)
+    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, (const float*)nullptr,
+                       W.e_partial, (const int*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, W.cand_e,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(set_eold_kernel, dim3(1), dim3(1), 0, st, W.gs, W.cand_e);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_gn_begin(SolverWork& W, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    const int nb_owner = nblk(S.nF, 256 / LANES);
+    hipLaunchKernelGGL(cg_begin_kernel, dim3(1), dim3(1), 0, st, W.gn_done, W.cg_done, W.cg, W.cg_max_iters,
+                       W.cg_tol * W.cg_tol);
+    hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial,
+                       W.cg_done);
+    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.cg_done);
+    // z = M^-1 r (mode 0: no update), rz, p = z
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, W.cg, W.hsol, W.r, W.p, W.q,
+                       W.binv, W.z, W.rc, W.partial2, W.cg_done);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z,
+                       W.partial3, W.cg_done);
+    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 0, W.partial3, S.n_agg, W.cg, W.partial, nb_owner,
+                       W.cg_done);
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.p, W.z, (int64_t)W.n, W.cg_done);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_cg_iteration(SolverWork& W, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    const int nb_owner = nblk(S.nF, 256 / LANES);
+    hipLaunchKernelGGL(apply_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.p, W.q, W.partial, W.cg_done);
+    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 1, W.partial, nb_owner, W.cg, (const double*)nullptr,
+                       0, W.cg_done);
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 1, W.cg, W.hsol, W.r, W.p, W.q,
+                       W.binv, W.z, W.rc, W.partial2, W.cg_done);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z,
+                       W.partial3, W.cg_done);
+    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 2, W.partial3, S.n_agg, W.cg, W.partial2, S.n_agg,
+                       W.cg_done);
+    hipLaunchKernelGGL(pupdate_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.n, W.cg, W.z, W.p, W.cg_done);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, W.n_cand), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial,
+                       W.gn_done);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, W.n_cand, W.cand_e,
+                       W.gn_done);
+    hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(256), 0, st, W.n, W.x, W.hsol, W.cand_e, W.n_cand, W.gs,
+                       energies, iter, stop_rule, W.gn_done);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, x, (const float*)nullptr, W.e_partial,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, out,
+                       (const int*)nullptr);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+}  // namespace arapgs

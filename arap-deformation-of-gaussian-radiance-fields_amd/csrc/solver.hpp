@@ -1,0 +1,92 @@
+// Gauss-Newton solver state (device side) for solver.hip / solver_host.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace arapgs {
+
+// Everything the residual / Jacobian kernels read. Plain device pointers,
+// passed to kernels by value.
+struct SolverDev {
+    int M, K;
+    int nF, nS, nC;        // free nodes, static nodes, constraint groups
+    int64_t n_groups;      // nF + nF*K + nS*K + nC
+    int n_agg, nc_pad;     // aggregates, padded coarse dimension (multiple of 64)
+    float sw_rot, sw_reg, sw_con;
+    const int* free_nodes;     // nF   node id of each free rank (ascending)
+    const int* static_nodes;   // nS
+    const int* free_id;        // M    rank among free nodes or -1
+    const int* static_id;      // M    rank among static nodes or -1
+    const uint32_t* nbr;       // M*K  graph neighbours
+    const float* edge_scale;   // M*K  weighting factor of (i, t) when i and k share a non-excluded block
+    const uint32_t* src_idx;   // M*K  KNN of each node's source vertex
+    const float* src_w;        // M*K
+    const int* con_node;       // nC   control node of each constraint group
+    const int* in_ptr;         // nF+1 in-edges (src*K + t) of each free node
+    const int* in_edge;
+    const int* cm_ptr;         // nF+1 constraint memberships of each free node
+    const int* cm_con;
+    const int* cm_slot;
+    const int* agg_ptr;        // n_agg+1 free ranks grouped by aggregate
+    const int* agg_member;
+    const int* agg_of;         // nF
+    float* agg_center;         // n_agg*3 (per frame)
+    const float* g;            // M*3 current node positions (per frame)
+    const float* aim;          // M*3 aims (per frame)
+};
+
+struct CgScalars {
+    double rz, alpha, beta, rr, bb, tol2;
+    int iters, max_iters;
+};
+
+struct GnScalars {
+    double e_old;
+    int iters_done;
+    int pad;
+};
+
+struct SolverWork {
+    SolverDev dev;
+    int n;  // 12 * nF
+    float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col;
+    double *partial, *partial2, *partial3, *e_partial, *cand_e;
+    int e_blocks, n_cand;
+    CgScalars* cg;
+    GnScalars* gs;
+    int* gn_done;
+    int* cg_done;
+    int cg_max_iters;
+    double cg_tol;
+    std::vector<void*> allocs;  // owned device buffers
+};
+
+struct SolverDesc {
+    int M, K;
+    const uint32_t* nbr;
+    const int* block_ptr;
+    const int* block_nodes;
+    const int* block_type;
+    int nblocks;
+    const uint32_t* src_idx;
+    const float* src_w;
+    const float* node_pos;
+    float w_rot, w_reg, w_con, wf;
+    int center;
+    int n_agg;
+};
+
+// Host-side setup; returns 0 or a negative error with a message in err.
+int solver_build(const SolverDesc& d, SolverWork& W, std::string& err);
+void solver_free(SolverWork& W);
+
+int solver_frame_setup(SolverWork& W, hipStream_t st);
+int solver_gn_begin(SolverWork& W, hipStream_t st);
+int solver_cg_iteration(SolverWork& W, hipStream_t st);
+int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st);
+int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st);
+
+}  // namespace arapgs

@@ -1,0 +1,206 @@
+// Host-side construction of the Gauss-Newton problem structure (once per edit,
+// i.e. whenever blocks / graph change — the per-frame inputs are only node
+// positions and aims). Mirrors the bookkeeping of Deform::Deform,
+// GetControlNums and BelongsToSameBlock (Deform.hpp:414-455, Deform.cpp:6-32)
+// and UpdateIndicies (GaussianView.cpp:1995-2021), precomputed as flat arrays.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "solver.hpp"
+
+namespace arapgs {
+
+namespace {
+
+template <typename T>
+T* upload(SolverWork& W, const std::vector<T>& v, int& rc) {
+    if (rc) return nullptr;
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) { rc = -2; return nullptr; }
+    W.allocs.push_back(p);
+    if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) rc = -2;
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+T* alloc(SolverWork& W, size_t n, int& rc) {
+    if (rc) return nullptr;
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) { rc = -2; return nullptr; }
+    if (hipMemset(p, 0, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) { rc = -2; return nullptr; }
+    W.allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+void solver_free(SolverWork& W) {
+    for (void* p : W.allocs) (void)hipFree(p);
+    W.allocs.clear();
+}
+
+int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
+    const int M = d.M, K = d.K;
+    if (M <= 0 || K <= 0 || K > KNN_MAX_T) { err = "solver: need M > 0 and 1 <= K <= 12"; return -1; }
+    if (d.center) { err = "solver: constraints-on-centre mode is not supported"; return -1; }
+    // static = nodes of blocks with type < 0 (UpdateIndicies)
+    std::vector<char> is_static(M, 0);
+    std::vector<std::vector<int>> node_blocks(M);
+    for (int b = 0; b < d.nblocks; b++) {
+        for (int q = d.block_ptr[b]; q < d.block_ptr[b + 1]; q++) {
+            const int v = d.block_nodes[q];
+            if (v < 0 || v >= M) { err = "solver: block node out of range"; return -1; }
+            if (d.block_type[b] < 0) is_static[v] = 1;
+            if (d.block_type[b] != -1) node_blocks[v].push_back(b);
+        }
+    }
+    std::vector<int> free_nodes, static_nodes, free_id(M, -1), static_id(M, -1);
+    for (int i = 0; i < M; i++) {
+        if (is_static[i]) { static_id[i] = (int)static_nodes.size(); static_nodes.push_back(i); }
+        else { free_id[i] = (int)free_nodes.size(); free_nodes.push_back(i); }
+    }
+    const int nF = (int)free_nodes.size(), nS = (int)static_nodes.size();
+    if (nF == 0) { err = "solver: no free nodes"; return -1; }
+    for (int64_t e = 0; e < (int64_t)M * K; e++)
+        if (d.nbr[e] >= (uint32_t)M || d.src_idx[e] >= (uint32_t)M) { err = "solver: neighbour index out of range"; return -1; }
+    // BelongsToSameBlock (Deform.cpp:20-32) per edge
+    std::vector<float> edge_scale((size_t)M * K, 1.0f);
+    for (int i = 0; i < M; i++)
+        for (int t = 0; t < K; t++) {
+            const int k = (int)d.nbr[(size_t)i * K + t];
+            bool same = false;
+            for (int b : node_blocks[i])
+                if (std::find(node_blocks[k].begin(), node_blocks[k].end(), b) != node_blocks[k].end()) same = true;
+            if (same) edge_scale[(size_t)i * K + t] = d.wf;
+        }
+    // in-edges of free nodes
+    std::vector<std::vector<int>> ins(nF);
+    for (int i = 0; i < M; i++)
+        for (int t = 0; t < K; t++) {
+            const int k = (int)d.nbr[(size_t)i * K + t];
+            if (free_id[k] >= 0) ins[free_id[k]].push_back(i * K + t);
+        }
+    std::vector<int> in_ptr(nF + 1, 0), in_edge;
+    for (int f = 0; f < nF; f++) {
+        in_ptr[f + 1] = in_ptr[f] + (int)ins[f].size();
+        in_edge.insert(in_edge.end(), ins[f].begin(), ins[f].end());
+    }
+    // constraints: every node of every non-excluded block, in block order
+    std::vector<int> con_node;
+    for (int b = 0; b < d.nblocks; b++) {
+        if (d.block_type[b] == -1) continue;
+        for (int q = d.block_ptr[b]; q < d.block_ptr[b + 1]; q++) con_node.push_back(d.block_nodes[q]);
+    }
+    const int nC = (int)con_node.size();
+    std::vector<std::vector<std::pair<int, int>>> cms(nF);
+    for (int c = 0; c < nC; c++)
+        for (int j = 0; j < K; j++) {
+            const int nj = (int)d.src_idx[(size_t)con_node[c] * K + j];
+            if (free_id[nj] >= 0) cms[free_id[nj]].push_back({c, j});
+        }
+    std::vector<int> cm_ptr(nF + 1, 0), cm_con, cm_slot;
+    for (int f = 0; f < nF; f++) {
+        cm_ptr[f + 1] = cm_ptr[f] + (int)cms[f].size();
+        for (auto& pr : cms[f]) { cm_con.push_back(pr.first); cm_slot.push_back(pr.second); }
+    }
+    // aggregates: farthest-point seeds over the free nodes' positions, then
+    // nearest-seed assignment
+    const int n_agg = std::max(1, std::min(d.n_agg > 0 ? d.n_agg : 64, nF));
+    std::vector<int> seeds;
+    {
+        std::vector<float> dist(nF, 3.4e38f);
+        int cur = 0;
+        for (int s = 0; s < n_agg; s++) {
+            seeds.push_back(cur);
+            const float* pc = d.node_pos + 3 * (size_t)free_nodes[cur];
+            int best = 0;
+            float bd = -1.0f;
+            for (int f = 0; f < nF; f++) {
+                const float* pf = d.node_pos + 3 * (size_t)free_nodes[f];
+                const float dx = pf[0] - pc[0], dy = pf[1] - pc[1], dz = pf[2] - pc[2];
+                dist[f] = std::min(dist[f], dx * dx + dy * dy + dz * dz);
+                if (dist[f] > bd) { bd = dist[f]; best = f; }
+            }
+            cur = best;
+        }
+    }
+    std::vector<int> agg_of(nF, 0);
+    for (int f = 0; f < nF; f++) {
+        const float* pf = d.node_pos + 3 * (size_t)free_nodes[f];
+        float bd = 3.4e38f;
+        for (int a = 0; a < n_agg; a++) {
+            const float* ps = d.node_pos + 3 * (size_t)free_nodes[seeds[a]];
+            const float dx = pf[0] - ps[0], dy = pf[1] - ps[1], dz = pf[2] - ps[2];
+            const float dd = dx * dx + dy * dy + dz * dz;
+            if (dd < bd) { bd = dd; agg_of[f] = a; }
+        }
+    }
+    std::vector<int> agg_ptr(n_agg + 1, 0), agg_member;
+    for (int a = 0; a < n_agg; a++) {
+        for (int f = 0; f < nF; f++)
+            if (agg_of[f] == a) agg_member.push_back(f);
+        agg_ptr[a + 1] = (int)agg_member.size();
+    }
+
+    int rc = 0;
+    SolverDev& S = W.dev;
+    S.M = M; S.K = K; S.nF = nF; S.nS = nS; S.nC = nC;
+    S.n_groups = (int64_t)nF + (int64_t)nF * K + (int64_t)nS * K + nC;
+    S.n_agg = n_agg;
+    S.nc_pad = ((12 * n_agg + 63) / 64) * 64;
+    S.sw_rot = std::sqrt(d.w_rot); S.sw_reg = std::sqrt(d.w_reg); S.sw_con = std::sqrt(d.w_con);
+    S.free_nodes = upload(W, free_nodes, rc);
+    S.static_nodes = upload(W, static_nodes, rc);
+    S.free_id = upload(W, free_id, rc);
+    S.static_id = upload(W, static_id, rc);
+    S.nbr = upload(W, std::vector<uint32_t>(d.nbr, d.nbr + (size_t)M * K), rc);
+    S.edge_scale = upload(W, edge_scale, rc);
+    S.src_idx = upload(W, std::vector<uint32_t>(d.src_idx, d.src_idx + (size_t)M * K), rc);
+    S.src_w = upload(W, std::vector<float>(d.src_w, d.src_w + (size_t)M * K), rc);
+    S.con_node = upload(W, con_node, rc);
+    S.in_ptr = upload(W, in_ptr, rc);
+    S.in_edge = upload(W, in_edge, rc);
+    S.cm_ptr = upload(W, cm_ptr, rc);
+    S.cm_con = upload(W, cm_con, rc);
+    S.cm_slot = upload(W, cm_slot, rc);
+    S.agg_ptr = upload(W, agg_ptr, rc);
+    S.agg_member = upload(W, agg_member, rc);
+    S.agg_of = upload(W, agg_of, rc);
+    S.agg_center = alloc<float>(W, 3 * (size_t)n_agg, rc);
+    S.g = nullptr;
+    S.aim = nullptr;
+
+    W.n = 12 * nF;
+    W.x = alloc<float>(W, W.n, rc);
+    W.hsol = alloc<float>(W, W.n, rc);
+    W.r = alloc<float>(W, W.n, rc);
+    W.z = alloc<float>(W, W.n, rc);
+    W.p = alloc<float>(W, W.n, rc);
+    W.q = alloc<float>(W, W.n, rc);
+    W.binv = alloc<float>(W, (size_t)144 * nF, rc);
+    W.rc = alloc<float>(W, 12 * (size_t)n_agg, rc);
+    W.acinv = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
+    W.gj_piv = alloc<float>(W, 64 * 64, rc);
+    W.gj_col = alloc<float>(W, (size_t)S.nc_pad * 64, rc);
+    const int nb_owner = (nF + 15) / 16;
+    W.partial = alloc<double>(W, std::max(nb_owner, 1), rc);
+    W.partial2 = alloc<double>(W, n_agg, rc);
+    W.partial3 = alloc<double>(W, n_agg, rc);
+    W.e_blocks = (int)std::min<int64_t>(512, (S.n_groups + 255) / 256);
+    W.n_cand = 50;
+    W.e_partial = alloc<double>(W, (size_t)W.e_blocks * W.n_cand, rc);
+    W.cand_e = alloc<double>(W, W.n_cand, rc);
+    W.cg = alloc<CgScalars>(W, 1, rc);
+    W.gs = alloc<GnScalars>(W, 1, rc);
+    W.gn_done = alloc<int>(W, 1, rc);
+    W.cg_done = alloc<int>(W, 1, rc);
+    W.cg_max_iters = 100;
+    W.cg_tol = 1e-4;
+    if (rc) { err = "solver: device allocation failed"; solver_free(W); return rc; }
+    return 0;
+}
+
+}  // namespace arapgs

@@ -1,0 +1,162 @@
+/*
+ * arapgs — MI355X (gfx950) per-frame "ARAP solve -> deform Gaussians -> render"
+ * path behind a plain C ABI.
+ *
+ * Conventions
+ *   - One context per GPU; all work is queued on the context's stream
+ *     (arapgs_set_stream) and is asynchronous unless stated otherwise.
+ *   - Pointer arguments are DEVICE pointers unless the name ends in `_h`.
+ *   - Every call returns 0 on success, a negative code on error; the message is
+ *     available from arapgs_last_error(ctx). Handles are not thread-safe.
+ *   - Gaussian attributes use the reference's activated SoA layout
+ *     (GaussianView.cpp:43-153 after activation): pos f32[N*3], rot f32[N*4]
+ *     as (w,x,y,z), scale f32[N*3], opacity f32[N], sh f32[N*48] with
+ *     coefficient-major interleaving (coef*3 + channel).
+ *   - K (nearest nodes per point) is 1..12 (KNN_MAX, helper.hpp:48).
+ *   - Steady-state per-frame calls do not allocate device memory once a call
+ *     of the same size has run (workspaces grow on first use).
+ */
+#ifndef ARAPGS_H
+#define ARAPGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct arapgs_ctx arapgs_ctx;
+typedef struct arapgs_solver arapgs_solver;
+
+/* Per-node deformation record (64 bytes): current node position g, rotation R
+ * row-major, translation t. Replaces DeformGraph::rot / trans / nodes[].Position
+ * (Deform.hpp:16-38) in device memory. */
+typedef struct arapgs_node_xf {
+    float g[3];
+    float R[9];
+    float t[3];
+    float pad;
+} arapgs_node_xf;
+
+/* ------------------------------------------------------------ context */
+int arapgs_create(int device, arapgs_ctx** out);
+void arapgs_destroy(arapgs_ctx* ctx);
+const char* arapgs_last_error(const arapgs_ctx* ctx);
+/* hip_stream is a hipStream_t (NULL = the null stream). */
+int arapgs_set_stream(arapgs_ctx* ctx, void* hip_stream);
+int arapgs_synchronize(arapgs_ctx* ctx);
+/* Library version string (static storage). */
+const char* arapgs_version(void);
+
+/* ------------------------------------------------------------ graph build */
+/* farthest_control_points_sampling + FetchFirstNodeIdx (helper.cpp:139-195).
+ * out_idx: device int[m+1]; *count_h receives the number written (min(m,N),
+ * or 2 in the reference's m==1 corner case). Synchronous. */
+int arapgs_fps(arapgs_ctx* ctx, const float* pos, int64_t n, int m, int* out_idx, int* count_h);
+
+/* DeformGraph::computeWeights / findNearestNodes (Deform.hpp:153-208) for Q
+ * query points against M rest node positions, with the reference's exact
+ * selection-sort tie semantics. out_idx u32[Q*(k+1)] (slot k is the normaliser
+ * node), out_w f32[Q*k] (may be NULL): (1 - d_j/d_k)^2 normalised, computed in
+ * FP64 and rounded. Requires M >= k+1. */
+int arapgs_knn(arapgs_ctx* ctx, const float* nodes, int m, const float* queries, int64_t q, int k,
+               uint32_t* out_idx, float* out_w);
+
+/* DeformGraph::setupEdges (Deform.hpp:84-95): out_nbr u32[M*k]. */
+int arapgs_graph_edges(arapgs_ctx* ctx, const float* nodes, int m, int k, uint32_t* out_nbr);
+
+/* GaussianView::GetEndPoints (GaussianView.cpp:4643-4668): ends f32[N*18] as
+ * [e0+, e0-, e1+, e1-, e2+, e2-] per Gaussian. */
+int arapgs_end_points(arapgs_ctx* ctx, const float* pos, const float* rot, const float* scale, int64_t n,
+                      float axis_padding, float end_coeff, float* ends);
+
+/* ------------------------------------------------------------ ARAP solve */
+/* Replaces the Deform constructor (Deform.hpp:414-455) for one block layout:
+ * host arrays describing the graph and the edit blocks. block_type: -1
+ * excluded/static, 0 pinned, 1 active (GaussianView.cpp:1995-2021). src_idx /
+ * src_w (M*k) are the KNN of each node's source vertex
+ * (cand_vertices[nodes[i].Vertex_index]). node_pos_h seeds the coarse
+ * aggregation of the preconditioner. */
+typedef struct arapgs_solver_desc {
+    int m, k;
+    const uint32_t* nbr_h;
+    const int* block_ptr_h;      /* nblocks+1 */
+    const int* block_nodes_h;
+    const int* block_type_h;     /* nblocks */
+    int nblocks;
+    const uint32_t* src_idx_h;
+    const float* src_w_h;
+    const float* node_pos_h;
+    float w_rot, w_reg, w_con;   /* DeformGraph weights (Deform.hpp:54-59): 1, 10, 100 */
+    float weighting_factor;      /* GaussianView::_weighting_factor (GaussianView.hpp:459) */
+    int constraints_on_center;   /* must be 0 (centre mode not supported) */
+    int n_aggregates;            /* coarse aggregates, 0 = 64 */
+    int cg_max_iters;            /* 0 = 100 */
+    float cg_tol;                /* relative residual, 0 = 1e-4 */
+} arapgs_solver_desc;
+
+int arapgs_solver_create(arapgs_ctx* ctx, const arapgs_solver_desc* desc_h, arapgs_solver** out);
+void arapgs_solver_destroy(arapgs_solver* s);
+/* n_free_h: free nodes; n_unknowns_h = 12 * n_free. */
+int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h);
+
+/* Deform::real_time_deform + optimize (Deform.cpp:77-169): GN from identity,
+ * `iters` iterations at most, stop rule |h| < (|x|+1e-6)1e-6 when stop_rule.
+ * node_pos f32[M*3] current node positions, aims f32[M*3].
+ * x_out f32[12*n_free] (optional), energies f64[iters+1] (energy at the start
+ * of each iteration; entry iters_done is the final energy), iters_done int[1].
+ * All outputs are device buffers; nothing is read back. */
+int arapgs_gn_solve(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, int iters,
+                    int stop_rule, float* x_out, double* energies, int* iters_done);
+
+/* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,
+ * Deform.cpp:378-581). energy: device f64[1]. */
+int arapgs_gn_energy(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
+                     double* energy);
+
+/* DeformGraph::putFreeInputs (Deform.hpp:140-151): node table from the solved
+ * unknowns (identity for non-free nodes). */
+int arapgs_put_free_inputs(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* x,
+                           arapgs_node_xf* table);
+
+/* ------------------------------------------------------------ deformation */
+/* DeformGraph::predict_mesh / predict_samples (Deform.hpp:230-268):
+ * out[i] = sum_j w_j (R_j (p_i - g_j) + t_j + g_j); skip (u8[P], optional)
+ * copies p_i unchanged. Used for samples (UpdatePositionforSamples,
+ * GaussianView.cpp:3060-3077) and nodes (GaussianView.cpp:3046-3056). */
+int arapgs_blend_points(arapgs_ctx* ctx, const float* pts, int64_t p, const uint32_t* idx, const float* w, int k,
+                        const arapgs_node_xf* table, int m, const uint8_t* skip, float* out);
+
+/* UpdatePosition's end-point blend + UpdateAsSixPointsWithdrawBad
+ * (GaussianView.cpp:3021-3043, 3081-3166), fused: ends (N*18) are blended in
+ * place with ends_idx u32[N*6*k] / ends_w f32[N*6*k]; for non-static Gaussians
+ * pos, rot, scale and sh are rewritten (polar in FP64). scale0 = backup scales. */
+int arapgs_six_point_update(arapgs_ctx* ctx, int64_t n, int k, const arapgs_node_xf* table, int m,
+                            const uint32_t* ends_idx, const float* ends_w, float* ends, const float* scale0,
+                            const uint8_t* is_static, float axis_padding, float end_coeff, float* pos, float* rot,
+                            float* scale, float* sh);
+
+/* FastUpdateSamplesSH (GaussianView.cpp:3169-3186): node quaternions
+ * q f32[M*4] (w,x,y,z) from the table's rotations (orthogonal polar factor). */
+int arapgs_node_quats(arapgs_ctx* ctx, const arapgs_node_xf* table, int m, float* q);
+
+/* RotateSamplesSHsCUDA / RotateSHs (cudakdtree.cuh:25, cudakdtree.cu:201-222):
+ * per non-static sample, running slerp of node quaternions, then rotation of
+ * the 48-float feature in place. */
+int arapgs_sample_sh(arapgs_ctx* ctx, int64_t s, int k, const float* w, const uint32_t* idx, const float* node_q,
+                     const uint8_t* is_static, float* feat);
+
+/* ------------------------------------------------------------ test hooks */
+/* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the
+ * GaussianView.cpp:3134-3156 sign wrapper); R9 f32[n*9] row-major. */
+int arapgs_sh_rotate(arapgs_ctx* ctx, int64_t n, const float* r9, float* sh);
+/* Polar decomposition M = R S in FP64 (helper.cpp:429-440): m9, r9 f64[n*9]
+ * row-major, sdiag f64[n*3] = diag(S). */
+int arapgs_polar(arapgs_ctx* ctx, int64_t n, const double* m9, double* r9, double* sdiag);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARAPGS_H */

@@ -1,0 +1,45 @@
+"""Small seeded scenes shared by the CPU and GPU tests (built with the oracle)."""
+import numpy as np
+
+import oracle as O
+from arapgs_amd import scenes as S
+
+
+def small_scene(n=20000, n_nodes=128, K=8, seed=0, kind="bar"):
+    sc = S.bar_scene(seed, n=n, n_nodes=n_nodes) if kind == "bar" else S.capsule_scene(seed, n=n, n_nodes=n_nodes)
+    g = sc.gaussians
+    mn, mx, step = O.scene_aabb(g.pos, 64)
+    g = g.take(O.grid_reorder(g.pos, mn, step, 64))
+    node_idx = O.fps(g.pos, n_nodes)
+    nodes = g.pos[node_idx].copy()
+    nbr = O.graph_edges(nodes, K)
+    nidx, nw = O.knn_weights(nodes, nodes, K)
+    blocks, types = S.blocks_for(sc, nodes)
+    ends = O.end_points(g.pos, g.rot, g.scale)
+    eidx, ew = O.knn_weights(nodes, ends.reshape(-1, 3), K)
+    return dict(scene=sc, g=g, nodes=nodes, node_idx=node_idx, nbr=nbr, src_idx=nidx[:, :K], src_w=nw,
+                blocks=blocks, types=types, ends=ends, ends_idx=eidx[:, :K], ends_w=ew, K=K)
+
+
+def random_node_xf(M, rng, rot_deg=10.0, trans=0.05):
+    """Random near-rigid node transforms: rot9 column-major (M,9) and t (M,3)."""
+    rot9 = np.zeros((M, 9))
+    for i in range(M):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        a = np.radians(rot_deg) * rng.normal()
+        Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+        R = np.eye(3) + np.sin(a) * Kx + (1 - np.cos(a)) * Kx @ Kx
+        R = R @ (np.eye(3) + 0.02 * rng.normal(size=(3, 3)))  # not exactly orthogonal, like GN output
+        rot9[i] = R.T.reshape(9)  # column-major
+    return rot9, trans * rng.normal(size=(M, 3))
+
+
+def table_from(nodes, rot9, t3):
+    """arapgs_node_xf rows (g, R row-major, t, pad) as float32 (M,16)."""
+    M = nodes.shape[0]
+    T = np.zeros((M, 16), np.float32)
+    T[:, 0:3] = nodes
+    T[:, 3:12] = rot9.reshape(M, 3, 3).transpose(0, 2, 1).reshape(M, 9)  # column-major -> row-major
+    T[:, 12:15] = t3
+    return T

@@ -1,0 +1,187 @@
+"""GPU parity: graph build and per-frame deformation through the C ABI vs the oracle.
+
+Bars: KNN indices / FPS indices / weights bit-exact; deformed means and
+covariances within 1e-5 relative; SH within 1e-5 of the coefficient scale.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import random_node_xf, small_scene, table_from
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return small_scene(n=20000, n_nodes=128)
+
+
+def test_fps_exact(ctx):
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-1, 1, size=(20000, 3)).astype(np.float32)
+    pos[100] = pos[200]  # duplicates do not change the argmax rule
+    for m in (1, 2, 64, 300):
+        ref = O.fps(pos, m)
+        got = H(ctx.fps(T(pos), m))
+        assert np.array_equal(ref, got), f"m={m}"
+
+
+@pytest.mark.parametrize("k", [8, 10, 1])
+def test_knn_exact_random(ctx, k):
+    rng = np.random.default_rng(k)
+    nodes = rng.uniform(-1, 1, size=(257, 3)).astype(np.float32)
+    q = rng.uniform(-1.2, 1.2, size=(30000, 3)).astype(np.float32)
+    ridx, rw = O.knn_weights(nodes, q, k)
+    gidx, gw = ctx.knn(T(nodes), T(q), k)
+    assert np.array_equal(ridx.astype(np.int64), H(gidx).astype(np.int64))
+    assert np.array_equal(rw.astype(np.float32), H(gw))
+
+
+def test_knn_exact_ties(ctx):
+    """Duplicated nodes and lattice-symmetric queries: the swap-history tie rule."""
+    g = np.stack(np.meshgrid(*[np.arange(5, dtype=np.float32)] * 3, indexing="ij"), -1).reshape(-1, 3) * 0.25
+    nodes = np.concatenate([g, g[:20], g[7:9]]).astype(np.float32)  # exact duplicates
+    rng = np.random.default_rng(0)
+    q = np.concatenate([g, g + 0.125, (g[:50] + g[1:51]) * 0.5,
+                        rng.uniform(0, 1, size=(2000, 3))]).astype(np.float32)
+    for k in (8, 12):
+        ridx, rw = O.knn_weights(nodes, q, k)
+        gidx, gw = ctx.knn(T(nodes), T(q), k)
+        assert np.array_equal(ridx.astype(np.int64), H(gidx).astype(np.int64))
+        both_nan = np.isnan(rw.astype(np.float32)) & np.isnan(H(gw))
+        assert np.array_equal(np.where(both_nan, 0, rw.astype(np.float32)), np.where(both_nan, 0, H(gw)))
+
+
+def test_graph_edges_exact(ctx, scene):
+    got = H(ctx.graph_edges(T(scene["nodes"]), 8))
+    assert np.array_equal(got.astype(np.uint32), scene["nbr"])
+
+
+def test_end_points(ctx, scene):
+    g = scene["g"]
+    got = H(ctx.end_points(T(g.pos), T(g.rot), T(g.scale)))
+    assert np.array_equal(got, scene["ends"])  # same float operations, no contraction
+
+
+def test_polar_fp64(ctx):
+    rng = np.random.default_rng(1)
+    Ms = rng.normal(size=(500, 3, 3))
+    Ms[:100] = np.diag([100.0, 5.0, 1.0])  # RunTestVKVt matrices (helper.cpp:869-920)
+    Ms[100:200] = [[100, 0.1, 0], [0, 5, 0], [0, 0, 1]]
+    Ms[200:300] = [[9, -7, 2], [-6, 3, 4], [5, 1, -8]]
+    R, S = ctx.polar(T(Ms.reshape(-1, 9)))
+    R, S = H(R).reshape(-1, 3, 3), H(S)
+    for i in range(0, 500, 7):
+        Rr, Sr = O.polar(Ms[i])
+        np.testing.assert_allclose(R[i], Rr, atol=1e-12)
+        np.testing.assert_allclose(S[i], np.diag(Sr), atol=1e-10 * max(1, np.abs(Sr).max()))
+
+
+def test_sh_rotate(ctx):
+    rng = np.random.default_rng(2)
+    n = 256
+    q = rng.normal(size=(n, 4)).astype(np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[0] = np.array([0.992198, -2.06755e-09, 0.124675, -1.99438e-09], np.float32)  # Test_SH_Rotation (helper.cpp:925)
+    R = np.stack([O.quat_to_mat(qq) for qq in q]).astype(np.float32)
+    sh = rng.normal(0, 0.3, size=(n, 48)).astype(np.float32)
+    ref = np.stack([O.sh_rotate(R[i], sh[i], True)[0] for i in range(n)])
+    shd = T(sh)
+    ctx.sh_rotate(T(R.reshape(n, 9)), shd)
+    np.testing.assert_allclose(H(shd), ref, atol=2e-6)
+
+
+def f32(a):
+    """Round to float and back: the inputs the device sees (float weights / transforms)."""
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+def test_blend_points(ctx, scene):
+    rng = np.random.default_rng(4)
+    M = scene["nodes"].shape[0]
+    rot9, t3 = random_node_xf(M, rng)
+    pts = scene["g"].pos
+    idx, w = O.knn_weights(scene["nodes"], pts, 8)
+    # float-rounded weights/transforms: the device then performs the reference's
+    # exact arithmetic (double terms, float accumulator), so the result is exact
+    ref = O.blend(pts, idx[:, :8], f32(w), scene["nodes"], f32(rot9), f32(t3))
+    tab = T(table_from(scene["nodes"], rot9, t3))
+    got = H(ctx.blend_points(T(pts), T(idx[:, :8].astype(np.int32)), T(w.astype(np.float32)), tab))
+    assert np.array_equal(got, ref)
+    # against the full-precision FP64 inputs of the reference: 1e-5 of the extent
+    ref64 = O.blend(pts, idx[:, :8], w, scene["nodes"], rot9, t3)
+    np.testing.assert_allclose(got, ref64, rtol=0, atol=1e-5 * 2.0)
+
+
+def _cov(rot, scale):
+    out = []
+    for q, s in zip(rot, scale):
+        R = O.quat_to_mat(q / np.linalg.norm(q)).astype(np.float64)
+        out.append(R @ np.diag(s.astype(np.float64) ** 2) @ R.T)
+    return np.stack(out)
+
+
+def test_six_point_update(ctx, scene):
+    rng = np.random.default_rng(5)
+    M = scene["nodes"].shape[0]
+    rot9, t3 = random_node_xf(M, rng)
+    g = scene["g"]
+    N = g.n
+    K = 8
+    is_static = np.zeros(N, np.uint8)
+    is_static[::7] = 1
+    ends = scene["ends"]
+    ref_ends = O.blend(ends.reshape(-1, 3), scene["ends_idx"], f32(scene["ends_w"]), scene["nodes"], f32(rot9),
+                       f32(t3)).reshape(N, 6, 3)
+    rpos, rrot, rscale, rsh = O.six_point(ref_ends, g.scale, is_static, g.pos, g.rot, g.scale, g.sh)
+    tab = T(table_from(scene["nodes"], rot9, t3))
+    ends_d = T(ends.copy())
+    pos_d, rot_d, scale_d, sh_d = T(g.pos.copy()), T(g.rot.copy()), T(g.scale.copy()), T(g.sh.copy())
+    ctx.six_point_update(tab, T(scene["ends_idx"].astype(np.int32)), T(scene["ends_w"].astype(np.float32)), ends_d,
+                         T(g.scale.copy()), T(is_static), pos_d, rot_d, scale_d, sh_d)
+    ext = 2.0  # scene extent for the absolute position tolerance
+    assert np.array_equal(H(ends_d), ref_ends)
+    np.testing.assert_allclose(H(pos_d), rpos, rtol=0, atol=1e-6 * ext)
+    st = is_static.astype(bool)
+    assert np.array_equal(H(rot_d)[st], g.rot[st]) and np.array_equal(H(sh_d)[st], g.sh[st])
+    cg, cr = _cov(H(rot_d), H(scale_d)), _cov(rrot, rscale)
+    rel = np.linalg.norm(cg - cr, axis=(1, 2)) / np.linalg.norm(cr, axis=(1, 2))
+    assert rel.max() < 1e-5, rel.max()
+    np.testing.assert_allclose(H(scale_d), rscale, rtol=1e-5)
+    scale_sh = np.maximum(1.0, np.abs(rsh).max(axis=1, keepdims=True))
+    assert (np.abs(H(sh_d) - rsh) / scale_sh).max() < 1e-5
+
+
+def test_node_quats_and_sample_sh(ctx, scene):
+    rng = np.random.default_rng(6)
+    M = scene["nodes"].shape[0]
+    rot9, t3 = random_node_xf(M, rng, rot_deg=20.0)
+    rq = O.node_quats(rot9)
+    tab = T(table_from(scene["nodes"], rot9, t3))
+    gq = H(ctx.node_quats(tab))
+    # quaternion sign is free: compare rotation matrices
+    for i in range(M):
+        np.testing.assert_allclose(O.quat_to_mat(gq[i]), O.quat_to_mat(rq[i]), atol=2e-6)
+    S = 5000
+    samples = rng.uniform(-2, 2, size=(S, 3)).astype(np.float32) * np.array([1, 0.125, 0.125], np.float32)
+    sidx, sw = O.knn_weights(scene["nodes"], samples, 8)
+    sw32 = sw.astype(np.float32)
+    feat = rng.normal(0, 0.2, size=(S, 48)).astype(np.float32)
+    st = (rng.uniform(size=S) < 0.1).astype(np.uint8)
+    ref = O.sample_sh(sw32, sidx[:, :8], rq, st, feat)
+    fd = T(feat.copy())
+    ctx.sample_sh(T(sw32), T(sidx[:, :8].astype(np.int32)), T(rq), T(st), fd)
+    np.testing.assert_allclose(H(fd), ref, atol=1e-5)
