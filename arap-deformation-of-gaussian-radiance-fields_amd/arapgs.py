@@ -29,6 +29,16 @@ class SolverDesc(C.Structure):
                 ("cg_tol", C.c_float)]
 
 
+class RenderDesc(C.Structure):
+    _fields_ = [("P", C.c_int), ("D", C.c_int), ("M", C.c_int), ("background", P), ("width", C.c_int),
+                ("height", C.c_int), ("means3D", P), ("shs", P), ("colors_precomp", P), ("opacities", P),
+                ("scales", P), ("scale_modifier", C.c_float), ("rotations", P), ("cov3D_precomp", P),
+                ("viewmatrix", P), ("projmatrix", P), ("cam_pos", P), ("tan_fovx", C.c_float),
+                ("tan_fovy", C.c_float), ("prefiltered", C.c_int), ("out_color", P), ("surface", P), ("radii", P),
+                ("rects", P), ("boxmin_h", P), ("boxmax_h", P), ("to_ortho", C.c_int), ("ortho_scale", C.c_float),
+                ("tile", C.c_int)]
+
+
 EXPORTS = {
     "arapgs_create": ([C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_destroy": ([P], None),
@@ -51,6 +61,8 @@ EXPORTS = {
                                 C.c_int),
     "arapgs_node_quats": ([P, P, C.c_int, P], C.c_int),
     "arapgs_sample_sh": ([P, C.c_int64, C.c_int, P, P, P, P, P], C.c_int),
+    "arapgs_render": ([P, C.POINTER(RenderDesc), C.POINTER(C.c_int)], C.c_int),
+    "arapgs_render_debug": ([P, P, P, C.c_int64, P, C.c_int64], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
 }
@@ -179,6 +191,39 @@ class Context:
     def sample_sh(self, w, idx, node_q, is_static, feat):
         self._check(lib().arapgs_sample_sh(self.h, feat.shape[0], idx.shape[-1], ptr(w), ptr(idx), ptr(node_q),
                                            ptr(is_static), ptr(feat)))
+
+    def render(self, means, shs, opac, scales, rots, view, proj, campos, tanfovx, tanfovy, W, H, bg, D=3,
+               scale_mod=1.0, fast_culling=True, boxmin=None, boxmax=None, tile=16, out=None, want_surface=False,
+               want_radii=False):
+        """Rasterizer::forward (GaussianView.cpp:1099-1128). view/proj/campos/bg are device tensors."""
+        t = self._torch
+        dev = means.device
+        P_ = means.shape[0]
+        out = t.empty((3, H, W), dtype=t.float32, device=dev) if out is None else out
+        surface = t.zeros(P_, dtype=t.int32, device=dev) if want_surface else None
+        radii = t.zeros(P_, dtype=t.int32, device=dev) if want_radii else None
+        rects = t.zeros((P_, 2), dtype=t.int32, device=dev) if fast_culling else None
+        import numpy as np
+        self._box = None
+        bmin = bmax = None
+        if boxmin is not None:
+            self._box = (np.ascontiguousarray(boxmin, np.float32), np.ascontiguousarray(boxmax, np.float32))
+            bmin, bmax = self._box[0].ctypes.data_as(P), self._box[1].ctypes.data_as(P)
+        d = RenderDesc(P_, D, 16, ptr(bg), W, H, ptr(means), ptr(shs), None, ptr(opac), ptr(scales), scale_mod,
+                       ptr(rots), None, ptr(view), ptr(proj), ptr(campos), tanfovx, tanfovy, 0, ptr(out), ptr(surface),
+                       ptr(radii), ptr(rects), bmin, bmax, 0, 1.0, tile)
+        n = C.c_int(0)
+        self._check(lib().arapgs_render(self.h, C.byref(d), C.byref(n)))
+        return dict(image=out, num_rendered=n.value, surface=surface, radii=radii, rects=rects)
+
+    def render_debug(self, n, ntiles):
+        import numpy as np
+        keys = np.zeros(n, np.uint64)
+        vals = np.zeros(n, np.uint32)
+        ranges = np.zeros((ntiles, 2), np.uint32)
+        self._check(lib().arapgs_render_debug(self.h, keys.ctypes.data_as(P), vals.ctypes.data_as(P), n,
+                                              ranges.ctypes.data_as(P), ntiles))
+        return keys, vals, ranges
 
     def sh_rotate(self, R9, sh):
         self._check(lib().arapgs_sh_rotate(self.h, sh.shape[0], ptr(R9), ptr(sh)))

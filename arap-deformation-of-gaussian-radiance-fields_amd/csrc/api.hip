@@ -9,7 +9,9 @@
 
 #include "../../include/arapgs.h"
 #include "common.hpp"
+#include "raster.hpp"
 #include "solver.hpp"
+#include "sort.hpp"
 
 using namespace arapgs;
 
@@ -29,6 +31,20 @@ struct arapgs_ctx {
     unsigned long long* fps_best = nullptr;
     uint32_t* edge_tmp = nullptr;
     size_t edge_tmp_cap = 0;
+    // rasteriser workspaces
+    void* r_geo = nullptr;
+    size_t r_geo_cap = 0;
+    void* r_scan = nullptr;
+    size_t r_scan_cap = 0;
+    void* r_bin = nullptr;          // keys, vals, keys_alt, vals_alt
+    size_t r_bin_cap = 0;
+    void* r_sort = nullptr;
+    size_t r_sort_cap = 0;
+    void* r_ranges = nullptr;
+    size_t r_ranges_cap = 0;
+    uint32_t* r_total = nullptr;
+    uint32_t last_rendered = 0;
+    int last_tiles = 0;
 };
 
 struct arapgs_solver {
@@ -97,7 +113,8 @@ int arapgs_create(int device, arapgs_ctx** out) {
     if ((e = hipMalloc(&c->knn_count, sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&c->fps_partial, sizeof(unsigned long long) * FPS_BLOCKS)) != hipSuccess ||
         (e = hipMalloc(&c->fps_counter, sizeof(unsigned int))) != hipSuccess ||
-        (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess) {
+        (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&c->r_total, sizeof(uint32_t))) != hipSuccess) {
         delete c;
         return -5;
     }
@@ -111,7 +128,8 @@ void arapgs_destroy(arapgs_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
-                    (void*)c->fps_best, (void*)c->edge_tmp})
+                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges,
+                    (void*)c->r_total})
         if (p) (void)hipFree(p);
     delete c;
 }
@@ -355,6 +373,82 @@ int arapgs_polar(arapgs_ctx* c, int64_t n, const double* m9, double* r9, double*
     CHECK_CTX(c);
     if (!m9 || !r9 || !sdiag || n < 0) return fail(c, -1, "polar: bad arguments");
     return check_launch(c, launch_polar(n, m9, r9, sdiag, c->stream), "polar");
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int arapgs_render(arapgs_ctx* c, const arapgs_render_desc* d, int* num_rendered_h) {
+    CHECK_CTX(c);
+    if (!d || !d->background || !d->means3D || !d->shs || !d->opacities || !d->scales || !d->rotations ||
+        !d->viewmatrix || !d->projmatrix || !d->cam_pos || !d->out_color || d->P < 0 || d->width <= 0 ||
+        d->height <= 0)
+        return fail(c, -1, "render: bad arguments");
+    if (d->colors_precomp || d->cov3D_precomp) return fail(c, -1, "render: precomputed colours/covariances are not supported");
+    if (d->prefiltered || d->to_ortho) return fail(c, -1, "render: prefiltered / orthographic modes are not supported");
+    if (d->D < 0 || d->D > 3) return fail(c, -1, "render: SH degree must be 0..3");
+    const int tile = d->tile ? d->tile : 16;
+    if (tile != 16 && tile != 8) return fail(c, -1, "render: tile must be 8 or 16");
+    (void)hipSetDevice(c->device);
+    RasterParams rp;
+    rp.P = d->P; rp.D = d->D; rp.W = d->width; rp.H = d->height; rp.tile = tile;
+    rp.gx = (d->width + tile - 1) / tile; rp.gy = (d->height + tile - 1) / tile;
+    rp.bg = d->background; rp.means = d->means3D; rp.shs = d->shs; rp.opac = d->opacities; rp.scales = d->scales;
+    rp.rots = d->rotations; rp.view = d->viewmatrix; rp.proj = d->projmatrix; rp.campos = d->cam_pos;
+    rp.scale_mod = d->scale_modifier; rp.tanfovx = d->tan_fovx; rp.tanfovy = d->tan_fovy;
+    rp.focal_x = (float)d->width / (2.0f * d->tan_fovx);
+    rp.focal_y = (float)d->height / (2.0f * d->tan_fovy);
+    rp.use_rects = d->rects != nullptr;
+    rp.use_box = (d->boxmin_h && d->boxmax_h) ? 1 : 0;
+    for (int i = 0; i < 3; i++) {
+        rp.boxmin[i] = rp.use_box ? d->boxmin_h[i] : 0.0f;
+        rp.boxmax[i] = rp.use_box ? d->boxmax_h[i] : 0.0f;
+    }
+    int rc = grow(c, &c->r_geo, &c->r_geo_cap, raster_geo_bytes(std::max(d->P, 1)));
+    if (!rc) rc = grow(c, &c->r_scan, &c->r_scan_cap, sizeof(uint32_t) * scan_workspace_words(std::max(d->P, 1)));
+    const int ntiles = rp.gx * rp.gy;
+    if (!rc) rc = grow(c, &c->r_ranges, &c->r_ranges_cap, sizeof(uint2) * (size_t)ntiles);
+    if (rc) return rc;
+    rc = check_launch(c, raster_preprocess(rp, c->r_geo, (uint32_t*)c->r_scan, c->r_total, d->radii, d->rects,
+                                           c->stream), "render preprocess");
+    if (rc) return rc;
+    uint32_t n = 0;
+    hipError_t e = hipMemcpyAsync(&n, c->r_total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "render");
+    if (n > 0x7fffffffu) return fail(c, -1, "render: too many tile instances (%u)", n);
+    const size_t nn = std::max<size_t>(n, 1);
+    rc = grow(c, &c->r_bin, &c->r_bin_cap, nn * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 1024);
+    if (!rc) rc = grow(c, &c->r_sort, &c->r_sort_cap, sizeof(uint32_t) * radix_workspace_words((int64_t)nn));
+    if (rc) return rc;
+    char* b = (char*)c->r_bin;
+    uint64_t* keys = (uint64_t*)b;
+    uint64_t* keys_alt = (uint64_t*)(b + nn * 8);
+    uint32_t* vals = (uint32_t*)(b + nn * 16);
+    uint32_t* vals_alt = (uint32_t*)(b + nn * 20);
+    rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, keys, vals, keys_alt, vals_alt,
+                                              (uint32_t*)c->r_sort, (uint2*)c->r_ranges, d->out_color, d->surface,
+                                              c->stream), "render");
+    if (rc) return rc;
+    c->last_rendered = n;
+    c->last_tiles = ntiles;
+    if (num_rendered_h) *num_rendered_h = (int)n;
+    return 0;
+}
+
+int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64_t cap, uint32_t* ranges_h,
+                        int64_t ranges_cap) {
+    CHECK_CTX(c);
+    const size_t nn = std::max<size_t>(c->last_rendered, 1);
+    const int64_t n = std::min<int64_t>(cap, c->last_rendered);
+    char* b = (char*)c->r_bin;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && keys_h && n > 0) e = hipMemcpy(keys_h, b, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vals_h && n > 0) e = hipMemcpy(vals_h, b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    const int64_t nr = std::min<int64_t>(ranges_cap, c->last_tiles);
+    if (e == hipSuccess && ranges_h && nr > 0) e = hipMemcpy(ranges_h, c->r_ranges, sizeof(uint2) * nr, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? 0 : hip_fail(c, e, "render_debug");
 }
 
 }  // extern "C"

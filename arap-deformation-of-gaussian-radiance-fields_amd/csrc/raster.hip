@@ -1,0 +1,322 @@
+// 3D Gaussian Splatting forward rasteriser for gfx950 (the path behind
+// CudaRasterizer::Rasterizer::forward, called at GaussianView.cpp:1099-1128).
+//
+//   preprocess  one thread per Gaussian: view/clip transform, EWA 2D covariance,
+//               conic, 3-sigma extent, tile rectangle, view-dependent SH colour
+//   bin         exclusive scan of tiles touched, (tile << 32 | depth bits) keys,
+//               stable radix sort, per-tile [start, end) ranges
+//   blend       one 16x16 workgroup per tile; Gaussians staged through LDS in
+//               256-wide batches, front-to-back alpha compositing with early
+//               termination; planar CHW float output
+//
+// Everything that feeds a key or a tile rectangle is evaluated with single
+// float operations in a fixed order and no contraction, so keys, tile ranges
+// and sort order are reproducible bit for bit on the CPU.
+#pragma clang fp contract(off)
+#include "common.hpp"
+#include "raster.hpp"
+#include "sort.hpp"
+
+namespace arapgs {
+
+namespace {
+
+__constant__ float kShC0 = 0.28209479177387814f;
+__constant__ float kShC1 = 0.4886025119029199f;
+
+struct Geo {
+    float* depth;      // P
+    float2* xy;        // P
+    float4* conic;     // P  (a, b, c of the inverse 2D covariance, opacity)
+    float4* rgb;       // P  (r, g, b, pad)
+    int4* rect;        // P  tile rectangle [x0, y0, x1, y1)
+    uint32_t* touched; // P
+};
+
+__global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo geo, int* __restrict__ radii,
+                                                         int* __restrict__ rects_out) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rp.P) return;
+    geo.touched[g] = 0;
+    if (radii) radii[g] = 0;
+    const float px = rp.means[3 * g], py = rp.means[3 * g + 1], pz = rp.means[3 * g + 2];
+    if (rp.use_box && (px < rp.boxmin[0] || py < rp.boxmin[1] || pz < rp.boxmin[2] || px > rp.boxmax[0] ||
+                       py > rp.boxmax[1] || pz > rp.boxmax[2]))
+        return;
+    const float* V = rp.view;
+    const float* Pm = rp.proj;
+    const float vx = V[0] * px + V[4] * py + V[8] * pz + V[12];
+    const float vy = V[1] * px + V[5] * py + V[9] * pz + V[13];
+    const float vz = V[2] * px + V[6] * py + V[10] * pz + V[14];
+    if (vz <= 0.2f) return;
+    const float hx = Pm[0] * px + Pm[4] * py + Pm[8] * pz + Pm[12];
+    const float hy = Pm[1] * px + Pm[5] * py + Pm[9] * pz + Pm[13];
+    const float hw = Pm[3] * px + Pm[7] * py + Pm[11] * pz + Pm[15];
+    const float pw = 1.0f / (hw + 0.0000001f);
+    const float ndx = hx * pw, ndy = hy * pw;
+    const float qw = rp.rots[4 * g], qx = rp.rots[4 * g + 1], qy = rp.rots[4 * g + 2], qz = rp.rots[4 * g + 3];
+    float R[3][3];
+    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+    R[0][1] = 2.0f * (qx * qy - qw * qz);
+    R[0][2] = 2.0f * (qx * qz + qw * qy);
+    R[1][0] = 2.0f * (qx * qy + qw * qz);
+    R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+    R[1][2] = 2.0f * (qy * qz - qw * qx);
+    R[2][0] = 2.0f * (qx * qz - qw * qy);
+    R[2][1] = 2.0f * (qy * qz + qw * qx);
+    R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+    float s2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float s = rp.scale_mod * rp.scales[3 * g + k];
+        s2[k] = s * s;
+    }
+    float S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            S[i][j] = R[i][0] * s2[0] * R[j][0] + R[i][1] * s2[1] * R[j][1] + R[i][2] * s2[2] * R[j][2];
+    const float limx = 1.3f * rp.tanfovx, limy = 1.3f * rp.tanfovy;
+    const float txtz = vx / vz, tytz = vy / vz;
+    const float tx = fminf(limx, fmaxf(-limx, txtz)) * vz;
+    const float ty = fminf(limy, fmaxf(-limy, tytz)) * vz;
+    const float j00 = rp.focal_x / vz, j02 = -(rp.focal_x * tx) / (vz * vz);
+    const float j11 = rp.focal_y / vz, j12 = -(rp.focal_y * ty) / (vz * vz);
+    float T[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        T[0][c] = j00 * V[c * 4 + 0] + j02 * V[c * 4 + 2];
+        T[1][c] = j11 * V[c * 4 + 1] + j12 * V[c * 4 + 2];
+    }
+    float TS[2][3];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) TS[r][c] = T[r][0] * S[0][c] + T[r][1] * S[1][c] + T[r][2] * S[2][c];
+    const float ca = TS[0][0] * T[0][0] + TS[0][1] * T[0][1] + TS[0][2] * T[0][2] + 0.3f;
+    const float cb = TS[0][0] * T[1][0] + TS[0][1] * T[1][1] + TS[0][2] * T[1][2];
+    const float cc = TS[1][0] * T[1][0] + TS[1][1] * T[1][1] + TS[1][2] * T[1][2] + 0.3f;
+    const float det = ca * cc - cb * cb;
+    if (det == 0.0f) return;
+    const float dinv = 1.0f / det;
+    const float mid = 0.5f * (ca + cc);
+    const float disc = __builtin_sqrtf(fmaxf(0.1f, mid * mid - det));
+    const float l1 = mid + disc, l2 = mid - disc;
+    const int rad = (int)ceilf(3.0f * __builtin_sqrtf(fmaxf(l1, l2)));
+    const float ix = ((ndx + 1.0f) * (float)rp.W - 1.0f) * 0.5f;
+    const float iy = ((ndy + 1.0f) * (float)rp.H - 1.0f) * 0.5f;
+    int ex = rad, ey = rad;
+    if (rp.use_rects) {
+        ex = (int)ceilf(3.0f * __builtin_sqrtf(ca));
+        ey = (int)ceilf(3.0f * __builtin_sqrtf(cc));
+    }
+    const float tf = (float)rp.tile;
+    const int r0x = min(rp.gx, max(0, (int)((ix - (float)ex) / tf)));
+    const int r0y = min(rp.gy, max(0, (int)((iy - (float)ey) / tf)));
+    const int r1x = min(rp.gx, max(0, (int)((ix + (float)ex + tf - 1.0f) / tf)));
+    const int r1y = min(rp.gy, max(0, (int)((iy + (float)ey + tf - 1.0f) / tf)));
+    const int area = (r1x - r0x) * (r1y - r0y);
+    if (area == 0) return;
+    // view-dependent colour (degree D)
+    float dx = px - rp.campos[0], dy = py - rp.campos[1], dz = pz - rp.campos[2];
+    const float dn = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    dx /= dn; dy /= dn; dz /= dn;
+    const float* sh = rp.shs + 48 * (int64_t)g;
+    float col[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        float r = kShC0 * sh[ch];
+        if (rp.D > 0) {
+            r = r - kShC1 * dy * sh[3 + ch] + kShC1 * dz * sh[6 + ch] - kShC1 * dx * sh[9 + ch];
+            if (rp.D > 1) {
+                const float xx = dx * dx, yy = dy * dy, zz = dz * dz, xy2 = dx * dy, yz = dy * dz, xz = dx * dz;
+                r = r + 1.0925484305920792f * xy2 * sh[12 + ch] + -1.0925484305920792f * yz * sh[15 + ch] +
+                    0.31539156525252005f * (2.0f * zz - xx - yy) * sh[18 + ch] +
+                    -1.0925484305920792f * xz * sh[21 + ch] + 0.5462742152960396f * (xx - yy) * sh[24 + ch];
+                if (rp.D > 2) {
+                    r = r + -0.5900435899266435f * dy * (3.0f * xx - yy) * sh[27 + ch] +
+                        2.890611442640554f * xy2 * dz * sh[30 + ch] +
+                        -0.4570457994644658f * dy * (4.0f * zz - xx - yy) * sh[33 + ch] +
+                        0.3731763325901154f * dz * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[36 + ch] +
+                        -0.4570457994644658f * dx * (4.0f * zz - xx - yy) * sh[39 + ch] +
+                        1.445305721320277f * dz * (xx - yy) * sh[42 + ch] +
+                        -0.5900435899266435f * dx * (xx - 3.0f * yy) * sh[45 + ch];
+                }
+            }
+        }
+        col[ch] = fmaxf(r + 0.5f, 0.0f);
+    }
+    geo.depth[g] = vz;
+    geo.xy[g] = make_float2(ix, iy);
+    geo.conic[g] = make_float4(cc * dinv, -cb * dinv, ca * dinv, rp.opac[g]);
+    geo.rgb[g] = make_float4(col[0], col[1], col[2], 0.0f);
+    geo.rect[g] = make_int4(r0x, r0y, r1x, r1y);
+    geo.touched[g] = (uint32_t)area;
+    if (radii) radii[g] = rad;
+    if (rects_out) { rects_out[2 * g] = ex; rects_out[2 * g + 1] = ey; }
+}
+
+__global__ void __launch_bounds__(256) emit_keys_kernel(int P, int gx, Geo geo, const uint32_t* __restrict__ offsets,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= P) return;
+    if (geo.touched[g] == 0) return;
+    uint32_t o = offsets[g];
+    const int4 r = geo.rect[g];
+    const uint64_t db = (uint64_t)__float_as_uint(geo.depth[g]);
+    for (int y = r.y; y < r.w; y++)
+        for (int x = r.x; x < r.z; x++) {
+            keys[o] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | db;
+            vals[o] = (uint32_t)g;
+            o++;
+        }
+}
+
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
+                              uint2* __restrict__ ranges) {
+    const uint32_t n = *n_ptr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t t = (uint32_t)(keys[i] >> 32);
+        if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != t) ranges[t].x = i;
+        if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != t) ranges[t].y = i + 1;
+    }
+}
+
+// One workgroup per tile (tile x tile threads, tile in {8, 16}); each thread
+// owns one pixel.
+template <int TILE>
+__global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
+                                                          const uint2* __restrict__ ranges, float* __restrict__ out,
+                                                          int* __restrict__ surface) {
+    constexpr int NT = TILE * TILE;
+    __shared__ float2 s_xy[NT];
+    __shared__ float4 s_co[NT];
+    __shared__ float4 s_rgb[NT];
+    __shared__ uint32_t s_id[NT];
+    const int tile = blockIdx.x;
+    const int tx = tile % rp.gx, ty = tile / rp.gx;
+    const int px = tx * TILE + (threadIdx.x % TILE), py = ty * TILE + (threadIdx.x / TILE);
+    const bool inside = px < rp.W && py < rp.H;
+    const float fxp = (float)px, fyp = (float)py;
+    const uint2 rg = ranges[tile];
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    bool done = !inside;
+    const int n = (int)(rg.y - rg.x);
+    for (int b0 = 0; b0 < n; b0 += NT) {
+        if (__syncthreads_count(done) == NT) break;
+        const int i = b0 + (int)threadIdx.x;
+        if (i < n) {
+            const uint32_t g = vals[rg.x + i];
+            s_id[threadIdx.x] = g;
+            s_xy[threadIdx.x] = geo.xy[g];
+            s_co[threadIdx.x] = geo.conic[g];
+            s_rgb[threadIdx.x] = geo.rgb[g];
+        }
+        __syncthreads();
+        const int m = min(NT, n - b0);
+        for (int j = 0; j < m && !done; j++) {
+            const float2 xy = s_xy[j];
+            const float4 co = s_co[j];
+            const float dx = xy.x - fxp, dy = xy.y - fyp;
+            const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+            if (power > 0.0f) continue;
+            const float alpha = fminf(0.99f, co.w * expf(power));
+            if (alpha < 1.0f / 255.0f) continue;
+            const float test_T = T * (1.0f - alpha);
+            if (test_T < 0.0001f) { done = true; break; }
+            const float4 c = s_rgb[j];
+            C0 += c.x * alpha * T;
+            C1 += c.y * alpha * T;
+            C2 += c.z * alpha * T;
+            if (surface) surface[s_id[j]] = 1;
+            T = test_T;
+        }
+    }
+    if (inside) {
+        const int64_t HW = (int64_t)rp.W * rp.H, o = (int64_t)py * rp.W + px;
+        out[o] = C0 + T * rp.bg[0];
+        out[HW + o] = C1 + T * rp.bg[1];
+        out[2 * HW + o] = C2 + T * rp.bg[2];
+    }
+}
+
+__global__ void copy_count_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) { *dst = *src; }
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+size_t raster_geo_bytes(int P) {
+    // depth, xy, conic, rgb, rect, touched, offsets (+ alignment slack)
+    return (size_t)P * (4 + 8 + 16 + 16 + 16 + 4 + 4) + 7 * 256;
+}
+
+static Geo carve_geo(void* base, int P) {
+    char* p = (char*)base;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    Geo g;
+    g.depth = (float*)take(4 * (size_t)P);
+    g.xy = (float2*)take(8 * (size_t)P);
+    g.conic = (float4*)take(16 * (size_t)P);
+    g.rgb = (float4*)take(16 * (size_t)P);
+    g.rect = (int4*)take(16 * (size_t)P);
+    g.touched = (uint32_t*)take(4 * (size_t)P);
+    return g;
+}
+
+static uint32_t* geo_offsets(void* base, int P) {
+    char* p = (char*)base;
+    size_t off = 0;
+    const size_t sz[6] = {4, 8, 16, 16, 16, 4};
+    for (int i = 0; i < 6; i++) off += (sz[i] * (size_t)P + 255) & ~(size_t)255;
+    return (uint32_t*)(p + off);
+}
+
+int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, int* radii,
+                      int* rects, hipStream_t st) {
+    Geo geo = carve_geo(geo_ws, rp.P);
+    if (rp.P > 0) {
+        hipLaunchKernelGGL(preprocess_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo, radii, rects);
+        HIP_RET(hipGetLastError());
+    }
+    return launch_exclusive_scan(geo.touched, rp.P, geo_offsets(geo_ws, rp.P), d_total, scan_ws, st);
+}
+
+int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
+                         uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* sort_ws,
+                         uint2* ranges, float* out, int* surface, hipStream_t st) {
+    Geo geo = carve_geo(geo_ws, rp.P);
+    const int ntiles = rp.gx * rp.gy;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(nblk(2 * (int64_t)ntiles, 256)), dim3(256), 0, st, (uint32_t*)ranges,
+                       2 * (int64_t)ntiles);
+    if (n_rendered > 0) {
+        hipLaunchKernelGGL(emit_keys_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp.P, rp.gx, geo,
+                           geo_offsets(geo_ws, rp.P), keys, vals);
+        int tile_bits = 0;
+        while ((1 << tile_bits) < ntiles) tile_bits++;
+        int rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, 32 + tile_bits, sort_ws, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
+                           keys, d_total, ranges);
+    }
+    if (rp.tile == 16)
+        hipLaunchKernelGGL(blend_kernel<16>, dim3(ntiles), dim3(256), 0, st, rp, geo, vals, ranges, out, surface);
+    else if (rp.tile == 8)
+        hipLaunchKernelGGL(blend_kernel<8>, dim3(ntiles), dim3(64), 0, st, rp, geo, vals, ranges, out, surface);
+    else
+        return -1;
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+}  // namespace arapgs

@@ -147,6 +147,59 @@ int arapgs_node_quats(arapgs_ctx* ctx, const arapgs_node_xf* table, int m, float
 int arapgs_sample_sh(arapgs_ctx* ctx, int64_t s, int k, const float* w, const uint32_t* idx, const float* node_q,
                      const uint8_t* is_static, float* feat);
 
+/* ------------------------------------------------------------ render */
+/* CudaRasterizer::Rasterizer::forward as called at GaussianView.cpp:1099-1128
+ * (positional order of that call): P Gaussians, SH degree D (M = 16
+ * coefficients per Gaussian), background bg (device f32[3]), W x H image,
+ * means3D, shs, opacities, scales (x scale_modifier), rotations (w,x,y,z),
+ * viewmatrix / projmatrix (device f32[16], column-major; projmatrix is the full
+ * world->clip transform), cam_pos (device f32[3]), tan_fovx/y.
+ * out_color: device f32[3*H*W], planar CHW (shaders/copy.frag:38-40).
+ * surface (device int[P], optional): set to 1 for every Gaussian that
+ * contributed to a pixel. radii (device int[P], optional). rects (device
+ * int[2*P], optional): when given, tile rectangles use per-axis 3-sigma extents
+ * (the "Fast culling" mode, GaussianView.cpp:1062) and the extents are written.
+ * boxmin_h / boxmax_h (host float[3], optional): crop box on the means.
+ * colors_precomp, cov3D_precomp must be NULL, prefiltered and to_ortho 0.
+ * tile: 16 (the 3DGS tile) or 8. Returns the number of (tile, Gaussian)
+ * instances in *num_rendered_h. Synchronises once (instance count). */
+typedef struct arapgs_render_desc {
+    int P, D, M;
+    const float* background;
+    int width, height;
+    const float* means3D;
+    const float* shs;
+    const float* colors_precomp;
+    const float* opacities;
+    const float* scales;
+    float scale_modifier;
+    const float* rotations;
+    const float* cov3D_precomp;
+    const float* viewmatrix;
+    const float* projmatrix;
+    const float* cam_pos;
+    float tan_fovx, tan_fovy;
+    int prefiltered;
+    float* out_color;
+    int* surface;
+    int* radii;
+    int* rects;
+    const float* boxmin_h;
+    const float* boxmax_h;
+    int to_ortho;
+    float ortho_scale;
+    int tile;
+} arapgs_render_desc;
+
+int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_rendered_h);
+
+/* Debug view of the last arapgs_render on this context: sorted keys
+ * (u64, tile << 32 | depth bits) and values (Gaussian ids) copied to host
+ * buffers of capacity cap; tile ranges (u32 pairs [start,end)) to ranges_h
+ * (capacity in tiles). Synchronous. */
+int arapgs_render_debug(arapgs_ctx* ctx, uint64_t* keys_h, uint32_t* vals_h, int64_t cap, uint32_t* ranges_h,
+                        int64_t ranges_cap);
+
 /* ------------------------------------------------------------ test hooks */
 /* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the
  * GaussianView.cpp:3134-3156 sign wrapper); R9 f32[n*9] row-major. */
