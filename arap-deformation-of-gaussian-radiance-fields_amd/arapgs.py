@@ -63,6 +63,14 @@ EXPORTS = {
     "arapgs_sample_sh": ([P, C.c_int64, C.c_int, P, P, P, P, P], C.c_int),
     "arapgs_render": ([P, C.POINTER(RenderDesc), C.POINTER(C.c_int)], C.c_int),
     "arapgs_render_debug": ([P, P, P, C.c_int64, P, C.c_int64], C.c_int),
+    "arapgs_scene_aabb": ([P, P, C.c_int64, C.c_int, P], C.c_int),
+    "arapgs_grid_reorder": ([P, P, C.c_int64, P, C.c_int, P], C.c_int),
+    "arapgs_gaussian_aabbs": ([P, P, P, P, P, C.c_int64, P], C.c_int),
+    "arapgs_grid_build": ([P, P, C.c_int64, P, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_grid_destroy": ([P], None),
+    "arapgs_grid_info": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                          C.POINTER(P)], C.c_int),
+    "arapgs_density_sample": ([P, P, P, P, P, P, P, C.c_float, P, P], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
 }
@@ -216,6 +224,36 @@ class Context:
         self._check(lib().arapgs_render(self.h, C.byref(d), C.byref(n)))
         return dict(image=out, num_rendered=n.value, surface=surface, radii=radii, rects=rects)
 
+    # ------------------------------------------------------------ grid
+    def scene_aabb(self, pos, G):
+        import numpy as np
+        out = np.zeros(7, np.float32)
+        self._check(lib().arapgs_scene_aabb(self.h, ptr(pos), pos.shape[0], G, out.ctypes.data_as(P)))
+        return out
+
+    def grid_reorder(self, pos, aabb7, G):
+        import numpy as np
+        t = self._torch
+        order = t.empty(pos.shape[0], dtype=t.int32, device=pos.device)
+        a = np.ascontiguousarray(aabb7, np.float32)
+        self._check(lib().arapgs_grid_reorder(self.h, ptr(pos), pos.shape[0], a.ctypes.data_as(P), G, ptr(order)))
+        return order
+
+    def gaussian_aabbs(self, pos, rot, scale, opacity):
+        t = self._torch
+        out = t.empty((pos.shape[0], 6), dtype=t.float32, device=pos.device)
+        self._check(lib().arapgs_gaussian_aabbs(self.h, ptr(pos), ptr(rot), ptr(scale), ptr(opacity), pos.shape[0],
+                                                ptr(out)))
+        return out
+
+    def grid_build(self, aabbs, aabb7, G, padding=1):
+        import numpy as np
+        a = np.ascontiguousarray(aabb7, np.float32)
+        h = P()
+        self._check(lib().arapgs_grid_build(self.h, ptr(aabbs), aabbs.shape[0], a.ctypes.data_as(P), G, padding,
+                                            C.byref(h)))
+        return Grid(self, h, G)
+
     def render_debug(self, n, ntiles):
         import numpy as np
         keys = np.zeros(n, np.uint64)
@@ -308,3 +346,73 @@ class Solver:
         table = node_table(self.M, node_pos.device) if table is None else table
         self.ctx._check(lib().arapgs_put_free_inputs(self.ctx.h, self.h, ptr(node_pos), ptr(x), ptr(table)))
         return table
+
+
+class Grid:
+    """arapgs_grid: per-cell Gaussian lists, valid cells and samples (device views)."""
+
+    def __init__(self, ctx, h, G):
+        self.ctx, self.h, self.G = ctx, h, G
+        n_pairs, n_valid = C.c_int64(0), C.c_int64(0)
+        pr, li, va, sa = P(), P(), P(), P()
+        lib().arapgs_grid_info(h, C.byref(n_pairs), C.byref(n_valid), C.byref(pr), C.byref(li), C.byref(va),
+                               C.byref(sa))
+        self.n_pairs, self.n_valid = n_pairs.value, n_valid.value
+        self._ptrs = (pr.value, li.value, va.value, sa.value)
+
+    def arrays(self):
+        """Copies of (prefix, list, valid, samples) on the host, via torch."""
+        import numpy as np
+        import torch
+        G3 = self.G ** 3
+        out = []
+        for p, n, dt, shp in ((self._ptrs[0], G3, np.int32, (G3,)), (self._ptrs[1], self.n_pairs, np.int32,
+                                                                      (self.n_pairs,)),
+                              (self._ptrs[2], self.n_valid, np.int32, (self.n_valid,)),
+                              (self._ptrs[3], self.n_valid * 64 * 3, np.float32, (self.n_valid * 64, 3))):
+            a = np.empty(shp, dt)
+            if n:
+                torch.cuda.synchronize()
+                rc = _hip_memcpy_d2h(a.ctypes.data, p, a.nbytes)
+                if rc != 0:
+                    raise ArapgsError(f"hipMemcpy failed ({rc})")
+            out.append(a)
+        return out
+
+    def samples_tensor(self):
+        """Device copy of the samples as a torch tensor (S, 3)."""
+        import numpy as np
+        import torch
+        return torch.from_numpy(self.arrays()[3]).cuda()
+
+    def density(self, pos, rot, scale, opacity, sh, lpf):
+        t = self.ctx._torch
+        S = self.n_valid * 64
+        feat = t.empty((S, 48), dtype=t.float32, device=pos.device)
+        opac = t.empty(S, dtype=t.float32, device=pos.device)
+        self.ctx._check(lib().arapgs_density_sample(self.ctx.h, self.h, ptr(pos), ptr(rot), ptr(scale), ptr(opacity),
+                                                    ptr(sh), lpf, ptr(feat), ptr(opac)))
+        return feat, opac
+
+    def close(self):
+        if self.h:
+            lib().arapgs_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip = None
+
+
+def _hip_memcpy_d2h(dst, src, nbytes):
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    return _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 2)  # hipMemcpyDeviceToHost

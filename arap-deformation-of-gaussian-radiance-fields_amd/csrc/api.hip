@@ -6,9 +6,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/arapgs.h"
 #include "common.hpp"
+#include "grid.hpp"
 #include "raster.hpp"
 #include "solver.hpp"
 #include "sort.hpp"
@@ -449,6 +451,198 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
     const int64_t nr = std::min<int64_t>(ranges_cap, c->last_tiles);
     if (e == hipSuccess && ranges_h && nr > 0) e = hipMemcpy(ranges_h, c->r_ranges, sizeof(uint2) * nr, hipMemcpyDeviceToHost);
     return e == hipSuccess ? 0 : hip_fail(c, e, "render_debug");
+}
+
+}  // extern "C"
+
+struct arapgs_grid {
+    int G = 0;
+    int64_t n_pairs = 0, n_valid = 0;
+    int* prefix = nullptr;
+    int* list = nullptr;
+    int* valid = nullptr;
+    float* samples = nullptr;
+    int device = 0;
+};
+
+extern "C" {
+
+int arapgs_scene_aabb(arapgs_ctx* c, const float* pos, int64_t n, int grid_num, float* out7_h) {
+    CHECK_CTX(c);
+    if (!pos || n <= 0 || grid_num <= 0 || !out7_h) return fail(c, -1, "scene_aabb: bad arguments");
+    (void)hipSetDevice(c->device);
+    uint32_t* ws = nullptr;
+    float* d6 = nullptr;
+    hipError_t e = hipMalloc(&ws, 64);
+    if (e == hipSuccess) e = hipMalloc(&d6, 64);
+    if (e != hipSuccess) return hip_fail(c, e, "scene_aabb");
+    int rc = check_launch(c, launch_minmax(pos, n, ws, d6, c->stream), "scene_aabb");
+    float mm[6];
+    if (!rc) {
+        e = hipMemcpyAsync(mm, d6, sizeof(mm), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "scene_aabb");
+    }
+    (void)hipFree(ws);
+    (void)hipFree(d6);
+    if (rc) return rc;
+    // expansion of getOverallAABB in float, then the step of getGridSamples
+    float step = 0.0f;
+    for (int k = 0; k < 3; k++) {
+        const float lo0 = mm[k], hi0 = mm[3 + k];
+        const float mean = (hi0 + lo0) / 2.0f;
+        const float lo = (lo0 - mean) * 1.1f + mean;
+        const float hi = ((hi0 - mean) * 1.1f + mean - lo) * 1.0078125f + lo;
+        out7_h[k] = std::min(lo, -0.75f);
+        out7_h[3 + k] = std::max(hi, 0.75f);
+        step = std::max(step, (out7_h[3 + k] - out7_h[k]) / (float)grid_num);
+    }
+    out7_h[6] = step;
+    return 0;
+}
+
+int arapgs_grid_reorder(arapgs_ctx* c, const float* pos, int64_t n, const float* aabb7_h, int grid_num,
+                        uint32_t* order) {
+    CHECK_CTX(c);
+    if (!pos || !aabb7_h || !order || n < 0 || grid_num <= 0) return fail(c, -1, "grid_reorder: bad arguments");
+    if (n == 0) return 0;
+    GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, 0};
+    void* buf = nullptr;
+    const size_t words = radix_workspace_words(n);
+    hipError_t e = hipMalloc(&buf, (size_t)n * 24 + words * 4);
+    if (e != hipSuccess) return hip_fail(c, e, "grid_reorder");
+    uint64_t* keys = (uint64_t*)buf;
+    uint64_t* keys_alt = keys + n;
+    uint32_t* vals_alt = (uint32_t*)(keys_alt + n);
+    uint32_t* ws = vals_alt + n;
+    int bits = 1;
+    while ((1ll << bits) < (int64_t)grid_num * grid_num * grid_num) bits++;
+    int rc = check_launch(c, launch_center_cells(pos, n, gg, keys, order, c->stream), "grid_reorder");
+    if (!rc) rc = check_launch(c, launch_radix_sort(keys, order, keys_alt, vals_alt, n, bits, ws, c->stream), "grid_reorder");
+    e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (!rc && e != hipSuccess) rc = hip_fail(c, e, "grid_reorder");
+    return rc;
+}
+
+int arapgs_gaussian_aabbs(arapgs_ctx* c, const float* pos, const float* rot, const float* scale, const float* opacity,
+                          int64_t n, float* aabbs) {
+    CHECK_CTX(c);
+    if (!pos || !rot || !scale || !opacity || !aabbs || n < 0) return fail(c, -1, "gaussian_aabbs: bad arguments");
+    return check_launch(c, launch_aabbs(pos, rot, scale, opacity, n, aabbs, c->stream), "gaussian_aabbs");
+}
+
+int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num, int padding,
+                      arapgs_grid** out) {
+    CHECK_CTX(c);
+    if (!aabbs || !aabb7_h || !out || n <= 0 || grid_num <= 0 || grid_num > 1024)
+        return fail(c, -1, "grid_build: bad arguments");
+    *out = nullptr;
+    (void)hipSetDevice(c->device);
+    const int64_t gc = (int64_t)grid_num * grid_num * grid_num;
+    GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, padding};
+    arapgs_grid* g = new arapgs_grid();
+    g->G = grid_num;
+    g->device = c->device;
+    std::vector<void*> tmp;
+    auto dmal = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    uint32_t* cnt = (uint32_t*)dmal(4 * (size_t)n);
+    uint32_t* offs = (uint32_t*)dmal(4 * (size_t)n);
+    uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(std::max(n, gc)));
+    uint32_t* total = (uint32_t*)dmal(8);
+    uint32_t* cell_count = (uint32_t*)dmal(4 * (size_t)gc);
+    uint32_t* cell_ex = (uint32_t*)dmal(4 * (size_t)gc);
+    uint32_t* flags = (uint32_t*)dmal(4 * (size_t)gc);
+    uint32_t* flag_ex = (uint32_t*)dmal(4 * (size_t)gc);
+    if (!cnt || !offs || !scan_ws || !total || !cell_count || !cell_ex || !flags || !flag_ex ||
+        hipMalloc(&g->prefix, 4 * (size_t)gc) != hipSuccess) {
+        cleanup();
+        delete g;
+        return fail(c, -2, "grid_build: allocation failed");
+    }
+    int rc = check_launch(c, launch_box_count(aabbs, n, gg, cnt, c->stream), "grid_build");
+    if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), "grid_build");
+    uint32_t npairs = 0;
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(&npairs, total, 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+    }
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    if (!rc) {
+        const size_t np = std::max<size_t>(npairs, 1);
+        keys = (uint64_t*)dmal(8 * np * 2);
+        uint32_t* vals_alt = (uint32_t*)dmal(4 * np);
+        uint32_t* sort_ws = (uint32_t*)dmal(4 * radix_workspace_words((int64_t)np));
+        if (!keys || !vals_alt || !sort_ws || hipMalloc(&g->list, 4 * np) != hipSuccess) rc = fail(c, -2, "grid_build: allocation failed");
+        vals = (uint32_t*)g->list;
+        if (!rc) rc = check_launch(c, launch_box_emit(aabbs, n, gg, offs, keys, vals, cell_count, c->stream), "grid_build");
+        int bits = 1;
+        while ((1ll << bits) < gc) bits++;
+        if (!rc) rc = check_launch(c, launch_radix_sort(keys, vals, keys + np, vals_alt, npairs, bits, sort_ws, c->stream), "grid_build");
+        if (!rc) rc = check_launch(c, launch_exclusive_scan(cell_count, gc, cell_ex, total, scan_ws, c->stream), "grid_build");
+        if (!rc) rc = check_launch(c, launch_cell_prefix(cell_ex, cell_count, gc, g->prefix, flags, c->stream), "grid_build");
+        if (!rc) rc = check_launch(c, launch_exclusive_scan(flags, gc, flag_ex, total + 1, scan_ws, c->stream), "grid_build");
+    }
+    uint32_t nvalid = 0;
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(&nvalid, total + 1, 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+    }
+    if (!rc) {
+        if (hipMalloc(&g->valid, 4 * std::max<size_t>(nvalid, 1)) != hipSuccess ||
+            hipMalloc(&g->samples, 12 * 64 * std::max<size_t>(nvalid, 1)) != hipSuccess)
+            rc = fail(c, -2, "grid_build: allocation failed");
+        if (!rc) rc = check_launch(c, launch_compact_valid(flags, flag_ex, gc, g->valid, c->stream), "grid_build");
+        if (!rc) rc = check_launch(c, launch_samples(g->valid, nvalid, gg, g->samples, c->stream), "grid_build");
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (!rc && e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+    }
+    cleanup();
+    if (rc) {
+        arapgs_grid_destroy(g);
+        return rc;
+    }
+    g->n_pairs = npairs;
+    g->n_valid = nvalid;
+    *out = g;
+    return 0;
+}
+
+void arapgs_grid_destroy(arapgs_grid* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    for (void* p : {(void*)g->prefix, (void*)g->list, (void*)g->valid, (void*)g->samples})
+        if (p) (void)hipFree(p);
+    delete g;
+}
+
+int arapgs_grid_info(const arapgs_grid* g, int64_t* n_pairs, int64_t* n_valid, const int** prefix, const int** list,
+                     const int** valid, const float** samples) {
+    if (!g) return -1;
+    if (n_pairs) *n_pairs = g->n_pairs;
+    if (n_valid) *n_valid = g->n_valid;
+    if (prefix) *prefix = g->prefix;
+    if (list) *list = g->list;
+    if (valid) *valid = g->valid;
+    if (samples) *samples = g->samples;
+    return 0;
+}
+
+int arapgs_density_sample(arapgs_ctx* c, const arapgs_grid* g, const float* pos, const float* rot, const float* scale,
+                          const float* opacity, const float* sh, float lpf, float* feat, float* opac) {
+    CHECK_CTX(c);
+    if (!g || !pos || !rot || !scale || !opacity || !sh || !feat || !opac) return fail(c, -1, "density_sample: bad arguments");
+    return check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, g->samples, pos, rot, scale,
+                                          opacity, sh, lpf, feat, opac, c->stream), "density_sample");
 }
 
 }  // extern "C"

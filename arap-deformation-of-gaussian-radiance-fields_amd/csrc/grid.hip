@@ -1,0 +1,343 @@
+// G^3 spatial grid over the Gaussians (getGridSamples, GaussianView.cpp:3870-4149;
+// GetGsGrid / GetBoxesGsGrid + scans, cudakdtree.cu:403-451,508-553) and the
+// density sample on its cells (the un-vendored forward3d_grid, called at
+// GaussianView.cpp:4159-4186), for gfx950.
+//
+//   * scene AABB: device min/max reduction, then the float expansion rule on the host
+//   * reorder: stable radix sort of the centre cells (the rank rule of
+//     GaussianView.cpp:3904-3953 is exactly a stable sort by cell)
+//   * lists: every Gaussian emits its (cell, id) pairs at a scanned offset, a
+//     stable radix sort by cell yields per-cell lists in ascending id order —
+//     the order of the reference's serial fill loop (GaussianView.cpp:4076-4100)
+//   * density: one workgroup per valid cell, its Gaussians staged through LDS,
+//     64 samples per cell (one lane each)
+#pragma clang fp contract(off)
+#include "common.hpp"
+#include "grid.hpp"
+#include "sort.hpp"
+
+namespace arapgs {
+
+namespace {
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t b) {
+    return __uint_as_float((b & 0x80000000u) ? (b & 0x7fffffffu) : ~b);
+}
+
+__global__ void minmax_kernel(const float* __restrict__ pos, int64_t n, uint32_t* __restrict__ mm) {
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const uint32_t b = ordered_bits(pos[3 * i + c]);
+            lo[c] = min(lo[c], b);
+            hi[c] = max(hi[c], b);
+        }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[c] = min(lo[c], (uint32_t)__shfl_xor((int)lo[c], o));
+            hi[c] = max(hi[c], (uint32_t)__shfl_xor((int)hi[c], o));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&mm[c], lo[c]);
+            atomicMax(&mm[3 + c], hi[c]);
+        }
+    }
+}
+
+__global__ void minmax_init_kernel(uint32_t* __restrict__ mm) {
+    if (threadIdx.x < 3) mm[threadIdx.x] = 0xffffffffu;
+    else if (threadIdx.x < 6) mm[threadIdx.x] = 0u;
+}
+
+__global__ void decode_minmax_kernel(const uint32_t* __restrict__ mm, float* __restrict__ out) {
+    if (threadIdx.x < 6) out[threadIdx.x] = from_ordered(mm[threadIdx.x]);
+}
+
+__device__ __forceinline__ int cell_coord(float p, float mn, float step) { return (int)floorf((p - mn) / step); }
+
+__global__ void center_cell_kernel(const float* __restrict__ pos, int64_t n, GridGeom gg, uint64_t* __restrict__ keys,
+                                   uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = cell_coord(pos[3 * i], gg.mn[0], gg.step);
+    const int y = cell_coord(pos[3 * i + 1], gg.mn[1], gg.step);
+    const int z = cell_coord(pos[3 * i + 2], gg.mn[2], gg.step);
+    keys[i] = (uint64_t)(uint32_t)(x * gg.G * gg.G + y * gg.G + z);
+    vals[i] = (uint32_t)i;
+}
+
+// alpha-cutoff AABB (GaussianView.cpp:3960-4019)
+__global__ void aabb_kernel(const float* __restrict__ pos, const float* __restrict__ rot,
+                            const float* __restrict__ scale, const float* __restrict__ opacity, int64_t n,
+                            float* __restrict__ out) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float q[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
+    quat_normalize(q);
+    float R[3][3];
+    quat_to_mat(q[0], q[1], q[2], q[3], R);
+    const float cutoff = 1.0f / 255.0f;
+    float s3[3] = {0.0f, 0.0f, 0.0f};
+    if (opacity[g] > cutoff) {
+        const float lg = (float)log((double)(cutoff / opacity[g]));
+        const float f = __builtin_sqrtf(-2.0f * lg);
+#pragma unroll
+        for (int i = 0; i < 3; i++) s3[i] = f * (scale[3 * g + i] + 0.0f);
+    }
+    float lo[3], hi[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) lo[c] = hi[c] = pos[3 * g + c];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float off = R[c][d] * s3[d];
+            const float a = pos[3 * g + c] + off, b = pos[3 * g + c] + (-off);
+            lo[c] = fminf(lo[c], fminf(a, b));
+            hi[c] = fmaxf(hi[c], fmaxf(a, b));
+        }
+#pragma unroll
+    for (int c = 0; c < 3; c++) { out[6 * g + c] = lo[c]; out[6 * g + 3 + c] = hi[c]; }
+}
+
+__device__ __forceinline__ void box_range(const float* a, const GridGeom& gg, int lo[3], int hi[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        lo[c] = max((int)(floorf((a[c] - gg.mn[c]) / gg.step) - (float)gg.padding), 0);
+        hi[c] = min((int)(floorf((a[3 + c] - gg.mn[c]) / gg.step) + (float)gg.padding), gg.G - 1);
+    }
+}
+
+__global__ void box_count_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg, uint32_t* __restrict__ cnt) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lo[3], hi[3];
+    box_range(aabbs + 6 * g, gg, lo, hi);
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    cnt[g] = (nx > 0 && ny > 0 && nz > 0) ? (uint32_t)(nx * ny * nz) : 0u;
+}
+
+__global__ void box_emit_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg,
+                                const uint32_t* __restrict__ offs, uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_count) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lo[3], hi[3];
+    box_range(aabbs + 6 * g, gg, lo, hi);
+    uint32_t o = offs[g];
+    for (int x = lo[0]; x <= hi[0]; x++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int z = lo[2]; z <= hi[2]; z++) {
+                const uint32_t c = (uint32_t)(x * gg.G * gg.G + y * gg.G + z);
+                keys[o] = c;
+                vals[o] = (uint32_t)g;
+                o++;
+                atomicAdd(&cell_count[c], 1u);
+            }
+}
+
+__global__ void inclusive_from_exclusive_kernel(const uint32_t* __restrict__ ex, const uint32_t* __restrict__ cnt,
+                                                int64_t n, int* __restrict__ incl, uint32_t* __restrict__ flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    incl[i] = (int)(ex[i] + cnt[i]);
+    flags[i] = cnt[i] ? 1u : 0u;
+}
+
+__global__ void compact_valid_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos_ex,
+                                     int64_t n, int* __restrict__ valid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flags[i]) valid[pos_ex[i]] = (int)i;
+}
+
+__global__ void samples_kernel(const int* __restrict__ valid, int64_t nv, GridGeom gg, float* __restrict__ samples) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * 64) return;
+    const int64_t v = t / 64;
+    const int s = (int)(t % 64);
+    const int c = valid[v];
+    const int G = gg.G;
+    const int xi = c / (G * G), yi = (c - xi * G * G) / G, zi = c % G;
+    const float interval = gg.step / 4;
+    const float x0 = (float)((double)(gg.mn[0] + xi * gg.step) + 0.5 * (double)interval);
+    const float y0 = (float)((double)(gg.mn[1] + yi * gg.step) + 0.5 * (double)interval);
+    const float z0 = (float)((double)(gg.mn[2] + zi * gg.step) + 0.5 * (double)interval);
+    const int sx = s / 16, sy = (s - sx * 16) / 4, sz = s % 4;
+    samples[3 * t] = x0 + sx * interval;
+    samples[3 * t + 1] = y0 + sy * interval;
+    samples[3 * t + 2] = z0 + sz * interval;
+}
+
+// Density sample: a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d),
+// feature = sum a_g sh_g, opacity = sum a_g. One workgroup (64 lanes) per cell.
+__global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __restrict__ valid,
+                                                     const int* __restrict__ prefix, const int* __restrict__ list,
+                                                     const float* __restrict__ samples, const float* __restrict__ pos,
+                                                     const float* __restrict__ rot, const float* __restrict__ scale,
+                                                     const float* __restrict__ opacity, const float* __restrict__ sh,
+                                                     float lpf, float* __restrict__ feat, float* __restrict__ opac) {
+#pragma clang fp contract(fast)
+    constexpr int B = 64;
+    __shared__ float s_q[B][10];   // mu(3), inverse covariance (6), opacity
+    __shared__ float s_sh[B][48];
+    const int64_t v = blockIdx.x;
+    if (v >= nv) return;
+    const int c = valid[v];
+    const int beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
+    const int64_t si = v * 64 + threadIdx.x;
+    const float x0 = samples[3 * si], x1 = samples[3 * si + 1], x2 = samples[3 * si + 2];
+    float acc[48];
+#pragma unroll
+    for (int k = 0; k < 48; k++) acc[k] = 0.0f;
+    float aop = 0.0f;
+    for (int b0 = beg; b0 < end; b0 += B) {
+        const int li = b0 + threadIdx.x;
+        if (li < end) {
+            const int g = list[li];
+            float q[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
+            quat_normalize(q);
+            float R[3][3];
+            quat_to_mat(q[0], q[1], q[2], q[3], R);
+            float S[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    float a = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const float sk = scale[3 * g + k];
+                        a += R[i][k] * sk * sk * R[j][k];
+                    }
+                    S[i][j] = a + (i == j ? lpf : 0.0f);
+                }
+            const float c00 = S[1][1] * S[2][2] - S[1][2] * S[2][1];
+            const float c01 = S[1][2] * S[2][0] - S[1][0] * S[2][2];
+            const float c02 = S[1][0] * S[2][1] - S[1][1] * S[2][0];
+            const float det = S[0][0] * c00 + S[0][1] * c01 + S[0][2] * c02;
+            const float id = 1.0f / det;
+            s_q[threadIdx.x][0] = pos[3 * g];
+            s_q[threadIdx.x][1] = pos[3 * g + 1];
+            s_q[threadIdx.x][2] = pos[3 * g + 2];
+            s_q[threadIdx.x][3] = c00 * id;
+            s_q[threadIdx.x][4] = c01 * id;
+            s_q[threadIdx.x][5] = c02 * id;
+            s_q[threadIdx.x][6] = (S[0][0] * S[2][2] - S[0][2] * S[2][0]) * id;
+            s_q[threadIdx.x][7] = (S[0][2] * S[1][0] - S[0][0] * S[1][2]) * id;
+            s_q[threadIdx.x][8] = (S[0][0] * S[1][1] - S[0][1] * S[1][0]) * id;
+            s_q[threadIdx.x][9] = opacity[g];
+            const float4* s4 = reinterpret_cast<const float4*>(sh + 48 * (int64_t)g);
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                const float4 t4 = s4[k];
+                s_sh[threadIdx.x][4 * k] = t4.x; s_sh[threadIdx.x][4 * k + 1] = t4.y;
+                s_sh[threadIdx.x][4 * k + 2] = t4.z; s_sh[threadIdx.x][4 * k + 3] = t4.w;
+            }
+        }
+        __syncthreads();
+        const int m = min(B, end - b0);
+        for (int j = 0; j < m; j++) {
+            const float d0 = x0 - s_q[j][0], d1 = x1 - s_q[j][1], d2 = x2 - s_q[j][2];
+            const float qf = s_q[j][3] * d0 * d0 + s_q[j][6] * d1 * d1 + s_q[j][8] * d2 * d2 +
+                             2.0f * (s_q[j][4] * d0 * d1 + s_q[j][5] * d0 * d2 + s_q[j][7] * d1 * d2);
+            const float a = s_q[j][9] * __expf(-0.5f * qf);
+            aop += a;
+#pragma unroll
+            for (int k = 0; k < 48; k++) acc[k] += a * s_sh[j][k];
+        }
+        __syncthreads();
+    }
+    float4* o4 = reinterpret_cast<float4*>(feat + 48 * si);
+#pragma unroll
+    for (int k = 0; k < 12; k++) o4[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
+    opac[si] = aop;
+}
+
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+}  // namespace
+
+int launch_minmax(const float* pos, int64_t n, uint32_t* mm_ws, float* out6, hipStream_t st) {
+    hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(64), 0, st, mm_ws);
+    hipLaunchKernelGGL(minmax_kernel, dim3(std::min<unsigned>(nblk(n, 256), 1024)), dim3(256), 0, st, pos, n, mm_ws);
+    hipLaunchKernelGGL(decode_minmax_kernel, dim3(1), dim3(64), 0, st, mm_ws, out6);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_center_cells(const float* pos, int64_t n, const GridGeom& gg, uint64_t* keys, uint32_t* vals,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(center_cell_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, pos, n, gg, keys, vals);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_aabbs(const float* pos, const float* rot, const float* scale, const float* opacity, int64_t n, float* out,
+                 hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(aabb_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, pos, rot, scale, opacity, n, out);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_box_count(const float* aabbs, int64_t n, const GridGeom& gg, uint32_t* cnt, hipStream_t st) {
+    hipLaunchKernelGGL(box_count_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, cnt);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_box_emit(const float* aabbs, int64_t n, const GridGeom& gg, const uint32_t* offs, uint64_t* keys,
+                    uint32_t* vals, uint32_t* cell_count, hipStream_t st) {
+    const int64_t gc = (int64_t)gg.G * gg.G * gg.G;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(std::min<unsigned>(nblk(gc, 256), 4096)), dim3(256), 0, st, cell_count,
+                       gc);
+    hipLaunchKernelGGL(box_emit_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, offs, keys, vals,
+                       cell_count);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_cell_prefix(const uint32_t* cell_ex, const uint32_t* cell_count, int64_t gc, int* prefix, uint32_t* flags,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(inclusive_from_exclusive_kernel, dim3(nblk(gc, 256)), dim3(256), 0, st, cell_ex, cell_count, gc,
+                       prefix, flags);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_compact_valid(const uint32_t* flags, const uint32_t* pos_ex, int64_t gc, int* valid, hipStream_t st) {
+    hipLaunchKernelGGL(compact_valid_kernel, dim3(nblk(gc, 256)), dim3(256), 0, st, flags, pos_ex, gc, valid);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_samples(const int* valid, int64_t nv, const GridGeom& gg, float* samples, hipStream_t st) {
+    if (nv == 0) return 0;
+    hipLaunchKernelGGL(samples_kernel, dim3(nblk(nv * 64, 256)), dim3(256), 0, st, valid, nv, gg, samples);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_density(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+                   const float* pos, const float* rot, const float* scale, const float* opacity, const float* sh,
+                   float lpf, float* feat, float* opac, hipStream_t st) {
+    if (nv == 0) return 0;
+    hipLaunchKernelGGL(density_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples, pos, rot,
+                       scale, opacity, sh, lpf, feat, opac);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+}  // namespace arapgs

@@ -147,6 +147,43 @@ int arapgs_node_quats(arapgs_ctx* ctx, const arapgs_node_xf* table, int m, float
 int arapgs_sample_sh(arapgs_ctx* ctx, int64_t s, int k, const float* w, const uint32_t* idx, const float* node_q,
                      const uint8_t* is_static, float* feat);
 
+/* ------------------------------------------------------------ grid + density */
+typedef struct arapgs_grid arapgs_grid;
+
+/* getOverallAABB (GaussianView.cpp:3601-3631) + the grid step of
+ * getGridSamples (:3873-3877): out7_h (host) = {min.xyz, max.xyz, step} for a
+ * grid_num^3 grid. Synchronous. */
+int arapgs_scene_aabb(arapgs_ctx* ctx, const float* pos, int64_t n, int grid_num, float* out7_h);
+
+/* The cell reorder of getGridSamples (GaussianView.cpp:3889-3953): order u32[n]
+ * with order[new] = old, a stable sort of the Gaussians by centre cell. */
+int arapgs_grid_reorder(arapgs_ctx* ctx, const float* pos, int64_t n, const float* aabb7_h, int grid_num,
+                        uint32_t* order);
+
+/* Per-Gaussian alpha-cutoff AABBs (GaussianView.cpp:3960-4019): f32[n*6]. */
+int arapgs_gaussian_aabbs(arapgs_ctx* ctx, const float* pos, const float* rot, const float* scale,
+                          const float* opacity, int64_t n, float* aabbs);
+
+/* Cell lists + valid cells + 4^3 samples per valid cell (GaussianView.cpp:4021-4133,
+ * GetBoxesGsGrid cudakdtree.cu:425-451). The grid owns its device buffers.
+ * Synchronous (list and valid-cell counts are data dependent). */
+int arapgs_grid_build(arapgs_ctx* ctx, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num,
+                      int padding, arapgs_grid** out);
+void arapgs_grid_destroy(arapgs_grid* grid);
+/* Device views of a grid: prefix int[G^3] (inclusive per-cell counts, the
+ * reference's grid_gs_prefix_sum), list int[n_pairs] (grided_gs_idx), valid
+ * int[n_valid] (valid_grid_idx), samples f32[n_valid*64*3] (sample_positions). */
+int arapgs_grid_info(const arapgs_grid* grid, int64_t* n_pairs, int64_t* n_valid, const int** prefix,
+                     const int** list, const int** valid, const float** samples);
+
+/* Density sample of every grid sample (the forward3d_grid pass,
+ * GaussianView.cpp:4159-4186; formula defined by this library, see DESIGN):
+ * a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d) over the cell's list,
+ * feat f32[S*48] = sum a_g sh_g, opac f32[S] = sum a_g. */
+int arapgs_density_sample(arapgs_ctx* ctx, const arapgs_grid* grid, const float* pos, const float* rot,
+                          const float* scale, const float* opacity, const float* sh, float lpf, float* feat,
+                          float* opac);
+
 /* ------------------------------------------------------------ render */
 /* CudaRasterizer::Rasterizer::forward as called at GaussianView.cpp:1099-1128
  * (positional order of that call): P Gaussians, SH degree D (M = 16
