@@ -21,6 +21,14 @@ class ArapgsError(RuntimeError):
     pass
 
 
+class FrameBuffers(C.Structure):
+    _fields_ = [("n", C.c_int64), ("k", C.c_int), ("m", C.c_int), ("pos", P), ("rot", P), ("scale", P), ("sh", P),
+                ("scale0", P), ("ends", P), ("ends_idx", P), ("ends_w", P), ("gs_static", P), ("node_pos", P),
+                ("node_src_idx", P), ("node_src_w", P), ("s", C.c_int64), ("samples", P), ("sample_idx", P),
+                ("sample_w", P), ("sample_static", P), ("sample_feat", P), ("axis_padding", C.c_float),
+                ("end_coeff", C.c_float)]
+
+
 class SolverDesc(C.Structure):
     _fields_ = [("m", C.c_int), ("k", C.c_int), ("nbr_h", P), ("block_ptr_h", P), ("block_nodes_h", P),
                 ("block_type_h", P), ("nblocks", C.c_int), ("src_idx_h", P), ("src_w_h", P), ("node_pos_h", P),
@@ -71,6 +79,12 @@ EXPORTS = {
     "arapgs_grid_info": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P), C.POINTER(P),
                           C.POINTER(P)], C.c_int),
     "arapgs_density_sample": ([P, P, P, P, P, P, P, C.c_float, P, P], C.c_int),
+    "arapgs_solver_set_host_checks": ([P, C.c_int], C.c_int),
+    "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
+    "arapgs_aims_translate": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
+    "arapgs_aims_twist": ([P, P, C.c_int, P, C.c_int, P, C.c_float, P], C.c_int),
+    "arapgs_profile_enable": ([P, C.c_int], C.c_int),
+    "arapgs_profile_read": ([P, P, C.c_int], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
 }
@@ -263,6 +277,29 @@ class Context:
                                               ranges.ctypes.data_as(P), ntiles))
         return keys, vals, ranges
 
+    def aims_translate(self, node_pos, active, delta, aims):
+        import numpy as np
+        d = np.ascontiguousarray(delta, np.float32)
+        self._check(lib().arapgs_aims_translate(self.h, ptr(node_pos), node_pos.shape[0], ptr(active),
+                                                active.shape[0], d.ctypes.data_as(P), ptr(aims)))
+
+    def aims_twist(self, node_pos, active, axis, radian, aims):
+        import numpy as np
+        a = np.ascontiguousarray(axis, np.float32)
+        self._check(lib().arapgs_aims_twist(self.h, ptr(node_pos), node_pos.shape[0], ptr(active), active.shape[0],
+                                            a.ctypes.data_as(P), radian, ptr(aims)))
+
+    def profile_enable(self, on=True):
+        self._check(lib().arapgs_profile_enable(self.h, int(on)))
+
+    def profile_read(self, cap=4096):
+        import numpy as np
+        buf = np.zeros(cap, np.float32)
+        n = lib().arapgs_profile_read(self.h, buf.ctypes.data_as(P), cap)
+        if n < 0:
+            self._check(n)
+        return buf[:n]
+
     def sh_rotate(self, R9, sh):
         self._check(lib().arapgs_sh_rotate(self.h, sh.shape[0], ptr(R9), ptr(sh)))
 
@@ -342,6 +379,18 @@ class Solver:
         self.ctx._check(lib().arapgs_gn_energy(self.ctx.h, self.h, ptr(node_pos), ptr(aims), ptr(x), ptr(e)))
         return e
 
+    def set_host_checks(self, every):
+        self.ctx._check(lib().arapgs_solver_set_host_checks(self.h, every))
+
+    def deform_frame(self, fb, aims, iters, stop_rule=True, energies=None, iters_done=None):
+        t = self.ctx._torch
+        dev = aims.device
+        energies = t.zeros(iters + 1, dtype=t.float64, device=dev) if energies is None else energies
+        iters_done = t.zeros(1, dtype=t.int32, device=dev) if iters_done is None else iters_done
+        self.ctx._check(lib().arapgs_deform_frame(self.ctx.h, self.h, C.byref(fb), ptr(aims), iters, int(stop_rule),
+                                                  ptr(energies), ptr(iters_done)))
+        return energies, iters_done
+
     def put_free_inputs(self, node_pos, x, table=None):
         table = node_table(self.M, node_pos.device) if table is None else table
         self.ctx._check(lib().arapgs_put_free_inputs(self.ctx.h, self.h, ptr(node_pos), ptr(x), ptr(table)))
@@ -416,3 +465,14 @@ def _hip_memcpy_d2h(dst, src, nbytes):
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipMemcpy.restype = C.c_int
     return _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 2)  # hipMemcpyDeviceToHost
+
+
+def frame_buffers(state: dict, k: int, axis_padding=1e-3, end_coeff=2.0) -> FrameBuffers:
+    """FrameBuffers over a dict of device tensors (keys as in arapgs_frame_buffers)."""
+    g = state.get
+    return FrameBuffers(int(g("pos").shape[0]), k, int(g("node_pos").shape[0]), ptr(g("pos")), ptr(g("rot")),
+                        ptr(g("scale")), ptr(g("sh")), ptr(g("scale0")), ptr(g("ends")), ptr(g("ends_idx")),
+                        ptr(g("ends_w")), ptr(g("gs_static")), ptr(g("node_pos")), ptr(g("node_src_idx")),
+                        ptr(g("node_src_w")), int(g("samples").shape[0]) if g("samples") is not None else 0,
+                        ptr(g("samples")), ptr(g("sample_idx")), ptr(g("sample_w")), ptr(g("sample_static")),
+                        ptr(g("sample_feat")), axis_padding, end_coeff)

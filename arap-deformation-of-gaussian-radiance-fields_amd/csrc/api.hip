@@ -47,11 +47,20 @@ struct arapgs_ctx {
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
+    // optional HIP-event timing of the fused six-point kernel inside deform_frame
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+    size_t prof_used = 0;
 };
 
 struct arapgs_solver {
     arapgs_ctx* ctx = nullptr;
     SolverWork w;
+    int host_check_every = 0;   // 0: device-side convergence only
+    // per-frame scratch (M nodes)
+    NodeXf* table = nullptr;
+    float* node_q = nullptr;
+    float* node_tmp = nullptr;
 };
 
 namespace {
@@ -129,6 +138,10 @@ void arapgs_destroy(arapgs_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
+    for (auto& pr : c->prof_events) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
                     (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges,
                     (void*)c->r_total})
@@ -252,6 +265,13 @@ int arapgs_solver_create(arapgs_ctx* c, const arapgs_solver_desc* d, arapgs_solv
     }
     if (d->cg_max_iters > 0) s->w.cg_max_iters = d->cg_max_iters;
     if (d->cg_tol > 0.0f) s->w.cg_tol = d->cg_tol;
+    hipError_t e = hipMalloc(&s->table, sizeof(NodeXf) * (size_t)d->m);
+    if (e == hipSuccess) e = hipMalloc(&s->node_q, sizeof(float) * 4 * (size_t)d->m);
+    if (e == hipSuccess) e = hipMalloc(&s->node_tmp, sizeof(float) * 3 * (size_t)d->m);
+    if (e != hipSuccess) {
+        arapgs_solver_destroy(s);
+        return hip_fail(c, e, "solver_create");
+    }
     *out = s;
     return 0;
 }
@@ -261,7 +281,15 @@ void arapgs_solver_destroy(arapgs_solver* s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipDeviceSynchronize();
     solver_free(s->w);
+    for (void* p : {(void*)s->table, (void*)s->node_q, (void*)s->node_tmp})
+        if (p) (void)hipFree(p);
     delete s;
+}
+
+int arapgs_solver_set_host_checks(arapgs_solver* s, int every) {
+    if (!s || every < 0) return -1;
+    s->host_check_every = every;
+    return 0;
 }
 
 int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h) {
@@ -291,15 +319,31 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     W.dev.aim = aims;
     int rc = check_launch(c, solver_frame_setup(W, c->stream), "gn_solve setup");
     if (rc) return rc;
+    const int every = s->host_check_every;
+    auto host_flag = [&](const int* dflag, int* out) -> int {
+        hipError_t e = hipMemcpyAsync(out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
+    };
     for (int it = 0; it < iters; it++) {
         rc = check_launch(c, solver_gn_begin(W, c->stream), "gn_solve rhs");
         if (rc) return rc;
         for (int k = 0; k < W.cg_max_iters; k++) {
             rc = check_launch(c, solver_cg_iteration(W, c->stream), "gn_solve cg");
             if (rc) return rc;
+            if (every > 0 && (k + 1) % every == 0) {
+                int done = 0;
+                if ((rc = host_flag(W.cg_done, &done))) return rc;
+                if (done) break;
+            }
         }
         rc = check_launch(c, solver_gn_end(W, it, stop_rule, energies, c->stream), "gn_solve line search");
         if (rc) return rc;
+        if (every > 0) {
+            int done = 0;
+            if ((rc = host_flag(W.gn_done, &done))) return rc;
+            if (done) break;
+        }
     }
     if (iters == 0) {
         hipError_t e = hipMemcpyAsync(energies, W.cand_e, sizeof(double), hipMemcpyDeviceToDevice, c->stream);
@@ -643,6 +687,89 @@ int arapgs_density_sample(arapgs_ctx* c, const arapgs_grid* g, const float* pos,
     if (!g || !pos || !rot || !scale || !opacity || !sh || !feat || !opac) return fail(c, -1, "density_sample: bad arguments");
     return check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, g->samples, pos, rot, scale,
                                           opacity, sh, lpf, feat, opac, c->stream), "density_sample");
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int arapgs_aims_translate(arapgs_ctx* c, const float* node_pos, int m, const int* active, int n_active,
+                          const float* delta_h, float* aims) {
+    CHECK_CTX(c);
+    if (!node_pos || !aims || !delta_h || m <= 0 || (n_active > 0 && !active)) return fail(c, -1, "aims_translate: bad arguments");
+    return check_launch(c, launch_aims_translate(node_pos, m, active, n_active, delta_h, aims, c->stream), "aims_translate");
+}
+
+int arapgs_aims_twist(arapgs_ctx* c, const float* node_pos, int m, const int* active, int n_active, const float* axis_h,
+                      float radian, float* aims) {
+    CHECK_CTX(c);
+    if (!node_pos || !aims || !axis_h || m <= 0 || n_active <= 0 || !active) return fail(c, -1, "aims_twist: bad arguments");
+    return check_launch(c, launch_aims_twist(node_pos, m, active, n_active, axis_h, radian, aims, c->stream), "aims_twist");
+}
+
+int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buffers* fb, const float* aims,
+                        int gn_iters, int stop_rule, double* energies, int* iters_done) {
+    CHECK_CTX(c);
+    if (!s || !fb || !aims || !energies) return fail(c, -1, "deform_frame: bad arguments");
+    const int m = s->w.dev.M, k = s->w.dev.K;
+    if (fb->m != m || fb->k != k) return fail(c, -1, "deform_frame: buffers do not match the solver (m, k)");
+    int rc = arapgs_gn_solve(c, s, fb->node_pos, aims, gn_iters, stop_rule, nullptr, energies, iters_done);
+    if (rc) return rc;
+    rc = check_launch(c, launch_build_node_table(m, fb->node_pos, s->w.x, s->w.dev.free_id, s->table, c->stream),
+                      "deform_frame table");
+    // samples first: they use the current node positions (GaussianView.cpp:1500-1504)
+    if (!rc && fb->s > 0)
+        rc = check_launch(c, launch_blend_points(fb->samples, fb->s, fb->sample_idx, fb->sample_w, k, s->table, m,
+                                                 fb->sample_static, fb->samples, c->stream), "deform_frame samples");
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    if (!rc && c->prof) {
+        if (c->prof_used == c->prof_events.size() && c->prof_events.size() < 4096) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) c->prof_events.push_back({a, b});
+        }
+        if (c->prof_used < c->prof_events.size()) ev = &c->prof_events[c->prof_used++];
+        if (ev) (void)hipEventRecord(ev->first, c->stream);
+    }
+    if (!rc)
+        rc = check_launch(c, launch_six_point(fb->n, k, s->table, m, fb->ends_idx, fb->ends_w, fb->ends, fb->scale0,
+                                              fb->gs_static, fb->axis_padding, fb->end_coeff, fb->pos, fb->rot,
+                                              fb->scale, fb->sh, c->shrot, c->stream), "deform_frame six-point");
+    if (ev) (void)hipEventRecord(ev->second, c->stream);
+    if (!rc)
+        rc = check_launch(c, launch_blend_points(fb->node_pos, m, fb->node_src_idx, fb->node_src_w, k, s->table, m,
+                                                 nullptr, s->node_tmp, c->stream), "deform_frame nodes");
+    if (!rc) rc = check_launch(c, launch_copy_f32(fb->node_pos, s->node_tmp, 3 * (int64_t)m, c->stream), "deform_frame");
+    if (!rc && fb->s > 0 && fb->sample_feat) {
+        rc = check_launch(c, launch_node_quats(m, s->table, s->node_q, c->stream), "deform_frame quats");
+        if (!rc)
+            rc = check_launch(c, launch_sample_sh(fb->s, k, fb->sample_w, fb->sample_idx, s->node_q, fb->sample_static,
+                                                  fb->sample_feat, c->shrot, c->stream), "deform_frame sample SH");
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int arapgs_profile_enable(arapgs_ctx* c, int on) {
+    CHECK_CTX(c);
+    c->prof = on != 0;
+    c->prof_used = 0;
+    return 0;
+}
+
+int arapgs_profile_read(arapgs_ctx* c, float* ms_h, int cap) {
+    CHECK_CTX(c);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "profile_read");
+    int n = 0;
+    for (size_t i = 0; i < c->prof_used && n < cap; i++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->prof_events[i].first, c->prof_events[i].second) == hipSuccess) ms_h[n++] = ms;
+    }
+    c->prof_used = 0;
+    return n;
 }
 
 }  // extern "C"

@@ -48,4 +48,11 @@ int launch_polar(int64_t n, const double* M9, double* R9, double* Sdiag, hipStre
 
 void build_shrot_tables(ShRotTables* T);
 
+// ----------------------------------------------------------------- frame.hip
+int launch_aims_translate(const float* node_pos, int m, const int* active, int n_active, const float d[3], float* aims,
+                          hipStream_t st);
+int launch_aims_twist(const float* node_pos, int m, const int* active, int n_active, const float axis[3], float radian,
+                      float* aims, hipStream_t st);
+int launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t st);
+
 }  // namespace arapgs

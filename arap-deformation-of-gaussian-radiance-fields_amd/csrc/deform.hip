@@ -106,12 +106,12 @@ __global__ void node_table_kernel(int M, const float* __restrict__ node_pos, con
 }
 
 template <int K>
-__global__ void __launch_bounds__(256) blend_points_kernel(const float* __restrict__ pts, int64_t P,
+__global__ void __launch_bounds__(256) blend_points_kernel(const float* pts, int64_t P,
                                                            const uint32_t* __restrict__ idx,
                                                            const float* __restrict__ w,
                                                            const NodeXf* __restrict__ table,
                                                            const uint8_t* __restrict__ skip,
-                                                           float* __restrict__ out) {
+                                                           float* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
@@ -124,12 +124,12 @@ __global__ void __launch_bounds__(256) blend_points_kernel(const float* __restri
     out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
 }
 
-__global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* __restrict__ pts, int64_t P,
+__global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* pts, int64_t P,
                                                                    const uint32_t* __restrict__ idx,
                                                                    const float* __restrict__ w, int k,
                                                                    const NodeXf* __restrict__ table,
                                                                    const uint8_t* __restrict__ skip,
-                                                                   float* __restrict__ out) {
+                                                                   float* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
     const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};

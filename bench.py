@@ -1,0 +1,283 @@
+#!/usr/bin/env python3
+"""Deform+render frames/s at 1M Gaussians on one MI355X.
+
+Workload (the interactive-scale capsule scene): 1M FP32 Gaussians inside a
+union of 9 seeded capsules in the unit sphere, 2048 farthest-point nodes, K=8,
+64^3 density grid with 4^3 samples per valid cell, one capsule pinned and another
+dragged 0.3 along a seeded unit vector (spread over a 30-frame drag), 10
+Gauss-Newton iterations (reference stop rule on), 4 views at 800x800, SH degree 3.
+
+One step = one drag frame: aims -> ARAP GN solve -> node table -> sample blend
+-> end-point blend + six-point update (polar, quaternion, scale, SH) -> node
+blend -> node quaternions + sample SH rotation -> 4 renders. All inputs are
+resident in HBM before the timed region.
+
+Two forms are timed on the same state:
+  plain  every stage synchronous, CG convergence read back to the host after
+         every CG iteration (the reference's call pattern: a host-driven solver
+         and a device sync after each launch);
+  fast   stages queued back to back; CG convergence checked on the host only
+         every 16 iterations; views rendered back to back.
+`value` is the fast form's frames/s; `plain_fps` and the per-form p50/p99 frame
+latencies are reported alongside.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "arap-deformation-of-gaussian-radiance-fields_amd")
+HBM_PEAK_GBS = 8000.0
+SIX_POINT_BYTES_PER_GAUSSIAN = 980  # 6 ends 72 + idx 192 + w 192 + s0 12 + q 16 + SH 192 read; mu 12 + q 16 + s 12 + SH 192 + ends 72 written
+
+
+def load_package():
+    if "arapgs_amd" in sys.modules:
+        return sys.modules["arapgs_amd"]
+    spec = importlib.util.spec_from_file_location("arapgs_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["arapgs_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup(args):
+    import numpy as np
+    import torch
+    load_package()
+    from arapgs_amd import arapgs, scenes
+
+    ctx = arapgs.Context(0)
+    sc = scenes.capsule_scene(args.seed, n=args.n, n_nodes=args.nodes, W=args.width, H=args.height,
+                              n_views=args.views)
+    g = sc.gaussians
+    K = sc.knn_k
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    pos, rot, scale, opac, sh = T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh)
+    log(f"scene: {g.n} Gaussians, {args.nodes} nodes, K={K}")
+    # grid reorder of the Gaussians (getGridSamples)
+    a7 = ctx.scene_aabb(pos, sc.grid)
+    order = ctx.grid_reorder(pos, a7, sc.grid).long()
+    pos, rot, scale, opac, sh = (t.index_select(0, order).contiguous() for t in (pos, rot, scale, opac, sh))
+    # graph: FPS nodes, node edges, per-point KNN
+    nidx = ctx.fps(pos, args.nodes).long()
+    node_pos = pos.index_select(0, nidx).contiguous()
+    nbr = ctx.graph_edges(node_pos, K)
+    src_idx_full, src_w = ctx.knn(node_pos, node_pos, K)
+    node_src_idx = src_idx_full[:, :K].contiguous()
+    ends = ctx.end_points(pos, rot, scale)
+    eidx_full, ends_w = ctx.knn(node_pos, ends.view(-1, 3), K)
+    ends_idx = eidx_full[:, :K].contiguous()
+    del eidx_full
+    # grid, samples and their initial density features (GPUSetupSamplesFeatures)
+    aabbs = ctx.gaussian_aabbs(pos, rot, scale, opac)
+    grid = ctx.grid_build(aabbs, a7, sc.grid, padding=1)
+    samples = grid.samples_tensor()
+    sidx_full, sample_w = ctx.knn(node_pos, samples, K)
+    sample_idx = sidx_full[:, :K].contiguous()
+    del sidx_full
+    lpf = float(np.float32(a7[6] / 4) ** 2 * np.float32(0.2))
+    feat, _ = grid.density(pos, rot, scale, opac, sh, lpf)
+    torch.cuda.synchronize()
+    log(f"grid: {grid.n_valid} valid cells, {samples.shape[0]} samples, {grid.n_pairs} cell-Gaussian pairs")
+    # edit blocks on the rest node positions
+    npos_h = node_pos.cpu().numpy()
+    blocks, types = scenes.blocks_for(sc, npos_h)
+    active = [b for b, t in zip(blocks, types) if t == 1]
+    active = T(np.concatenate(active).astype(np.int32)) if active else torch.zeros(0, dtype=torch.int32, device=dev)
+    solver = arapgs.Solver(ctx, nbr.cpu().numpy(), blocks, types, node_src_idx.cpu().numpy(), src_w.cpu().numpy(),
+                           npos_h)
+    log(f"solver: {solver.n_free} free nodes, {solver.n_unknowns} unknowns; blocks {[len(b) for b in blocks]}")
+    state = dict(pos=pos, rot=rot, scale=scale, sh=sh, scale0=scale.clone(), ends=ends, ends_idx=ends_idx,
+                 ends_w=ends_w, gs_static=None, node_pos=node_pos, node_src_idx=node_src_idx, node_src_w=src_w,
+                 samples=samples, sample_idx=sample_idx, sample_w=sample_w, sample_static=None, sample_feat=feat)
+    fb = arapgs.frame_buffers(state, K)
+    cams = []
+    for c in sc.cameras:
+        cams.append(dict(view=T(c.view), proj=T(c.proj), campos=T(c.campos), tx=c.tanfovx, ty=c.tanfovy))
+    bg = T(np.array([0.8, 0.8, 0.8], np.float32))
+    images = [torch.empty((3, args.height, args.width), dtype=torch.float32, device=dev) for _ in cams]
+    step = np.float32(sc.edit["distance"] / args.drag_frames) * sc.edit["direction"].astype(np.float32)
+    return dict(ctx=ctx, sc=sc, state=state, fb=fb, solver=solver, aims=torch.empty_like(node_pos), active=active,
+                delta=step, cams=cams, bg=bg, images=images, opac=opac, iters=sc.gn_iters, grid=grid, K=K)
+
+
+def frame(B, plain):
+    ctx, st = B["ctx"], B["state"]
+    sync = ctx.synchronize if plain else (lambda: None)
+    ctx.aims_translate(st["node_pos"], B["active"], B["delta"], B["aims"])
+    sync()
+    B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=True)
+    sync()
+    n_inst = 0
+    for cam, img in zip(B["cams"], B["images"]):
+        out = ctx.render(st["pos"], st["sh"], B["opac"], st["scale"], st["rot"], cam["view"], cam["proj"],
+                         cam["campos"], cam["tx"], cam["ty"], img.shape[2], img.shape[1], B["bg"], out=img)
+        n_inst += out["num_rendered"]
+        sync()
+    return n_inst
+
+
+def timed(B, plain, steps, warmup):
+    import torch
+    B["solver"].set_host_checks(1 if plain else 16)
+    for _ in range(warmup):
+        frame(B, plain)
+    torch.cuda.synchronize()
+    ctx = B["ctx"]
+    ctx.profile_enable(True)
+    times = []
+    n_inst = 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        a = time.perf_counter()
+        n_inst = frame(B, plain)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - a) * 1e3)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    kernel_ms = ctx.profile_read()
+    ctx.profile_enable(False)
+    return dict(total_s=total, frame_ms=times, kernel_ms=kernel_ms, instances=n_inst)
+
+
+def pct(v, q):
+    s = sorted(v)
+    if not s:
+        return float("nan")
+    k = min(len(s) - 1, max(0, int(math.ceil(q / 100.0 * len(s))) - 1))
+    return s[k]
+
+
+def cpu_baseline(args):
+    """The oracle (CPU restatement of the reference path, C++/OpenMP, FP64
+    solve) timed on a bounded sample of the same workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    load_package()
+    from arapgs_amd import scenes
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sc = scenes.capsule_scene(args.seed, n=args.n, n_nodes=args.nodes)
+    g = sc.gaussians
+    K = sc.knn_k
+    rng = np.random.default_rng(0)
+    sub = np.sort(rng.choice(g.n, size=min(args.cpu_sample, g.n), replace=False))
+    # graph on the full scene (not timed)
+    ni = O.fps(g.pos, args.nodes)
+    nodes = g.pos[ni]
+    nbr = O.graph_edges(nodes, K)
+    nidx, nw = O.knn_weights(nodes, nodes, K)
+    blocks, types = scenes.blocks_for(sc, nodes)
+    aims = scenes.frame_aims(sc, nodes, blocks, types, 1.0 / args.drag_frames)
+    prob = O.GNProblem(nodes, nbr, blocks, types, nidx[:, :K], nw, aims)
+    log(f"cpu baseline: 1 GN iteration at {prob.n} unknowns ...")
+    t = time.perf_counter()
+    prob.solve(1, stop_rule=False)
+    t_gn1 = time.perf_counter() - t
+    # deform of a slice: end-point blend + six-point update (+ SH)
+    pos, rot, scale, sh = g.pos[sub], g.rot[sub], g.scale[sub], g.sh[sub]
+    ends = O.end_points(pos, rot, scale)
+    eidx, ew = O.knn_weights(nodes, ends.reshape(-1, 3), K)
+    rot9 = np.tile(np.eye(3).reshape(1, 9), (nodes.shape[0], 1))
+    t3 = np.zeros((nodes.shape[0], 3)) + 0.001
+    t = time.perf_counter()
+    new_ends = O.blend(ends.reshape(-1, 3), eidx[:, :K], ew, nodes, rot9, t3).reshape(-1, 6, 3)
+    O.six_point(new_ends, scale, None, pos, rot, scale, sh)
+    t_def = time.perf_counter() - t
+    frame_s = sc.gn_iters * t_gn1 + t_def * g.n / len(sub)
+    return dict(value=1.0 / frame_s, unit="frames/s", cores=threads, kind="port",
+                sample=(f"oracle (C++/OpenMP, FP64 envelope-Cholesky GN) on the same scene: 1 of {sc.gn_iters} GN "
+                        f"iterations at {prob.n} unknowns ({t_gn1:.1f}s, x{sc.gn_iters}) + end-point blend and "
+                        f"six-point update of {len(sub)} of {g.n} Gaussians ({t_def:.2f}s, scaled); no render "
+                        f"(the reference renders on the GPU); sample update not included"))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nodes", type=int, default=2048)
+    ap.add_argument("--views", type=int, default=4)
+    ap.add_argument("--width", type=int, default=800)
+    ap.add_argument("--height", type=int, default=800)
+    ap.add_argument("--drag-frames", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,fast")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the per-frame path is single-GPU. "
+              "See DESIGN.md for how it would shard.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    B = setup(args)
+    res = {}
+    for form in args.forms.split(","):
+        log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")
+        res[form] = timed(B, form == "plain", args.steps, args.warmup)
+        r = res[form]
+        log(f"  {form}: {args.steps / r['total_s']:.2f} frames/s, p50 {pct(r['frame_ms'], 50):.2f} ms, "
+            f"p99 {pct(r['frame_ms'], 99):.2f} ms, {r['instances']} tile instances/frame")
+    best = "fast" if "fast" in res else next(iter(res))
+    r = res[best]
+    fps = args.steps / r["total_s"]
+    kms = r["kernel_ms"]
+    avg_k = float(sum(kms) / len(kms)) if len(kms) else float("nan")
+    bytes_per_launch = SIX_POINT_BYTES_PER_GAUSSIAN * args.n
+    achieved = bytes_per_launch / (avg_k * 1e-3) / 1e9 if len(kms) else None
+    line = {
+        "metric": "deform+render frames/s @1M Gaussians, adjoint+batched vs plain; p50/p99 ms",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["total_s"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "capsules-1M", "gaussians": args.n, "nodes": args.nodes, "knn_k": 8,
+                   "grid": 64, "gn_iters": 10, "views": args.views, "resolution": [args.width, args.height],
+                   "sh_degree": 3, "form": best},
+        "p50_ms": round(pct(r["frame_ms"], 50), 3),
+        "p99_ms": round(pct(r["frame_ms"], 99), 3),
+    }
+    if "plain" in res:
+        rp = res["plain"]
+        line["plain_fps"] = round(args.steps / rp["total_s"], 3)
+        line["plain_p50_ms"] = round(pct(rp["frame_ms"], 50), 3)
+        line["plain_p99_ms"] = round(pct(rp["frame_ms"], 99), 3)
+        line["speedup_vs_plain"] = round(fps / line["plain_fps"], 3)
+    line["roofline"] = {"bound": "hbm", "kernel": "six_point_kernel<8> (fused end-point blend + polar + SH)",
+                        "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                        "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
+                        "launches": int(len(kms))}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,11 @@ int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h)
 int arapgs_gn_solve(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, int iters,
                     int stop_rule, float* x_out, double* energies, int* iters_done);
 
+/* Convergence checks on the host every `every` CG iterations and after every
+ * GN iteration (synchronising; the loop then stops early). 0 = none: the
+ * whole solve is queued with device-side early exit (graph-capturable). */
+int arapgs_solver_set_host_checks(arapgs_solver* s, int every);
+
 /* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,
  * Deform.cpp:378-581). energy: device f64[1]. */
 int arapgs_gn_energy(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
@@ -236,6 +241,55 @@ int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_re
  * (capacity in tiles). Synchronous. */
 int arapgs_render_debug(arapgs_ctx* ctx, uint64_t* keys_h, uint32_t* vals_h, int64_t cap, uint32_t* ranges_h,
                         int64_t ranges_cap);
+
+/* ------------------------------------------------------------ one frame */
+/* Device state of a deforming scene (all device pointers). */
+typedef struct arapgs_frame_buffers {
+    int64_t n;               /* Gaussians */
+    int k, m;                /* neighbours per point, nodes */
+    float* pos;              /* N*3  (pos_vector) */
+    float* rot;              /* N*4  (rot_vector, w,x,y,z) */
+    float* scale;            /* N*3  (scale_vector) */
+    float* sh;               /* N*48 (shs_vector) */
+    const float* scale0;     /* N*3  (scale_backup_vector) */
+    float* ends;             /* N*18 (ends_vector) */
+    const uint32_t* ends_idx;/* N*6*k (ends_vertices[].Neighbor_Nodes) */
+    const float* ends_w;     /* N*6*k */
+    const uint8_t* gs_static;/* N, optional (static_gaussians) */
+    float* node_pos;         /* M*3  (deform_graph.nodes[].Position) */
+    const uint32_t* node_src_idx; /* M*k (cand_vertices[Vertex_index]) */
+    const float* node_src_w;
+    int64_t s;               /* samples (0 = none) */
+    float* samples;          /* S*3  (sample_positions) */
+    const uint32_t* sample_idx;   /* S*k */
+    const float* sample_w;        /* S*k */
+    const uint8_t* sample_static; /* S, optional (static_samples) */
+    float* sample_feat;      /* S*48 (aim_feature_cuda), optional */
+    float axis_padding, end_coeff;   /* 1e-3, 2 (GaussianView.hpp:116-117) */
+} arapgs_frame_buffers;
+
+/* One drag frame of GaussianView::onUpdate (GaussianView.cpp:1481-1522) after
+ * the aims are set: Deform + real_time_deform, putFreeInputs,
+ * UpdatePositionforSamples, UpdatePosition (end points + nodes),
+ * UpdateAsSixPointsWithdrawBad with FastUpdateSamplesSH. */
+int arapgs_deform_frame(arapgs_ctx* ctx, arapgs_solver* s, const arapgs_frame_buffers* fb_h, const float* aims,
+                        int gn_iters, int stop_rule, double* energies, int* iters_done);
+
+/* UpdateAimPosition (GaussianView.cpp:2920-2934) after ReloadAimPositions:
+ * aims = node_pos, then aims[active] += delta_h (host float[3]). */
+int arapgs_aims_translate(arapgs_ctx* ctx, const float* node_pos, int m, const int* active, int n_active,
+                          const float* delta_h, float* aims);
+/* UpdateAimPositionTwist (GaussianView.cpp:2936-2962): rotate the active aims
+ * by `radian` about axis_h (host float[3]) through the active nodes' centroid. */
+int arapgs_aims_twist(arapgs_ctx* ctx, const float* node_pos, int m, const int* active, int n_active,
+                      const float* axis_h, float radian, float* aims);
+
+/* Measurement hook: when enabled, arapgs_deform_frame brackets its fused
+ * six-point kernel with HIP events on the context stream; arapgs_profile_read
+ * synchronises, returns the per-launch durations (ms) and resets. Returns the
+ * count written (>= 0) or a negative error. */
+int arapgs_profile_enable(arapgs_ctx* ctx, int on);
+int arapgs_profile_read(arapgs_ctx* ctx, float* ms_h, int cap);
 
 /* ------------------------------------------------------------ test hooks */
 /* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the

@@ -287,6 +287,122 @@ bool dense_solve(int n, std::vector<double>& A, std::vector<double>& b) {
     return true;
 }
 
+// Sparse direct solve for large systems: reverse Cuthill-McKee ordering of the
+// node graph of J^T J (12x12 blocks), then an envelope (profile) Cholesky in
+// FP64 — the same class of solver as the reference's SimplicialCholesky
+// (sparse LDL^T with a fill-reducing ordering).
+struct Envelope {
+    int n = 0;
+    std::vector<int> first;      // first stored column of each row
+    std::vector<size_t> start;   // offset of row i (entry j at start[i] + j - first[i])
+    std::vector<double> v;
+    double& at(int i, int j) { return v[start[i] + (size_t)(j - first[i])]; }
+};
+
+std::vector<int> rcm_order(int nb, const std::vector<std::vector<int>>& adj) {
+    std::vector<int> order, deg(nb);
+    std::vector<char> seen(nb, 0);
+    for (int i = 0; i < nb; i++) deg[i] = (int)adj[i].size();
+    for (int s0 = 0; s0 < nb; s0++) {
+        // start each component at a minimum-degree node
+        int best = -1;
+        for (int i = 0; i < nb; i++)
+            if (!seen[i] && (best < 0 || deg[i] < deg[best])) best = i;
+        if (best < 0) break;
+        size_t head = order.size();
+        order.push_back(best);
+        seen[best] = 1;
+        while (head < order.size()) {
+            const int u = order[head++];
+            std::vector<int> nb_list;
+            for (int w : adj[u])
+                if (!seen[w]) { seen[w] = 1; nb_list.push_back(w); }
+            std::sort(nb_list.begin(), nb_list.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+            order.insert(order.end(), nb_list.begin(), nb_list.end());
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// Solve (J^T J) h = rhs from merged rows of J (each row: list of (col, val)).
+bool envelope_solve(int n, const std::vector<std::vector<std::pair<int, double>>>& rows, std::vector<double>& rhs) {
+    const int nb = n / 12;
+    std::vector<std::vector<int>> adj(nb);
+    for (const auto& r : rows)
+        for (size_t a = 0; a < r.size(); a++)
+            for (size_t b = 0; b < r.size(); b++) {
+                const int u = r[a].first / 12, w = r[b].first / 12;
+                if (u != w) adj[u].push_back(w);
+            }
+    for (auto& a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    const std::vector<int> ord = rcm_order(nb, adj);
+    std::vector<int> inv(nb);
+    for (int i = 0; i < nb; i++) inv[ord[i]] = i;
+    auto perm = [&](int c) { return inv[c / 12] * 12 + c % 12; };
+    Envelope E;
+    E.n = n;
+    E.first.assign(n, 0);
+    for (int i = 0; i < n; i++) E.first[i] = i;
+    for (const auto& r : rows)
+        for (const auto& a : r)
+            for (const auto& b : r) {
+                const int pa = perm(a.first), pb = perm(b.first);
+                if (pb < pa) E.first[pa] = std::min(E.first[pa], pb);
+            }
+    E.start.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) E.start[i + 1] = E.start[i] + (size_t)(i - E.first[i] + 1);
+    E.v.assign(E.start[n], 0.0);
+    for (const auto& r : rows)
+        for (const auto& a : r)
+            for (const auto& b : r) {
+                const int pa = perm(a.first), pb = perm(b.first);
+                if (pb <= pa) E.at(pa, pb) += a.second * b.second;
+            }
+    // envelope Cholesky, row by row
+    for (int i = 0; i < n; i++) {
+        const int fi = E.first[i];
+        double* Li = &E.v[E.start[i]];
+        for (int j = fi; j <= i; j++) {
+            const int fj = E.first[j];
+            const int k0 = std::max(fi, fj);
+            const double* Lj = &E.v[E.start[j]];
+            double s = Li[j - fi];
+            const double* a = Li + (k0 - fi);
+            const double* b = Lj + (k0 - fj);
+            const int len = j - k0;
+            double acc = 0.0;
+            for (int k = 0; k < len; k++) acc += a[k] * b[k];
+            s -= acc;
+            if (j < i) Li[j - fi] = s / Lj[j - fj];
+            else {
+                if (!(s > 0.0)) return false;
+                Li[j - fi] = std::sqrt(s);
+            }
+        }
+    }
+    std::vector<double> y(n);
+    for (int c = 0; c < n; c++) y[perm(c)] = rhs[c];
+    for (int i = 0; i < n; i++) {
+        const int fi = E.first[i];
+        const double* Li = &E.v[E.start[i]];
+        double s = y[i];
+        for (int k = fi; k < i; k++) s -= Li[k - fi] * y[k];
+        y[i] = s / Li[i - fi];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        y[i] /= E.v[E.start[i] + (size_t)(i - E.first[i])];
+        const int fi = E.first[i];
+        const double* Li = &E.v[E.start[i]];
+        for (int k = fi; k < i; k++) y[k] -= Li[k - fi] * y[i];
+    }
+    for (int c = 0; c < n; c++) rhs[c] = y[perm(c)];
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -358,7 +474,8 @@ int orc_gn_solve(const orc_gn_args* a, int max_iters, int stop_rule, double* x, 
     const long m = P.rows;
     for (int i = 0; i < n; i++) x[i] = 0.0;
     for (int i = 0; i < n / 12; i++) x[12 * i] = x[12 * i + 4] = x[12 * i + 8] = 1.0;
-    std::vector<double> f(m), f1(m), xt(n), A((size_t)n * n), g(n);
+    const bool sparse = n > 3072;
+    std::vector<double> f(m), f1(m), xt(n), A(sparse ? 0 : (size_t)n * n), g(n);
     std::vector<Trip> T;
     int it = 0;
     for (; it < max_iters; it++) {
@@ -366,6 +483,7 @@ int orc_gn_solve(const orc_gn_args* a, int max_iters, int stop_rule, double* x, 
         residual(P, x, f.data());
         // J^T J and g = -J^T f via row-grouped triplets
         std::fill(A.begin(), A.end(), 0.0);
+        std::vector<std::vector<std::pair<int, double>>> jrows;
         std::fill(g.begin(), g.end(), 0.0);
         std::vector<Trip> Ts(T);
         std::stable_sort(Ts.begin(), Ts.end(), [](const Trip& p, const Trip& q) { return p.r < q.r; });
@@ -385,15 +503,17 @@ int orc_gn_solve(const orc_gn_args* a, int max_iters, int stop_rule, double* x, 
             const double fr = f[Ts[s].r];
             for (auto& p1 : mrg) {
                 g[p1.first] -= p1.second * fr;
-                for (auto& p2 : mrg) A[(size_t)p1.first * n + p2.first] += p1.second * p2.second;
+                if (!sparse)
+                    for (auto& p2 : mrg) A[(size_t)p1.first * n + p2.first] += p1.second * p2.second;
             }
+            if (sparse) jrows.push_back(std::move(mrg));
             s = e;
         }
         double old_e = 0.0;
         for (long r = 0; r < m; r++) old_e += f[r] * f[r];
         energies[it] = old_e;
         std::vector<double> h(g);
-        if (!dense_solve(n, A, h)) return -1;
+        if (sparse ? !envelope_solve(n, jrows, h) : !dense_solve(n, A, h)) return -1;
         double normv = 0.0;
         for (int i = 0; i < n; i++) normv += x[i] * x[i];
         normv = std::sqrt(normv);
