@@ -57,6 +57,8 @@ struct arapgs_solver {
     arapgs_ctx* ctx = nullptr;
     SolverWork w;
     int host_check_every = 0;   // 0: device-side convergence only
+    int coarse_refresh = 8;     // frames between coarse-operator rebuilds
+    int64_t solves = 0;
     // per-frame scratch (M nodes)
     NodeXf* table = nullptr;
     float* node_q = nullptr;
@@ -292,6 +294,13 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every) {
     return 0;
 }
 
+int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames) {
+    if (!s || frames < 1) return -1;
+    s->coarse_refresh = frames;
+    s->solves = 0;
+    return 0;
+}
+
 int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h) {
     if (!s) return -1;
     if (n_free_h) *n_free_h = s->w.dev.nF;
@@ -317,7 +326,9 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     SolverWork& W = s->w;
     W.dev.g = node_pos;
     W.dev.aim = aims;
-    int rc = check_launch(c, solver_frame_setup(W, c->stream), "gn_solve setup");
+    const bool refresh = (s->solves % std::max(1, s->coarse_refresh)) == 0;
+    s->solves++;
+    int rc = check_launch(c, solver_frame_setup(W, refresh, c->stream), "gn_solve setup");
     if (rc) return rc;
     const int every = s->host_check_every;
     auto host_flag = [&](const int* dflag, int* out) -> int {
@@ -329,7 +340,7 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
         rc = check_launch(c, solver_gn_begin(W, c->stream), "gn_solve rhs");
         if (rc) return rc;
         for (int k = 0; k < W.cg_max_iters; k++) {
-            rc = check_launch(c, solver_cg_iteration(W, c->stream), "gn_solve cg");
+            rc = check_launch(c, solver_cg_iteration(W, k, c->stream), "gn_solve cg");
             if (rc) return rc;
             if (every > 0 && (k + 1) % every == 0) {
                 int done = 0;

@@ -187,12 +187,12 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         for (int j = 0; j < 3; j++)
 #pragma unroll
             for (int c = 0; c < 3; c++) Ji[3 + j][3 * j + c] = 2.0f * S.sw_rot * xi[3 * j + c];
-        const float* vi = v + 12 * (int64_t)fi;
+        const int64_t vo = 12 * (int64_t)fi;
 #pragma unroll
         for (int a = 0; a < 6; a++) {
             float s = 0.0f;
 #pragma unroll
-            for (int b = 0; b < 9; b++) s += Ji[a][b] * vi[b];
+            for (int b = 0; b < 9; b++) s += Ji[a][b] * v[vo + b];
             Jv[a] = s;
         }
         fn(6, Ji, Jv);
@@ -207,13 +207,13 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         const float* gi = S.g + 3 * i;
         const float* gk = S.g + 3 * k;
         const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
-        const float* vi = v + 12 * (int64_t)fi;
+        const int64_t vo = 12 * (int64_t)fi;
 #pragma unroll
         for (int r = 0; r < 3; r++) {
 #pragma unroll
             for (int c = 0; c < 3; c++) Ji[r][3 * c + r] = S.sw_reg * s * d[c];
             Ji[r][9 + r] = S.sw_reg;
-            float jv = S.sw_reg * (s * (vi[r] * d[0] + vi[3 + r] * d[1] + vi[6 + r] * d[2]) + vi[9 + r]);
+            float jv = S.sw_reg * (s * (v[vo + r] * d[0] + v[vo + 3 + r] * d[1] + v[vo + 6 + r] * d[2]) + v[vo + 9 + r]);
             if (fk >= 0) jv -= S.sw_reg * s * v[12 * (int64_t)fk + 9 + r];
             Jv[r] = jv;
         }
@@ -238,11 +238,11 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
             const float* gs = S.g + 3 * src;
             const float* gi = S.g + 3 * i;
             const float d[3] = {gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2]};
-            const float* vs = v + 12 * (int64_t)fs;
+            const int64_t so = 12 * (int64_t)fs;
 #pragma unroll
             for (int r = 0; r < 3; r++) {
                 Ji[r][9 + r] = -S.sw_reg * s;
-                Jv[r] = S.sw_reg * (s * (vs[r] * d[0] + vs[3 + r] * d[1] + vs[6 + r] * d[2]) + vs[9 + r]) -
+                Jv[r] = S.sw_reg * (s * (v[so + r] * d[0] + v[so + 3 + r] * d[1] + v[so + 6 + r] * d[2]) + v[so + 9 + r]) -
                         S.sw_reg * s * v[12 * (int64_t)fi + 9 + r];
             }
         }
@@ -273,9 +273,9 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         const float w = S.src_w[(int64_t)c * S.K + j];
         const float* gj = S.g + 3 * nj;
         const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
-        const float* vj = v + 12 * (int64_t)fj;
+        const int64_t jo = 12 * (int64_t)fj;
 #pragma unroll
-        for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (vj[r] * dd[0] + vj[3 + r] * dd[1] + vj[6 + r] * dd[2] + vj[9 + r]);
+        for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
     }
     fn(3, Ji, Jv);
 }
@@ -406,12 +406,52 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// q = A p for all free nodes; partial p.q
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += p[i];
+    return s;
+}
+
+// Sum of n FP64 partials by the whole workgroup (result in every thread).
+__device__ __forceinline__ double block_sum_partials(const double* __restrict__ p, int n, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; w++) r += sh[w];
+    __syncthreads();
+    return r;
+}
+
+// PCG, three kernels per iteration k (no host round trips, no scalar kernels).
+// The search direction's product is carried recursively (Chronopoulos-Gear):
+// with s = A z,  p_k = z + beta_k p_{k-1}  and  q_k = A p_k = s + beta_k q_{k-1},
+// so the operator only ever reads z, and p, q are updated node-locally.
+//   apply   beta_k = rz_k / rz_{k-1} (0 at k = 0); s = A z; p, q as above
+//           (in place); partial p_k.q_k
+//   update  alpha = rz_k / (p_k.q_k); h += alpha p_k; r -= alpha q_k; z = Binv r;
+//           rc = P^T r; partial |r|^2               (one workgroup per aggregate)
+//   coarse  z += P (Ac^-1 rc); partial rz_{k+1} = r.z; workgroup 0 tests
+//           |r|^2 <= tol^2 |b|^2 and the iteration cap and raises `done`
+// Every workgroup reduces the (few dozen) FP64 partials it needs itself.
 __global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
-                                                    const float* __restrict__ p, float* __restrict__ q,
+                                                    const float* __restrict__ z, const double* __restrict__ rz_cur,
+                                                    const double* __restrict__ rz_prev, int first,
+                                                    float* __restrict__ p, float* __restrict__ q,
                                                     double* __restrict__ partial, const int* __restrict__ skip) {
     if (*skip) return;
     __shared__ double shd[8];
+    float beta = 0.0f;
+    if (!first) {
+        const double den = block_sum_partials(rz_prev, S.n_agg, shd);
+        const double num = block_sum_partials(rz_cur, S.n_agg, shd);
+        beta = den != 0.0 ? (float)(num / den) : 0.0f;
+    }
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     const int fi = blockIdx.x * (256 / LANES) + grp;
     float acc[12];
@@ -420,61 +460,36 @@ __global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __
     if (fi < S.nF) {
         const int ni = owner_items(S, fi);
         for (int it = lane; it < ni; it += LANES) {
-            owner_item(S, fi, it, x, p, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
+            owner_item(S, fi, it, x, z, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
                 for (int r = 0; r < rows; r++)
 #pragma unroll
                     for (int a = 0; a < 12; a++) acc[a] += Ji[r][a] * Jv[r];
             });
         }
     }
+    float mine = 0.0f;
 #pragma unroll
-    for (int a = 0; a < 12; a++) acc[a] = group_reduce16(acc[a]);
+    for (int a = 0; a < 12; a++) {
+        const float t = group_reduce16(acc[a]);
+        if (lane == a) mine = t;
+    }
     double pq = 0.0;
-    if (fi < S.nF && lane == 0) {
-        for (int a = 0; a < 12; a++) {
-            q[12 * (int64_t)fi + a] = acc[a];
-            pq += (double)acc[a] * p[12 * (int64_t)fi + a];
-        }
+    if (fi < S.nF && lane < 12) {
+        const int64_t o = 12 * (int64_t)fi + lane;
+        const float pn = first ? z[o] : z[o] + beta * p[o];
+        const float qn = first ? mine : mine + beta * q[o];
+        p[o] = pn;
+        q[o] = qn;
+        pq = (double)pn * qn;
     }
     const double t = block_reduce_d(pq, shd);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-// Scalar bookkeeping of PCG in one thread (all FP64):
-//   mode 0: init   rz = r.z (from partial), p = z handled elsewhere
-//   mode 1: alpha  alpha = rz / pq
-//   mode 2: beta   rz_new, beta = rz_new / rz, convergence test on |r|^2
-__global__ void scalar_kernel(int mode, const double* __restrict__ partial, int np, CgScalars* __restrict__ s,
-                              const double* __restrict__ partial2, int np2, int* __restrict__ skip) {
-    if (*skip) return;
-    if (threadIdx.x != 0) return;
-    double a = 0.0;
-    for (int i = 0; i < np; i++) a += partial[i];
-    if (mode == 0) {
-        s->rz = a;
-        s->beta = 0.0;
-        double rr = 0.0;
-        for (int i = 0; i < np2; i++) rr += partial2[i];
-        s->rr = rr;
-        s->bb = rr;
-        s->iters = 0;
-        if (!(rr > 0.0)) *skip = 1;
-    } else if (mode == 1) {
-        s->alpha = (a != 0.0) ? s->rz / a : 0.0;
-    } else {
-        double rr = 0.0;
-        for (int i = 0; i < np2; i++) rr += partial2[i];
-        s->rr = rr;
-        s->beta = (s->rz != 0.0) ? a / s->rz : 0.0;
-        s->rz = a;
-        s->iters += 1;
-        if (rr <= s->tol2 * s->bb || s->iters >= s->max_iters) *skip = 1;
-    }
-}
-
-// One workgroup per aggregate: h += alpha p, r -= alpha q (mode 1), then
-// z1 = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
-__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const CgScalars* __restrict__ s,
+// One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
+// z = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
+__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const double* __restrict__ pq_part,
+                                                              int n_pq, const double* __restrict__ rz_cur,
                                                               float* __restrict__ hsol, float* __restrict__ r,
                                                               const float* __restrict__ p, const float* __restrict__ q,
                                                               const float* __restrict__ binv, float* __restrict__ z,
@@ -483,9 +498,14 @@ __global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int m
     if (*skip) return;
     const int a = blockIdx.x;
     const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
-    const float alpha = mode == 1 ? (float)s->alpha : 0.0f;
     __shared__ float sred[256 / LANES][12];
     __shared__ double shd[8];
+    float alpha = 0.0f;
+    if (mode == 1) {
+        const double pq = block_sum_partials(pq_part, n_pq, shd);
+        const double rz = block_sum_partials(rz_cur, S.n_agg, shd);
+        alpha = pq != 0.0 ? (float)(rz / pq) : 0.0f;
+    }
     const float* ca = S.agg_center + 3 * a;
     float rcl[12];
 #pragma unroll
@@ -505,7 +525,7 @@ __global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int m
             rv = r[o];
             rr += (double)rv * rv;
         }
-        // z1 = Binv r (row c), needs all 12 r of the node: gather within 16 lanes
+        // z = Binv r (row c), needs all 12 r of the node: gather within 16 lanes
         float rn[12];
 #pragma unroll
         for (int j = 0; j < 12; j++) rn[j] = __shfl(rv, j, 16);
@@ -544,10 +564,12 @@ __global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int m
 }
 
 // One workgroup per aggregate: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z.
+// Workgroup 0 also runs the convergence test on the |r|^2 partials.
 __global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const float* __restrict__ acinv,
                                                              const float* __restrict__ rc, const float* __restrict__ r,
-                                                             float* __restrict__ z, double* __restrict__ partial,
-                                                             const int* __restrict__ skip) {
+                                                             float* __restrict__ z, double* __restrict__ rz_out,
+                                                             const double* __restrict__ rr_part,
+                                                             CgScalars* __restrict__ cs, int test, int* __restrict__ skip) {
     if (*skip) return;
     const int a = blockIdx.x;
     const int nc = 12 * S.n_agg;
@@ -585,15 +607,27 @@ __global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const 
         rz += (double)zz * r[o];
     }
     const double t = block_reduce_d(rz, shd);
-    if (threadIdx.x == 0) partial[a] = t;
+    const double rr = (a == 0 && test) ? block_sum_partials(rr_part, S.n_agg, shd) : 0.0;
+    if (threadIdx.x == 0) {
+        rz_out[a] = t;
+        if (a == 0 && test) {
+            cs->rr = rr;
+            cs->iters += 1;
+            if (rr <= cs->tol2 * cs->bb || cs->iters >= cs->max_iters) *skip = 1;
+        }
+    }
 }
 
-// p = z + beta p
-__global__ void pupdate_kernel(int n, const CgScalars* __restrict__ s, const float* __restrict__ z,
-                               float* __restrict__ p, const int* __restrict__ skip) {
+// |b|^2 from the rhs partials; an all-zero right-hand side ends the solve.
+__global__ void cg_init_kernel(const double* __restrict__ bb_part, int n, CgScalars* __restrict__ cs,
+                               int* __restrict__ skip) {
     if (*skip) return;
-    const float beta = (float)s->beta;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+    if (threadIdx.x != 0) return;
+    const double bb = sum_partials(bb_part, n);
+    cs->bb = bb;
+    cs->rr = bb;
+    cs->iters = 0;
+    if (!(bb > 0.0)) *skip = 1;
 }
 
 __global__ void zero_kernel(float* __restrict__ v, int64_t n, const int* __restrict__ skip) {
@@ -937,22 +971,26 @@ __global__ void cg_begin_kernel(const int* __restrict__ gn_done, int* __restrict
 // ---------------------------------------------------------------- host driver
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-int solver_frame_setup(SolverWork& W, hipStream_t st) {
+int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
     const SolverDev& S = W.dev;
-    hipLaunchKernelGGL(agg_center_kernel, dim3(S.n_agg), dim3(256), 0, st, S);
-    const int nc = 12 * S.n_agg;
-    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.acinv, (int64_t)S.nc_pad * S.nc_pad,
-                       (const int*)nullptr);
     hipLaunchKernelGGL(init_x_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.n, W.x, W.gs, W.gn_done,
                        (const double*)nullptr);
-    hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.acinv);
-    const int np = S.nc_pad;
-    hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
-    const int nt = np / GJ_T;
-    for (int kb = 0; kb < nt; kb++) {
-        hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
-        hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
-        hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+    if (refresh_coarse) {
+        // Galerkin coarse operator at x0 = identity and its inverse; it is only a
+        // preconditioner, so callers may keep it across frames.
+        hipLaunchKernelGGL(agg_center_kernel, dim3(S.n_agg), dim3(256), 0, st, S);
+        const int nc = 12 * S.n_agg;
+        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.acinv, (int64_t)S.nc_pad * S.nc_pad,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.acinv);
+        const int np = S.nc_pad;
+        hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
+        const int nt = np / GJ_T;
+        for (int kb = 0; kb < nt; kb++) {
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
+            hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+            hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+        }
     }
     // E(x0)
     hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, (const float*)nullptr,
@@ -971,32 +1009,29 @@ int solver_gn_begin(SolverWork& W, hipStream_t st) {
                        W.cg_tol * W.cg_tol);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial,
                        W.cg_done);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, W.partial, nb_owner, W.cg, W.cg_done);
     hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.cg_done);
-    // z = M^-1 r (mode 0: no update), rz, p = z
-    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, W.cg, W.hsol, W.r, W.p, W.q,
-                       W.binv, W.z, W.rc, W.partial2, W.cg_done);
-    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z,
-                       W.partial3, W.cg_done);
-    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 0, W.partial3, S.n_agg, W.cg, W.partial, nb_owner,
-                       W.cg_done);
-    hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.p, W.z, (int64_t)W.n, W.cg_done);
+    // z0 = M^-1 r0, rz_0 -> rz[0]
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, (const double*)nullptr, 0,
+                       (const double*)nullptr, W.hsol, W.r, (const float*)nullptr, (const float*)nullptr, W.binv, W.z,
+                       W.rc, W.partial2, W.cg_done);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z, W.rzp,
+                       W.partial2, W.cg, 0, W.cg_done);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int solver_cg_iteration(SolverWork& W, hipStream_t st) {
+int solver_cg_iteration(SolverWork& W, int k, hipStream_t st) {
     const SolverDev& S = W.dev;
     const int nb_owner = nblk(S.nF, 256 / LANES);
-    hipLaunchKernelGGL(apply_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.p, W.q, W.partial, W.cg_done);
-    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 1, W.partial, nb_owner, W.cg, (const double*)nullptr,
-                       0, W.cg_done);
-    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 1, W.cg, W.hsol, W.r, W.p, W.q,
-                       W.binv, W.z, W.rc, W.partial2, W.cg_done);
-    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z,
-                       W.partial3, W.cg_done);
-    hipLaunchKernelGGL(scalar_kernel, dim3(1), dim3(64), 0, st, 2, W.partial3, S.n_agg, W.cg, W.partial2, S.n_agg,
-                       W.cg_done);
-    hipLaunchKernelGGL(pupdate_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.n, W.cg, W.z, W.p, W.cg_done);
+    double* rz_cur = W.rzp + (k & 1) * S.n_agg;
+    double* rz_prev = W.rzp + ((k + 1) & 1) * S.n_agg;
+    hipLaunchKernelGGL(apply_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.z, rz_cur, rz_prev,
+                       k == 0 ? 1 : 0, W.p, W.q, W.partial, W.cg_done);
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 1, W.partial, nb_owner, rz_cur,
+                       W.hsol, W.r, W.p, W.q, W.binv, W.z, W.rc, W.partial2, W.cg_done);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z, rz_prev,
+                       W.partial2, W.cg, 1, W.cg_done);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -53,7 +53,7 @@ struct SolverWork {
     SolverDev dev;
     int n;  // 12 * nF
     float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col;
-    double *partial, *partial2, *partial3, *e_partial, *cand_e;
+    double *partial, *partial2, *rzp, *e_partial, *cand_e;  // rzp: 2 x n_agg r.z partials
     int e_blocks, n_cand;
     CgScalars* cg;
     GnScalars* gs;
@@ -83,9 +83,9 @@ struct SolverDesc {
 int solver_build(const SolverDesc& d, SolverWork& W, std::string& err);
 void solver_free(SolverWork& W);
 
-int solver_frame_setup(SolverWork& W, hipStream_t st);
+int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st);
 int solver_gn_begin(SolverWork& W, hipStream_t st);
-int solver_cg_iteration(SolverWork& W, hipStream_t st);
+int solver_cg_iteration(SolverWork& W, int k, hipStream_t st);
 int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st);
 int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st);
 

@@ -188,7 +188,7 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     const int nb_owner = (nF + 15) / 16;
     W.partial = alloc<double>(W, std::max(nb_owner, 1), rc);
     W.partial2 = alloc<double>(W, n_agg, rc);
-    W.partial3 = alloc<double>(W, n_agg, rc);
+    W.rzp = alloc<double>(W, 2 * (size_t)n_agg, rc);
     W.e_blocks = (int)std::min<int64_t>(512, (S.n_groups + 255) / 256);
     W.n_cand = 50;
     W.e_partial = alloc<double>(W, (size_t)W.e_blocks * W.n_cand, rc);

@@ -115,6 +115,12 @@ int arapgs_gn_solve(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, co
  * whole solve is queued with device-side early exit (graph-capturable). */
 int arapgs_solver_set_host_checks(arapgs_solver* s, int every);
 
+/* The coarse level of the preconditioner (Galerkin operator of the node
+ * aggregates and its inverse) is rebuilt on the first solve and then every
+ * `frames` solves (default 8); in between it is reused. It only affects the
+ * CG iteration count, never the solution. */
+int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
+
 /* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,
  * Deform.cpp:378-581). energy: device f64[1]. */
 int arapgs_gn_energy(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
