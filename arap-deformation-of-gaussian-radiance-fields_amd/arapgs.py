@@ -88,6 +88,7 @@ EXPORTS = {
     "arapgs_profile_read": ([P, P, C.c_int], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
+    "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
 }
 
 
@@ -310,6 +311,23 @@ class Context:
         S = t.empty((M9.shape[0], 3), dtype=t.float64, device=M9.device)
         self._check(lib().arapgs_polar(self.h, M9.shape[0], ptr(M9), ptr(R), ptr(S)))
         return R, S
+
+
+def tile_test_host(xy, conic, box):
+    """Blend's conservative Gaussian-vs-tile test evaluated on the host (numpy
+    float32 arrays xy (n, 2), conic (n, 4), box (n, 4)); returns uint8 keep."""
+    import numpy as np
+    xy = np.ascontiguousarray(xy, dtype=np.float32)
+    conic = np.ascontiguousarray(conic, dtype=np.float32)
+    box = np.ascontiguousarray(box, dtype=np.float32)
+    n = xy.shape[0]
+    if conic.shape != (n, 4) or box.shape != (n, 4) or xy.shape != (n, 2):
+        raise ArapgsError("tile_test_host: shapes must be (n,2), (n,4), (n,4)")
+    keep = np.zeros(n, dtype=np.uint8)
+    rc = lib().arapgs_tile_test_host(n, xy.ctypes.data, conic.ctypes.data, box.ctypes.data, keep.ctypes.data)
+    if rc != 0:
+        raise ArapgsError(f"tile_test_host failed ({rc})")
+    return keep
 
 
 def node_table(n, device):

@@ -432,6 +432,19 @@ int arapgs_polar(arapgs_ctx* c, int64_t n, const double* m9, double* r9, double*
     return check_launch(c, launch_polar(n, m9, r9, sdiag, c->stream), "polar");
 }
 
+int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const float* box, uint8_t* keep) {
+    if (n < 0 || (n > 0 && (!xy || !conic || !box || !keep))) return -1;
+    for (int64_t i = 0; i < n; i++) {
+        const float* b = box + 4 * i;
+        keep[i] = tile_may_contribute(make_float2(xy[2 * i], xy[2 * i + 1]),
+                                      make_float4(conic[4 * i], conic[4 * i + 1], conic[4 * i + 2], conic[4 * i + 3]),
+                                      b[0], b[1], b[2], b[3])
+                      ? 1
+                      : 0;
+    }
+    return 0;
+}
+
 }  // extern "C"
 
 extern "C" {

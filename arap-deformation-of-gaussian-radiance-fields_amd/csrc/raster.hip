@@ -189,37 +189,63 @@ __global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t*
 }
 
 // One workgroup per tile (tile x tile threads, tile in {8, 16}); each thread
-// owns one pixel.
+// owns one pixel. Each batch of the tile's Gaussians is filtered with the
+// conservative tile test and compacted in LDS (order kept), so pixels only
+// loop over Gaussians that can contribute to this tile.
 template <int TILE>
 __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
                                                           const uint2* __restrict__ ranges, float* __restrict__ out,
                                                           int* __restrict__ surface) {
     constexpr int NT = TILE * TILE;
+    constexpr int NW = (NT + 63) / 64;
     __shared__ float2 s_xy[NT];
     __shared__ float4 s_co[NT];
     __shared__ float4 s_rgb[NT];
     __shared__ uint32_t s_id[NT];
+    __shared__ int s_cnt[NW];
     const int tile = blockIdx.x;
     const int tx = tile % rp.gx, ty = tile / rp.gx;
     const int px = tx * TILE + (threadIdx.x % TILE), py = ty * TILE + (threadIdx.x / TILE);
     const bool inside = px < rp.W && py < rp.H;
     const float fxp = (float)px, fyp = (float)py;
+    const float bx0 = (float)(tx * TILE), by0 = (float)(ty * TILE);
+    const float bx1 = (float)min(tx * TILE + TILE - 1, rp.W - 1), by1 = (float)min(ty * TILE + TILE - 1, rp.H - 1);
     const uint2 rg = ranges[tile];
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     bool done = !inside;
     const int n = (int)(rg.y - rg.x);
+    const int wid = threadIdx.x >> 6, wl = threadIdx.x & 63;
     for (int b0 = 0; b0 < n; b0 += NT) {
         if (__syncthreads_count(done) == NT) break;
         const int i = b0 + (int)threadIdx.x;
+        bool keep = false;
+        uint32_t g = 0;
+        float2 xy;
+        float4 co;
         if (i < n) {
-            const uint32_t g = vals[rg.x + i];
-            s_id[threadIdx.x] = g;
-            s_xy[threadIdx.x] = geo.xy[g];
-            s_co[threadIdx.x] = geo.conic[g];
-            s_rgb[threadIdx.x] = geo.rgb[g];
+            g = vals[rg.x + i];
+            xy = geo.xy[g];
+            co = geo.conic[g];
+            keep = tile_may_contribute(xy, co, bx0, by0, bx1, by1);
+        }
+        const uint64_t bal = __ballot(keep);
+        if (wl == 0) s_cnt[wid] = __popcll(bal);
+        __syncthreads();
+        int base = 0, m = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const int cw = s_cnt[w];
+            base += (w < wid) ? cw : 0;
+            m += cw;
+        }
+        if (keep) {
+            const int pos = base + __popcll(bal & ((1ull << wl) - 1ull));
+            s_id[pos] = g;
+            s_xy[pos] = xy;
+            s_co[pos] = co;
+            s_rgb[pos] = geo.rgb[g];
         }
         __syncthreads();
-        const int m = min(NT, n - b0);
         for (int j = 0; j < m && !done; j++) {
             const float2 xy = s_xy[j];
             const float4 co = s_co[j];
@@ -236,6 +262,131 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
             C2 += c.z * alpha * T;
             if (surface) surface[s_id[j]] = 1;
             T = test_T;
+        }
+    }
+    if (inside) {
+        const int64_t HW = (int64_t)rp.W * rp.H, o = (int64_t)py * rp.W + px;
+        out[o] = C0 + T * rp.bg[0];
+        out[HW + o] = C1 + T * rp.bg[1];
+        out[2 * HW + o] = C2 + T * rp.bg[2];
+    }
+}
+
+// 16x16 tiles: one workgroup per tile, one wave per 8x8 quadrant. Each batch
+// of 256 of the tile's Gaussians is staged in LDS once; every loader thread
+// tests its Gaussian against the four quadrant boxes and the per-quadrant
+// survivors are compacted into index lists (depth order kept). Each wave then
+// walks only its own quadrant's list, four Gaussians per step: the four
+// power / exp / alpha evaluations are independent and overlap, and the
+// front-to-back compositing that follows runs in order with exactly the
+// per-pixel arithmetic of the one-pixel-per-thread form, so the image is the
+// same bit for bit; only the work on Gaussians that cannot reach 1/255 in the
+// quadrant is skipped.
+__global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
+                                                         const uint2* __restrict__ ranges, float* __restrict__ out,
+                                                         int* __restrict__ surface) {
+    constexpr int NT = 256;
+    __shared__ float2 s_xy[NT];
+    __shared__ float4 s_co[NT];
+    __shared__ float4 s_rgb[NT];
+    __shared__ uint32_t s_id[NT];
+    __shared__ __align__(4) uint8_t s_list[4][NT + 4];
+    __shared__ int s_cnt[4][4];  // [quadrant][wave]
+    const int tile = blockIdx.x;
+    const int tx = tile % rp.gx, ty = tile / rp.gx;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = tx * 16 + (wv & 1) * 8 + (lane & 7), py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    const bool inside = px < rp.W && py < rp.H;
+    const float fxp = (float)px, fyp = (float)py;
+    // quadrant boxes (pixel centres inside the image)
+    float qb[4][4];
+    bool qok[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int x0 = tx * 16 + (q & 1) * 8, y0 = ty * 16 + (q >> 1) * 8;
+        qok[q] = x0 < rp.W && y0 < rp.H;
+        qb[q][0] = (float)x0;
+        qb[q][1] = (float)y0;
+        qb[q][2] = (float)min(x0 + 7, rp.W - 1);
+        qb[q][3] = (float)min(y0 + 7, rp.H - 1);
+    }
+    const uint2 rg = ranges[tile];
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    bool done = !inside;
+    const int n = (int)(rg.y - rg.x);
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int b0 = 0; b0 < n; b0 += NT) {
+        if (__syncthreads_count(done) == NT) break;
+        const int i = b0 + (int)threadIdx.x;
+        int qmask = 0;
+        if (i < n) {
+            const uint32_t g = vals[rg.x + i];
+            const float2 xy = geo.xy[g];
+            const float4 co = geo.conic[g];
+            const float lthr = cull_log_threshold(co.w);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (qok[q] && box_may_contribute(xy, co, lthr, qb[q][0], qb[q][1], qb[q][2], qb[q][3])) qmask |= 1 << q;
+            if (qmask) {
+                s_id[threadIdx.x] = g;
+                s_xy[threadIdx.x] = xy;
+                s_co[threadIdx.x] = co;
+                s_rgb[threadIdx.x] = geo.rgb[g];
+            }
+        }
+        uint64_t bal[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            bal[q] = __ballot((qmask >> q) & 1);
+            if (lane == 0) s_cnt[q][wv] = __popcll(bal[q]);
+        }
+        __syncthreads();
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int base = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                const int cw = s_cnt[q][w];
+                base += (w < wv) ? cw : 0;
+                if (q == wv) m += cw;
+            }
+            if ((qmask >> q) & 1) s_list[q][base + __popcll(bal[q] & lt_mask)] = (uint8_t)threadIdx.x;
+        }
+        __syncthreads();
+        const uint8_t* lst = s_list[wv];
+        for (int j = 0; j < m; j += 4) {
+            if (__all(done)) break;
+            const uint32_t k4 = *reinterpret_cast<const uint32_t*>(lst + j);
+            int k[4];
+            float alpha[4];
+            bool use[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                k[u] = (int)((k4 >> (8 * u)) & 0xffu);
+                const float2 xy = s_xy[k[u]];
+                const float4 co = s_co[k[u]];
+                const float dx = xy.x - fxp, dy = xy.y - fyp;
+                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                alpha[u] = fminf(0.99f, co.w * expf(power));
+                use[u] = (j + u < m) && !(power > 0.0f) && !(alpha[u] < 1.0f / 255.0f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (use[u] && !done) {
+                    const float test_T = T * (1.0f - alpha[u]);
+                    if (test_T < 0.0001f) {
+                        done = true;
+                    } else {
+                        const float4 c = s_rgb[k[u]];
+                        C0 += c.x * alpha[u] * T;
+                        C1 += c.y * alpha[u] * T;
+                        C2 += c.z * alpha[u] * T;
+                        if (surface) surface[s_id[k[u]]] = 1;
+                        T = test_T;
+                    }
+                }
+            }
         }
     }
     if (inside) {
@@ -310,7 +461,7 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
                            keys, d_total, ranges);
     }
     if (rp.tile == 16)
-        hipLaunchKernelGGL(blend_kernel<16>, dim3(ntiles), dim3(256), 0, st, rp, geo, vals, ranges, out, surface);
+        hipLaunchKernelGGL(blend_quad_kernel, dim3(ntiles), dim3(256), 0, st, rp, geo, vals, ranges, out, surface);
     else if (rp.tile == 8)
         hipLaunchKernelGGL(blend_kernel<8>, dim3(ntiles), dim3(64), 0, st, rp, geo, vals, ranges, out, surface);
     else

@@ -1,6 +1,7 @@
 // Rasteriser launchers (raster.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -21,6 +22,41 @@ struct RasterParams {
     int use_rects, use_box;
     float boxmin[3], boxmax[3];
 };
+
+// Conservative test whether a Gaussian can reach alpha >= 1/255 at any pixel
+// centre of the box [x0, x1] x [y0, y1]. The blend skips a pixel when
+// o * exp(power) < 1/255 with power = -q/2, q = a dx^2 + 2 b dx dy + c dy^2
+// (conic (a, b, c), d = mean - pixel). The minimum of the convex q over the box
+// is 0 when the mean lies inside, otherwise it sits on one of the four edges at
+// the clamped 1-D minimiser. A Gaussian is dropped only when that minimum
+// exceeds the threshold by a margin (1e-3 + 1e-5 * sum of |terms|) far above
+// the float rounding of the per-pixel evaluation, so every pixel would have
+// skipped it too and the composite is unchanged bit for bit.
+__host__ __device__ inline float cull_log_threshold(float opacity) { return logf(255.0f * opacity) + 1e-3f; }
+
+__host__ __device__ inline bool box_may_contribute(float2 xy, float4 co, float lthr, float x0, float y0, float x1,
+                                                   float y1) {
+    const float a = co.x, b = co.y, c = co.z;
+    const float dxl = xy.x - x1, dxh = xy.x - x0, dyl = xy.y - y1, dyh = xy.y - y0;
+    if (dxl <= 0.0f && dxh >= 0.0f && dyl <= 0.0f && dyh >= 0.0f) return true;
+    if (!(a > 0.0f && c > 0.0f && a * c - b * b > 0.0f)) return true;
+    const float y_l = fminf(dyh, fmaxf(dyl, -b * dxl / c));
+    const float y_h = fminf(dyh, fmaxf(dyl, -b * dxh / c));
+    const float x_l = fminf(dxh, fmaxf(dxl, -b * dyl / a));
+    const float x_h = fminf(dxh, fmaxf(dxl, -b * dyh / a));
+    const float q0 = a * dxl * dxl + 2.0f * b * dxl * y_l + c * y_l * y_l;
+    const float q1 = a * dxh * dxh + 2.0f * b * dxh * y_h + c * y_h * y_h;
+    const float q2 = a * x_l * x_l + 2.0f * b * x_l * dyl + c * dyl * dyl;
+    const float q3 = a * x_h * x_h + 2.0f * b * x_h * dyh + c * dyh * dyh;
+    const float qmin = fminf(fminf(q0, q1), fminf(q2, q3));
+    const float mx = fmaxf(fabsf(dxl), fabsf(dxh)), my = fmaxf(fabsf(dyl), fabsf(dyh));
+    const float mag = a * mx * mx + c * my * my + 2.0f * fabsf(b) * mx * my;
+    return !(0.5f * qmin > lthr + 1e-5f * mag);
+}
+
+__host__ __device__ inline bool tile_may_contribute(float2 xy, float4 co, float x0, float y0, float x1, float y1) {
+    return box_may_contribute(xy, co, cull_log_threshold(co.w), x0, y0, x1, y1);
+}
 
 size_t raster_geo_bytes(int P);
 // Preprocess + scan of tiles touched; *d_total (device) = instances to render.

@@ -304,6 +304,13 @@ int arapgs_sh_rotate(arapgs_ctx* ctx, int64_t n, const float* r9, float* sh);
 /* Polar decomposition M = R S in FP64 (helper.cpp:429-440): m9, r9 f64[n*9]
  * row-major, sdiag f64[n*3] = diag(S). */
 int arapgs_polar(arapgs_ctx* ctx, int64_t n, const double* m9, double* r9, double* sdiag);
+/* Host-side evaluation (no GPU, no context) of the blend's conservative
+ * Gaussian-vs-tile test: keep[i] = 0 only if Gaussian i (xy f32[n*2], conic
+ * f32[n*4] = a, b, c, opacity) has alpha < 1/255 at every pixel centre of
+ * box f32[n*4] = x0, y0, x1, y1. Skipping such pairs leaves the composite of
+ * Rasterizer::forward (un-vendored XinhaoT/CudaRasterizer@96ea96c, called at
+ * GaussianView.cpp:1099-1128) unchanged. */
+int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const float* box, uint8_t* keep);
 
 #ifdef __cplusplus
 }
