@@ -22,6 +22,7 @@ struct arapgs_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     ShRotTables shrot;
+    ShRotTables* shrot_dev = nullptr;  // device copy read by the SH kernels
     // growable workspaces
     void* knn_list = nullptr;
     size_t knn_list_cap = 0;
@@ -127,8 +128,10 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipMalloc(&c->fps_partial, sizeof(unsigned long long) * FPS_BLOCKS)) != hipSuccess ||
         (e = hipMalloc(&c->fps_counter, sizeof(unsigned int))) != hipSuccess ||
         (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&c->r_total, sizeof(uint32_t))) != hipSuccess) {
-        delete c;
+        (e = hipMalloc(&c->r_total, sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&c->shrot_dev, sizeof(ShRotTables))) != hipSuccess ||
+        (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
+        arapgs_destroy(c);
         return -5;
     }
     (void)hipMemset(c->fps_counter, 0, sizeof(unsigned int));
@@ -146,7 +149,7 @@ void arapgs_destroy(arapgs_ctx* c) {
     }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
                     (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges,
-                    (void*)c->r_total})
+                    (void*)c->r_total, (void*)c->shrot_dev})
         if (p) (void)hipFree(p);
     delete c;
 }
@@ -402,8 +405,11 @@ int arapgs_six_point_update(arapgs_ctx* c, int64_t n, int k, const arapgs_node_x
     if (!table || !ends_idx || !ends_w || !ends || !scale0 || !pos || !rot || !scale || !sh || n < 0 || k < 1 ||
         k > KNN_MAX_T)
         return fail(c, -1, "six_point_update: bad arguments");
+    if (m < 1) return fail(c, -1, "six_point_update: need m >= 1 nodes");
+    if ((uintptr_t)rot % 16) return fail(c, -1, "six_point_update: rot must be 16-byte aligned");
+    if ((uintptr_t)sh % 16) return fail(c, -1, "six_point_update: sh must be 16-byte aligned");
     return check_launch(c, launch_six_point(n, k, (const NodeXf*)table, m, ends_idx, ends_w, ends, scale0, is_static,
-                                            pad, coeff, pos, rot, scale, sh, c->shrot, c->stream),
+                                            pad, coeff, pos, rot, scale, sh, c->shrot_dev, c->stream),
                         "six_point_update");
 }
 
@@ -417,13 +423,13 @@ int arapgs_sample_sh(arapgs_ctx* c, int64_t s, int k, const float* w, const uint
                      const uint8_t* is_static, float* feat) {
     CHECK_CTX(c);
     if (!w || !idx || !node_q || !feat || s < 0 || k < 1 || k > KNN_MAX_T) return fail(c, -1, "sample_sh: bad arguments");
-    return check_launch(c, launch_sample_sh(s, k, w, idx, node_q, is_static, feat, c->shrot, c->stream), "sample_sh");
+    return check_launch(c, launch_sample_sh(s, k, w, idx, node_q, is_static, feat, c->shrot_dev, c->stream), "sample_sh");
 }
 
 int arapgs_sh_rotate(arapgs_ctx* c, int64_t n, const float* r9, float* sh) {
     CHECK_CTX(c);
     if (!r9 || !sh || n < 0) return fail(c, -1, "sh_rotate: bad arguments");
-    return check_launch(c, launch_sh_rotate(n, r9, sh, c->shrot, c->stream), "sh_rotate");
+    return check_launch(c, launch_sh_rotate(n, r9, sh, c->shrot_dev, c->stream), "sh_rotate");
 }
 
 int arapgs_polar(arapgs_ctx* c, int64_t n, const double* m9, double* r9, double* sdiag) {
@@ -757,7 +763,8 @@ int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buff
     if (!rc)
         rc = check_launch(c, launch_six_point(fb->n, k, s->table, m, fb->ends_idx, fb->ends_w, fb->ends, fb->scale0,
                                               fb->gs_static, fb->axis_padding, fb->end_coeff, fb->pos, fb->rot,
-                                              fb->scale, fb->sh, c->shrot, c->stream), "deform_frame six-point");
+                                              fb->scale, fb->sh, c->shrot_dev, c->stream),
+                          "deform_frame six-point");
     if (ev) (void)hipEventRecord(ev->second, c->stream);
     if (!rc)
         rc = check_launch(c, launch_blend_points(fb->node_pos, m, fb->node_src_idx, fb->node_src_w, k, s->table, m,
@@ -767,7 +774,7 @@ int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buff
         rc = check_launch(c, launch_node_quats(m, s->table, s->node_q, c->stream), "deform_frame quats");
         if (!rc)
             rc = check_launch(c, launch_sample_sh(fb->s, k, fb->sample_w, fb->sample_idx, s->node_q, fb->sample_static,
-                                                  fb->sample_feat, c->shrot, c->stream), "deform_frame sample SH");
+                                                  fb->sample_feat, c->shrot_dev, c->stream), "deform_frame sample SH");
     }
     return rc;
 }

@@ -31,6 +31,7 @@ struct NodeXf {
 };
 static_assert(sizeof(NodeXf) == 64, "NodeXf is one 64-byte record");
 
+
 int launch_end_points(const float* pos, const float* rot, const float* scale, int64_t N, float pad, float coeff,
                       float* ends, hipStream_t st);
 int launch_build_node_table(int M, const float* node_pos, const float* x_free, const int* free_id, NodeXf* table,
@@ -39,11 +40,11 @@ int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const 
                         int M, const uint8_t* skip, float* out, hipStream_t st);
 int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
-                     float* rot, float* scale, float* sh, const ShRotTables& T, hipStream_t st);
+                     float* rot, float* scale, float* sh, const ShRotTables* Td, hipStream_t st);
 int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st);
 int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
-                     const uint8_t* is_static, float* feat, const ShRotTables& T, hipStream_t st);
-int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables& T, hipStream_t st);
+                     const uint8_t* is_static, float* feat, const ShRotTables* Td, hipStream_t st);
+int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables* Td, hipStream_t st);
 int launch_polar(int64_t n, const double* M9, double* R9, double* Sdiag, hipStream_t st);
 
 void build_shrot_tables(ShRotTables* T);

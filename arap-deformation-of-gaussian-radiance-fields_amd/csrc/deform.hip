@@ -39,13 +39,9 @@ __device__ __forceinline__ void blend_point(const NodeXf* __restrict__ table, co
 }
 
 template <int K>
-__device__ __forceinline__ void blend_point_k(const NodeXf* __restrict__ table, const uint32_t* __restrict__ idx,
-                                              const float* __restrict__ w, const float p[3], float o[3]) {
+__device__ __forceinline__ void blend_point_regs(const NodeXf* __restrict__ table, const uint32_t (&id)[K],
+                                                 const float (&wt)[K], const float p[3], float o[3]) {
 #pragma clang fp contract(off)
-    uint32_t id[K];
-    float wt[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) { id[j] = idx[j]; wt[j] = w[j]; }
     o[0] = o[1] = o[2] = 0.0f;
 #pragma unroll
     for (int j = 0; j < K; j++) {
@@ -58,6 +54,34 @@ __device__ __forceinline__ void blend_point_k(const NodeXf* __restrict__ table, 
         o[1] = (float)((double)o[1] + wj * ((double)b.z * tx + (double)b.w * ty + (double)c.x * tz + (double)d.y + (double)a.y));
         o[2] = (float)((double)o[2] + wj * ((double)c.y * tx + (double)c.z * ty + (double)c.w * tz + (double)d.z + (double)a.z));
     }
+}
+
+// K neighbour ids and weights of one point; 16-byte vector loads when the
+// rows are 16-byte aligned (K % 4 == 0 and aligned base pointers).
+template <int K, bool VEC>
+__device__ __forceinline__ void load_nbrs(const uint32_t* __restrict__ idx, const float* __restrict__ w,
+                                          uint32_t (&id)[K], float (&wt)[K]) {
+    if constexpr (VEC && K % 4 == 0) {
+#pragma unroll
+        for (int j = 0; j < K; j += 4) {
+            const uint4 a = *reinterpret_cast<const uint4*>(idx + j);
+            const float4 b = *reinterpret_cast<const float4*>(w + j);
+            id[j] = a.x; id[j + 1] = a.y; id[j + 2] = a.z; id[j + 3] = a.w;
+            wt[j] = b.x; wt[j + 1] = b.y; wt[j + 2] = b.z; wt[j + 3] = b.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) { id[j] = idx[j]; wt[j] = w[j]; }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void blend_point_k(const NodeXf* __restrict__ table, const uint32_t* __restrict__ idx,
+                                              const float* __restrict__ w, const float p[3], float o[3]) {
+    uint32_t id[K];
+    float wt[K];
+    load_nbrs<K, false>(idx, w, id, wt);
+    blend_point_regs<K>(table, id, wt, p, o);
 }
 
 __global__ void end_points_kernel(const float* __restrict__ pos, const float* __restrict__ rot,
@@ -142,10 +166,15 @@ __global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* 
     out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
 }
 
-// The fused per-Gaussian update. Static Gaussians (every end-point neighbour a
-// static node) still blend their end points, as the reference does, but keep
-// their attributes.
-template <int K>
+// The fused per-Gaussian update, one workgroup per 256 Gaussians in two
+// phases. Phase 1 is point-parallel: each thread blends 6 of the block's 1536
+// end points, so consecutive lanes read consecutive neighbour rows (32 B each
+// at K = 8) and the rows are consumed right after they arrive; blended points
+// go to LDS and back to `ends`. Phase 2 is Gaussian-parallel: polar factor,
+// quaternion, scale, centre and the SH rotation of one Gaussian per thread.
+// Static Gaussians (every end-point neighbour a static node) still blend their
+// end points, as the reference does, but keep their attributes.
+template <int K, bool VEC>
 __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf* __restrict__ table,
                                                         const uint32_t* __restrict__ ends_idx,
                                                         const float* __restrict__ ends_w, float* __restrict__ ends,
@@ -153,20 +182,37 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
                                                         const uint8_t* __restrict__ is_static, float pad, float coeff,
                                                         float* __restrict__ pos, float* __restrict__ rot,
                                                         float* __restrict__ scale, float* __restrict__ sh,
-                                                        const ShRotTables T) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= N) return;
-    float e[6][3];
+                                                        const ShRotTables* __restrict__ T) {
+    __shared__ float s_e[256 * 18];
+    const int64_t g0 = (int64_t)blockIdx.x * 256;
+    const int64_t npts = 6 * min((int64_t)256, N - g0);
+#pragma unroll 1
+    for (int r = 0; r < 6; r++) {
+        const int k = r * 256 + (int)threadIdx.x;
+        if (k < npts) {
+            const int64_t pt = g0 * 6 + k;
+            uint32_t id[K];
+            float wt[K];
+            load_nbrs<K, VEC>(ends_idx + pt * K, ends_w + pt * K, id, wt);
+            const float in[3] = {ends[pt * 3], ends[pt * 3 + 1], ends[pt * 3 + 2]};
+            float o[3];
+            blend_point_regs<K>(table, id, wt, in, o);
 #pragma unroll
-    for (int p = 0; p < 6; p++) {
-        float in[3] = {ends[g * 18 + p * 3], ends[g * 18 + p * 3 + 1], ends[g * 18 + p * 3 + 2]};
-        blend_point_k<K>(table, ends_idx + (g * 6 + p) * K, ends_w + (g * 6 + p) * K, in, e[p]);
+            for (int c = 0; c < 3; c++) {
+                ends[pt * 3 + c] = o[c];
+                s_e[k * 3 + c] = o[c];
+            }
+        }
     }
+    __syncthreads();
+    const int64_t g = g0 + threadIdx.x;
+    if (g >= N) return;
+    if (is_static && is_static[g]) return;
+    float e[6][3];
 #pragma unroll
     for (int p = 0; p < 6; p++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) ends[g * 18 + p * 3 + c] = e[p][c];
-    if (is_static && is_static[g]) return;
+        for (int c = 0; c < 3; c++) e[p][c] = s_e[threadIdx.x * 18 + p * 3 + c];
 
     float center[3];
 #pragma unroll
@@ -207,17 +253,7 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
     quat_normalize(qr);
     float Rr[3][3];
     quat_to_mat(qr[0], qr[1], qr[2], qr[3], Rr);
-    float c[48];
-    const float4* s4 = reinterpret_cast<const float4*>(sh + g * 48);
-#pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const float4 v = s4[i];
-        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
-    }
-    sh_rotate_resample(T, Rr, c);
-    float4* o4 = reinterpret_cast<float4*>(sh + g * 48);
-#pragma unroll
-    for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
+    sh_rotate_f4(*T, Rr, sh + g * 48);
 }
 
 // Node rotation -> orthogonal polar factor -> unit quaternion (w,x,y,z).
@@ -246,7 +282,7 @@ __global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* 
                                                         const uint32_t* __restrict__ idx,
                                                         const float* __restrict__ node_q,
                                                         const uint8_t* __restrict__ is_static,
-                                                        float* __restrict__ feat, const ShRotTables T) {
+                                                        float* __restrict__ feat, const ShRotTables* __restrict__ T) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     if (is_static && is_static[s]) return;
@@ -281,31 +317,17 @@ __global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* 
     quat_normalize(acc);
     float R[3][3];
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
-    float c[48];
-    const float4* s4 = reinterpret_cast<const float4*>(feat + s * 48);
-#pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const float4 v = s4[i];
-        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
-    }
-    sh_rotate_resample(T, R, c);
-    float4* o4 = reinterpret_cast<float4*>(feat + s * 48);
-#pragma unroll
-    for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
+    sh_rotate_f4(*T, R, feat + s * 48);
 }
 
-__global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh, const ShRotTables T) {
+__global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh,
+                                 const ShRotTables* __restrict__ T) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float R[3][3];
 #pragma unroll
     for (int k = 0; k < 9; k++) R[k / 3][k % 3] = R9[9 * i + k];
-    float c[48];
-#pragma unroll
-    for (int k = 0; k < 48; k++) c[k] = sh[48 * i + k];
-    sh_rotate_resample(T, R, c);
-#pragma unroll
-    for (int k = 0; k < 48; k++) sh[48 * i + k] = c[k];
+    sh_rotate_f4(*T, R, sh + 48 * i);
 }
 
 __global__ void polar_kernel(int64_t n, const double* __restrict__ M9, double* __restrict__ R9,
@@ -356,14 +378,19 @@ int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const 
 
 int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
-                     float* rot, float* scale, float* sh, const ShRotTables& T, hipStream_t st) {
+                     float* rot, float* scale, float* sh, const ShRotTables* T, hipStream_t st) {
     (void)M;
     if (N == 0) return 0;
+    const bool vec = ((uintptr_t)ends_idx % 16 == 0) && ((uintptr_t)ends_w % 16 == 0);
     switch (k) {
-#define ARAPGS_SP_CASE(KV)                                                                                         \
-    case KV:                                                                                                       \
-        hipLaunchKernelGGL(six_point_kernel<KV>, dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx, ends_w, \
-                           ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);                          \
+#define ARAPGS_SP_CASE(KV)                                                                                             \
+    case KV:                                                                                                           \
+        if (vec)                                                                                                       \
+            hipLaunchKernelGGL((six_point_kernel<KV, true>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx, \
+                               ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);                  \
+        else                                                                                                           \
+            hipLaunchKernelGGL((six_point_kernel<KV, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,          \
+                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);         \
         break;
         ARAPGS_SP_CASE(1) ARAPGS_SP_CASE(2) ARAPGS_SP_CASE(3) ARAPGS_SP_CASE(4) ARAPGS_SP_CASE(5) ARAPGS_SP_CASE(6)
         ARAPGS_SP_CASE(7) ARAPGS_SP_CASE(8) ARAPGS_SP_CASE(9) ARAPGS_SP_CASE(10) ARAPGS_SP_CASE(11) ARAPGS_SP_CASE(12)
@@ -382,7 +409,7 @@ int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st) {
 }
 
 int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
-                     const uint8_t* is_static, float* feat, const ShRotTables& T, hipStream_t st) {
+                     const uint8_t* is_static, float* feat, const ShRotTables* T, hipStream_t st) {
     if (S == 0) return 0;
     switch (k) {
 #define ARAPGS_SS_CASE(KV)                                                                                        \
@@ -400,7 +427,7 @@ int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, cons
     return 0;
 }
 
-int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables& T, hipStream_t st) {
+int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables* T, hipStream_t st) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(sh_rotate_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, n, R9, sh, T);
     HIP_RET(hipGetLastError());

@@ -32,6 +32,22 @@ __device__ __forceinline__ void quat_normalize(float q[4]) {
 }
 
 // Shoemake's matrix -> quaternion (SIGGRAPH 1985), float, out (w,x,y,z).
+// Largest-diagonal branch of mat_to_quat for a fixed pivot I (static indices,
+// so the matrix stays in registers).
+template <int I>
+__device__ __forceinline__ void mat_to_quat_pivot(const float m[3][3], float q[4]) {
+#pragma clang fp contract(off)
+    constexpr int J = (I + 1) % 3, K = (J + 1) % 3;
+    float s = __builtin_sqrtf(m[I][I] - m[J][J] - m[K][K] + 1.0f);
+    float v[3];
+    v[I] = 0.5f * s;
+    s = 0.5f / s;
+    q[0] = (m[K][J] - m[J][K]) * s;
+    v[J] = (m[J][I] + m[I][J]) * s;
+    v[K] = (m[K][I] + m[I][K]) * s;
+    q[1] = v[0]; q[2] = v[1]; q[3] = v[2];
+}
+
 __device__ __forceinline__ void mat_to_quat(const float m[3][3], float q[4]) {
 #pragma clang fp contract(off)
     const float t = m[0][0] + m[1][1] + m[2][2];
@@ -45,16 +61,10 @@ __device__ __forceinline__ void mat_to_quat(const float m[3][3], float q[4]) {
     } else {
         int i = 0;
         if (m[1][1] > m[0][0]) i = 1;
-        if (m[2][2] > m[i][i]) i = 2;
-        const int j = (i + 1) % 3, k = (j + 1) % 3;
-        float s = __builtin_sqrtf(m[i][i] - m[j][j] - m[k][k] + 1.0f);
-        float v[3];
-        v[i] = 0.5f * s;
-        s = 0.5f / s;
-        q[0] = (m[k][j] - m[j][k]) * s;
-        v[j] = (m[j][i] + m[i][j]) * s;
-        v[k] = (m[k][i] + m[i][k]) * s;
-        q[1] = v[0]; q[2] = v[1]; q[3] = v[2];
+        if (m[2][2] > (i == 1 ? m[1][1] : m[0][0])) i = 2;
+        if (i == 0) mat_to_quat_pivot<0>(m, q);
+        else if (i == 1) mat_to_quat_pivot<1>(m, q);
+        else mat_to_quat_pivot<2>(m, q);
     }
 }
 
@@ -176,86 +186,119 @@ struct ShRotTables {
     float inv3[7][7];
 };
 
-// sh: 48 floats (coef*3 + ch) in registers, rotated in place by R (row-major).
-__device__ __forceinline__ void sh_rotate_resample(const ShRotTables& T, const float R[3][3], float sh[48]) {
-    // band 1: basis (-y, z, -x) * C1; f'(d) = f(R^T d) => c' = M c with
-    // M[m][n] = s_m s_n R[a_m][a_n], a = (y,z,x), s = (-1, 1, -1).
-    {
-        const int ax[3] = {1, 2, 0};
-        const float sg[3] = {-1.0f, 1.0f, -1.0f};
-        float o[3][3];
+// SH rotation f'(d) = f(R^T d) band by band (R row-major, coefficients
+// coef-major, channel-minor: band l holds coefficients l^2 .. (l+1)^2 - 1, so
+// band 1 = floats 3..11, band 2 = 12..26, band 3 = 27..47; band 0 is invariant).
+// Band 1: basis (-y, z, -x) * C1, so c' = M c with M[m][n] = s_m s_n R[a_m][a_n],
+// a = (y, z, x), s = (-1, 1, -1). Bands 2 and 3 resample: evaluate the band at
+// the 2l+1 rotated table directions R^T d_k and map the values back to
+// coefficients with the table's inverse basis matrix.
+__device__ __forceinline__ void sh_rotate_band1(const float R[3][3], float (&b)[9]) {
+    const int ax[3] = {1, 2, 0};
+    const float sg[3] = {-1.0f, 1.0f, -1.0f};
+    float o[3][3];
 #pragma unroll
-        for (int m = 0; m < 3; m++)
+    for (int m = 0; m < 3; m++)
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float acc = 0.0f;
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = 0.0f;
 #pragma unroll
-                for (int n = 0; n < 3; n++) acc += sg[m] * sg[n] * R[ax[m]][ax[n]] * sh[(1 + n) * 3 + ch];
-                o[m][ch] = acc;
-            }
-#pragma unroll
-        for (int m = 0; m < 3; m++)
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) sh[(1 + m) * 3 + ch] = o[m][ch];
-    }
-    // band 2
-    {
-        float F[5][3];
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const float dx = T.dir2[k][0], dy = T.dir2[k][1], dz = T.dir2[k][2];
-            const float rx = R[0][0] * dx + R[1][0] * dy + R[2][0] * dz;
-            const float ry = R[0][1] * dx + R[1][1] * dy + R[2][1] * dz;
-            const float rz = R[0][2] * dx + R[1][2] * dy + R[2][2] * dz;
-            float b[5];
-            sh_band2(rx, ry, rz, b);
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int m = 0; m < 5; m++) acc += b[m] * sh[(4 + m) * 3 + ch];
-                F[k][ch] = acc;
-            }
+            for (int n = 0; n < 3; n++) acc += sg[m] * sg[n] * R[ax[m]][ax[n]] * b[n * 3 + ch];
+            o[m][ch] = acc;
         }
 #pragma unroll
-        for (int m = 0; m < 5; m++)
+    for (int m = 0; m < 3; m++)
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float acc = 0.0f;
+        for (int ch = 0; ch < 3; ch++) b[m * 3 + ch] = o[m][ch];
+}
+
+__device__ __forceinline__ void sh_rotate_band2(const ShRotTables& T, const float R[3][3], float (&b)[15]) {
+    float F[5][3];
 #pragma unroll
-                for (int k = 0; k < 5; k++) acc += T.inv2[m][k] * F[k][ch];
-                sh[(4 + m) * 3 + ch] = acc;
-            }
-    }
-    // band 3
-    {
-        float F[7][3];
+    for (int k = 0; k < 5; k++) {
+        const float dx = T.dir2[k][0], dy = T.dir2[k][1], dz = T.dir2[k][2];
+        const float rx = R[0][0] * dx + R[1][0] * dy + R[2][0] * dz;
+        const float ry = R[0][1] * dx + R[1][1] * dy + R[2][1] * dz;
+        const float rz = R[0][2] * dx + R[1][2] * dy + R[2][2] * dz;
+        float y[5];
+        sh_band2(rx, ry, rz, y);
 #pragma unroll
-        for (int k = 0; k < 7; k++) {
-            const float dx = T.dir3[k][0], dy = T.dir3[k][1], dz = T.dir3[k][2];
-            const float rx = R[0][0] * dx + R[1][0] * dy + R[2][0] * dz;
-            const float ry = R[0][1] * dx + R[1][1] * dy + R[2][1] * dz;
-            const float rz = R[0][2] * dx + R[1][2] * dy + R[2][2] * dz;
-            float b[7];
-            sh_band3(rx, ry, rz, b);
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = 0.0f;
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int m = 0; m < 7; m++) acc += b[m] * sh[(9 + m) * 3 + ch];
-                F[k][ch] = acc;
-            }
+            for (int m = 0; m < 5; m++) acc += y[m] * b[m * 3 + ch];
+            F[k][ch] = acc;
         }
-#pragma unroll
-        for (int m = 0; m < 7; m++)
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int k = 0; k < 7; k++) acc += T.inv3[m][k] * F[k][ch];
-                sh[(9 + m) * 3 + ch] = acc;
-            }
     }
+#pragma unroll
+    for (int m = 0; m < 5; m++)
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 5; k++) acc += T.inv2[m][k] * F[k][ch];
+            b[m * 3 + ch] = acc;
+        }
+}
+
+__device__ __forceinline__ void sh_rotate_band3(const ShRotTables& T, const float R[3][3], float (&b)[21]) {
+    float F[7][3];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const float dx = T.dir3[k][0], dy = T.dir3[k][1], dz = T.dir3[k][2];
+        const float rx = R[0][0] * dx + R[1][0] * dy + R[2][0] * dz;
+        const float ry = R[0][1] * dx + R[1][1] * dy + R[2][1] * dz;
+        const float rz = R[0][2] * dx + R[1][2] * dy + R[2][2] * dz;
+        float y[7];
+        sh_band3(rx, ry, rz, y);
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int m = 0; m < 7; m++) acc += y[m] * b[m * 3 + ch];
+            F[k][ch] = acc;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 7; m++)
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 7; k++) acc += T.inv3[m][k] * F[k][ch];
+            b[m * 3 + ch] = acc;
+        }
+}
+
+// Rotates one 48-float SH set in global memory in place: twelve 16-byte loads
+// and stores (sh must be 16-byte aligned), bands 1-3 rotated in registers.
+__device__ __forceinline__ void sh_rotate_f4(const ShRotTables& T, const float R[3][3], float* __restrict__ sh) {
+    float c[48];
+    const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 v = s4[i];
+        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+    }
+    float b1[9], b2[15], b3[21];
+#pragma unroll
+    for (int i = 0; i < 9; i++) b1[i] = c[3 + i];
+#pragma unroll
+    for (int i = 0; i < 15; i++) b2[i] = c[12 + i];
+#pragma unroll
+    for (int i = 0; i < 21; i++) b3[i] = c[27 + i];
+    sh_rotate_band1(R, b1);
+    sh_rotate_band2(T, R, b2);
+    sh_rotate_band3(T, R, b3);
+#pragma unroll
+    for (int i = 0; i < 9; i++) c[3 + i] = b1[i];
+#pragma unroll
+    for (int i = 0; i < 15; i++) c[12 + i] = b2[i];
+#pragma unroll
+    for (int i = 0; i < 21; i++) c[27 + i] = b3[i];
+    float4* o4 = reinterpret_cast<float4*>(sh);
+#pragma unroll
+    for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
 }
 
 }  // namespace arapgs

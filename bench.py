@@ -36,7 +36,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "arap-deformation-of-gaussian-radiance-fields_amd")
 HBM_PEAK_GBS = 8000.0
-SIX_POINT_BYTES_PER_GAUSSIAN = 980  # 6 ends 72 + idx 192 + w 192 + s0 12 + q 16 + SH 192 read; mu 12 + q 16 + s 12 + SH 192 + ends 72 written
+# 6 ends 72 + idx 192 + w 192 + s0 12 + static flag 1 + q 16 + SH 192 read;
+# mu 12 + q 16 + s 12 + SH 192 + ends 72 written
+SIX_POINT_BYTES_PER_GAUSSIAN = 981
 
 
 def load_package():
@@ -222,6 +224,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,fast")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the per-frame path is single-GPU. "
@@ -229,6 +233,16 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.rand(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
     B = setup(args)
     res = {}
     for form in args.forms.split(","):
@@ -269,7 +283,7 @@ def main():
         line["plain_p50_ms"] = round(pct(rp["frame_ms"], 50), 3)
         line["plain_p99_ms"] = round(pct(rp["frame_ms"], 99), 3)
         line["speedup_vs_plain"] = round(fps / line["plain_fps"], 3)
-    line["roofline"] = {"bound": "hbm", "kernel": "six_point_kernel<8> (fused end-point blend + polar + SH)",
+    line["roofline"] = {"bound": "hbm", "kernel": "six_point_kernel<8, true> (fused end-point blend + polar + SH)",
                         "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
                         "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
