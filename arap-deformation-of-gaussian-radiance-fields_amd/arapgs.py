@@ -81,6 +81,7 @@ EXPORTS = {
     "arapgs_density_sample": ([P, P, P, P, P, P, P, C.c_float, P, P], C.c_int),
     "arapgs_solver_set_host_checks": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
     "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
     "arapgs_aims_translate": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_aims_twist": ([P, P, C.c_int, P, C.c_int, P, C.c_float, P], C.c_int),
@@ -403,6 +404,10 @@ class Solver:
 
     def set_coarse_refresh(self, frames):
         self.ctx._check(lib().arapgs_solver_set_coarse_refresh(self.h, frames))
+
+    def set_cg_mode(self, mode):
+        """0 = three kernels per CG iteration (default), 1 = persistent cooperative PCG kernel."""
+        self.ctx._check(lib().arapgs_solver_set_cg_mode(self.h, mode))
 
     def deform_frame(self, fb, aims, iters, stop_rule=True, energies=None, iters_done=None):
         t = self.ctx._torch

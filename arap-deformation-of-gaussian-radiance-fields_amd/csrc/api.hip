@@ -58,6 +58,7 @@ struct arapgs_solver {
     arapgs_ctx* ctx = nullptr;
     SolverWork w;
     int host_check_every = 0;   // 0: device-side convergence only
+    int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG
     int coarse_refresh = 8;     // frames between coarse-operator rebuilds
     int64_t solves = 0;
     // per-frame scratch (M nodes)
@@ -297,6 +298,12 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every) {
     return 0;
 }
 
+int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode) {
+    if (!s || mode < 0 || mode > 1) return -1;
+    s->cg_mode = mode;
+    return 0;
+}
+
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames) {
     if (!s || frames < 1) return -1;
     s->coarse_refresh = frames;
@@ -317,7 +324,12 @@ namespace {
 __global__ void copy_f32_kernel(float* __restrict__ dst, const float* __restrict__ src, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
 }
-__global__ void copy_done_kernel(const arapgs::GnScalars* __restrict__ gs, int* __restrict__ out) { *out = gs->iters_done; }
+// iters_done for the caller; -1 when the persistent PCG kernel's grid barrier
+// gave up waiting (the solve is then invalid).
+__global__ void copy_done_kernel(const arapgs::GnScalars* __restrict__ gs, const arapgs::GridBarrier* __restrict__ gb,
+                                 int* __restrict__ out) {
+    *out = gb->err ? -1 : gs->iters_done;
+}
 }  // namespace
 
 extern "C" {
@@ -342,13 +354,18 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     for (int it = 0; it < iters; it++) {
         rc = check_launch(c, solver_gn_begin(W, c->stream), "gn_solve rhs");
         if (rc) return rc;
-        for (int k = 0; k < W.cg_max_iters; k++) {
-            rc = check_launch(c, solver_cg_iteration(W, k, c->stream), "gn_solve cg");
+        if (s->cg_mode == 1) {
+            rc = check_launch(c, solver_cg_persistent(W, c->stream), "gn_solve cg");
             if (rc) return rc;
-            if (every > 0 && (k + 1) % every == 0) {
-                int done = 0;
-                if ((rc = host_flag(W.cg_done, &done))) return rc;
-                if (done) break;
+        } else {
+            for (int k = 0; k < W.cg_max_iters; k++) {
+                rc = check_launch(c, solver_cg_iteration(W, k, c->stream), "gn_solve cg");
+                if (rc) return rc;
+                if (every > 0 && (k + 1) % every == 0) {
+                    int done = 0;
+                    if ((rc = host_flag(W.cg_done, &done))) return rc;
+                    if (done) break;
+                }
             }
         }
         rc = check_launch(c, solver_gn_end(W, it, stop_rule, energies, c->stream), "gn_solve line search");
@@ -365,7 +382,9 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     }
     if (x_out)
         hipLaunchKernelGGL(copy_f32_kernel, dim3(64), dim3(256), 0, c->stream, x_out, (const float*)W.x, W.n);
-    if (iters_done) hipLaunchKernelGGL(copy_done_kernel, dim3(1), dim3(1), 0, c->stream, W.gs, iters_done);
+    if (iters_done)
+        hipLaunchKernelGGL(copy_done_kernel, dim3(1), dim3(1), 0, c->stream, W.gs,
+                           (const GridBarrier*)W.gbar, iters_done);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
 }

@@ -19,8 +19,14 @@
 // whose 12n_a x 12n_a Galerkin matrix is assembled and inverted once per frame.
 // The step-halving line search evaluates all 50 halvings of the reference
 // (alpha = 1 .. 2^-49) in one pass and takes the first with E <= E_old.
+//
+// No FMA contraction: every kernel that shares a phase body rounds the same
+// way, so the persistent and the per-phase launch paths agree bit for bit.
+#pragma clang fp contract(off)
 #include "common.hpp"
 #include "solver.hpp"
+
+#include <algorithm>
 
 namespace arapgs {
 
@@ -428,32 +434,27 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
     return r;
 }
 
-// PCG, three kernels per iteration k (no host round trips, no scalar kernels).
+// PCG iteration k in three phases (no host round trips, no scalar kernels).
 // The search direction's product is carried recursively (Chronopoulos-Gear):
 // with s = A z,  p_k = z + beta_k p_{k-1}  and  q_k = A p_k = s + beta_k q_{k-1},
 // so the operator only ever reads z, and p, q are updated node-locally.
 //   apply   beta_k = rz_k / rz_{k-1} (0 at k = 0); s = A z; p, q as above
-//           (in place); partial p_k.q_k
+//           (in place); partial p_k.q_k per owner block
 //   update  alpha = rz_k / (p_k.q_k); h += alpha p_k; r -= alpha q_k; z = Binv r;
 //           rc = P^T r; partial |r|^2               (one workgroup per aggregate)
-//   coarse  z += P (Ac^-1 rc); partial rz_{k+1} = r.z; workgroup 0 tests
-//           |r|^2 <= tol^2 |b|^2 and the iteration cap and raises `done`
-// Every workgroup reduces the (few dozen) FP64 partials it needs itself.
-__global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
-                                                    const float* __restrict__ z, const double* __restrict__ rz_cur,
-                                                    const double* __restrict__ rz_prev, int first,
-                                                    float* __restrict__ p, float* __restrict__ q,
-                                                    double* __restrict__ partial, const int* __restrict__ skip) {
-    if (*skip) return;
-    __shared__ double shd[8];
-    float beta = 0.0f;
-    if (!first) {
-        const double den = block_sum_partials(rz_prev, S.n_agg, shd);
-        const double num = block_sum_partials(rz_cur, S.n_agg, shd);
-        beta = den != 0.0 ? (float)(num / den) : 0.0f;
-    }
+//   coarse  z += P (Ac^-1 rc); partial rz_{k+1} = r.z
+// then |r|^2 <= tol^2 |b|^2 or the iteration cap ends the solve. The phases run
+// either as three kernels per iteration (solver_cg_iteration) or inside one
+// persistent cooperative kernel per solve with grid barriers between them
+// (solver_cg_persistent). Every workgroup reduces the (few dozen) FP64 partials
+// it needs itself, in the same order, so all of them take the same decisions.
+
+// apply for owner block ob (16 free nodes, 16 lanes each)
+__device__ __forceinline__ void apply_body(const SolverDev& S, const float* __restrict__ x,
+                                           const float* __restrict__ z, float beta, bool first, float* __restrict__ p,
+                                           float* __restrict__ q, double* __restrict__ partial, int ob, double* shd) {
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    const int fi = blockIdx.x * (256 / LANES) + grp;
+    const int fi = ob * (256 / LANES) + grp;
     float acc[12];
 #pragma unroll
     for (int a = 0; a < 12; a++) acc[a] = 0.0f;
@@ -483,29 +484,16 @@ __global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __
         pq = (double)pn * qn;
     }
     const double t = block_reduce_d(pq, shd);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (threadIdx.x == 0) partial[ob] = t;
 }
 
-// One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
-// z = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
-__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const double* __restrict__ pq_part,
-                                                              int n_pq, const double* __restrict__ rz_cur,
-                                                              float* __restrict__ hsol, float* __restrict__ r,
-                                                              const float* __restrict__ p, const float* __restrict__ q,
-                                                              const float* __restrict__ binv, float* __restrict__ z,
-                                                              float* __restrict__ rc, double* __restrict__ partial,
-                                                              const int* __restrict__ skip) {
-    if (*skip) return;
-    const int a = blockIdx.x;
+// update + restriction for aggregate a; mode 0 skips the h, r update (z0 = M^-1 r0)
+__device__ __forceinline__ void update_body(const SolverDev& S, int mode, float alpha, float* __restrict__ hsol,
+                                            float* __restrict__ r, const float* __restrict__ p,
+                                            const float* __restrict__ q, const float* __restrict__ binv,
+                                            float* __restrict__ z, float* __restrict__ rc,
+                                            double* __restrict__ partial, int a, float (*sred)[12], double* shd) {
     const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
-    __shared__ float sred[256 / LANES][12];
-    __shared__ double shd[8];
-    float alpha = 0.0f;
-    if (mode == 1) {
-        const double pq = block_sum_partials(pq_part, n_pq, shd);
-        const double rz = block_sum_partials(rz_cur, S.n_agg, shd);
-        alpha = pq != 0.0 ? (float)(rz / pq) : 0.0f;
-    }
     const float* ca = S.agg_center + 3 * a;
     float rcl[12];
 #pragma unroll
@@ -563,19 +551,13 @@ __global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int m
     if (threadIdx.x == 0) partial[a] = tr;
 }
 
-// One workgroup per aggregate: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z.
-// Workgroup 0 also runs the convergence test on the |r|^2 partials.
-__global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const float* __restrict__ acinv,
-                                                             const float* __restrict__ rc, const float* __restrict__ r,
-                                                             float* __restrict__ z, double* __restrict__ rz_out,
-                                                             const double* __restrict__ rr_part,
-                                                             CgScalars* __restrict__ cs, int test, int* __restrict__ skip) {
-    if (*skip) return;
-    const int a = blockIdx.x;
+// coarse correction for aggregate a: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z
+__device__ __forceinline__ void prolong_body(const SolverDev& S, const float* __restrict__ acinv,
+                                             const float* __restrict__ rc, const float* __restrict__ r,
+                                             float* __restrict__ z, double* __restrict__ rz_out, int a, float* zc,
+                                             double* shd) {
     const int nc = 12 * S.n_agg;
     const int ld = S.nc_pad;
-    __shared__ float zc[12];
-    __shared__ double shd[8];
     // 12 rows of Ac^-1 times rc: 16 lanes per row
     {
         const int row = threadIdx.x / LANES, lane = threadIdx.x % LANES;
@@ -607,15 +589,162 @@ __global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const 
         rz += (double)zz * r[o];
     }
     const double t = block_reduce_d(rz, shd);
+    if (threadIdx.x == 0) rz_out[a] = t;
+    __syncthreads();  // zc is reused by the caller's next aggregate
+}
+
+__device__ __forceinline__ float cg_beta(const SolverDev& S, const double* rz_cur, const double* rz_prev, bool first,
+                                         double* shd) {
+    if (first) return 0.0f;
+    const double den = block_sum_partials(rz_prev, S.n_agg, shd);
+    const double num = block_sum_partials(rz_cur, S.n_agg, shd);
+    return den != 0.0 ? (float)(num / den) : 0.0f;
+}
+
+__device__ __forceinline__ float cg_alpha(const SolverDev& S, const double* pq_part, int n_pq, const double* rz_cur,
+                                          double* shd) {
+    const double pq = block_sum_partials(pq_part, n_pq, shd);
+    const double rz = block_sum_partials(rz_cur, S.n_agg, shd);
+    return pq != 0.0 ? (float)(rz / pq) : 0.0f;
+}
+
+__global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
+                                                    const float* __restrict__ z, const double* __restrict__ rz_cur,
+                                                    const double* __restrict__ rz_prev, int first,
+                                                    float* __restrict__ p, float* __restrict__ q,
+                                                    double* __restrict__ partial, const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ double shd[8];
+    const float beta = cg_beta(S, rz_cur, rz_prev, first != 0, shd);
+    apply_body(S, x, z, beta, first != 0, p, q, partial, blockIdx.x, shd);
+}
+
+// One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
+// z = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
+__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const double* __restrict__ pq_part,
+                                                              int n_pq, const double* __restrict__ rz_cur,
+                                                              float* __restrict__ hsol, float* __restrict__ r,
+                                                              const float* __restrict__ p, const float* __restrict__ q,
+                                                              const float* __restrict__ binv, float* __restrict__ z,
+                                                              float* __restrict__ rc, double* __restrict__ partial,
+                                                              const int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ float sred[256 / LANES][12];
+    __shared__ double shd[8];
+    const float alpha = mode == 1 ? cg_alpha(S, pq_part, n_pq, rz_cur, shd) : 0.0f;
+    update_body(S, mode, alpha, hsol, r, p, q, binv, z, rc, partial, blockIdx.x, sred, shd);
+}
+
+// One workgroup per aggregate: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z.
+// Workgroup 0 also runs the convergence test on the |r|^2 partials.
+__global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const float* __restrict__ acinv,
+                                                             const float* __restrict__ rc, const float* __restrict__ r,
+                                                             float* __restrict__ z, double* __restrict__ rz_out,
+                                                             const double* __restrict__ rr_part,
+                                                             CgScalars* __restrict__ cs, int test, int* __restrict__ skip) {
+    if (*skip) return;
+    __shared__ float zc[12];
+    __shared__ double shd[8];
+    const int a = blockIdx.x;
+    prolong_body(S, acinv, rc, r, z, rz_out, a, zc, shd);
     const double rr = (a == 0 && test) ? block_sum_partials(rr_part, S.n_agg, shd) : 0.0;
-    if (threadIdx.x == 0) {
-        rz_out[a] = t;
-        if (a == 0 && test) {
-            cs->rr = rr;
-            cs->iters += 1;
-            if (rr <= cs->tol2 * cs->bb || cs->iters >= cs->max_iters) *skip = 1;
-        }
+    if (threadIdx.x == 0 && a == 0 && test) {
+        cs->rr = rr;
+        cs->iters += 1;
+        if (rr <= cs->tol2 * cs->bb || cs->iters >= cs->max_iters) *skip = 1;
     }
+}
+
+// ------------------------------------------------------------ grid barrier
+// Generation barrier for a cooperative launch (all workgroups co-resident).
+// Every thread first fences its own global writes at device scope (a fence
+// only waits for the issuing wave's stores, so one per wave is needed), then
+// thread 0 arrives on the monotone counter and waits for the generation to
+// reach its epoch + 1; after the workgroup barrier every thread fences again
+// (acquire) before reading other workgroups' data. The wait is bounded: after
+// GRID_SPIN_LIMIT polls, or when another workgroup has raised `err`, the
+// workgroup raises `err` and leaves, so the kernel always drains.
+constexpr unsigned GRID_SPIN_LIMIT = 1u << 24;
+
+__device__ __forceinline__ bool grid_sync(GridBarrier* gb, unsigned& epoch) {
+    __shared__ int s_ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned target = epoch + 1;
+        const unsigned arrived = __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == target * gridDim.x - 1) {
+            __hip_atomic_store(&gb->gen, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins >= GRID_SPIN_LIMIT ||
+                    __hip_atomic_load(&gb->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    __hip_atomic_store(&gb->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __threadfence();
+    epoch += 1;
+    return s_ok != 0;
+}
+
+// The whole PCG solve of one Gauss-Newton iteration in one cooperative launch.
+// Preconditions (queued before it): r = b, z0 = M^-1 r0 and rz_0 in rzp[0..n_agg),
+// cs->bb, cs->iters = 0; gb zeroed. Owner blocks and aggregates are strided
+// over the grid, so any grid size up to the co-resident limit works.
+__global__ void __launch_bounds__(256) pcg_persistent_kernel(SolverDev S, const float* __restrict__ x,
+                                                             float* __restrict__ hsol, float* __restrict__ r,
+                                                             float* __restrict__ z, float* __restrict__ p,
+                                                             float* __restrict__ q, const float* __restrict__ binv,
+                                                             float* __restrict__ rc, const float* __restrict__ acinv,
+                                                             double* __restrict__ pq_part, double* __restrict__ rr_part,
+                                                             double* __restrict__ rzp, CgScalars* __restrict__ cs,
+                                                             GridBarrier* __restrict__ gb, int* __restrict__ skip) {
+    if (*skip) return;  // read by every workgroup before anything is written
+    __shared__ float sred[256 / LANES][12];
+    __shared__ float zc[12];
+    __shared__ double shd[8];
+    const int n_ob = (S.nF + (256 / LANES) - 1) / (256 / LANES);
+    const int max_iters = cs->max_iters;
+    const double tol2bb = cs->tol2 * cs->bb;
+    unsigned epoch = 0;
+    int k = 0;
+    double rr = 0.0;
+    for (;; k++) {
+        const double* rz_cur = rzp + (k & 1) * S.n_agg;
+        double* rz_next = rzp + ((k + 1) & 1) * S.n_agg;
+        const float beta = cg_beta(S, rz_cur, rz_next, k == 0, shd);
+        for (int ob = blockIdx.x; ob < n_ob; ob += gridDim.x) apply_body(S, x, z, beta, k == 0, p, q, pq_part, ob, shd);
+        if (!grid_sync(gb, epoch)) return;
+        const float alpha = cg_alpha(S, pq_part, n_ob, rz_cur, shd);
+        for (int a = blockIdx.x; a < S.n_agg; a += gridDim.x)
+            update_body(S, 1, alpha, hsol, r, p, q, binv, z, rc, rr_part, a, sred, shd);
+        if (!grid_sync(gb, epoch)) return;
+        rr = block_sum_partials(rr_part, S.n_agg, shd);
+        if (rr <= tol2bb || k + 1 >= max_iters) break;
+        for (int a = blockIdx.x; a < S.n_agg; a += gridDim.x) prolong_body(S, acinv, rc, r, z, rz_next, a, zc, shd);
+        if (!grid_sync(gb, epoch)) return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cs->rr = rr;
+        cs->iters = k + 1;
+        *skip = 1;
+    }
+}
+
+__global__ void grid_barrier_reset_kernel(GridBarrier* gb) {
+    gb->count = 0u;
+    gb->gen = 0u;
+    gb->err = 0;
+    gb->pad = 0;
 }
 
 // |b|^2 from the rhs partials; an all-zero right-hand side ends the solve.
@@ -1033,6 +1162,37 @@ int solver_cg_iteration(SolverWork& W, int k, hipStream_t st) {
     hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z, rz_prev,
                        W.partial2, W.cg, 1, W.cg_done);
     HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_cg_persistent(SolverWork& W, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    if (W.coop_grid <= 0) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        HIP_RET(hipGetDevice(&dev));
+        HIP_RET(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_RET(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_persistent_kernel),
+                                                             256, 0));
+        const int want = std::max((int)nblk(S.nF, 256 / LANES), S.n_agg);
+        W.coop_grid = std::max(1, std::min(want, per_cu * ncu));
+    }
+    GridBarrier* gb = reinterpret_cast<GridBarrier*>(W.gbar);
+    hipLaunchKernelGGL(grid_barrier_reset_kernel, dim3(1), dim3(1), 0, st, gb);
+    const float* x = W.x;
+    const float* binv = W.binv;
+    const float* acinv = W.acinv;
+    SolverDev Sv = S;
+    void* args[] = {&Sv,  (void*)&x,   &W.hsol,    &W.r,     &W.z,   &W.p, &W.q,       (void*)&binv,
+                    &W.rc, (void*)&acinv, &W.partial, &W.partial2, &W.rzp, &W.cg, &gb, &W.cg_done};
+    HIP_RET(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pcg_persistent_kernel), dim3(W.coop_grid),
+                                       dim3(256), args, 0, st));
+    return 0;
+}
+
+int solver_barrier_error(SolverWork& W, int* err) {
+    GridBarrier gb;
+    HIP_RET(hipMemcpy(&gb, W.gbar, sizeof gb, hipMemcpyDeviceToHost));
+    *err = gb.err;
     return 0;
 }
 

@@ -49,6 +49,14 @@ struct GnScalars {
     int pad;
 };
 
+// Counter, generation and error flag of the persistent PCG kernel's grid barrier.
+struct GridBarrier {
+    unsigned count;
+    unsigned gen;
+    int err;
+    int pad;
+};
+
 struct SolverWork {
     SolverDev dev;
     int n;  // 12 * nF
@@ -61,6 +69,8 @@ struct SolverWork {
     int* cg_done;
     int cg_max_iters;
     double cg_tol;
+    void* gbar;        // grid barrier of the persistent PCG kernel
+    int coop_grid;     // its grid size (0: not yet chosen)
     std::vector<void*> allocs;  // owned device buffers
 };
 
@@ -86,6 +96,11 @@ void solver_free(SolverWork& W);
 int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st);
 int solver_gn_begin(SolverWork& W, hipStream_t st);
 int solver_cg_iteration(SolverWork& W, int k, hipStream_t st);
+// The whole PCG solve of one GN iteration as one cooperative launch (after
+// solver_gn_begin); same arithmetic and stopping rule as solver_cg_iteration.
+int solver_cg_persistent(SolverWork& W, hipStream_t st);
+// Reads the persistent kernel's barrier error flag (synchronous).
+int solver_barrier_error(SolverWork& W, int* err);
 int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st);
 int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st);
 

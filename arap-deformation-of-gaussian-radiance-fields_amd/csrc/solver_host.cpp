@@ -197,6 +197,8 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.gs = alloc<GnScalars>(W, 1, rc);
     W.gn_done = alloc<int>(W, 1, rc);
     W.cg_done = alloc<int>(W, 1, rc);
+    W.gbar = alloc<int>(W, 4, rc);
+    W.coop_grid = 0;
     W.cg_max_iters = 100;
     W.cg_tol = 1e-4;
     if (rc) { err = "solver: device allocation failed"; solver_free(W); return rc; }

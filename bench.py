@@ -13,11 +13,14 @@ blend -> node quaternions + sample SH rotation -> 4 renders. All inputs are
 resident in HBM before the timed region.
 
 Two forms are timed on the same state:
-  plain  every stage synchronous, CG convergence read back to the host after
-         every CG iteration (the reference's call pattern: a host-driven solver
-         and a device sync after each launch);
-  fast   stages queued back to back; CG convergence checked on the host only
-         every 16 iterations; views rendered back to back.
+  plain  every stage synchronous, three kernels per CG iteration with the
+         convergence flag read back to the host after every CG iteration (the
+         reference's call pattern: a host-driven solver and a device sync after
+         each launch);
+  fast   stages queued back to back; the CG phases carry a device-side
+         convergence flag (kernels queued past convergence exit at once) and
+         the host looks at it only every 16 CG iterations; views rendered back
+         to back.
 `value` is the fast form's frames/s; `plain_fps` and the per-form p50/p99 frame
 latencies are reported alongside.
 
@@ -136,6 +139,7 @@ def frame(B, plain):
 
 def timed(B, plain, steps, warmup):
     import torch
+    B["solver"].set_cg_mode(0)
     B["solver"].set_host_checks(1 if plain else 16)
     for _ in range(warmup):
         frame(B, plain)

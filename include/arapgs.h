@@ -106,7 +106,8 @@ int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h)
  * node_pos f32[M*3] current node positions, aims f32[M*3].
  * x_out f32[12*n_free] (optional), energies f64[iters+1] (energy at the start
  * of each iteration; entry iters_done is the final energy), iters_done int[1].
- * All outputs are device buffers; nothing is read back. */
+ * All outputs are device buffers; nothing is read back. iters_done is -1 when
+ * the persistent PCG kernel's grid barrier gave up waiting (solve invalid). */
 int arapgs_gn_solve(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, int iters,
                     int stop_rule, float* x_out, double* energies, int* iters_done);
 
@@ -120,6 +121,14 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every);
  * `frames` solves (default 8); in between it is reused. It only affects the
  * CG iteration count, never the solution. */
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
+
+/* How each GN iteration's PCG solve (the dense Eigen solve of Deform.cpp:118-128
+ * in the reference) is launched: 0 (default) = three kernels per CG iteration
+ * (combine with set_host_checks to bound the launches queued past convergence);
+ * 1 = one persistent cooperative kernel per solve with grid barriers between
+ * the CG phases. Both compute the same iterates bit for bit; on MI355X a grid
+ * barrier costs more than a kernel boundary, so mode 0 is the faster one. */
+int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode);
 
 /* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,
  * Deform.cpp:378-581). energy: device f64[1]. */

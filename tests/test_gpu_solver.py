@@ -90,3 +90,26 @@ def test_static_blocks(ctx, prob):
     x, e, done = s.solve(T(prob["nodes"]), T(aims), 6, stop_rule=True)
     e = H(e)[: int(H(done)[0]) + 1]
     assert abs(e[-1] - e_ref[-1]) <= 1e-5 * e_ref[-1]
+
+
+@pytest.mark.parametrize("n_nodes", [128, 1024])
+def test_cg_launch_modes_identical(ctx, n_nodes):
+    """The persistent cooperative PCG kernel and the three-kernels-per-iteration
+    path (with and without a host-driven loop) compute the same iterates bit for
+    bit, and repeated persistent solves are deterministic. One solver instance
+    is used so that all solves share one coarse operator (assembled with float
+    atomics once, on the first solve)."""
+    sc = small_scene(n=20000 if n_nodes == 128 else 60000, n_nodes=n_nodes, seed=1)
+    aims, _ = _problem(sc, 0.7)
+    s = _solver(ctx, sc)
+    s.set_coarse_refresh(1000)
+    outs = []
+    for mode, every in ((1, 0), (0, 1), (0, 0), (1, 0)):
+        s.set_cg_mode(mode)
+        s.set_host_checks(every)
+        x, e, done = s.solve(T(sc["nodes"]), T(aims), 6, stop_rule=True)
+        outs.append((H(x), H(e), int(H(done)[0])))
+    for x, e, done in outs[1:]:
+        assert done == outs[0][2] and done >= 1
+        assert np.array_equal(x, outs[0][0])
+        assert np.array_equal(e[: done + 1], outs[0][1][: done + 1])
