@@ -161,6 +161,21 @@ def timed(B, plain, steps, warmup):
     return dict(total_s=total, frame_ms=times, kernel_ms=kernel_ms, instances=n_inst)
 
 
+def committed_traffic(args):
+    """HBM bytes per six-point launch measured with rocprofv3 PMC counters
+    (FETCH_SIZE, WRITE_SIZE, scaled by a plain-copy calibration) on the default
+    workload, read from the committed summary; None for other workloads."""
+    default = (args.n, args.nodes, args.seed) == (1_000_000, 2048, 0)
+    path = os.path.join(ROOT, "profiles", "six_point_kernel_capsules-1M_traffic.md")
+    if not default or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch six_point_kernel:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def pct(v, q):
     s = sorted(v)
     if not s:
@@ -289,7 +304,8 @@ def main():
         line["speedup_vs_plain"] = round(fps / line["plain_fps"], 3)
     line["roofline"] = {"bound": "hbm", "kernel": "six_point_kernel<8, true> (fused end-point blend + polar + SH)",
                         "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                        "traffic": committed_traffic(args),
                         "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
                         "launches": int(len(kms))}
     if not args.no_cpu:
