@@ -1,0 +1,164 @@
+"""Oracle rasteriser (the 3DGS forward behind Rasterizer::forward, called at
+GaussianView.cpp:1099-1128; the fork itself is un-vendored) pinned on the CPU by
+a vectorised numpy restatement of the 3DGS equations (Kerbl et al. 2023):
+
+  projection   p_view = V p, p_ndc = (P p) / w, pixel = ((ndc + 1) W - 1) / 2
+  EWA          Sigma2 = J W Sigma3 W^T J^T + 0.3 I, J clamped at 1.3 tan(fov)
+  extent       radius = ceil(3 sqrt(lambda_max)); per-axis rects ceil(3 sqrt(Sigma2_ii))
+  colour       degree-3 SH at (p - campos)/|p - campos|, + 0.5, clamped at 0
+  binning      one (tile << 32 | depth bits) key per touched tile, emitted in
+               Gaussian order, stable sort, per-tile [start, end)
+  blend        front to back per pixel: alpha = min(0.99, o exp(power)), skip
+               power > 0 or alpha < 1/255, stop before T drops below 1e-4
+
+Keys, values and ranges must match exactly; pixels within 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from arapgs_amd import scenes as S
+
+C0 = 0.28209479177387814
+C1 = 0.4886025119029199
+C2 = [1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792, 0.5462742152960396]
+C3 = [-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154, -0.4570457994644658,
+      1.445305721320277, -0.5900435899266435]
+
+
+def quat_mats(q):
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def sh_color(sh, d):
+    x, y, z = d[:, 0:1], d[:, 1:2], d[:, 2:3]
+    xx, yy, zz = x * x, y * y, z * z
+    c = sh.reshape(-1, 16, 3)
+    b = [np.full_like(x, C0), -C1 * y, C1 * z, -C1 * x, C2[0] * x * y, C2[1] * y * z, C2[2] * (2 * zz - xx - yy),
+         C2[3] * x * z, C2[4] * (xx - yy), C3[0] * y * (3 * xx - yy), C3[1] * x * y * z,
+         C3[2] * y * (4 * zz - xx - yy), C3[3] * z * (2 * zz - 3 * xx - 3 * yy), C3[4] * x * (4 * zz - xx - yy),
+         C3[5] * z * (xx - yy), C3[6] * x * (xx - 3 * yy)]
+    out = sum(b[i] * c[:, i, :] for i in range(16))
+    return np.maximum(out + 0.5, 0.0)
+
+
+def restatement(g, cam, W, H, tile=16):
+    V = cam.view.reshape(4, 4).T.astype(np.float64)      # column-major storage
+    P = cam.proj.reshape(4, 4).T.astype(np.float64)
+    p = g.pos.astype(np.float64)
+    ph = np.c_[p, np.ones(len(p))]
+    pv = ph @ V.T
+    pp = ph @ P.T
+    ndc = pp[:, :2] / (pp[:, 3:4] + 1e-7)
+    z = pv[:, 2]
+    fx = W / (2 * cam.tanfovx)
+    fy = H / (2 * cam.tanfovy)
+    R = quat_mats(g.rot.astype(np.float64))
+    Sg = R @ (np.square(g.scale.astype(np.float64))[:, :, None] * np.transpose(R, (0, 2, 1)))
+    tx = np.clip(pv[:, 0] / z, -1.3 * cam.tanfovx, 1.3 * cam.tanfovx) * z
+    ty = np.clip(pv[:, 1] / z, -1.3 * cam.tanfovy, 1.3 * cam.tanfovy) * z
+    J = np.zeros((len(p), 2, 3))
+    J[:, 0, 0] = fx / z
+    J[:, 0, 2] = -fx * tx / z ** 2
+    J[:, 1, 1] = fy / z
+    J[:, 1, 2] = -fy * ty / z ** 2
+    T = J @ V[:3, :3]
+    cov = T @ Sg @ np.transpose(T, (0, 2, 1))
+    a = cov[:, 0, 0] + 0.3
+    b = cov[:, 0, 1]
+    c = cov[:, 1, 1] + 0.3
+    det = a * c - b * b
+    conic = np.stack([c / det, -b / det, a / det], 1)
+    ix = ((ndc[:, 0] + 1) * W - 1) * 0.5
+    iy = ((ndc[:, 1] + 1) * H - 1) * 0.5
+    ex = np.ceil(3 * np.sqrt(a))
+    ey = np.ceil(3 * np.sqrt(c))
+    gx, gy = (W + tile - 1) // tile, (H + tile - 1) // tile
+    r0x = np.clip(((ix - ex) / tile).astype(np.int64), 0, gx)
+    r0y = np.clip(((iy - ey) / tile).astype(np.int64), 0, gy)
+    r1x = np.clip(((ix + ex + tile - 1) / tile).astype(np.int64), 0, gx)
+    r1y = np.clip(((iy + ey + tile - 1) / tile).astype(np.int64), 0, gy)
+    vis = (z > 0.2) & ((r1x - r0x) * (r1y - r0y) > 0)
+    d = p - cam.campos.astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rgb = sh_color(g.sh.astype(np.float64), d)
+    return dict(z=z, ix=ix, iy=iy, conic=conic, rect=np.stack([r0x, r0y, r1x, r1y], 1), vis=vis, rgb=rgb,
+                opac=g.opacity.astype(np.float64), gx=gx, gy=gy)
+
+
+@pytest.fixture(scope="module")
+def small():
+    sc = S.capsule_scene(3, n=3000, n_nodes=64, W=96, H=80, n_views=2)
+    return sc
+
+
+def _oracle(g, cam, W, H):
+    return O.raster(g.pos, g.sh, g.opacity, g.scale, g.rot, cam.view, cam.proj, cam.campos, cam.tanfovx,
+                    cam.tanfovy, W, H, np.array([0.8, 0.8, 0.8], np.float32))
+
+
+def test_preprocess_and_binning_exact(small):
+    g = small.gaussians
+    for cam in small.cameras:
+        W, H = cam.W, cam.H
+        r = _oracle(g, cam, W, H)
+        ref = restatement(g, cam, W, H)
+        vis = r["touched"] > 0
+        np.testing.assert_array_equal(vis, ref["vis"])
+        np.testing.assert_allclose(r["xy"][vis, 0], ref["ix"][vis], atol=2e-4)
+        np.testing.assert_allclose(r["conic"][vis, :3], ref["conic"][vis], rtol=2e-4, atol=1e-6)
+        np.testing.assert_array_equal(r["rect"][vis], ref["rect"][vis])
+        # keys emitted per Gaussian (ascending) and per tile row-major, stable-sorted
+        keys, vals = [], []
+        for i in np.nonzero(vis)[0]:
+            x0, y0, x1, y1 = r["rect"][i]
+            db = int(np.float32(r["depth"][i]).view(np.uint32))
+            for y in range(y0, y1):
+                for x in range(x0, x1):
+                    keys.append(((y * ref["gx"] + x) << 32) | db)
+                    vals.append(i)
+        keys = np.array(keys, np.uint64)
+        order = np.argsort(keys, kind="stable")
+        np.testing.assert_array_equal(r["keys"], keys[order])
+        np.testing.assert_array_equal(r["vals"], np.array(vals, np.uint32)[order])
+        tiles = (r["keys"] >> np.uint64(32)).astype(np.int64)
+        for t in range(ref["gx"] * ref["gy"]):
+            idx = np.nonzero(tiles == t)[0]
+            if idx.size:
+                assert tuple(r["ranges"][t]) == (idx[0], idx[-1] + 1)
+            else:
+                assert r["ranges"][t][1] - r["ranges"][t][0] == 0
+
+
+def test_blend_per_pixel_brute_force(small):
+    g = small.gaussians
+    cam = small.cameras[0]
+    W, H = cam.W, cam.H
+    r = _oracle(g, cam, W, H)
+    xy, conic, rgb = r["xy"].astype(np.float64), r["conic"].astype(np.float64), r["rgb"].astype(np.float64)
+    img = np.zeros((3, H, W))
+    bg = 0.8
+    for py in range(H):
+        for px in range(W):
+            t = (py // 16) * ((W + 15) // 16) + px // 16
+            s, e = r["ranges"][t]
+            T, C = 1.0, np.zeros(3)
+            for gi in r["vals"][s:e]:
+                dx, dy = xy[gi, 0] - px, xy[gi, 1] - py
+                power = -0.5 * (conic[gi, 0] * dx * dx + conic[gi, 2] * dy * dy) - conic[gi, 1] * dx * dy
+                if power > 0:
+                    continue
+                alpha = min(0.99, conic[gi, 3] * np.exp(power))
+                if alpha < 1.0 / 255.0:
+                    continue
+                tt = T * (1 - alpha)
+                if tt < 1e-4:
+                    break
+                C += rgb[gi] * alpha * T
+                T = tt
+            img[:, py, px] = C + T * bg
+    np.testing.assert_allclose(r["image"], img, atol=1e-5)
