@@ -55,7 +55,7 @@ struct arapgs_ctx {
 };
 
 struct arapgs_solver {
-    arapgs_ctx* ctx = nullptr;
+    int device = 0;  // the creating context's device; a solver never touches its context again
     SolverWork w;
     int host_check_every = 0;   // 0: device-side convergence only
     int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG
@@ -257,7 +257,7 @@ int arapgs_solver_create(arapgs_ctx* c, const arapgs_solver_desc* d, arapgs_solv
     *out = nullptr;
     (void)hipSetDevice(c->device);
     arapgs_solver* s = new arapgs_solver();
-    s->ctx = c;
+    s->device = c->device;
     SolverDesc sd;
     sd.M = d->m; sd.K = d->k; sd.nbr = d->nbr_h; sd.block_ptr = d->block_ptr_h; sd.block_nodes = d->block_nodes_h;
     sd.block_type = d->block_type_h; sd.nblocks = d->nblocks; sd.src_idx = d->src_idx_h; sd.src_w = d->src_w_h;
@@ -284,7 +284,7 @@ int arapgs_solver_create(arapgs_ctx* c, const arapgs_solver_desc* d, arapgs_solv
 
 void arapgs_solver_destroy(arapgs_solver* s) {
     if (!s) return;
-    (void)hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
     solver_free(s->w);
     for (void* p : {(void*)s->table, (void*)s->node_q, (void*)s->node_tmp})

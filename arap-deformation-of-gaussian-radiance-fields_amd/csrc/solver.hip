@@ -800,8 +800,23 @@ __global__ void agg_center_kernel(SolverDev S) {
 
 // Galerkin coarse matrix Ac = P^T (J^T J) P accumulated per residual group:
 // each group's J projected onto the coarse dofs of the aggregates it touches.
+// Coarse Galerkin entries are accumulated as 64-bit fixed point (2^-24 units):
+// integer additions commute, so the assembled operator does not depend on the
+// order in which the residual groups arrive. Range +-2^39 (5.5e11), resolution
+// 6e-8 against diagonal entries >= w_reg K; it is only a preconditioner.
+constexpr double COARSE_FX = 16777216.0;
+
+__device__ __forceinline__ void coarse_add(unsigned long long* Ac, int64_t at, float s) {
+    atomicAdd(Ac + at, (unsigned long long)__double2ll_rn((double)s * COARSE_FX));
+}
+
+__global__ void coarse_to_float_kernel(const unsigned long long* __restrict__ Ac64, float* __restrict__ Ac, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        Ac[i] = (float)((double)(long long)Ac64[i] / COARSE_FX);
+}
+
 __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
-                                                              float* __restrict__ Ac) {
+                                                              unsigned long long* __restrict__ Ac) {
     const int nc = S.nc_pad;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
          g += (int64_t)gridDim.x * blockDim.x) {
@@ -830,7 +845,7 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
                 for (int w = 0; w < 9; w++) {
                     float s = 0.0f;
                     for (int rr = 0; rr < 6; rr++) s += Jrot[rr][u] * Jrot[rr][w];
-                    if (s != 0.0f) atomicAdd(&Ac[(int64_t)(12 * a + u) * nc + 12 * a + w], s);
+                    if (s != 0.0f) coarse_add(Ac, (int64_t)(12 * a + u) * nc + 12 * a + w, s);
                 }
             continue;
         }
@@ -923,7 +938,7 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
                     for (int w = 0; w < 12; w++) {
                         float s = 0.0f;
                         for (int r = 0; r < rows; r++) s += JP[q][r][u] * JP[q2][r][w];
-                        if (s != 0.0f) atomicAdd(&Ac[(int64_t)(12 * aggs[q] + u) * nc + 12 * aggs[q2] + w], s);
+                        if (s != 0.0f) coarse_add(Ac, (int64_t)(12 * aggs[q] + u) * nc + 12 * aggs[q2] + w, s);
                     }
             }
         }
@@ -1109,9 +1124,12 @@ int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
         // preconditioner, so callers may keep it across frames.
         hipLaunchKernelGGL(agg_center_kernel, dim3(S.n_agg), dim3(256), 0, st, S);
         const int nc = 12 * S.n_agg;
-        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.acinv, (int64_t)S.nc_pad * S.nc_pad,
+        const int64_t n2 = (int64_t)S.nc_pad * S.nc_pad;
+        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, reinterpret_cast<float*>(W.ac64), 2 * n2,
                            (const int*)nullptr);
-        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.acinv);
+        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.ac64);
+        hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256), dim3(256), 0, st, (const unsigned long long*)W.ac64,
+                           W.acinv, n2);
         const int np = S.nc_pad;
         hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
         const int nt = np / GJ_T;

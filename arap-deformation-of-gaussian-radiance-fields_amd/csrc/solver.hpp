@@ -61,6 +61,7 @@ struct SolverWork {
     SolverDev dev;
     int n;  // 12 * nF
     float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col;
+    unsigned long long* ac64;  // fixed-point coarse assembly (order independent)
     double *partial, *partial2, *rzp, *e_partial, *cand_e;  // rzp: 2 x n_agg r.z partials
     int e_blocks, n_cand;
     CgScalars* cg;

@@ -183,6 +183,7 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.binv = alloc<float>(W, (size_t)144 * nF, rc);
     W.rc = alloc<float>(W, 12 * (size_t)n_agg, rc);
     W.acinv = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
+    W.ac64 = alloc<unsigned long long>(W, (size_t)S.nc_pad * S.nc_pad, rc);
     W.gj_piv = alloc<float>(W, 64 * 64, rc);
     W.gj_col = alloc<float>(W, (size_t)S.nc_pad * 64, rc);
     const int nb_owner = (nF + 15) / 16;

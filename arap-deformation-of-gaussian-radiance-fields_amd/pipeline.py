@@ -1,0 +1,152 @@
+"""Pipelined drag frames over the arapgs C ABI.
+
+One drag frame of GaussianView::onUpdate (GaussianView.cpp:1481-1522) is, in
+order: aims; Gauss-Newton solve; node table (putFreeInputs); sample blend
+(UpdatePositionforSamples); end-point blend + six-point update
+(UpdatePosition, UpdateAsSixPointsWithdrawBad); node blend; node quaternions +
+sample SH rotation (FastUpdateSamplesSH); renders.
+
+The only thing frame t+1 needs from frame t is the blended node positions. So
+the frame splits into two halves on two streams:
+
+  solve side   (stream A)  aims(t), solve(t), node table(t) -> slot t%2,
+                           node blend(t) (node positions for t+1); event A[t%2]
+  deform side  (stream B)  wait A[t%2]; sample blend, six-point update, node
+                           quaternions, sample SH with table slot t%2; event
+                           B[t%2] (slot free again); V renders
+
+and frame t's deform side runs while frame t+1's solve runs. Each side has its
+own host thread, because each side synchronises with the host (the solve's CG
+convergence checks, the renderer's instance count) and those waits must not
+stall the other side. The two node-table slots bound the lead of the solve side
+to one frame. Every kernel sees exactly the inputs of the sequential frame, so
+the results are identical bit for bit (GPU test).
+"""
+from __future__ import annotations
+
+import queue
+import threading
+
+from . import arapgs
+
+
+class DragPipeline:
+    def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
+                 stop_rule=True, render_ctx=None, sh_degree=3):
+        import torch
+        self.torch = torch
+        self.solver = solver
+        self.ctx_a = solver.ctx
+        dev = state["pos"].device
+        self.stream_a = self.ctx_a.stream
+        self.stream_b = torch.cuda.Stream(device=dev)
+        self.ctx_b = render_ctx if render_ctx is not None else arapgs.Context(dev.index or 0, stream=self.stream_b)
+        if render_ctx is not None:
+            self.stream_b = render_ctx.stream
+        self.st = state
+        self.k = k
+        self.active, self.delta = active, delta
+        self.iters, self.stop_rule = gn_iters, stop_rule
+        self.cams, self.images, self.bg, self.opac, self.D = cameras, images, bg, opacity, sh_degree
+        M = state["node_pos"].shape[0]
+        with torch.cuda.stream(self.stream_a):
+            self.aims = torch.empty_like(state["node_pos"])
+            self.tables = [arapgs.node_table(M, dev) for _ in range(2)]
+            self.x = [torch.empty(solver.n_unknowns, dtype=torch.float32, device=dev) for _ in range(2)]
+            self.energies = [torch.zeros(gn_iters + 1, dtype=torch.float64, device=dev) for _ in range(2)]
+            self.iters_done = [torch.zeros(1, dtype=torch.int32, device=dev) for _ in range(2)]
+        self.ev_a = [torch.cuda.Event() for _ in range(2)]
+        self.ev_b = [torch.cuda.Event() for _ in range(2)]
+        self.ev_b_recorded = [False, False]
+        self.instances = 0
+        torch.cuda.synchronize(dev)
+
+    # ------------------------------------------------------------ the two halves
+    def _solve_half(self, slot):
+        st, torch = self.st, self.torch
+        with torch.cuda.stream(self.stream_a):
+            if self.ev_b_recorded[slot]:
+                self.stream_a.wait_event(self.ev_b[slot])
+            self.ctx_a.aims_translate(st["node_pos"], self.active, self.delta, self.aims)
+            self.solver.solve(st["node_pos"], self.aims, self.iters, self.stop_rule, x_out=self.x[slot],
+                              energies=self.energies[slot], iters_done=self.iters_done[slot])
+            self.solver.put_free_inputs(st["node_pos"], self.x[slot], table=self.tables[slot])
+            self.ctx_a.blend_points(st["node_pos"], st["node_src_idx"], st["node_src_w"], self.tables[slot],
+                                    out=st["node_pos"])
+            self.ev_a[slot].record(self.stream_a)
+
+    def _deform_half(self, slot, render=True):
+        st, torch = self.st, self.torch
+        ctx = self.ctx_b
+        with torch.cuda.stream(self.stream_b):
+            self.stream_b.wait_event(self.ev_a[slot])
+            table = self.tables[slot]
+            if st.get("samples") is not None and st["samples"].shape[0] > 0:
+                ctx.blend_points(st["samples"], st["sample_idx"], st["sample_w"], table, skip=st.get("sample_static"),
+                                 out=st["samples"])
+            ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"], st.get("gs_static"),
+                                 st["pos"], st["rot"], st["scale"], st["sh"])
+            if st.get("sample_feat") is not None and st["samples"].shape[0] > 0:
+                q = ctx.node_quats(table)
+                ctx.sample_sh(st["sample_w"], st["sample_idx"], q, st.get("sample_static"), st["sample_feat"])
+            self.ev_b[slot].record(self.stream_b)
+            self.ev_b_recorded[slot] = True
+            n_inst = 0
+            if render:
+                for cam, img in zip(self.cams, self.images):
+                    out = ctx.render(st["pos"], st["sh"], self.opac, st["scale"], st["rot"], cam["view"],
+                                     cam["proj"], cam["campos"], cam["tx"], cam["ty"], img.shape[2], img.shape[1],
+                                     self.bg, D=self.D, out=img)
+                    n_inst += out["num_rendered"]
+            return n_inst
+
+    # ------------------------------------------------------------ driver
+    def run(self, n_frames: int, render=True, on_frame=None):
+        """Runs n_frames drag frames; on_frame(t, instances) is called on the
+        deform thread after frame t's renders were queued (images complete once
+        the deform stream reaches that point)."""
+        slots = queue.Queue()
+        free = threading.Semaphore(2)
+        err = []
+
+        def solve_side():
+            try:
+                for t in range(n_frames):
+                    free.acquire()
+                    if err:
+                        return
+                    self._solve_half(t % 2)
+                    slots.put(t % 2)
+            except BaseException as e:  # noqa: BLE001 - re-raised on the caller's thread
+                err.append(e)
+                slots.put(None)
+
+        def deform_side():
+            try:
+                for t in range(n_frames):
+                    slot = slots.get()
+                    if slot is None:
+                        return
+                    n = self._deform_half(slot, render)
+                    free.release()
+                    self.instances = n
+                    if on_frame is not None:
+                        on_frame(t, n)
+            except BaseException as e:  # noqa: BLE001
+                err.append(e)
+                for _ in range(n_frames + 2):
+                    free.release()
+
+        ta = threading.Thread(target=solve_side, name="arapgs-solve")
+        tb = threading.Thread(target=deform_side, name="arapgs-deform")
+        ta.start()
+        tb.start()
+        ta.join()
+        tb.join()
+        if err:
+            raise err[0]
+        self.torch.cuda.synchronize(self.st["pos"].device)
+
+    def close(self):
+        if self.ctx_b is not None:
+            self.ctx_b.close()

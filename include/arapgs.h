@@ -15,6 +15,9 @@
  *   - K (nearest nodes per point) is 1..12 (KNN_MAX, helper.hpp:48).
  *   - Steady-state per-frame calls do not allocate device memory once a call
  *     of the same size has run (workspaces grow on first use).
+ *   - Solvers and grids keep their own device buffers and may be destroyed
+ *     before or after the context that created them; calls that take both need
+ *     both alive.
  */
 #ifndef ARAPGS_H
 #define ARAPGS_H
