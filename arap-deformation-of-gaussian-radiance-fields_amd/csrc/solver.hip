@@ -39,26 +39,34 @@ __device__ __forceinline__ float xR(const float* x, int r, int c) { return x[3 *
 // ---------------------------------------------------------------- residuals
 // Residual of one group at unknowns x (+ alpha*h when h != nullptr). Returns
 // the number of rows written to f (<= 6).
-__device__ int group_residual(const SolverDev& S, const float* __restrict__ x, const float* __restrict__ h,
-                              float alpha, int64_t g, float f[6]) {
-    auto X = [&](int node_free, int comp) -> float {
+// Residual rows of group g at x + alpha h (h == nullptr: at x), evaluated in
+// FP64 from the FP32 unknowns (each candidate unknown is first rounded to the
+// FP32 value the accepted step would store). FP64 keeps the cancellations of
+// the affine parts (R d + g_i + t_i - g_k - t_k, the constraint sums) out of
+// the gradient, so near the minimum the step shrinks the way the reference's
+// double-precision solve does and its stop rule fires at the same iteration.
+__device__ __forceinline__ int group_residual_rows(const SolverDev& S, const float* __restrict__ x,
+                                                   const float* __restrict__ h, float alpha, int64_t g, double& f0,
+                                                   double& f1, double& f2, double& f3, double& f4, double& f5) {
+    auto X = [&](int node_free, int comp) -> double {
         const int64_t o = 12 * (int64_t)node_free + comp;
-        return h ? x[o] + alpha * h[o] : x[o];
+        return (double)(h ? x[o] + alpha * h[o] : x[o]);
     };
+    const double sw_rot = S.sw_rot, sw_reg = S.sw_reg, sw_con = S.sw_con;
     if (g < S.nF) {
         const int fi = (int)g;
-        float c[3][3];
+        double c[3][3];
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
             for (int r = 0; r < 3; r++) c[j][r] = X(fi, 3 * j + r);
         auto dot = [&](int a, int b) { return c[a][0] * c[b][0] + c[a][1] * c[b][1] + c[a][2] * c[b][2]; };
-        f[0] = S.sw_rot * dot(0, 1);
-        f[1] = S.sw_rot * dot(0, 2);
-        f[2] = S.sw_rot * dot(1, 2);
-        f[3] = S.sw_rot * (dot(0, 0) - 1.0f);
-        f[4] = S.sw_rot * (dot(1, 1) - 1.0f);
-        f[5] = S.sw_rot * (dot(2, 2) - 1.0f);
+        f0 = sw_rot * dot(0, 1);
+        f1 = sw_rot * dot(0, 2);
+        f2 = sw_rot * dot(1, 2);
+        f3 = sw_rot * (dot(0, 0) - 1.0);
+        f4 = sw_rot * (dot(1, 1) - 1.0);
+        f5 = sw_rot * (dot(2, 2) - 1.0);
         return 6;
     }
     g -= S.nF;
@@ -67,17 +75,20 @@ __device__ int group_residual(const SolverDev& S, const float* __restrict__ x, c
         const int i = S.free_nodes[fi];
         const int k = (int)S.nbr[(int64_t)i * S.K + t];
         const int fk = S.free_id[k];
-        const float s = S.edge_scale[(int64_t)i * S.K + t];
+        const double s = S.edge_scale[(int64_t)i * S.K + t];
         const float* gi = S.g + 3 * i;
         const float* gk = S.g + 3 * k;
-        const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const float tk = fk >= 0 ? X(fk, 9 + r) : 0.0f;
-            const float v = X(fi, r) * d[0] + X(fi, 3 + r) * d[1] + X(fi, 6 + r) * d[2] + gi[r] + X(fi, 9 + r) -
-                            gk[r] - tk;
-            f[r] = S.sw_reg * v * s;
-        }
+        const double d[3] = {(double)gk[0] - gi[0], (double)gk[1] - gi[1], (double)gk[2] - gi[2]};
+        auto reg = [&](int r) {
+            const double tk = fk >= 0 ? X(fk, 9 + r) : 0.0;
+            const double v = X(fi, r) * d[0] + X(fi, 3 + r) * d[1] + X(fi, 6 + r) * d[2] + (double)gi[r] +
+                             X(fi, 9 + r) - (double)gk[r] - tk;
+            return sw_reg * v * s;
+        };
+        f0 = reg(0);
+        f1 = reg(1);
+        f2 = reg(2);
+        f3 = f4 = f5 = 0.0;
         return 3;
     }
     g -= (int64_t)S.nF * S.K;
@@ -86,34 +97,49 @@ __device__ int group_residual(const SolverDev& S, const float* __restrict__ x, c
         const int i = S.static_nodes[si];
         const int k = (int)S.nbr[(int64_t)i * S.K + t];
         const int fk = S.free_id[k];
-#pragma unroll
-        for (int r = 0; r < 3; r++) f[r] = S.sw_reg * -(fk >= 0 ? X(fk, 9 + r) : 0.0f);
+        f0 = sw_reg * -(fk >= 0 ? X(fk, 9) : 0.0);
+        f1 = sw_reg * -(fk >= 0 ? X(fk, 10) : 0.0);
+        f2 = sw_reg * -(fk >= 0 ? X(fk, 11) : 0.0);
+        f3 = f4 = f5 = 0.0;
         return 3;
     }
     g -= (int64_t)S.nS * S.K;
     // constraint
     const int cid = (int)g;
     const int c = S.con_node[cid];
-    const float v[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
-    float nv[3] = {0.0f, 0.0f, 0.0f};
+    const double v0 = S.g[3 * c], v1 = S.g[3 * c + 1], v2 = S.g[3 * c + 2];
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
     for (int j = 0; j < S.K; j++) {
         const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
-        const float w = S.src_w[(int64_t)c * S.K + j];
+        const double w = S.src_w[(int64_t)c * S.K + j];
         const int fj = S.free_id[nj];
         if (fj < 0) {
-#pragma unroll
-            for (int r = 0; r < 3; r++) nv[r] += w * v[r];
+            n0 += w * v0;
+            n1 += w * v1;
+            n2 += w * v2;
         } else {
             const float* gj = S.g + 3 * nj;
-            const float dd[3] = {v[0] - gj[0], v[1] - gj[1], v[2] - gj[2]};
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-                nv[r] += w * (X(fj, r) * dd[0] + X(fj, 3 + r) * dd[1] + X(fj, 6 + r) * dd[2] + gj[r] + X(fj, 9 + r));
+            const double d0 = v0 - gj[0], d1 = v1 - gj[1], d2 = v2 - gj[2];
+            n0 += w * (X(fj, 0) * d0 + X(fj, 3) * d1 + X(fj, 6) * d2 + (double)gj[0] + X(fj, 9));
+            n1 += w * (X(fj, 1) * d0 + X(fj, 4) * d1 + X(fj, 7) * d2 + (double)gj[1] + X(fj, 10));
+            n2 += w * (X(fj, 2) * d0 + X(fj, 5) * d1 + X(fj, 8) * d2 + (double)gj[2] + X(fj, 11));
         }
     }
-#pragma unroll
-    for (int r = 0; r < 3; r++) f[r] = S.sw_con * (nv[r] - S.aim[3 * c + r]);
+    f0 = sw_con * (n0 - (double)S.aim[3 * c]);
+    f1 = sw_con * (n1 - (double)S.aim[3 * c + 1]);
+    f2 = sw_con * (n2 - (double)S.aim[3 * c + 2]);
+    f3 = f4 = f5 = 0.0;
     return 3;
+}
+
+// The rows travel as six scalars (3-row groups set rows 3-5 to zero) so that
+// every branch keeps them in registers.
+__device__ __forceinline__ int group_residual(const SolverDev& S, const float* __restrict__ x,
+                                              const float* __restrict__ h, float alpha, int64_t g, double f[6]) {
+    double f0, f1, f2, f3, f4, f5;
+    const int nr = group_residual_rows(S, x, h, alpha, g, f0, f1, f2, f3, f4, f5);
+    f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3; f[4] = f4; f[5] = f5;
+    return nr;
 }
 
 __device__ __forceinline__ double block_reduce_d(double v, double* sh) {
@@ -141,9 +167,11 @@ __global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* _
     double acc = 0.0;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
          g += (int64_t)gridDim.x * blockDim.x) {
-        float f[6];
+        double f[6];
         const int nr = group_residual(S, x, h, alpha, g, f);
-        for (int r = 0; r < nr; r++) acc += (double)f[r] * (double)f[r];
+#pragma unroll
+        for (int r = 0; r < 6; r++)
+            if (r < nr) acc += f[r] * f[r];
     }
     const double t = block_reduce_d(acc, sh);
     if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = t;
@@ -166,7 +194,9 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 // wrt fi and the group's J*v for a given vector v (12 per free node):
 //   fn(rows, Ji[rows][12], Jv[rows]).
 // Items: 0 = rot, 1..K = out-edges, then in-edges, then constraint slots.
-template <typename Fn>
+// KT > 0: the neighbour count is a compile-time constant (== S.K), so the
+// constraint item's K-node loop unrolls and its gathers issue together.
+template <int KT = 0, typename Fn>
 __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item, const float* __restrict__ x,
                                            const float* __restrict__ v, Fn&& fn) {
     const int i = S.free_nodes[fi];
@@ -272,11 +302,14 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
             Ji[r][9 + r] = S.sw_con * wi;
         }
     }
-    for (int j = 0; j < S.K; j++) {
-        const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+    const int KK = KT > 0 ? KT : S.K;
+#pragma unroll
+    for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
+        if (KT == 0 && j >= KK) break;
+        const int nj = (int)S.src_idx[(int64_t)c * KK + j];
         const int fj = S.free_id[nj];
         if (fj < 0) continue;
-        const float w = S.src_w[(int64_t)c * S.K + j];
+        const float w = S.src_w[(int64_t)c * KK + j];
         const float* gj = S.g + 3 * nj;
         const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
         const int64_t jo = 12 * (int64_t)fj;
@@ -291,6 +324,14 @@ __device__ __forceinline__ int owner_items(const SolverDev& S, int fi) {
 }
 
 __device__ __forceinline__ float group_reduce16(float v) {
+    v += __shfl_xor(v, 8, 16);
+    v += __shfl_xor(v, 4, 16);
+    v += __shfl_xor(v, 2, 16);
+    v += __shfl_xor(v, 1, 16);
+    return v;
+}
+
+__device__ __forceinline__ double group_reduce16d(double v) {
     v += __shfl_xor(v, 8, 16);
     v += __shfl_xor(v, 4, 16);
     v += __shfl_xor(v, 2, 16);
@@ -339,9 +380,10 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
     __shared__ double shd[8];
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     const int fi = blockIdx.x * (256 / LANES) + grp;
-    float bl[12], D[78];
+    double bl[12];
+    float D[78];
 #pragma unroll
-    for (int a = 0; a < 12; a++) bl[a] = 0.0f;
+    for (int a = 0; a < 12; a++) bl[a] = 0.0;
 #pragma unroll
     for (int a = 0; a < 78; a++) D[a] = 0.0f;
     if (fi < S.nF) {
@@ -349,7 +391,7 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
         for (int it = lane; it < ni; it += LANES) {
             // residual value of the group for the gradient: evaluate f via J*x
             // identities is not possible for the affine parts, so recompute f.
-            float f[6];
+            double f[6];
             int64_t gid;
             // map item -> global group id for the residual
             {
@@ -376,9 +418,11 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
             group_residual(S, x, nullptr, 0.0f, gid, f);
             owner_item(S, fi, it, x, x, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
                 (void)Jv;
-                for (int r = 0; r < rows; r++) {
 #pragma unroll
-                    for (int a = 0; a < 12; a++) bl[a] -= Ji[r][a] * f[r];
+                for (int r = 0; r < 6; r++) {
+                    if (r >= rows) break;
+#pragma unroll
+                    for (int a = 0; a < 12; a++) bl[a] -= (double)Ji[r][a] * f[r];
                     int o = 0;
 #pragma unroll
                     for (int a = 0; a < 12; a++)
@@ -389,7 +433,7 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
         }
     }
 #pragma unroll
-    for (int a = 0; a < 12; a++) bl[a] = group_reduce16(bl[a]);
+    for (int a = 0; a < 12; a++) bl[a] = group_reduce16d(bl[a]);
 #pragma unroll
     for (int a = 0; a < 78; a++) D[a] = group_reduce16(D[a]);
     double nb = 0.0;
@@ -402,8 +446,9 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
                 o++;
             }
         for (int a = 0; a < 12; a++) {
-            b[12 * (int64_t)fi + a] = bl[a];
-            nb += (double)bl[a] * bl[a];
+            const float ba = (float)bl[a];
+            b[12 * (int64_t)fi + a] = ba;
+            nb += (double)ba * ba;
         }
         invert12(sD[grp]);
         for (int a = 0; a < 144; a++) binv[144 * (int64_t)fi + a] = sD[grp][a];
@@ -450,6 +495,7 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 // it needs itself, in the same order, so all of them take the same decisions.
 
 // apply for owner block ob (16 free nodes, 16 lanes each)
+template <int KT>
 __device__ __forceinline__ void apply_body(const SolverDev& S, const float* __restrict__ x,
                                            const float* __restrict__ z, float beta, bool first, float* __restrict__ p,
                                            float* __restrict__ q, double* __restrict__ partial, int ob, double* shd) {
@@ -461,7 +507,7 @@ __device__ __forceinline__ void apply_body(const SolverDev& S, const float* __re
     if (fi < S.nF) {
         const int ni = owner_items(S, fi);
         for (int it = lane; it < ni; it += LANES) {
-            owner_item(S, fi, it, x, z, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
+            owner_item<KT>(S, fi, it, x, z, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
                 for (int r = 0; r < rows; r++)
 #pragma unroll
                     for (int a = 0; a < 12; a++) acc[a] += Ji[r][a] * Jv[r];
@@ -555,17 +601,32 @@ __device__ __forceinline__ void update_body(const SolverDev& S, int mode, float 
 __device__ __forceinline__ void prolong_body(const SolverDev& S, const float* __restrict__ acinv,
                                              const float* __restrict__ rc, const float* __restrict__ r,
                                              float* __restrict__ z, double* __restrict__ rz_out, int a, float* zc,
-                                             double* shd) {
+                                             float (*zred)[12], double* shd) {
     const int nc = 12 * S.n_agg;
     const int ld = S.nc_pad;
-    // 12 rows of Ac^-1 times rc: 16 lanes per row
+    // 12 rows of Ac^-1 times rc: every thread takes columns j = tid, tid+256, ...
+    // (coalesced row loads, rc read once), then a block reduction of 12 sums
     {
-        const int row = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-        float acc = 0.0f;
-        if (row < 12)
-            for (int j = lane; j < nc; j += LANES) acc += acinv[(int64_t)(12 * a + row) * ld + j] * rc[j];
-        acc = group_reduce16(acc);
-        if (row < 12 && lane == 0) zc[row] = acc;
+        float part[12];
+#pragma unroll
+        for (int row = 0; row < 12; row++) part[row] = 0.0f;
+        const float* A0 = acinv + (int64_t)(12 * a) * ld;
+#pragma unroll 4
+        for (int j = threadIdx.x; j < nc; j += 256) {
+            const float rj = rc[j];
+#pragma unroll
+            for (int row = 0; row < 12; row++) part[row] += A0[(int64_t)row * ld + j] * rj;
+        }
+        const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int row = 0; row < 12; row++) {
+            float v = part[row];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (wl == 0) zred[wv][row] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 12) zc[threadIdx.x] = (zred[0][threadIdx.x] + zred[1][threadIdx.x]) +
+                                              (zred[2][threadIdx.x] + zred[3][threadIdx.x]);
     }
     __syncthreads();
     const float* ca = S.agg_center + 3 * a;
@@ -608,6 +669,7 @@ __device__ __forceinline__ float cg_alpha(const SolverDev& S, const double* pq_p
     return pq != 0.0 ? (float)(rz / pq) : 0.0f;
 }
 
+template <int KT>
 __global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
                                                     const float* __restrict__ z, const double* __restrict__ rz_cur,
                                                     const double* __restrict__ rz_prev, int first,
@@ -616,7 +678,7 @@ __global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __
     if (*skip) return;
     __shared__ double shd[8];
     const float beta = cg_beta(S, rz_cur, rz_prev, first != 0, shd);
-    apply_body(S, x, z, beta, first != 0, p, q, partial, blockIdx.x, shd);
+    apply_body<KT>(S, x, z, beta, first != 0, p, q, partial, blockIdx.x, shd);
 }
 
 // One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
@@ -644,14 +706,15 @@ __global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const 
                                                              CgScalars* __restrict__ cs, int test, int* __restrict__ skip) {
     if (*skip) return;
     __shared__ float zc[12];
+    __shared__ float zred[4][12];
     __shared__ double shd[8];
     const int a = blockIdx.x;
-    prolong_body(S, acinv, rc, r, z, rz_out, a, zc, shd);
+    prolong_body(S, acinv, rc, r, z, rz_out, a, zc, zred, shd);
     const double rr = (a == 0 && test) ? block_sum_partials(rr_part, S.n_agg, shd) : 0.0;
     if (threadIdx.x == 0 && a == 0 && test) {
         cs->rr = rr;
         cs->iters += 1;
-        if (rr <= cs->tol2 * cs->bb || cs->iters >= cs->max_iters) *skip = 1;
+        if (rr <= cs->rr_stop || cs->iters >= cs->max_iters) *skip = 1;
     }
 }
 
@@ -700,6 +763,7 @@ __device__ __forceinline__ bool grid_sync(GridBarrier* gb, unsigned& epoch) {
 // Preconditions (queued before it): r = b, z0 = M^-1 r0 and rz_0 in rzp[0..n_agg),
 // cs->bb, cs->iters = 0; gb zeroed. Owner blocks and aggregates are strided
 // over the grid, so any grid size up to the co-resident limit works.
+template <int KT>
 __global__ void __launch_bounds__(256) pcg_persistent_kernel(SolverDev S, const float* __restrict__ x,
                                                              float* __restrict__ hsol, float* __restrict__ r,
                                                              float* __restrict__ z, float* __restrict__ p,
@@ -711,10 +775,11 @@ __global__ void __launch_bounds__(256) pcg_persistent_kernel(SolverDev S, const 
     if (*skip) return;  // read by every workgroup before anything is written
     __shared__ float sred[256 / LANES][12];
     __shared__ float zc[12];
+    __shared__ float zred[4][12];
     __shared__ double shd[8];
     const int n_ob = (S.nF + (256 / LANES) - 1) / (256 / LANES);
     const int max_iters = cs->max_iters;
-    const double tol2bb = cs->tol2 * cs->bb;
+    const double tol2bb = cs->rr_stop;
     unsigned epoch = 0;
     int k = 0;
     double rr = 0.0;
@@ -722,7 +787,8 @@ __global__ void __launch_bounds__(256) pcg_persistent_kernel(SolverDev S, const 
         const double* rz_cur = rzp + (k & 1) * S.n_agg;
         double* rz_next = rzp + ((k + 1) & 1) * S.n_agg;
         const float beta = cg_beta(S, rz_cur, rz_next, k == 0, shd);
-        for (int ob = blockIdx.x; ob < n_ob; ob += gridDim.x) apply_body(S, x, z, beta, k == 0, p, q, pq_part, ob, shd);
+        for (int ob = blockIdx.x; ob < n_ob; ob += gridDim.x)
+            apply_body<KT>(S, x, z, beta, k == 0, p, q, pq_part, ob, shd);
         if (!grid_sync(gb, epoch)) return;
         const float alpha = cg_alpha(S, pq_part, n_ob, rz_cur, shd);
         for (int a = blockIdx.x; a < S.n_agg; a += gridDim.x)
@@ -730,7 +796,8 @@ __global__ void __launch_bounds__(256) pcg_persistent_kernel(SolverDev S, const 
         if (!grid_sync(gb, epoch)) return;
         rr = block_sum_partials(rr_part, S.n_agg, shd);
         if (rr <= tol2bb || k + 1 >= max_iters) break;
-        for (int a = blockIdx.x; a < S.n_agg; a += gridDim.x) prolong_body(S, acinv, rc, r, z, rz_next, a, zc, shd);
+        for (int a = blockIdx.x; a < S.n_agg; a += gridDim.x)
+            prolong_body(S, acinv, rc, r, z, rz_next, a, zc, zred, shd);
         if (!grid_sync(gb, epoch)) return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -747,16 +814,24 @@ __global__ void grid_barrier_reset_kernel(GridBarrier* gb) {
     gb->pad = 0;
 }
 
-// |b|^2 from the rhs partials; an all-zero right-hand side ends the solve.
+// |b|^2 from the rhs partials and the CG stopping level: relative residual
+// tol, but never below CG_TOL_FLOOR times the first Gauss-Newton gradient of
+// this solve (inexact Newton: once the gradient itself is below that level the
+// step is negligible, the solve returns at once and the stop rule fires). An
+// all-zero right-hand side ends the solve.
+constexpr double CG_TOL_FLOOR = 1e-6;
+
 __global__ void cg_init_kernel(const double* __restrict__ bb_part, int n, CgScalars* __restrict__ cs,
-                               int* __restrict__ skip) {
+                               GnScalars* __restrict__ gs, int* __restrict__ skip) {
     if (*skip) return;
     if (threadIdx.x != 0) return;
     const double bb = sum_partials(bb_part, n);
+    if (gs->iters_done == 0) gs->bb0 = bb;
     cs->bb = bb;
     cs->rr = bb;
     cs->iters = 0;
-    if (!(bb > 0.0)) *skip = 1;
+    cs->rr_stop = fmax(cs->tol2 * bb, CG_TOL_FLOOR * CG_TOL_FLOOR * gs->bb0);
+    if (!(bb > 0.0) || bb <= cs->rr_stop * (1.0 - 1e-12)) *skip = 1;
 }
 
 __global__ void zero_kernel(float* __restrict__ v, int64_t n, const int* __restrict__ skip) {
@@ -1156,8 +1231,9 @@ int solver_gn_begin(SolverWork& W, hipStream_t st) {
                        W.cg_tol * W.cg_tol);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial,
                        W.cg_done);
-    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, W.partial, nb_owner, W.cg, W.cg_done);
-    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.cg_done);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, W.partial, nb_owner, W.cg, W.gs, W.cg_done);
+    // h = 0 also when the CG is skipped for a negligible gradient (the line search then takes a zero step)
+    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.gn_done);
     // z0 = M^-1 r0, rz_0 -> rz[0]
     hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, (const double*)nullptr, 0,
                        (const double*)nullptr, W.hsol, W.r, (const float*)nullptr, (const float*)nullptr, W.binv, W.z,
@@ -1173,7 +1249,8 @@ int solver_cg_iteration(SolverWork& W, int k, hipStream_t st) {
     const int nb_owner = nblk(S.nF, 256 / LANES);
     double* rz_cur = W.rzp + (k & 1) * S.n_agg;
     double* rz_prev = W.rzp + ((k + 1) & 1) * S.n_agg;
-    hipLaunchKernelGGL(apply_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.z, rz_cur, rz_prev,
+    auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
+    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.z, rz_cur, rz_prev,
                        k == 0 ? 1 : 0, W.p, W.q, W.partial, W.cg_done);
     hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 1, W.partial, nb_owner, rz_cur,
                        W.hsol, W.r, W.p, W.q, W.binv, W.z, W.rc, W.partial2, W.cg_done);
@@ -1189,7 +1266,7 @@ int solver_cg_persistent(SolverWork& W, hipStream_t st) {
         int per_cu = 0, dev = 0, ncu = 0;
         HIP_RET(hipGetDevice(&dev));
         HIP_RET(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_RET(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_persistent_kernel),
+        HIP_RET(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(pcg_persistent_kernel<0>),
                                                              256, 0));
         const int want = std::max((int)nblk(S.nF, 256 / LANES), S.n_agg);
         W.coop_grid = std::max(1, std::min(want, per_cu * ncu));
@@ -1202,7 +1279,9 @@ int solver_cg_persistent(SolverWork& W, hipStream_t st) {
     SolverDev Sv = S;
     void* args[] = {&Sv,  (void*)&x,   &W.hsol,    &W.r,     &W.z,   &W.p, &W.q,       (void*)&binv,
                     &W.rc, (void*)&acinv, &W.partial, &W.partial2, &W.rzp, &W.cg, &gb, &W.cg_done};
-    HIP_RET(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(pcg_persistent_kernel), dim3(W.coop_grid),
+    const void* kern = S.K == 8 ? reinterpret_cast<const void*>(pcg_persistent_kernel<8>)
+                                : reinterpret_cast<const void*>(pcg_persistent_kernel<0>);
+    HIP_RET(hipLaunchCooperativeKernel(kern, dim3(W.coop_grid),
                                        dim3(256), args, 0, st));
     return 0;
 }

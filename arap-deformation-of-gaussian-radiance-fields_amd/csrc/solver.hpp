@@ -40,11 +40,13 @@ struct SolverDev {
 
 struct CgScalars {
     double rz, alpha, beta, rr, bb, tol2;
+    double rr_stop;   // max(tol^2 |b|^2, tol_floor^2 |b_0|^2), b_0 = the solve's first gradient
     int iters, max_iters;
 };
 
 struct GnScalars {
     double e_old;
+    double bb0;       // |b|^2 of the solve's first Gauss-Newton iteration
     int iters_done;
     int pad;
 };

@@ -101,7 +101,7 @@ def _frame(B, plain):
     sync = ctx.synchronize if plain else (lambda: None)
     ctx.aims_translate(st["node_pos"], B["active"], B["delta"], B["aims"])
     sync()
-    B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=True)
+    _, B["iters_done"] = B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=True)
     sync()
     n_inst = 0
     for cam, img in zip(B["cams"], B["images"]):
@@ -155,6 +155,7 @@ def timed(B, plain, steps, warmup):
     total = time.perf_counter() - t0
     kernel_ms = ctx.profile_read()
     ctx.profile_enable(False)
+    log(f"  GN iterations in the last frame: {int(B['iters_done'].item())}")
     return dict(total_s=total, frame_ms=times, kernel_ms=kernel_ms, instances=n_inst)
 
 
