@@ -90,6 +90,7 @@ EXPORTS = {
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
+    "arapgs_render_views": ([P, P, C.c_int, P], C.c_int),
 }
 
 
@@ -240,6 +241,28 @@ class Context:
         n = C.c_int(0)
         self._check(lib().arapgs_render(self.h, C.byref(d), C.byref(n)))
         return dict(image=out, num_rendered=n.value, surface=surface, radii=radii, rects=rects)
+
+    def render_views(self, means, shs, opac, scales, rots, cams, W, H, bg, outs, D=3, scale_mod=1.0,
+                     fast_culling=True, tile=16):
+        """All views in one pass (arapgs_render_views); cams: dicts with view, proj, campos (device
+        tensors), tx, ty; outs: (3, H, W) device tensors. Returns the total instance count."""
+        t = self._torch
+        dev = means.device
+        P_ = means.shape[0]
+        V = len(cams)
+        if not hasattr(self, "_rects") or self._rects is None or self._rects.shape[0] != V * P_:
+            self._rects = t.zeros((V * P_, 2), dtype=t.int32, device=dev) if fast_culling else None
+        arr = (RenderDesc * V)()
+        for v, (cam, out) in enumerate(zip(cams, outs)):
+            rects = None
+            if fast_culling:
+                rects = C.c_void_p(self._rects.data_ptr() + v * P_ * 8)
+            arr[v] = RenderDesc(P_, D, 16, ptr(bg), W, H, ptr(means), ptr(shs), None, ptr(opac), ptr(scales),
+                                scale_mod, ptr(rots), None, ptr(cam["view"]), ptr(cam["proj"]), ptr(cam["campos"]),
+                                cam["tx"], cam["ty"], 0, ptr(out), None, None, rects, None, None, 0, 1.0, tile)
+        n = C.c_int(0)
+        self._check(lib().arapgs_render_views(self.h, arr, V, C.byref(n)))
+        return n.value
 
     # ------------------------------------------------------------ grid
     def scene_aabb(self, pos, G):

@@ -48,6 +48,8 @@ struct arapgs_ctx {
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
+    RasterView* r_views = nullptr;    // device copy of the views of the current render
+    RasterView* r_views_h = nullptr;  // pinned host staging for it
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -131,6 +133,8 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMalloc(&c->r_total, sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&c->shrot_dev, sizeof(ShRotTables))) != hipSuccess ||
+        (e = hipMalloc(&c->r_views, sizeof(RasterView) * RASTER_MAX_VIEWS)) != hipSuccess ||
+        (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
         arapgs_destroy(c);
         return -5;
@@ -150,8 +154,9 @@ void arapgs_destroy(arapgs_ctx* c) {
     }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
                     (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges,
-                    (void*)c->r_total, (void*)c->shrot_dev})
+                    (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views})
         if (p) (void)hipFree(p);
+    if (c->r_views_h) (void)hipHostFree(c->r_views_h);
     delete c;
 }
 
@@ -474,42 +479,68 @@ int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const 
 
 extern "C" {
 
-int arapgs_render(arapgs_ctx* c, const arapgs_render_desc* d, int* num_rendered_h) {
+static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int* num_rendered_h) {
     CHECK_CTX(c);
-    if (!d || !d->background || !d->means3D || !d->shs || !d->opacities || !d->scales || !d->rotations ||
-        !d->viewmatrix || !d->projmatrix || !d->cam_pos || !d->out_color || d->P < 0 || d->width <= 0 ||
-        d->height <= 0)
+    if (!dv || V < 1 || V > RASTER_MAX_VIEWS) return fail(c, -1, "render: need 1..%d views", RASTER_MAX_VIEWS);
+    const arapgs_render_desc* d = dv;
+    if (!d->background || !d->means3D || !d->shs || !d->opacities || !d->scales || !d->rotations || d->P < 0 ||
+        d->width <= 0 || d->height <= 0)
         return fail(c, -1, "render: bad arguments");
-    if (d->colors_precomp || d->cov3D_precomp) return fail(c, -1, "render: precomputed colours/covariances are not supported");
-    if (d->prefiltered || d->to_ortho) return fail(c, -1, "render: prefiltered / orthographic modes are not supported");
     if (d->D < 0 || d->D > 3) return fail(c, -1, "render: SH degree must be 0..3");
     const int tile = d->tile ? d->tile : 16;
     if (tile != 16 && tile != 8) return fail(c, -1, "render: tile must be 8 or 16");
+    for (int v = 0; v < V; v++) {
+        const arapgs_render_desc* e = dv + v;
+        if (!e->viewmatrix || !e->projmatrix || !e->cam_pos || !e->out_color)
+            return fail(c, -1, "render: view %d: bad arguments", v);
+        if (e->colors_precomp || e->cov3D_precomp)
+            return fail(c, -1, "render: precomputed colours/covariances are not supported");
+        if (e->prefiltered || e->to_ortho) return fail(c, -1, "render: prefiltered / orthographic modes are not supported");
+        if (v > 0 && (e->P != d->P || e->D != d->D || e->width != d->width || e->height != d->height ||
+                      (e->tile ? e->tile : 16) != tile || e->means3D != d->means3D || e->shs != d->shs ||
+                      e->opacities != d->opacities || e->scales != d->scales || e->rotations != d->rotations ||
+                      e->background != d->background || e->scale_modifier != d->scale_modifier ||
+                      (e->rects != nullptr) != (d->rects != nullptr) || e->boxmin_h != d->boxmin_h ||
+                      e->boxmax_h != d->boxmax_h))
+            return fail(c, -1, "render: view %d differs from view 0 in a shared argument", v);
+    }
     (void)hipSetDevice(c->device);
     RasterParams rp;
-    rp.P = d->P; rp.D = d->D; rp.W = d->width; rp.H = d->height; rp.tile = tile;
+    rp.P = d->P; rp.D = d->D; rp.W = d->width; rp.H = d->height; rp.tile = tile; rp.V = V;
     rp.gx = (d->width + tile - 1) / tile; rp.gy = (d->height + tile - 1) / tile;
     rp.bg = d->background; rp.means = d->means3D; rp.shs = d->shs; rp.opac = d->opacities; rp.scales = d->scales;
-    rp.rots = d->rotations; rp.view = d->viewmatrix; rp.proj = d->projmatrix; rp.campos = d->cam_pos;
-    rp.scale_mod = d->scale_modifier; rp.tanfovx = d->tan_fovx; rp.tanfovy = d->tan_fovy;
-    rp.focal_x = (float)d->width / (2.0f * d->tan_fovx);
-    rp.focal_y = (float)d->height / (2.0f * d->tan_fovy);
+    rp.rots = d->rotations; rp.scale_mod = d->scale_modifier;
     rp.use_rects = d->rects != nullptr;
     rp.use_box = (d->boxmin_h && d->boxmax_h) ? 1 : 0;
     for (int i = 0; i < 3; i++) {
         rp.boxmin[i] = rp.use_box ? d->boxmin_h[i] : 0.0f;
         rp.boxmax[i] = rp.use_box ? d->boxmax_h[i] : 0.0f;
     }
-    int rc = grow(c, &c->r_geo, &c->r_geo_cap, raster_geo_bytes(std::max(d->P, 1)));
-    if (!rc) rc = grow(c, &c->r_scan, &c->r_scan_cap, sizeof(uint32_t) * scan_workspace_words(std::max(d->P, 1)));
+    RasterView* hv = c->r_views_h;
+    for (int v = 0; v < V; v++) {
+        const arapgs_render_desc* e = dv + v;
+        hv[v].view = e->viewmatrix; hv[v].proj = e->projmatrix; hv[v].campos = e->cam_pos;
+        hv[v].tanfovx = e->tan_fovx; hv[v].tanfovy = e->tan_fovy;
+        hv[v].focal_x = (float)e->width / (2.0f * e->tan_fovx);
+        hv[v].focal_y = (float)e->height / (2.0f * e->tan_fovy);
+        hv[v].out = e->out_color; hv[v].surface = e->surface; hv[v].radii = e->radii; hv[v].rects = e->rects;
+    }
+    // the staging copy completes before this call returns (the stream is
+    // synchronised for the instance count below), so the pinned buffer is free again
+    hipError_t e = hipMemcpyAsync(c->r_views, hv, sizeof(RasterView) * V, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "render views");
+    rp.views = c->r_views;
+    const int64_t PV = (int64_t)std::max(d->P, 1) * V;
+    int rc = grow(c, &c->r_geo, &c->r_geo_cap, raster_geo_bytes(PV));
+    if (!rc) rc = grow(c, &c->r_scan, &c->r_scan_cap, sizeof(uint32_t) * scan_workspace_words(PV));
     const int ntiles = rp.gx * rp.gy;
-    if (!rc) rc = grow(c, &c->r_ranges, &c->r_ranges_cap, sizeof(uint2) * (size_t)ntiles);
+    if (!rc) rc = grow(c, &c->r_ranges, &c->r_ranges_cap, sizeof(uint2) * (size_t)ntiles * V);
     if (rc) return rc;
-    rc = check_launch(c, raster_preprocess(rp, c->r_geo, (uint32_t*)c->r_scan, c->r_total, d->radii, d->rects,
-                                           c->stream), "render preprocess");
+    rc = check_launch(c, raster_preprocess(rp, c->r_geo, (uint32_t*)c->r_scan, c->r_total, c->stream),
+                      "render preprocess");
     if (rc) return rc;
     uint32_t n = 0;
-    hipError_t e = hipMemcpyAsync(&n, c->r_total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(&n, c->r_total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "render");
     if (n > 0x7fffffffu) return fail(c, -1, "render: too many tile instances (%u)", n);
@@ -523,13 +554,20 @@ int arapgs_render(arapgs_ctx* c, const arapgs_render_desc* d, int* num_rendered_
     uint32_t* vals = (uint32_t*)(b + nn * 16);
     uint32_t* vals_alt = (uint32_t*)(b + nn * 20);
     rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, keys, vals, keys_alt, vals_alt,
-                                              (uint32_t*)c->r_sort, (uint2*)c->r_ranges, d->out_color, d->surface,
-                                              c->stream), "render");
+                                              (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->stream), "render");
     if (rc) return rc;
     c->last_rendered = n;
-    c->last_tiles = ntiles;
+    c->last_tiles = ntiles * V;
     if (num_rendered_h) *num_rendered_h = (int)n;
     return 0;
+}
+
+int arapgs_render(arapgs_ctx* c, const arapgs_render_desc* d, int* num_rendered_h) {
+    return render_views(c, d, 1, num_rendered_h);
+}
+
+int arapgs_render_views(arapgs_ctx* c, const arapgs_render_desc* descs, int n_views, int* num_rendered_h) {
+    return render_views(c, descs, n_views, num_rendered_h);
 }
 
 int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64_t cap, uint32_t* ranges_h,

@@ -33,18 +33,19 @@ struct Geo {
     uint32_t* touched; // P
 };
 
-__global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo geo, int* __restrict__ radii,
-                                                         int* __restrict__ rects_out) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rp.P) return;
-    geo.touched[g] = 0;
+// One Gaussian in one view (record index gv = v*P + g).
+__device__ __forceinline__ void project_one(const RasterParams& rp, const RasterView& vw, int g, int64_t gv, Geo& geo,
+                                            const float* __restrict__ sh) {
+    int* radii = vw.radii;
+    int* rects_out = vw.rects;
+    geo.touched[gv] = 0;
     if (radii) radii[g] = 0;
     const float px = rp.means[3 * g], py = rp.means[3 * g + 1], pz = rp.means[3 * g + 2];
     if (rp.use_box && (px < rp.boxmin[0] || py < rp.boxmin[1] || pz < rp.boxmin[2] || px > rp.boxmax[0] ||
                        py > rp.boxmax[1] || pz > rp.boxmax[2]))
         return;
-    const float* V = rp.view;
-    const float* Pm = rp.proj;
+    const float* V = vw.view;
+    const float* Pm = vw.proj;
     const float vx = V[0] * px + V[4] * py + V[8] * pz + V[12];
     const float vy = V[1] * px + V[5] * py + V[9] * pz + V[13];
     const float vz = V[2] * px + V[6] * py + V[10] * pz + V[14];
@@ -77,12 +78,12 @@ __global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo ge
 #pragma unroll
         for (int j = 0; j < 3; j++)
             S[i][j] = R[i][0] * s2[0] * R[j][0] + R[i][1] * s2[1] * R[j][1] + R[i][2] * s2[2] * R[j][2];
-    const float limx = 1.3f * rp.tanfovx, limy = 1.3f * rp.tanfovy;
+    const float limx = 1.3f * vw.tanfovx, limy = 1.3f * vw.tanfovy;
     const float txtz = vx / vz, tytz = vy / vz;
     const float tx = fminf(limx, fmaxf(-limx, txtz)) * vz;
     const float ty = fminf(limy, fmaxf(-limy, tytz)) * vz;
-    const float j00 = rp.focal_x / vz, j02 = -(rp.focal_x * tx) / (vz * vz);
-    const float j11 = rp.focal_y / vz, j12 = -(rp.focal_y * ty) / (vz * vz);
+    const float j00 = vw.focal_x / vz, j02 = -(vw.focal_x * tx) / (vz * vz);
+    const float j11 = vw.focal_y / vz, j12 = -(vw.focal_y * ty) / (vz * vz);
     float T[2][3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -119,10 +120,9 @@ __global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo ge
     const int area = (r1x - r0x) * (r1y - r0y);
     if (area == 0) return;
     // view-dependent colour (degree D)
-    float dx = px - rp.campos[0], dy = py - rp.campos[1], dz = pz - rp.campos[2];
+    float dx = px - vw.campos[0], dy = py - vw.campos[1], dz = pz - vw.campos[2];
     const float dn = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
     dx /= dn; dy /= dn; dz /= dn;
-    const float* sh = rp.shs + 48 * (int64_t)g;
     float col[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ch++) {
@@ -147,28 +147,49 @@ __global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo ge
         }
         col[ch] = fmaxf(r + 0.5f, 0.0f);
     }
-    geo.depth[g] = vz;
-    geo.xy[g] = make_float2(ix, iy);
-    geo.conic[g] = make_float4(cc * dinv, -cb * dinv, ca * dinv, rp.opac[g]);
-    geo.rgb[g] = make_float4(col[0], col[1], col[2], 0.0f);
-    geo.rect[g] = make_int4(r0x, r0y, r1x, r1y);
-    geo.touched[g] = (uint32_t)area;
+    geo.depth[gv] = vz;
+    geo.xy[gv] = make_float2(ix, iy);
+    geo.conic[gv] = make_float4(cc * dinv, -cb * dinv, ca * dinv, rp.opac[g]);
+    geo.rgb[gv] = make_float4(col[0], col[1], col[2], 0.0f);
+    geo.rect[gv] = make_int4(r0x, r0y, r1x, r1y);
+    geo.touched[gv] = (uint32_t)area;
     if (radii) radii[g] = rad;
     if (rects_out) { rects_out[2 * g] = ex; rects_out[2 * g + 1] = ey; }
 }
 
-__global__ void __launch_bounds__(256) emit_keys_kernel(int P, int gx, Geo geo, const uint32_t* __restrict__ offsets,
-                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// One thread per Gaussian for all V views: the 48 SH coefficients are read once.
+__global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo geo) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= P) return;
-    if (geo.touched[g] == 0) return;
-    uint32_t o = offsets[g];
-    const int4 r = geo.rect[g];
-    const uint64_t db = (uint64_t)__float_as_uint(geo.depth[g]);
+    if (g >= rp.P) return;
+    float sh[48];
+    const float4* s4 = reinterpret_cast<const float4*>(rp.shs + 48 * (int64_t)g);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 q = s4[i];
+        sh[4 * i] = q.x; sh[4 * i + 1] = q.y; sh[4 * i + 2] = q.z; sh[4 * i + 3] = q.w;
+    }
+    for (int v = 0; v < rp.V; v++) {
+        const RasterView vw = rp.views[v];
+        project_one(rp, vw, g, (int64_t)v * rp.P + g, geo, sh);
+    }
+}
+
+// One thread per (view, Gaussian) record gv = v*P + g: one key per touched
+// tile, (v*ntiles + tile) << 32 | depth bits; value = the record index gv.
+__global__ void __launch_bounds__(256) emit_keys_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t gv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gv >= PV) return;
+    if (geo.touched[gv] == 0) return;
+    const uint32_t base = (uint32_t)(gv / P) * (uint32_t)ntiles;
+    uint32_t o = offsets[gv];
+    const int4 r = geo.rect[gv];
+    const uint64_t db = (uint64_t)__float_as_uint(geo.depth[gv]);
     for (int y = r.y; y < r.w; y++)
         for (int x = r.x; x < r.z; x++) {
-            keys[o] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | db;
-            vals[o] = (uint32_t)g;
+            keys[o] = ((uint64_t)(base + (uint32_t)(y * gx + x)) << 32) | db;
+            vals[o] = (uint32_t)gv;
             o++;
         }
 }
@@ -194,8 +215,7 @@ __global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t*
 // loop over Gaussians that can contribute to this tile.
 template <int TILE>
 __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
-                                                          const uint2* __restrict__ ranges, float* __restrict__ out,
-                                                          int* __restrict__ surface) {
+                                                          const uint2* __restrict__ ranges) {
     constexpr int NT = TILE * TILE;
     constexpr int NW = (NT + 63) / 64;
     __shared__ float2 s_xy[NT];
@@ -203,14 +223,19 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
     __shared__ float4 s_rgb[NT];
     __shared__ uint32_t s_id[NT];
     __shared__ int s_cnt[NW];
-    const int tile = blockIdx.x;
+    const int ntl = rp.gx * rp.gy;
+    const int view = blockIdx.x / ntl;
+    const int tile = blockIdx.x - view * ntl;
+    float* __restrict__ out = rp.views[view].out;
+    int* __restrict__ surface = rp.views[view].surface;
+    const uint32_t vbase = (uint32_t)view * (uint32_t)rp.P;
     const int tx = tile % rp.gx, ty = tile / rp.gx;
     const int px = tx * TILE + (threadIdx.x % TILE), py = ty * TILE + (threadIdx.x / TILE);
     const bool inside = px < rp.W && py < rp.H;
     const float fxp = (float)px, fyp = (float)py;
     const float bx0 = (float)(tx * TILE), by0 = (float)(ty * TILE);
     const float bx1 = (float)min(tx * TILE + TILE - 1, rp.W - 1), by1 = (float)min(ty * TILE + TILE - 1, rp.H - 1);
-    const uint2 rg = ranges[tile];
+    const uint2 rg = ranges[blockIdx.x];
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     bool done = !inside;
     const int n = (int)(rg.y - rg.x);
@@ -260,7 +285,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
             C0 += c.x * alpha * T;
             C1 += c.y * alpha * T;
             C2 += c.z * alpha * T;
-            if (surface) surface[s_id[j]] = 1;
+            if (surface) surface[s_id[j] - vbase] = 1;
             T = test_T;
         }
     }
@@ -283,8 +308,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
 // same bit for bit; only the work on Gaussians that cannot reach 1/255 in the
 // quadrant is skipped.
 __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
-                                                         const uint2* __restrict__ ranges, float* __restrict__ out,
-                                                         int* __restrict__ surface) {
+                                                         const uint2* __restrict__ ranges) {
     constexpr int NT = 256;
     __shared__ float2 s_xy[NT];
     __shared__ float4 s_co[NT];
@@ -292,7 +316,12 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
     __shared__ uint32_t s_id[NT];
     __shared__ __align__(4) uint8_t s_list[4][NT + 4];
     __shared__ int s_cnt[4][4];  // [quadrant][wave]
-    const int tile = blockIdx.x;
+    const int ntl = rp.gx * rp.gy;
+    const int view = blockIdx.x / ntl;
+    const int tile = blockIdx.x - view * ntl;
+    float* __restrict__ out = rp.views[view].out;
+    int* __restrict__ surface = rp.views[view].surface;
+    const uint32_t vbase = (uint32_t)view * (uint32_t)rp.P;
     const int tx = tile % rp.gx, ty = tile / rp.gx;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = tx * 16 + (wv & 1) * 8 + (lane & 7), py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
@@ -310,7 +339,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
         qb[q][2] = (float)min(x0 + 7, rp.W - 1);
         qb[q][3] = (float)min(y0 + 7, rp.H - 1);
     }
-    const uint2 rg = ranges[tile];
+    const uint2 rg = ranges[blockIdx.x];
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     bool done = !inside;
     const int n = (int)(rg.y - rg.x);
@@ -382,7 +411,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
                         C0 += c.x * alpha[u] * T;
                         C1 += c.y * alpha[u] * T;
                         C2 += c.z * alpha[u] * T;
-                        if (surface) surface[s_id[k[u]]] = 1;
+                        if (surface) surface[s_id[k[u]] - vbase] = 1;
                         T = test_T;
                     }
                 }
@@ -403,12 +432,12 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-size_t raster_geo_bytes(int P) {
+size_t raster_geo_bytes(int64_t PV) {
     // depth, xy, conic, rgb, rect, touched, offsets (+ alignment slack)
-    return (size_t)P * (4 + 8 + 16 + 16 + 16 + 4 + 4) + 7 * 256;
+    return (size_t)PV * (4 + 8 + 16 + 16 + 16 + 4 + 4) + 7 * 256;
 }
 
-static Geo carve_geo(void* base, int P) {
+static Geo carve_geo(void* base, int64_t PV) {
     char* p = (char*)base;
     auto take = [&](size_t bytes) {
         char* r = p;
@@ -416,54 +445,55 @@ static Geo carve_geo(void* base, int P) {
         return r;
     };
     Geo g;
-    g.depth = (float*)take(4 * (size_t)P);
-    g.xy = (float2*)take(8 * (size_t)P);
-    g.conic = (float4*)take(16 * (size_t)P);
-    g.rgb = (float4*)take(16 * (size_t)P);
-    g.rect = (int4*)take(16 * (size_t)P);
-    g.touched = (uint32_t*)take(4 * (size_t)P);
+    g.depth = (float*)take(4 * (size_t)PV);
+    g.xy = (float2*)take(8 * (size_t)PV);
+    g.conic = (float4*)take(16 * (size_t)PV);
+    g.rgb = (float4*)take(16 * (size_t)PV);
+    g.rect = (int4*)take(16 * (size_t)PV);
+    g.touched = (uint32_t*)take(4 * (size_t)PV);
     return g;
 }
 
-static uint32_t* geo_offsets(void* base, int P) {
+static uint32_t* geo_offsets(void* base, int64_t PV) {
     char* p = (char*)base;
     size_t off = 0;
     const size_t sz[6] = {4, 8, 16, 16, 16, 4};
-    for (int i = 0; i < 6; i++) off += (sz[i] * (size_t)P + 255) & ~(size_t)255;
+    for (int i = 0; i < 6; i++) off += (sz[i] * (size_t)PV + 255) & ~(size_t)255;
     return (uint32_t*)(p + off);
 }
 
-int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, int* radii,
-                      int* rects, hipStream_t st) {
-    Geo geo = carve_geo(geo_ws, rp.P);
+int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, hipStream_t st) {
+    const int64_t PV = (int64_t)rp.P * rp.V;
+    Geo geo = carve_geo(geo_ws, PV);
     if (rp.P > 0) {
-        hipLaunchKernelGGL(preprocess_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo, radii, rects);
+        hipLaunchKernelGGL(preprocess_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo);
         HIP_RET(hipGetLastError());
     }
-    return launch_exclusive_scan(geo.touched, rp.P, geo_offsets(geo_ws, rp.P), d_total, scan_ws, st);
+    return launch_exclusive_scan(geo.touched, PV, geo_offsets(geo_ws, PV), d_total, scan_ws, st);
 }
 
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* sort_ws,
-                         uint2* ranges, float* out, int* surface, hipStream_t st) {
-    Geo geo = carve_geo(geo_ws, rp.P);
+                         uint2* ranges, hipStream_t st) {
+    const int64_t PV = (int64_t)rp.P * rp.V;
+    Geo geo = carve_geo(geo_ws, PV);
     const int ntiles = rp.gx * rp.gy;
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(nblk(2 * (int64_t)ntiles, 256)), dim3(256), 0, st, (uint32_t*)ranges,
-                       2 * (int64_t)ntiles);
+    const int64_t nt = (int64_t)ntiles * rp.V;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(nblk(2 * nt, 256)), dim3(256), 0, st, (uint32_t*)ranges, 2 * nt);
     if (n_rendered > 0) {
-        hipLaunchKernelGGL(emit_keys_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp.P, rp.gx, geo,
-                           geo_offsets(geo_ws, rp.P), keys, vals);
+        hipLaunchKernelGGL(emit_keys_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.gx, ntiles, geo,
+                           geo_offsets(geo_ws, PV), keys, vals);
         int tile_bits = 0;
-        while ((1 << tile_bits) < ntiles) tile_bits++;
+        while ((int64_t(1) << tile_bits) < nt) tile_bits++;
         int rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, 32 + tile_bits, sort_ws, st);
         if (rc) return rc;
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
                            keys, d_total, ranges);
     }
     if (rp.tile == 16)
-        hipLaunchKernelGGL(blend_quad_kernel, dim3(ntiles), dim3(256), 0, st, rp, geo, vals, ranges, out, surface);
+        hipLaunchKernelGGL(blend_quad_kernel, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
     else if (rp.tile == 8)
-        hipLaunchKernelGGL(blend_kernel<8>, dim3(ntiles), dim3(64), 0, st, rp, geo, vals, ranges, out, surface);
+        hipLaunchKernelGGL(blend_kernel<8>, dim3((unsigned)nt), dim3(64), 0, st, rp, geo, vals, ranges);
     else
         return -1;
     HIP_RET(hipGetLastError());

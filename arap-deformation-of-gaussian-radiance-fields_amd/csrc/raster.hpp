@@ -7,20 +7,33 @@
 
 namespace arapgs {
 
+// One camera of a (multi-view) render. Lives in device memory.
+struct RasterView {
+    const float* view;     // device, 16 column-major
+    const float* proj;     // device, 16 column-major
+    const float* campos;   // device, 3
+    float tanfovx, tanfovy, focal_x, focal_y;
+    float* out;            // device, 3*H*W planar CHW
+    int* surface;          // optional, P
+    int* radii;            // optional, P
+    int* rects;            // optional, 2*P
+};
+constexpr int RASTER_MAX_VIEWS = 16;
+
+// V views of one set of Gaussians at one resolution. Per-view records are laid
+// out view-major (index v*P + g); keys carry the combined tile id v*ntiles + t.
 struct RasterParams {
-    int P, D, W, H, tile, gx, gy;
+    int P, D, W, H, tile, gx, gy, V;
     const float* bg;       // device, 3
     const float* means;    // P*3
     const float* shs;      // P*48
     const float* opac;     // P
     const float* scales;   // P*3
     const float* rots;     // P*4 (w,x,y,z)
-    const float* view;     // device, 16 column-major
-    const float* proj;     // device, 16 column-major
-    const float* campos;   // device, 3
-    float scale_mod, tanfovx, tanfovy, focal_x, focal_y;
+    float scale_mod;
     int use_rects, use_box;
     float boxmin[3], boxmax[3];
+    const RasterView* views;  // device, V
 };
 
 // Conservative test whether a Gaussian can reach alpha >= 1/255 at any pixel
@@ -58,13 +71,13 @@ __host__ __device__ inline bool tile_may_contribute(float2 xy, float4 co, float 
     return box_may_contribute(xy, co, cull_log_threshold(co.w), x0, y0, x1, y1);
 }
 
-size_t raster_geo_bytes(int P);
-// Preprocess + scan of tiles touched; *d_total (device) = instances to render.
-int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, int* radii,
-                      int* rects, hipStream_t st);
+size_t raster_geo_bytes(int64_t PV);
+// Preprocess of all V views + scan of tiles touched; *d_total (device) =
+// instances to render. scan_ws sized for V*P.
+int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, hipStream_t st);
 // Keys, sort, ranges and blend for n_rendered instances.
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* sort_ws,
-                         uint2* ranges, float* out, int* surface, hipStream_t st);
+                         uint2* ranges, hipStream_t st);
 
 }  // namespace arapgs

@@ -13,7 +13,8 @@ the frame splits into two halves on two streams:
                            node blend(t) (node positions for t+1); event A[t%2]
   deform side  (stream B)  wait A[t%2]; sample blend, six-point update, node
                            quaternions, sample SH with table slot t%2; event
-                           B[t%2] (slot free again); V renders
+                           B[t%2] (slot free again); the V views in one
+                           render pass
 
 and frame t's deform side runs while frame t+1's solve runs. Each side has its
 own host thread, because each side synchronises with the host (the solve's CG
@@ -92,12 +93,10 @@ class DragPipeline:
             self.ev_b[slot].record(self.stream_b)
             self.ev_b_recorded[slot] = True
             n_inst = 0
-            if render:
-                for cam, img in zip(self.cams, self.images):
-                    out = ctx.render(st["pos"], st["sh"], self.opac, st["scale"], st["rot"], cam["view"],
-                                     cam["proj"], cam["campos"], cam["tx"], cam["ty"], img.shape[2], img.shape[1],
-                                     self.bg, D=self.D, out=img)
-                    n_inst += out["num_rendered"]
+            if render and self.images:
+                img = self.images[0]
+                n_inst = ctx.render_views(st["pos"], st["sh"], self.opac, st["scale"], st["rot"], self.cams,
+                                          img.shape[2], img.shape[1], self.bg, self.images, D=self.D)
             return n_inst
 
     # ------------------------------------------------------------ driver

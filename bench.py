@@ -19,7 +19,8 @@ Three forms are timed on the same state (the drag continues across them):
              sync after each launch);
   fast       stages queued back to back; the CG phases carry a device-side
              convergence flag (kernels queued past convergence exit at once)
-             and the host looks at it only every 16 CG iterations;
+             and the host looks at it only every 16 CG iterations; the views
+             are rendered in one pass (arapgs_render_views);
   pipelined  as fast, with frame t's deformation and renders on a second stream
              overlapping frame t+1's solve (pipeline.DragPipeline: two node-table
              slots, events between the halves, one host thread per stream);
@@ -103,6 +104,11 @@ def _frame(B, plain):
     sync()
     _, B["iters_done"] = B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=True)
     sync()
+    if not plain:
+        # all views in one pass (one sort, one blend launch; bit-identical images)
+        img = B["images"][0]
+        return ctx.render_views(st["pos"], st["sh"], B["opac"], st["scale"], st["rot"], B["cams"], img.shape[2],
+                                img.shape[1], B["bg"], B["images"])
     n_inst = 0
     for cam, img in zip(B["cams"], B["images"]):
         out = ctx.render(st["pos"], st["sh"], B["opac"], st["scale"], st["rot"], cam["view"], cam["proj"],

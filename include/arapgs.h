@@ -253,6 +253,17 @@ typedef struct arapgs_render_desc {
 
 int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_rendered_h);
 
+/* The same forward pass for n_views cameras of one Gaussian set in one go
+ * (GaussianView renders its views one Rasterizer::forward call at a time):
+ * descs_h[v] differ only in viewmatrix, projmatrix, cam_pos, tan_fovx/y,
+ * out_color, surface, radii and rects; every other field must equal view 0's.
+ * Each Gaussian's SH is read once for all views, the (tile, depth) keys of all
+ * views (combined tile id view * ntiles + tile) are sorted together and one
+ * blend launch covers every tile of every view, so the deepest tiles of the
+ * views overlap. Images are identical bit for bit to n_views separate
+ * arapgs_render calls. n_views <= 16; *num_rendered_h = total instances. */
+int arapgs_render_views(arapgs_ctx* ctx, const arapgs_render_desc* descs_h, int n_views, int* num_rendered_h);
+
 /* Debug view of the last arapgs_render on this context: sorted keys
  * (u64, tile << 32 | depth bits) and values (Gaussian ids) copied to host
  * buffers of capacity cap; tile ranges (u32 pairs [start,end)) to ranges_h

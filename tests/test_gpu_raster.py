@@ -65,3 +65,27 @@ def test_render_crop_box(ctx, scene):
                      T(cam.campos), cam.tanfovx, cam.tanfovy, 400, 400, T(bg), boxmin=bmin, boxmax=bmax)
     assert out["num_rendered"] == ref["keys"].shape[0]
     assert np.abs(H(out["image"]) - ref["image"]).max() <= 1.0 / 255.0
+
+
+@pytest.mark.parametrize("tile", [16, 8])
+def test_render_views_equals_separate_renders(ctx, scene, tile):
+    """arapgs_render_views (all views in one pass) gives each view's image bit for
+    bit as its own arapgs_render call, and the instance count is the sum."""
+    import torch
+    g = scene["g"]
+    W, H_ = 320, 256
+    cams = [S.look_at_camera((0.3 * i, -4.0 + i, 0.8), (0.0, 0.0, 0.0), 50.0, W, H_) for i in range(3)]
+    bg = T(np.array([0.8, 0.8, 0.8], np.float32))
+    args = (T(g.pos), T(g.sh), T(g.opacity), T(g.scale), T(g.rot))
+    sep, total = [], 0
+    for cam in cams:
+        out = ctx.render(*args, T(cam.view), T(cam.proj), T(cam.campos), cam.tanfovx, cam.tanfovy, W, H_, bg,
+                         tile=tile)
+        sep.append(H(out["image"]))
+        total += out["num_rendered"]
+    outs = [torch.empty((3, H_, W), device="cuda") for _ in cams]
+    cd = [dict(view=T(c.view), proj=T(c.proj), campos=T(c.campos), tx=c.tanfovx, ty=c.tanfovy) for c in cams]
+    n = ctx.render_views(*args, cd, W, H_, bg, outs, tile=tile)
+    assert n == total
+    for a, b in zip(sep, outs):
+        assert np.array_equal(a, H(b))
