@@ -48,6 +48,8 @@ struct arapgs_ctx {
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
+    uint32_t last_dmin = 0;   // depth-bit offset and width of the last render's keys
+    int last_dbits = 32;
     RasterView* r_views = nullptr;    // device copy of the views of the current render
     RasterView* r_views_h = nullptr;  // pinned host staging for it
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
@@ -131,7 +133,7 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipMalloc(&c->fps_partial, sizeof(unsigned long long) * FPS_BLOCKS)) != hipSuccess ||
         (e = hipMalloc(&c->fps_counter, sizeof(unsigned int))) != hipSuccess ||
         (e = hipMalloc(&c->fps_best, sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&c->r_total, sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&c->r_total, 4 * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&c->shrot_dev, sizeof(ShRotTables))) != hipSuccess ||
         (e = hipMalloc(&c->r_views, sizeof(RasterView) * RASTER_MAX_VIEWS)) != hipSuccess ||
         (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
@@ -539,10 +541,14 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     rc = check_launch(c, raster_preprocess(rp, c->r_geo, (uint32_t*)c->r_scan, c->r_total, c->stream),
                       "render preprocess");
     if (rc) return rc;
-    uint32_t n = 0;
-    e = hipMemcpyAsync(&n, c->r_total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    uint32_t cnt[3] = {0, 0, 0};
+    e = hipMemcpyAsync(cnt, c->r_total, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "render");
+    const uint32_t n = cnt[0];
+    const uint32_t dmin = n > 0 ? cnt[1] : 0u;
+    int dbits = 1;
+    while (n > 0 && dbits < 32 && ((uint64_t)(cnt[2] - cnt[1]) >> dbits) != 0) dbits++;
     if (n > 0x7fffffffu) return fail(c, -1, "render: too many tile instances (%u)", n);
     const size_t nn = std::max<size_t>(n, 1);
     rc = grow(c, &c->r_bin, &c->r_bin_cap, nn * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 1024);
@@ -553,9 +559,12 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     uint64_t* keys_alt = (uint64_t*)(b + nn * 8);
     uint32_t* vals = (uint32_t*)(b + nn * 16);
     uint32_t* vals_alt = (uint32_t*)(b + nn * 20);
-    rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, keys, vals, keys_alt, vals_alt,
-                                              (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->stream), "render");
+    rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, dmin, dbits, keys, vals, keys_alt,
+                                              vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->stream),
+                      "render");
     if (rc) return rc;
+    c->last_dmin = dmin;
+    c->last_dbits = dbits;
     c->last_rendered = n;
     c->last_tiles = ntiles * V;
     if (num_rendered_h) *num_rendered_h = (int)n;
@@ -577,7 +586,13 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
     const int64_t n = std::min<int64_t>(cap, c->last_rendered);
     char* b = (char*)c->r_bin;
     hipError_t e = hipStreamSynchronize(c->stream);
-    if (e == hipSuccess && keys_h && n > 0) e = hipMemcpy(keys_h, b, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && keys_h && n > 0) {
+        e = hipMemcpy(keys_h, b, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+        // back to the reference key layout, tile << 32 | depth bits
+        const uint64_t mask = (c->last_dbits >= 64) ? ~0ull : ((1ull << c->last_dbits) - 1ull);
+        for (int64_t i = 0; e == hipSuccess && i < n; i++)
+            keys_h[i] = ((keys_h[i] >> c->last_dbits) << 32) | (uint64_t)((uint32_t)(keys_h[i] & mask) + c->last_dmin);
+    }
     if (e == hipSuccess && vals_h && n > 0) e = hipMemcpy(vals_h, b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
     const int64_t nr = std::min<int64_t>(ranges_cap, c->last_tiles);
     if (e == hipSuccess && ranges_h && nr > 0) e = hipMemcpy(ranges_h, c->r_ranges, sizeof(uint2) * nr, hipMemcpyDeviceToHost);

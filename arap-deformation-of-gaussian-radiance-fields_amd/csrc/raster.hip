@@ -158,37 +158,71 @@ __device__ __forceinline__ void project_one(const RasterParams& rp, const Raster
 }
 
 // One thread per Gaussian for all V views: the 48 SH coefficients are read once.
-__global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo geo) {
+// Also folds the depth-bit range of the visible records into dminmax[0..1]
+// (bit patterns of positive floats order like the floats).
+__global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo geo, uint32_t* __restrict__ dminmax) {
+    __shared__ uint32_t s_mn[4], s_mx[4];
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rp.P) return;
-    float sh[48];
-    const float4* s4 = reinterpret_cast<const float4*>(rp.shs + 48 * (int64_t)g);
+    uint32_t mn = 0xffffffffu, mx = 0u;
+    if (g < rp.P) {
+        float sh[48];
+        const float4* s4 = reinterpret_cast<const float4*>(rp.shs + 48 * (int64_t)g);
 #pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const float4 q = s4[i];
-        sh[4 * i] = q.x; sh[4 * i + 1] = q.y; sh[4 * i + 2] = q.z; sh[4 * i + 3] = q.w;
+        for (int i = 0; i < 12; i++) {
+            const float4 q = s4[i];
+            sh[4 * i] = q.x; sh[4 * i + 1] = q.y; sh[4 * i + 2] = q.z; sh[4 * i + 3] = q.w;
+        }
+        for (int v = 0; v < rp.V; v++) {
+            const RasterView vw = rp.views[v];
+            const int64_t gv = (int64_t)v * rp.P + g;
+            project_one(rp, vw, g, gv, geo, sh);
+            if (geo.touched[gv]) {
+                const uint32_t db = __float_as_uint(geo.depth[gv]);
+                mn = min(mn, db);
+                mx = max(mx, db);
+            }
+        }
     }
-    for (int v = 0; v < rp.V; v++) {
-        const RasterView vw = rp.views[v];
-        project_one(rp, vw, g, (int64_t)v * rp.P + g, geo, sh);
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { s_mn[wv] = mn; s_mx[wv] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) { mn = min(mn, s_mn[w]); mx = max(mx, s_mx[w]); }
+        if (mn <= mx) {
+            atomicMin(&dminmax[0], mn);
+            atomicMax(&dminmax[1], mx);
+        }
     }
 }
 
+__global__ void depth_range_init_kernel(uint32_t* __restrict__ dminmax) {
+    dminmax[0] = 0xffffffffu;
+    dminmax[1] = 0u;
+}
+
 // One thread per (view, Gaussian) record gv = v*P + g: one key per touched
-// tile, (v*ntiles + tile) << 32 | depth bits; value = the record index gv.
+// tile, (v*ntiles + tile) << dbits | (depth bits - dmin); value = the record
+// index gv. dbits covers the frame's depth-bit range, so the significant key
+// bits are contiguous and the sort needs fewer passes; the order is the order
+// of (tile, depth bits), since subtracting dmin is monotone.
 __global__ void __launch_bounds__(256) emit_keys_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
-                                                        const uint32_t* __restrict__ offsets,
-                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                        const uint32_t* __restrict__ offsets, uint32_t dmin,
+                                                        int dbits, uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals) {
     const int64_t gv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gv >= PV) return;
     if (geo.touched[gv] == 0) return;
     const uint32_t base = (uint32_t)(gv / P) * (uint32_t)ntiles;
     uint32_t o = offsets[gv];
     const int4 r = geo.rect[gv];
-    const uint64_t db = (uint64_t)__float_as_uint(geo.depth[gv]);
+    const uint64_t db = (uint64_t)(__float_as_uint(geo.depth[gv]) - dmin);
     for (int y = r.y; y < r.w; y++)
         for (int x = r.x; x < r.z; x++) {
-            keys[o] = ((uint64_t)(base + (uint32_t)(y * gx + x)) << 32) | db;
+            keys[o] = ((uint64_t)(base + (uint32_t)(y * gx + x)) << dbits) | db;
             vals[o] = (uint32_t)gv;
             o++;
         }
@@ -199,13 +233,13 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
         p[i] = 0u;
 }
 
-__global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
+__global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, int dbits,
                               uint2* __restrict__ ranges) {
     const uint32_t n = *n_ptr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t t = (uint32_t)(keys[i] >> 32);
-        if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != t) ranges[t].x = i;
-        if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != t) ranges[t].y = i + 1;
+        const uint32_t t = (uint32_t)(keys[i] >> dbits);
+        if (i == 0 || (uint32_t)(keys[i - 1] >> dbits) != t) ranges[t].x = i;
+        if (i == n - 1 || (uint32_t)(keys[i + 1] >> dbits) != t) ranges[t].y = i + 1;
     }
 }
 
@@ -462,19 +496,20 @@ static uint32_t* geo_offsets(void* base, int64_t PV) {
     return (uint32_t*)(p + off);
 }
 
-int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, hipStream_t st) {
+int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_counts, hipStream_t st) {
     const int64_t PV = (int64_t)rp.P * rp.V;
     Geo geo = carve_geo(geo_ws, PV);
+    hipLaunchKernelGGL(depth_range_init_kernel, dim3(1), dim3(1), 0, st, d_counts + 1);
     if (rp.P > 0) {
-        hipLaunchKernelGGL(preprocess_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo);
+        hipLaunchKernelGGL(preprocess_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo, d_counts + 1);
         HIP_RET(hipGetLastError());
     }
-    return launch_exclusive_scan(geo.touched, PV, geo_offsets(geo_ws, PV), d_total, scan_ws, st);
+    return launch_exclusive_scan(geo.touched, PV, geo_offsets(geo_ws, PV), d_counts, scan_ws, st);
 }
 
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
-                         uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* sort_ws,
-                         uint2* ranges, hipStream_t st) {
+                         uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, hipStream_t st) {
     const int64_t PV = (int64_t)rp.P * rp.V;
     Geo geo = carve_geo(geo_ws, PV);
     const int ntiles = rp.gx * rp.gy;
@@ -482,13 +517,13 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
     hipLaunchKernelGGL(zero_u32_kernel, dim3(nblk(2 * nt, 256)), dim3(256), 0, st, (uint32_t*)ranges, 2 * nt);
     if (n_rendered > 0) {
         hipLaunchKernelGGL(emit_keys_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.gx, ntiles, geo,
-                           geo_offsets(geo_ws, PV), keys, vals);
+                           geo_offsets(geo_ws, PV), dmin, dbits, keys, vals);
         int tile_bits = 0;
         while ((int64_t(1) << tile_bits) < nt) tile_bits++;
-        int rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, 32 + tile_bits, sort_ws, st);
+        int rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, dbits + tile_bits, sort_ws, st);
         if (rc) return rc;
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
-                           keys, d_total, ranges);
+                           keys, d_total, dbits, ranges);
     }
     if (rp.tile == 16)
         hipLaunchKernelGGL(blend_quad_kernel, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);

@@ -72,12 +72,14 @@ __host__ __device__ inline bool tile_may_contribute(float2 xy, float4 co, float 
 }
 
 size_t raster_geo_bytes(int64_t PV);
-// Preprocess of all V views + scan of tiles touched; *d_total (device) =
-// instances to render. scan_ws sized for V*P.
-int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_total, hipStream_t st);
+// Preprocess of all V views + scan of tiles touched. d_counts (device, 3 words):
+// [0] instances to render, [1] / [2] min / max depth bits of the visible
+// records. scan_ws sized for V*P.
+int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_counts, hipStream_t st);
 // Keys, sort, ranges and blend for n_rendered instances.
+// Keys (tile << dbits | depth bits - dmin), sort, ranges and blend.
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
-                         uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t* sort_ws,
-                         uint2* ranges, hipStream_t st);
+                         uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, hipStream_t st);
 
 }  // namespace arapgs

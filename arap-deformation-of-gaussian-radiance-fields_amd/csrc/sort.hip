@@ -186,7 +186,8 @@ int launch_exclusive_scan(const uint32_t* in, int64_t n, uint32_t* out, uint32_t
                           hipStream_t st) {
     const int nb = (int)((n + SCAN_TILE - 1) / SCAN_TILE);
     if (nb == 0) {
-        HIP_RET(hipMemsetAsync(total, 0, sizeof(uint32_t), st));
+        hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, ws, 0, total);  // *total = 0
+        HIP_RET(hipGetLastError());
         return 0;
     }
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(SCAN_THREADS), 0, st, in, n, ws);
