@@ -41,3 +41,14 @@ def test_version_string_without_gpu():
     lib = ctypes.CDLL(arapgs.LIB_PATH)
     lib.arapgs_version.restype = ctypes.c_char_p
     assert lib.arapgs_version().startswith(b"arapgs")
+
+
+def test_dropin_program_links_against_the_library():
+    """tests/dropin/dropin_check (the C++ caller using include/arapgs_dropin.hpp)
+    resolves libarapgs.so and libamdhip64 and starts: without arguments it prints
+    its usage and exits 2 before any HIP call."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_check")
+    assert os.path.exists(exe), "build the C++ caller first (make -C tests/dropin)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr, (r.returncode, r.stderr)
