@@ -47,6 +47,14 @@ class RenderDesc(C.Structure):
                 ("tile", C.c_int)]
 
 
+class ChainDesc(C.Structure):
+    _fields_ = [("n", C.c_int64), ("k", C.c_int), ("m", C.c_int), ("pos0", P), ("sh0", P), ("opacity", P),
+                ("ends_idx", P), ("ends_w", P), ("grid", P), ("sample_idx", P), ("sample_w", P), ("lpf", C.c_float),
+                ("n_views", C.c_int), ("width", C.c_int), ("height", C.c_int), ("sh_degree", C.c_int),
+                ("background", P), ("viewmatrix_h", P), ("projmatrix_h", P), ("cam_pos_h", P),
+                ("tan_fovx_h", P), ("tan_fovy_h", P), ("axis_padding", C.c_float), ("end_coeff", C.c_float)]
+
+
 EXPORTS = {
     "arapgs_create": ([C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_destroy": ([P], None),
@@ -91,6 +99,11 @@ EXPORTS = {
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
     "arapgs_render_views": ([P, P, C.c_int, P], C.c_int),
+    "arapgs_chain_create": ([P, C.POINTER(ChainDesc), C.POINTER(P)], C.c_int),
+    "arapgs_chain_destroy": ([P], None),
+    "arapgs_chain_forward": ([P, P, P, P, P, P, P, P], C.c_int),
+    "arapgs_chain_backward": ([P, P, P, P, P, P, P, P, P, P, P], C.c_int),
+    "arapgs_chain_state": ([P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)], C.c_int),
 }
 
 
@@ -526,3 +539,87 @@ def frame_buffers(state: dict, k: int, axis_padding=1e-3, end_coeff=2.0) -> Fram
                         ptr(g("node_src_w")), int(g("samples").shape[0]) if g("samples") is not None else 0,
                         ptr(g("samples")), ptr(g("sample_idx")), ptr(g("sample_w")), ptr(g("sample_static")),
                         ptr(g("sample_feat")), axis_padding, end_coeff)
+
+
+class Chain:
+    """arapgs_chain: the differentiable deform -> density-sample -> render chain.
+
+    forward(table, rot0, scale0) -> (images [V x (3,H,W)], feat (S,48), dens (S,))
+    backward(table, rot0, scale0, dimages, dfeat, ddens) -> (dtable (M,16), drot0 (N,4), dscale0 (N,3))
+    cams: dicts with device tensors view, proj, campos and floats tx, ty (tan of the half FoVs).
+    """
+
+    def __init__(self, ctx: Context, m, pos0, sh0, opacity, ends_idx, ends_w, cams, W, H, bg, grid=None,
+                 sample_idx=None, sample_w=None, lpf=0.0, sh_degree=3, axis_padding=1e-3, end_coeff=2.0):
+        t = ctx._torch
+        for name, x, dts in (("pos0", pos0, (t.float32,)), ("sh0", sh0, (t.float32,)),
+                             ("opacity", opacity, (t.float32,)), ("ends_w", ends_w, (t.float32,)),
+                             ("ends_idx", ends_idx, (t.int32, t.uint32)), ("bg", bg, (t.float32,)),
+                             ("sample_w", sample_w, (t.float32,)), ("sample_idx", sample_idx, (t.int32, t.uint32))):
+            if x is not None and x.dtype not in dts:
+                raise ArapgsError(f"Chain: {name} must be {dts}, got {x.dtype}")
+        self.ctx = ctx
+        self.N, self.M = pos0.shape[0], m
+        self.K = ends_w.numel() // (6 * self.N)
+        self.V, self.W, self.H = len(cams), W, H
+        self.S = 0 if grid is None else 64 * grid.n_valid
+        self.grid = grid
+        self._keep = [pos0, sh0, opacity, ends_idx, ends_w, bg, sample_idx, sample_w, cams]
+        V = len(cams)
+        self._vh = (P * V)(*[ptr(cm["view"]) for cm in cams])
+        self._ph = (P * V)(*[ptr(cm["proj"]) for cm in cams])
+        self._ch = (P * V)(*[ptr(cm["campos"]) for cm in cams])
+        self._tx = (C.c_float * V)(*[cm["tx"] for cm in cams])
+        self._ty = (C.c_float * V)(*[cm["ty"] for cm in cams])
+        d = ChainDesc(self.N, self.K, m, ptr(pos0), ptr(sh0), ptr(opacity), ptr(ends_idx), ptr(ends_w),
+                      grid.h if grid is not None else None, ptr(sample_idx), ptr(sample_w), lpf, V, W, H,
+                      sh_degree, ptr(bg), C.cast(self._vh, P), C.cast(self._ph, P), C.cast(self._ch, P),
+                      C.cast(self._tx, P), C.cast(self._ty, P), axis_padding, end_coeff)
+        h = P()
+        ctx._check(lib().arapgs_chain_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().arapgs_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, table, rot0, scale0, images=None, feat=None, dens=None):
+        t = self.ctx._torch
+        dev = rot0.device
+        images = images or [t.empty((3, self.H, self.W), dtype=t.float32, device=dev) for _ in range(self.V)]
+        if self.S > 0:
+            feat = t.empty((self.S, 48), dtype=t.float32, device=dev) if feat is None else feat
+            dens = t.empty(self.S, dtype=t.float32, device=dev) if dens is None else dens
+        ih = (P * self.V)(*[ptr(i) for i in images])
+        self.ctx._check(lib().arapgs_chain_forward(self.ctx.h, self.h, ptr(table), ptr(rot0), ptr(scale0),
+                                                   C.cast(ih, P), ptr(feat), ptr(dens)))
+        return images, feat, dens
+
+    def backward(self, table, rot0, scale0, dimages=None, dfeat=None, ddens=None):
+        t = self.ctx._torch
+        dev = rot0.device
+        dtable = t.empty((self.M, 16), dtype=t.float32, device=dev)
+        drot0 = t.empty((self.N, 4), dtype=t.float32, device=dev)
+        dscale0 = t.empty((self.N, 3), dtype=t.float32, device=dev)
+        dih = None
+        if dimages is not None:
+            dih = (P * self.V)(*[ptr(i.contiguous()) for i in dimages])
+            self._dkeep = dimages
+        self.ctx._check(lib().arapgs_chain_backward(self.ctx.h, self.h, ptr(table), ptr(rot0), ptr(scale0),
+                                                    C.cast(dih, P) if dih is not None else None, ptr(dfeat),
+                                                    ptr(ddens), ptr(dtable), ptr(drot0), ptr(dscale0)))
+        return dtable, drot0, dscale0
+
+    def state(self):
+        """Device pointers of the deformed (pos, rot, scale, sh, samples) as torch views."""
+        t = self.ctx._torch
+        ps = [P() for _ in range(5)]
+        lib().arapgs_chain_state(self.h, *[C.byref(p) for p in ps])
+        return ps

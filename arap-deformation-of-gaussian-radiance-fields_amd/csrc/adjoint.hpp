@@ -1,0 +1,43 @@
+// Launchers of the chain's reverse-mode kernels (adjoint.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace arapgs {
+
+int launch_zero_f32(float* p, int64_t n, hipStream_t st);
+
+// Density backward over the grid's valid cells at the given (deformed) sample
+// positions: gsamp (S*3) written, gdens (N*57: d mu 3, d Q 6, d sh 48)
+// accumulated (zeroed by the caller). dfeat / ddens may be null.
+int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+                            const float* pos, const float* rot, const float* scale, const float* opacity,
+                            const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
+                            float* gdens, hipStream_t st);
+
+// Per-Gaussian adjoint from (mu', q', s', sh') to the blended end points (gends,
+// N*18), q0 (drot0, N*4) and s0 (dscale0, N*3). gr: N*58 renderer gradients;
+// gd: N*57 density gradients or null.
+int launch_six_point_backward(int64_t N, int K, const NodeXf* table, const uint32_t* ends_idx, const float* ends_w,
+                              const float* ends, const float* rot0, const float* scale0, const float* sh0,
+                              const float* rot1, const float* scale1, const float* gr, const float* gd, float lpf,
+                              float pad, float coeff, const ShRotTables* Tsh, float* gends, float* drot0,
+                              float* dscale0, hipStream_t st);
+
+// Node -> (end point | sample) x slot inverse index: entries (n_entries u32,
+// sorted by node, stable) and per-node [begin, end) ranges. ws holds
+// inverse_index_words(n_entries) words.
+size_t inverse_index_words(int64_t n_entries);
+int build_inverse_index(int64_t n_end_entries, int64_t n_entries, int M, const uint32_t* ends_idx,
+                        const uint32_t* sample_idx, uint32_t* ws, uint32_t* entries, uint2* rng, hipStream_t st);
+
+// d R_j, d t_j of every node (dtable, NodeXf layout, g and pad zero).
+int launch_node_gather(int M, int K, int64_t n_end_entries, const uint2* rng, const uint32_t* entries,
+                       const NodeXf* table, const float* ends_w, const float* ends0, const float* gends,
+                       const float* sample_w, const float* samples0, const float* gsamp, NodeXf* dtable,
+                       hipStream_t st);
+
+}  // namespace arapgs

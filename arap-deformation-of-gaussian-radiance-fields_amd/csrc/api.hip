@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/arapgs.h"
+#include "adjoint.hpp"
 #include "common.hpp"
 #include "grid.hpp"
 #include "raster.hpp"
@@ -52,6 +53,7 @@ struct arapgs_ctx {
     int last_dbits = 32;
     RasterView* r_views = nullptr;    // device copy of the views of the current render
     RasterView* r_views_h = nullptr;  // pinned host staging for it
+    RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -526,6 +528,7 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
         hv[v].focal_x = (float)e->width / (2.0f * e->tan_fovx);
         hv[v].focal_y = (float)e->height / (2.0f * e->tan_fovy);
         hv[v].out = e->out_color; hv[v].surface = e->surface; hv[v].radii = e->radii; hv[v].rects = e->rects;
+        hv[v].dout = nullptr;
     }
     // the staging copy completes before this call returns (the stream is
     // synchronised for the instance count below), so the pinned buffer is free again
@@ -563,6 +566,7 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
                                               vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->stream),
                       "render");
     if (rc) return rc;
+    c->last_rp = rp;
     c->last_dmin = dmin;
     c->last_dbits = dbits;
     c->last_rendered = n;
@@ -873,6 +877,284 @@ int arapgs_profile_read(arapgs_ctx* c, float* ms_h, int cap) {
     }
     c->prof_used = 0;
     return n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- adjoint chain
+struct arapgs_chain {
+    int device = 0;
+    int64_t N = 0, S = 0, nv = 0;
+    int K = 0, M = 0, V = 0, W = 0, H = 0, D = 3;
+    arapgs_chain_desc d{};
+    std::vector<const float*> view, proj, campos;
+    std::vector<float> tfx, tfy;
+    const int* g_prefix = nullptr;
+    const int* g_list = nullptr;
+    const int* g_valid = nullptr;
+    const float* samples0 = nullptr;
+    // deformed state and its inputs
+    float *ends0 = nullptr, *ends = nullptr, *pos = nullptr, *rot = nullptr, *scale = nullptr, *sh = nullptr;
+    float* samples = nullptr;
+    float* img = nullptr;  // V*3*H*W render scratch of the backward
+    // gradients
+    float *grec = nullptr, *gout = nullptr, *gdens = nullptr, *gends = nullptr, *gsamp = nullptr;
+    // node -> (point, slot) inverse index
+    uint32_t* entries = nullptr;
+    uint2* rng = nullptr;
+    int64_t n_end_entries = 0, n_entries = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int chain_alloc(arapgs_ctx* c, arapgs_chain* ch, T** p, size_t count) {
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1));
+    if (e != hipSuccess) { *p = nullptr; return hip_fail(c, e, "chain allocation"); }
+    ch->allocs.push_back((void*)*p);
+    return 0;
+}
+
+void chain_free(arapgs_chain* ch) {
+    if (!ch) return;
+    (void)hipSetDevice(ch->device);
+    for (void* p : ch->allocs) (void)hipFree(p);
+    delete ch;
+}
+
+// Deformed state of the chain from (table, q0, s0): end points, copies of the
+// rest attributes, the fused blend + six-point update and the sample blend.
+int chain_deform(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* table, const float* rot0,
+                 const float* scale0) {
+    const arapgs_chain_desc& d = ch->d;
+    hipStream_t st = c->stream;
+    int rc = check_launch(c, launch_end_points(d.pos0, rot0, scale0, ch->N, d.axis_padding, d.end_coeff, ch->ends0, st),
+                          "chain end points");
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(ch->ends, ch->ends0, sizeof(float) * 18 * ch->N, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ch->pos, d.pos0, sizeof(float) * 3 * ch->N, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ch->rot, rot0, sizeof(float) * 4 * ch->N, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ch->scale, scale0, sizeof(float) * 3 * ch->N, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ch->sh, d.sh0, sizeof(float) * 48 * ch->N, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(c, e, "chain copies");
+    rc = check_launch(c, launch_six_point(ch->N, ch->K, (const NodeXf*)table, ch->M, d.ends_idx, d.ends_w, ch->ends,
+                                          scale0, nullptr, d.axis_padding, d.end_coeff, ch->pos, ch->rot, ch->scale,
+                                          ch->sh, c->shrot_dev, st),
+                      "chain six-point");
+    if (rc) return rc;
+    if (ch->S > 0)
+        rc = check_launch(c, launch_blend_points(ch->samples0, ch->S, d.sample_idx, d.sample_w, ch->K,
+                                                 (const NodeXf*)table, ch->M, nullptr, ch->samples, st),
+                          "chain sample blend");
+    return rc;
+}
+
+int chain_render(arapgs_ctx* c, arapgs_chain* ch, float* const* outs) {
+    std::vector<arapgs_render_desc> rd((size_t)ch->V);
+    for (int v = 0; v < ch->V; v++) {
+        arapgs_render_desc& r = rd[v];
+        r = arapgs_render_desc{};
+        r.P = (int)ch->N; r.D = ch->D; r.M = 16; r.background = ch->d.background;
+        r.width = ch->W; r.height = ch->H;
+        r.means3D = ch->pos; r.shs = ch->sh; r.opacities = ch->d.opacity; r.scales = ch->scale;
+        r.scale_modifier = 1.0f; r.rotations = ch->rot;
+        r.viewmatrix = ch->view[v]; r.projmatrix = ch->proj[v]; r.cam_pos = ch->campos[v];
+        r.tan_fovx = ch->tfx[v]; r.tan_fovy = ch->tfy[v];
+        r.out_color = outs[v];
+        r.ortho_scale = 1.0f; r.tile = 16;
+    }
+    return render_views(c, rd.data(), ch->V, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int arapgs_chain_create(arapgs_ctx* c, const arapgs_chain_desc* d, arapgs_chain** out) {
+    CHECK_CTX(c);
+    if (!d || !out) return fail(c, -1, "chain_create: bad arguments");
+    *out = nullptr;
+    if (d->n < 1 || d->k < 1 || d->k > KNN_MAX_T || d->m < 1 || !d->pos0 || !d->sh0 || !d->opacity || !d->ends_idx ||
+        !d->ends_w || !d->background)
+        return fail(c, -1, "chain_create: bad arguments");
+    if (d->n > 0x7fffffff) return fail(c, -1, "chain_create: too many Gaussians");
+    if (d->n_views < 1 || d->n_views > RASTER_MAX_VIEWS || d->width < 1 || d->height < 1 || d->sh_degree < 0 ||
+        d->sh_degree > 3 || !d->viewmatrix_h || !d->projmatrix_h || !d->cam_pos_h || !d->tan_fovx_h ||
+        !d->tan_fovy_h)
+        return fail(c, -1, "chain_create: bad camera arguments");
+    if (d->grid && (!d->sample_idx || !d->sample_w)) return fail(c, -1, "chain_create: a grid needs sample ids/weights");
+    if ((uintptr_t)d->sh0 % 16) return fail(c, -1, "chain_create: sh0 must be 16-byte aligned");
+    (void)hipSetDevice(c->device);
+    arapgs_chain* ch = new arapgs_chain();
+    ch->device = c->device;
+    ch->d = *d;
+    ch->N = d->n; ch->K = d->k; ch->M = d->m; ch->V = d->n_views; ch->W = d->width; ch->H = d->height;
+    ch->D = d->sh_degree;
+    for (int v = 0; v < ch->V; v++) {
+        if (!d->viewmatrix_h[v] || !d->projmatrix_h[v] || !d->cam_pos_h[v]) {
+            chain_free(ch);
+            return fail(c, -1, "chain_create: view %d: bad camera pointers", v);
+        }
+        ch->view.push_back(d->viewmatrix_h[v]);
+        ch->proj.push_back(d->projmatrix_h[v]);
+        ch->campos.push_back(d->cam_pos_h[v]);
+        ch->tfx.push_back(d->tan_fovx_h[v]);
+        ch->tfy.push_back(d->tan_fovy_h[v]);
+    }
+    ch->d.viewmatrix_h = nullptr; ch->d.projmatrix_h = nullptr; ch->d.cam_pos_h = nullptr;
+    ch->d.tan_fovx_h = nullptr; ch->d.tan_fovy_h = nullptr;
+    if (d->grid) {
+        ch->nv = d->grid->n_valid;
+        ch->S = 64 * ch->nv;
+        ch->g_prefix = d->grid->prefix; ch->g_list = d->grid->list; ch->g_valid = d->grid->valid;
+        ch->samples0 = d->grid->samples;
+    }
+    ch->n_end_entries = 6 * ch->N * ch->K;
+    ch->n_entries = ch->n_end_entries + ch->S * ch->K;
+    if (ch->n_entries > 0xffffffffll) {
+        chain_free(ch);
+        return fail(c, -1, "chain_create: too many (point, neighbour) entries");
+    }
+    const size_t N = (size_t)ch->N, S = (size_t)ch->S, V = (size_t)ch->V;
+    int rc = 0;
+    rc = rc ? rc : chain_alloc(c, ch, &ch->ends0, 18 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->ends, 18 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->pos, 3 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->rot, 4 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->scale, 3 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->sh, 48 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->samples, 3 * S);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->img, V * 3 * (size_t)ch->W * ch->H);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->grec, V * N * 8);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->gout, 58 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->gdens, 57 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->gends, 18 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->gsamp, 3 * S);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->entries, (size_t)ch->n_entries);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->rng, (size_t)ch->M);
+    uint32_t* ws = nullptr;
+    if (!rc) {
+        hipError_t e = hipMalloc((void**)&ws, sizeof(uint32_t) * inverse_index_words(ch->n_entries));
+        if (e != hipSuccess) rc = hip_fail(c, e, "chain allocation");
+    }
+    if (!rc)
+        rc = check_launch(c, build_inverse_index(ch->n_end_entries, ch->n_entries, ch->M, d->ends_idx,
+                                                 d->sample_idx, ws, ch->entries, ch->rng, c->stream),
+                          "chain inverse index");
+    if (ws) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        (void)hipFree(ws);
+        if (!rc && e != hipSuccess) rc = hip_fail(c, e, "chain inverse index");
+    }
+    if (rc) {
+        chain_free(ch);
+        return rc;
+    }
+    *out = ch;
+    return 0;
+}
+
+void arapgs_chain_destroy(arapgs_chain* ch) { chain_free(ch); }
+
+int arapgs_chain_forward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* table, const float* rot0,
+                         const float* scale0, float* const* images_h, float* feat, float* dens) {
+    CHECK_CTX(c);
+    if (!ch || !table || !rot0 || !scale0 || !images_h) return fail(c, -1, "chain_forward: bad arguments");
+    if (ch->device != c->device) return fail(c, -1, "chain_forward: chain of another device");
+    if ((uintptr_t)rot0 % 16) return fail(c, -1, "chain_forward: rot0 must be 16-byte aligned");
+    for (int v = 0; v < ch->V; v++)
+        if (!images_h[v]) return fail(c, -1, "chain_forward: image %d is NULL", v);
+    if (ch->S > 0 && (!feat || !dens)) return fail(c, -1, "chain_forward: the grid needs feat and dens outputs");
+    (void)hipSetDevice(c->device);
+    int rc = chain_deform(c, ch, table, rot0, scale0);
+    if (rc) return rc;
+    rc = chain_render(c, ch, images_h);
+    if (rc) return rc;
+    if (ch->S > 0)
+        rc = check_launch(c, launch_density(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples, ch->pos,
+                                            ch->rot, ch->scale, ch->d.opacity, ch->sh, ch->d.lpf, feat, dens,
+                                            c->stream),
+                          "chain density");
+    return rc;
+}
+
+int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* table, const float* rot0,
+                          const float* scale0, const float* const* dimages_h, const float* dfeat,
+                          const float* ddens, arapgs_node_xf* dtable, float* drot0, float* dscale0) {
+    CHECK_CTX(c);
+    if (!ch || !table || !rot0 || !scale0 || !dtable || !drot0 || !dscale0)
+        return fail(c, -1, "chain_backward: bad arguments");
+    if (ch->device != c->device) return fail(c, -1, "chain_backward: chain of another device");
+    if ((uintptr_t)rot0 % 16) return fail(c, -1, "chain_backward: rot0 must be 16-byte aligned");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = c->stream;
+    int rc = chain_deform(c, ch, table, rot0, scale0);
+    if (rc) return rc;
+    // renderer: recompute the binning, then the blend and preprocess adjoints
+    std::vector<float*> outs((size_t)ch->V);
+    const size_t img = 3 * (size_t)ch->W * ch->H;
+    for (int v = 0; v < ch->V; v++) outs[v] = ch->img + v * img;
+    rc = chain_render(c, ch, outs.data());
+    if (rc) return rc;
+    rc = check_launch(c, launch_zero_f32(ch->grec, (int64_t)ch->V * ch->N * 8, st), "chain backward");
+    if (rc) return rc;
+    if (dimages_h) {
+        RasterView* hv = c->r_views_h;
+        for (int v = 0; v < ch->V; v++) hv[v].dout = dimages_h[v];
+        hipError_t e = hipMemcpyAsync(c->r_views, hv, sizeof(RasterView) * ch->V, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);  // the pinned staging is reused by the next render
+        if (e != hipSuccess) return hip_fail(c, e, "chain backward views");
+        RasterParams rp = c->last_rp;
+        rp.views = c->r_views;
+        rc = check_launch(c, raster_blend_backward(rp, c->r_geo, (const uint32_t*)((char*)c->r_bin +
+                                                                                     std::max<size_t>(c->last_rendered, 1) * 16),
+                                                   (const uint2*)c->r_ranges, ch->grec, st),
+                          "chain blend backward");
+        if (rc) return rc;
+    }
+    {
+        RasterParams rp = c->last_rp;
+        rp.views = c->r_views;
+        rc = check_launch(c, raster_preprocess_backward(rp, c->r_geo, ch->grec, ch->gout, st),
+                          "chain preprocess backward");
+        if (rc) return rc;
+    }
+    // density
+    const float* gd = nullptr;
+    if (ch->S > 0) {
+        rc = check_launch(c, launch_zero_f32(ch->gdens, 57 * ch->N, st), "chain backward");
+        if (!rc && (dfeat || ddens))
+            rc = check_launch(c, launch_density_backward(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples,
+                                                         ch->pos, ch->rot, ch->scale, ch->d.opacity, ch->sh,
+                                                         ch->d.lpf, dfeat, ddens, ch->gsamp, ch->gdens, st),
+                              "chain density backward");
+        else if (!rc)
+            rc = check_launch(c, launch_zero_f32(ch->gsamp, 3 * ch->S, st), "chain backward");
+        if (rc) return rc;
+        gd = ch->gdens;
+    }
+    rc = check_launch(c, launch_six_point_backward(ch->N, ch->K, (const NodeXf*)table, ch->d.ends_idx, ch->d.ends_w,
+                                                   ch->ends, rot0, scale0, ch->d.sh0, ch->rot, ch->scale, ch->gout,
+                                                   gd, ch->d.lpf, ch->d.axis_padding, ch->d.end_coeff, c->shrot_dev,
+                                                   ch->gends, drot0, dscale0, st),
+                      "chain six-point backward");
+    if (rc) return rc;
+    return check_launch(c, launch_node_gather(ch->M, ch->K, ch->n_end_entries, ch->rng, ch->entries,
+                                              (const NodeXf*)table, ch->d.ends_w, ch->ends0, ch->gends,
+                                              ch->d.sample_w, ch->samples0, ch->gsamp, (NodeXf*)dtable, st),
+                        "chain node gather");
+}
+
+int arapgs_chain_state(const arapgs_chain* ch, const float** pos, const float** rot, const float** scale,
+                       const float** sh, const float** samples) {
+    if (!ch) return -1;
+    if (pos) *pos = ch->pos;
+    if (rot) *rot = ch->rot;
+    if (scale) *scale = ch->scale;
+    if (sh) *sh = ch->sh;
+    if (samples) *samples = ch->samples;
+    return 0;
 }
 
 }  // extern "C"

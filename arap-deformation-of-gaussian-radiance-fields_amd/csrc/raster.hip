@@ -460,6 +460,376 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
     }
 }
 
+// ----------------------------------------------------------------- backward
+// Gradient of one pixel's composite C = sum_i c_i a_i T_i + T_final bg with
+// respect to the Gaussians it composited, walking them back to front. With
+// S_i = sum_{j>i} c_j a_j T_j + T_final bg (the colour behind i):
+//   dL/dc_i = a_i T_i dL/dC,   dL/da_i = sum_ch dL/dC_ch (c_i,ch T_i - S_i,ch / (1 - a_i)),
+// T_i = T_{i+1} / (1 - a_i). a_i = min(0.99, o_i G_i) with G_i = exp(power_i)
+// and power_i = -(A dx^2 + C dy^2)/2 - B dx dy (d = mean - pixel): the clamp
+// has zero slope. Opacities are not differentiated.
+//
+// One workgroup per tile, one thread per pixel. The tile first re-runs the
+// forward pass (same float operations, so the same contributors, final
+// transmittance and stopping point per pixel) to find each pixel's last
+// contributor; the reverse pass then restages the tile's batches in LDS from
+// the deepest one up. For every Gaussian each wave sums its 64 pixels'
+// gradients with lane shuffles into a per-wave LDS slot; after the batch one
+// thread per Gaussian adds the wave slots and issues one set of 8 global
+// atomic adds per (tile, Gaussian).
+template <int TILE>
+__global__ void __launch_bounds__(TILE* TILE) blend_backward_kernel(RasterParams rp, Geo geo,
+                                                                   const uint32_t* __restrict__ vals,
+                                                                   const uint2* __restrict__ ranges,
+                                                                   float* __restrict__ grec) {
+    constexpr int NT = TILE * TILE;
+    constexpr int NW = NT / 64;
+    static_assert(NT % 64 == 0, "whole waves per tile");
+    __shared__ float2 s_xy[NT];
+    __shared__ float4 s_co[NT];
+    __shared__ float4 s_rgb[NT];
+    __shared__ uint32_t s_id[NT];
+    __shared__ float s_g[NW][NT][8];
+    __shared__ int s_last;
+    const int ntl = rp.gx * rp.gy;
+    const int view = blockIdx.x / ntl;
+    const int tile = blockIdx.x - view * ntl;
+    const float* __restrict__ dout = rp.views[view].dout;
+    const int tx = tile % rp.gx, ty = tile / rp.gx;
+    const int px = tx * TILE + (threadIdx.x % TILE), py = ty * TILE + (threadIdx.x / TILE);
+    const bool inside = px < rp.W && py < rp.H;
+    const float fxp = (float)px, fyp = (float)py;
+    const uint2 rg = ranges[blockIdx.x];
+    const int n = (int)(rg.y - rg.x);
+    const int wid = threadIdx.x >> 6, wl = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_last = -1;
+
+    // forward pass: final transmittance and last contributor of this pixel
+    float T = 1.0f;
+    int last = -1;
+    bool done = !inside;
+    for (int b0 = 0; b0 < n; b0 += NT) {
+        if (__syncthreads_count(done) == NT) break;
+        const int i = b0 + (int)threadIdx.x;
+        if (i < n) {
+            const uint32_t g = vals[rg.x + i];
+            s_xy[threadIdx.x] = geo.xy[g];
+            s_co[threadIdx.x] = geo.conic[g];
+        }
+        __syncthreads();
+        const int m = min(NT, n - b0);
+        for (int j = 0; j < m && !done; j++) {
+            const float2 xy = s_xy[j];
+            const float4 co = s_co[j];
+            const float dx = xy.x - fxp, dy = xy.y - fyp;
+            const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+            if (power > 0.0f) continue;
+            const float alpha = fminf(0.99f, co.w * expf(power));
+            if (alpha < 1.0f / 255.0f) continue;
+            const float test_T = T * (1.0f - alpha);
+            if (test_T < 0.0001f) { done = true; break; }
+            T = test_T;
+            last = b0 + j;
+        }
+    }
+    __syncthreads();
+    if (last >= 0) atomicMax(&s_last, last);
+    __syncthreads();
+    const int tile_last = s_last;
+    if (tile_last < 0) return;  // uniform: nothing in this tile contributed
+
+    float dC[3] = {0.0f, 0.0f, 0.0f};
+    if (inside && dout) {
+        const int64_t HW = (int64_t)rp.W * rp.H, o = (int64_t)py * rp.W + px;
+        dC[0] = dout[o]; dC[1] = dout[HW + o]; dC[2] = dout[2 * HW + o];
+    }
+    float S0 = T * rp.bg[0], S1 = T * rp.bg[1], S2 = T * rp.bg[2];
+    float Tc = T;
+    for (int b0 = (tile_last / NT) * NT; b0 >= 0; b0 -= NT) {
+        __syncthreads();
+        const int i = b0 + (int)threadIdx.x;
+        if (i < n) {
+            const uint32_t g = vals[rg.x + i];
+            s_id[threadIdx.x] = g;
+            s_xy[threadIdx.x] = geo.xy[g];
+            s_co[threadIdx.x] = geo.conic[g];
+            s_rgb[threadIdx.x] = geo.rgb[g];
+        }
+        __syncthreads();
+        const int jtop = min(NT - 1, tile_last - b0);
+        for (int j = jtop; j >= 0; j--) {
+            float gv[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            bool act = false;
+            if (b0 + j <= last) {
+                const float2 xy = s_xy[j];
+                const float4 co = s_co[j];
+                const float dx = xy.x - fxp, dy = xy.y - fyp;
+                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                if (power <= 0.0f) {
+                    const float G = expf(power);
+                    const float alpha = fminf(0.99f, co.w * G);
+                    if (alpha >= 1.0f / 255.0f) {
+                        act = true;
+                        const float4 c = s_rgb[j];
+                        const float om = 1.0f - alpha;
+                        const float Ti = Tc / om;
+                        const float wgt = alpha * Ti;
+                        gv[5] = wgt * dC[0];
+                        gv[6] = wgt * dC[1];
+                        gv[7] = wgt * dC[2];
+                        const float da = dC[0] * (c.x * Ti - S0 / om) + dC[1] * (c.y * Ti - S1 / om) +
+                                         dC[2] * (c.z * Ti - S2 / om);
+                        S0 += c.x * wgt;
+                        S1 += c.y * wgt;
+                        S2 += c.z * wgt;
+                        Tc = Ti;
+                        if (co.w * G < 0.99f) {
+                            const float dpow = da * co.w * G;
+                            gv[0] = dpow * (-co.x * dx - co.y * dy);
+                            gv[1] = dpow * (-co.z * dy - co.y * dx);
+                            gv[2] = -0.5f * dx * dx * dpow;
+                            gv[3] = -dx * dy * dpow;
+                            gv[4] = -0.5f * dy * dy * dpow;
+                        }
+                    }
+                }
+            }
+            if (__ballot(act)) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) gv[k] = wave_sum(gv[k]);
+            }
+            if (wl == 0) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) s_g[wid][j][k] = gv[k];
+            }
+        }
+        __syncthreads();
+        const int t = (int)threadIdx.x;
+        if (t <= jtop) {
+            float acc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                float a = 0.0f;
+#pragma unroll
+                for (int w = 0; w < NW; w++) a += s_g[w][t][k];
+                acc[k] = a;
+            }
+            float* dst = grec + 8 * (int64_t)s_id[t];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (acc[k] != 0.0f) unsafeAtomicAdd(dst + k, acc[k]);
+        }
+    }
+}
+
+// Backward of project_one for one (view, Gaussian) record whose blend
+// gradient is gr = d(x, y, conic a, b, c, r, g, b). The forward quantities
+// are recomputed with the forward's float operations; the clamps of the
+// forward (view-space tangent limits, colour >= 0) pass no gradient where
+// they are active.
+__device__ __forceinline__ void project_backward(const RasterParams& rp, const RasterView& vw, int g,
+                                                 const float* __restrict__ sh, const float gr[8], float dp[3],
+                                                 float dq[4], float ds[3], float dsh[48]) {
+    const float px = rp.means[3 * g], py = rp.means[3 * g + 1], pz = rp.means[3 * g + 2];
+    const float* V = vw.view;
+    const float* Pm = vw.proj;
+    const float vx = V[0] * px + V[4] * py + V[8] * pz + V[12];
+    const float vy = V[1] * px + V[5] * py + V[9] * pz + V[13];
+    const float vz = V[2] * px + V[6] * py + V[10] * pz + V[14];
+    const float hx = Pm[0] * px + Pm[4] * py + Pm[8] * pz + Pm[12];
+    const float hy = Pm[1] * px + Pm[5] * py + Pm[9] * pz + Pm[13];
+    const float hw = Pm[3] * px + Pm[7] * py + Pm[11] * pz + Pm[15];
+    const float pw = 1.0f / (hw + 0.0000001f);
+    const float q[4] = {rp.rots[4 * g], rp.rots[4 * g + 1], rp.rots[4 * g + 2], rp.rots[4 * g + 3]};
+    const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+    float R[3][3];
+    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+    R[0][1] = 2.0f * (qx * qy - qw * qz);
+    R[0][2] = 2.0f * (qx * qz + qw * qy);
+    R[1][0] = 2.0f * (qx * qy + qw * qz);
+    R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+    R[1][2] = 2.0f * (qy * qz - qw * qx);
+    R[2][0] = 2.0f * (qx * qz - qw * qy);
+    R[2][1] = 2.0f * (qy * qz + qw * qx);
+    R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+    float sc[3], s2[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        sc[k] = rp.scale_mod * rp.scales[3 * g + k];
+        s2[k] = sc[k] * sc[k];
+    }
+    float S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            S[i][j] = R[i][0] * s2[0] * R[j][0] + R[i][1] * s2[1] * R[j][1] + R[i][2] * s2[2] * R[j][2];
+    const float limx = 1.3f * vw.tanfovx, limy = 1.3f * vw.tanfovy;
+    const float txtz = vx / vz, tytz = vy / vz;
+    const float tx = fminf(limx, fmaxf(-limx, txtz)) * vz;
+    const float ty = fminf(limy, fmaxf(-limy, tytz)) * vz;
+    const float fx = vw.focal_x, fy = vw.focal_y;
+    const float j00 = fx / vz, j02 = -(fx * tx) / (vz * vz);
+    const float j11 = fy / vz, j12 = -(fy * ty) / (vz * vz);
+    float T[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        T[0][c] = j00 * V[c * 4 + 0] + j02 * V[c * 4 + 2];
+        T[1][c] = j11 * V[c * 4 + 1] + j12 * V[c * 4 + 2];
+    }
+    float ST0[3], ST1[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        ST0[i] = S[i][0] * T[0][0] + S[i][1] * T[0][1] + S[i][2] * T[0][2];
+        ST1[i] = S[i][0] * T[1][0] + S[i][1] * T[1][1] + S[i][2] * T[1][2];
+    }
+    const float ca = T[0][0] * ST0[0] + T[0][1] * ST0[1] + T[0][2] * ST0[2] + 0.3f;
+    const float cb = T[0][0] * ST1[0] + T[0][1] * ST1[1] + T[0][2] * ST1[2];
+    const float cc = T[1][0] * ST1[0] + T[1][1] * ST1[1] + T[1][2] * ST1[2] + 0.3f;
+    const float det = ca * cc - cb * cb;
+
+    // colour
+    const float ux0 = px - vw.campos[0], uy0 = py - vw.campos[1], uz0 = pz - vw.campos[2];
+    const float dn = __builtin_sqrtf(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
+    const float ux = ux0 / dn, uy = uy0 / dn, uz = uz0 / dn;
+    float b[16], db[16][3];
+    sh_basis_and_grad(ux, uy, uz, b, db);
+    const int nb = (rp.D + 1) * (rp.D + 1);
+    float dcol[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        float r = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (k < nb) r += b[k] * sh[3 * k + ch];
+        dcol[ch] = (r + 0.5f > 0.0f) ? gr[5 + ch] : 0.0f;
+    }
+    float ddir[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (k < nb) {
+            float w = 0.0f;
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) {
+                dsh[3 * k + ch] += b[k] * dcol[ch];
+                w += sh[3 * k + ch] * dcol[ch];
+            }
+            ddir[0] += w * db[k][0];
+            ddir[1] += w * db[k][1];
+            ddir[2] += w * db[k][2];
+        }
+    }
+    const float dd = ux * ddir[0] + uy * ddir[1] + uz * ddir[2];
+    dp[0] += (ddir[0] - ux * dd) / dn;
+    dp[1] += (ddir[1] - uy * dd) / dn;
+    dp[2] += (ddir[2] - uz * dd) / dn;
+
+    // conic (a, b, c) = (cc, -cb, ca) / det
+    const float gA = gr[2], gB = gr[3], gC = gr[4];
+    const float id = 1.0f / det, id2 = id * id;
+    const float dca = -gA * cc * cc * id2 + gB * cb * cc * id2 + gC * (id - ca * cc * id2);
+    const float dcb = 2.0f * gA * cb * cc * id2 - gB * (id + 2.0f * cb * cb * id2) + 2.0f * gC * cb * ca * id2;
+    const float dcc = gA * (id - cc * ca * id2) + gB * cb * ca * id2 - gC * ca * ca * id2;
+    // cov2D = T Sigma T^T
+    float dS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            dS[i][j] = dca * T[0][i] * T[0][j] + dcb * T[0][i] * T[1][j] + dcc * T[1][i] * T[1][j];
+    float dj00 = 0.0f, dj02 = 0.0f, dj11 = 0.0f, dj12 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float dT0 = 2.0f * dca * ST0[c] + dcb * ST1[c];
+        const float dT1 = 2.0f * dcc * ST1[c] + dcb * ST0[c];
+        dj00 += dT0 * V[c * 4 + 0];
+        dj02 += dT0 * V[c * 4 + 2];
+        dj11 += dT1 * V[c * 4 + 1];
+        dj12 += dT1 * V[c * 4 + 2];
+    }
+    const float iz = 1.0f / vz, iz2 = iz * iz, iz3 = iz2 * iz;
+    float dvx = 0.0f, dvy = 0.0f;
+    float dvz = -dj00 * fx * iz2 - dj11 * fy * iz2 + 2.0f * dj02 * fx * tx * iz3 + 2.0f * dj12 * fy * ty * iz3;
+    const float dtx = -dj02 * fx * iz2, dty = -dj12 * fy * iz2;
+    if (txtz < -limx) dvz -= dtx * limx;
+    else if (txtz > limx) dvz += dtx * limx;
+    else dvx += dtx;
+    if (tytz < -limy) dvz -= dty * limy;
+    else if (tytz > limy) dvz += dty * limy;
+    else dvy += dty;
+    // Sigma = R diag(s^2) R^T
+    float dR[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += (dS[a][j] + dS[j][a]) * R[j][bb];
+            dR[a][bb] = s2[bb] * acc;
+        }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float dD = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dD += dS[i][j] * R[i][k] * R[j][k];
+        ds[k] += dD * 2.0f * sc[k] * rp.scale_mod;
+    }
+    quat_to_mat_backward(q, dR, dq);
+    // image position ((ndc + 1) * size - 1) / 2 through the projection
+    const float dndx = gr[0] * 0.5f * (float)rp.W, dndy = gr[1] * 0.5f * (float)rp.H;
+    const float dhx = dndx * pw, dhy = dndy * pw;
+    const float dpw = dndx * hx + dndy * hy;
+    const float dhw = -dpw * pw * pw;
+    dp[0] += dhx * Pm[0] + dhy * Pm[1] + dhw * Pm[3] + dvx * V[0] + dvy * V[1] + dvz * V[2];
+    dp[1] += dhx * Pm[4] + dhy * Pm[5] + dhw * Pm[7] + dvx * V[4] + dvy * V[5] + dvz * V[6];
+    dp[2] += dhx * Pm[8] + dhy * Pm[9] + dhw * Pm[11] + dvx * V[8] + dvy * V[9] + dvz * V[10];
+}
+
+// One thread per Gaussian over all views (its SH read once).
+__global__ void __launch_bounds__(256) preprocess_backward_kernel(RasterParams rp, Geo geo,
+                                                                  const float* __restrict__ grec,
+                                                                  float* __restrict__ gout) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rp.P) return;
+    float dp[3] = {0.0f, 0.0f, 0.0f}, dq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ds[3] = {0.0f, 0.0f, 0.0f};
+    float dsh[48];
+#pragma unroll
+    for (int k = 0; k < 48; k++) dsh[k] = 0.0f;
+    float sh[48];
+    const float4* s4 = reinterpret_cast<const float4*>(rp.shs + 48 * (int64_t)g);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 v = s4[i];
+        sh[4 * i] = v.x; sh[4 * i + 1] = v.y; sh[4 * i + 2] = v.z; sh[4 * i + 3] = v.w;
+    }
+    for (int v = 0; v < rp.V; v++) {
+        const int64_t gv = (int64_t)v * rp.P + g;
+        if (geo.touched[gv] == 0) continue;
+        float gr[8];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            gr[k] = grec[8 * gv + k];
+            any = any || gr[k] != 0.0f;
+        }
+        if (!any) continue;
+        const RasterView vw = rp.views[v];
+        project_backward(rp, vw, g, sh, gr, dp, dq, ds, dsh);
+    }
+    float* o = gout + 58 * (int64_t)g;
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[k] = dp[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[3 + k] = dq[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[7 + k] = ds[k];
+#pragma unroll
+    for (int k = 0; k < 48; k++) o[10 + k] = dsh[k];
+}
+
 __global__ void copy_count_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) { *dst = *src; }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -531,6 +901,34 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
         hipLaunchKernelGGL(blend_kernel<8>, dim3((unsigned)nt), dim3(64), 0, st, rp, geo, vals, ranges);
     else
         return -1;
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int raster_blend_backward(const RasterParams& rp, void* geo_ws, const uint32_t* vals, const uint2* ranges,
+                          float* grec, hipStream_t st) {
+    const int64_t PV = (int64_t)rp.P * rp.V;
+    Geo geo = carve_geo(geo_ws, PV);
+    const int64_t nt = (int64_t)rp.gx * rp.gy * rp.V;
+    if (nt == 0) return 0;
+    if (rp.tile == 16)
+        hipLaunchKernelGGL(blend_backward_kernel<16>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges,
+                           grec);
+    else if (rp.tile == 8)
+        hipLaunchKernelGGL(blend_backward_kernel<8>, dim3((unsigned)nt), dim3(64), 0, st, rp, geo, vals, ranges,
+                           grec);
+    else
+        return -1;
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int raster_preprocess_backward(const RasterParams& rp, void* geo_ws, const float* grec, float* gout,
+                               hipStream_t st) {
+    if (rp.P <= 0) return 0;
+    const int64_t PV = (int64_t)rp.P * rp.V;
+    Geo geo = carve_geo(geo_ws, PV);
+    hipLaunchKernelGGL(preprocess_backward_kernel, dim3(nblk(rp.P, 256)), dim3(256), 0, st, rp, geo, grec, gout);
     HIP_RET(hipGetLastError());
     return 0;
 }

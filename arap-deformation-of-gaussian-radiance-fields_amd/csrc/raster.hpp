@@ -17,6 +17,7 @@ struct RasterView {
     int* surface;          // optional, P
     int* radii;            // optional, P
     int* rects;            // optional, 2*P
+    const float* dout;     // backward only: dL/d out (3*H*W planar CHW)
 };
 constexpr int RASTER_MAX_VIEWS = 16;
 
@@ -81,5 +82,17 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, hipStream_t st);
+
+// Backward of the blend for every view of the last binning (geo, vals,
+// ranges as left by raster_bin_and_blend): per record gv the gradient of
+// (x, y, conic a, b, c, r, g, b) accumulates into grec[8*gv..] (zeroed by the
+// caller). Each tile re-runs its forward pass to recover every pixel's final
+// transmittance and last contributor, then walks its list back to front.
+int raster_blend_backward(const RasterParams& rp, void* geo_ws, const uint32_t* vals, const uint2* ranges,
+                          float* grec, hipStream_t st);
+// Backward of the preprocess: per Gaussian, over all views, from grec to the
+// gradients of means (3), rotations (4, as given, unnormalised), scales (3)
+// and SH (48), written (not accumulated) to gout[58*g..] in that order.
+int raster_preprocess_backward(const RasterParams& rp, void* geo_ws, const float* grec, float* gout, hipStream_t st);
 
 }  // namespace arapgs

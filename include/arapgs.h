@@ -320,6 +320,68 @@ int arapgs_aims_twist(arapgs_ctx* ctx, const float* node_pos, int m, const int* 
 int arapgs_profile_enable(arapgs_ctx* ctx, int on);
 int arapgs_profile_read(arapgs_ctx* ctx, float* ms_h, int cap);
 
+/* ------------------------------------------------------------ adjoint chain */
+/* The differentiable deform -> density-sample -> render chain from rest
+ * state: end points of every Gaussian (GetEndPoints, GaussianView.cpp:
+ * 4643-4668) from its rest rotation q0 and scale s0, the K-node blend of end
+ * points and samples (predict_mesh / predict_samples, Deform.hpp:230-268), the
+ * six-point update (UpdateAsSixPointsWithdrawBad, GaussianView.cpp:3081-3166,
+ * with q_old = q0 and the backup scales = s0), the renders of n_views cameras
+ * (Rasterizer::forward as in arapgs_render, 16-pixel tiles, no fast culling)
+ * and the density sample of the deformed Gaussians at the blended samples of a
+ * grid (arapgs_density_sample's formula over the grid's fixed cell lists).
+ * Inputs differentiated: the node transforms (R, t of an arapgs_node_xf table;
+ * g is a constant), q0 and s0. arapgs_chain_backward takes the gradients of a
+ * scalar loss with respect to the outputs and returns those of the inputs; it
+ * recomputes the forward (including every tile's blend state) instead of
+ * keeping it. Replaces the Jacobians the reference obtains by finite
+ * differences of its CUDA forward for the density-preserving step
+ * (L1loss3d / backward3d_grid / update3d, GaussianView.cpp:1695-1733). */
+typedef struct arapgs_chain arapgs_chain;
+typedef struct arapgs_chain_desc {
+    int64_t n;                    /* Gaussians */
+    int k, m;                     /* neighbours per point, nodes */
+    const float* pos0;            /* N*3 rest means */
+    const float* sh0;             /* N*48 rest SH */
+    const float* opacity;         /* N activated opacities */
+    const uint32_t* ends_idx;     /* N*6*k blend ids of the end points */
+    const float* ends_w;          /* N*6*k */
+    const arapgs_grid* grid;      /* density grid (NULL: no density term); S = 64 * valid cells */
+    const uint32_t* sample_idx;   /* S*k blend ids of the grid's samples */
+    const float* sample_w;        /* S*k */
+    float lpf;                    /* density low-pass variance */
+    int n_views, width, height, sh_degree;
+    const float* background;      /* device f32[3] */
+    const float* const* viewmatrix_h;  /* host arrays of n_views device pointers */
+    const float* const* projmatrix_h;
+    const float* const* cam_pos_h;
+    const float* tan_fovx_h;      /* host arrays of n_views */
+    const float* tan_fovy_h;
+    float axis_padding, end_coeff;  /* 1e-3, 2 */
+} arapgs_chain_desc;
+
+/* Validates the description and builds the node -> (end point | sample, slot)
+ * inverse index used by the backward (a stable sort by node id). The device
+ * arrays of the description must outlive the chain. */
+int arapgs_chain_create(arapgs_ctx* ctx, const arapgs_chain_desc* desc_h, arapgs_chain** out);
+void arapgs_chain_destroy(arapgs_chain* ch);
+/* images_h: host array of n_views device pointers (f32[3*H*W] each), feat
+ * f32[S*48] and dens f32[S] (both NULL when the chain has no grid). */
+int arapgs_chain_forward(arapgs_ctx* ctx, arapgs_chain* ch, const arapgs_node_xf* table, const float* rot0,
+                         const float* scale0, float* const* images_h, float* feat, float* dens);
+/* dimages_h: host array of n_views device pointers (dL/d image) or NULL;
+ * dfeat / ddens may be NULL. Outputs (all written): dtable (R and t
+ * gradients in the table layout, g and pad 0), drot0 f32[N*4], dscale0
+ * f32[N*3]. Float atomics accumulate over tiles and cells, so the last bits
+ * vary from run to run. */
+int arapgs_chain_backward(arapgs_ctx* ctx, arapgs_chain* ch, const arapgs_node_xf* table, const float* rot0,
+                          const float* scale0, const float* const* dimages_h, const float* dfeat,
+                          const float* ddens, arapgs_node_xf* dtable, float* drot0, float* dscale0);
+/* Device views of the deformed state of the last forward or backward call:
+ * pos f32[N*3], rot f32[N*4], scale f32[N*3], sh f32[N*48], samples f32[S*3]. */
+int arapgs_chain_state(const arapgs_chain* ch, const float** pos, const float** rot, const float** scale,
+                       const float** sh, const float** samples);
+
 /* ------------------------------------------------------------ test hooks */
 /* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the
  * GaussianView.cpp:3134-3156 sign wrapper); R9 f32[n*9] row-major. */

@@ -82,6 +82,10 @@ def _declare(L):
     L.orc_grid_lists.restype = C.c_long
     L.orc_density_sample.argtypes = [C.c_long, P, P, P, P, P, P, P, P, P, C.c_float, P, P]
     L.orc_raster_preprocess.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P, P]
+    L.orc_chain_jvp.argtypes = [C.c_void_p] + [P] * 19
+    L.orc_chain_jvp.restype = None
+    L.orc_chain_forward.argtypes = [C.c_void_p] + [P] * 7
+    L.orc_chain_forward.restype = None
     L.orc_raster_bin.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P]
     L.orc_raster_bin.restype = C.c_long
     L.orc_raster_blend.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P, P]
@@ -337,3 +341,68 @@ def raster(means, shs, opac, scales, rots, view, proj, campos, tanfovx, tanfovy,
     lib().orc_raster_blend(C.byref(a), _p(vals), _p(ranges), _p(xy), _p(conic), _p(rgb), _p(img), _p(surface))
     return dict(image=img, keys=keys, vals=vals, ranges=ranges, depth=depth, radius=radius, xy=xy, conic=conic,
                 rgb=rgb, rect=rect, touched=touched, surface=surface)
+
+
+# --------------------------------------------------------------------------- adjoint chain
+class ChainArgs(C.Structure):
+    _fields_ = [("N", C.c_long), ("K", C.c_int), ("M", C.c_int), ("pos0", P), ("sh0", P), ("opac", P),
+                ("ends_idx", P), ("ends_w", P), ("node_g", P), ("nv", C.c_long), ("valid", P), ("prefix", P),
+                ("list", P), ("samples0", P), ("sample_idx", P), ("sample_w", P), ("lpf", C.c_float),
+                ("V", C.c_int), ("W", C.c_int), ("H", C.c_int), ("D", C.c_int), ("bg", P), ("view", P),
+                ("proj", P), ("campos", P), ("tanfov", P), ("pad", C.c_float), ("coeff", C.c_float)]
+
+
+class Chain:
+    """FP64 forward and forward-mode directional derivative of the deform ->
+    density-sample -> render chain (oracle/chain.cpp). cams: list of
+    (view16, proj16, campos3, tanfovx, tanfovy); grid: (valid, prefix, list,
+    samples0) or None."""
+
+    def __init__(self, pos0, sh0, opac, ends_idx, ends_w, node_g, cams, W, H, bg, D=3, grid=None,
+                 sample_idx=None, sample_w=None, lpf=0.0, pad=1e-3, coeff=2.0):
+        self.N = pos0.shape[0]
+        self.M = node_g.shape[0]
+        self.K = np.asarray(ends_idx).size // (6 * self.N)
+        self.V, self.W, self.H = len(cams), W, H
+        k = dict(pos0=_f32(pos0), sh0=_f32(sh0), opac=_f32(opac), ends_idx=_u32(ends_idx), ends_w=_f32(ends_w),
+                 node_g=_f32(node_g), bg=_f32(bg),
+                 view=_f32(np.stack([np.asarray(c[0], np.float32).reshape(16) for c in cams])),
+                 proj=_f32(np.stack([np.asarray(c[1], np.float32).reshape(16) for c in cams])),
+                 campos=_f32(np.stack([np.asarray(c[2], np.float32).reshape(3) for c in cams])),
+                 tanfov=_f32(np.array([[c[3], c[4]] for c in cams], np.float32)))
+        nv = 0
+        if grid is not None:
+            valid, prefix, lst, samples0 = grid
+            k.update(valid=_i32(valid), prefix=_i32(prefix), list=_i32(lst), samples0=_f32(samples0),
+                     sample_idx=_u32(sample_idx), sample_w=_f32(sample_w))
+            nv = k["valid"].shape[0]
+        else:
+            k.update(valid=None, prefix=None, list=None, samples0=None, sample_idx=None, sample_w=None)
+        self.nv, self.S = nv, 64 * nv
+        self._k = k
+        self.args = ChainArgs(self.N, self.K, self.M, _p(k["pos0"]), _p(k["sh0"]), _p(k["opac"]),
+                              _p(k["ends_idx"]), _p(k["ends_w"]), _p(k["node_g"]), nv, _p(k["valid"]),
+                              _p(k["prefix"]), _p(k["list"]), _p(k["samples0"]), _p(k["sample_idx"]),
+                              _p(k["sample_w"]), lpf, self.V, W, H, D, _p(k["bg"]), _p(k["view"]), _p(k["proj"]),
+                              _p(k["campos"]), _p(k["tanfov"]), pad, coeff)
+
+    def jvp(self, R, t, q0, s0, dR=None, dt=None, dq0=None, ds0=None):
+        L = lib()
+        V, H, W, S, N = self.V, self.H, self.W, self.S, self.N
+        out = dict(images=np.zeros((V, 3, H, W)), feat=np.zeros((S, 48)), dens=np.zeros(S),
+                   d_images=np.zeros((V, 3, H, W)), d_feat=np.zeros((S, 48)), d_dens=np.zeros(S),
+                   pos=np.zeros((N, 3)), rot=np.zeros((N, 4)), scale=np.zeros((N, 3)), sh=np.zeros((N, 48)),
+                   samples=np.zeros((S, 3)))
+        d = [None if x is None else _f64(x) for x in (dR, dt, dq0, ds0)]
+        L.orc_chain_jvp(C.byref(self.args), _p(_f64(R)), _p(_f64(t)), _p(_f64(q0)), _p(_f64(s0)),
+                        *[_p(x) for x in d], _p(out["images"]), _p(out["feat"]), _p(out["dens"]),
+                        _p(out["d_images"]), _p(out["d_feat"]), _p(out["d_dens"]), _p(out["pos"]), _p(out["rot"]),
+                        _p(out["scale"]), _p(out["sh"]), _p(out["samples"]))
+        return out
+
+    def forward(self, R, t, q0, s0):
+        V, H, W, S = self.V, self.H, self.W, self.S
+        images, feat, dens = np.zeros((V, 3, H, W)), np.zeros((S, 48)), np.zeros(S)
+        lib().orc_chain_forward(C.byref(self.args), _p(_f64(R)), _p(_f64(t)), _p(_f64(q0)), _p(_f64(s0)),
+                                _p(images), _p(feat), _p(dens))
+        return images, feat, dens
