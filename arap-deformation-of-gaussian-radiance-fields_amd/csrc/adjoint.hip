@@ -84,9 +84,9 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
 // Threads own Gaussians of the cell's list (64 at a time, parameters and SH
 // in registers) and walk the cell's 64 samples, whose positions and output
 // gradients sit in LDS (every lane reads the same sample: broadcast). The
-// sample gradient of each step is a wave sum. gdens per Gaussian: d mu (3),
-// d Q (q00, q11, q22, then q01, q02, q12 each counted for both symmetric
-// entries), d c (48).
+// sample gradient of each step is a wave sum. gdens is component-major
+// (component k of Gaussian g at k * gstride + g): d mu (3), d Q (q00, q11,
+// q22, then q01, q02, q12 each counted for both symmetric entries), d c (48).
 __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const int* __restrict__ valid,
                                                               const int* __restrict__ prefix,
                                                               const int* __restrict__ list,
@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
                                                               const float* __restrict__ sh, float lpf,
                                                               const float* __restrict__ dfeat,
                                                               const float* __restrict__ ddens,
-                                                              float* __restrict__ gsamp, float* __restrict__ gdens) {
+                                                              float* __restrict__ gsamp, float* __restrict__ gdens,
+                                                              int64_t gstride) {
 #pragma clang fp contract(fast)
     __shared__ float s_x[64][3];
     __shared__ float s_df[64][49];
@@ -170,13 +171,15 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
             }
         }
         if (has) {
-            float* dst = gdens + 57 * (int64_t)g;
+            // component-major: the wave's atomics of one component land on
+            // neighbouring addresses (list entries are sorted by Gaussian)
+            float* dst = gdens + g;
 #pragma unroll
-            for (int k = 0; k < 3; k++) if (gmu[k] != 0.0f) unsafeAtomicAdd(dst + k, gmu[k]);
+            for (int k = 0; k < 3; k++) if (gmu[k] != 0.0f) unsafeAtomicAdd(dst + k * gstride, gmu[k]);
 #pragma unroll
-            for (int k = 0; k < 6; k++) if (gQ[k] != 0.0f) unsafeAtomicAdd(dst + 3 + k, gQ[k]);
+            for (int k = 0; k < 6; k++) if (gQ[k] != 0.0f) unsafeAtomicAdd(dst + (3 + k) * gstride, gQ[k]);
 #pragma unroll
-            for (int k = 0; k < 48; k++) if (gc[k] != 0.0f) unsafeAtomicAdd(dst + 9 + k, gc[k]);
+            for (int k = 0; k < 48; k++) if (gc[k] != 0.0f) unsafeAtomicAdd(dst + (9 + k) * gstride, gc[k]);
         }
     }
     __syncthreads();
@@ -186,8 +189,8 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
 
 // One thread per Gaussian: the adjoint of everything between the rest
 // attributes and (mu', q', s', c'). gr: the renderer's per-Gaussian gradient
-// (d mu' 3, d q' 4, d s' 3, d c' 48); gd: the density's (d mu' 3, d Q 6, d c' 48)
-// or null. Writes d e' (18) and d q0, d s0.
+// (d mu' 3, d q' 4, d s' 3, d c' 48); gd: the density's, component-major
+// (57 x N: d mu' 3, d Q 6, d c' 48) or null. Writes d e' (18) and d q0, d s0.
 __global__ void __launch_bounds__(128) six_point_backward_kernel(
     int64_t N, int K, const NodeXf* __restrict__ table, const uint32_t* __restrict__ ends_idx,
     const float* __restrict__ ends_w, const float* __restrict__ ends, const float* __restrict__ rot0,
@@ -207,7 +210,9 @@ __global__ void __launch_bounds__(128) six_point_backward_kernel(
     const float q1[4] = {rot1[4 * g], rot1[4 * g + 1], rot1[4 * g + 2], rot1[4 * g + 3]};
     const float s1[3] = {scale1[3 * g], scale1[3 * g + 1], scale1[3 * g + 2]};
     if (gd) {
-        const float* d = gd + 57 * g;
+        float d[57];
+#pragma unroll
+        for (int k = 0; k < 57; k++) d[k] = gd[k * N + g];
 #pragma unroll
         for (int k = 0; k < 3; k++) dmu[k] += d[k];
 #pragma unroll
@@ -505,10 +510,10 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st) {
 int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
-                            float* gdens, hipStream_t st) {
+                            float* gdens, int64_t n_gauss, hipStream_t st) {
     if (nv <= 0) return 0;
     hipLaunchKernelGGL(density_backward_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples,
-                       pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens);
+                       pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens, n_gauss);
     HIP_RET(hipGetLastError());
     return 0;
 }

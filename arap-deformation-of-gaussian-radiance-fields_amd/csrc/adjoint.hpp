@@ -11,16 +11,16 @@ namespace arapgs {
 int launch_zero_f32(float* p, int64_t n, hipStream_t st);
 
 // Density backward over the grid's valid cells at the given (deformed) sample
-// positions: gsamp (S*3) written, gdens (N*57: d mu 3, d Q 6, d sh 48)
-// accumulated (zeroed by the caller). dfeat / ddens may be null.
+// positions: gsamp (S*3) written, gdens (57 x N component-major: d mu 3,
+// d Q 6, d sh 48) accumulated (zeroed by the caller). dfeat / ddens may be null.
 int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
-                            float* gdens, hipStream_t st);
+                            float* gdens, int64_t n_gauss, hipStream_t st);
 
 // Per-Gaussian adjoint from (mu', q', s', sh') to the blended end points (gends,
 // N*18), q0 (drot0, N*4) and s0 (dscale0, N*3). gr: N*58 renderer gradients;
-// gd: N*57 density gradients or null.
+// gd: density gradients (57 x N component-major) or null.
 int launch_six_point_backward(int64_t N, int K, const NodeXf* table, const uint32_t* ends_idx, const float* ends_w,
                               const float* ends, const float* rot0, const float* scale0, const float* sh0,
                               const float* rot1, const float* scale1, const float* gr, const float* gd, float lpf,

@@ -1127,7 +1127,8 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
         if (!rc && (dfeat || ddens))
             rc = check_launch(c, launch_density_backward(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples,
                                                          ch->pos, ch->rot, ch->scale, ch->d.opacity, ch->sh,
-                                                         ch->d.lpf, dfeat, ddens, ch->gsamp, ch->gdens, st),
+                                                         ch->d.lpf, dfeat, ddens, ch->gsamp, ch->gdens, ch->N,
+                                                         st),
                               "chain density backward");
         else if (!rc)
             rc = check_launch(c, launch_zero_f32(ch->gsamp, 3 * ch->S, st), "chain backward");

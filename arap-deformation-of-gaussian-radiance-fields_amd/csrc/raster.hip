@@ -614,10 +614,12 @@ __global__ void __launch_bounds__(TILE* TILE) blend_backward_kernel(RasterParams
                 for (int w = 0; w < NW; w++) a += s_g[w][t][k];
                 acc[k] = a;
             }
-            float* dst = grec + 8 * (int64_t)s_id[t];
+            // component-major: component k of record gv at k * V * P + gv
+            const int64_t PV = (int64_t)rp.V * rp.P;
+            float* dst = grec + s_id[t];
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                if (acc[k] != 0.0f) unsafeAtomicAdd(dst + k, acc[k]);
+                if (acc[k] != 0.0f) unsafeAtomicAdd(dst + k * PV, acc[k]);
         }
     }
 }
@@ -812,7 +814,7 @@ __global__ void __launch_bounds__(256) preprocess_backward_kernel(RasterParams r
         bool any = false;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            gr[k] = grec[8 * gv + k];
+            gr[k] = grec[k * (int64_t)rp.V * rp.P + gv];
             any = any || gr[k] != 0.0f;
         }
         if (!any) continue;

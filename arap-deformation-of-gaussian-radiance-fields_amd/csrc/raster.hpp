@@ -85,8 +85,8 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
 
 // Backward of the blend for every view of the last binning (geo, vals,
 // ranges as left by raster_bin_and_blend): per record gv the gradient of
-// (x, y, conic a, b, c, r, g, b) accumulates into grec[8*gv..] (zeroed by the
-// caller). Each tile re-runs its forward pass to recover every pixel's final
+// (x, y, conic a, b, c, r, g, b) accumulates into grec, component-major
+// (component k at k * V * P + gv; zeroed by the caller). Each tile re-runs its forward pass to recover every pixel's final
 // transmittance and last contributor, then walks its list back to front.
 int raster_blend_backward(const RasterParams& rp, void* geo_ws, const uint32_t* vals, const uint2* ranges,
                           float* grec, hipStream_t st);

@@ -13,7 +13,7 @@ from . import scenes
 
 
 def prepare(ctx, sc, device="cuda:0", log=None):
-    """Returns dict(state, fb, solver_args, active, delta_per_frame, cams, bg, opac, grid, K)."""
+    """Returns dict(state, solver_args, active, cams, bg, opac, grid, K, lpf)."""
     import torch
     g = sc.gaussians
     K = sc.knn_k
@@ -54,7 +54,8 @@ def prepare(ctx, sc, device="cuda:0", log=None):
     cams = [dict(view=T(c.view), proj=T(c.proj), campos=T(c.campos), tx=c.tanfovx, ty=c.tanfovy, W=c.W, H=c.H)
             for c in sc.cameras]
     bg = T(np.array([0.8, 0.8, 0.8], np.float32))
-    return dict(state=state, solver_args=solver_args, active=active, cams=cams, bg=bg, opac=opac, grid=grid, K=K)
+    return dict(state=state, solver_args=solver_args, active=active, cams=cams, bg=bg, opac=opac, grid=grid, K=K,
+                lpf=lpf)
 
 
 def clone_state(state: dict) -> dict:

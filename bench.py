@@ -232,6 +232,81 @@ def cpu_baseline(args):
                         f"(the reference renders on the GPU); sample update not included"))
 
 
+def adjoint_leg(args):
+    """The differentiation workload: 500k Gaussians (capsule scene), 1024
+    nodes, K=8, 64^3 grid, one 800x800 view. Targets are the chain's outputs
+    under a seeded node perturbation (rotations N(0, 5 deg), translations
+    N(0, 0.02)); one step = chain forward at the rest table, the L2 loss
+    gradient of images + density features + densities, chain backward."""
+    import numpy as np
+    import torch
+    load_package()
+    from arapgs_amd import arapgs, scenes, workload
+    ctx = arapgs.Context(0, stream=torch.cuda.Stream())
+    sc = scenes.capsule_scene(args.seed, n=args.adjoint_n, n_nodes=args.adjoint_nodes, W=args.width, H=args.height,
+                              n_views=1)
+    with torch.cuda.stream(ctx.stream):
+        W = workload.prepare(ctx, sc, log=log)
+        st = W["state"]
+        M = st["node_pos"].shape[0]
+        chain = arapgs.Chain(ctx, M, st["pos"], st["sh"], W["opac"], st["ends_idx"], st["ends_w"], W["cams"],
+                             args.width, args.height, W["bg"], grid=W["grid"], sample_idx=st["sample_idx"],
+                             sample_w=st["sample_w"], lpf=W["lpf"])
+        rng = np.random.default_rng(args.seed + 1)
+        table0 = np.zeros((M, 16), np.float32)
+        table0[:, 0:3] = st["node_pos"].cpu().numpy()
+        table0[:, 3:12] = np.eye(3, dtype=np.float32).reshape(9)
+        pert = table0.copy()
+        for i in range(M):
+            ax = rng.normal(size=3)
+            ax /= np.linalg.norm(ax)
+            ang = np.radians(5.0) * rng.normal()
+            Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+            pert[i, 3:12] = (np.eye(3) + np.sin(ang) * Kx + (1 - np.cos(ang)) * Kx @ Kx).reshape(9)
+        pert[:, 12:15] = 0.02 * rng.normal(size=(M, 3))
+        table0, pert = torch.from_numpy(table0).cuda(), torch.from_numpy(pert).cuda()
+        rot0, scale0 = st["rot"], st["scale0"]
+        tgt = chain.forward(pert, rot0, scale0)
+        tgt = ([x.clone() for x in tgt[0]], tgt[1].clone(), tgt[2].clone())
+    torch.cuda.synchronize()
+    log(f"adjoint: {sc.gaussians.n} Gaussians, {M} nodes, {chain.S} samples")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd, bwd, tot = [], [], []
+
+    def step():
+        with torch.cuda.stream(ctx.stream):
+            ev[0].record(ctx.stream)
+            imgs, feat, dens = chain.forward(table0, rot0, scale0)
+            ev[1].record(ctx.stream)
+            dimgs = [2.0 * (a - b) for a, b in zip(imgs, tgt[0])]
+            grads = chain.backward(table0, rot0, scale0, dimages=dimgs, dfeat=2.0 * (feat - tgt[1]),
+                                   ddens=2.0 * (dens - tgt[2]))
+            ev[2].record(ctx.stream)
+            return grads
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.adjoint_steps):
+        a = time.perf_counter()
+        grads = step()
+        torch.cuda.synchronize()
+        tot.append((time.perf_counter() - a) * 1e3)
+        fwd.append(ev[0].elapsed_time(ev[1]))
+        bwd.append(ev[1].elapsed_time(ev[2]))
+    total = time.perf_counter() - t0
+    gnorm = float(torch.linalg.vector_norm(grads[0][:, 3:15]).item())
+    chain.close()
+    ctx.close()
+    return {"workload": f"capsules-{args.adjoint_n // 1000}k differentiation", "gaussians": args.adjoint_n,
+            "nodes": M, "samples": int(chain.S), "views": 1, "resolution": [args.width, args.height],
+            "steps": args.adjoint_steps, "steps_per_s": round(args.adjoint_steps / total, 3),
+            "step_p50_ms": round(pct(tot, 50), 3), "forward_p50_ms": round(pct(fwd, 50), 3),
+            "backward_p50_ms": round(pct(bwd, 50), 3), "node_grad_norm": gnorm,
+            "check": "vector-Jacobian products vs FP64 forward-mode oracle along 64 directions (tests/test_gpu_chain.py)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +322,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,fast,pipelined")
+    ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the differentiation leg (0: skip)")
+    ap.add_argument("--adjoint-n", type=int, default=500_000)
+    ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -319,6 +397,11 @@ def main():
                         "traffic": committed_traffic(args),
                         "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
                         "launches": int(len(kms))}
+    if args.adjoint_steps > 0:
+        log(f"timing the differentiation leg: {args.warmup} warmup + {args.adjoint_steps} steps")
+        line["adjoint"] = adjoint_leg(args)
+        log(f"  adjoint: {line['adjoint']['steps_per_s']} steps/s, forward p50 {line['adjoint']['forward_p50_ms']} ms, "
+            f"backward p50 {line['adjoint']['backward_p50_ms']} ms")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line), flush=True)
