@@ -108,7 +108,7 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     }
     // aggregates: farthest-point seeds over the free nodes' positions, then
     // nearest-seed assignment
-    const int n_agg = std::max(1, std::min(d.n_agg > 0 ? d.n_agg : 64, nF));
+    const int n_agg = std::max(1, std::min(d.n_agg > 0 ? d.n_agg : 128, nF));
     std::vector<int> seeds;
     {
         std::vector<float> dist(nF, 3.4e38f);

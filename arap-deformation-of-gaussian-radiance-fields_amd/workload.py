@@ -67,5 +67,5 @@ def drag_delta(sc, drag_frames: int):
     return np.float32(sc.edit["distance"] / drag_frames) * sc.edit["direction"].astype(np.float32)
 
 
-def make_solver(ctx, solver_args):
-    return arapgs.Solver(ctx, *solver_args)
+def make_solver(ctx, solver_args, **kw):
+    return arapgs.Solver(ctx, *solver_args, **kw)

@@ -77,7 +77,7 @@ def setup(args):
     log(f"scene: {sc.gaussians.n} Gaussians, {args.nodes} nodes, K={sc.knn_k}")
     with torch.cuda.stream(ctx.stream):
         W = workload.prepare(ctx, sc, log=log)
-        solver = workload.make_solver(ctx, W["solver_args"])
+        solver = workload.make_solver(ctx, W["solver_args"], n_aggregates=args.aggregates)
         log(f"solver: {solver.n_free} free nodes, {solver.n_unknowns} unknowns; "
             f"blocks {[len(b) for b in W['solver_args'][1]]}")
         state = W["state"]
@@ -323,6 +323,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain,fast,pipelined")
     ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the differentiation leg (0: skip)")
+    ap.add_argument("--aggregates", type=int, default=0, help="coarse aggregates of the CG preconditioner (0: 128)")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -94,7 +94,7 @@ typedef struct arapgs_solver_desc {
     float w_rot, w_reg, w_con;   /* DeformGraph weights (Deform.hpp:54-59): 1, 10, 100 */
     float weighting_factor;      /* GaussianView::_weighting_factor (GaussianView.hpp:459) */
     int constraints_on_center;   /* must be 0 (centre mode not supported) */
-    int n_aggregates;            /* coarse aggregates, 0 = 64 */
+    int n_aggregates;            /* coarse aggregates, 0 = 128 */
     int cg_max_iters;            /* 0 = 100 */
     float cg_tol;                /* relative residual, 0 = 1e-4 */
 } arapgs_solver_desc;

@@ -281,7 +281,7 @@ private:
 
 /* Settings of the linear solve inside each GN iteration (the reference solves
  * J^T J h = -J^T f exactly with Eigen; here it is preconditioned CG). Zero
- * selects the library defaults (100 iterations, relative residual 1e-4, 64
+ * selects the library defaults (100 iterations, relative residual 1e-4, 128
  * coarse aggregates). */
 struct SolverOptions {
     int cg_max_iters = 0;
