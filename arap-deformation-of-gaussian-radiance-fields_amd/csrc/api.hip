@@ -46,6 +46,8 @@ struct arapgs_ctx {
     size_t r_sort_cap = 0;
     void* r_ranges = nullptr;
     size_t r_ranges_cap = 0;
+    void* r_rec = nullptr;          // depth sort of the records
+    size_t r_rec_cap = 0;
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
@@ -157,7 +159,7 @@ void arapgs_destroy(arapgs_ctx* c) {
         (void)hipEventDestroy(pr.second);
     }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
-                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges,
+                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec,
                     (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views})
         if (p) (void)hipFree(p);
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
@@ -540,6 +542,7 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     if (!rc) rc = grow(c, &c->r_scan, &c->r_scan_cap, sizeof(uint32_t) * scan_workspace_words(PV));
     const int ntiles = rp.gx * rp.gy;
     if (!rc) rc = grow(c, &c->r_ranges, &c->r_ranges_cap, sizeof(uint2) * (size_t)ntiles * V);
+    if (!rc) rc = grow(c, &c->r_rec, &c->r_rec_cap, raster_record_bytes(PV));
     if (rc) return rc;
     rc = check_launch(c, raster_preprocess(rp, c->r_geo, (uint32_t*)c->r_scan, c->r_total, c->stream),
                       "render preprocess");
@@ -563,7 +566,8 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     uint32_t* vals = (uint32_t*)(b + nn * 16);
     uint32_t* vals_alt = (uint32_t*)(b + nn * 20);
     rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, dmin, dbits, keys, vals, keys_alt,
-                                              vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->stream),
+                                              vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->r_rec,
+                                              (uint32_t*)c->r_scan, c->stream),
                       "render");
     if (rc) return rc;
     c->last_rp = rp;
@@ -591,11 +595,14 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
     char* b = (char*)c->r_bin;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && keys_h && n > 0) {
+        // the sorted keys hold the tile id; the reference layout is tile << 32 | depth bits of the value's record
+        std::vector<uint32_t> v((size_t)n);
+        const int64_t PV = (int64_t)std::max(c->last_rp.P, 1) * c->last_rp.V;
+        std::vector<uint32_t> depth((size_t)PV);
         e = hipMemcpy(keys_h, b, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
-        // back to the reference key layout, tile << 32 | depth bits
-        const uint64_t mask = (c->last_dbits >= 64) ? ~0ull : ((1ull << c->last_dbits) - 1ull);
-        for (int64_t i = 0; e == hipSuccess && i < n; i++)
-            keys_h[i] = ((keys_h[i] >> c->last_dbits) << 32) | (uint64_t)((uint32_t)(keys_h[i] & mask) + c->last_dmin);
+        if (e == hipSuccess) e = hipMemcpy(v.data(), b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(depth.data(), c->r_geo, sizeof(uint32_t) * PV, hipMemcpyDeviceToHost);
+        for (int64_t i = 0; e == hipSuccess && i < n; i++) keys_h[i] = (keys_h[i] << 32) | depth[v[i]];
     }
     if (e == hipSuccess && vals_h && n > 0) e = hipMemcpy(vals_h, b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
     const int64_t nr = std::min<int64_t>(ranges_cap, c->last_tiles);

@@ -204,26 +204,47 @@ __global__ void depth_range_init_kernel(uint32_t* __restrict__ dminmax) {
     dminmax[1] = 0u;
 }
 
-// One thread per (view, Gaussian) record gv = v*P + g: one key per touched
-// tile, (v*ntiles + tile) << dbits | (depth bits - dmin); value = the record
-// index gv. dbits covers the frame's depth-bit range, so the significant key
-// bits are contiguous and the sort needs fewer passes; the order is the order
-// of (tile, depth bits), since subtracting dmin is monotone.
-__global__ void __launch_bounds__(256) emit_keys_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
-                                                        const uint32_t* __restrict__ offsets, uint32_t dmin,
-                                                        int dbits, uint64_t* __restrict__ keys,
-                                                        uint32_t* __restrict__ vals) {
+// Depth-first binning. The records are sorted once by depth (their key is the
+// depth bit pattern minus the frame's minimum, so the order is the float order;
+// the stable sort keeps equal depths in record order), then every record emits
+// its tiles in that order and the instances are stably sorted by tile alone:
+// each tile's list ends up ordered by (depth bits, record), exactly the order of
+// a stable sort of (tile << 32 | depth bits) keys emitted in record order. The
+// wide sort runs on the V*P records instead of the instances, and the instance
+// sort needs only the tile-id bits.
+__global__ void __launch_bounds__(256) record_keys_kernel(int64_t PV, Geo geo, uint32_t dmin,
+                                                          uint64_t* __restrict__ rkeys, uint32_t* __restrict__ rvals) {
     const int64_t gv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gv >= PV) return;
+    rkeys[gv] = geo.touched[gv] ? (uint64_t)(__float_as_uint(geo.depth[gv]) - dmin) : 0ull;
+    rvals[gv] = (uint32_t)gv;
+}
+
+__global__ void __launch_bounds__(256) gather_touched_kernel(int64_t PV, const uint32_t* __restrict__ rvals,
+                                                             const uint32_t* __restrict__ touched,
+                                                             uint32_t* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= PV) return;
+    out[r] = touched[rvals[r]];
+}
+
+// One thread per depth-ordered record r: its tiles' keys (view * ntiles + tile)
+// and values (the record index) from offsets[r] on.
+__global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
+                                                         const uint32_t* __restrict__ rvals,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= PV) return;
+    const uint32_t gv = rvals[r];
     if (geo.touched[gv] == 0) return;
-    const uint32_t base = (uint32_t)(gv / P) * (uint32_t)ntiles;
-    uint32_t o = offsets[gv];
-    const int4 r = geo.rect[gv];
-    const uint64_t db = (uint64_t)(__float_as_uint(geo.depth[gv]) - dmin);
-    for (int y = r.y; y < r.w; y++)
-        for (int x = r.x; x < r.z; x++) {
-            keys[o] = ((uint64_t)(base + (uint32_t)(y * gx + x)) << dbits) | db;
-            vals[o] = (uint32_t)gv;
+    const uint32_t base = (gv / (uint32_t)P) * (uint32_t)ntiles;
+    uint32_t o = offsets[r];
+    const int4 rc = geo.rect[gv];
+    for (int y = rc.y; y < rc.w; y++)
+        for (int x = rc.x; x < rc.z; x++) {
+            keys[o] = (uint64_t)(base + (uint32_t)(y * gx + x));
+            vals[o] = gv;
             o++;
         }
 }
@@ -879,23 +900,50 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
     return launch_exclusive_scan(geo.touched, PV, geo_offsets(geo_ws, PV), d_counts, scan_ws, st);
 }
 
+size_t raster_record_bytes(int64_t PV) {
+    return (size_t)PV * (8 + 8 + 4 + 4 + 4) + sizeof(uint32_t) * radix_workspace_words(PV) + 6 * 256;
+}
+
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
-                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, hipStream_t st) {
+                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
+                         hipStream_t st) {
     const int64_t PV = (int64_t)rp.P * rp.V;
     Geo geo = carve_geo(geo_ws, PV);
     const int ntiles = rp.gx * rp.gy;
     const int64_t nt = (int64_t)ntiles * rp.V;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(nblk(2 * nt, 256)), dim3(256), 0, st, (uint32_t*)ranges, 2 * nt);
     if (n_rendered > 0) {
-        hipLaunchKernelGGL(emit_keys_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.gx, ntiles, geo,
-                           geo_offsets(geo_ws, PV), dmin, dbits, keys, vals);
+        // records by depth
+        char* p = (char*)rec_ws;
+        auto take = [&](size_t bytes) {
+            char* r = p;
+            p += (bytes + 255) & ~(size_t)255;
+            return r;
+        };
+        uint64_t* rkeys = (uint64_t*)take(8 * (size_t)PV);
+        uint64_t* rkeys_alt = (uint64_t*)take(8 * (size_t)PV);
+        uint32_t* rvals = (uint32_t*)take(4 * (size_t)PV);
+        uint32_t* rvals_alt = (uint32_t*)take(4 * (size_t)PV);
+        uint32_t* rtouched = (uint32_t*)take(4 * (size_t)PV);
+        uint32_t* rsort_ws = (uint32_t*)take(sizeof(uint32_t) * radix_workspace_words(PV));
+        hipLaunchKernelGGL(record_keys_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, geo, dmin, rkeys, rvals);
+        int rc = launch_radix_sort(rkeys, rvals, rkeys_alt, rvals_alt, PV, dbits, rsort_ws, st);
+        if (rc) return rc;
+        // instance offsets in depth order, then the tiles of every record
+        hipLaunchKernelGGL(gather_touched_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rvals, geo.touched,
+                           rtouched);
+        uint32_t* offs = geo_offsets(geo_ws, PV);
+        rc = launch_exclusive_scan(rtouched, PV, offs, (uint32_t*)d_total, scan_ws, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(emit_tiles_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.gx, ntiles, geo,
+                           rvals, offs, keys, vals);
         int tile_bits = 0;
         while ((int64_t(1) << tile_bits) < nt) tile_bits++;
-        int rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, dbits + tile_bits, sort_ws, st);
+        rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, tile_bits, sort_ws, st);
         if (rc) return rc;
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
-                           keys, d_total, dbits, ranges);
+                           keys, d_total, 0, ranges);
     }
     if (rp.tile == 16)
         hipLaunchKernelGGL(blend_quad_kernel, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);

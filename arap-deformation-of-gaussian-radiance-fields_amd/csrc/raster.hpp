@@ -77,11 +77,16 @@ size_t raster_geo_bytes(int64_t PV);
 // [0] instances to render, [1] / [2] min / max depth bits of the visible
 // records. scan_ws sized for V*P.
 int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_counts, hipStream_t st);
-// Keys, sort, ranges and blend for n_rendered instances.
-// Keys (tile << dbits | depth bits - dmin), sort, ranges and blend.
+// Binning, ranges and blend for n_rendered instances: the V*P records are
+// sorted by depth bits - dmin (dbits wide), the records' tiles are emitted in
+// that order and the instances are stably sorted by tile id (keys = view *
+// ntiles + tile). rec_ws holds raster_record_bytes(V*P) bytes, scan_ws is the
+// preprocess's scan workspace (sized for V*P).
+size_t raster_record_bytes(int64_t PV);
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
-                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, hipStream_t st);
+                         uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
+                         hipStream_t st);
 
 // Backward of the blend for every view of the last binning (geo, vals,
 // ranges as left by raster_bin_and_blend): per record gv the gradient of
