@@ -99,6 +99,7 @@ EXPORTS = {
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
     "arapgs_render_views": ([P, P, C.c_int, P], C.c_int),
+    "arapgs_set_blend_unroll": ([P, C.c_int], C.c_int),
     "arapgs_chain_create": ([P, C.POINTER(ChainDesc), C.POINTER(P)], C.c_int),
     "arapgs_chain_destroy": ([P], None),
     "arapgs_chain_forward": ([P, P, P, P, P, P, P, P], C.c_int),
@@ -254,6 +255,10 @@ class Context:
         n = C.c_int(0)
         self._check(lib().arapgs_render(self.h, C.byref(d), C.byref(n)))
         return dict(image=out, num_rendered=n.value, surface=surface, radii=radii, rects=rects)
+
+    def set_blend_unroll(self, unroll):
+        """Gaussians per compositing step of the 16x16 blend (2, 4, 8); images do not change."""
+        self._check(lib().arapgs_set_blend_unroll(self.h, unroll))
 
     def render_views(self, means, shs, opac, scales, rots, cams, W, H, bg, outs, D=3, scale_mod=1.0,
                      fast_culling=True, tile=16):

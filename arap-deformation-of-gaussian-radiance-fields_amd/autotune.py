@@ -1,0 +1,72 @@
+"""Autotuner of the drag-frame path.
+
+The tunables change only speed, never the images or the solve's result beyond
+its tolerance:
+  tile          render tile edge: 16 (the 3DGS tile) or 8
+  blend_unroll  Gaussians per compositing step of the 16x16 blend: 2, 4, 8
+  aggregates    node aggregates of the CG preconditioner's coarse space
+  refresh       solves between rebuilds of the coarse operator
+  host_checks   CG iterations between host reads of the convergence flag
+
+`tune` runs a coordinate descent from the defaults: for each parameter in turn
+it times every candidate value (the others held at their current best) with a
+caller-supplied `measure(config) -> seconds per frame`, and keeps the fastest.
+Results are cached per workload key in a JSON file so a session tunes once.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16)
+CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(4, 8, 16),
+                  host_checks=(8, 16, 32))
+ORDER = ("aggregates", "refresh", "host_checks", "tile", "blend_unroll")
+
+
+def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):
+    """Coordinate descent over `candidates` (default CANDIDATES). Returns
+    (best_config, trials) where trials lists (config, seconds) in the order run."""
+    cands = dict(CANDIDATES if candidates is None else candidates)
+    if cache_path and key and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+        if key in cache:
+            if log:
+                log(f"autotune: cached configuration for {key}: {cache[key]}")
+            return dict(cache[key]), []
+    best = {k: DEFAULTS[k] for k in cands}
+    trials = []
+    memo = {}
+
+    def run(cfg):
+        k = tuple(sorted(cfg.items()))
+        if k not in memo:
+            memo[k] = float(measure(dict(cfg)))
+            trials.append((dict(cfg), memo[k]))
+            if log:
+                log(f"autotune: {cfg} -> {memo[k] * 1e3:.3f} ms/frame")
+        return memo[k]
+
+    best_t = run(best)
+    for _ in range(rounds):
+        for name in order:
+            if name not in cands:
+                continue
+            for v in cands[name]:
+                if v == best[name]:
+                    continue
+                cfg = dict(best)
+                cfg[name] = v
+                t = run(cfg)
+                if t < best_t:
+                    best, best_t = cfg, t
+    if cache_path and key:
+        cache = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        cache[key] = best
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return best, trials

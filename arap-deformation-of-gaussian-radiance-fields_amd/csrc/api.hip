@@ -56,6 +56,7 @@ struct arapgs_ctx {
     RasterView* r_views = nullptr;    // device copy of the views of the current render
     RasterView* r_views_h = nullptr;  // pinned host staging for it
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
+    int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
@@ -99,6 +100,8 @@ int check_launch(arapgs_ctx* c, int rc, const char* what) {
     return fail(c, rc, "%s: unsupported arguments", what);
 }
 
+// Workspaces grow with 25 % headroom: per-frame sizes (tile instances) drift
+// from frame to frame, and a re-allocation synchronises the device.
 int grow(arapgs_ctx* c, void** p, size_t* cap, size_t bytes) {
     if (*cap >= bytes) return 0;
     if (*p) {
@@ -107,9 +110,10 @@ int grow(arapgs_ctx* c, void** p, size_t* cap, size_t bytes) {
         (void)hipFree(*p);
         *p = nullptr;
     }
-    hipError_t e = hipMalloc(p, bytes);
+    const size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(p, want);
     if (e != hipSuccess) { *cap = 0; return hip_fail(c, e, "workspace allocation"); }
-    *cap = bytes;
+    *cap = want;
     return 0;
 }
 
@@ -518,6 +522,7 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     rp.rots = d->rotations; rp.scale_mod = d->scale_modifier;
     rp.use_rects = d->rects != nullptr;
     rp.use_box = (d->boxmin_h && d->boxmax_h) ? 1 : 0;
+    rp.unroll = c->blend_unroll;
     for (int i = 0; i < 3; i++) {
         rp.boxmin[i] = rp.use_box ? d->boxmin_h[i] : 0.0f;
         rp.boxmax[i] = rp.use_box ? d->boxmax_h[i] : 0.0f;
@@ -576,6 +581,13 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     c->last_rendered = n;
     c->last_tiles = ntiles * V;
     if (num_rendered_h) *num_rendered_h = (int)n;
+    return 0;
+}
+
+int arapgs_set_blend_unroll(arapgs_ctx* c, int unroll) {
+    CHECK_CTX(c);
+    if (unroll != 2 && unroll != 4 && unroll != 8) return fail(c, -1, "set_blend_unroll: 2, 4 or 8");
+    c->blend_unroll = unroll;
     return 0;
 }
 

@@ -362,6 +362,9 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
 // per-pixel arithmetic of the one-pixel-per-thread form, so the image is the
 // same bit for bit; only the work on Gaussians that cannot reach 1/255 in the
 // quadrant is skipped.
+// U (2, 4 or 8) is the number of Gaussians whose alphas are evaluated together
+// before compositing them in order; it changes only the instruction schedule.
+template <int U>
 __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
                                                          const uint2* __restrict__ ranges) {
     constexpr int NT = 256;
@@ -369,7 +372,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
     __shared__ float4 s_co[NT];
     __shared__ float4 s_rgb[NT];
     __shared__ uint32_t s_id[NT];
-    __shared__ __align__(4) uint8_t s_list[4][NT + 4];
+    __shared__ __align__(8) uint8_t s_list[4][NT + 8];
     __shared__ int s_cnt[4][4];  // [quadrant][wave]
     const int ntl = rp.gx * rp.gy;
     const int view = blockIdx.x / ntl;
@@ -439,15 +442,14 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
         }
         __syncthreads();
         const uint8_t* lst = s_list[wv];
-        for (int j = 0; j < m; j += 4) {
+        for (int j = 0; j < m; j += U) {
             if (__all(done)) break;
-            const uint32_t k4 = *reinterpret_cast<const uint32_t*>(lst + j);
-            int k[4];
-            float alpha[4];
-            bool use[4];
+            int k[U];
+            float alpha[U];
+            bool use[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                k[u] = (int)((k4 >> (8 * u)) & 0xffu);
+            for (int u = 0; u < U; u++) {
+                k[u] = (int)lst[j + u];  // entries past m are stale bytes: in range, and unused
                 const float2 xy = s_xy[k[u]];
                 const float4 co = s_co[k[u]];
                 const float dx = xy.x - fxp, dy = xy.y - fyp;
@@ -456,7 +458,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
                 use[u] = (j + u < m) && !(power > 0.0f) && !(alpha[u] < 1.0f / 255.0f);
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 if (use[u] && !done) {
                     const float test_T = T * (1.0f - alpha[u]);
                     if (test_T < 0.0001f) {
@@ -945,9 +947,14 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
                            keys, d_total, 0, ranges);
     }
-    if (rp.tile == 16)
-        hipLaunchKernelGGL(blend_quad_kernel, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
-    else if (rp.tile == 8)
+    if (rp.tile == 16) {
+        if (rp.unroll == 2)
+            hipLaunchKernelGGL(blend_quad_kernel<2>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+        else if (rp.unroll == 8)
+            hipLaunchKernelGGL(blend_quad_kernel<8>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+        else
+            hipLaunchKernelGGL(blend_quad_kernel<4>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+    } else if (rp.tile == 8)
         hipLaunchKernelGGL(blend_kernel<8>, dim3((unsigned)nt), dim3(64), 0, st, rp, geo, vals, ranges);
     else
         return -1;

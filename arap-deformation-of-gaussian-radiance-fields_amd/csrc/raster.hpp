@@ -33,6 +33,7 @@ struct RasterParams {
     const float* rots;     // P*4 (w,x,y,z)
     float scale_mod;
     int use_rects, use_box;
+    int unroll;            // Gaussians per compositing step of the 16x16 blend (2, 4, 8)
     float boxmin[3], boxmax[3];
     const RasterView* views;  // device, V
 };

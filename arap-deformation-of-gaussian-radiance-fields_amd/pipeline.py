@@ -33,7 +33,7 @@ from . import arapgs
 
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
-                 stop_rule=True, render_ctx=None, sh_degree=3):
+                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16):
         import torch
         self.torch = torch
         self.solver = solver
@@ -49,6 +49,7 @@ class DragPipeline:
         self.active, self.delta = active, delta
         self.iters, self.stop_rule = gn_iters, stop_rule
         self.cams, self.images, self.bg, self.opac, self.D = cameras, images, bg, opacity, sh_degree
+        self.tile = tile
         M = state["node_pos"].shape[0]
         with torch.cuda.stream(self.stream_a):
             self.aims = torch.empty_like(state["node_pos"])
@@ -96,7 +97,7 @@ class DragPipeline:
             if render and self.images:
                 img = self.images[0]
                 n_inst = ctx.render_views(st["pos"], st["sh"], self.opac, st["scale"], st["rot"], self.cams,
-                                          img.shape[2], img.shape[1], self.bg, self.images, D=self.D)
+                                          img.shape[2], img.shape[1], self.bg, self.images, D=self.D, tile=self.tile)
             return n_inst
 
     # ------------------------------------------------------------ driver

@@ -12,7 +12,7 @@ One step = one drag frame: aims -> ARAP GN solve -> node table -> sample blend
 blend -> node quaternions + sample SH rotation -> 4 renders. All inputs are
 resident in HBM before the timed region.
 
-Three forms are timed on the same state (the drag continues across them):
+Three forms are timed from the same starting state (the drag is reset before each):
   plain      every stage synchronous, three kernels per CG iteration with the
              convergence flag read back to the host after every CG iteration
              (the reference's call pattern: a host-driven solver and a device
@@ -67,7 +67,7 @@ def log(msg):
 def setup(args):
     import torch
     load_package()
-    from arapgs_amd import arapgs, scenes, workload
+    from arapgs_amd import arapgs, autotune, scenes, workload
 
     # the solve side runs on its own (non-null) stream so that the pipelined form
     # can overlap it with the deform/render stream
@@ -85,9 +85,27 @@ def setup(args):
         images = [torch.empty((3, args.height, args.width), dtype=torch.float32, device="cuda:0") for _ in W["cams"]]
         aims = torch.empty_like(state["node_pos"])
     torch.cuda.synchronize()
-    return dict(ctx=ctx, sc=sc, state=state, fb=fb, solver=solver, aims=aims, active=W["active"],
+    return dict(ctx=ctx, sc=sc, state=state, state0=workload.clone_state(state), fb=fb, solver=solver, aims=aims,
+                active=W["active"],
+                solver_args=W["solver_args"], cfg=dict(autotune.DEFAULTS, aggregates=args.aggregates or 128),
                 delta=workload.drag_delta(sc, args.drag_frames), cams=W["cams"], bg=W["bg"], images=images,
                 opac=W["opac"], iters=sc.gn_iters, grid=W["grid"], K=W["K"])
+
+
+def reset_state(B):
+    """Every form (and every autotuner candidate) starts the drag from the same
+    state, so all of them time the same frames. Python's garbage collector runs
+    here, outside the timed frames, and is paused until the next reset."""
+    import gc
+    import torch
+    gc.enable()
+    gc.collect()
+    gc.disable()
+    with torch.cuda.stream(B["ctx"].stream):
+        for k, v in B["state"].items():
+            if v is not None:
+                v.copy_(B["state0"][k])
+    torch.cuda.synchronize()
 
 
 def frame(B, plain):
@@ -108,11 +126,12 @@ def _frame(B, plain):
         # all views in one pass (one sort, one blend launch; bit-identical images)
         img = B["images"][0]
         return ctx.render_views(st["pos"], st["sh"], B["opac"], st["scale"], st["rot"], B["cams"], img.shape[2],
-                                img.shape[1], B["bg"], B["images"])
+                                img.shape[1], B["bg"], B["images"], tile=B["cfg"]["tile"])
     n_inst = 0
     for cam, img in zip(B["cams"], B["images"]):
         out = ctx.render(st["pos"], st["sh"], B["opac"], st["scale"], st["rot"], cam["view"], cam["proj"],
-                         cam["campos"], cam["tx"], cam["ty"], img.shape[2], img.shape[1], B["bg"], out=img)
+                         cam["campos"], cam["tx"], cam["ty"], img.shape[2], img.shape[1], B["bg"], out=img,
+                         tile=B["cfg"]["tile"])
         n_inst += out["num_rendered"]
         sync()
     return n_inst
@@ -123,11 +142,13 @@ def timed_pipelined(B, steps, warmup):
     two host threads; pipeline.DragPipeline)."""
     import torch
     from arapgs_amd.pipeline import DragPipeline
+    reset_state(B)
     B["solver"].set_cg_mode(0)
-    B["solver"].set_host_checks(16)
+    B["solver"].set_host_checks(B["cfg"]["host_checks"])
     if "pipe" not in B:
         B["pipe"] = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
-                                 B["images"], B["bg"], B["opac"])
+                                 B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"])
+        B["pipe"].ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
     pipe = B["pipe"]
     pipe.run(warmup)
     torch.cuda.synchronize()
@@ -142,8 +163,9 @@ def timed_pipelined(B, steps, warmup):
 
 def timed(B, plain, steps, warmup):
     import torch
+    reset_state(B)
     B["solver"].set_cg_mode(0)
-    B["solver"].set_host_checks(1 if plain else 16)
+    B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
     for _ in range(warmup):
         frame(B, plain)
     torch.cuda.synchronize()
@@ -163,6 +185,37 @@ def timed(B, plain, steps, warmup):
     ctx.profile_enable(False)
     log(f"  GN iterations in the last frame: {int(B['iters_done'].item())}")
     return dict(total_s=total, frame_ms=times, kernel_ms=kernel_ms, instances=n_inst)
+
+
+def apply_config(B, cfg):
+    """Puts a tuning configuration in place (the solver is rebuilt when the
+    aggregate count changes; everything else is a setting)."""
+    import torch
+    from arapgs_amd import workload
+    ctx = B["ctx"]
+    if cfg["aggregates"] != B["cfg"]["aggregates"]:
+        with torch.cuda.stream(ctx.stream):
+            B["solver"].close()
+            B["solver"] = workload.make_solver(ctx, B["solver_args"], n_aggregates=cfg["aggregates"])
+        torch.cuda.synchronize()
+    B["solver"].set_coarse_refresh(cfg["refresh"])
+    ctx.set_blend_unroll(cfg["blend_unroll"])
+    B["cfg"] = dict(cfg)
+
+
+def autotune_leg(B, args):
+    """Coordinate descent over the tunables on the fast form (a few frames per
+    candidate, each from the same starting state)."""
+    from arapgs_amd import autotune
+
+    def measure(cfg):
+        apply_config(B, cfg)
+        r = timed(B, False, args.tune_steps, 2)
+        return r["total_s"] / args.tune_steps
+
+    best, trials = autotune.tune(measure, log=log)
+    apply_config(B, best)
+    return best, trials
 
 
 def committed_traffic(args):
@@ -324,6 +377,9 @@ def main():
     ap.add_argument("--forms", default="plain,fast,pipelined")
     ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the differentiation leg (0: skip)")
     ap.add_argument("--aggregates", type=int, default=0, help="coarse aggregates of the CG preconditioner (0: 128)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner (tile, blend unroll, aggregates, coarse refresh, CG host checks)")
+    ap.add_argument("--tune-steps", type=int, default=6, help="timed frames per autotuner candidate")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -346,6 +402,14 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 3}), flush=True)
         return
     B = setup(args)
+    apply_config(B, B["cfg"])
+    tuning = None
+    if args.autotune:
+        log(f"autotuning on the fast form ({args.tune_steps} frames per candidate)")
+        best, trials = autotune_leg(B, args)
+        tuning = {"config": best, "candidates_timed": len(trials),
+                  "ms_per_frame": {json.dumps(c, sort_keys=True): round(t * 1e3, 3) for c, t in trials}}
+        log(f"  autotune: {best}")
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")
@@ -354,6 +418,7 @@ def main():
         else:
             res[form] = timed(B, form == "plain", args.steps, args.warmup)
         r = res[form]
+        log(f"  {form} frame ms: {' '.join(f'{t:.1f}' for t in r['frame_ms'])}")
         log(f"  {form}: {args.steps / r['total_s']:.2f} frames/s, p50 {pct(r['frame_ms'], 50):.2f} ms, "
             f"p99 {pct(r['frame_ms'], 99):.2f} ms, {r['instances']} tile instances/frame")
     best = max(res, key=lambda f: args.steps / res[f]["total_s"])
@@ -398,6 +463,11 @@ def main():
                         "traffic": committed_traffic(args),
                         "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
                         "launches": int(len(kms))}
+    import gc
+    gc.enable()
+    line["config"]["tuning"] = B["cfg"]
+    if tuning:
+        line["autotune"] = tuning
     if args.adjoint_steps > 0:
         log(f"timing the differentiation leg: {args.warmup} warmup + {args.adjoint_steps} steps")
         line["adjoint"] = adjoint_leg(args)

@@ -264,6 +264,11 @@ int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_re
  * arapgs_render calls. n_views <= 16; *num_rendered_h = total instances. */
 int arapgs_render_views(arapgs_ctx* ctx, const arapgs_render_desc* descs_h, int n_views, int* num_rendered_h);
 
+/* Tuning of the 16x16 blend: how many Gaussians (2, 4 or 8; default 4) have
+ * their alphas evaluated together before being composited in order. Images do
+ * not depend on it. No counterpart in the reference (set by the autotuner). */
+int arapgs_set_blend_unroll(arapgs_ctx* ctx, int unroll);
+
 /* Debug view of the last arapgs_render on this context: sorted keys
  * (u64, tile << 32 | depth bits) and values (Gaussian ids) copied to host
  * buffers of capacity cap; tile ranges (u32 pairs [start,end)) to ranges_h

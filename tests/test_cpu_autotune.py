@@ -1,0 +1,39 @@
+"""The autotuner's coordinate descent (host logic, no GPU): it finds the
+separable optimum, never measures a configuration twice, keeps the defaults
+when nothing is faster, and reuses a cached result."""
+import json
+
+from arapgs_amd import autotune as A
+
+
+def _cost(cfg):
+    # separable cost with its minimum away from the defaults
+    return (abs(cfg["aggregates"] - 192) / 64 + abs(cfg["refresh"] - 16) / 4 + abs(cfg["host_checks"] - 8) / 8
+            + (0.5 if cfg["tile"] == 8 else 0.0) + abs(cfg["blend_unroll"] - 8) / 4 + 1.0)
+
+
+def test_finds_the_separable_optimum():
+    seen = []
+
+    def measure(cfg):
+        seen.append(tuple(sorted(cfg.items())))
+        return _cost(cfg)
+
+    best, trials = A.tune(measure)
+    assert best == dict(tile=16, blend_unroll=8, aggregates=192, refresh=16, host_checks=8)
+    assert len(seen) == len(set(seen)) == len(trials)
+    assert min(t for _, t in trials) == _cost(best)
+
+
+def test_keeps_defaults_when_nothing_is_faster():
+    best, _ = A.tune(lambda cfg: 1.0 + (0.0 if cfg == A.DEFAULTS else 0.1))
+    assert best == A.DEFAULTS
+
+
+def test_cache_is_reused(tmp_path):
+    path = str(tmp_path / "tune.json")
+    best, trials = A.tune(_cost, key="w1", cache_path=path)
+    assert trials
+    assert json.load(open(path))["w1"] == best
+    best2, trials2 = A.tune(lambda cfg: 1 / 0, key="w1", cache_path=path)  # never measured
+    assert best2 == best and trials2 == []

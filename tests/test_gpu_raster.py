@@ -89,3 +89,19 @@ def test_render_views_equals_separate_renders(ctx, scene, tile):
     assert n == total
     for a, b in zip(sep, outs):
         assert np.array_equal(a, H(b))
+
+
+def test_blend_unroll_does_not_change_images(ctx, scene):
+    """The 16x16 blend's unroll (Gaussians per compositing step) is a schedule:
+    images are identical bit for bit for 2, 4 and 8."""
+    g = scene["g"]
+    cam = S.look_at_camera((0.3, -4.0, 0.8), (0.0, 0.0, 0.0), 50.0, 320, 240)
+    bg = np.array([0.8, 0.8, 0.8], np.float32)
+    imgs = []
+    for u in (2, 4, 8):
+        ctx.set_blend_unroll(u)
+        out = ctx.render(T(g.pos), T(g.sh), T(g.opacity), T(g.scale), T(g.rot), T(cam.view), T(cam.proj),
+                         T(cam.campos), cam.tanfovx, cam.tanfovy, 320, 240, T(bg), tile=16)
+        imgs.append(H(out["image"]))
+    ctx.set_blend_unroll(4)
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[1], imgs[2])
