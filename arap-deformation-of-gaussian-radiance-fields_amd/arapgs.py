@@ -55,6 +55,12 @@ class ChainDesc(C.Structure):
                 ("tan_fovx_h", P), ("tan_fovy_h", P), ("axis_padding", C.c_float), ("end_coeff", C.c_float)]
 
 
+class RealignDesc(C.Structure):
+    _fields_ = [("lr_pos", C.c_float), ("lr_rot", C.c_float), ("lr_scale", C.c_float), ("lr_opacity", C.c_float),
+                ("lr_sh", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("w_density", C.c_float)]
+
+
 EXPORTS = {
     "arapgs_create": ([C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_destroy": ([P], None),
@@ -100,6 +106,10 @@ EXPORTS = {
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
     "arapgs_render_views": ([P, P, C.c_int, P], C.c_int),
     "arapgs_set_blend_unroll": ([P, C.c_int], C.c_int),
+    "arapgs_density_backward": ([P, P, P, P, P, P, P, P, C.c_float, P, P, P, P, P, P, P, P], C.c_int),
+    "arapgs_realign_create": ([P, P, C.c_int64, C.POINTER(RealignDesc), C.POINTER(P)], C.c_int),
+    "arapgs_realign_destroy": ([P], None),
+    "arapgs_realign_step": ([P, P, P, P, P, C.c_float, P, P, P, P, P, P], C.c_int),
     "arapgs_chain_create": ([P, C.POINTER(ChainDesc), C.POINTER(P)], C.c_int),
     "arapgs_chain_destroy": ([P], None),
     "arapgs_chain_forward": ([P, P, P, P, P, P, P, P], C.c_int),
@@ -502,6 +512,21 @@ class Grid:
         import torch
         return torch.from_numpy(self.arrays()[3]).cuda()
 
+    def density_backward(self, pos, rot, scale, opacity, sh, lpf, dfeat=None, dopac=None, samples=None,
+                         want_samples=False):
+        """VJP of the density sample (arapgs_density_backward): returns
+        (dpos, drot, dscale, dopacity, dsh[, dsamples])."""
+        t = self.ctx._torch
+        dev = pos.device
+        N = pos.shape[0]
+        out = [t.empty((N, 3), device=dev), t.empty((N, 4), device=dev), t.empty((N, 3), device=dev),
+               t.empty(N, device=dev), t.empty((N, 48), device=dev)]
+        ds = t.empty((self.n_valid * 64, 3), device=dev) if want_samples else None
+        self.ctx._check(lib().arapgs_density_backward(self.ctx.h, self.h, ptr(samples), ptr(pos), ptr(rot), ptr(scale),
+                                                      ptr(opacity), ptr(sh), lpf, ptr(dfeat), ptr(dopac),
+                                                      *[ptr(o) for o in out], ptr(ds)))
+        return tuple(out) + ((ds,) if want_samples else ())
+
     def density(self, pos, rot, scale, opacity, sh, lpf):
         t = self.ctx._torch
         S = self.n_valid * 64
@@ -628,3 +653,32 @@ class Chain:
         ps = [P() for _ in range(5)]
         lib().arapgs_chain_state(self.h, *[C.byref(p) for p in ps])
         return ps
+
+
+class Realign:
+    """arapgs_realign: Adam steps of the density-preserving realignment over the
+    raw Gaussian parameters (pos, rot, log scales, logit opacities, SH)."""
+
+    def __init__(self, ctx: Context, grid, n, lr_pos=1.6e-4, lr_rot=1e-3, lr_scale=5e-3, lr_opacity=5e-2, lr_sh=2.5e-3,
+                 beta1=0.9, beta2=0.999, eps=1e-15, w_density=1.0):
+        self.ctx, self.grid = ctx, grid
+        d = RealignDesc(lr_pos, lr_rot, lr_scale, lr_opacity, lr_sh, beta1, beta2, eps, w_density)
+        h = P()
+        ctx._check(lib().arapgs_realign_create(ctx.h, grid.h, n, C.byref(d), C.byref(h)))
+        self.h = h
+
+    def step(self, samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, loss=None):
+        self.ctx._check(lib().arapgs_realign_step(self.ctx.h, self.h, ptr(samples), ptr(aim_feat), ptr(aim_dens), lpf,
+                                                  ptr(pos), ptr(rot), ptr(log_scale), ptr(logit_opacity), ptr(sh),
+                                                  ptr(loss)))
+
+    def close(self):
+        if self.h:
+            lib().arapgs_realign_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

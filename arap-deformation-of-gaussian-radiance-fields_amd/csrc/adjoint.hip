@@ -86,7 +86,8 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
 // gradients sit in LDS (every lane reads the same sample: broadcast). The
 // sample gradient of each step is a wave sum. gdens is component-major
 // (component k of Gaussian g at k * gstride + g): d mu (3), d Q (q00, q11,
-// q22, then q01, q02, q12 each counted for both symmetric entries), d c (48).
+// q22, then q01, q02, q12 each counted for both symmetric entries), d c (48),
+// d o (1).
 __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const int* __restrict__ valid,
                                                               const int* __restrict__ prefix,
                                                               const int* __restrict__ list,
@@ -139,19 +140,21 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
                 cg[4 * k] = q4.x; cg[4 * k + 1] = q4.y; cg[4 * k + 2] = q4.z; cg[4 * k + 3] = q4.w;
             }
         }
-        float gmu[3] = {0.0f, 0.0f, 0.0f}, gQ[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, gc[48];
+        float gmu[3] = {0.0f, 0.0f, 0.0f}, gQ[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, gc[48], go = 0.0f;
 #pragma unroll
         for (int k = 0; k < 48; k++) gc[k] = 0.0f;
         for (int s = 0; s < 64; s++) {
             const float d0 = s_x[s][0] - mu[0], d1 = s_x[s][1] - mu[1], d2 = s_x[s][2] - mu[2];
             const float qf = Q[0] * d0 * d0 + Q[1] * d1 * d1 + Q[2] * d2 * d2 +
                              2.0f * (Q[3] * d0 * d1 + Q[4] * d0 * d2 + Q[5] * d1 * d2);
-            const float a = o * __expf(-0.5f * qf);
+            const float ex = __expf(-0.5f * qf);
+            const float a = o * ex;
             float da = s_df[s][48];
 #pragma unroll
             for (int k = 0; k < 48; k++) da += s_df[s][k] * cg[k];
 #pragma unroll
             for (int k = 0; k < 48; k++) gc[k] += a * s_df[s][k];
+            go += ex * da;
             const float dq = -0.5f * a * da;
             gQ[0] += dq * d0 * d0;
             gQ[1] += dq * d1 * d1;
@@ -180,17 +183,95 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
             for (int k = 0; k < 6; k++) if (gQ[k] != 0.0f) unsafeAtomicAdd(dst + (3 + k) * gstride, gQ[k]);
 #pragma unroll
             for (int k = 0; k < 48; k++) if (gc[k] != 0.0f) unsafeAtomicAdd(dst + (9 + k) * gstride, gc[k]);
+            if (go != 0.0f) unsafeAtomicAdd(dst + 57 * gstride, go);
         }
     }
     __syncthreads();
+    if (gsamp)
 #pragma unroll
-    for (int k = 0; k < 3; k++) gsamp[3 * si + k] = s_dx[t][k];
+        for (int k = 0; k < 3; k++) gsamp[3 * si + k] = s_dx[t][k];
+}
+
+// dL/dQ of a density Gaussian (d[0..5] as accumulated by the density backward)
+// back to its rotation q (unnormalised, as stored) and scale s: Sigma = R(q/|q|)
+// diag(s^2) R^T + lpf I, Q = Sigma^-1, dSigma = -Q dQ Q. Adds into dq and ds.
+__device__ __forceinline__ void density_cov_backward(const float* __restrict__ rot, const float* __restrict__ scale,
+                                                     int64_t g, float lpf, const float dq6[6], float dq[4],
+                                                     float ds[3]) {
+    float Qv[6], R[3][3], S[3][3];
+    density_inverse_cov(rot, scale, g, lpf, Qv, R, S);
+    const float s1[3] = {scale[3 * g], scale[3 * g + 1], scale[3 * g + 2]};
+    const float q1[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
+    const float Qm[3][3] = {{Qv[0], Qv[3], Qv[4]}, {Qv[3], Qv[1], Qv[5]}, {Qv[4], Qv[5], Qv[2]}};
+    const float dQ[3][3] = {{dq6[0], 0.5f * dq6[3], 0.5f * dq6[4]},
+                            {0.5f * dq6[3], dq6[1], 0.5f * dq6[5]},
+                            {0.5f * dq6[4], 0.5f * dq6[5], dq6[2]}};
+    float tmp[3][3], dS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) tmp[i][j] = dQ[i][0] * Qm[0][j] + dQ[i][1] * Qm[1][j] + dQ[i][2] * Qm[2][j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) dS[i][j] = -(Qm[i][0] * tmp[0][j] + Qm[i][1] * tmp[1][j] + Qm[i][2] * tmp[2][j]);
+    float dR[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; j++) acc += (dS[a][j] + dS[j][a]) * R[j][b];
+            dR[a][b] = s1[b] * s1[b] * acc;
+        }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float dD = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) dD += dS[i][j] * R[i][k] * R[j][k];
+        ds[k] += dD * 2.0f * s1[k];
+    }
+    float qn[4] = {q1[0], q1[1], q1[2], q1[3]};
+    quat_normalize(qn);
+    float dqn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    quat_to_mat_backward(qn, dR, dqn);
+    quat_normalize_backward(q1, dqn, dq);
+}
+
+// The density gradients of every Gaussian's own attributes (the VJP of the
+// density sample, arapgs_density_backward): position, rotation (as stored),
+// scale, opacity and SH from the component-major accumulators gd (58 x N).
+__global__ void __launch_bounds__(256) density_attr_grad_kernel(int64_t N, const float* __restrict__ rot,
+                                                                const float* __restrict__ scale,
+                                                                const float* __restrict__ gd, float lpf,
+                                                                float* __restrict__ dpos, float* __restrict__ drot,
+                                                                float* __restrict__ dscale, float* __restrict__ dopac,
+                                                                float* __restrict__ dsh) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float dq6[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) dq6[k] = gd[(3 + k) * N + g];
+    float dq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ds[3] = {0.0f, 0.0f, 0.0f};
+    density_cov_backward(rot, scale, g, lpf, dq6, dq, ds);
+#pragma unroll
+    for (int k = 0; k < 3; k++) dpos[3 * g + k] = gd[k * N + g];
+#pragma unroll
+    for (int k = 0; k < 4; k++) drot[4 * g + k] = dq[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dscale[3 * g + k] = ds[k];
+    dopac[g] = gd[57 * N + g];
+#pragma unroll
+    for (int k = 0; k < 48; k++) dsh[48 * g + k] = gd[(9 + k) * N + g];
 }
 
 // One thread per Gaussian: the adjoint of everything between the rest
 // attributes and (mu', q', s', c'). gr: the renderer's per-Gaussian gradient
 // (d mu' 3, d q' 4, d s' 3, d c' 48); gd: the density's, component-major
-// (57 x N: d mu' 3, d Q 6, d c' 48) or null. Writes d e' (18) and d q0, d s0.
+// (58 x N: d mu' 3, d Q 6, d c' 48, d o) or null; opacities are constants here. Writes d e' (18) and d q0, d s0.
 __global__ void __launch_bounds__(128) six_point_backward_kernel(
     int64_t N, int K, const NodeXf* __restrict__ table, const uint32_t* __restrict__ ends_idx,
     const float* __restrict__ ends_w, const float* __restrict__ ends, const float* __restrict__ rot0,
@@ -207,8 +288,6 @@ __global__ void __launch_bounds__(128) six_point_backward_kernel(
     float dc1[48];
 #pragma unroll
     for (int k = 0; k < 48; k++) dc1[k] = r[10 + k];
-    const float q1[4] = {rot1[4 * g], rot1[4 * g + 1], rot1[4 * g + 2], rot1[4 * g + 3]};
-    const float s1[3] = {scale1[3 * g], scale1[3 * g + 1], scale1[3 * g + 2]};
     if (gd) {
         float d[57];
 #pragma unroll
@@ -217,46 +296,7 @@ __global__ void __launch_bounds__(128) six_point_backward_kernel(
         for (int k = 0; k < 3; k++) dmu[k] += d[k];
 #pragma unroll
         for (int k = 0; k < 48; k++) dc1[k] += d[9 + k];
-        // d Q -> d Sigma = -Q dQ Q -> (R(q'/|q'|), s')
-        float Qv[6], R[3][3], S[3][3];
-        density_inverse_cov(rot1, scale1, g, lpf, Qv, R, S);
-        const float Qm[3][3] = {{Qv[0], Qv[3], Qv[4]}, {Qv[3], Qv[1], Qv[5]}, {Qv[4], Qv[5], Qv[2]}};
-        const float dQ[3][3] = {{d[3], 0.5f * d[6], 0.5f * d[7]},
-                                {0.5f * d[6], d[4], 0.5f * d[8]},
-                                {0.5f * d[7], 0.5f * d[8], d[5]}};
-        float tmp[3][3], dS[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) tmp[i][j] = dQ[i][0] * Qm[0][j] + dQ[i][1] * Qm[1][j] + dQ[i][2] * Qm[2][j];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) dS[i][j] = -(Qm[i][0] * tmp[0][j] + Qm[i][1] * tmp[1][j] + Qm[i][2] * tmp[2][j]);
-        float dR[3][3];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int j = 0; j < 3; j++) acc += (dS[a][j] + dS[j][a]) * R[j][b];
-                dR[a][b] = s1[b] * s1[b] * acc;
-            }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float dD = 0.0f;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) dD += dS[i][j] * R[i][k] * R[j][k];
-            ds1[k] += dD * 2.0f * s1[k];
-        }
-        float qn[4] = {q1[0], q1[1], q1[2], q1[3]};
-        quat_normalize(qn);
-        float dqn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        quat_to_mat_backward(qn, dR, dqn);
-        quat_normalize_backward(q1, dqn, dq1);
+        density_cov_backward(rot1, scale1, g, lpf, d + 3, dq1, ds1);
     }
 
     // recompute the six-point forward from the blended end points
@@ -498,7 +538,110 @@ __global__ void __launch_bounds__(256) node_gather_kernel(int M, int K, int64_t 
     }
 }
 
+// ---------------------------------------------------------------- realignment
+// raw parameters -> activated: s = exp(log s), o = sigmoid(logit o)
+__global__ void __launch_bounds__(256) activate_kernel(int64_t N, const float* __restrict__ log_scale,
+                                                       const float* __restrict__ logit_opac, float* __restrict__ scale,
+                                                       float* __restrict__ opac) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) scale[3 * g + k] = expf(log_scale[3 * g + k]);
+    opac[g] = 1.0f / (1.0f + expf(-logit_opac[g]));
+}
+
+// L1 loss sum |feat - aim| + w sum |dens - aim_dens| and its subgradient
+// (sign, 0 at equality); the loss is reduced in FP64 (block sum, one atomic).
+__global__ void __launch_bounds__(256) l1_grad_kernel(int64_t S, const float* __restrict__ feat,
+                                                      const float* __restrict__ dens, const float* __restrict__ aim_feat,
+                                                      const float* __restrict__ aim_dens, float w,
+                                                      float* __restrict__ dfeat, float* __restrict__ ddens,
+                                                      double* __restrict__ loss) {
+    __shared__ double s_red[4];
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (s < S) {
+#pragma unroll 4
+        for (int k = 0; k < 48; k++) {
+            const float d = feat[48 * s + k] - aim_feat[48 * s + k];
+            dfeat[48 * s + k] = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+            acc += fabs((double)d);
+        }
+        const float d = dens[s] - aim_dens[s];
+        ddens[s] = d > 0.0f ? w : (d < 0.0f ? -w : 0.0f);
+        acc += (double)w * fabs((double)d);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0 && loss) unsafeAtomicAdd(loss, s_red[0] + s_red[1] + s_red[2] + s_red[3]);
+}
+
+// One Adam step per Gaussian over its 59 raw parameters (pos 3, rot 4,
+// log-scale 3, logit-opacity 1, SH 48); the activated gradients are chained to
+// the raw ones (d log s = d s * s, d logit o = d o * o (1 - o)). m and v are
+// component-major (k * N + g); bc1 / bc2 are the bias corrections 1 - beta^t.
+__global__ void __launch_bounds__(256) adam_kernel(int64_t N, const float* __restrict__ dpos,
+                                                   const float* __restrict__ drot, const float* __restrict__ dscale,
+                                                   const float* __restrict__ dopac, const float* __restrict__ dsh,
+                                                   const float* __restrict__ scale, const float* __restrict__ opac,
+                                                   float* __restrict__ pos, float* __restrict__ rot,
+                                                   float* __restrict__ log_scale, float* __restrict__ logit_opac,
+                                                   float* __restrict__ sh, float* __restrict__ m, float* __restrict__ v,
+                                                   float lr_pos, float lr_rot, float lr_scale, float lr_opac,
+                                                   float lr_sh, float b1, float b2, float eps, float bc1, float bc2) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    auto step = [&](int k, float grad, float lr, float& p) {
+        float& mk = m[k * N + g];
+        float& vk = v[k * N + g];
+        mk = b1 * mk + (1.0f - b1) * grad;
+        vk = b2 * vk + (1.0f - b2) * grad * grad;
+        p -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+    };
+#pragma unroll
+    for (int c = 0; c < 3; c++) step(c, dpos[3 * g + c], lr_pos, pos[3 * g + c]);
+#pragma unroll
+    for (int c = 0; c < 4; c++) step(3 + c, drot[4 * g + c], lr_rot, rot[4 * g + c]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) step(7 + c, dscale[3 * g + c] * scale[3 * g + c], lr_scale, log_scale[3 * g + c]);
+    const float o = opac[g];
+    step(10, dopac[g] * o * (1.0f - o), lr_opac, logit_opac[g]);
+#pragma unroll 4
+    for (int c = 0; c < 48; c++) step(11 + c, dsh[48 * g + c], lr_sh, sh[48 * g + c]);
+}
+
 }  // namespace
+
+int launch_activate(int64_t N, const float* log_scale, const float* logit_opac, float* scale, float* opac,
+                    hipStream_t st) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(activate_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, log_scale, logit_opac, scale, opac);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float* aim_feat, const float* aim_dens,
+                   float w, float* dfeat, float* ddens, double* loss, hipStream_t st) {
+    if (S <= 0) return 0;
+    hipLaunchKernelGGL(l1_grad_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, S, feat, dens, aim_feat, aim_dens, w,
+                       dfeat, ddens, loss);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_adam(int64_t N, const float* dpos, const float* drot, const float* dscale, const float* dopac,
+                const float* dsh, const float* scale, const float* opac, float* pos, float* rot, float* log_scale,
+                float* logit_opac, float* sh, float* m, float* v, const float lr[5], float b1, float b2, float eps,
+                float bc1, float bc2, hipStream_t st) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, dpos, drot, dscale, dopac, dsh, scale,
+                       opac, pos, rot, log_scale, logit_opac, sh, m, v, lr[0], lr[1], lr[2], lr[3], lr[4], b1, b2, eps,
+                       bc1, bc2);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
 
 int launch_zero_f32(float* p, int64_t n, hipStream_t st) {
     if (n <= 0) return 0;
@@ -514,6 +657,15 @@ int launch_density_backward(int64_t nv, const int* valid, const int* prefix, con
     if (nv <= 0) return 0;
     hipLaunchKernelGGL(density_backward_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples,
                        pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens, n_gauss);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, float lpf, float* dpos,
+                             float* drot, float* dscale, float* dopac, float* dsh, hipStream_t st) {
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(density_attr_grad_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, rot, scale, gd, lpf, dpos,
+                       drot, dscale, dopac, dsh);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -11,12 +11,19 @@ namespace arapgs {
 int launch_zero_f32(float* p, int64_t n, hipStream_t st);
 
 // Density backward over the grid's valid cells at the given (deformed) sample
-// positions: gsamp (S*3) written, gdens (57 x N component-major: d mu 3,
-// d Q 6, d sh 48) accumulated (zeroed by the caller). dfeat / ddens may be null.
+// positions: gsamp (S*3, may be null) written, gdens (58 x N component-major:
+// d mu 3, d Q 6, d sh 48, d opacity) accumulated (zeroed by the caller).
+// dfeat / ddens may be null.
 int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
                             float* gdens, int64_t n_gauss, hipStream_t st);
+
+// Gradients of each Gaussian's own attributes from the density accumulators
+// (58 x N): dpos N*3, drot N*4 (w.r.t. the stored rotation), dscale N*3,
+// dopac N, dsh N*48.
+int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, float lpf, float* dpos,
+                             float* drot, float* dscale, float* dopac, float* dsh, hipStream_t st);
 
 // Per-Gaussian adjoint from (mu', q', s', sh') to the blended end points (gends,
 // N*18), q0 (drot0, N*4) and s0 (dscale0, N*3). gr: N*58 renderer gradients;
@@ -39,5 +46,17 @@ int launch_node_gather(int M, int K, int64_t n_end_entries, const uint2* rng, co
                        const NodeXf* table, const float* ends_w, const float* ends0, const float* gends,
                        const float* sample_w, const float* samples0, const float* gsamp, NodeXf* dtable,
                        hipStream_t st);
+
+// Realignment (density-preserving step): activation of the raw parameters,
+// the L1 loss subgradient at the samples, one Adam step over the 59 raw
+// parameters per Gaussian (lr: pos, rot, log-scale, logit-opacity, SH).
+int launch_activate(int64_t N, const float* log_scale, const float* logit_opac, float* scale, float* opac,
+                    hipStream_t st);
+int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float* aim_feat, const float* aim_dens,
+                   float w, float* dfeat, float* ddens, double* loss, hipStream_t st);
+int launch_adam(int64_t N, const float* dpos, const float* drot, const float* dscale, const float* dopac,
+                const float* dsh, const float* scale, const float* opac, float* pos, float* rot, float* log_scale,
+                float* logit_opac, float* sh, float* m, float* v, const float lr[5], float b1, float b2, float eps,
+                float bc1, float bc2, hipStream_t st);
 
 }  // namespace arapgs

@@ -48,6 +48,8 @@ struct arapgs_ctx {
     size_t r_ranges_cap = 0;
     void* r_rec = nullptr;          // depth sort of the records
     size_t r_rec_cap = 0;
+    void* db_ws = nullptr;          // density backward accumulators
+    size_t db_cap = 0;
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
@@ -163,7 +165,7 @@ void arapgs_destroy(arapgs_ctx* c) {
         (void)hipEventDestroy(pr.second);
     }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
-                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec,
+                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec, c->db_ws,
                     (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views})
         if (p) (void)hipFree(p);
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
@@ -627,6 +629,7 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
 struct arapgs_grid {
     int G = 0;
     int64_t n_pairs = 0, n_valid = 0;
+    int64_t n_gauss = 0;  // Gaussians the grid was built for
     int* prefix = nullptr;
     int* list = nullptr;
     int* valid = nullptr;
@@ -712,6 +715,7 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, padding};
     arapgs_grid* g = new arapgs_grid();
     g->G = grid_num;
+    g->n_gauss = n;
     g->device = c->device;
     std::vector<void*> tmp;
     auto dmal = [&](size_t bytes) -> void* {
@@ -804,6 +808,34 @@ int arapgs_grid_info(const arapgs_grid* g, int64_t* n_pairs, int64_t* n_valid, c
     if (valid) *valid = g->valid;
     if (samples) *samples = g->samples;
     return 0;
+}
+
+int arapgs_density_backward(arapgs_ctx* c, const arapgs_grid* g, const float* samples, const float* pos,
+                            const float* rot, const float* scale, const float* opacity, const float* sh, float lpf,
+                            const float* dfeat, const float* dopac, float* dpos, float* drot, float* dscale,
+                            float* dopacity, float* dsh, float* dsamples) {
+    CHECK_CTX(c);
+    if (!g || !pos || !rot || !scale || !opacity || !sh || !dpos || !drot || !dscale || !dopacity || !dsh)
+        return fail(c, -1, "density_backward: bad arguments");
+    if ((uintptr_t)sh % 16) return fail(c, -1, "density_backward: sh must be 16-byte aligned");
+    (void)hipSetDevice(c->device);
+    const int64_t N = g->n_gauss;  // the Gaussians the grid was built for
+    int rc = grow(c, &c->db_ws, &c->db_cap, sizeof(float) * 58 * (size_t)std::max<int64_t>(N, 1));
+    if (rc) return rc;
+    float* gd = (float*)c->db_ws;
+    rc = check_launch(c, launch_zero_f32(gd, 58 * N, c->stream), "density_backward");
+    if (!rc && (dfeat || dopac))
+        rc = check_launch(c, launch_density_backward(g->n_valid, g->valid, g->prefix, g->list,
+                                                     samples ? samples : g->samples, pos, rot, scale, opacity, sh,
+                                                     lpf, dfeat, dopac, dsamples, gd, N, c->stream),
+                          "density_backward");
+    else if (!rc && dsamples)
+        rc = check_launch(c, launch_zero_f32(dsamples, 3 * 64 * g->n_valid, c->stream), "density_backward");
+    if (!rc)
+        rc = check_launch(c, launch_density_attr_grad(N, rot, scale, gd, lpf, dpos, drot, dscale, dopacity, dsh,
+                                                      c->stream),
+                          "density_backward");
+    return rc;
 }
 
 int arapgs_density_sample(arapgs_ctx* c, const arapgs_grid* g, const float* pos, const float* rot, const float* scale,
@@ -1047,7 +1079,7 @@ int arapgs_chain_create(arapgs_ctx* c, const arapgs_chain_desc* d, arapgs_chain*
     rc = rc ? rc : chain_alloc(c, ch, &ch->img, V * 3 * (size_t)ch->W * ch->H);
     rc = rc ? rc : chain_alloc(c, ch, &ch->grec, V * N * 8);
     rc = rc ? rc : chain_alloc(c, ch, &ch->gout, 58 * N);
-    rc = rc ? rc : chain_alloc(c, ch, &ch->gdens, 57 * N);
+    rc = rc ? rc : chain_alloc(c, ch, &ch->gdens, 58 * N);
     rc = rc ? rc : chain_alloc(c, ch, &ch->gends, 18 * N);
     rc = rc ? rc : chain_alloc(c, ch, &ch->gsamp, 3 * S);
     rc = rc ? rc : chain_alloc(c, ch, &ch->entries, (size_t)ch->n_entries);
@@ -1142,7 +1174,7 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
     // density
     const float* gd = nullptr;
     if (ch->S > 0) {
-        rc = check_launch(c, launch_zero_f32(ch->gdens, 57 * ch->N, st), "chain backward");
+        rc = check_launch(c, launch_zero_f32(ch->gdens, 58 * ch->N, st), "chain backward");
         if (!rc && (dfeat || ddens))
             rc = check_launch(c, launch_density_backward(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples,
                                                          ch->pos, ch->rot, ch->scale, ch->d.opacity, ch->sh,
@@ -1175,6 +1207,117 @@ int arapgs_chain_state(const arapgs_chain* ch, const float** pos, const float** 
     if (sh) *sh = ch->sh;
     if (samples) *samples = ch->samples;
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- realignment
+struct arapgs_realign {
+    int device = 0;
+    const arapgs_grid* grid = nullptr;
+    int64_t N = 0, S = 0;
+    arapgs_realign_desc d{};
+    int64_t t = 0;  // Adam step
+    float *scale = nullptr, *opac = nullptr, *feat = nullptr, *dens = nullptr, *dfeat = nullptr, *ddens = nullptr;
+    float *gd = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+void realign_free(arapgs_realign* r) {
+    if (!r) return;
+    (void)hipSetDevice(r->device);
+    for (void* p : r->allocs) (void)hipFree(p);
+    delete r;
+}
+}  // namespace
+
+extern "C" {
+
+int arapgs_realign_create(arapgs_ctx* c, const arapgs_grid* grid, int64_t n, const arapgs_realign_desc* d,
+                          arapgs_realign** out) {
+    CHECK_CTX(c);
+    if (!grid || !d || !out || n < 1 || n < grid->n_gauss) return fail(c, -1, "realign_create: bad arguments");
+    *out = nullptr;
+    (void)hipSetDevice(c->device);
+    arapgs_realign* r = new arapgs_realign();
+    r->device = c->device;
+    r->grid = grid;
+    r->N = n;
+    r->S = 64 * grid->n_valid;
+    r->d = *d;
+    if (r->d.beta1 == 0.0f) r->d.beta1 = 0.9f;
+    if (r->d.beta2 == 0.0f) r->d.beta2 = 0.999f;
+    if (r->d.eps == 0.0f) r->d.eps = 1e-15f;
+    const size_t N = (size_t)n, S = (size_t)std::max<int64_t>(r->S, 1);
+    struct { float** p; size_t count; } bufs[] = {
+        {&r->scale, 3 * N}, {&r->opac, N}, {&r->feat, 48 * S}, {&r->dens, S}, {&r->dfeat, 48 * S}, {&r->ddens, S},
+        {&r->gd, 58 * N}, {&r->grad, 59 * N}, {&r->m, 59 * N}, {&r->v, 59 * N}};
+    for (auto& b : bufs) {
+        hipError_t e = hipMalloc((void**)b.p, sizeof(float) * b.count);
+        if (e != hipSuccess) {
+            realign_free(r);
+            return hip_fail(c, e, "realign allocation");
+        }
+        r->allocs.push_back((void*)*b.p);
+    }
+    int rc = check_launch(c, launch_zero_f32(r->m, 59 * (int64_t)N, c->stream), "realign_create");
+    if (!rc) rc = check_launch(c, launch_zero_f32(r->v, 59 * (int64_t)N, c->stream), "realign_create");
+    if (rc) {
+        realign_free(r);
+        return rc;
+    }
+    *out = r;
+    return 0;
+}
+
+void arapgs_realign_destroy(arapgs_realign* r) { realign_free(r); }
+
+int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, const float* aim_feat,
+                        const float* aim_dens, float lpf, float* pos, float* rot, float* log_scale,
+                        float* logit_opacity, float* sh, double* loss) {
+    CHECK_CTX(c);
+    if (!r || !samples || !aim_feat || !aim_dens || !pos || !rot || !log_scale || !logit_opacity || !sh)
+        return fail(c, -1, "realign_step: bad arguments");
+    if (r->device != c->device) return fail(c, -1, "realign_step: object of another device");
+    if ((uintptr_t)sh % 16) return fail(c, -1, "realign_step: sh must be 16-byte aligned");
+    (void)hipSetDevice(c->device);
+    hipStream_t st = c->stream;
+    const arapgs_grid* g = r->grid;
+    const int64_t N = r->N;
+    int rc = check_launch(c, launch_activate(N, log_scale, logit_opacity, r->scale, r->opac, st), "realign activate");
+    if (!rc)
+        rc = check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot, r->scale,
+                                            r->opac, sh, lpf, r->feat, r->dens, st),
+                          "realign density");
+    if (!rc && loss) rc = check_launch(c, launch_zero_f32((float*)loss, 2, st), "realign loss");  // 0.0 as f64
+    if (!rc)
+        rc = check_launch(c, launch_l1_grad(r->S, r->feat, r->dens, aim_feat, aim_dens, r->d.w_density, r->dfeat,
+                                            r->ddens, loss, st),
+                          "realign loss");
+    if (!rc) rc = check_launch(c, launch_zero_f32(r->gd, 58 * N, st), "realign backward");
+    if (!rc)
+        rc = check_launch(c, launch_density_backward(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot,
+                                                     r->scale, r->opac, sh, lpf, r->dfeat, r->ddens, nullptr, r->gd,
+                                                     N, st),
+                          "realign backward");
+    float* dpos = r->grad;
+    float* drot = dpos + 3 * N;
+    float* dscale = drot + 4 * N;
+    float* dopac = dscale + 3 * N;
+    float* dsh = dopac + N;
+    if (!rc)
+        rc = check_launch(c, launch_density_attr_grad(N, rot, r->scale, r->gd, lpf, dpos, drot, dscale, dopac, dsh, st),
+                          "realign backward");
+    if (rc) return rc;
+    r->t++;
+    const float lr[5] = {r->d.lr_pos, r->d.lr_rot, r->d.lr_scale, r->d.lr_opacity, r->d.lr_sh};
+    const float bc1 = 1.0f - (float)std::pow((double)r->d.beta1, (double)r->t);
+    const float bc2 = 1.0f - (float)std::pow((double)r->d.beta2, (double)r->t);
+    return check_launch(c, launch_adam(N, dpos, drot, dscale, dopac, dsh, r->scale, r->opac, pos, rot, log_scale,
+                                       logit_opacity, sh, r->m, r->v, lr, r->d.beta1, r->d.beta2, r->d.eps, bc1, bc2,
+                                       st),
+                        "realign adam");
 }
 
 }  // extern "C"

@@ -207,6 +207,42 @@ int arapgs_density_sample(arapgs_ctx* ctx, const arapgs_grid* grid, const float*
                           const float* scale, const float* opacity, const float* sh, float lpf, float* feat,
                           float* opac);
 
+/* The vector-Jacobian product of arapgs_density_sample (the role of the
+ * un-vendored backward3d_grid, called at GaussianView.cpp:4415-4447) at sample
+ * positions `samples` (device f32[S*3]; NULL = the grid's own samples), with
+ * the grid's cell lists held fixed: given dL/dfeat (f32[S*48], may be NULL) and
+ * dL/dopac (f32[S], may be NULL), writes dL/d of every Gaussian's pos (N*3),
+ * rot (N*4, with respect to the stored, unnormalised quaternion), scale (N*3,
+ * activated), opacity (N, activated) and sh (N*48); dsamples (S*3) optional.
+ * Float atomics accumulate over cells, so the last bits vary between runs. */
+int arapgs_density_backward(arapgs_ctx* ctx, const arapgs_grid* grid, const float* samples, const float* pos,
+                            const float* rot, const float* scale, const float* opacity, const float* sh, float lpf,
+                            const float* dfeat, const float* dopac, float* dpos, float* drot, float* dscale,
+                            float* dopacity, float* dsh, float* dsamples);
+
+/* Density-preserving realignment (the F6 loop of GaussianView.cpp:1695-1733,
+ * GPUOptimize at :4363-4486: forward3d_grid, L1loss3d, backward3d_grid,
+ * update3d). The Gaussians' raw parameters — pos, rot (stored quaternion), log
+ * scales, logit opacities, SH — are moved by Adam so that the density sample
+ * at the (deformed) samples matches the aim features / densities in L1:
+ * L = sum |feat - aim_feat| + w_density sum |dens - aim_dens|. The cell lists
+ * of the grid stay fixed during the loop. The per-cell convergence flags,
+ * ExpandOpRange and adaptive low-pass of the reference are not modelled. */
+typedef struct arapgs_realign_desc {
+    float lr_pos, lr_rot, lr_scale, lr_opacity, lr_sh;  /* Adam step sizes (raw parameters) */
+    float beta1, beta2, eps;                             /* 0 = 0.9, 0.999, 1e-15 */
+    float w_density;                                     /* weight of the density term */
+} arapgs_realign_desc;
+typedef struct arapgs_realign arapgs_realign;
+int arapgs_realign_create(arapgs_ctx* ctx, const arapgs_grid* grid, int64_t n, const arapgs_realign_desc* desc_h,
+                          arapgs_realign** out);
+void arapgs_realign_destroy(arapgs_realign* r);
+/* One Adam step; loss (device f64[1], optional) receives L before the step.
+ * samples f32[S*3], aim_feat f32[S*48], aim_dens f32[S] (S = 64 * valid cells). */
+int arapgs_realign_step(arapgs_ctx* ctx, arapgs_realign* r, const float* samples, const float* aim_feat,
+                        const float* aim_dens, float lpf, float* pos, float* rot, float* log_scale,
+                        float* logit_opacity, float* sh, double* loss);
+
 /* ------------------------------------------------------------ render */
 /* CudaRasterizer::Rasterizer::forward as called at GaussianView.cpp:1099-1128
  * (positional order of that call): P Gaussians, SH degree D (M = 16

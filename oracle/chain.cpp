@@ -300,13 +300,15 @@ void blend_t(const orc_chain_args* a, const T* Rn, const T* tn, const uint32_t* 
 
 template <class T>
 struct Deformed {
-    std::vector<T> pos, rot, scale, sh, samples;
+    std::vector<T> pos, rot, scale, sh, samples, opac;
 };
 
 template <class T>
 void deform_t(const orc_chain_args* a, const T* Rn, const T* tn, const T* q0, const T* s0, Deformed<T>& D) {
     const long N = a->N;
     D.pos.assign(3 * N, T(0.0)); D.rot.assign(4 * N, T(0.0)); D.scale.assign(3 * N, T(0.0)); D.sh.assign(48 * N, T(0.0));
+    D.opac.resize(N);
+    for (long g = 0; g < N; g++) D.opac[g] = (double)a->opac[g];
 #pragma omp parallel for schedule(static)
     for (long g = 0; g < N; g++) {
         T qn[4] = {q0[4 * g], q0[4 * g + 1], q0[4 * g + 2], q0[4 * g + 3]};
@@ -527,7 +529,7 @@ void density_t(const orc_chain_args* a, const Deformed<T>& D, T* feat, T* dens) 
                 const T* q = &Q[6 * g];
                 const T qf = q[0] * d0 * d0 + q[1] * d1 * d1 + q[2] * d2 * d2 +
                              2.0 * (q[3] * d0 * d1 + q[4] * d0 * d2 + q[5] * d1 * d2);
-                const T al = (double)a->opac[g] * exp(-0.5 * qf);
+                const T al = D.opac[g] * exp(-0.5 * qf);
                 ao += al;
                 for (int k = 0; k < 48; k++) acc[k] += al * D.sh[48 * g + k];
             }
@@ -594,6 +596,50 @@ void orc_chain_jvp(const orc_chain_args* a, const double* Rn, const double* tn, 
 void orc_chain_forward(const orc_chain_args* a, const double* Rn, const double* tn, const double* q0,
                        const double* s0, double* images, double* feat, double* dens) {
     chain_t<double>(a, Rn, tn, q0, s0, images, feat, dens, nullptr);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// The density sample alone (the formula of arapgs_density_sample) over its own
+// Gaussian attributes and sample positions, with forward-mode derivatives
+// along (dpos, drot, dscale, dopac, dsh, dsamples) (each may be NULL = 0).
+void orc_density_jvp(long nv, const int* valid, const int* prefix, const int* list, long N, const float* samples,
+                     const float* pos, const float* rot, const float* scale, const float* opac, const float* sh,
+                     float lpf, const double* dpos, const double* drot, const double* dscale, const double* dopac,
+                     const double* dsh, const double* dsamples, double* feat, double* dens, double* d_feat,
+                     double* d_dens) {
+    orc_chain_args a{};
+    a.N = N;
+    a.nv = nv;
+    a.valid = valid;
+    a.prefix = prefix;
+    a.list = list;
+    a.lpf = lpf;
+    a.opac = opac;
+    const long S = 64 * nv;
+    Deformed<Dual> D;
+    auto fill = [](std::vector<Dual>& out, const float* v, const double* dv, long n) {
+        out.resize(n);
+        for (long i = 0; i < n; i++) out[i] = Dual((double)v[i], dv ? dv[i] : 0.0);
+    };
+    fill(D.pos, pos, dpos, 3 * N);
+    fill(D.rot, rot, drot, 4 * N);
+    fill(D.scale, scale, dscale, 3 * N);
+    fill(D.opac, opac, dopac, N);
+    fill(D.sh, sh, dsh, 48 * N);
+    fill(D.samples, samples, dsamples, 3 * S);
+    std::vector<Dual> fe(48 * S), de(S);
+    density_t(&a, D, fe.data(), de.data());
+    for (long i = 0; i < 48 * S; i++) {
+        feat[i] = fe[i].v;
+        d_feat[i] = fe[i].d;
+    }
+    for (long i = 0; i < S; i++) {
+        dens[i] = de[i].v;
+        d_dens[i] = de[i].d;
+    }
 }
 
 }  // extern "C"

@@ -86,6 +86,8 @@ def _declare(L):
     L.orc_chain_jvp.restype = None
     L.orc_chain_forward.argtypes = [C.c_void_p] + [P] * 7
     L.orc_chain_forward.restype = None
+    L.orc_density_jvp.argtypes = [C.c_long, P, P, P, C.c_long, P, P, P, P, P, P, C.c_float] + [P] * 10
+    L.orc_density_jvp.restype = None
     L.orc_raster_bin.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P]
     L.orc_raster_bin.restype = C.c_long
     L.orc_raster_blend.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P, P]
@@ -406,3 +408,21 @@ class Chain:
         lib().orc_chain_forward(C.byref(self.args), _p(_f64(R)), _p(_f64(t)), _p(_f64(q0)), _p(_f64(s0)),
                                 _p(images), _p(feat), _p(dens))
         return images, feat, dens
+
+
+def density_jvp(valid, prefix, lst, samples, pos, rot, scale, opac, sh, lpf, dpos=None, drot=None, dscale=None,
+                dopac=None, dsh=None, dsamples=None):
+    """FP64 density sample and its directional derivative along the given
+    directions of the Gaussian attributes and sample positions (oracle/chain.cpp)."""
+    nv = np.asarray(valid).shape[0]
+    N = np.asarray(pos).shape[0]
+    S = 64 * nv
+    feat, dens = np.zeros((S, 48)), np.zeros(S)
+    d_feat, d_dens = np.zeros((S, 48)), np.zeros(S)
+    keep = [_i32(valid), _i32(prefix), _i32(lst), _f32(samples), _f32(pos), _f32(rot), _f32(scale), _f32(opac),
+            _f32(sh)]
+    dirs = [None if d is None else _f64(d) for d in (dpos, drot, dscale, dopac, dsh, dsamples)]
+    lib().orc_density_jvp(nv, _p(keep[0]), _p(keep[1]), _p(keep[2]), N, _p(keep[3]), _p(keep[4]), _p(keep[5]),
+                          _p(keep[6]), _p(keep[7]), _p(keep[8]), lpf, *[_p(d) for d in dirs], _p(feat), _p(dens),
+                          _p(d_feat), _p(d_dens))
+    return feat, dens, d_feat, d_dens

@@ -81,3 +81,21 @@ def test_directional_derivative_matches_central_differences(sc, term, eps):
                 for x, dx in zip((R, t, q0, s0), d)]
         fd = (loss(ch.forward(*[s[0] for s in step])) - loss(ch.forward(*[s[1] for s in step]))) / (2 * eps)
         assert abs(jv - fd) <= 1e-5 * max(abs(fd), 1.0), (trial, jv, fd)
+
+
+def test_density_jvp_is_linear_in_opacity_and_sh(sc):
+    """The density sample is linear in the opacities and the SH: the forward-mode
+    derivative along a direction equals the difference of two evaluations."""
+    g = sc["g"]
+    rng = np.random.default_rng(3)
+    args = (sc["valid"], sc["prefix"], sc["list"], sc["samples"], g.pos, g.rot, g.scale)
+    dop = (0.01 * rng.normal(size=g.opacity.shape)).astype(np.float32)
+    dsh = (0.01 * rng.normal(size=g.sh.shape)).astype(np.float32)
+    f0, d0, jf, jd = O.density_jvp(*args, g.opacity, g.sh, sc["lpf"], dopac=dop)
+    f1, d1, _, _ = O.density_jvp(*args, g.opacity + dop, g.sh, sc["lpf"])
+    assert np.abs((d1 - d0) - jd).max() <= 1e-5 * np.abs(jd).max()
+    assert np.abs((f1 - f0) - jf).max() <= 1e-5 * np.abs(jf).max()
+    f0, d0, jf, jd = O.density_jvp(*args, g.opacity, g.sh, sc["lpf"], dsh=dsh)
+    f1, _, _, _ = O.density_jvp(*args, g.opacity, g.sh + dsh, sc["lpf"])
+    assert np.abs(jd).max() == 0.0
+    assert np.abs((f1 - f0) - jf).max() <= 1e-5 * np.abs(jf).max()
