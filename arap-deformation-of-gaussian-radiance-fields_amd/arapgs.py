@@ -659,8 +659,11 @@ class Realign:
     """arapgs_realign: Adam steps of the density-preserving realignment over the
     raw Gaussian parameters (pos, rot, log scales, logit opacities, SH)."""
 
-    def __init__(self, ctx: Context, grid, n, lr_pos=1.6e-4, lr_rot=1e-3, lr_scale=5e-3, lr_opacity=5e-2, lr_sh=2.5e-3,
+    def __init__(self, ctx: Context, grid, n, lr_pos=1e-5, lr_rot=1e-4, lr_scale=5e-4, lr_opacity=5e-3, lr_sh=2.5e-4,
                  beta1=0.9, beta2=0.999, eps=1e-15, w_density=1.0):
+        # Adam moves every parameter by about lr per step whatever the loss scale;
+        # where ~1000 Gaussians overlap a sample their steps add up, so the step
+        # sizes here are a tenth of the 3DGS training rates.
         self.ctx, self.grid = ctx, grid
         d = RealignDesc(lr_pos, lr_rot, lr_scale, lr_opacity, lr_sh, beta1, beta2, eps, w_density)
         h = P()

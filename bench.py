@@ -89,7 +89,7 @@ def setup(args):
                 active=W["active"],
                 solver_args=W["solver_args"], cfg=dict(autotune.DEFAULTS, aggregates=args.aggregates or 128),
                 delta=workload.drag_delta(sc, args.drag_frames), cams=W["cams"], bg=W["bg"], images=images,
-                opac=W["opac"], iters=sc.gn_iters, grid=W["grid"], K=W["K"])
+                opac=W["opac"], iters=sc.gn_iters, grid=W["grid"], K=W["K"], lpf=W["lpf"])
 
 
 def reset_state(B):
@@ -360,6 +360,42 @@ def adjoint_leg(args):
             "check": "vector-Jacobian products vs FP64 forward-mode oracle along 64 directions (tests/test_gpu_chain.py)"}
 
 
+def realign_leg(B, args):
+    """Density-preserving realignment on the main workload after the timed drag:
+    the deformed Gaussians are moved by Adam so that their density at the
+    deformed samples matches the rotated rest features (the F6 loop). One step =
+    activation + density sample + L1 subgradient + density backward + Adam."""
+    import torch
+    load_package()
+    from arapgs_amd import arapgs
+    ctx, st, grid = B["ctx"], B["state"], B["grid"]
+    with torch.cuda.stream(ctx.stream):
+        pos, rot, sh = st["pos"].clone(), st["rot"].clone(), st["sh"].clone()
+        log_scale = torch.log(st["scale"])
+        o = B["opac"].clamp(1e-6, 1 - 1e-6)
+        logit = torch.log(o / (1 - o))
+        _, aim_dens = grid.density(B["state0"]["pos"], B["state0"]["rot"], B["state0"]["scale"], B["opac"],
+                                   B["state0"]["sh"], B["lpf"])
+        r = arapgs.Realign(ctx, grid, pos.shape[0])
+        loss = torch.zeros(1, dtype=torch.float64, device=pos.device)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms, losses = [], []
+    for i in range(args.realign_steps + 1):
+        with torch.cuda.stream(ctx.stream):
+            ev[0].record(ctx.stream)
+            r.step(st["samples"], st["sample_feat"], aim_dens, B["lpf"], pos, rot, log_scale, logit, sh, loss=loss)
+            ev[1].record(ctx.stream)
+        torch.cuda.synchronize()
+        losses.append(float(loss.item()))
+        if i > 0:
+            ms.append(ev[0].elapsed_time(ev[1]))
+    r.close()
+    return {"gaussians": int(pos.shape[0]), "samples": int(st["samples"].shape[0]),
+            "cell_gaussian_pairs": int(grid.n_pairs), "steps": args.realign_steps,
+            "step_p50_ms": round(pct(ms, 50), 3), "l1_loss_first_last": [losses[0], losses[-1]]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,6 +417,7 @@ def main():
                     help="skip the autotuner (tile, blend unroll, aggregates, coarse refresh, CG host checks)")
     ap.add_argument("--tune-steps", type=int, default=6, help="timed frames per autotuner candidate")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
+    ap.add_argument("--realign-steps", type=int, default=5, help="steps of the realignment leg (0: skip)")
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
@@ -468,6 +505,10 @@ def main():
     line["config"]["tuning"] = B["cfg"]
     if tuning:
         line["autotune"] = tuning
+    if args.realign_steps > 0:
+        log(f"timing the realignment leg: {args.realign_steps} steps")
+        line["realign"] = realign_leg(B, args)
+        log(f"  realign: {line['realign']['step_p50_ms']} ms/step, L1 {line['realign']['l1_loss_first_last']}")
     if args.adjoint_steps > 0:
         log(f"timing the differentiation leg: {args.warmup} warmup + {args.adjoint_steps} steps")
         line["adjoint"] = adjoint_leg(args)
