@@ -115,6 +115,9 @@ EXPORTS = {
     "arapgs_chain_forward": ([P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_chain_backward": ([P, P, P, P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_chain_state": ([P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)], C.c_int),
+    "arapgs_ply_info": ([C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int)], C.c_int),
+    "arapgs_ply_load": ([P, C.c_char_p, C.c_int, P, P, P, P, P, P, P], C.c_int),
+    "arapgs_ply_save": ([P, C.c_char_p, C.c_int64, P, P, P, P, P, P, P, P, C.POINTER(C.c_int64)], C.c_int),
 }
 
 
@@ -149,6 +152,15 @@ def ptr(t):
     if not t.is_contiguous():
         raise ArapgsError("arapgs calls take contiguous tensors")
     return C.c_void_p(t.data_ptr())
+
+
+def ply_info(path):
+    """(vertex count, has_index) of a 3DGS PLY (header check only, no GPU)."""
+    n = C.c_int64(0)
+    has = C.c_int(0)
+    if lib().arapgs_ply_info(os.fsencode(path), C.byref(n), C.byref(has)) != 0:
+        raise ArapgsError(f"{path}: not a 3DGS PLY of 62 (or 63 with index) float properties")
+    return n.value, bool(has.value)
 
 
 class Context:
@@ -291,6 +303,45 @@ class Context:
         n = C.c_int(0)
         self._check(lib().arapgs_render_views(self.h, arr, V, C.byref(n)))
         return n.value
+
+    # ------------------------------------------------------------ PLY codec
+    def ply_load(self, path, morton=True):
+        """Loads a 3DGS PLY (loadPly, GaussianView.cpp:43-268) into device
+        tensors with the activations applied; Morton-ordered unless morton is
+        False. Returns dict(pos, rot, scale, opacity, sh (N, 48), index, box)."""
+        import numpy as np
+        t = self._torch
+        n, _ = ply_info(path)
+        dev = t.device("cuda", self.device)
+        out = dict(pos=t.empty((n, 3), dtype=t.float32, device=dev), rot=t.empty((n, 4), dtype=t.float32, device=dev),
+                   scale=t.empty((n, 3), dtype=t.float32, device=dev),
+                   opacity=t.empty(n, dtype=t.float32, device=dev), sh=t.empty((n, 48), dtype=t.float32, device=dev),
+                   index=t.empty(n, dtype=t.int32, device=dev))
+        box = np.zeros(6, np.float32)
+        self._check(lib().arapgs_ply_load(self.h, os.fsencode(path), 1 if morton else 0, ptr(out["pos"]),
+                                          ptr(out["rot"]), ptr(out["scale"]), ptr(out["opacity"]), ptr(out["sh"]),
+                                          ptr(out["index"]), box.ctypes.data_as(P)))
+        out["box"] = box
+        return out
+
+    def ply_save(self, path, pos, rot, scale, opacity, sh, box=None, too_small=None):
+        """Writes the Gaussians inside box ((min xyz), (max xyz)) and not flagged
+        in too_small (device bool/uint8) as a 3DGS PLY (savePly,
+        GaussianView.cpp:270-339). Returns the number written."""
+        import numpy as np
+        t = self._torch
+        lo = hi = None
+        if box is not None:
+            lo = np.ascontiguousarray(box[0], np.float32)
+            hi = np.ascontiguousarray(box[1], np.float32)
+        ts = None
+        if too_small is not None:
+            ts = too_small.to(t.uint8).contiguous()
+        written = C.c_int64(0)
+        self._check(lib().arapgs_ply_save(self.h, os.fsencode(path), pos.shape[0], ptr(pos), ptr(rot), ptr(scale),
+                                          ptr(opacity), ptr(sh), None if lo is None else lo.ctypes.data_as(P),
+                                          None if hi is None else hi.ctypes.data_as(P), ptr(ts), C.byref(written)))
+        return written.value
 
     # ------------------------------------------------------------ grid
     def scene_aabb(self, pos, G):

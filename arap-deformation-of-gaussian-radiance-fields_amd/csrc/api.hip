@@ -12,6 +12,7 @@
 #include "adjoint.hpp"
 #include "common.hpp"
 #include "grid.hpp"
+#include "ply.hpp"
 #include "raster.hpp"
 #include "solver.hpp"
 #include "sort.hpp"
@@ -1318,6 +1319,122 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
                                        logit_opacity, sh, r->m, r->v, lr, r->d.beta1, r->d.beta2, r->d.eps, bc1, bc2,
                                        st),
                         "realign adam");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- PLY codec
+extern "C" {
+
+int arapgs_ply_info(const char* path, int64_t* count_h, int* has_index_h) {
+    if (!path) return -1;
+    PlyHeader h;
+    std::string err;
+    if (ply_parse_header(path, &h, &err)) return -1;
+    if (count_h) *count_h = h.count;
+    if (has_index_h) *has_index_h = h.has_index;
+    return 0;
+}
+
+int arapgs_ply_load(arapgs_ctx* c, const char* path, int morton, float* pos, float* rot, float* scale, float* opacity,
+                    float* sh, int* index, float* minmax6_h) {
+    CHECK_CTX(c);
+    if (!path) return fail(c, -1, "ply_load: bad arguments");
+    PlyHeader h;
+    std::string err;
+    if (ply_parse_header(path, &h, &err)) return fail(c, -1, "ply_load: %s", err.c_str());
+    const int64_t n = h.count;
+    if (n == 0) return 0;
+    if (!pos || !rot || !scale || !opacity || !sh) return fail(c, -1, "ply_load: bad arguments");
+    if (n > 0xffffffffll) return fail(c, -1, "ply_load: too many vertices");
+    std::vector<float> recs((size_t)n * h.stride);
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(c, -1, "ply_load: cannot open %s", path);
+    std::fseek(f, h.data_offset, SEEK_SET);
+    const size_t got = std::fread(recs.data(), sizeof(float) * h.stride, (size_t)n, f);
+    std::fclose(f);
+    if (got != (size_t)n) return fail(c, -1, "ply_load: %s is truncated (%zu of %lld vertices)", path, got, (long long)n);
+    (void)hipSetDevice(c->device);
+    float* drec = nullptr;
+    uint32_t* ws = nullptr;
+    float* mm = nullptr;
+    hipError_t e = hipMalloc(&drec, sizeof(float) * recs.size());
+    if (e == hipSuccess) e = hipMalloc(&ws, sizeof(uint32_t) * ply_workspace_words(n));
+    if (e == hipSuccess) e = hipMalloc(&mm, sizeof(float) * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(drec, recs.data(), sizeof(float) * recs.size(), hipMemcpyHostToDevice, c->stream);
+    int rc = 0;
+    if (e != hipSuccess) rc = hip_fail(c, e, "ply_load");
+    if (!rc)
+        rc = check_launch(c, launch_ply_unpack(n, h.stride, drec, morton, ws, mm, pos, rot, scale, opacity, sh, index,
+                                               c->stream),
+                          "ply_load");
+    float mm_h[6] = {0, 0, 0, 0, 0, 0};
+    if (!rc && morton) {
+        e = hipMemcpyAsync(mm_h, mm, sizeof(mm_h), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "ply_load");
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(c, e, "ply_load");
+    if (!rc && minmax6_h) std::memcpy(minmax6_h, mm_h, sizeof(mm_h));
+    (void)hipFree(drec);
+    (void)hipFree(ws);
+    (void)hipFree(mm);
+    return rc;
+}
+
+int arapgs_ply_save(arapgs_ctx* c, const char* path, int64_t n, const float* pos, const float* rot, const float* scale,
+                    const float* opacity, const float* sh, const float* boxmin_h, const float* boxmax_h,
+                    const uint8_t* too_small, int64_t* written_h) {
+    CHECK_CTX(c);
+    if (!path || n < 0 || (n > 0 && (!pos || !rot || !scale || !opacity || !sh)))
+        return fail(c, -1, "ply_save: bad arguments");
+    if (n > 0xffffffffll) return fail(c, -1, "ply_save: too many Gaussians");
+    (void)hipSetDevice(c->device);
+    float* drec = nullptr;
+    uint32_t* ws = nullptr;
+    uint32_t* dcount = nullptr;
+    uint32_t count = 0;
+    hipError_t e = hipSuccess;
+    int rc = 0;
+    std::vector<float> recs;
+    if (n > 0) {
+        e = hipMalloc(&drec, sizeof(float) * 62 * (size_t)n);
+        if (e == hipSuccess) e = hipMalloc(&ws, sizeof(uint32_t) * ply_pack_workspace_words(n));
+        if (e == hipSuccess) e = hipMalloc(&dcount, sizeof(uint32_t) * 4);
+        if (e != hipSuccess) rc = hip_fail(c, e, "ply_save");
+        if (!rc)
+            rc = check_launch(c, launch_ply_pack(n, pos, rot, scale, opacity, sh, boxmin_h, boxmax_h, too_small, ws,
+                                                 drec, dcount, c->stream),
+                              "ply_save");
+        if (!rc) {
+            e = hipMemcpyAsync(&count, dcount, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(c, e, "ply_save");
+        }
+        if (!rc) {
+            recs.resize((size_t)count * 62);
+            e = hipMemcpy(recs.data(), drec, sizeof(float) * recs.size(), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(c, e, "ply_save");
+        }
+        (void)hipFree(drec);
+        (void)hipFree(ws);
+        (void)hipFree(dcount);
+        if (rc) return rc;
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(c, -1, "ply_save: cannot create %s", path);
+    std::fprintf(f, "ply\nformat binary_little_endian 1.0\nelement vertex %u\n", count);
+    const char* head[] = {"x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"};
+    for (const char* p : head) std::fprintf(f, "property float %s\n", p);
+    for (int i = 0; i < 45; i++) std::fprintf(f, "property float f_rest_%d\n", i);
+    const char* tail[] = {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"};
+    for (const char* p : tail) std::fprintf(f, "property float %s\n", p);
+    std::fprintf(f, "end_header\n");
+    const size_t put = recs.empty() ? 0 : std::fwrite(recs.data(), sizeof(float) * 62, count, f);
+    std::fclose(f);
+    if (put != count) return fail(c, -1, "ply_save: short write to %s", path);
+    if (written_h) *written_h = count;
+    return 0;
 }
 
 }  // extern "C"

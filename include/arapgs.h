@@ -423,6 +423,30 @@ int arapgs_chain_backward(arapgs_ctx* ctx, arapgs_chain* ch, const arapgs_node_x
 int arapgs_chain_state(const arapgs_chain* ch, const float** pos, const float** rot, const float** scale,
                        const float** sh, const float** samples);
 
+/* ------------------------------------------------------------ PLY codec */
+/* The 3DGS PLY of loadPly / savePly (GaussianView.cpp:43-339): binary little-
+ * endian, 62 float properties per vertex (x y z nx ny nz f_dc_0..2
+ * f_rest_0..44 opacity scale_0..2 rot_0..3), or 63 with a trailing `index`
+ * (the RichPoint_Soup variant). Header check only, no GPU: */
+int arapgs_ply_info(const char* path, int64_t* count_h, int* has_index_h);
+/* Reads the file into device SoA with the reference's activations (unit
+ * quaternion, exp(scale), sigmoid(opacity), SH interleaved coefficient * 3 +
+ * channel). morton != 0 reorders by the 3D Morton code of the positions over
+ * their bounding box (21 bits per axis, as the reference; ties keep file order,
+ * where the reference's std::sort leaves them unspecified) and returns the box
+ * in minmax6_h (host, optional). index (device int[count], optional): the
+ * `index` property, or each vertex's position in the file. Buffers sized for
+ * arapgs_ply_info's count. Synchronous. */
+int arapgs_ply_load(arapgs_ctx* ctx, const char* path, int morton, float* pos, float* rot, float* scale,
+                    float* opacity, float* sh, int* index, float* minmax6_h);
+/* Writes the Gaussians inside the crop box (host float[3] each, optional) and
+ * not flagged in too_small (device u8[n], optional), in their order, with the
+ * inverse activations (log(scale), logit(opacity)) and normals 0.
+ * *written_h receives the vertex count. Synchronous. */
+int arapgs_ply_save(arapgs_ctx* ctx, const char* path, int64_t n, const float* pos, const float* rot,
+                    const float* scale, const float* opacity, const float* sh, const float* boxmin_h,
+                    const float* boxmax_h, const uint8_t* too_small, int64_t* written_h);
+
 /* ------------------------------------------------------------ test hooks */
 /* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the
  * GaussianView.cpp:3134-3156 sign wrapper); R9 f32[n*9] row-major. */

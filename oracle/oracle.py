@@ -426,3 +426,107 @@ def density_jvp(valid, prefix, lst, samples, pos, rot, scale, opac, sh, lpf, dpo
                           _p(keep[6]), _p(keep[7]), _p(keep[8]), lpf, *[_p(d) for d in dirs], _p(feat), _p(dens),
                           _p(d_feat), _p(d_dens))
     return feat, dens, d_feat, d_dens
+
+
+# ---------------------------------------------------------------- PLY codec
+PLY_PROPS = (["x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"] + [f"f_rest_{i}" for i in range(45)] +
+             ["opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"])
+
+
+def ply_write(path, recs):
+    """Writes float32 records (n x 62, or n x 63 with a trailing index) as a
+    binary little-endian PLY with the 3DGS header (savePly's header,
+    GaussianView.cpp:296-307)."""
+    recs = np.ascontiguousarray(recs, dtype="<f4")
+    props = PLY_PROPS + (["index"] if recs.shape[1] == 63 else [])
+    head = f"ply\nformat binary_little_endian 1.0\nelement vertex {recs.shape[0]}\n"
+    head += "".join(f"property float {p}\n" for p in props) + "end_header\n"
+    with open(path, "wb") as f:
+        f.write(head.encode())
+        f.write(recs.tobytes())
+
+
+def ply_read(path):
+    """(records float32 (n x stride), stride) of a 3DGS PLY; the header walk of
+    loadPly (GaussianView.cpp:53-76) with the property list checked."""
+    with open(path, "rb") as f:
+        data = f.read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    lines = data[:end].decode().splitlines()
+    n = int(lines[2].split()[2])
+    props = [ln.split()[2] for ln in lines if ln.startswith("property float ")]
+    assert props[:62] == PLY_PROPS and len(props) in (62, 63)
+    stride = len(props)
+    return np.frombuffer(data, dtype="<f4", count=n * stride, offset=end).reshape(n, stride), stride
+
+
+def morton_codes(pos):
+    """63-bit Morton codes of float32 positions over their bounding box, 21 bits
+    per axis of trunc((2^21 - 1) * (p - min) / (max - min)), x in bit 3b, y in
+    3b + 1, z in 3b + 2 (GaussianView.cpp:205-225). Returns (codes, min, max)."""
+    pos = np.asarray(pos, np.float32)
+    mn, mx = pos.min(0), pos.max(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rel = (pos - mn) / (mx - mn)
+        scaled = np.float32((1 << 21) - 1) * rel
+    xyz = np.where(np.isfinite(scaled), scaled, 0).astype(np.int64)
+    code = np.zeros(pos.shape[0], np.uint64)
+    for b in range(21):
+        for a in range(3):
+            code |= ((xyz[:, a] >> b) & 1).astype(np.uint64) << np.uint64(3 * b + a)
+    return code, mn, mx
+
+
+def ply_load(recs, morton=True):
+    """Activated SoA of PLY records (loadPly, GaussianView.cpp:226-266): Morton
+    order (ties in file order), unit quaternion, exp(scale), sigmoid(opacity),
+    SH interleaved as coefficient * 3 + channel; index: the `index` property
+    rounded half away from zero, or the file position."""
+    recs = np.asarray(recs, np.float32)
+    n, stride = recs.shape
+    if morton and n:
+        code, mn, mx = morton_codes(recs[:, 0:3])
+        order = np.argsort(code, kind="stable")
+        box = np.concatenate([mn, mx])
+    else:
+        order = np.arange(n)
+        box = np.zeros(6, np.float32)
+    r = recs[order]
+    q = r[:, 58:62]
+    length = np.sqrt((q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1]) + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+    sh = np.zeros((n, 16, 3), np.float32)
+    sh[:, 0] = r[:, 6:9]
+    for c in range(3):
+        sh[:, 1:, c] = r[:, 9 + 15 * c: 24 + 15 * c]
+    if stride == 63:
+        index = (np.sign(r[:, 62]) * np.floor(np.abs(r[:, 62]) + 0.5)).astype(np.int32)
+    else:
+        index = order.astype(np.int32)
+    return dict(pos=r[:, 0:3].copy(), rot=q / length[:, None], scale=np.exp(r[:, 55:58]),
+                opacity=(1.0 / (1.0 + np.exp(-r[:, 54]))).astype(np.float32), sh=sh.reshape(n, 48), index=index,
+                box=box)
+
+
+def ply_save_records(pos, rot, scale, opacity, sh, box=None, too_small=None):
+    """62-float records savePly writes (GaussianView.cpp:270-339): the Gaussians
+    inside the box (inclusive) and not too small, in order, normals 0,
+    log(scale), log(o / (1 - o)), SH back to channel-major."""
+    pos = np.asarray(pos, np.float32)
+    keep = np.ones(pos.shape[0], bool)
+    if box is not None:
+        lo, hi = np.asarray(box[0], np.float32), np.asarray(box[1], np.float32)
+        keep &= ~((pos < lo) | (pos > hi)).any(1)
+    if too_small is not None:
+        keep &= ~np.asarray(too_small, bool)
+    sh = np.asarray(sh, np.float32).reshape(-1, 16, 3)[keep]
+    o = np.asarray(opacity, np.float32)[keep]
+    m = int(keep.sum())
+    r = np.zeros((m, 62), np.float32)
+    r[:, 0:3] = pos[keep]
+    r[:, 6:9] = sh[:, 0]
+    for c in range(3):
+        r[:, 9 + 15 * c: 24 + 15 * c] = sh[:, 1:, c]
+    r[:, 54] = np.log(o / (np.float32(1.0) - o))
+    r[:, 55:58] = np.log(np.asarray(scale, np.float32)[keep])
+    r[:, 58:62] = np.asarray(rot, np.float32)[keep]
+    return r
