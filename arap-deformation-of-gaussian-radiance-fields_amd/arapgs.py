@@ -443,7 +443,8 @@ class Solver:
     """arapgs_solver for one block layout (Deform::Deform, Deform.hpp:414-455)."""
 
     def __init__(self, ctx: Context, nbr, blocks, block_types, src_idx, src_w, node_pos, w_rot=1.0, w_reg=10.0,
-                 w_con=100.0, weighting_factor=1.0, n_aggregates=0, cg_max_iters=0, cg_tol=0.0):
+                 w_con=100.0, weighting_factor=1.0, n_aggregates=0, cg_max_iters=0, cg_tol=0.0,
+                 constraints_on_center=False):
         import numpy as np
         self.ctx = ctx
         nbr = np.ascontiguousarray(nbr, np.uint32)
@@ -465,7 +466,8 @@ class Solver:
         d = SolverDesc(M, K, kp["nbr"].ctypes.data_as(P), kp["bptr"].ctypes.data_as(P),
                        kp["bnodes"].ctypes.data_as(P), kp["btype"].ctypes.data_as(P), len(blocks),
                        kp["src_idx"].ctypes.data_as(P), kp["src_w"].ctypes.data_as(P), kp["pos"].ctypes.data_as(P),
-                       w_rot, w_reg, w_con, weighting_factor, 0, n_aggregates, cg_max_iters, cg_tol)
+                       w_rot, w_reg, w_con, weighting_factor, int(bool(constraints_on_center)), n_aggregates,
+                       cg_max_iters, cg_tol)
         h = P()
         ctx._check(lib().arapgs_solver_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -5,7 +5,11 @@
 //   rot(i)      6 rows  {c0.c1, c0.c2, c1.c2, |c0|^2-1, |c1|^2-1, |c2|^2-1}
 //   reg(i,t)    3 rows  s_ik (R_i d + g_i + t_i - g_k - t_k), d = g_k - g_i
 //   static(i,t) 3 rows  -t_k   (static i, free k)
-//   con(c)      3 rows  sum_j w_j (free j ? R_j (v - g_j) + g_j + t_j : v) - aim_c
+//   con(c)      3 rows  sum over members v of
+//                          [sum_j w_j (free j ? R_j (v - g_j) + g_j + t_j : v) - centre_c]
+//                       plain mode: one member, centre = its aim; centre mode
+//                       (Deform.cpp:483-529): the block's key nodes, centre =
+//                       the float mean of their aims
 // The Jacobian keeps the reference's asymmetry: d reg / d t_i carries no s_ik
 // (Deform.cpp:257 vs :261).
 //
@@ -104,30 +108,42 @@ __device__ __forceinline__ int group_residual_rows(const SolverDev& S, const flo
         return 3;
     }
     g -= (int64_t)S.nS * S.K;
-    // constraint
+    // constraint group: sum over its members of sw_con (blend(member) - centre)
     const int cid = (int)g;
-    const int c = S.con_node[cid];
-    const double v0 = S.g[3 * c], v1 = S.g[3 * c + 1], v2 = S.g[3 * c + 2];
-    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
-    for (int j = 0; j < S.K; j++) {
-        const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
-        const double w = S.src_w[(int64_t)c * S.K + j];
-        const int fj = S.free_id[nj];
-        if (fj < 0) {
-            n0 += w * v0;
-            n1 += w * v1;
-            n2 += w * v2;
-        } else {
-            const float* gj = S.g + 3 * nj;
-            const double d0 = v0 - gj[0], d1 = v1 - gj[1], d2 = v2 - gj[2];
-            n0 += w * (X(fj, 0) * d0 + X(fj, 3) * d1 + X(fj, 6) * d2 + (double)gj[0] + X(fj, 9));
-            n1 += w * (X(fj, 1) * d0 + X(fj, 4) * d1 + X(fj, 7) * d2 + (double)gj[1] + X(fj, 10));
-            n2 += w * (X(fj, 2) * d0 + X(fj, 5) * d1 + X(fj, 8) * d2 + (double)gj[2] + X(fj, 11));
-        }
+    float ctr[3] = {0.0f, 0.0f, 0.0f};
+    const int ab = S.caim_ptr[cid], ae = S.caim_ptr[cid + 1];
+    for (int q = ab; q < ae; q++) {
+        const int a = S.caim_node[q];
+#pragma unroll
+        for (int r = 0; r < 3; r++) ctr[r] += S.aim[3 * a + r];
     }
-    f0 = sw_con * (n0 - (double)S.aim[3 * c]);
-    f1 = sw_con * (n1 - (double)S.aim[3 * c + 1]);
-    f2 = sw_con * (n2 - (double)S.aim[3 * c + 2]);
+#pragma unroll
+    for (int r = 0; r < 3; r++) ctr[r] = ctr[r] / (float)(ae - ab);
+    f0 = f1 = f2 = 0.0;
+    for (int q = S.con_ptr[cid]; q < S.con_ptr[cid + 1]; q++) {
+        const int c = S.con_mem[q];
+        const double v0 = S.g[3 * c], v1 = S.g[3 * c + 1], v2 = S.g[3 * c + 2];
+        double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+        for (int j = 0; j < S.K; j++) {
+            const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+            const double w = S.src_w[(int64_t)c * S.K + j];
+            const int fj = S.free_id[nj];
+            if (fj < 0) {
+                n0 += w * v0;
+                n1 += w * v1;
+                n2 += w * v2;
+            } else {
+                const float* gj = S.g + 3 * nj;
+                const double d0 = v0 - gj[0], d1 = v1 - gj[1], d2 = v2 - gj[2];
+                n0 += w * (X(fj, 0) * d0 + X(fj, 3) * d1 + X(fj, 6) * d2 + (double)gj[0] + X(fj, 9));
+                n1 += w * (X(fj, 1) * d0 + X(fj, 4) * d1 + X(fj, 7) * d2 + (double)gj[1] + X(fj, 10));
+                n2 += w * (X(fj, 2) * d0 + X(fj, 5) * d1 + X(fj, 8) * d2 + (double)gj[2] + X(fj, 11));
+            }
+        }
+        f0 += sw_con * (n0 - (double)ctr[0]);
+        f1 += sw_con * (n1 - (double)ctr[1]);
+        f2 += sw_con * (n2 - (double)ctr[2]);
+    }
     f3 = f4 = f5 = 0.0;
     return 3;
 }
@@ -288,33 +304,39 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
     item -= n_in;
     const int ent = S.cm_ptr[fi] + item;
     const int cid = S.cm_con[ent];
-    const int slot = S.cm_slot[ent];
-    const int c = S.con_node[cid];
-    const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
     {
-        const float wi = S.src_w[(int64_t)c * S.K + slot];
+        // J of the group wrt node i: sum over the (member, slot) entries that reach i
         const float* gi = S.g + 3 * i;
-        const float dd[3] = {vv[0] - gi[0], vv[1] - gi[1], vv[2] - gi[2]};
+        for (int q = S.cm_sbeg[ent]; q < S.cm_sbeg[ent + 1]; q++) {
+            const int e = S.cm_src[q];
+            const int c = e / S.K;
+            const float wi = S.src_w[e];
+            const float dd[3] = {S.g[3 * c] - gi[0], S.g[3 * c + 1] - gi[1], S.g[3 * c + 2] - gi[2]};
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
+            for (int r = 0; r < 3; r++) {
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++) Ji[r][3 * cc + r] = S.sw_con * wi * dd[cc];
-            Ji[r][9 + r] = S.sw_con * wi;
+                for (int cc = 0; cc < 3; cc++) Ji[r][3 * cc + r] += S.sw_con * wi * dd[cc];
+                Ji[r][9 + r] += S.sw_con * wi;
+            }
         }
     }
     const int KK = KT > 0 ? KT : S.K;
+    for (int m = S.con_ptr[cid]; m < S.con_ptr[cid + 1]; m++) {
+        const int c = S.con_mem[m];
+        const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
 #pragma unroll
-    for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
-        if (KT == 0 && j >= KK) break;
-        const int nj = (int)S.src_idx[(int64_t)c * KK + j];
-        const int fj = S.free_id[nj];
-        if (fj < 0) continue;
-        const float w = S.src_w[(int64_t)c * KK + j];
-        const float* gj = S.g + 3 * nj;
-        const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
-        const int64_t jo = 12 * (int64_t)fj;
+        for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
+            if (KT == 0 && j >= KK) break;
+            const int nj = (int)S.src_idx[(int64_t)c * KK + j];
+            const int fj = S.free_id[nj];
+            if (fj < 0) continue;
+            const float w = S.src_w[(int64_t)c * KK + j];
+            const float* gj = S.g + 3 * nj;
+            const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
+            const int64_t jo = 12 * (int64_t)fj;
 #pragma unroll
-        for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
+            for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
+        }
     }
     fn(3, Ji, Jv);
 }
@@ -890,6 +912,66 @@ __global__ void coarse_to_float_kernel(const unsigned long long* __restrict__ Ac
         Ac[i] = (float)((double)(long long)Ac64[i] / COARSE_FX);
 }
 
+// Coarse contribution of the constraint groups with several members (centre
+// mode, one per control block). Every (member, slot) entry reaching a free node
+// of aggregate a adds to the aggregate's projected row coefficients
+//   R part: sw_con w (v_m - g_j) + sw_con w (g_j - c_a),   t part: sw_con w,
+// the same per-entry values as the one-member path, summed by one thread in
+// entry order (deterministic); then every pair of touched aggregates adds its
+// 3-row outer product. One workgroup per group; LDS: 24 bytes per aggregate.
+__global__ void __launch_bounds__(256) coarse_multi_kernel(SolverDev S, unsigned long long* __restrict__ Ac) {
+    extern __shared__ float lds[];
+    float* coef = lds;                                   // 4 per aggregate
+    int* list = reinterpret_cast<int*>(coef + 4 * S.n_agg);  // touched aggregates
+    int* mark = list + S.n_agg;
+    __shared__ int nt;
+    for (int i = threadIdx.x; i < 4 * S.n_agg; i += blockDim.x) coef[i] = 0.0f;
+    for (int i = threadIdx.x; i < S.n_agg; i += blockDim.x) mark[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int cid = S.multi_con[blockIdx.x];
+        int cnt = 0;
+        for (int m = S.con_ptr[cid]; m < S.con_ptr[cid + 1]; m++) {
+            const int c = S.con_mem[m];
+            const float v[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
+            for (int j = 0; j < S.K; j++) {
+                const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
+                const int fj = S.free_id[nj];
+                if (fj < 0) continue;
+                const float w = S.src_w[(int64_t)c * S.K + j];
+                const int a = S.agg_of[fj];
+                const float jt = S.sw_con * w;
+                for (int cc = 0; cc < 3; cc++) {
+                    const float dd = v[cc] - S.g[3 * nj + cc];
+                    const float dj = S.g[3 * nj + cc] - S.agg_center[3 * a + cc];
+                    coef[4 * a + cc] += S.sw_con * w * dd + jt * dj;
+                }
+                coef[4 * a + 3] += jt;
+                if (!mark[a]) {
+                    mark[a] = 1;
+                    list[cnt++] = a;
+                }
+            }
+        }
+        nt = cnt;
+    }
+    __syncthreads();
+    const int nc = S.nc_pad;
+    const int n = nt;
+    for (int p = threadIdx.x; p < n * n; p += blockDim.x) {
+        const int a1 = list[p / n], a2 = list[p % n];
+        for (int r = 0; r < 3; r++)
+            for (int p1 = 0; p1 < 4; p1++)
+                for (int p2 = 0; p2 < 4; p2++) {
+                    const float val = coef[4 * a1 + p1] * coef[4 * a2 + p2];
+                    if (val == 0.0f) continue;
+                    const int u = p1 < 3 ? 3 * p1 + r : 9 + r;
+                    const int w = p2 < 3 ? 3 * p2 + r : 9 + r;
+                    coarse_add(Ac, (int64_t)(12 * a1 + u) * nc + 12 * a2 + w, val);
+                }
+    }
+}
+
 __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
                                                               unsigned long long* __restrict__ Ac) {
     const int nc = S.nc_pad;
@@ -959,7 +1041,9 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
                 nn++;
             } else {
                 gg -= (int64_t)S.nS * S.K;
-                const int c = S.con_node[(int)gg];
+                // groups of several members: coarse_multi_kernel
+                if (S.con_ptr[gg + 1] - S.con_ptr[gg] != 1) continue;
+                const int c = S.con_mem[S.con_ptr[gg]];
                 const float v[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
                 for (int j = 0; j < S.K; j++) {
                     const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
@@ -1203,6 +1287,8 @@ int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
         hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, reinterpret_cast<float*>(W.ac64), 2 * n2,
                            (const int*)nullptr);
         hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.ac64);
+        if (S.n_multi > 0)
+            hipLaunchKernelGGL(coarse_multi_kernel, dim3(S.n_multi), dim3(256), (size_t)24 * S.n_agg, st, S, W.ac64);
         hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256), dim3(256), 0, st, (const unsigned long long*)W.ac64,
                            W.acinv, n2);
         const int np = S.nc_pad;

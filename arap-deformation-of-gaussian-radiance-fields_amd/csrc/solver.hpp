@@ -24,12 +24,22 @@ struct SolverDev {
     const float* edge_scale;   // M*K  weighting factor of (i, t) when i and k share a non-excluded block
     const uint32_t* src_idx;   // M*K  KNN of each node's source vertex
     const float* src_w;        // M*K
-    const int* con_node;       // nC   control node of each constraint group
+    // Constraint group c: rows sw_con * sum over members m of (blend(m) - centre_c),
+    // centre_c = (float) sum of the aims of its aim nodes / their count. Plain
+    // mode: one member, aim node = the member. Centre mode: the ascending key
+    // nodes of a block and its first min(20, |block|) nodes.
+    const int* con_ptr;        // nC+1 members of each group
+    const int* con_mem;
+    const int* caim_ptr;       // nC+1 aim nodes of each group
+    const int* caim_node;
     const int* in_ptr;         // nF+1 in-edges (src*K + t) of each free node
     const int* in_edge;
-    const int* cm_ptr;         // nF+1 constraint memberships of each free node
-    const int* cm_con;
-    const int* cm_slot;
+    const int* cm_ptr;         // nF+1 constraint memberships of each free node, one per group
+    const int* cm_con;         // group of each membership
+    const int* cm_sbeg;        // (memberships+1) range of the membership in cm_src
+    const int* cm_src;         // source-weight entries (member*K + slot) that reach the free node
+    const int* multi_con;      // n_multi groups with more than one member
+    int n_multi;
     const int* agg_ptr;        // n_agg+1 free ranks grouped by aggregate
     const int* agg_member;
     const int* agg_of;         // nF

@@ -45,7 +45,6 @@ void solver_free(SolverWork& W) {
 int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     const int M = d.M, K = d.K;
     if (M <= 0 || K <= 0 || K > KNN_MAX_T) { err = "solver: need M > 0 and 1 <= K <= 12"; return -1; }
-    if (d.center) { err = "solver: constraints-on-centre mode is not supported"; return -1; }
     // static = nodes of blocks with type < 0 (UpdateIndicies)
     std::vector<char> is_static(M, 0);
     std::vector<std::vector<int>> node_blocks(M);
@@ -88,27 +87,67 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
         in_ptr[f + 1] = in_ptr[f] + (int)ins[f].size();
         in_edge.insert(in_edge.end(), ins[f].begin(), ins[f].end());
     }
-    // constraints: every node of every non-excluded block, in block order
-    std::vector<int> con_node;
+    // constraint groups of the non-excluded blocks (type != -1), in block order:
+    // plain mode, one per block node (CalcEnergyFunc, Deform.cpp:531-579);
+    // centre mode, one per block (Deform.cpp:483-529) whose members are the key
+    // controls of SelectKeyControls (Deform.cpp:34-75): the reference samples
+    // min(20, |block|) nodes by FPS but keeps the FIRST that many block nodes
+    // (c_indices[i]), as an ascending set; the aim centre averages their aims.
+    std::vector<int> con_ptr{0}, con_mem, caim_ptr{0}, caim_node, multi_con;
     for (int b = 0; b < d.nblocks; b++) {
         if (d.block_type[b] == -1) continue;
-        for (int q = d.block_ptr[b]; q < d.block_ptr[b + 1]; q++) con_node.push_back(d.block_nodes[q]);
+        const int beg = d.block_ptr[b], end = d.block_ptr[b + 1];
+        if (!d.center) {
+            for (int q = beg; q < end; q++) {
+                con_mem.push_back(d.block_nodes[q]);
+                con_ptr.push_back((int)con_mem.size());
+                caim_node.push_back(d.block_nodes[q]);
+                caim_ptr.push_back((int)caim_node.size());
+            }
+        } else {
+            if (end == beg) { err = "solver: empty control block in centre mode"; return -1; }
+            const int cnt = std::min(20, end - beg);
+            std::vector<int> keys(d.block_nodes + beg, d.block_nodes + beg + cnt);
+            caim_node.insert(caim_node.end(), keys.begin(), keys.end());
+            std::sort(keys.begin(), keys.end());
+            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+            if (keys.size() > 1) multi_con.push_back((int)con_ptr.size() - 1);
+            con_mem.insert(con_mem.end(), keys.begin(), keys.end());
+            con_ptr.push_back((int)con_mem.size());
+            caim_ptr.push_back((int)caim_node.size());
+        }
     }
-    const int nC = (int)con_node.size();
+    const int nC = (int)con_ptr.size() - 1;
+    // memberships: one per (free node, group), listing the source-weight
+    // entries member*K + slot through which the group's rows reach the node
     std::vector<std::vector<std::pair<int, int>>> cms(nF);
     for (int c = 0; c < nC; c++)
-        for (int j = 0; j < K; j++) {
-            const int nj = (int)d.src_idx[(size_t)con_node[c] * K + j];
-            if (free_id[nj] >= 0) cms[free_id[nj]].push_back({c, j});
-        }
-    std::vector<int> cm_ptr(nF + 1, 0), cm_con, cm_slot;
+        for (int q = con_ptr[c]; q < con_ptr[c + 1]; q++)
+            for (int j = 0; j < K; j++) {
+                const int e = con_mem[q] * K + j;
+                const int nj = (int)d.src_idx[e];
+                if (free_id[nj] >= 0) cms[free_id[nj]].push_back({c, e});
+            }
+    std::vector<int> cm_ptr(nF + 1, 0), cm_con, cm_sbeg, cm_src;
     for (int f = 0; f < nF; f++) {
-        cm_ptr[f + 1] = cm_ptr[f] + (int)cms[f].size();
-        for (auto& pr : cms[f]) { cm_con.push_back(pr.first); cm_slot.push_back(pr.second); }
+        auto& v = cms[f];
+        std::stable_sort(v.begin(), v.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+            return a.first < b.first;
+        });
+        for (size_t q = 0; q < v.size(); q++) {
+            if (q == 0 || v[q].first != v[q - 1].first) {
+                cm_con.push_back(v[q].first);
+                cm_sbeg.push_back((int)cm_src.size());
+            }
+            cm_src.push_back(v[q].second);
+        }
+        cm_ptr[f + 1] = (int)cm_con.size();
     }
+    cm_sbeg.push_back((int)cm_src.size());
     // aggregates: farthest-point seeds over the free nodes' positions, then
     // nearest-seed assignment
     const int n_agg = std::max(1, std::min(d.n_agg > 0 ? d.n_agg : 128, nF));
+    if (!multi_con.empty() && n_agg > 2048) { err = "solver: centre mode supports at most 2048 aggregates"; return -1; }
     std::vector<int> seeds;
     {
         std::vector<float> dist(nF, 3.4e38f);
@@ -160,12 +199,18 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     S.edge_scale = upload(W, edge_scale, rc);
     S.src_idx = upload(W, std::vector<uint32_t>(d.src_idx, d.src_idx + (size_t)M * K), rc);
     S.src_w = upload(W, std::vector<float>(d.src_w, d.src_w + (size_t)M * K), rc);
-    S.con_node = upload(W, con_node, rc);
+    S.con_ptr = upload(W, con_ptr, rc);
+    S.con_mem = upload(W, con_mem, rc);
+    S.caim_ptr = upload(W, caim_ptr, rc);
+    S.caim_node = upload(W, caim_node, rc);
     S.in_ptr = upload(W, in_ptr, rc);
     S.in_edge = upload(W, in_edge, rc);
     S.cm_ptr = upload(W, cm_ptr, rc);
     S.cm_con = upload(W, cm_con, rc);
-    S.cm_slot = upload(W, cm_slot, rc);
+    S.cm_sbeg = upload(W, cm_sbeg, rc);
+    S.cm_src = upload(W, cm_src, rc);
+    S.multi_con = upload(W, multi_con, rc);
+    S.n_multi = (int)multi_con.size();
     S.agg_ptr = upload(W, agg_ptr, rc);
     S.agg_member = upload(W, agg_member, rc);
     S.agg_of = upload(W, agg_of, rc);

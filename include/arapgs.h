@@ -93,7 +93,9 @@ typedef struct arapgs_solver_desc {
     const float* node_pos_h;
     float w_rot, w_reg, w_con;   /* DeformGraph weights (Deform.hpp:54-59): 1, 10, 100 */
     float weighting_factor;      /* GaussianView::_weighting_factor (GaussianView.hpp:459) */
-    int constraints_on_center;   /* must be 0 (centre mode not supported) */
+    int constraints_on_center;   /* Deform::_constraints_on_center: one 3-row constraint per control
+                                    block, on the centre of its first min(20, |block|) nodes
+                                    (Deform.cpp:34-75, 483-529) */
     int n_aggregates;            /* coarse aggregates, 0 = 128 */
     int cg_max_iters;            /* 0 = 100 */
     float cg_tol;                /* relative residual, 0 = 1e-4 */

@@ -21,7 +21,7 @@
  * types below with the same field names; errors (which the reference does not
  * report) throw arapgs::dropin::Error; the three scratch-buffer allocators of
  * Rasterizer::forward are accepted and not called (the context owns growable
- * workspaces); the constraints-on-centre mode is rejected.
+ * workspaces).
  */
 #ifndef ARAPGS_DROPIN_HPP
 #define ARAPGS_DROPIN_HPP
@@ -305,7 +305,6 @@ public:
         (void)control_idx;
         (void)static_idx;
         (void)show_efficiency;
-        if (cons_on_center) throw Error("Deform: the constraints-on-centre mode is not supported");
         if (knn_k != deform_graph.k_nearest) throw Error("Deform: knn_k differs from the graph's k_nearest");
         const auto& nodes = deform_graph.nodes;
         const int m = (int)nodes.size(), k = knn_k;
@@ -344,6 +343,7 @@ public:
         d.w_reg = (float)deform_graph.w_reg;
         d.w_con = (float)deform_graph.w_con;
         d.weighting_factor = weighting_factor;
+        d.constraints_on_center = cons_on_center ? 1 : 0;
         d.cg_max_iters = options.cg_max_iters;
         d.cg_tol = options.cg_tol;
         d.n_aggregates = options.n_aggregates;

@@ -10,6 +10,12 @@
              (alpha = 1 .. 2^-49) and stop rule |h| < (|x| + 1e-6) 1e-6
   invariant  zero energy at identity when the aims are the current node positions
   fixture    the stripes graph (200 nodes) with the node blocks of LoadDeformScript0
+  centre     the constraints-on-centre mode (Deform.cpp:34-75, 483-529): the same
+             restatement with one 3-row group per control block over its first
+             min(20, |block|) nodes; central differences, the dense GN loop, the
+             identity "centre rows = sum of the plain rows of the block's nodes"
+             (blocks of <= 20 nodes, up to the float rounding of the centre) and
+             the reference's first-20 selection (the 21st node's aim is ignored)
 """
 import json
 import os
@@ -26,7 +32,9 @@ class Ref:
     """Independent restatement of the reference energy."""
 
     def __init__(self, nodes, nbr, blocks, types, src_idx, src_w, aims, w_rot=1.0, w_reg=10.0, w_con=100.0,
-                 wf=1.0):
+                 wf=1.0, center=False):
+        self.center = center
+        self.aims32 = np.asarray(aims, np.float32)
         self.g = nodes.astype(np.float64)
         self.nbr, self.blocks, self.types = nbr, blocks, list(types)
         self.src_idx, self.src_w, self.aims = src_idx, src_w, aims.astype(np.float64)
@@ -69,19 +77,34 @@ class Ref:
         for blk, ty in zip(self.blocks, self.types):
             if ty == -1:
                 continue
+            if self.center:
+                cnt = min(20, len(blk))
+                ctr = np.zeros(3, np.float32)
+                for c in blk[:cnt]:
+                    ctr = (ctr + self.aims32[int(c)]).astype(np.float32)
+                ctr = (ctr / np.float32(cnt)).astype(np.float64)
+                acc = np.zeros(3)
+                for c in sorted({int(c) for c in blk[:cnt]}):
+                    acc += self.sc * (self.blend(x, c) - ctr)
+                out += list(acc)
+                continue
             for c in blk:
-                ve = self.g[c]
-                v = np.zeros(3)
-                for jj in range(K):
-                    nj = int(self.src_idx[c, jj])
-                    w = float(self.src_w[c, jj])
-                    if nj in self.fid:
-                        f = self.fid[nj]
-                        v += w * (self.R(x, f) @ (ve - self.g[nj]) + self.g[nj] + self.t(x, f))
-                    else:
-                        v += w * ve
-                out += list(self.sc * (v - self.aims[c]))
+                out += list(self.sc * (self.blend(x, c) - self.aims[c]))
         return np.array(out)
+
+    def blend(self, x, c):
+        K = self.nbr.shape[1]
+        ve = self.g[c]
+        v = np.zeros(3)
+        for jj in range(K):
+            nj = int(self.src_idx[c, jj])
+            w = float(self.src_w[c, jj])
+            if nj in self.fid:
+                f = self.fid[nj]
+                v += w * (self.R(x, f) @ (ve - self.g[nj]) + self.g[nj] + self.t(x, f))
+            else:
+                v += w * ve
+        return v
 
     def energy(self, x):
         f = self.residual(x)
@@ -101,20 +124,21 @@ class Ref:
         return rows
 
 
-def tiny(seed=0, M=24, K=4, wf=1.0, aims_at_nodes=False):
+def tiny(seed=0, M=24, K=4, wf=1.0, aims_at_nodes=False, center=False, blocks=None, types=None):
     rng = np.random.default_rng(seed)
     nodes = rng.uniform(-1, 1, size=(M, 3)).astype(np.float32)
     nbr = O.graph_edges(nodes, K)
     src = nodes + rng.normal(0, 0.05, size=nodes.shape).astype(np.float32)
     sidx, sw = O.knn_weights(nodes, src, K)
-    blocks = [np.arange(0, 6), np.arange(6, 12), np.arange(12, 15)]
-    types = [0, 1, -1]
+    if blocks is None:
+        blocks = [np.arange(0, 6), np.arange(6, 12), np.arange(12, 15)]
+        types = [0, 1, -1]
     aims = nodes.astype(np.float64).copy()
     if not aims_at_nodes:
-        aims[6:12] += np.array([0.2, -0.1, 0.3])
+        aims[blocks[1]] += np.array([0.2, -0.1, 0.3])
     aims = aims.astype(np.float32)          # aims are float positions in the reference
     args = (nodes, nbr, blocks, types, sidx[:, :K], sw, aims)
-    return O.GNProblem(*args, wf=wf), Ref(*args, wf=wf)
+    return O.GNProblem(*args, wf=wf, center=center), Ref(*args, wf=wf, center=center)
 
 
 @pytest.mark.parametrize("wf", [1.0, 3.0])
@@ -220,3 +244,64 @@ def test_stripes_fixture_gn():
     x_ref, e_ref = gn_numpy(prob, ref, 6)
     np.testing.assert_allclose(e, e_ref, rtol=1e-8)
     assert np.all(np.diff(e) <= 0) and e[-1] < 0.5 * e[0]
+
+
+@pytest.mark.parametrize("wf", [1.0, 3.0])
+def test_center_mode_residual_and_jacobian(wf):
+    prob, ref = tiny(seed=7, wf=wf, center=True)
+    rng = np.random.default_rng(8)
+    x = prob.identity_x() + 0.1 * rng.normal(size=prob.n)
+    f = prob.residual(x)
+    np.testing.assert_allclose(f, ref.residual(x), rtol=0, atol=1e-12)
+    nf = prob.n // 12
+    assert len(f) == 6 * nf + 3 * (nf * 4 + 3 * 4) + 3 * 2       # two control blocks -> 6 centre rows
+    eps = 1e-6
+    J_fd = np.zeros((len(f), prob.n))
+    for c in range(prob.n):
+        e = np.zeros(prob.n)
+        e[c] = eps
+        J_fd[:, c] = (ref.residual(x + e) - ref.residual(x - e)) / (2 * eps)
+    for r, c in ref.quirk_columns():
+        J_fd[r, c] /= wf
+    np.testing.assert_allclose(prob.jacobian(x), J_fd, atol=2e-6)
+
+
+def test_center_rows_are_sums_of_plain_rows():
+    """Blocks of <= 20 distinct nodes: sum_keys (blend - centre) = sum_keys (blend - aim)."""
+    pc, _ = tiny(seed=9, center=True)
+    pp, _ = tiny(seed=9, center=False)
+    rng = np.random.default_rng(10)
+    x = pc.identity_x() + 0.1 * rng.normal(size=pc.n)
+    fc, fp = pc.residual(x), pp.residual(x)
+    nf = pc.n // 12
+    head = 6 * nf + 3 * (nf * 4 + 3 * 4)
+    np.testing.assert_array_equal(fc[:head], fp[:head])
+    plain = fp[head:].reshape(-1, 3)
+    np.testing.assert_allclose(fc[head:head + 3], plain[0:6].sum(0), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(fc[head + 3:head + 6], plain[6:12].sum(0), rtol=1e-5, atol=1e-5)
+
+
+def test_center_mode_keeps_the_first_20_block_nodes():
+    """SelectKeyControls runs FPS but keeps c_indices[i] (the first 20 nodes):
+    the aim of a block's 21st node does not enter the centre residual."""
+    blocks = [np.arange(0, 4), np.arange(4, 29)]
+    prob, ref = tiny(seed=11, M=40, center=True, blocks=blocks, types=[-1, 1])
+    x = prob.identity_x()
+    f0 = prob.residual(x)
+    np.testing.assert_allclose(f0, ref.residual(x), rtol=0, atol=1e-12)
+    aims = prob.aims.copy()
+    aims[24] += 1.0                      # the 21st node of block 1
+    moved = O.GNProblem(prob.node_pos, prob.nbr, blocks, [-1, 1], prob.src_idx, prob.src_w, aims, center=True)
+    np.testing.assert_array_equal(moved.residual(x), f0)
+    aims[23] += 1.0                      # the 20th node counts
+    moved = O.GNProblem(prob.node_pos, prob.nbr, blocks, [-1, 1], prob.src_idx, prob.src_w, aims, center=True)
+    assert np.abs(moved.residual(x) - f0).max() > 1.0
+
+
+def test_center_mode_gn_matches_dense_restatement():
+    prob, ref = tiny(seed=12, center=True)
+    x_ref, e_ref = gn_numpy(prob, ref, 8)
+    x, e, it = prob.solve(8)
+    assert it == len(e_ref) - 1
+    np.testing.assert_allclose(e, e_ref, rtol=1e-9, atol=1e-14)
+    np.testing.assert_allclose(x, x_ref, atol=1e-8)

@@ -31,23 +31,40 @@ def prob():
     return sc
 
 
-def _problem(sc, fraction, wf=1.0):
+def _translate_aims(nodes, blocks, types, fraction):
+    """Aims of a translating drag (UpdateAimPosition, GaussianView.cpp:2920-2931):
+    the centre-mode constraint sees only the mean of the block's aims, which a
+    twist about the block's centre leaves in place."""
+    aims = np.array(nodes, np.float32, copy=True)
+    for blk, t in zip(blocks, types):
+        if t == 1:
+            aims[blk] += np.float32(fraction) * np.array([0.12, 0.05, -0.08], np.float32)
+    return aims
+
+
+def _problem(sc, fraction, wf=1.0, center=False):
     nodes = sc["nodes"]
-    aims = S.frame_aims(sc["scene"], nodes, sc["blocks"], sc["types"], fraction)
-    ref = O.GNProblem(nodes, sc["nbr"], sc["blocks"], sc["types"], sc["src_idx"], sc["src_w"], aims, wf=wf)
+    if center:
+        aims = _translate_aims(nodes, sc["blocks"], sc["types"], fraction)
+    else:
+        aims = S.frame_aims(sc["scene"], nodes, sc["blocks"], sc["types"], fraction)
+    ref = O.GNProblem(nodes, sc["nbr"], sc["blocks"], sc["types"], sc["src_idx"], sc["src_w"], aims, wf=wf,
+                      center=center)
     return aims, ref
 
 
-def _solver(ctx, sc, wf=1.0, blocks=None, types=None):
+def _solver(ctx, sc, wf=1.0, blocks=None, types=None, center=False):
     return arapgs.Solver(ctx, sc["nbr"], sc["blocks"] if blocks is None else blocks,
                          sc["types"] if types is None else types, sc["src_idx"], sc["src_w"].astype(np.float32),
-                         sc["nodes"], weighting_factor=wf, cg_max_iters=200, cg_tol=1e-5)
+                         sc["nodes"], weighting_factor=wf, cg_max_iters=200, cg_tol=1e-5,
+                         constraints_on_center=center)
 
 
-@pytest.mark.parametrize("wf", [1.0, 3.0])
-def test_energy_function(ctx, prob, wf):
-    aims, ref = _problem(prob, 0.5, wf)
-    s = _solver(ctx, prob, wf)
+@pytest.mark.parametrize("wf,center", [(1.0, False), (3.0, False), (3.0, True)])
+def test_energy_function(ctx, prob, wf, center):
+    print(f"solver energy wf={wf} center={center}", flush=True)
+    aims, ref = _problem(prob, 0.5, wf, center)
+    s = _solver(ctx, prob, wf, center=center)
     rng = np.random.default_rng(0)
     for trial in range(3):
         x = ref.identity_x() + 0.05 * rng.normal(size=ref.n)
@@ -115,3 +132,65 @@ def test_cg_launch_modes_identical(ctx, n_nodes):
         assert done == outs[0][2] and done >= 1
         assert np.array_equal(x, outs[0][0])
         assert np.array_equal(e[: done + 1], outs[0][1][: done + 1])
+
+
+@pytest.mark.parametrize("fraction", [1.0, 0.3])
+def test_gn_solve_center_mode_matches_oracle(ctx, prob, fraction):
+    """Constraints on the centre of each control block (Deform.cpp:34-75,
+    483-529): one 3-row group per block over its first min(20, |block|) nodes."""
+    print(f"solver centre mode fraction={fraction} blocks={[len(b) for b in prob['blocks']]}", flush=True)
+    assert max(len(b) for b, t in zip(prob["blocks"], prob["types"]) if t != -1) > 1
+    aims, ref = _problem(prob, fraction, center=True)
+    x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+    s = _solver(ctx, prob, center=True)
+    assert s.n_free == ref.n // 12
+    x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+    x, e, done = H(x).astype(np.float64), H(e), int(H(done)[0])
+    assert 1 <= done <= 8
+    e = e[: done + 1]
+    assert abs(e[0] - e_ref[0]) <= 1e-6 * e_ref[0]
+    assert np.all(np.diff(e) <= 1e-9 * e[0])
+    assert abs(e[-1] - e_ref[-1]) <= 1e-5 * e_ref[-1], (e, e_ref)
+    e_at_x = ref.energy(x)
+    assert abs(e_at_x - e[-1]) <= 1e-5 * e_at_x
+    assert abs(done - it_ref) <= 1, (done, it_ref)
+
+
+def test_center_mode_launch_modes_identical(ctx, prob):
+    """Centre mode through the persistent and the per-phase PCG paths: same bits."""
+    aims, _ = _problem(prob, 0.7, center=True)
+    s = _solver(ctx, prob, center=True)
+    s.set_coarse_refresh(1000)
+    outs = []
+    for mode, every in ((1, 0), (0, 1)):
+        s.set_cg_mode(mode)
+        s.set_host_checks(every)
+        x, e, done = s.solve(T(prob["nodes"]), T(aims), 6, stop_rule=True)
+        outs.append((H(x), H(e), int(H(done)[0])))
+    assert outs[0][2] == outs[1][2] >= 1
+    assert np.array_equal(outs[0][0], outs[1][0])
+
+
+def test_center_mode_large_block_keeps_first_20(ctx, prob):
+    """A 30-node control block: the centre group uses its first 20 nodes (the
+    reference's c_indices[i] selection); energies match the oracle."""
+    nodes = prob["nodes"]
+    d0 = np.linalg.norm(nodes - nodes[0], axis=1)
+    far = int(np.argmax(d0))
+    ctrl = np.argsort(d0, kind="stable")[:30].astype(np.int32)
+    stat = np.argsort(np.linalg.norm(nodes - nodes[far], axis=1), kind="stable")[:10].astype(np.int32)
+    blocks, types = [stat, ctrl], np.array([-1, 1], np.int32)
+    aims = _translate_aims(nodes, blocks, types, 0.6)
+    ref = O.GNProblem(nodes, prob["nbr"], blocks, types, prob["src_idx"], prob["src_w"], aims, center=True)
+    x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+    s = _solver(ctx, prob, blocks=blocks, types=types, center=True)
+    x, e, done = s.solve(T(nodes), T(aims), 8, stop_rule=True)
+    e = H(e)[: int(H(done)[0]) + 1]
+    assert abs(e[0] - e_ref[0]) <= 1e-6 * e_ref[0]
+    assert abs(e[-1] - e_ref[-1]) <= 1e-5 * e_ref[-1], (e, e_ref)
+    # the aims of nodes 21..30 of the block do not enter
+    aims2 = aims.copy()
+    aims2[ctrl[20:]] += 0.5
+    e2 = float(H(s.energy(T(nodes), T(aims2), T(ref.identity_x().astype(np.float32))))[0])
+    e1 = float(H(s.energy(T(nodes), T(aims), T(ref.identity_x().astype(np.float32))))[0])
+    assert e1 == e2
