@@ -99,6 +99,9 @@ EXPORTS = {
     "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
     "arapgs_aims_translate": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_aims_twist": ([P, P, C.c_int, P, C.c_int, P, C.c_float, P], C.c_int),
+    "arapgs_aims_scale": ([P, P, C.c_int, P, C.c_int, C.c_float, P], C.c_int),
+    "arapgs_static_rows": ([P, P, P, C.c_int64, C.c_int, P], C.c_int),
+    "arapgs_aims_set": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_profile_enable": ([P, C.c_int], C.c_int),
     "arapgs_profile_read": ([P, P, C.c_int], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
@@ -393,6 +396,24 @@ class Context:
         a = np.ascontiguousarray(axis, np.float32)
         self._check(lib().arapgs_aims_twist(self.h, ptr(node_pos), node_pos.shape[0], ptr(active), active.shape[0],
                                             a.ctypes.data_as(P), radian, ptr(aims)))
+
+    def aims_scale(self, node_pos, active, s, aims):
+        self._check(lib().arapgs_aims_scale(self.h, ptr(node_pos), node_pos.shape[0], ptr(active), active.shape[0],
+                                            float(s), ptr(aims)))
+
+    def aims_set(self, node_pos, idx, vals, aims):
+        self._check(lib().arapgs_aims_set(self.h, ptr(node_pos), node_pos.shape[0], ptr(idx), idx.shape[0],
+                                          ptr(vals), ptr(aims)))
+
+    def static_rows(self, node_static, idx, out=None):
+        """uint8 per row of idx (n, len): 1 when all its node ids are static."""
+        t = self._torch
+        n = idx.shape[0]
+        length = idx[0].numel() if n else 1
+        if out is None:
+            out = t.empty(n, dtype=t.uint8, device=idx.device)
+        self._check(lib().arapgs_static_rows(self.h, ptr(node_static), ptr(idx), n, length, ptr(out)))
+        return out
 
     def profile_enable(self, on=True):
         self._check(lib().arapgs_profile_enable(self.h, int(on)))

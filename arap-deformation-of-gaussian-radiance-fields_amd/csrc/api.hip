@@ -865,6 +865,27 @@ int arapgs_aims_twist(arapgs_ctx* c, const float* node_pos, int m, const int* ac
     return check_launch(c, launch_aims_twist(node_pos, m, active, n_active, axis_h, radian, aims, c->stream), "aims_twist");
 }
 
+int arapgs_aims_scale(arapgs_ctx* c, const float* node_pos, int m, const int* active, int n_active, float s,
+                      float* aims) {
+    CHECK_CTX(c);
+    if (!node_pos || !aims || m <= 0 || n_active <= 0 || !active) return fail(c, -1, "aims_scale: bad arguments");
+    return check_launch(c, launch_aims_scale(node_pos, m, active, n_active, s, aims, c->stream), "aims_scale");
+}
+
+int arapgs_aims_set(arapgs_ctx* c, const float* node_pos, int m, const int* idx, int n, const float* vals,
+                    float* aims) {
+    CHECK_CTX(c);
+    if (!node_pos || !aims || m <= 0 || n < 0 || (n > 0 && (!idx || !vals))) return fail(c, -1, "aims_set: bad arguments");
+    return check_launch(c, launch_aims_set(node_pos, m, idx, n, vals, aims, c->stream), "aims_set");
+}
+
+int arapgs_static_rows(arapgs_ctx* c, const uint8_t* node_static, const uint32_t* idx, int64_t n, int len,
+                       uint8_t* out) {
+    CHECK_CTX(c);
+    if (n < 0 || len <= 0 || (n > 0 && (!node_static || !idx || !out))) return fail(c, -1, "static_rows: bad arguments");
+    return check_launch(c, launch_static_rows(node_static, idx, n, len, out, c->stream), "static_rows");
+}
+
 int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buffers* fb, const float* aims,
                         int gn_iters, int stop_rule, double* energies, int* iters_done) {
     CHECK_CTX(c);

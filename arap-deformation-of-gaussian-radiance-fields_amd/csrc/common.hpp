@@ -54,6 +54,12 @@ int launch_aims_translate(const float* node_pos, int m, const int* active, int n
                           hipStream_t st);
 int launch_aims_twist(const float* node_pos, int m, const int* active, int n_active, const float axis[3], float radian,
                       float* aims, hipStream_t st);
+int launch_aims_scale(const float* node_pos, int m, const int* active, int n_active, float sc, float* aims,
+                      hipStream_t st);
+int launch_aims_set(const float* node_pos, int m, const int* idx, int n, const float* vals, float* aims,
+                    hipStream_t st);
+int launch_static_rows(const uint8_t* node_static, const uint32_t* idx, int64_t n, int len, uint8_t* out,
+                       hipStream_t st);
 int launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t st);
 
 }  // namespace arapgs

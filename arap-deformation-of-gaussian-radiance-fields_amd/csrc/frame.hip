@@ -67,6 +67,65 @@ __global__ void __launch_bounds__(256) aims_twist_kernel(const float* __restrict
     }
 }
 
+// UpdateAimPositionScale (GaussianView.cpp:2961-2983): aims of the active nodes
+// scaled by s about their float centroid (summed in node order, one thread).
+__global__ void __launch_bounds__(256) aims_scale_kernel(const float* __restrict__ node_pos, int m,
+                                                         const int* __restrict__ active, int n_active, float sc,
+                                                         float* __restrict__ aims) {
+#pragma clang fp contract(off)
+    __shared__ float cen[3];
+    if (threadIdx.x == 0) {
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        for (int j = 0; j < n_active; j++) {
+            const int i = active[j];
+            cx += node_pos[3 * i]; cy += node_pos[3 * i + 1]; cz += node_pos[3 * i + 2];
+        }
+        const float cnt = (float)n_active;
+        cen[0] = cx / cnt; cen[1] = cy / cnt; cen[2] = cz / cnt;
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        aims[3 * i] = node_pos[3 * i];
+        aims[3 * i + 1] = node_pos[3 * i + 1];
+        aims[3 * i + 2] = node_pos[3 * i + 2];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < n_active; j += blockDim.x) {
+        const int i = active[j];
+#pragma unroll
+        for (int c = 0; c < 3; c++) aims[3 * i + c] = cen[c] + sc * (node_pos[3 * i + c] - cen[c]);
+    }
+}
+
+// aims = node_pos, then aims[idx[j]] = vals[j] for j in order (one thread, so
+// a node listed twice keeps its last value, like the reference's loop).
+__global__ void __launch_bounds__(256) aims_set_kernel(const float* __restrict__ node_pos, int m,
+                                                       const int* __restrict__ idx, int n,
+                                                       const float* __restrict__ vals, float* __restrict__ aims) {
+    for (int i = threadIdx.x; i < 3 * m; i += blockDim.x) aims[i] = node_pos[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int j = 0; j < n; j++) {
+            const int i = idx[j];
+            if (i < 0 || i >= m) continue;
+            aims[3 * i] = vals[3 * j];
+            aims[3 * i + 1] = vals[3 * j + 1];
+            aims[3 * i + 2] = vals[3 * j + 2];
+        }
+}
+
+// CheckStaticSamples / CheckMovedGaussians (GaussianView.cpp:2023-2109): row r
+// of n rows of len node ids is static when every id is a static node.
+__global__ void __launch_bounds__(256) static_rows_kernel(const uint8_t* __restrict__ node_static,
+                                                          const uint32_t* __restrict__ idx, int64_t n, int len,
+                                                          uint8_t* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint8_t all = 1;
+    for (int j = 0; j < len; j++)
+        if (!node_static[idx[r * len + j]]) all = 0;
+    out[r] = all;
+}
+
 __global__ void copy_f32_n_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -89,6 +148,29 @@ int launch_aims_twist(const float* node_pos, int m, const int* active, int n_act
                       float* aims, hipStream_t st) {
     hipLaunchKernelGGL(aims_twist_kernel, dim3(1), dim3(256), 0, st, node_pos, m, active, n_active, axis[0], axis[1],
                        axis[2], radian, aims);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_aims_scale(const float* node_pos, int m, const int* active, int n_active, float sc, float* aims,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(aims_scale_kernel, dim3(1), dim3(256), 0, st, node_pos, m, active, n_active, sc, aims);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_aims_set(const float* node_pos, int m, const int* idx, int n, const float* vals, float* aims,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(aims_set_kernel, dim3(1), dim3(256), 0, st, node_pos, m, idx, n, vals, aims);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_static_rows(const uint8_t* node_static, const uint32_t* idx, int64_t n, int len, uint8_t* out,
+                       hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(static_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, node_static, idx, n,
+                       len, out);
     HIP_RET(hipGetLastError());
     return 0;
 }

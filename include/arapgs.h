@@ -355,6 +355,21 @@ int arapgs_aims_translate(arapgs_ctx* ctx, const float* node_pos, int m, const i
  * by `radian` about axis_h (host float[3]) through the active nodes' centroid. */
 int arapgs_aims_twist(arapgs_ctx* ctx, const float* node_pos, int m, const int* active, int n_active,
                       const float* axis_h, float radian, float* aims);
+/* UpdateAimPositionScale (GaussianView.cpp:2961-2983): aims = node_pos, then
+ * the active aims scaled by s about the active nodes' centroid. */
+int arapgs_aims_scale(arapgs_ctx* ctx, const float* node_pos, int m, const int* active, int n_active, float s,
+                      float* aims);
+/* Aims of a DeformScript step (GaussianView.cpp:1921-1935): aims = node_pos,
+ * then aims[idx[j]] = vals[j*3 .. j*3+3) in order (device idx int[n], vals
+ * f32[n*3]; ids outside [0, m) are skipped). */
+int arapgs_aims_set(arapgs_ctx* ctx, const float* node_pos, int m, const int* idx, int n, const float* vals,
+                    float* aims);
+/* CheckStaticSamples (GaussianView.cpp:2023-2090): out[r] = 1 when every one of
+ * the len node ids idx[r*len .. r*len+len) is static (node_static u8[m]).
+ * Samples: len = k (static_samples); Gaussians: the 6k ids of their end points
+ * (static_gaussians). All device pointers. */
+int arapgs_static_rows(arapgs_ctx* ctx, const uint8_t* node_static, const uint32_t* idx, int64_t n, int len,
+                       uint8_t* out);
 
 /* Measurement hook: when enabled, arapgs_deform_frame brackets its fused
  * six-point kernel with HIP events on the context stream; arapgs_profile_read
