@@ -96,6 +96,7 @@ EXPORTS = {
     "arapgs_solver_set_host_checks": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
     "arapgs_aims_translate": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_aims_twist": ([P, P, C.c_int, P, C.c_int, P, C.c_float, P], C.c_int),
@@ -529,6 +530,10 @@ class Solver:
 
     def set_coarse_refresh(self, frames):
         self.ctx._check(lib().arapgs_solver_set_coarse_refresh(self.h, frames))
+
+    def set_warm_start(self, on=True):
+        if lib().arapgs_solver_set_warm_start(self.h, int(bool(on))) != 0:
+            raise ArapgsError("set_warm_start failed")
 
     def set_cg_mode(self, mode):
         """0 = three kernels per CG iteration (default), 1 = persistent cooperative PCG kernel."""

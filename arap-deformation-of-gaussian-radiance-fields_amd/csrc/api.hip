@@ -322,6 +322,17 @@ int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode) {
     return 0;
 }
 
+int arapgs_solver_set_warm_start(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    (void)hipSetDevice(s->device);
+    // every (re)enable starts from zero steps, so a timed run does not depend on earlier ones
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemset(s->w.hwarm, 0, sizeof(float) * (size_t)s->w.warm_slots * s->w.n) != hipSuccess)
+        return -1000;
+    s->w.warm = on;
+    return 0;
+}
+
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames) {
     if (!s || frames < 1) return -1;
     s->coarse_refresh = frames;
@@ -370,7 +381,7 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
         return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
     };
     for (int it = 0; it < iters; it++) {
-        rc = check_launch(c, solver_gn_begin(W, c->stream), "gn_solve rhs");
+        rc = check_launch(c, solver_gn_begin(W, it, c->stream), "gn_solve rhs");
         if (rc) return rc;
         if (s->cg_mode == 1) {
             rc = check_launch(c, solver_cg_persistent(W, c->stream), "gn_solve cg");

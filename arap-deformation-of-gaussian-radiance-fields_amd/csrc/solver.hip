@@ -171,14 +171,25 @@ __device__ __forceinline__ double block_reduce_d(double v, double* sh) {
     return r;  // valid in thread 0
 }
 
-// Energies of x + 2^-c h for c in [0, NC) (h == nullptr: the energy of x).
-// blockIdx.y = c. Per-block FP64 partials, reduced by reduce_partials_kernel.
+// The line search's candidates past the full step are evaluated only when the
+// full step was rejected: with first_e set, a launch for candidates c0 >= 1
+// returns at once when E(x + h) = first_e[0] <= E_old.
+__device__ __forceinline__ bool first_accepted(const double* first_e, const GnScalars* gs) {
+    return first_e && !(first_e[0] > gs->e_old);
+}
+
+// Energies of x + 2^-c h for c in [c0, c0 + gridDim.y) (h == nullptr: the
+// energy of x). blockIdx.y = c - c0. Per-block FP64 partials at
+// partial[c * gridDim.x + block], reduced by reduce_partials_kernel.
 __global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* __restrict__ x,
                                                      const float* __restrict__ h, double* __restrict__ partial,
-                                                     const int* __restrict__ skip) {
+                                                     const int* __restrict__ skip, int c0,
+                                                     const double* __restrict__ first_e,
+                                                     const GnScalars* __restrict__ gs) {
     if (skip && *skip) return;
+    if (first_accepted(first_e, gs)) return;
     __shared__ double sh[8];
-    const int c = blockIdx.y;
+    const int c = blockIdx.y + c0;
     const float alpha = ldexpf(1.0f, -c);
     double acc = 0.0;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
@@ -193,11 +204,13 @@ __global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* _
     if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = t;
 }
 
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int ncand,
-                                       double* __restrict__ out, const int* __restrict__ skip) {
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int c0, int ncand,
+                                       double* __restrict__ out, const int* __restrict__ skip,
+                                       const double* __restrict__ first_e, const GnScalars* __restrict__ gs) {
     if (skip && *skip) return;
+    if (first_accepted(first_e, gs)) return;
     __shared__ double sh[8];
-    for (int c = 0; c < ncand; c++) {
+    for (int c = c0; c < c0 + ncand; c++) {
         double acc = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partial[(int64_t)c * nblocks + b];
         const double t = block_reduce_d(acc, sh);
@@ -862,6 +875,23 @@ __global__ void zero_kernel(float* __restrict__ v, int64_t n, const int* __restr
         v[i] = 0.0f;
 }
 
+// Warm-started CG: h = h0 and r = b - A h0 (q holds A h0); a CG skipped for a
+// negligible gradient keeps h = 0 like the cold start.
+__global__ void warm_init_kernel(int64_t n, float* __restrict__ hsol, const float* __restrict__ h0,
+                                 float* __restrict__ r, const float* __restrict__ q, const int* __restrict__ cg_done,
+                                 const int* __restrict__ gn_done) {
+    if (*gn_done) return;
+    const bool skip = *cg_done != 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (skip) {
+            hsol[i] = 0.0f;
+        } else {
+            hsol[i] = h0[i];
+            r[i] = r[i] - q[i];
+        }
+    }
+}
+
 __global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, int64_t n,
                             const int* __restrict__ skip) {
     if (skip && *skip) return;
@@ -972,17 +1002,23 @@ __global__ void __launch_bounds__(256) coarse_multi_kernel(SolverDev S, unsigned
     }
 }
 
-__global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
+// Every 3-row group's Jacobian block for a node is I_3-structured: row r holds
+// (a_0, a_1, a_2) at the R columns 3c + r and b at the t column 9 + r. Its
+// projection onto the node's aggregate keeps the structure with
+// a_c + b (g_j - c_a)_c, so a group contributes, per pair of touched
+// aggregates, the 16 products of their 4-coefficient rows placed in the three
+// rows. One thread per group; the (aggregate, coefficients) list of a group
+// lives in LDS and nodes of one aggregate are summed in the group's node order.
+__global__ void __launch_bounds__(128) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
                                                               unsigned long long* __restrict__ Ac) {
+    __shared__ float cv[128][KNN_MAX_T + 1][4];
+    __shared__ int ca[128][KNN_MAX_T + 1];
+    const int tid = threadIdx.x;
     const int nc = S.nc_pad;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
          g += (int64_t)gridDim.x * blockDim.x) {
-        // collect (free node, Jblock) pairs of the group
-        int nodes[KNN_MAX_T + 1];
-        float J[KNN_MAX_T + 1][3][12];
-        int rows = 3, nn = 0;
-        float Jrot[6][9];
         int64_t gg = g;
+        float Jrot[6][9];
         if (gg < S.nF) {
             const int fi = (int)gg;
             const float* xi = x + 12 * (int64_t)fi;
@@ -1007,9 +1043,22 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
             continue;
         }
         gg -= S.nF;
-        for (int q = 0; q < KNN_MAX_T + 1; q++)
-            for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 12; c++) J[q][r][c] = 0.0f;
+        int nn = 0;
+        auto push = [&](int fi, float r0, float r1, float r2, float t) {
+            const int a = S.agg_of[fi];
+            const int node = S.free_nodes[fi];
+            const float v[4] = {r0 + t * (S.g[3 * node] - S.agg_center[3 * a]),
+                                r1 + t * (S.g[3 * node + 1] - S.agg_center[3 * a + 1]),
+                                r2 + t * (S.g[3 * node + 2] - S.agg_center[3 * a + 2]), t};
+            for (int q = 0; q < nn; q++)
+                if (ca[tid][q] == a) {
+                    for (int c = 0; c < 4; c++) cv[tid][q][c] += v[c];
+                    return;
+                }
+            ca[tid][nn] = a;
+            for (int c = 0; c < 4; c++) cv[tid][nn][c] = v[c];
+            nn++;
+        };
         if (gg < (int64_t)S.nF * S.K) {
             const int fi = (int)(gg / S.K), t = (int)(gg % S.K);
             const int i = S.free_nodes[fi];
@@ -1017,17 +1066,8 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
             const int fk = S.free_id[k];
             const float s = S.edge_scale[(int64_t)i * S.K + t];
             const float d[3] = {S.g[3 * k] - S.g[3 * i], S.g[3 * k + 1] - S.g[3 * i + 1], S.g[3 * k + 2] - S.g[3 * i + 2]};
-            nodes[nn] = fi;
-            for (int r = 0; r < 3; r++) {
-                for (int c = 0; c < 3; c++) J[nn][r][3 * c + r] = S.sw_reg * s * d[c];
-                J[nn][r][9 + r] = S.sw_reg;
-            }
-            nn++;
-            if (fk >= 0) {
-                nodes[nn] = fk;
-                for (int r = 0; r < 3; r++) J[nn][r][9 + r] = -S.sw_reg * s;
-                nn++;
-            }
+            push(fi, S.sw_reg * s * d[0], S.sw_reg * s * d[1], S.sw_reg * s * d[2], S.sw_reg);
+            if (fk >= 0) push(fk, 0.0f, 0.0f, 0.0f, -S.sw_reg * s);
         } else {
             gg -= (int64_t)S.nF * S.K;
             if (gg < (int64_t)S.nS * S.K) {
@@ -1036,9 +1076,7 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
                 const int k = (int)S.nbr[(int64_t)i * S.K + t];
                 const int fk = S.free_id[k];
                 if (fk < 0) continue;
-                nodes[nn] = fk;
-                for (int r = 0; r < 3; r++) J[nn][r][9 + r] = -S.sw_reg;
-                nn++;
+                push(fk, 0.0f, 0.0f, 0.0f, -S.sw_reg);
             } else {
                 gg -= (int64_t)S.nS * S.K;
                 // groups of several members: coarse_multi_kernel
@@ -1051,54 +1089,26 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
                     if (fj < 0) continue;
                     const float w = S.src_w[(int64_t)c * S.K + j];
                     const float dd[3] = {v[0] - S.g[3 * nj], v[1] - S.g[3 * nj + 1], v[2] - S.g[3 * nj + 2]};
-                    nodes[nn] = fj;
-                    for (int r = 0; r < 3; r++) {
-                        for (int cc = 0; cc < 3; cc++) J[nn][r][3 * cc + r] = S.sw_con * w * dd[cc];
-                        J[nn][r][9 + r] = S.sw_con * w;
-                    }
-                    nn++;
+                    push(fj, S.sw_con * w * dd[0], S.sw_con * w * dd[1], S.sw_con * w * dd[2], S.sw_con * w);
                 }
             }
         }
-        // project each node's block onto its aggregate: JP[q][r][12]
-        float JP[KNN_MAX_T + 1][3][12];
-        int aggs[KNN_MAX_T + 1];
         for (int q = 0; q < nn; q++) {
-            const int fi = nodes[q];
-            const int a = S.agg_of[fi];
-            aggs[q] = a;
-            const int node = S.free_nodes[fi];
-            const float dj[3] = {S.g[3 * node] - S.agg_center[3 * a], S.g[3 * node + 1] - S.agg_center[3 * a + 1],
-                                 S.g[3 * node + 2] - S.agg_center[3 * a + 2]};
-            for (int r = 0; r < rows; r++) {
-                for (int u = 0; u < 9; u++) JP[q][r][u] = J[q][r][u];
-                for (int rr3 = 0; rr3 < 3; rr3++) {
-                    const float jt = J[q][r][9 + rr3];
-                    for (int cc = 0; cc < 3; cc++) JP[q][r][3 * cc + rr3] += jt * dj[cc];
-                    JP[q][r][9 + rr3] = jt;
-                }
-            }
-        }
-        // merge nodes sharing an aggregate
-        for (int q = 0; q < nn; q++) {
-            if (aggs[q] < 0) continue;
-            for (int q2 = q + 1; q2 < nn; q2++)
-                if (aggs[q2] == aggs[q]) {
-                    for (int r = 0; r < rows; r++)
-                        for (int u = 0; u < 12; u++) JP[q][r][u] += JP[q2][r][u];
-                    aggs[q2] = -1;
-                }
-        }
-        for (int q = 0; q < nn; q++) {
-            if (aggs[q] < 0) continue;
+            const int a1 = ca[tid][q];
             for (int q2 = 0; q2 < nn; q2++) {
-                if (aggs[q2] < 0) continue;
-                for (int u = 0; u < 12; u++)
-                    for (int w = 0; w < 12; w++) {
-                        float s = 0.0f;
-                        for (int r = 0; r < rows; r++) s += JP[q][r][u] * JP[q2][r][w];
-                        if (s != 0.0f) coarse_add(Ac, (int64_t)(12 * aggs[q] + u) * nc + 12 * aggs[q2] + w, s);
+                const int a2 = ca[tid][q2];
+                for (int p1 = 0; p1 < 4; p1++) {
+                    const float c1 = cv[tid][q][p1];
+                    for (int p2 = 0; p2 < 4; p2++) {
+                        const float val = c1 * cv[tid][q2][p2];
+                        if (val == 0.0f) continue;
+                        for (int r = 0; r < 3; r++) {
+                            const int u = p1 < 3 ? 3 * p1 + r : 9 + r;
+                            const int w = p2 < 3 ? 3 * p2 + r : 9 + r;
+                            coarse_add(Ac, (int64_t)(12 * a1 + u) * nc + 12 * a2 + w, val);
+                        }
                     }
+                }
             }
         }
     }
@@ -1111,35 +1121,47 @@ __global__ void __launch_bounds__(256) coarse_assemble_kernel(SolverDev S, const
 // column panel -A_iK A_KK^-1.
 constexpr int GJ_T = 64;
 
-__global__ void __launch_bounds__(256) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
-    __shared__ float P[GJ_T][GJ_T + 1];
-    const int k0 = kb * GJ_T;
-    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) P[e / GJ_T][e % GJ_T] = A[(int64_t)(k0 + e / GJ_T) * n + k0 + e % GJ_T];
-    __syncthreads();
-    __shared__ float inv;
-    for (int k = 0; k < GJ_T; k++) {
-        if (threadIdx.x == 0) {
-            const float d = P[k][k];
-            inv = (fabsf(d) > 1e-30f) ? 1.0f / d : 0.0f;
-        }
-        __syncthreads();
-        const float iv = inv;
-        for (int j = threadIdx.x; j < GJ_T; j += blockDim.x)
-            if (j != k) P[k][j] *= iv;
-        __syncthreads();
-        for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
-            const int i = e / GJ_T, j = e % GJ_T;
-            if (i == k || j == k) continue;
-            P[i][j] -= P[i][k] * P[k][j];
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < GJ_T; i += blockDim.x)
-            if (i != k) P[i][k] *= -iv;
-        __syncthreads();
-        if (threadIdx.x == 0) P[k][k] = iv;
-        __syncthreads();
+// Inverse of the 64x64 pivot tile by in-place Gauss-Jordan, one wave: lane j
+// keeps column j in registers; step k reads the pivot column from lane k with
+// v_readlane (the loops are unrolled, so every register index is static).
+// Per entry the same operations in the same order as the textbook in-place
+// form: row k scaled by 1/d, P_ij -= P_ik (P_kj / d), column k -> -P_ik / d.
+template <int K>
+__device__ __forceinline__ void gj_step(float (&c)[GJ_T], int lane) {
+    float col[GJ_T];
+#pragma unroll
+    for (int i = 0; i < GJ_T; i++) col[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[i]), K));
+    const float d = col[K];
+    const float iv = (fabsf(d) > 1e-30f) ? 1.0f / d : 0.0f;
+    if (lane == K) {
+#pragma unroll
+        for (int i = 0; i < GJ_T; i++) c[i] = (i == K) ? iv : col[i] * -iv;
+    } else {
+        const float rk = c[K] * iv;
+#pragma unroll
+        for (int i = 0; i < GJ_T; i++)
+            if (i != K) c[i] -= col[i] * rk;
+        c[K] = rk;
     }
-    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) piv[e] = P[e / GJ_T][e % GJ_T];
+}
+
+template <int K>
+__device__ __forceinline__ void gj_steps(float (&c)[GJ_T], int lane) {
+    if constexpr (K < GJ_T) {
+        gj_step<K>(c, lane);
+        gj_steps<K + 1>(c, lane);
+    }
+}
+
+__global__ void __launch_bounds__(64) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
+    const int k0 = kb * GJ_T;
+    const int lane = threadIdx.x;
+    float c[GJ_T];
+#pragma unroll
+    for (int i = 0; i < GJ_T; i++) c[i] = A[(int64_t)(k0 + i) * n + k0 + lane];
+    gj_steps<0>(c, lane);
+#pragma unroll
+    for (int i = 0; i < GJ_T; i++) piv[i * GJ_T + lane] = c[i];
 }
 
 // blockIdx.x = tile column jb != kb: row panel A_Kj <- piv * A_Kj; also copies
@@ -1202,44 +1224,60 @@ __global__ void pad_identity_kernel(float* __restrict__ A, int n, int nc) {
 
 // ---------------------------------------------------------------- GN step
 // Choose the first candidate with E <= E_old, update x, record energies and the
-// stop rule |h_final| < (|x_old| + 1e-6) 1e-6 (Deform.cpp:142-166).
-__global__ void accept_kernel(int n, float* __restrict__ x, const float* __restrict__ hsol,
-                              const double* __restrict__ cand_e, int ncand, GnScalars* __restrict__ gs,
-                              double* __restrict__ energies, int iter, int stop_rule, int* __restrict__ gn_done) {
+// stop rule |h_final| < (|x_old| + 1e-6) 1e-6 (Deform.cpp:142-166). Two
+// kernels: every block finds the chosen candidate, accumulates its share of
+// |x_old|^2 and |h|^2 in FP64 and moves its share of x; one block then sums
+// the partials in block order and decides.
+__device__ __forceinline__ int chosen_candidate(const double* cand_e, int ncand, const GnScalars* gs) {
+    for (int c = 0; c < ncand; c++)
+        if (!(cand_e[c] > gs->e_old)) return c;
+    return -1;
+}
+
+__global__ void __launch_bounds__(256) accept_partial_kernel(int n, float* __restrict__ x,
+                                                             const float* __restrict__ hsol,
+                                                             const double* __restrict__ cand_e, int ncand,
+                                                             const GnScalars* __restrict__ gs,
+                                                             double* __restrict__ partial, const int* __restrict__ gn_done) {
     if (*gn_done) return;
     __shared__ int chosen;
-    __shared__ double red[256];
-    if (threadIdx.x == 0) {
-        chosen = -1;
-        for (int c = 0; c < ncand; c++)
-            if (!(cand_e[c] > gs->e_old)) { chosen = c; break; }
-        energies[iter] = gs->e_old;
-    }
+    __shared__ double sh[8];
+    if (threadIdx.x == 0) chosen = chosen_candidate(cand_e, ncand, gs);
     __syncthreads();
-    double nx = 0.0, nh = 0.0;
     const float a = chosen >= 0 ? ldexpf(1.0f, -chosen) : 0.0f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double nx = 0.0, nh = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         nx += (double)x[i] * x[i];
         nh += (double)hsol[i] * hsol[i];
         if (chosen >= 0) x[i] += a * hsol[i];
     }
-    red[threadIdx.x] = nx;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    const double normv = sqrt(red[0]);
-    __syncthreads();
-    red[threadIdx.x] = nh;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
+    const double tx = block_reduce_d(nx, sh);
+    const double th = block_reduce_d(nh, sh);
     if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = tx;
+        partial[2 * blockIdx.x + 1] = th;
+    }
+}
+
+__global__ void __launch_bounds__(256) accept_final_kernel(int nblocks, const double* __restrict__ partial,
+                                                           const double* __restrict__ cand_e, int ncand,
+                                                           GnScalars* __restrict__ gs, double* __restrict__ energies,
+                                                           int iter, int stop_rule, int* __restrict__ gn_done) {
+    if (*gn_done) return;
+    __shared__ double sh[8];
+    double nx = 0.0, nh = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        nx += partial[2 * b];
+        nh += partial[2 * b + 1];
+    }
+    const double tx = block_reduce_d(nx, sh);
+    const double th = block_reduce_d(nh, sh);
+    if (threadIdx.x == 0) {
+        const int chosen = chosen_candidate(cand_e, ncand, gs);
+        energies[iter] = gs->e_old;
         const double scale = chosen >= 0 ? ldexp(1.0, -chosen) : ldexp(1.0, -ncand);
-        const double normh = sqrt(red[0]) * scale;
+        const double normv = sqrt(tx);
+        const double normh = sqrt(th) * scale;
         if (chosen >= 0) gs->e_old = cand_e[chosen];
         gs->iters_done = iter + 1;
         energies[iter + 1] = gs->e_old;
@@ -1295,22 +1333,22 @@ int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
         hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
         const int nt = np / GJ_T;
         for (int kb = 0; kb < nt; kb++) {
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, kb, W.gj_piv);
             hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
             hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
         }
     }
     // E(x0)
     hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, (const float*)nullptr,
-                       W.e_partial, (const int*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, W.cand_e,
-                       (const int*)nullptr);
+                       W.e_partial, (const int*)nullptr, 0, (const double*)nullptr, (const GnScalars*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, W.cand_e,
+                       (const int*)nullptr, (const double*)nullptr, (const GnScalars*)nullptr);
     hipLaunchKernelGGL(set_eold_kernel, dim3(1), dim3(1), 0, st, W.gs, W.cand_e);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int solver_gn_begin(SolverWork& W, hipStream_t st) {
+int solver_gn_begin(SolverWork& W, int iter, hipStream_t st) {
     const SolverDev& S = W.dev;
     const int nb_owner = nblk(S.nF, 256 / LANES);
     hipLaunchKernelGGL(cg_begin_kernel, dim3(1), dim3(1), 0, st, W.gn_done, W.cg_done, W.cg, W.cg_max_iters,
@@ -1318,8 +1356,18 @@ int solver_gn_begin(SolverWork& W, hipStream_t st) {
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial,
                        W.cg_done);
     hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, W.partial, nb_owner, W.cg, W.gs, W.cg_done);
-    // h = 0 also when the CG is skipped for a negligible gradient (the line search then takes a zero step)
-    hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.gn_done);
+    if (W.warm && iter < W.warm_slots) {
+        // q = A h0 (the first-iteration form of the apply), then h = h0, r = b - A h0
+        const float* h0 = W.hwarm + (int64_t)iter * W.n;
+        auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
+        hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, h0, (const double*)W.rzp,
+                           (const double*)W.rzp, 1, W.p, W.q, W.partial, W.cg_done);
+        hipLaunchKernelGGL(warm_init_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, (int64_t)W.n, W.hsol, h0, W.r,
+                           W.q, W.cg_done, W.gn_done);
+    } else {
+        // h = 0 also when the CG is skipped for a negligible gradient (the line search then takes a zero step)
+        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.gn_done);
+    }
     // z0 = M^-1 r0, rz_0 -> rz[0]
     hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, (const double*)nullptr, 0,
                        (const double*)nullptr, W.hsol, W.r, (const float*)nullptr, (const float*)nullptr, W.binv, W.z,
@@ -1381,12 +1429,22 @@ int solver_barrier_error(SolverWork& W, int* err) {
 
 int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st) {
     const SolverDev& S = W.dev;
-    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, W.n_cand), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial,
-                       W.gn_done);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, W.n_cand, W.cand_e,
-                       W.gn_done);
-    hipLaunchKernelGGL(accept_kernel, dim3(1), dim3(256), 0, st, W.n, W.x, W.hsol, W.cand_e, W.n_cand, W.gs,
-                       energies, iter, stop_rule, W.gn_done);
+    if (W.warm && iter < W.warm_slots)
+        hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.hwarm + (int64_t)iter * W.n,
+                           (const float*)W.hsol, (int64_t)W.n, (const int*)W.gn_done);
+    // the full step first; the 49 halvings only if it was rejected
+    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial, W.gn_done,
+                       0, (const double*)nullptr, (const GnScalars*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, W.cand_e,
+                       W.gn_done, (const double*)nullptr, (const GnScalars*)nullptr);
+    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, W.n_cand - 1), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial,
+                       W.gn_done, 1, (const double*)W.cand_e, (const GnScalars*)W.gs);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, W.n_cand - 1,
+                       W.cand_e, W.gn_done, (const double*)W.cand_e, (const GnScalars*)W.gs);
+    hipLaunchKernelGGL(accept_partial_kernel, dim3(W.acc_blocks), dim3(256), 0, st, W.n, W.x, W.hsol, W.cand_e,
+                       W.n_cand, W.gs, W.acc_partial, W.gn_done);
+    hipLaunchKernelGGL(accept_final_kernel, dim3(1), dim3(256), 0, st, W.acc_blocks, W.acc_partial, W.cand_e,
+                       W.n_cand, W.gs, energies, iter, stop_rule, W.gn_done);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -1394,9 +1452,9 @@ int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipS
 int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st) {
     const SolverDev& S = W.dev;
     hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, x, (const float*)nullptr, W.e_partial,
-                       (const int*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, out,
-                       (const int*)nullptr);
+                       (const int*)nullptr, 0, (const double*)nullptr, (const GnScalars*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, out,
+                       (const int*)nullptr, (const double*)nullptr, (const GnScalars*)nullptr);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -76,6 +76,13 @@ struct SolverWork {
     unsigned long long* ac64;  // fixed-point coarse assembly (order independent)
     double *partial, *partial2, *rzp, *e_partial, *cand_e;  // rzp: 2 x n_agg r.z partials
     int e_blocks, n_cand;
+    double* acc_partial;  // 2 x acc_blocks: |x|^2, |h|^2 partials of the step acceptance
+    int acc_blocks;
+    // Warm start: the CG of GN iteration k starts from the step of GN iteration
+    // k of the previous solve (hwarm[k], zero before the first), when warm != 0.
+    float* hwarm;
+    int warm;
+    int warm_slots;
     CgScalars* cg;
     GnScalars* gs;
     int* gn_done;
@@ -107,7 +114,7 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err);
 void solver_free(SolverWork& W);
 
 int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st);
-int solver_gn_begin(SolverWork& W, hipStream_t st);
+int solver_gn_begin(SolverWork& W, int iter, hipStream_t st);
 int solver_cg_iteration(SolverWork& W, int k, hipStream_t st);
 // The whole PCG solve of one GN iteration as one cooperative launch (after
 // solver_gn_begin); same arithmetic and stopping rule as solver_cg_iteration.

@@ -239,6 +239,11 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.n_cand = 50;
     W.e_partial = alloc<double>(W, (size_t)W.e_blocks * W.n_cand, rc);
     W.cand_e = alloc<double>(W, W.n_cand, rc);
+    W.acc_blocks = (int)std::min<int64_t>(128, std::max<int64_t>(1, (W.n + 255) / 256));
+    W.acc_partial = alloc<double>(W, 2 * (size_t)W.acc_blocks, rc);
+    W.warm = 0;
+    W.warm_slots = 8;
+    W.hwarm = alloc<float>(W, (size_t)W.warm_slots * W.n, rc);
     W.cg = alloc<CgScalars>(W, 1, rc);
     W.gs = alloc<GnScalars>(W, 1, rc);
     W.gn_done = alloc<int>(W, 1, rc);

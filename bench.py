@@ -145,6 +145,7 @@ def timed_pipelined(B, steps, warmup):
     reset_state(B)
     B["solver"].set_cg_mode(0)
     B["solver"].set_host_checks(B["cfg"]["host_checks"])
+    B["solver"].set_warm_start(B["cfg"]["warm"])
     if "pipe" not in B:
         B["pipe"] = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                                  B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"])
@@ -166,6 +167,7 @@ def timed(B, plain, steps, warmup):
     reset_state(B)
     B["solver"].set_cg_mode(0)
     B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
+    B["solver"].set_warm_start(0 if plain else B["cfg"]["warm"])
     for _ in range(warmup):
         frame(B, plain)
     torch.cuda.synchronize()

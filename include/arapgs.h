@@ -127,6 +127,14 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every);
  * CG iteration count, never the solution. */
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
 
+/* on = 1: the CG of GN iteration k starts from the step that GN iteration k of
+ * the previous solve on this solver found (zero before the first), instead of
+ * from zero. Consecutive drag frames solve nearly the same systems, so the CG
+ * starts closer to its tolerance; the tolerance and stopping rule are
+ * unchanged. Default 0 (every solve independent of the ones before). Every
+ * call clears the stored steps (synchronous). */
+int arapgs_solver_set_warm_start(arapgs_solver* s, int on);
+
 /* How each GN iteration's PCG solve (the dense Eigen solve of Deform.cpp:118-128
  * in the reference) is launched: 0 (default) = three kernels per CG iteration
  * (combine with set_host_checks to bound the launches queued past convergence);

@@ -194,3 +194,19 @@ def test_center_mode_large_block_keeps_first_20(ctx, prob):
     e2 = float(H(s.energy(T(nodes), T(aims2), T(ref.identity_x().astype(np.float32))))[0])
     e1 = float(H(s.energy(T(nodes), T(aims), T(ref.identity_x().astype(np.float32))))[0])
     assert e1 == e2
+
+
+def test_warm_start_solves_match_oracle(ctx, prob):
+    """Warm-started CG (each GN iteration starts from the previous solve's step):
+    a sequence of drag frames keeps the oracle's energies (1e-5) and GN
+    iteration counts (within 1)."""
+    s = _solver(ctx, prob)
+    s.set_warm_start(True)
+    for fraction in (0.30, 0.35, 0.40, 0.45):
+        aims, ref = _problem(prob, fraction)
+        x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+        x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+        e, done = H(e), int(H(done)[0])
+        assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (fraction, e[: done + 1], e_ref)
+        assert abs(done - it_ref) <= 1, (fraction, done, it_ref)
+    s.set_warm_start(False)
