@@ -67,6 +67,8 @@ EXPORTS = {
     "arapgs_last_error": ([P], C.c_char_p),
     "arapgs_set_stream": ([P, P], C.c_int),
     "arapgs_synchronize": ([P], C.c_int),
+    "arapgs_stream_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_stream_destroy": ([P], C.c_int),
     "arapgs_version": ([], C.c_char_p),
     "arapgs_fps": ([P, P, C.c_int64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
     "arapgs_knn": ([P, P, C.c_int, P, C.c_int64, C.c_int, P, P], C.c_int),
@@ -156,6 +158,25 @@ def ptr(t):
     if not t.is_contiguous():
         raise ArapgsError("arapgs calls take contiguous tensors")
     return C.c_void_p(t.data_ptr())
+
+
+class CuStream:
+    """A HIP stream limited to compute units [cu_begin, cu_begin + cu_count)
+    (all of them when cu_count is 0), usable as a torch stream (.torch)."""
+
+    def __init__(self, device=0, cu_begin=0, cu_count=0):
+        import torch
+        h = P()
+        rc = lib().arapgs_stream_create(device, cu_begin, cu_count, C.byref(h))
+        if rc != 0:
+            raise ArapgsError(f"arapgs_stream_create({cu_begin}, {cu_count}) failed ({rc})")
+        self.h = h
+        self.torch = torch.cuda.ExternalStream(h.value, device=torch.device("cuda", device))
+
+    def close(self):
+        if self.h:
+            lib().arapgs_stream_destroy(self.h)
+            self.h = None
 
 
 def ply_info(path):

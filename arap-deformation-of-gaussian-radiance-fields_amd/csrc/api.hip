@@ -19,6 +19,9 @@
 
 using namespace arapgs;
 
+// byte offset (per instance) of the sorted record ids in arapgs_ctx::r_bin
+constexpr size_t BIN_VALS_AT = 8;
+
 struct arapgs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -179,6 +182,33 @@ int arapgs_set_stream(arapgs_ctx* c, void* s) {
     CHECK_CTX(c);
     c->stream = (hipStream_t)s;
     return 0;
+}
+
+int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_out) {
+    if (!stream_out || cu_begin < 0 || cu_count < 0) return -1;
+    *stream_out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    hipStream_t st = nullptr;
+    hipError_t e;
+    if (cu_count == 0) {
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    } else {
+        int ncu = 0;
+        e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess) return -1000 - (int)e;
+        if (cu_begin + cu_count > ncu) return -1;
+        std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+        for (int cu = cu_begin; cu < cu_begin + cu_count; cu++) mask[cu / 32] |= 1u << (cu % 32);
+        e = hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data());
+    }
+    if (e != hipSuccess) return -1000 - (int)e;
+    *stream_out = st;
+    return 0;
+}
+
+int arapgs_stream_destroy(void* stream) {
+    if (!stream) return -1;
+    return hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : -1;
 }
 
 int arapgs_synchronize(arapgs_ctx* c) {
@@ -576,14 +606,15 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     while (n > 0 && dbits < 32 && ((uint64_t)(cnt[2] - cnt[1]) >> dbits) != 0) dbits++;
     if (n > 0x7fffffffu) return fail(c, -1, "render: too many tile instances (%u)", n);
     const size_t nn = std::max<size_t>(n, 1);
-    rc = grow(c, &c->r_bin, &c->r_bin_cap, nn * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t)) + 1024);
+    rc = grow(c, &c->r_bin, &c->r_bin_cap, nn * 4 * sizeof(uint32_t) + 1024);
     if (!rc) rc = grow(c, &c->r_sort, &c->r_sort_cap, sizeof(uint32_t) * radix_workspace_words((int64_t)nn));
     if (rc) return rc;
+    // r_bin: keys (tile ids), keys_alt, vals (record ids), vals_alt; BIN_VALS_AT is the vals offset
     char* b = (char*)c->r_bin;
-    uint64_t* keys = (uint64_t*)b;
-    uint64_t* keys_alt = (uint64_t*)(b + nn * 8);
-    uint32_t* vals = (uint32_t*)(b + nn * 16);
-    uint32_t* vals_alt = (uint32_t*)(b + nn * 20);
+    uint32_t* keys = (uint32_t*)b;
+    uint32_t* keys_alt = (uint32_t*)(b + nn * 4);
+    uint32_t* vals = (uint32_t*)(b + nn * BIN_VALS_AT);
+    uint32_t* vals_alt = (uint32_t*)(b + nn * 12);
     rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, dmin, dbits, keys, vals, keys_alt,
                                               vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->r_rec,
                                               (uint32_t*)c->r_scan, c->stream),
@@ -622,15 +653,16 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess && keys_h && n > 0) {
         // the sorted keys hold the tile id; the reference layout is tile << 32 | depth bits of the value's record
-        std::vector<uint32_t> v((size_t)n);
+        std::vector<uint32_t> v((size_t)n), tile((size_t)n);
         const int64_t PV = (int64_t)std::max(c->last_rp.P, 1) * c->last_rp.V;
         std::vector<uint32_t> depth((size_t)PV);
-        e = hipMemcpy(keys_h, b, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(v.data(), b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+        e = hipMemcpy(tile.data(), b, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(v.data(), b + nn * BIN_VALS_AT, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(depth.data(), c->r_geo, sizeof(uint32_t) * PV, hipMemcpyDeviceToHost);
-        for (int64_t i = 0; e == hipSuccess && i < n; i++) keys_h[i] = (keys_h[i] << 32) | depth[v[i]];
+        for (int64_t i = 0; e == hipSuccess && i < n; i++) keys_h[i] = ((uint64_t)tile[i] << 32) | depth[v[i]];
     }
-    if (e == hipSuccess && vals_h && n > 0) e = hipMemcpy(vals_h, b + nn * 16, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && vals_h && n > 0)
+        e = hipMemcpy(vals_h, b + nn * BIN_VALS_AT, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
     const int64_t nr = std::min<int64_t>(ranges_cap, c->last_tiles);
     if (e == hipSuccess && ranges_h && nr > 0) e = hipMemcpy(ranges_h, c->r_ranges, sizeof(uint2) * nr, hipMemcpyDeviceToHost);
     return e == hipSuccess ? 0 : hip_fail(c, e, "render_debug");
@@ -1192,7 +1224,8 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
         RasterParams rp = c->last_rp;
         rp.views = c->r_views;
         rc = check_launch(c, raster_blend_backward(rp, c->r_geo, (const uint32_t*)((char*)c->r_bin +
-                                                                                     std::max<size_t>(c->last_rendered, 1) * 16),
+                                                                                     std::max<size_t>(c->last_rendered, 1) *
+                                                                                         BIN_VALS_AT),
                                                    (const uint2*)c->r_ranges, ch->grec, st),
                           "chain blend backward");
         if (rc) return rc;

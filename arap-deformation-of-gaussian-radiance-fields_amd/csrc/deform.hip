@@ -276,7 +276,14 @@ __global__ void node_quats_kernel(int M, const NodeXf* __restrict__ table, float
 
 // Running slerp from identity towards each neighbour node's quaternion with
 // t = w_j / (w_j + sum of previous weights) (Q_SlerpCUDA, cudakdtree.cu:113-148),
-// then SH rotation of the sample's 48-float feature.
+// then SH rotation of the sample's 48-float feature. (Staging the features
+// through LDS for coalesced rows measured no faster: the kernel is bound by
+// the slerp's transcendentals and the node gathers, not by the row loads.)
+template <int K>
+__device__ __forceinline__ void sample_sh_one(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
+                                              const float* __restrict__ node_q, const ShRotTables* __restrict__ T,
+                                              float* __restrict__ row);
+
 template <int K>
 __global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* __restrict__ w,
                                                         const uint32_t* __restrict__ idx,
@@ -286,6 +293,13 @@ __global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* 
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     if (is_static && is_static[s]) return;
+    sample_sh_one<K>(s, w, idx, node_q, T, feat + s * 48);
+}
+
+template <int K>
+__device__ __forceinline__ void sample_sh_one(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
+                                              const float* __restrict__ node_q, const ShRotTables* __restrict__ T,
+                                              float* __restrict__ row) {
     float acc[4] = {1.0f, 0.0f, 0.0f, 0.0f};
     float last = 0.0f;
 #pragma unroll
@@ -317,7 +331,7 @@ __global__ void __launch_bounds__(256) sample_sh_kernel(int64_t S, const float* 
     quat_normalize(acc);
     float R[3][3];
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
-    sh_rotate_f4(*T, R, feat + s * 48);
+    sh_rotate_f4(*T, R, row);
 }
 
 __global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh,

@@ -447,14 +447,7 @@ __device__ __forceinline__ void sh_rotate_band3(const ShRotTables& T, const floa
 
 // Rotates one 48-float SH set in global memory in place: twelve 16-byte loads
 // and stores (sh must be 16-byte aligned), bands 1-3 rotated in registers.
-__device__ __forceinline__ void sh_rotate_f4(const ShRotTables& T, const float R[3][3], float* __restrict__ sh) {
-    float c[48];
-    const float4* s4 = reinterpret_cast<const float4*>(sh);
-#pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const float4 v = s4[i];
-        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
-    }
+__device__ __forceinline__ void sh_rotate_regs(const ShRotTables& T, const float R[3][3], float (&c)[48]) {
     float b1[9], b2[15], b3[21];
 #pragma unroll
     for (int i = 0; i < 9; i++) b1[i] = c[3 + i];
@@ -471,6 +464,17 @@ __device__ __forceinline__ void sh_rotate_f4(const ShRotTables& T, const float R
     for (int i = 0; i < 15; i++) c[12 + i] = b2[i];
 #pragma unroll
     for (int i = 0; i < 21; i++) c[27 + i] = b3[i];
+}
+
+__device__ __forceinline__ void sh_rotate_f4(const ShRotTables& T, const float R[3][3], float* __restrict__ sh) {
+    float c[48];
+    const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const float4 v = s4[i];
+        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+    }
+    sh_rotate_regs(T, R, c);
     float4* o4 = reinterpret_cast<float4*>(sh);
 #pragma unroll
     for (int i = 0; i < 12; i++) o4[i] = make_float4(c[4 * i], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);

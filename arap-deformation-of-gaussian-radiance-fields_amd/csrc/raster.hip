@@ -213,10 +213,10 @@ __global__ void depth_range_init_kernel(uint32_t* __restrict__ dminmax) {
 // wide sort runs on the V*P records instead of the instances, and the instance
 // sort needs only the tile-id bits.
 __global__ void __launch_bounds__(256) record_keys_kernel(int64_t PV, Geo geo, uint32_t dmin,
-                                                          uint64_t* __restrict__ rkeys, uint32_t* __restrict__ rvals) {
+                                                          uint32_t* __restrict__ rkeys, uint32_t* __restrict__ rvals) {
     const int64_t gv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gv >= PV) return;
-    rkeys[gv] = geo.touched[gv] ? (uint64_t)(__float_as_uint(geo.depth[gv]) - dmin) : 0ull;
+    rkeys[gv] = geo.touched[gv] ? __float_as_uint(geo.depth[gv]) - dmin : 0u;
     rvals[gv] = (uint32_t)gv;
 }
 
@@ -228,25 +228,60 @@ __global__ void __launch_bounds__(256) gather_touched_kernel(int64_t PV, const u
     out[r] = touched[rvals[r]];
 }
 
-// One thread per depth-ordered record r: its tiles' keys (view * ntiles + tile)
-// and values (the record index) from offsets[r] on.
+// The tiles of the depth-ordered records: the keys (view * ntiles + tile) and
+// values (the record index) of record r fill slots [offsets[r], offsets[r] +
+// count) in row-major rect order. One wave per 64 consecutive records writes
+// their slots, which are consecutive, with consecutive lanes on consecutive
+// slots: the lane of slot j finds its record (the last of the 64 whose first
+// slot is <= j) by a binary search over the lanes' first slots.
 __global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
                                                          const uint32_t* __restrict__ rvals,
                                                          const uint32_t* __restrict__ offsets,
-                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int lane = threadIdx.x & 63;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= PV) return;
-    const uint32_t gv = rvals[r];
-    if (geo.touched[gv] == 0) return;
-    const uint32_t base = (gv / (uint32_t)P) * (uint32_t)ntiles;
-    uint32_t o = offsets[r];
-    const int4 rc = geo.rect[gv];
-    for (int y = rc.y; y < rc.w; y++)
-        for (int x = rc.x; x < rc.z; x++) {
-            keys[o] = (uint64_t)(base + (uint32_t)(y * gx + x));
-            vals[o] = gv;
-            o++;
+    const bool valid = r < PV;
+    uint32_t gv = 0, first = 0xffffffffu, cnt = 0, base = 0;
+    int rx = 0, ry = 0, rw = 1;
+    if (valid) {
+        gv = rvals[r];
+        first = offsets[r];
+        cnt = geo.touched[gv];
+        if (cnt) {
+            const int4 rc = geo.rect[gv];
+            rx = rc.x;
+            ry = rc.y;
+            rw = rc.z - rc.x;
+            base = (gv / (uint32_t)P) * (uint32_t)ntiles;
         }
+    }
+    const uint64_t vmask = __ballot(valid);
+    if (vmask == 0ull) return;
+    const int last = 63 - __builtin_clzll(vmask);
+    const uint32_t begin = __shfl(first, 0);
+    const uint32_t end = __shfl(first + cnt, last);
+    // uniform trip count: every lane takes part in the shuffles (a shuffle from
+    // a lane that left a divergent loop does not return that lane's value)
+    for (uint32_t jb = begin; jb < end; jb += 64) {
+        const uint32_t j = jb + (uint32_t)lane;
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const uint32_t f = __shfl(first, lo + step);
+            if (f <= j) lo += step;
+        }
+        const uint32_t f0 = __shfl(first, lo);
+        const int w = __shfl(rw, lo);
+        const int x0 = __shfl(rx, lo), y0 = __shfl(ry, lo);
+        const uint32_t b = __shfl(base, lo);
+        const uint32_t g = __shfl(gv, lo);
+        if (j < end) {
+            const int q = (int)(j - f0);
+            const int y = y0 + q / w, x = x0 + q % w;
+            keys[j] = b + (uint32_t)(y * gx + x);
+            vals[j] = g;
+        }
+    }
 }
 
 __global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
@@ -254,13 +289,13 @@ __global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
         p[i] = 0u;
 }
 
-__global__ void ranges_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr, int dbits,
+__global__ void ranges_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ n_ptr,
                               uint2* __restrict__ ranges) {
     const uint32_t n = *n_ptr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t t = (uint32_t)(keys[i] >> dbits);
-        if (i == 0 || (uint32_t)(keys[i - 1] >> dbits) != t) ranges[t].x = i;
-        if (i == n - 1 || (uint32_t)(keys[i + 1] >> dbits) != t) ranges[t].y = i + 1;
+        const uint32_t t = keys[i];
+        if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
+        if (i == n - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
     }
 }
 
@@ -903,11 +938,11 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
 }
 
 size_t raster_record_bytes(int64_t PV) {
-    return (size_t)PV * (8 + 8 + 4 + 4 + 4) + sizeof(uint32_t) * radix_workspace_words(PV) + 6 * 256;
+    return (size_t)PV * (4 + 4 + 4 + 4 + 4) + sizeof(uint32_t) * radix_workspace_words(PV) + 6 * 256;
 }
 
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
-                         uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                         uint32_t dmin, int dbits, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
                          hipStream_t st) {
     const int64_t PV = (int64_t)rp.P * rp.V;
@@ -923,14 +958,14 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
             p += (bytes + 255) & ~(size_t)255;
             return r;
         };
-        uint64_t* rkeys = (uint64_t*)take(8 * (size_t)PV);
-        uint64_t* rkeys_alt = (uint64_t*)take(8 * (size_t)PV);
+        uint32_t* rkeys = (uint32_t*)take(4 * (size_t)PV);
+        uint32_t* rkeys_alt = (uint32_t*)take(4 * (size_t)PV);
         uint32_t* rvals = (uint32_t*)take(4 * (size_t)PV);
         uint32_t* rvals_alt = (uint32_t*)take(4 * (size_t)PV);
         uint32_t* rtouched = (uint32_t*)take(4 * (size_t)PV);
         uint32_t* rsort_ws = (uint32_t*)take(sizeof(uint32_t) * radix_workspace_words(PV));
         hipLaunchKernelGGL(record_keys_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, geo, dmin, rkeys, rvals);
-        int rc = launch_radix_sort(rkeys, rvals, rkeys_alt, rvals_alt, PV, dbits, rsort_ws, st);
+        int rc = launch_radix_sort32(rkeys, rvals, rkeys_alt, rvals_alt, PV, dbits, rsort_ws, st);
         if (rc) return rc;
         // instance offsets in depth order, then the tiles of every record
         hipLaunchKernelGGL(gather_touched_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rvals, geo.touched,
@@ -942,10 +977,10 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
                            rvals, offs, keys, vals);
         int tile_bits = 0;
         while ((int64_t(1) << tile_bits) < nt) tile_bits++;
-        rc = launch_radix_sort(keys, vals, keys_alt, vals_alt, n_rendered, tile_bits, sort_ws, st);
+        rc = launch_radix_sort32(keys, vals, keys_alt, vals_alt, n_rendered, tile_bits, sort_ws, st);
         if (rc) return rc;
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
-                           keys, d_total, 0, ranges);
+                           keys, d_total, ranges);
     }
     if (rp.tile == 16) {
         if (rp.unroll == 2)

@@ -85,7 +85,7 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
 // preprocess's scan workspace (sized for V*P).
 size_t raster_record_bytes(int64_t PV);
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
-                         uint32_t dmin, int dbits, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                         uint32_t dmin, int dbits, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
                          hipStream_t st);
 

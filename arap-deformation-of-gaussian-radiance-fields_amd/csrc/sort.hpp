@@ -19,5 +19,8 @@ int launch_exclusive_scan(const uint32_t* in, int64_t n, uint32_t* out, uint32_t
 size_t radix_workspace_words(int64_t n);
 int launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
                       uint32_t* ws, hipStream_t st);
+// The same for u32 keys (same workspace).
+int launch_radix_sort32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int64_t n, int bits,
+                        uint32_t* ws, hipStream_t st);
 
 }  // namespace arapgs

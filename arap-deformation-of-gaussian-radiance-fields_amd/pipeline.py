@@ -33,14 +33,16 @@ from . import arapgs
 
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
-                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16):
+                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None):
+        """The solve side runs on solver.ctx's stream, the deform side on
+        stream_b (a new stream when None), e.g. two arapgs.CuStream partitions."""
         import torch
         self.torch = torch
         self.solver = solver
         self.ctx_a = solver.ctx
         dev = state["pos"].device
         self.stream_a = self.ctx_a.stream
-        self.stream_b = torch.cuda.Stream(device=dev)
+        self.stream_b = stream_b if stream_b is not None else torch.cuda.Stream(device=dev)
         self.ctx_b = render_ctx if render_ctx is not None else arapgs.Context(dev.index or 0, stream=self.stream_b)
         if render_ctx is not None:
             self.stream_b = render_ctx.stream

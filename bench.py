@@ -137,29 +137,59 @@ def _frame(B, plain):
     return n_inst
 
 
-def timed_pipelined(B, steps, warmup):
+def timed_pipelined(B, steps, warmup, solve_cus=None):
     """Frame t's deformation and renders overlap frame t+1's solve (two streams,
-    two host threads; pipeline.DragPipeline)."""
+    two host threads; pipeline.DragPipeline). solve_cus > 0: the solve stream
+    runs on CUs [0, solve_cus) and the deform stream on the others."""
     import torch
+    from arapgs_amd import arapgs
     from arapgs_amd.pipeline import DragPipeline
+    cus = B["cfg"].get("solve_cus", 0) if solve_cus is None else solve_cus
     reset_state(B)
-    B["solver"].set_cg_mode(0)
-    B["solver"].set_host_checks(B["cfg"]["host_checks"])
-    B["solver"].set_warm_start(B["cfg"]["warm"])
-    if "pipe" not in B:
-        B["pipe"] = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
-                                 B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"])
-        B["pipe"].ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
-    pipe = B["pipe"]
-    pipe.run(warmup)
-    torch.cuda.synchronize()
-    stamps = []
-    t0 = time.perf_counter()
-    pipe.run(steps, on_frame=lambda t, n: stamps.append(time.perf_counter()))
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
+    ctx = B["ctx"]
+    base_stream = ctx.stream
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    streams = []
+    if cus > 0:
+        streams = [arapgs.CuStream(0, 0, cus), arapgs.CuStream(0, cus, n_cu - cus)]
+        ctx.set_stream(streams[0].torch)
+    try:
+        B["solver"].set_cg_mode(0)
+        B["solver"].set_host_checks(B["cfg"]["host_checks"])
+        B["solver"].set_warm_start(B["cfg"]["warm"])
+        pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
+                            B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
+                            stream_b=streams[1].torch if streams else None)
+        pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
+        pipe.run(warmup)
+        torch.cuda.synchronize()
+        stamps = []
+        t0 = time.perf_counter()
+        pipe.run(steps, on_frame=lambda t, n: stamps.append(time.perf_counter()))
+        torch.cuda.synchronize()
+        total = time.perf_counter() - t0
+        instances = pipe.instances
+        pipe.close()
+    finally:
+        torch.cuda.synchronize()
+        ctx.set_stream(base_stream)
+        for s_ in streams:
+            s_.close()
     times = [(b - a) * 1e3 for a, b in zip([t0] + stamps[:-1], stamps)]
-    return dict(total_s=total, frame_ms=times, kernel_ms=[], instances=pipe.instances)
+    return dict(total_s=total, frame_ms=times, kernel_ms=[], instances=instances)
+
+
+def partition_leg(B, args):
+    """Picks the solve stream's CU partition for the pipelined form (0 = both
+    streams on all CUs) from a few pipelined frames per candidate."""
+    best, best_t = 0, None
+    for cus in (0, 16, 32, 48, 64):
+        r = timed_pipelined(B, args.tune_steps * 2, 2, solve_cus=cus)
+        t = r["total_s"]
+        log(f"  partition: solve on {cus} CUs -> {t / (args.tune_steps * 2) * 1e3:.3f} ms/frame")
+        if best_t is None or t < best_t:
+            best, best_t = cus, t
+    return best
 
 
 def timed(B, plain, steps, warmup):
@@ -418,6 +448,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the autotuner (tile, blend unroll, aggregates, coarse refresh, CG host checks)")
     ap.add_argument("--tune-steps", type=int, default=6, help="timed frames per autotuner candidate")
+    ap.add_argument("--partition", action="store_true",
+                    help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
     ap.add_argument("--realign-steps", type=int, default=5, help="steps of the realignment leg (0: skip)")
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
@@ -453,6 +485,11 @@ def main():
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")
         if form == "pipelined":
+            if args.partition and "solve_cus" not in B["cfg"]:
+                B["cfg"]["solve_cus"] = partition_leg(B, args)
+                log(f"  partition: solve stream on {B['cfg']['solve_cus']} CUs")
+                if tuning is not None:
+                    tuning["config"] = dict(B["cfg"])
             res[form] = timed_pipelined(B, args.steps, args.warmup)
         else:
             res[form] = timed(B, form == "plain", args.steps, args.warmup)

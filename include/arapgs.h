@@ -49,6 +49,13 @@ const char* arapgs_last_error(const arapgs_ctx* ctx);
 /* hip_stream is a hipStream_t (NULL = the null stream). */
 int arapgs_set_stream(arapgs_ctx* ctx, void* hip_stream);
 int arapgs_synchronize(arapgs_ctx* ctx);
+/* A HIP stream on `device` whose kernels run only on the compute units
+ * [cu_begin, cu_begin + cu_count) (hipExtStreamCreateWithCUMask); cu_count = 0:
+ * an ordinary non-blocking stream. Lets a latency-bound stage (the solve) keep
+ * CUs of its own while a bandwidth-bound stage fills the rest. No counterpart
+ * in the reference. Destroy with arapgs_stream_destroy. */
+int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_out);
+int arapgs_stream_destroy(void* stream);
 /* Library version string (static storage). */
 const char* arapgs_version(void);
 
