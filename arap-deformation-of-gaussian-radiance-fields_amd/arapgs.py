@@ -99,6 +99,8 @@ EXPORTS = {
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
+    "arapgs_solver_last_stats": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "arapgs_solver_apply": ([P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
     "arapgs_aims_translate": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_aims_twist": ([P, P, C.c_int, P, C.c_int, P, C.c_float, P], C.c_int),
@@ -551,6 +553,22 @@ class Solver:
 
     def set_coarse_refresh(self, frames):
         self.ctx._check(lib().arapgs_solver_set_coarse_refresh(self.h, frames))
+
+    def apply(self, node_pos, aims, x, v):
+        """(J^T J v, -J^T f) at unknowns x (test hook)."""
+        t = self.ctx._torch
+        q = t.empty_like(v)
+        b = t.empty_like(v)
+        self.ctx._check(lib().arapgs_solver_apply(self.ctx.h, self.h, ptr(node_pos), ptr(aims), ptr(x), ptr(v), ptr(q),
+                                                  ptr(b)))
+        return q, b
+
+    def last_stats(self):
+        """(GN iterations, total CG iterations) of the last solve (synchronous)."""
+        gn, cg = C.c_int(0), C.c_int(0)
+        if lib().arapgs_solver_last_stats(self.h, C.byref(gn), C.byref(cg)) != 0:
+            raise ArapgsError("last_stats failed")
+        return gn.value, cg.value
 
     def set_warm_start(self, on=True):
         if lib().arapgs_solver_set_warm_start(self.h, int(bool(on))) != 0:

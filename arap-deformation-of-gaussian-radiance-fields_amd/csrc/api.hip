@@ -352,6 +352,26 @@ int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode) {
     return 0;
 }
 
+int arapgs_solver_apply(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
+                        const float* v, float* q, float* b) {
+    CHECK_CTX(c);
+    if (!s || !node_pos || !aims || !x || !v || !q || !b) return fail(c, -1, "solver_apply: bad arguments");
+    s->w.dev.g = node_pos;
+    s->w.dev.aim = aims;
+    return check_launch(c, solver_apply_test(s->w, x, v, q, b, c->stream), "solver_apply");
+}
+
+int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h) {
+    if (!s) return -1;
+    (void)hipSetDevice(s->device);
+    GnScalars gs;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&gs, s->w.gs, sizeof gs, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1000;
+    if (gn_iters_h) *gn_iters_h = gs.iters_done;
+    if (cg_iters_h) *cg_iters_h = gs.cg_total;
+    return 0;
+}
+
 int arapgs_solver_set_warm_start(arapgs_solver* s, int on) {
     if (!s || on < 0 || on > 1) return -1;
     (void)hipSetDevice(s->device);

@@ -220,25 +220,26 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 
 // ---------------------------------------------------------------- owner walk
 // Visit every residual group touching free node fi with its Jacobian block
-// wrt fi and the group's J*v for a given vector v (12 per free node):
-//   fn(rows, Ji[rows][12], Jv[rows]).
-// Items: 0 = rot, 1..K = out-edges, then in-edges, then constraint slots.
+// wrt fi and (WITH_JV) the group's J*v for a given vector v (12 per free node).
+// Item 0 is the rotation group: fr(Ji[6][9] (the R columns), Jv[6]). Every
+// other group has 3 rows whose block wrt a node is I_3-structured — row r
+// holds (a_0, a_1, a_2) at the R columns 3c + r and b at the t column 9 + r —
+// so it is passed as f3(a[3], b, Jv[3]).
+// Items: 0 = rot, 1..K = out-edges, then in-edges, then constraint memberships.
 // KT > 0: the neighbour count is a compile-time constant (== S.K), so the
 // constraint item's K-node loop unrolls and its gathers issue together.
-template <int KT = 0, typename Fn>
+template <int KT = 0, bool WITH_JV = true, typename FnRot, typename Fn3>
 __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item, const float* __restrict__ x,
-                                           const float* __restrict__ v, Fn&& fn) {
+                                           const float* __restrict__ v, FnRot&& fr, Fn3&& f3) {
     const int i = S.free_nodes[fi];
-    float Ji[6][12];
-    float Jv[6];
-#pragma unroll
-    for (int a = 0; a < 6; a++) {
-        Jv[a] = 0.0f;
-#pragma unroll
-        for (int b = 0; b < 12; b++) Ji[a][b] = 0.0f;
-    }
     if (item == 0) {
         // rot(i): d(c_a . c_b)/d c_a = c_b, d|c_j|^2 / d c_j = 2 c_j
+        float Ji[6][9];
+        float Jv[6];
+#pragma unroll
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int b = 0; b < 9; b++) Ji[a][b] = 0.0f;
         const float* xi = x + 12 * (int64_t)fi;
         const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
 #pragma unroll
@@ -256,13 +257,17 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
 #pragma unroll
         for (int a = 0; a < 6; a++) {
             float s = 0.0f;
+            if (WITH_JV) {
 #pragma unroll
-            for (int b = 0; b < 9; b++) s += Ji[a][b] * v[vo + b];
+                for (int b = 0; b < 9; b++) s += Ji[a][b] * v[vo + b];
+            }
             Jv[a] = s;
         }
-        fn(6, Ji, Jv);
+        fr(Ji, Jv);
         return;
     }
+    float a[3] = {0.0f, 0.0f, 0.0f}, b = 0.0f;
+    float Jv[3] = {0.0f, 0.0f, 0.0f};
     item -= 1;
     if (item < S.K) {
         const int t = item;
@@ -272,17 +277,19 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         const float* gi = S.g + 3 * i;
         const float* gk = S.g + 3 * k;
         const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
-        const int64_t vo = 12 * (int64_t)fi;
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) a[c] = S.sw_reg * s * d[c];
+        b = S.sw_reg;
+        if (WITH_JV) {
+            const int64_t vo = 12 * (int64_t)fi;
 #pragma unroll
-            for (int c = 0; c < 3; c++) Ji[r][3 * c + r] = S.sw_reg * s * d[c];
-            Ji[r][9 + r] = S.sw_reg;
-            float jv = S.sw_reg * (s * (v[vo + r] * d[0] + v[vo + 3 + r] * d[1] + v[vo + 6 + r] * d[2]) + v[vo + 9 + r]);
-            if (fk >= 0) jv -= S.sw_reg * s * v[12 * (int64_t)fk + 9 + r];
-            Jv[r] = jv;
+            for (int r = 0; r < 3; r++) {
+                float jv = S.sw_reg * (s * (v[vo + r] * d[0] + v[vo + 3 + r] * d[1] + v[vo + 6 + r] * d[2]) + v[vo + 9 + r]);
+                if (fk >= 0) jv -= S.sw_reg * s * v[12 * (int64_t)fk + 9 + r];
+                Jv[r] = jv;
+            }
         }
-        fn(3, Ji, Jv);
+        f3(a, b, Jv);
         return;
     }
     item -= S.K;
@@ -293,25 +300,26 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         const int fs = S.free_id[src];
         if (fs < 0) {
             // static(src, t): rows -sw_reg t_i
+            b = -S.sw_reg;
+            if (WITH_JV) {
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                Ji[r][9 + r] = -S.sw_reg;
-                Jv[r] = -S.sw_reg * v[12 * (int64_t)fi + 9 + r];
+                for (int r = 0; r < 3; r++) Jv[r] = -S.sw_reg * v[12 * (int64_t)fi + 9 + r];
             }
         } else {
             const float s = S.edge_scale[e];
-            const float* gs = S.g + 3 * src;
-            const float* gi = S.g + 3 * i;
-            const float d[3] = {gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2]};
-            const int64_t so = 12 * (int64_t)fs;
+            b = -S.sw_reg * s;
+            if (WITH_JV) {
+                const float* gs = S.g + 3 * src;
+                const float* gi = S.g + 3 * i;
+                const float d[3] = {gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2]};
+                const int64_t so = 12 * (int64_t)fs;
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                Ji[r][9 + r] = -S.sw_reg * s;
-                Jv[r] = S.sw_reg * (s * (v[so + r] * d[0] + v[so + 3 + r] * d[1] + v[so + 6 + r] * d[2]) + v[so + 9 + r]) -
-                        S.sw_reg * s * v[12 * (int64_t)fi + 9 + r];
+                for (int r = 0; r < 3; r++)
+                    Jv[r] = S.sw_reg * (s * (v[so + r] * d[0] + v[so + 3 + r] * d[1] + v[so + 6 + r] * d[2]) + v[so + 9 + r]) -
+                            S.sw_reg * s * v[12 * (int64_t)fi + 9 + r];
             }
         }
-        fn(3, Ji, Jv);
+        f3(a, b, Jv);
         return;
     }
     item -= n_in;
@@ -326,32 +334,31 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
             const float wi = S.src_w[e];
             const float dd[3] = {S.g[3 * c] - gi[0], S.g[3 * c + 1] - gi[1], S.g[3 * c + 2] - gi[2]};
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
+            for (int cc = 0; cc < 3; cc++) a[cc] += S.sw_con * wi * dd[cc];
+            b += S.sw_con * wi;
+        }
+    }
+    if (WITH_JV) {
+        const int KK = KT > 0 ? KT : S.K;
+        for (int m = S.con_ptr[cid]; m < S.con_ptr[cid + 1]; m++) {
+            const int c = S.con_mem[m];
+            const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
 #pragma unroll
-                for (int cc = 0; cc < 3; cc++) Ji[r][3 * cc + r] += S.sw_con * wi * dd[cc];
-                Ji[r][9 + r] += S.sw_con * wi;
+            for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
+                if (KT == 0 && j >= KK) break;
+                const int nj = (int)S.src_idx[(int64_t)c * KK + j];
+                const int fj = S.free_id[nj];
+                if (fj < 0) continue;
+                const float w = S.src_w[(int64_t)c * KK + j];
+                const float* gj = S.g + 3 * nj;
+                const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
+                const int64_t jo = 12 * (int64_t)fj;
+#pragma unroll
+                for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
             }
         }
     }
-    const int KK = KT > 0 ? KT : S.K;
-    for (int m = S.con_ptr[cid]; m < S.con_ptr[cid + 1]; m++) {
-        const int c = S.con_mem[m];
-        const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
-#pragma unroll
-        for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
-            if (KT == 0 && j >= KK) break;
-            const int nj = (int)S.src_idx[(int64_t)c * KK + j];
-            const int fj = S.free_id[nj];
-            if (fj < 0) continue;
-            const float w = S.src_w[(int64_t)c * KK + j];
-            const float* gj = S.g + 3 * nj;
-            const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
-            const int64_t jo = 12 * (int64_t)fj;
-#pragma unroll
-            for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
-        }
-    }
-    fn(3, Ji, Jv);
+    f3(a, b, Jv);
 }
 
 __device__ __forceinline__ int owner_items(const SolverDev& S, int fi) {
@@ -407,6 +414,10 @@ __device__ void invert12(float A[144]) {
 }
 
 // b = -J^T f, diagonal block D_i = sum J_i^T J_i and its inverse; |b|^2 partials.
+// b = -J^T f, diagonal block D_i = sum J_i^T J_i and its inverse; |b|^2 partials.
+// D_i = (the rotation group's 9x9 block, lane 0 only) + the I_3-structured sum
+// of the 3-row groups, C[p][q] = sum a'_p a'_q with a' = (a_0, a_1, a_2, b),
+// placed at (3p + r, 3q + r) (index 3 -> 9 + r): 10 sums per lane to reduce.
 __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float* __restrict__ x,
                                                        float* __restrict__ b, float* __restrict__ binv,
                                                        double* __restrict__ partial, const int* __restrict__ skip) {
@@ -416,11 +427,14 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     const int fi = blockIdx.x * (256 / LANES) + grp;
     double bl[12];
-    float D[78];
+    float C[10];
+    float D9[45];
 #pragma unroll
     for (int a = 0; a < 12; a++) bl[a] = 0.0;
 #pragma unroll
-    for (int a = 0; a < 78; a++) D[a] = 0.0f;
+    for (int a = 0; a < 10; a++) C[a] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 45; a++) D9[a] = 0.0f;
     if (fi < S.nF) {
         const int ni = owner_items(S, fi);
         for (int it = lane; it < ni; it += LANES) {
@@ -430,7 +444,6 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
             int64_t gid;
             // map item -> global group id for the residual
             {
-                const int i = S.free_nodes[fi];
                 int item = it;
                 if (item == 0) gid = fi;
                 else if ((item -= 1) < S.K) gid = S.nF + (int64_t)fi * S.K + item;
@@ -448,36 +461,62 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float*
                         gid = S.nF + (int64_t)(S.nF + S.nS) * S.K + S.cm_con[S.cm_ptr[fi] + item];
                     }
                 }
-                (void)i;
             }
             group_residual(S, x, nullptr, 0.0f, gid, f);
-            owner_item(S, fi, it, x, x, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
-                (void)Jv;
+            owner_item<0, false>(
+                S, fi, it, x, x,
+                [&](float (&Ji)[6][9], float (&)[6]) {
 #pragma unroll
-                for (int r = 0; r < 6; r++) {
-                    if (r >= rows) break;
+                    for (int r = 0; r < 6; r++) {
 #pragma unroll
-                    for (int a = 0; a < 12; a++) bl[a] -= (double)Ji[r][a] * f[r];
+                        for (int a = 0; a < 9; a++) bl[a] -= (double)Ji[r][a] * f[r];
+                        int o = 0;
+#pragma unroll
+                        for (int a = 0; a < 9; a++)
+#pragma unroll
+                            for (int c = a; c < 9; c++) D9[o++] += Ji[r][a] * Ji[r][c];
+                    }
+                },
+                [&](float (&a3)[3], float bb, float (&)[3]) {
+                    const float v4[4] = {a3[0], a3[1], a3[2], bb};
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) bl[3 * c + r] -= (double)v4[c] * f[r];
+                        bl[9 + r] -= (double)bb * f[r];
+                    }
                     int o = 0;
 #pragma unroll
-                    for (int a = 0; a < 12; a++)
+                    for (int p = 0; p < 4; p++)
 #pragma unroll
-                        for (int c = a; c < 12; c++) D[o++] += Ji[r][a] * Ji[r][c];
-                }
-            });
+                        for (int q = p; q < 4; q++) C[o++] += v4[p] * v4[q];
+                });
         }
     }
 #pragma unroll
     for (int a = 0; a < 12; a++) bl[a] = group_reduce16d(bl[a]);
 #pragma unroll
-    for (int a = 0; a < 78; a++) D[a] = group_reduce16(D[a]);
+    for (int a = 0; a < 10; a++) C[a] = group_reduce16(C[a]);
     double nb = 0.0;
     if (fi < S.nF && lane == 0) {
+        float* Dm = sD[grp];
+        for (int e = 0; e < 144; e++) Dm[e] = 0.0f;
         int o = 0;
-        for (int a = 0; a < 12; a++)
-            for (int c = a; c < 12; c++) {
-                sD[grp][a * 12 + c] = D[o];
-                sD[grp][c * 12 + a] = D[o];
+        for (int a = 0; a < 9; a++)
+            for (int c = a; c < 9; c++) {
+                Dm[a * 12 + c] = D9[o];
+                Dm[c * 12 + a] = D9[o];
+                o++;
+            }
+        o = 0;
+        for (int p = 0; p < 4; p++)
+            for (int q = p; q < 4; q++) {
+                for (int r = 0; r < 3; r++) {
+                    const int u = p < 3 ? 3 * p + r : 9 + r;
+                    const int w = q < 3 ? 3 * q + r : 9 + r;
+                    Dm[u * 12 + w] += C[o];
+                    if (u != w) Dm[w * 12 + u] += C[o];
+                }
                 o++;
             }
         for (int a = 0; a < 12; a++) {
@@ -542,11 +581,22 @@ __device__ __forceinline__ void apply_body(const SolverDev& S, const float* __re
     if (fi < S.nF) {
         const int ni = owner_items(S, fi);
         for (int it = lane; it < ni; it += LANES) {
-            owner_item<KT>(S, fi, it, x, z, [&](int rows, float (&Ji)[6][12], float (&Jv)[6]) {
-                for (int r = 0; r < rows; r++)
+            owner_item<KT>(
+                S, fi, it, x, z,
+                [&](float (&Ji)[6][9], float (&Jv)[6]) {
 #pragma unroll
-                    for (int a = 0; a < 12; a++) acc[a] += Ji[r][a] * Jv[r];
-            });
+                    for (int r = 0; r < 6; r++)
+#pragma unroll
+                        for (int a = 0; a < 9; a++) acc[a] += Ji[r][a] * Jv[r];
+                },
+                [&](float (&a3)[3], float bb, float (&Jv)[3]) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) acc[3 * c + r] += a3[c] * Jv[r];
+                        acc[9 + r] += bb * Jv[r];
+                    }
+                });
         }
     }
     float mine = 0.0f;
@@ -1261,8 +1311,9 @@ __global__ void __launch_bounds__(256) accept_partial_kernel(int n, float* __res
 
 __global__ void __launch_bounds__(256) accept_final_kernel(int nblocks, const double* __restrict__ partial,
                                                            const double* __restrict__ cand_e, int ncand,
-                                                           GnScalars* __restrict__ gs, double* __restrict__ energies,
-                                                           int iter, int stop_rule, int* __restrict__ gn_done) {
+                                                           GnScalars* __restrict__ gs, const CgScalars* __restrict__ cs,
+                                                           double* __restrict__ energies, int iter, int stop_rule,
+                                                           int* __restrict__ gn_done) {
     if (*gn_done) return;
     __shared__ double sh[8];
     double nx = 0.0, nh = 0.0;
@@ -1280,6 +1331,7 @@ __global__ void __launch_bounds__(256) accept_final_kernel(int nblocks, const do
         const double normh = sqrt(th) * scale;
         if (chosen >= 0) gs->e_old = cand_e[chosen];
         gs->iters_done = iter + 1;
+        gs->cg_total += cs->iters;
         energies[iter + 1] = gs->e_old;
         if (stop_rule && normh < (normv + 1e-6) * 1e-6) *gn_done = 1;
     }
@@ -1294,6 +1346,7 @@ __global__ void init_x_kernel(int n, float* __restrict__ x, GnScalars* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *gn_done = 0;
         gs->iters_done = 0;
+        gs->cg_total = 0;
         (void)e0;
     }
 }
@@ -1444,7 +1497,25 @@ int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipS
     hipLaunchKernelGGL(accept_partial_kernel, dim3(W.acc_blocks), dim3(256), 0, st, W.n, W.x, W.hsol, W.cand_e,
                        W.n_cand, W.gs, W.acc_partial, W.gn_done);
     hipLaunchKernelGGL(accept_final_kernel, dim3(1), dim3(256), 0, st, W.acc_blocks, W.acc_partial, W.cand_e,
-                       W.n_cand, W.gs, energies, iter, stop_rule, W.gn_done);
+                       W.n_cand, W.gs, (const CgScalars*)W.cg, energies, iter, stop_rule, W.gn_done);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_apply_test(SolverWork& W, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st) {
+    const SolverDev& S = W.dev;
+    const int nb_owner = nblk(S.nF, 256 / LANES);
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.x, x, (int64_t)W.n, (const int*)nullptr);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial, W.cg_done);
+    auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
+    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, v, (const double*)W.rzp, (const double*)W.rzp,
+                       1, W.p, W.q, W.partial, W.cg_done);
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, q_out, (const float*)W.q, (int64_t)W.n,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, b_out, (const float*)W.r, (int64_t)W.n,
+                       (const int*)nullptr);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -58,7 +58,7 @@ struct GnScalars {
     double e_old;
     double bb0;       // |b|^2 of the solve's first Gauss-Newton iteration
     int iters_done;
-    int pad;
+    int cg_total;     // CG iterations of the solve so far
 };
 
 // Counter, generation and error flag of the persistent PCG kernel's grid barrier.
@@ -123,5 +123,7 @@ int solver_cg_persistent(SolverWork& W, hipStream_t st);
 int solver_barrier_error(SolverWork& W, int* err);
 int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st);
 int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st);
+// Test hook: q = J^T J v and b = -J^T f at x (the CG operator and right-hand side).
+int solver_apply_test(SolverWork& W, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st);
 
 }  // namespace arapgs

@@ -142,6 +142,10 @@ int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
  * call clears the stored steps (synchronous). */
 int arapgs_solver_set_warm_start(arapgs_solver* s, int on);
 
+/* GN iterations and total CG iterations of the solver's last solve
+ * (synchronises the device). Diagnostics; no counterpart in the reference. */
+int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h);
+
 /* How each GN iteration's PCG solve (the dense Eigen solve of Deform.cpp:118-128
  * in the reference) is launched: 0 (default) = three kernels per CG iteration
  * (combine with set_host_checks to bound the launches queued past convergence);
@@ -480,6 +484,11 @@ int arapgs_ply_save(arapgs_ctx* ctx, const char* path, int64_t n, const float* p
                     const float* boxmax_h, const uint8_t* too_small, int64_t* written_h);
 
 /* ------------------------------------------------------------ test hooks */
+/* The solver's CG operator and right-hand side at unknowns x (f32[n]):
+ * q = J^T J v and b = -J^T f (device f32[n] each), J and f those of
+ * Deform::FastCalcJacobiMat / CalcEnergyFunc (Deform.cpp:180-581). */
+int arapgs_solver_apply(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
+                        const float* v, float* q, float* b);
 /* SH rotation f'(d) = f(R^T d) of n 48-float sets (helper.cpp:938-1075 with the
  * GaussianView.cpp:3134-3156 sign wrapper); R9 f32[n*9] row-major. */
 int arapgs_sh_rotate(arapgs_ctx* ctx, int64_t n, const float* r9, float* sh);
