@@ -215,7 +215,8 @@ def timed(B, plain, steps, warmup):
     total = time.perf_counter() - t0
     kernel_ms = ctx.profile_read()
     ctx.profile_enable(False)
-    log(f"  GN iterations in the last frame: {int(B['iters_done'].item())}")
+    gn, cg = B["solver"].last_stats()
+    log(f"  last frame: {gn} GN iterations, {cg} CG iterations")
     return dict(total_s=total, frame_ms=times, kernel_ms=kernel_ms, instances=n_inst)
 
 
