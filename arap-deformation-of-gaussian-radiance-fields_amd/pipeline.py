@@ -11,12 +11,14 @@ the frame splits into two halves on two streams:
 
   solve side   (stream A)  aims(t), solve(t), node table(t) -> slot t%2,
                            node blend(t) (node positions for t+1); event A[t%2]
-  deform side  (stream B)  wait A[t%2]; sample blend, six-point update, node
-                           quaternions, sample SH with table slot t%2; event
-                           B[t%2] (slot free again); the V views in one
-                           render pass
+  deform side  (stream B)  wait A[t%2]; six-point update with table slot t%2;
+                           event B[t%2]; the V views in one render pass
+               (stream C)  wait A[t%2]; sample blend, node quaternions, sample
+                           SH with table slot t%2; event C[t%2]
+                           (the slot is free again after B[t%2] and C[t%2])
 
-and frame t's deform side runs while frame t+1's solve runs. Each side has its
+and frame t's deform side runs while frame t+1's solve runs; the samples (they
+read only the node table) run beside the six-point update and the renders. Each side has its
 own host thread, because each side synchronises with the host (the solve's CG
 convergence checks, the renderer's instance count) and those waits must not
 stall the other side. The two node-table slots bound the lead of the solve side
@@ -33,9 +35,12 @@ from . import arapgs
 
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
-                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None):
+                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True):
         """The solve side runs on solver.ctx's stream, the deform side on
-        stream_b (a new stream when None), e.g. two arapgs.CuStream partitions."""
+        stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
+        split_samples: the samples' blend and SH rotation run on a third stream
+        beside the six-point update and the renders (they read only the node
+        table), so they fill the CUs the renders' deep tiles leave idle."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -46,6 +51,10 @@ class DragPipeline:
         self.ctx_b = render_ctx if render_ctx is not None else arapgs.Context(dev.index or 0, stream=self.stream_b)
         if render_ctx is not None:
             self.stream_b = render_ctx.stream
+        self.split = split_samples and state.get("samples") is not None and state["samples"].shape[0] > 0
+        self.stream_c = torch.cuda.Stream(device=dev) if self.split else None
+        self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
+        self.ev_c = [torch.cuda.Event() for _ in range(2)]
         self.st = state
         self.k = k
         self.active, self.delta = active, delta
@@ -70,7 +79,10 @@ class DragPipeline:
         st, torch = self.st, self.torch
         with torch.cuda.stream(self.stream_a):
             if self.ev_b_recorded[slot]:
+                # the slot is free once both deform streams are done with it
                 self.stream_a.wait_event(self.ev_b[slot])
+                if self.split:
+                    self.stream_a.wait_event(self.ev_c[slot])
             self.ctx_a.aims_translate(st["node_pos"], self.active, self.delta, self.aims)
             self.solver.solve(st["node_pos"], self.aims, self.iters, self.stop_rule, x_out=self.x[slot],
                               energies=self.energies[slot], iters_done=self.iters_done[slot])
@@ -79,20 +91,30 @@ class DragPipeline:
                                     out=st["node_pos"])
             self.ev_a[slot].record(self.stream_a)
 
+    def _samples(self, ctx, table):
+        st = self.st
+        if st.get("samples") is not None and st["samples"].shape[0] > 0:
+            ctx.blend_points(st["samples"], st["sample_idx"], st["sample_w"], table, skip=st.get("sample_static"),
+                             out=st["samples"])
+            if st.get("sample_feat") is not None:
+                q = ctx.node_quats(table)
+                ctx.sample_sh(st["sample_w"], st["sample_idx"], q, st.get("sample_static"), st["sample_feat"])
+
     def _deform_half(self, slot, render=True):
         st, torch = self.st, self.torch
         ctx = self.ctx_b
+        table = self.tables[slot]
+        if self.split:
+            with torch.cuda.stream(self.stream_c):
+                self.stream_c.wait_event(self.ev_a[slot])
+                self._samples(self.ctx_c, table)
+                self.ev_c[slot].record(self.stream_c)
         with torch.cuda.stream(self.stream_b):
             self.stream_b.wait_event(self.ev_a[slot])
-            table = self.tables[slot]
-            if st.get("samples") is not None and st["samples"].shape[0] > 0:
-                ctx.blend_points(st["samples"], st["sample_idx"], st["sample_w"], table, skip=st.get("sample_static"),
-                                 out=st["samples"])
+            if not self.split:
+                self._samples(ctx, table)
             ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"], st.get("gs_static"),
                                  st["pos"], st["rot"], st["scale"], st["sh"])
-            if st.get("sample_feat") is not None and st["samples"].shape[0] > 0:
-                q = ctx.node_quats(table)
-                ctx.sample_sh(st["sample_w"], st["sample_idx"], q, st.get("sample_static"), st["sample_feat"])
             self.ev_b[slot].record(self.stream_b)
             self.ev_b_recorded[slot] = True
             n_inst = 0
@@ -150,5 +172,8 @@ class DragPipeline:
         self.torch.cuda.synchronize(self.st["pos"].device)
 
     def close(self):
+        if self.ctx_c is not None:
+            self.ctx_c.close()
+            self.ctx_c = None
         if self.ctx_b is not None:
             self.ctx_b.close()
