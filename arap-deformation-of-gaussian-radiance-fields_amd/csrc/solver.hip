@@ -1171,47 +1171,72 @@ __global__ void __launch_bounds__(128) coarse_assemble_kernel(SolverDev S, const
 // column panel -A_iK A_KK^-1.
 constexpr int GJ_T = 64;
 
-// Inverse of the 64x64 pivot tile by in-place Gauss-Jordan, one wave: lane j
-// keeps column j in registers; step k reads the pivot column from lane k with
-// v_readlane (the loops are unrolled, so every register index is static).
-// Per entry the same operations in the same order as the textbook in-place
-// form: row k scaled by 1/d, P_ij -= P_ik (P_kj / d), column k -> -P_ik / d.
+// Inverse of the 64x64 pivot tile by in-place Gauss-Jordan, four waves: lane j
+// of wave w keeps rows 16w..16w+15 of column j in registers. Step k: the wave
+// holding row k publishes it in LDS (one barrier), every wave reads its rows of
+// the pivot column from lane k with v_readlane (the loops are unrolled, so
+// every register index is static). Per entry the same operations in the same
+// order as the textbook in-place form: row k scaled by 1/d,
+// P_ij -= P_ik (P_kj / d), column k -> -P_ik / d.
+constexpr int GJ_R = GJ_T / 4;  // rows per wave
+
 template <int K>
-__device__ __forceinline__ void gj_step(float (&c)[GJ_T], int lane) {
-    float col[GJ_T];
-#pragma unroll
-    for (int i = 0; i < GJ_T; i++) col[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[i]), K));
-    const float d = col[K];
+__device__ __forceinline__ void gj_step4(float (&c)[GJ_R], int lane, int wave, float* prow) {
+    if (wave == K / GJ_R) prow[lane] = c[K % GJ_R];
+    __syncthreads();
+    const float d = prow[K];
     const float iv = (fabsf(d) > 1e-30f) ? 1.0f / d : 0.0f;
-    if (lane == K) {
+    const float rk = prow[lane] * iv;
 #pragma unroll
-        for (int i = 0; i < GJ_T; i++) c[i] = (i == K) ? iv : col[i] * -iv;
-    } else {
-        const float rk = c[K] * iv;
-#pragma unroll
-        for (int i = 0; i < GJ_T; i++)
-            if (i != K) c[i] -= col[i] * rk;
-        c[K] = rk;
+    for (int r = 0; r < GJ_R; r++) {
+        const float colv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[r]), K));
+        const bool row_k = wave * GJ_R + r == K;
+        if (lane == K) c[r] = row_k ? iv : colv * -iv;
+        else c[r] = row_k ? rk : c[r] - colv * rk;
     }
+    __syncthreads();
 }
 
 template <int K>
-__device__ __forceinline__ void gj_steps(float (&c)[GJ_T], int lane) {
+__device__ __forceinline__ void gj_steps4(float (&c)[GJ_R], int lane, int wave, float* prow) {
     if constexpr (K < GJ_T) {
-        gj_step<K>(c, lane);
-        gj_steps<K + 1>(c, lane);
+        gj_step4<K>(c, lane, wave, prow);
+        gj_steps4<K + 1>(c, lane, wave, prow);
     }
 }
 
-__global__ void __launch_bounds__(64) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
+__global__ void __launch_bounds__(256) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
+    __shared__ float prow[GJ_T];
     const int k0 = kb * GJ_T;
-    const int lane = threadIdx.x;
-    float c[GJ_T];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float c[GJ_R];
 #pragma unroll
-    for (int i = 0; i < GJ_T; i++) c[i] = A[(int64_t)(k0 + i) * n + k0 + lane];
-    gj_steps<0>(c, lane);
+    for (int r = 0; r < GJ_R; r++) c[r] = A[(int64_t)(k0 + wave * GJ_R + r) * n + k0 + lane];
+    gj_steps4<0>(c, lane, wave, prow);
 #pragma unroll
-    for (int i = 0; i < GJ_T; i++) piv[i * GJ_T + lane] = c[i];
+    for (int r = 0; r < GJ_R; r++) piv[(wave * GJ_R + r) * GJ_T + lane] = c[r];
+}
+
+// 64x64 = (64x64) (64x64) products from LDS tiles: thread t accumulates the 4x4
+// block at rows 4 (t / 16), columns 4 (t % 16), with fused multiply-adds.
+__device__ __forceinline__ void tile_mm(const float (*X)[GJ_T + 1], const float (*Y)[GJ_T + 1], float (&acc)[4][4]) {
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0f;
+#pragma unroll 4
+    for (int t = 0; t < GJ_T; t++) {
+        float a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = X[4 * tr + i][t];
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = Y[t][4 * tc + j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_fmaf(a[i], b[j], acc[i][j]);
+    }
 }
 
 // blockIdx.x = tile column jb != kb: row panel A_Kj <- piv * A_Kj; also copies
@@ -1230,12 +1255,13 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(float* __restrict__ A, in
         colbuf[(int64_t)(j0 + r) * GJ_T + c] = A[(int64_t)(j0 + r) * n + k0 + c];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
-        const int r = e / GJ_T, c = e % GJ_T;
-        float acc = 0.0f;
-        for (int t = 0; t < GJ_T; t++) acc += Pv[r][t] * R[t][c];
-        A[(int64_t)(k0 + r) * n + j0 + c] = acc;
-    }
+    float acc[4][4];
+    tile_mm(Pv, R, acc);
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) A[(int64_t)(k0 + 4 * tr + i) * n + j0 + 4 * tc + j] = acc[i][j];
 }
 
 // blockIdx = (ib, jb): A_ij -= C_iK * A_Kj (new row panel) for ib != kb;
@@ -1258,14 +1284,17 @@ __global__ void __launch_bounds__(256) gj_update_kernel(float* __restrict__ A, i
         Bs[r][c] = (jb == kb) ? piv[e] : A[(int64_t)(k0 + r) * n + j0 + c];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
-        const int r = e / GJ_T, c = e % GJ_T;
-        float acc = 0.0f;
-        for (int t = 0; t < GJ_T; t++) acc += Cs[r][t] * Bs[t][c];
-        const int64_t o = (int64_t)(i0 + r) * n + j0 + c;
-        if (jb == kb) A[o] = -acc;
-        else A[o] -= acc;
-    }
+    float acc[4][4];
+    tile_mm(Cs, Bs, acc);
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t o = (int64_t)(i0 + 4 * tr + i) * n + j0 + 4 * tc + j;
+            if (jb == kb) A[o] = -acc[i][j];
+            else A[o] -= acc[i][j];
+        }
 }
 
 __global__ void pad_identity_kernel(float* __restrict__ A, int n, int nc) {
@@ -1386,7 +1415,7 @@ int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
         hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
         const int nt = np / GJ_T;
         for (int kb = 0; kb < nt; kb++) {
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, kb, W.gj_piv);
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
             hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
             hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
         }

@@ -258,3 +258,20 @@ def test_cg_operator_and_rhs_match_oracle_jacobian(ctx, prob, center):
           f"|b - b_ref| / |b_ref| = {np.linalg.norm(H(b) - b_ref) / np.linalg.norm(b_ref):.3e}", flush=True)
     assert np.linalg.norm(H(q) - q_ref) <= 1e-5 * np.linalg.norm(q_ref)
     assert np.linalg.norm(H(b) - b_ref) <= 1e-5 * np.linalg.norm(b_ref)
+
+
+def test_coarse_inverse_is_exact_when_every_node_is_an_aggregate(ctx, prob):
+    """With one aggregate per free node the coarse space is the whole space and
+    the two-level preconditioner contains (J^T J)^-1 (the blocked Gauss-Jordan
+    inverse), so the first GN step's CG needs only a few iterations; a wrong
+    inverse shows up as a CG that runs to its cap."""
+    aims, ref = _problem(prob, 1.0)
+    s = arapgs.Solver(ctx, prob["nbr"], prob["blocks"], prob["types"], prob["src_idx"],
+                      prob["src_w"].astype(np.float32), prob["nodes"], cg_max_iters=200, cg_tol=1e-7,
+                      n_aggregates=prob["nodes"].shape[0])
+    x, e, done = s.solve(T(prob["nodes"]), T(aims), 1, stop_rule=True)
+    gn, cg = s.last_stats()
+    x_ref, e_ref, _ = ref.solve(1, stop_rule=True)
+    print(f"one GN step: {cg} CG iterations, E {float(H(e)[1]):.6e} vs oracle {e_ref[1]:.6e}", flush=True)
+    assert gn == 1 and cg <= 20
+    assert abs(float(H(e)[1]) - e_ref[1]) <= 1e-5 * e_ref[1]
