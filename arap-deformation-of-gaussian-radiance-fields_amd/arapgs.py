@@ -80,6 +80,11 @@ EXPORTS = {
     "arapgs_gn_solve": ([P, P, P, P, C.c_int, C.c_int, P, P, P], C.c_int),
     "arapgs_gn_energy": ([P, P, P, P, P, P], C.c_int),
     "arapgs_put_free_inputs": ([P, P, P, P, P], C.c_int),
+    "arapgs_solver_batch_create": ([P, P, C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_solver_batch_destroy": ([P], None),
+    "arapgs_solver_batch_set_host_checks": ([P, C.c_int], C.c_int),
+    "arapgs_solver_batch_set_coarse_refresh": ([P, C.c_int], C.c_int),
+    "arapgs_solver_batch_solve": ([P, P, P, P, C.c_int, C.c_int, P, P, P, P], C.c_int),
     "arapgs_blend_points": ([P, P, C.c_int64, P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_six_point_update": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float, P, P, P, P],
                                 C.c_int),
@@ -113,6 +118,7 @@ EXPORTS = {
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
     "arapgs_render_views": ([P, P, C.c_int, P], C.c_int),
+    "arapgs_render_segments": ([P, P, C.c_int, P, P], C.c_int),
     "arapgs_set_blend_unroll": ([P, C.c_int], C.c_int),
     "arapgs_density_backward": ([P, P, P, P, P, P, P, P, C.c_float, P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_realign_create": ([P, P, C.c_int64, C.POINTER(RealignDesc), C.POINTER(P)], C.c_int),
@@ -329,6 +335,26 @@ class Context:
                                 cam["tx"], cam["ty"], 0, ptr(out), None, None, rects, None, None, 0, 1.0, tile)
         n = C.c_int(0)
         self._check(lib().arapgs_render_views(self.h, arr, V, C.byref(n)))
+        return n.value
+
+    def render_segments(self, means, shs, opac, scales, rots, cams, W, H, bg, outs, seg, D=3, scale_mod=1.0,
+                        tile=16):
+        """Scenes stored back to back, one camera each (arapgs_render_segments):
+        cams[v] draws Gaussians seg[v]..seg[v+1]-1 into outs[v]. Returns the total
+        instance count."""
+        import numpy as np
+        P_ = means.shape[0]
+        V = len(cams)
+        seg = np.ascontiguousarray(seg, np.int64)
+        if seg.shape != (V + 1,):
+            raise ArapgsError("render_segments: need len(cams) + 1 segment offsets")
+        arr = (RenderDesc * V)()
+        for v, (cam, out) in enumerate(zip(cams, outs)):
+            arr[v] = RenderDesc(P_, D, 16, ptr(bg), W, H, ptr(means), ptr(shs), None, ptr(opac), ptr(scales),
+                                scale_mod, ptr(rots), None, ptr(cam["view"]), ptr(cam["proj"]), ptr(cam["campos"]),
+                                cam["tx"], cam["ty"], 0, ptr(out), None, None, None, None, None, 0, 1.0, tile)
+        n = C.c_int(0)
+        self._check(lib().arapgs_render_segments(self.h, arr, V, seg.ctypes.data_as(P), C.byref(n)))
         return n.value
 
     # ------------------------------------------------------------ PLY codec
@@ -591,6 +617,63 @@ class Solver:
         table = node_table(self.M, node_pos.device) if table is None else table
         self.ctx._check(lib().arapgs_put_free_inputs(self.ctx.h, self.h, ptr(node_pos), ptr(x), ptr(table)))
         return table
+
+
+def _ptr_array(tensors):
+    """Host array of device pointers (None -> null entry)."""
+    arr = (P * len(tensors))()
+    for i, t in enumerate(tensors):
+        arr[i] = None if t is None else ptr(t)
+    return arr
+
+
+class SolverBatch:
+    """arapgs_solver_batch: independent scenes whose Gauss-Newton solves run as
+    one launch per solver phase (scene = blockIdx.z). Per scene, the result is
+    bit-identical to Solver.solve + Solver.put_free_inputs."""
+
+    def __init__(self, ctx: Context, solvers):
+        self.ctx = ctx
+        self.solvers = list(solvers)
+        arr = (P * len(self.solvers))(*[s.h for s in self.solvers])
+        h = P()
+        ctx._check(lib().arapgs_solver_batch_create(ctx.h, arr, len(self.solvers), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().arapgs_solver_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_host_checks(self, every):
+        self.ctx._check(lib().arapgs_solver_batch_set_host_checks(self.h, every))
+
+    def set_coarse_refresh(self, frames):
+        self.ctx._check(lib().arapgs_solver_batch_set_coarse_refresh(self.h, frames))
+
+    def solve(self, node_pos, aims, iters, stop_rule=True, x_out=None, energies=None, iters_done=None, tables=None):
+        """Lists (one entry per scene) of device tensors; outputs default to new
+        tensors (tables: None = no node tables)."""
+        t = self.ctx._torch
+        n = len(self.solvers)
+        dev = node_pos[0].device
+        if x_out is None:
+            x_out = [t.empty(s.n_unknowns, dtype=t.float32, device=dev) for s in self.solvers]
+        if energies is None:
+            energies = [t.zeros(iters + 1, dtype=t.float64, device=dev) for _ in range(n)]
+        if iters_done is None:
+            iters_done = [t.zeros(1, dtype=t.int32, device=dev) for _ in range(n)]
+        tb = _ptr_array(tables) if tables is not None else None
+        self.ctx._check(lib().arapgs_solver_batch_solve(self.ctx.h, self.h, _ptr_array(node_pos), _ptr_array(aims),
+                                                        iters, int(stop_rule), _ptr_array(x_out),
+                                                        _ptr_array(energies), _ptr_array(iters_done), tb))
+        return x_out, energies, iters_done
 
 
 class Grid:

@@ -61,6 +61,8 @@ struct arapgs_ctx {
     int last_dbits = 32;
     RasterView* r_views = nullptr;    // device copy of the views of the current render
     RasterView* r_views_h = nullptr;  // pinned host staging for it
+    int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
+    int* r_seg_h = nullptr;           // pinned host staging for it
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
@@ -76,6 +78,7 @@ struct arapgs_solver {
     int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG
     int coarse_refresh = 8;     // frames between coarse-operator rebuilds
     int64_t solves = 0;
+    SolverBatch one;            // this solver as a launch group of one
     // per-frame scratch (M nodes)
     NodeXf* table = nullptr;
     float* node_q = nullptr;
@@ -151,6 +154,8 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipMalloc(&c->shrot_dev, sizeof(ShRotTables))) != hipSuccess ||
         (e = hipMalloc(&c->r_views, sizeof(RasterView) * RASTER_MAX_VIEWS)) != hipSuccess ||
         (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc(&c->r_seg, sizeof(int) * (RASTER_MAX_VIEWS + 1))) != hipSuccess ||
+        (e = hipHostMalloc(&c->r_seg_h, sizeof(int) * (RASTER_MAX_VIEWS + 1), hipHostMallocDefault)) != hipSuccess ||
         (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
         arapgs_destroy(c);
         return -5;
@@ -170,9 +175,10 @@ void arapgs_destroy(arapgs_ctx* c) {
     }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
                     (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec, c->db_ws,
-                    (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views})
+                    (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views, (void*)c->r_seg})
         if (p) (void)hipFree(p);
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
+    if (c->r_seg_h) (void)hipHostFree(c->r_seg_h);
     delete c;
 }
 
@@ -319,6 +325,12 @@ int arapgs_solver_create(arapgs_ctx* c, const arapgs_solver_desc* d, arapgs_solv
     }
     if (d->cg_max_iters > 0) s->w.cg_max_iters = d->cg_max_iters;
     if (d->cg_tol > 0.0f) s->w.cg_tol = d->cg_tol;
+    rc = solver_batch_init(s->one, {&s->w}, err);
+    if (rc) {
+        solver_free(s->w);
+        delete s;
+        return fail(c, rc, "%s", err.c_str());
+    }
     hipError_t e = hipMalloc(&s->table, sizeof(NodeXf) * (size_t)d->m);
     if (e == hipSuccess) e = hipMalloc(&s->node_q, sizeof(float) * 4 * (size_t)d->m);
     if (e == hipSuccess) e = hipMalloc(&s->node_tmp, sizeof(float) * 3 * (size_t)d->m);
@@ -334,6 +346,7 @@ void arapgs_solver_destroy(arapgs_solver* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
+    solver_batch_free(s->one);
     solver_free(s->w);
     for (void* p : {(void*)s->table, (void*)s->node_q, (void*)s->node_tmp})
         if (p) (void)hipFree(p);
@@ -356,9 +369,9 @@ int arapgs_solver_apply(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, 
                         const float* v, float* q, float* b) {
     CHECK_CTX(c);
     if (!s || !node_pos || !aims || !x || !v || !q || !b) return fail(c, -1, "solver_apply: bad arguments");
-    s->w.dev.g = node_pos;
-    s->w.dev.aim = aims;
-    return check_launch(c, solver_apply_test(s->w, x, v, q, b, c->stream), "solver_apply");
+    int rc = check_launch(c, solver_batch_set_frame(s->one, &node_pos, &aims, c->stream), "solver_apply");
+    if (rc) return rc;
+    return check_launch(c, solver_apply_test(s->one, x, v, q, b, c->stream), "solver_apply");
 }
 
 int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h) {
@@ -400,16 +413,58 @@ int arapgs_solver_dims(const arapgs_solver* s, int* n_free_h, int* n_unknowns_h)
 }  // extern "C"
 
 namespace {
-__global__ void copy_f32_kernel(float* __restrict__ dst, const float* __restrict__ src, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
-}
-// iters_done for the caller; -1 when the persistent PCG kernel's grid barrier
-// gave up waiting (the solve is then invalid).
-__global__ void copy_done_kernel(const arapgs::GnScalars* __restrict__ gs, const arapgs::GridBarrier* __restrict__ gb,
-                                 int* __restrict__ out) {
-    *out = gb->err ? -1 : gs->iters_done;
+// The Gauss-Newton loop of Deform::optimize (Deform.cpp:95-169) over a launch
+// group of solvers: every phase is one launch for all scenes; the host reads
+// the groups' convergence flags every `every` CG iterations and after every
+// GN iteration (every == 0: device-side flags only, all launches queued).
+int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, int iters, int stop_rule) {
+    int rc = check_launch(c, solver_frame_setup(B, refresh, c->stream), "gn_solve setup");
+    if (rc) return rc;
+    auto host_flag = [&](int which, int* out) -> int {
+        const int* dflag = nullptr;
+        int r = check_launch(c, solver_batch_flag(B, which, c->stream, &dflag), "gn_solve");
+        if (r) return r;
+        hipError_t e = hipMemcpyAsync(out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
+    };
+    for (int it = 0; it < iters; it++) {
+        rc = check_launch(c, solver_gn_begin(B, it, c->stream), "gn_solve rhs");
+        if (rc) return rc;
+        if (cg_mode == 1) {
+            rc = check_launch(c, solver_cg_persistent(*B.w[0], c->stream), "gn_solve cg");
+            if (rc) return rc;
+        } else {
+            const int cap = B.w[0]->cg_max_iters;
+            for (int k = 0; k < cap; k++) {
+                rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
+                if (rc) return rc;
+                if (every > 0 && (k + 1) % every == 0) {
+                    int done = 0;
+                    if ((rc = host_flag(0, &done))) return rc;
+                    if (done) break;
+                }
+            }
+        }
+        rc = check_launch(c, solver_gn_end(B, it, stop_rule, c->stream), "gn_solve line search");
+        if (rc) return rc;
+        if (every > 0) {
+            int done = 0;
+            if ((rc = host_flag(1, &done))) return rc;
+            if (done) break;
+        }
+    }
+    return 0;
 }
 }  // namespace
+
+struct arapgs_solver_batch {
+    int device = 0;
+    SolverBatch b;
+    int host_check_every = 8;
+    int coarse_refresh = 8;
+    int64_t solves = 0;
+};
 
 extern "C" {
 
@@ -417,64 +472,100 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
                     int stop_rule, float* x_out, double* energies, int* iters_done) {
     CHECK_CTX(c);
     if (!s || !node_pos || !aims || !energies || iters < 0) return fail(c, -1, "gn_solve: bad arguments");
-    SolverWork& W = s->w;
-    W.dev.g = node_pos;
-    W.dev.aim = aims;
+    if (iters > GN_MAX_ITERS) return fail(c, -1, "gn_solve: at most %d Gauss-Newton iterations", GN_MAX_ITERS);
+    int rc = check_launch(c, solver_batch_set_frame(s->one, &node_pos, &aims, c->stream), "gn_solve");
+    if (rc) return rc;
     const bool refresh = (s->solves % std::max(1, s->coarse_refresh)) == 0;
     s->solves++;
-    int rc = check_launch(c, solver_frame_setup(W, refresh, c->stream), "gn_solve setup");
+    if ((rc = run_gn(c, s->one, s->host_check_every, s->cg_mode, refresh, iters, stop_rule))) return rc;
+    SolverOutPtrs out = {};
+    out.x[0] = x_out;
+    out.energies[0] = energies;
+    out.iters_done[0] = iters_done;
+    return check_launch(c, solver_batch_finish(s->one, out, c->stream), "gn_solve");
+}
+
+int arapgs_solver_batch_create(arapgs_ctx* c, arapgs_solver* const* solvers, int n, arapgs_solver_batch** out) {
+    CHECK_CTX(c);
+    if (!solvers || !out || n < 1 || n > SOLVER_BATCH_MAX)
+        return fail(c, -1, "solver_batch_create: need 1..%d solvers", SOLVER_BATCH_MAX);
+    *out = nullptr;
+    std::vector<SolverWork*> w;
+    for (int i = 0; i < n; i++) {
+        if (!solvers[i]) return fail(c, -1, "solver_batch_create: solver %d is null", i);
+        if (solvers[i]->device != c->device) return fail(c, -1, "solver_batch_create: solver %d is on another device", i);
+        for (int j = 0; j < i; j++)
+            if (solvers[j] == solvers[i]) return fail(c, -1, "solver_batch_create: solver %d appears twice", i);
+        w.push_back(&solvers[i]->w);
+    }
+    (void)hipSetDevice(c->device);
+    arapgs_solver_batch* b = new arapgs_solver_batch();
+    b->device = c->device;
+    std::string err;
+    const int rc = solver_batch_init(b->b, w, err);
+    if (rc) {
+        delete b;
+        return fail(c, rc, "%s", err.c_str());
+    }
+    *out = b;
+    return 0;
+}
+
+void arapgs_solver_batch_destroy(arapgs_solver_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    solver_batch_free(b->b);
+    delete b;
+}
+
+int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every) {
+    if (!b || every < 0) return -1;
+    b->host_check_every = every;
+    return 0;
+}
+
+int arapgs_solver_batch_set_coarse_refresh(arapgs_solver_batch* b, int frames) {
+    if (!b || frames < 1) return -1;
+    b->coarse_refresh = frames;
+    b->solves = 0;
+    return 0;
+}
+
+int arapgs_solver_batch_solve(arapgs_ctx* c, arapgs_solver_batch* b, const float* const* node_pos, const float* const* aims,
+                              int iters, int stop_rule, float* const* x_out, double* const* energies,
+                              int* const* iters_done, arapgs_node_xf* const* tables) {
+    CHECK_CTX(c);
+    if (!b || !node_pos || !aims || iters < 0) return fail(c, -1, "solver_batch_solve: bad arguments");
+    if (iters > GN_MAX_ITERS) return fail(c, -1, "solver_batch_solve: at most %d Gauss-Newton iterations", GN_MAX_ITERS);
+    if (b->device != c->device) return fail(c, -1, "solver_batch_solve: the batch is on another device");
+    const int n = (int)b->b.w.size();
+    for (int i = 0; i < n; i++)
+        if (!node_pos[i] || !aims[i]) return fail(c, -1, "solver_batch_solve: scene %d lacks node positions or aims", i);
+    if (!solver_batch_warm_uniform(b->b))
+        return fail(c, -1, "solver_batch_solve: every solver of a batch needs the same warm-start setting");
+    int rc = check_launch(c, solver_batch_set_frame(b->b, node_pos, aims, c->stream), "solver_batch_solve");
     if (rc) return rc;
-    const int every = s->host_check_every;
-    auto host_flag = [&](const int* dflag, int* out) -> int {
-        hipError_t e = hipMemcpyAsync(out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
-    };
-    for (int it = 0; it < iters; it++) {
-        rc = check_launch(c, solver_gn_begin(W, it, c->stream), "gn_solve rhs");
-        if (rc) return rc;
-        if (s->cg_mode == 1) {
-            rc = check_launch(c, solver_cg_persistent(W, c->stream), "gn_solve cg");
-            if (rc) return rc;
-        } else {
-            for (int k = 0; k < W.cg_max_iters; k++) {
-                rc = check_launch(c, solver_cg_iteration(W, k, c->stream), "gn_solve cg");
-                if (rc) return rc;
-                if (every > 0 && (k + 1) % every == 0) {
-                    int done = 0;
-                    if ((rc = host_flag(W.cg_done, &done))) return rc;
-                    if (done) break;
-                }
-            }
-        }
-        rc = check_launch(c, solver_gn_end(W, it, stop_rule, energies, c->stream), "gn_solve line search");
-        if (rc) return rc;
-        if (every > 0) {
-            int done = 0;
-            if ((rc = host_flag(W.gn_done, &done))) return rc;
-            if (done) break;
-        }
+    const bool refresh = (b->solves % std::max(1, b->coarse_refresh)) == 0;
+    b->solves++;
+    if ((rc = run_gn(c, b->b, b->host_check_every, 0, refresh, iters, stop_rule))) return rc;
+    SolverOutPtrs out = {};
+    for (int i = 0; i < n; i++) {
+        out.x[i] = x_out ? x_out[i] : nullptr;
+        out.energies[i] = energies ? energies[i] : nullptr;
+        out.iters_done[i] = iters_done ? iters_done[i] : nullptr;
+        out.table[i] = tables ? tables[i] : nullptr;
     }
-    if (iters == 0) {
-        hipError_t e = hipMemcpyAsync(energies, W.cand_e, sizeof(double), hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
-    }
-    if (x_out)
-        hipLaunchKernelGGL(copy_f32_kernel, dim3(64), dim3(256), 0, c->stream, x_out, (const float*)W.x, W.n);
-    if (iters_done)
-        hipLaunchKernelGGL(copy_done_kernel, dim3(1), dim3(1), 0, c->stream, W.gs,
-                           (const GridBarrier*)W.gbar, iters_done);
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
+    return check_launch(c, solver_batch_finish(b->b, out, c->stream), "solver_batch_solve");
 }
 
 int arapgs_gn_energy(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* aims, const float* x,
                      double* energy) {
     CHECK_CTX(c);
     if (!s || !node_pos || !aims || !x || !energy) return fail(c, -1, "gn_energy: bad arguments");
-    s->w.dev.g = node_pos;
-    s->w.dev.aim = aims;
-    return check_launch(c, solver_energy(s->w, x, energy, c->stream), "gn_energy");
+    int rc = check_launch(c, solver_batch_set_frame(s->one, &node_pos, &aims, c->stream), "gn_energy");
+    if (rc) return rc;
+    return check_launch(c, solver_energy(s->one, x, energy, c->stream), "gn_energy");
 }
 
 int arapgs_put_free_inputs(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, const float* x,
@@ -553,7 +644,10 @@ int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const 
 
 extern "C" {
 
-static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int* num_rendered_h) {
+// seg_h (segment mode, host int64[V+1] or null): view v draws Gaussians
+// [seg_h[v], seg_h[v+1]) only.
+static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int* num_rendered_h,
+                        const int64_t* seg_h = nullptr) {
     CHECK_CTX(c);
     if (!dv || V < 1 || V > RASTER_MAX_VIEWS) return fail(c, -1, "render: need 1..%d views", RASTER_MAX_VIEWS);
     const arapgs_render_desc* d = dv;
@@ -578,8 +672,14 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
                       e->boxmax_h != d->boxmax_h))
             return fail(c, -1, "render: view %d differs from view 0 in a shared argument", v);
     }
+    if (seg_h) {
+        if (seg_h[0] != 0 || seg_h[V] != d->P) return fail(c, -1, "render_segments: segments must run from 0 to P");
+        for (int v = 0; v < V; v++)
+            if (seg_h[v + 1] < seg_h[v]) return fail(c, -1, "render_segments: segment %d ends before it starts", v);
+    }
     (void)hipSetDevice(c->device);
     RasterParams rp;
+    rp.seg = nullptr;
     rp.P = d->P; rp.D = d->D; rp.W = d->width; rp.H = d->height; rp.tile = tile; rp.V = V;
     rp.gx = (d->width + tile - 1) / tile; rp.gy = (d->height + tile - 1) / tile;
     rp.bg = d->background; rp.means = d->means3D; rp.shs = d->shs; rp.opac = d->opacities; rp.scales = d->scales;
@@ -606,7 +706,13 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     hipError_t e = hipMemcpyAsync(c->r_views, hv, sizeof(RasterView) * V, hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "render views");
     rp.views = c->r_views;
-    const int64_t PV = (int64_t)std::max(d->P, 1) * V;
+    if (seg_h) {
+        for (int v = 0; v <= V; v++) c->r_seg_h[v] = (int)seg_h[v];
+        e = hipMemcpyAsync(c->r_seg, c->r_seg_h, sizeof(int) * (V + 1), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "render segments");
+        rp.seg = c->r_seg;
+    }
+    const int64_t PV = seg_h ? (int64_t)std::max(d->P, 1) : (int64_t)std::max(d->P, 1) * V;
     int rc = grow(c, &c->r_geo, &c->r_geo_cap, raster_geo_bytes(PV));
     if (!rc) rc = grow(c, &c->r_scan, &c->r_scan_cap, sizeof(uint32_t) * scan_workspace_words(PV));
     const int ntiles = rp.gx * rp.gy;
@@ -664,6 +770,15 @@ int arapgs_render_views(arapgs_ctx* c, const arapgs_render_desc* descs, int n_vi
     return render_views(c, descs, n_views, num_rendered_h);
 }
 
+int arapgs_render_segments(arapgs_ctx* c, const arapgs_render_desc* descs, int n_views, const int64_t* seg_h,
+                           int* num_rendered_h) {
+    if (!seg_h) {
+        CHECK_CTX(c);
+        return fail(c, -1, "render_segments: segments required");
+    }
+    return render_views(c, descs, n_views, num_rendered_h, seg_h);
+}
+
 int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64_t cap, uint32_t* ranges_h,
                         int64_t ranges_cap) {
     CHECK_CTX(c);
@@ -674,7 +789,8 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
     if (e == hipSuccess && keys_h && n > 0) {
         // the sorted keys hold the tile id; the reference layout is tile << 32 | depth bits of the value's record
         std::vector<uint32_t> v((size_t)n), tile((size_t)n);
-        const int64_t PV = (int64_t)std::max(c->last_rp.P, 1) * c->last_rp.V;
+        const int64_t PV = c->last_rp.seg ? (int64_t)std::max(c->last_rp.P, 1)
+                                          : (int64_t)std::max(c->last_rp.P, 1) * c->last_rp.V;
         std::vector<uint32_t> depth((size_t)PV);
         e = hipMemcpy(tile.data(), b, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(v.data(), b + nn * BIN_VALS_AT, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);

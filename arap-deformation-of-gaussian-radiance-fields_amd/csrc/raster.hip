@@ -33,13 +33,15 @@ struct Geo {
     uint32_t* touched; // P
 };
 
-// One Gaussian in one view (record index gv = v*P + g).
+// One Gaussian in one view (record index gv = v*P + g, or g in segment mode);
+// the view's optional per-Gaussian outputs are indexed by gl (g, or in segment
+// mode the Gaussian's index within its view's segment).
 __device__ __forceinline__ void project_one(const RasterParams& rp, const RasterView& vw, int g, int64_t gv, Geo& geo,
-                                            const float* __restrict__ sh) {
+                                            const float* __restrict__ sh, int gl) {
     int* radii = vw.radii;
     int* rects_out = vw.rects;
     geo.touched[gv] = 0;
-    if (radii) radii[g] = 0;
+    if (radii) radii[gl] = 0;
     const float px = rp.means[3 * g], py = rp.means[3 * g + 1], pz = rp.means[3 * g + 2];
     if (rp.use_box && (px < rp.boxmin[0] || py < rp.boxmin[1] || pz < rp.boxmin[2] || px > rp.boxmax[0] ||
                        py > rp.boxmax[1] || pz > rp.boxmax[2]))
@@ -153,8 +155,8 @@ __device__ __forceinline__ void project_one(const RasterParams& rp, const Raster
     geo.rgb[gv] = make_float4(col[0], col[1], col[2], 0.0f);
     geo.rect[gv] = make_int4(r0x, r0y, r1x, r1y);
     geo.touched[gv] = (uint32_t)area;
-    if (radii) radii[g] = rad;
-    if (rects_out) { rects_out[2 * g] = ex; rects_out[2 * g + 1] = ey; }
+    if (radii) radii[gl] = rad;
+    if (rects_out) { rects_out[2 * gl] = ex; rects_out[2 * gl + 1] = ey; }
 }
 
 // One thread per Gaussian for all V views: the 48 SH coefficients are read once.
@@ -172,14 +174,26 @@ __global__ void __launch_bounds__(256) preprocess_kernel(RasterParams rp, Geo ge
             const float4 q = s4[i];
             sh[4 * i] = q.x; sh[4 * i + 1] = q.y; sh[4 * i + 2] = q.z; sh[4 * i + 3] = q.w;
         }
-        for (int v = 0; v < rp.V; v++) {
+        if (rp.seg) {
+            // segment mode: one record per Gaussian, in the view whose segment holds it
+            const int v = segment_of(rp.seg, rp.V, g);
             const RasterView vw = rp.views[v];
-            const int64_t gv = (int64_t)v * rp.P + g;
-            project_one(rp, vw, g, gv, geo, sh);
-            if (geo.touched[gv]) {
-                const uint32_t db = __float_as_uint(geo.depth[gv]);
+            project_one(rp, vw, g, g, geo, sh, g - rp.seg[v]);
+            if (geo.touched[g]) {
+                const uint32_t db = __float_as_uint(geo.depth[g]);
                 mn = min(mn, db);
                 mx = max(mx, db);
+            }
+        } else {
+            for (int v = 0; v < rp.V; v++) {
+                const RasterView vw = rp.views[v];
+                const int64_t gv = (int64_t)v * rp.P + g;
+                project_one(rp, vw, g, gv, geo, sh, g);
+                if (geo.touched[gv]) {
+                    const uint32_t db = __float_as_uint(geo.depth[gv]);
+                    mn = min(mn, db);
+                    mx = max(mx, db);
+                }
             }
         }
     }
@@ -234,7 +248,8 @@ __global__ void __launch_bounds__(256) gather_touched_kernel(int64_t PV, const u
 // their slots, which are consecutive, with consecutive lanes on consecutive
 // slots: the lane of slot j finds its record (the last of the 64 whose first
 // slot is <= j) by a binary search over the lanes' first slots.
-__global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, int gx, int ntiles, Geo geo,
+__global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, const int* __restrict__ seg, int V,
+                                                         int gx, int ntiles, Geo geo,
                                                          const uint32_t* __restrict__ rvals,
                                                          const uint32_t* __restrict__ offsets,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -252,7 +267,7 @@ __global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, int 
             rx = rc.x;
             ry = rc.y;
             rw = rc.z - rc.x;
-            base = (gv / (uint32_t)P) * (uint32_t)ntiles;
+            base = (seg ? (uint32_t)segment_of(seg, V, (int)gv) : gv / (uint32_t)P) * (uint32_t)ntiles;
         }
     }
     const uint64_t vmask = __ballot(valid);
@@ -318,7 +333,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
     const int tile = blockIdx.x - view * ntl;
     float* __restrict__ out = rp.views[view].out;
     int* __restrict__ surface = rp.views[view].surface;
-    const uint32_t vbase = (uint32_t)view * (uint32_t)rp.P;
+    const uint32_t vbase = rp.seg ? (uint32_t)rp.seg[view] : (uint32_t)view * (uint32_t)rp.P;
     const int tx = tile % rp.gx, ty = tile / rp.gx;
     const int px = tx * TILE + (threadIdx.x % TILE), py = ty * TILE + (threadIdx.x / TILE);
     const bool inside = px < rp.W && py < rp.H;
@@ -414,7 +429,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
     const int tile = blockIdx.x - view * ntl;
     float* __restrict__ out = rp.views[view].out;
     int* __restrict__ surface = rp.views[view].surface;
-    const uint32_t vbase = (uint32_t)view * (uint32_t)rp.P;
+    const uint32_t vbase = rp.seg ? (uint32_t)rp.seg[view] : (uint32_t)view * (uint32_t)rp.P;
     const int tx = tile % rp.gx, ty = tile / rp.gx;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int px = tx * 16 + (wv & 1) * 8 + (lane & 7), py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
@@ -927,7 +942,7 @@ static uint32_t* geo_offsets(void* base, int64_t PV) {
 }
 
 int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, uint32_t* d_counts, hipStream_t st) {
-    const int64_t PV = (int64_t)rp.P * rp.V;
+    const int64_t PV = raster_records(rp);
     Geo geo = carve_geo(geo_ws, PV);
     hipLaunchKernelGGL(depth_range_init_kernel, dim3(1), dim3(1), 0, st, d_counts + 1);
     if (rp.P > 0) {
@@ -945,7 +960,7 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
                          uint32_t dmin, int dbits, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
                          hipStream_t st) {
-    const int64_t PV = (int64_t)rp.P * rp.V;
+    const int64_t PV = raster_records(rp);
     Geo geo = carve_geo(geo_ws, PV);
     const int ntiles = rp.gx * rp.gy;
     const int64_t nt = (int64_t)ntiles * rp.V;
@@ -973,8 +988,8 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
         uint32_t* offs = geo_offsets(geo_ws, PV);
         rc = launch_exclusive_scan(rtouched, PV, offs, (uint32_t*)d_total, scan_ws, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(emit_tiles_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.gx, ntiles, geo,
-                           rvals, offs, keys, vals);
+        hipLaunchKernelGGL(emit_tiles_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.seg, rp.V, rp.gx,
+                           ntiles, geo, rvals, offs, keys, vals);
         int tile_bits = 0;
         while ((int64_t(1) << tile_bits) < nt) tile_bits++;
         rc = launch_radix_sort32(keys, vals, keys_alt, vals_alt, n_rendered, tile_bits, sort_ws, st);
@@ -999,6 +1014,7 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
 
 int raster_blend_backward(const RasterParams& rp, void* geo_ws, const uint32_t* vals, const uint2* ranges,
                           float* grec, hipStream_t st) {
+    if (rp.seg) return -1;  // segment-mode renders have no backward
     const int64_t PV = (int64_t)rp.P * rp.V;
     Geo geo = carve_geo(geo_ws, PV);
     const int64_t nt = (int64_t)rp.gx * rp.gy * rp.V;
@@ -1017,6 +1033,7 @@ int raster_blend_backward(const RasterParams& rp, void* geo_ws, const uint32_t* 
 
 int raster_preprocess_backward(const RasterParams& rp, void* geo_ws, const float* grec, float* gout,
                                hipStream_t st) {
+    if (rp.seg) return -1;
     if (rp.P <= 0) return 0;
     const int64_t PV = (int64_t)rp.P * rp.V;
     Geo geo = carve_geo(geo_ws, PV);

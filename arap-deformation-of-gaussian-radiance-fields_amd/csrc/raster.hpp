@@ -19,10 +19,14 @@ struct RasterView {
     int* rects;            // optional, 2*P
     const float* dout;     // backward only: dL/d out (3*H*W planar CHW)
 };
-constexpr int RASTER_MAX_VIEWS = 16;
+constexpr int RASTER_MAX_VIEWS = 64;
 
 // V views of one set of Gaussians at one resolution. Per-view records are laid
 // out view-major (index v*P + g); keys carry the combined tile id v*ntiles + t.
+// Segment mode (seg != null, device int[V+1] ascending Gaussian offsets with
+// seg[0] = 0, seg[V] = P): view v draws only Gaussians [seg[v], seg[v+1]) —
+// independent scenes stored back to back — and each Gaussian has one record
+// (index g).
 struct RasterParams {
     int P, D, W, H, tile, gx, gy, V;
     const float* bg;       // device, 3
@@ -36,7 +40,21 @@ struct RasterParams {
     int unroll;            // Gaussians per compositing step of the 16x16 blend (2, 4, 8)
     float boxmin[3], boxmax[3];
     const RasterView* views;  // device, V
+    const int* seg;           // device, V+1, or null
 };
+
+inline int64_t raster_records(const RasterParams& rp) { return rp.seg ? (int64_t)rp.P : (int64_t)rp.P * rp.V; }
+
+// The view whose segment holds Gaussian g: the last v with seg[v] <= g.
+__host__ __device__ inline int segment_of(const int* seg, int V, int g) {
+    int lo = 0, hi = V - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // Conservative test whether a Gaussian can reach alpha >= 1/255 at any pixel
 // centre of the box [x0, x1] x [y0, y1]. The blend skips a pixel when

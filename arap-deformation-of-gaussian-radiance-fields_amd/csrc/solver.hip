@@ -38,6 +38,37 @@ namespace {
 
 constexpr int LANES = 16;  // lanes per owner node
 
+// Pointers read from device memory are generic (flat) pointers to the
+// compiler, and flat loads and stores cost more than global ones. A pointer
+// rebuilt as an integer in the global address space lets every access through
+// it compile to a global load or store (all solver buffers are device memory).
+template <typename T>
+__device__ __forceinline__ T* gptr(T* p) {
+    typedef __attribute__((address_space(1))) T GT;
+    return (T*)(GT*)(uintptr_t)p;
+}
+
+// This block's entry, every pointer in the global address space.
+__device__ __forceinline__ SolverEntry load_entry(const SolverEntry* __restrict__ B) {
+    SolverEntry E = B[blockIdx.z];
+    SolverDev& S = E.S;
+    S.free_nodes = gptr(S.free_nodes); S.static_nodes = gptr(S.static_nodes); S.free_id = gptr(S.free_id);
+    S.static_id = gptr(S.static_id); S.nbr = gptr(S.nbr); S.edge_scale = gptr(S.edge_scale);
+    S.src_idx = gptr(S.src_idx); S.src_w = gptr(S.src_w); S.con_ptr = gptr(S.con_ptr); S.con_mem = gptr(S.con_mem);
+    S.caim_ptr = gptr(S.caim_ptr); S.caim_node = gptr(S.caim_node); S.in_ptr = gptr(S.in_ptr);
+    S.in_edge = gptr(S.in_edge); S.cm_ptr = gptr(S.cm_ptr); S.cm_con = gptr(S.cm_con); S.cm_sbeg = gptr(S.cm_sbeg);
+    S.cm_src = gptr(S.cm_src); S.multi_con = gptr(S.multi_con); S.agg_ptr = gptr(S.agg_ptr);
+    S.agg_member = gptr(S.agg_member); S.agg_of = gptr(S.agg_of); S.agg_center = gptr(S.agg_center);
+    S.g = gptr(S.g); S.aim = gptr(S.aim);
+    E.x = gptr(E.x); E.hsol = gptr(E.hsol); E.r = gptr(E.r); E.z = gptr(E.z); E.p = gptr(E.p); E.q = gptr(E.q);
+    E.binv = gptr(E.binv); E.rc = gptr(E.rc); E.acinv = gptr(E.acinv); E.gj_piv = gptr(E.gj_piv);
+    E.gj_col = gptr(E.gj_col); E.hwarm = gptr(E.hwarm); E.ac64 = gptr(E.ac64); E.partial = gptr(E.partial);
+    E.partial2 = gptr(E.partial2); E.rzp = gptr(E.rzp); E.e_partial = gptr(E.e_partial); E.cand_e = gptr(E.cand_e);
+    E.acc_partial = gptr(E.acc_partial); E.energies = gptr(E.energies); E.cg = gptr(E.cg); E.gs = gptr(E.gs);
+    E.gn_done = gptr(E.gn_done); E.cg_done = gptr(E.cg_done); E.gbar = gptr(E.gbar);
+    return E;
+}
+
 __device__ __forceinline__ float xR(const float* x, int r, int c) { return x[3 * c + r]; }
 
 // ---------------------------------------------------------------- residuals
@@ -178,22 +209,27 @@ __device__ __forceinline__ bool first_accepted(const double* first_e, const GnSc
     return first_e && !(first_e[0] > gs->e_old);
 }
 
-// Energies of x + 2^-c h for c in [c0, c0 + gridDim.y) (h == nullptr: the
+// Energies of x + 2^-c h for c in [c0, c0 + gridDim.y) (use_h == 0: the
 // energy of x). blockIdx.y = c - c0. Per-block FP64 partials at
-// partial[c * gridDim.x + block], reduced by reduce_partials_kernel.
-__global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* __restrict__ x,
-                                                     const float* __restrict__ h, double* __restrict__ partial,
-                                                     const int* __restrict__ skip, int c0,
-                                                     const double* __restrict__ first_e,
-                                                     const GnScalars* __restrict__ gs) {
-    if (skip && *skip) return;
-    if (first_accepted(first_e, gs)) return;
+// partial[c * e_blocks + block], reduced by reduce_partials_kernel. use_skip:
+// nothing once the solve stopped; lazy: nothing when the full step (cand_e[0])
+// was accepted. x_ext (single-scene calls) replaces the entry's x.
+__global__ void __launch_bounds__(256) energy_kernel(const SolverEntry* __restrict__ B, int use_h, int use_skip, int c0,
+                                                     int lazy, const float* __restrict__ x_ext) {
+    const SolverEntry E = load_entry(B);
+    if ((int)blockIdx.x >= E.e_blocks) return;
+    if (use_skip && *E.gn_done) return;
+    if (lazy && first_accepted(E.cand_e, E.gs)) return;
+    const SolverDev& S = E.S;
+    const float* __restrict__ x = x_ext ? x_ext : E.x;
+    const float* __restrict__ h = use_h ? E.hsol : nullptr;
+    double* __restrict__ partial = E.e_partial;
     __shared__ double sh[8];
     const int c = blockIdx.y + c0;
     const float alpha = ldexpf(1.0f, -c);
     double acc = 0.0;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
-         g += (int64_t)gridDim.x * blockDim.x) {
+         g += (int64_t)E.e_blocks * blockDim.x) {
         double f[6];
         const int nr = group_residual(S, x, h, alpha, g, f);
 #pragma unroll
@@ -201,14 +237,18 @@ __global__ void __launch_bounds__(256) energy_kernel(SolverDev S, const float* _
             if (r < nr) acc += f[r] * f[r];
     }
     const double t = block_reduce_d(acc, sh);
-    if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = t;
+    if (threadIdx.x == 0) partial[(int64_t)c * E.e_blocks + blockIdx.x] = t;
 }
 
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int c0, int ncand,
-                                       double* __restrict__ out, const int* __restrict__ skip,
-                                       const double* __restrict__ first_e, const GnScalars* __restrict__ gs) {
-    if (skip && *skip) return;
-    if (first_accepted(first_e, gs)) return;
+// out_ext (single-scene calls) replaces the entry's cand_e.
+__global__ void reduce_partials_kernel(const SolverEntry* __restrict__ B, int c0, int ncand, int use_skip, int lazy,
+                                       double* __restrict__ out_ext) {
+    const SolverEntry E = load_entry(B);
+    if (use_skip && *E.gn_done) return;
+    if (lazy && first_accepted(E.cand_e, E.gs)) return;
+    const double* __restrict__ partial = E.e_partial;
+    const int nblocks = E.e_blocks;
+    double* __restrict__ out = out_ext ? out_ext : E.cand_e;
     __shared__ double sh[8];
     for (int c = c0; c < c0 + ncand; c++) {
         double acc = 0.0;
@@ -418,10 +458,14 @@ __device__ void invert12(float A[144]) {
 // D_i = (the rotation group's 9x9 block, lane 0 only) + the I_3-structured sum
 // of the 3-row groups, C[p][q] = sum a'_p a'_q with a' = (a_0, a_1, a_2, b),
 // placed at (3p + r, 3q + r) (index 3 -> 9 + r): 10 sums per lane to reduce.
-__global__ void __launch_bounds__(256) rhs_diag_kernel(SolverDev S, const float* __restrict__ x,
-                                                       float* __restrict__ b, float* __restrict__ binv,
-                                                       double* __restrict__ partial, const int* __restrict__ skip) {
-    if (*skip) return;
+__global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    if ((int)blockIdx.x >= E.nb_owner || *E.cg_done) return;
+    const SolverDev& S = E.S;
+    const float* __restrict__ x = E.x;
+    float* __restrict__ b = E.r;
+    float* __restrict__ binv = E.binv;
+    double* __restrict__ partial = E.partial;
     __shared__ float sD[256 / LANES][144];
     __shared__ double shd[8];
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
@@ -754,52 +798,57 @@ __device__ __forceinline__ float cg_alpha(const SolverDev& S, const double* pq_p
     return pq != 0.0 ? (float)(rz / pq) : 0.0f;
 }
 
+// CG iteration k (r.z partials of iterations k and k-1 in the two halves of
+// rzp). The vector is z, or (warm_iter >= 0) the warm-start step of that GN
+// iteration, or v_ext (single-scene test calls).
 template <int KT>
-__global__ void __launch_bounds__(256) apply_kernel(SolverDev S, const float* __restrict__ x,
-                                                    const float* __restrict__ z, const double* __restrict__ rz_cur,
-                                                    const double* __restrict__ rz_prev, int first,
-                                                    float* __restrict__ p, float* __restrict__ q,
-                                                    double* __restrict__ partial, const int* __restrict__ skip) {
-    if (*skip) return;
+__global__ void __launch_bounds__(256) apply_kernel(const SolverEntry* __restrict__ B, int k, int first, int warm_iter,
+                                                    const float* __restrict__ v_ext) {
+    const SolverEntry E = load_entry(B);
+    if ((int)blockIdx.x >= E.nb_owner || *E.cg_done) return;
+    const SolverDev& S = E.S;
+    const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
+    const double* rz_prev = E.rzp + ((k + 1) & 1) * S.n_agg;
+    const float* z = v_ext ? v_ext : (warm_iter >= 0 ? E.hwarm + (int64_t)warm_iter * E.n : E.z);
     __shared__ double shd[8];
     const float beta = cg_beta(S, rz_cur, rz_prev, first != 0, shd);
-    apply_body<KT>(S, x, z, beta, first != 0, p, q, partial, blockIdx.x, shd);
+    apply_body<KT>(S, E.x, z, beta, first != 0, E.p, E.q, E.partial, blockIdx.x, shd);
 }
 
 // One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
 // z = Binv r and the coarse restriction rc_a = P_a^T r; partial |r|^2.
-__global__ void __launch_bounds__(256) update_restrict_kernel(SolverDev S, int mode, const double* __restrict__ pq_part,
-                                                              int n_pq, const double* __restrict__ rz_cur,
-                                                              float* __restrict__ hsol, float* __restrict__ r,
-                                                              const float* __restrict__ p, const float* __restrict__ q,
-                                                              const float* __restrict__ binv, float* __restrict__ z,
-                                                              float* __restrict__ rc, double* __restrict__ partial,
-                                                              const int* __restrict__ skip) {
-    if (*skip) return;
+__global__ void __launch_bounds__(256) update_restrict_kernel(const SolverEntry* __restrict__ B, int mode, int k) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    if ((int)blockIdx.x >= S.n_agg || *E.cg_done) return;
     __shared__ float sred[256 / LANES][12];
     __shared__ double shd[8];
-    const float alpha = mode == 1 ? cg_alpha(S, pq_part, n_pq, rz_cur, shd) : 0.0f;
-    update_body(S, mode, alpha, hsol, r, p, q, binv, z, rc, partial, blockIdx.x, sred, shd);
+    const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
+    const float alpha = mode == 1 ? cg_alpha(S, E.partial, E.nb_owner, rz_cur, shd) : 0.0f;
+    update_body(S, mode, alpha, E.hsol, E.r, mode == 1 ? E.p : nullptr, mode == 1 ? E.q : nullptr, E.binv, E.z, E.rc,
+                E.partial2, blockIdx.x, sred, shd);
 }
 
 // One workgroup per aggregate: zc_a = (Ac^-1 rc)_a, z += P_a zc_a; partial r.z.
 // Workgroup 0 also runs the convergence test on the |r|^2 partials.
-__global__ void __launch_bounds__(256) coarse_prolong_kernel(SolverDev S, const float* __restrict__ acinv,
-                                                             const float* __restrict__ rc, const float* __restrict__ r,
-                                                             float* __restrict__ z, double* __restrict__ rz_out,
-                                                             const double* __restrict__ rr_part,
-                                                             CgScalars* __restrict__ cs, int test, int* __restrict__ skip) {
-    if (*skip) return;
+// test == 0 (the preconditioned start): rz_0 into the first half of rzp, no
+// convergence test; test == 1: CG iteration k, rz_{k+1} into half (k+1) & 1.
+__global__ void __launch_bounds__(256) coarse_prolong_kernel(const SolverEntry* __restrict__ B, int k, int test) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    if ((int)blockIdx.x >= S.n_agg || *E.cg_done) return;
     __shared__ float zc[12];
     __shared__ float zred[4][12];
     __shared__ double shd[8];
     const int a = blockIdx.x;
-    prolong_body(S, acinv, rc, r, z, rz_out, a, zc, zred, shd);
-    const double rr = (a == 0 && test) ? block_sum_partials(rr_part, S.n_agg, shd) : 0.0;
+    double* rz_out = E.rzp + (test ? ((k + 1) & 1) : 0) * S.n_agg;
+    prolong_body(S, E.acinv, E.rc, E.r, E.z, rz_out, a, zc, zred, shd);
+    const double rr = (a == 0 && test) ? block_sum_partials(E.partial2, S.n_agg, shd) : 0.0;
     if (threadIdx.x == 0 && a == 0 && test) {
+        CgScalars* cs = E.cg;
         cs->rr = rr;
         cs->iters += 1;
-        if (rr <= cs->rr_stop || cs->iters >= cs->max_iters) *skip = 1;
+        if (rr <= cs->rr_stop || cs->iters >= cs->max_iters) *E.cg_done = 1;
     }
 }
 
@@ -906,11 +955,14 @@ __global__ void grid_barrier_reset_kernel(GridBarrier* gb) {
 // all-zero right-hand side ends the solve.
 constexpr double CG_TOL_FLOOR = 1e-6;
 
-__global__ void cg_init_kernel(const double* __restrict__ bb_part, int n, CgScalars* __restrict__ cs,
-                               GnScalars* __restrict__ gs, int* __restrict__ skip) {
+__global__ void cg_init_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    int* __restrict__ skip = E.cg_done;
     if (*skip) return;
     if (threadIdx.x != 0) return;
-    const double bb = sum_partials(bb_part, n);
+    CgScalars* __restrict__ cs = E.cg;
+    GnScalars* __restrict__ gs = E.gs;
+    const double bb = sum_partials(E.partial, E.nb_owner);
     if (gs->iters_done == 0) gs->bb0 = bb;
     cs->bb = bb;
     cs->rr = bb;
@@ -925,13 +977,27 @@ __global__ void zero_kernel(float* __restrict__ v, int64_t n, const int* __restr
         v[i] = 0.0f;
 }
 
+// h = 0 for a cold start (also when the CG is skipped for a negligible gradient:
+// the line search then takes a zero step).
+__global__ void zero_step_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    if (*E.gn_done) return;
+    float* __restrict__ v = E.hsol;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E.n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = 0.0f;
+}
+
 // Warm-started CG: h = h0 and r = b - A h0 (q holds A h0); a CG skipped for a
 // negligible gradient keeps h = 0 like the cold start.
-__global__ void warm_init_kernel(int64_t n, float* __restrict__ hsol, const float* __restrict__ h0,
-                                 float* __restrict__ r, const float* __restrict__ q, const int* __restrict__ cg_done,
-                                 const int* __restrict__ gn_done) {
-    if (*gn_done) return;
-    const bool skip = *cg_done != 0;
+__global__ void warm_init_kernel(const SolverEntry* __restrict__ B, int iter) {
+    const SolverEntry E = load_entry(B);
+    if (*E.gn_done) return;
+    const int64_t n = E.n;
+    float* __restrict__ hsol = E.hsol;
+    const float* __restrict__ h0 = E.hwarm + (int64_t)iter * n;
+    float* __restrict__ r = E.r;
+    const float* __restrict__ q = E.q;
+    const bool skip = *E.cg_done != 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (skip) {
             hsol[i] = 0.0f;
@@ -949,9 +1015,22 @@ __global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ s
         dst[i] = src[i];
 }
 
+// The step of GN iteration iter becomes next solve's warm start for it.
+__global__ void save_warm_kernel(const SolverEntry* __restrict__ B, int iter) {
+    const SolverEntry E = load_entry(B);
+    if (*E.gn_done) return;
+    float* __restrict__ dst = E.hwarm + (int64_t)iter * E.n;
+    const float* __restrict__ src = E.hsol;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E.n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- coarse space
-__global__ void agg_center_kernel(SolverDev S) {
+__global__ void agg_center_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
     const int a = blockIdx.x;
+    if (a >= S.n_agg) return;
     __shared__ float sx[256], sy[256], sz[256];
     float x = 0.0f, y = 0.0f, z = 0.0f;
     const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
@@ -987,7 +1066,19 @@ __device__ __forceinline__ void coarse_add(unsigned long long* Ac, int64_t at, f
     atomicAdd(Ac + at, (unsigned long long)__double2ll_rn((double)s * COARSE_FX));
 }
 
-__global__ void coarse_to_float_kernel(const unsigned long long* __restrict__ Ac64, float* __restrict__ Ac, int64_t n) {
+__global__ void zero_coarse_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const int64_t n = 2 * (int64_t)E.S.nc_pad * E.S.nc_pad;  // the 64-bit entries as floats
+    float* __restrict__ v = reinterpret_cast<float*>(E.ac64);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = 0.0f;
+}
+
+__global__ void coarse_to_float_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const unsigned long long* __restrict__ Ac64 = E.ac64;
+    float* __restrict__ Ac = E.acinv;
+    const int64_t n = (int64_t)E.S.nc_pad * E.S.nc_pad;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         Ac[i] = (float)((double)(long long)Ac64[i] / COARSE_FX);
 }
@@ -999,7 +1090,11 @@ __global__ void coarse_to_float_kernel(const unsigned long long* __restrict__ Ac
 // the same per-entry values as the one-member path, summed by one thread in
 // entry order (deterministic); then every pair of touched aggregates adds its
 // 3-row outer product. One workgroup per group; LDS: 24 bytes per aggregate.
-__global__ void __launch_bounds__(256) coarse_multi_kernel(SolverDev S, unsigned long long* __restrict__ Ac) {
+__global__ void __launch_bounds__(256) coarse_multi_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    if ((int)blockIdx.x >= S.n_multi) return;
+    unsigned long long* __restrict__ Ac = E.ac64;
     extern __shared__ float lds[];
     float* coef = lds;                                   // 4 per aggregate
     int* list = reinterpret_cast<int*>(coef + 4 * S.n_agg);  // touched aggregates
@@ -1059,14 +1154,19 @@ __global__ void __launch_bounds__(256) coarse_multi_kernel(SolverDev S, unsigned
 // aggregates, the 16 products of their 4-coefficient rows placed in the three
 // rows. One thread per group; the (aggregate, coefficients) list of a group
 // lives in LDS and nodes of one aggregate are summed in the group's node order.
-__global__ void __launch_bounds__(128) coarse_assemble_kernel(SolverDev S, const float* __restrict__ x,
-                                                              unsigned long long* __restrict__ Ac) {
+__global__ void __launch_bounds__(128) coarse_assemble_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    const int64_t nb = (S.n_groups + 127) / 128;
+    if ((int64_t)blockIdx.x >= nb) return;
+    const float* __restrict__ x = E.x;
+    unsigned long long* __restrict__ Ac = E.ac64;
     __shared__ float cv[128][KNN_MAX_T + 1][4];
     __shared__ int ca[128][KNN_MAX_T + 1];
     const int tid = threadIdx.x;
     const int nc = S.nc_pad;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S.n_groups;
-         g += (int64_t)gridDim.x * blockDim.x) {
+         g += nb * blockDim.x) {
         int64_t gg = g;
         float Jrot[6][9];
         if (gg < S.nF) {
@@ -1205,7 +1305,12 @@ __device__ __forceinline__ void gj_steps4(float (&c)[GJ_R], int lane, int wave, 
     }
 }
 
-__global__ void __launch_bounds__(256) gj_pivot_kernel(float* __restrict__ A, int n, int kb, float* __restrict__ piv) {
+__global__ void __launch_bounds__(256) gj_pivot_kernel(const SolverEntry* __restrict__ B, int kb) {
+    const SolverEntry E = load_entry(B);
+    if (kb >= E.nt) return;
+    float* __restrict__ A = E.acinv;
+    const int n = E.S.nc_pad;
+    float* __restrict__ piv = E.gj_piv;
     __shared__ float prow[GJ_T];
     const int k0 = kb * GJ_T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1241,10 +1346,14 @@ __device__ __forceinline__ void tile_mm(const float (*X)[GJ_T + 1], const float 
 
 // blockIdx.x = tile column jb != kb: row panel A_Kj <- piv * A_Kj; also copies
 // the old column panel tile A_{jb,K} into colbuf.
-__global__ void __launch_bounds__(256) gj_panel_kernel(float* __restrict__ A, int n, int kb, const float* __restrict__ piv,
-                                                       float* __restrict__ colbuf) {
+__global__ void __launch_bounds__(256) gj_panel_kernel(const SolverEntry* __restrict__ B, int kb) {
+    const SolverEntry E = load_entry(B);
     const int jb = blockIdx.x;
-    if (jb == kb) return;
+    if (kb >= E.nt || jb >= E.nt || jb == kb) return;
+    float* __restrict__ A = E.acinv;
+    const int n = E.S.nc_pad;
+    const float* __restrict__ piv = E.gj_piv;
+    float* __restrict__ colbuf = E.gj_col;
     __shared__ float Pv[GJ_T][GJ_T + 1];
     __shared__ float R[GJ_T][GJ_T + 1];
     const int k0 = kb * GJ_T, j0 = jb * GJ_T;
@@ -1266,9 +1375,14 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(float* __restrict__ A, in
 
 // blockIdx = (ib, jb): A_ij -= C_iK * A_Kj (new row panel) for ib != kb;
 // for jb == kb the tile becomes -C_iK * piv; tile (kb, kb) becomes piv.
-__global__ void __launch_bounds__(256) gj_update_kernel(float* __restrict__ A, int n, int kb, const float* __restrict__ piv,
-                                                        const float* __restrict__ colbuf) {
+__global__ void __launch_bounds__(256) gj_update_kernel(const SolverEntry* __restrict__ B, int kb) {
+    const SolverEntry E = load_entry(B);
     const int ib = blockIdx.y, jb = blockIdx.x;
+    if (kb >= E.nt || ib >= E.nt || jb >= E.nt) return;
+    float* __restrict__ A = E.acinv;
+    const int n = E.S.nc_pad;
+    const float* __restrict__ piv = E.gj_piv;
+    const float* __restrict__ colbuf = E.gj_col;
     const int k0 = kb * GJ_T, i0 = ib * GJ_T, j0 = jb * GJ_T;
     if (ib == kb) {
         if (jb == kb)
@@ -1297,7 +1411,10 @@ __global__ void __launch_bounds__(256) gj_update_kernel(float* __restrict__ A, i
         }
 }
 
-__global__ void pad_identity_kernel(float* __restrict__ A, int n, int nc) {
+__global__ void pad_identity_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    float* __restrict__ A = E.acinv;
+    const int n = E.S.nc_pad, nc = 12 * E.S.n_agg;
     for (int i = nc + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) A[(int64_t)i * n + i] = 1.0f;
 }
 
@@ -1313,19 +1430,22 @@ __device__ __forceinline__ int chosen_candidate(const double* cand_e, int ncand,
     return -1;
 }
 
-__global__ void __launch_bounds__(256) accept_partial_kernel(int n, float* __restrict__ x,
-                                                             const float* __restrict__ hsol,
-                                                             const double* __restrict__ cand_e, int ncand,
-                                                             const GnScalars* __restrict__ gs,
-                                                             double* __restrict__ partial, const int* __restrict__ gn_done) {
-    if (*gn_done) return;
+__global__ void __launch_bounds__(256) accept_partial_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    if ((int)blockIdx.x >= E.acc_blocks || *E.gn_done) return;
+    const int n = E.n, ncand = E.n_cand, nblocks = E.acc_blocks;
+    float* __restrict__ x = E.x;
+    const float* __restrict__ hsol = E.hsol;
+    const double* __restrict__ cand_e = E.cand_e;
+    const GnScalars* __restrict__ gs = E.gs;
+    double* __restrict__ partial = E.acc_partial;
     __shared__ int chosen;
     __shared__ double sh[8];
     if (threadIdx.x == 0) chosen = chosen_candidate(cand_e, ncand, gs);
     __syncthreads();
     const float a = chosen >= 0 ? ldexpf(1.0f, -chosen) : 0.0f;
     double nx = 0.0, nh = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x) {
         nx += (double)x[i] * x[i];
         nh += (double)hsol[i] * hsol[i];
         if (chosen >= 0) x[i] += a * hsol[i];
@@ -1338,12 +1458,16 @@ __global__ void __launch_bounds__(256) accept_partial_kernel(int n, float* __res
     }
 }
 
-__global__ void __launch_bounds__(256) accept_final_kernel(int nblocks, const double* __restrict__ partial,
-                                                           const double* __restrict__ cand_e, int ncand,
-                                                           GnScalars* __restrict__ gs, const CgScalars* __restrict__ cs,
-                                                           double* __restrict__ energies, int iter, int stop_rule,
-                                                           int* __restrict__ gn_done) {
+__global__ void __launch_bounds__(256) accept_final_kernel(const SolverEntry* __restrict__ B, int iter, int stop_rule) {
+    const SolverEntry E = load_entry(B);
+    int* __restrict__ gn_done = E.gn_done;
     if (*gn_done) return;
+    const int nblocks = E.acc_blocks, ncand = E.n_cand;
+    const double* __restrict__ partial = E.acc_partial;
+    const double* __restrict__ cand_e = E.cand_e;
+    GnScalars* __restrict__ gs = E.gs;
+    const CgScalars* __restrict__ cs = E.cg;
+    double* __restrict__ energies = E.energies;
     __shared__ double sh[8];
     double nx = 0.0, nh = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
@@ -1366,8 +1490,12 @@ __global__ void __launch_bounds__(256) accept_final_kernel(int nblocks, const do
     }
 }
 
-__global__ void init_x_kernel(int n, float* __restrict__ x, GnScalars* __restrict__ gs, int* __restrict__ gn_done,
-                              const double* __restrict__ e0) {
+__global__ void init_x_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const int n = E.n;
+    float* __restrict__ x = E.x;
+    GnScalars* __restrict__ gs = E.gs;
+    int* __restrict__ gn_done = E.gn_done;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int c = i % 12;
         x[i] = (c == 0 || c == 4 || c == 8) ? 1.0f : 0.0f;
@@ -1376,17 +1504,81 @@ __global__ void init_x_kernel(int n, float* __restrict__ x, GnScalars* __restric
         *gn_done = 0;
         gs->iters_done = 0;
         gs->cg_total = 0;
-        (void)e0;
     }
 }
 
-__global__ void set_eold_kernel(GnScalars* __restrict__ gs, const double* __restrict__ e) { gs->e_old = e[0]; }
+__global__ void set_eold_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    E.gs->e_old = E.cand_e[0];
+    E.energies[0] = E.cand_e[0];
+}
 
-__global__ void cg_begin_kernel(const int* __restrict__ gn_done, int* __restrict__ cg_done, CgScalars* __restrict__ s,
-                                int max_iters, double tol2) {
-    *cg_done = *gn_done;
-    s->max_iters = max_iters;
-    s->tol2 = tol2;
+__global__ void cg_begin_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    *E.cg_done = *E.gn_done;
+    E.cg->max_iters = E.cg_max_iters;
+    E.cg->tol2 = E.tol2;
+}
+
+// Entries read this frame's node positions and aims.
+struct FramePtrs {
+    const float* g[SOLVER_BATCH_MAX];
+    const float* aim[SOLVER_BATCH_MAX];
+};
+
+__global__ void set_frame_kernel(SolverEntry* __restrict__ B, int n, FramePtrs f) {
+    const int s = threadIdx.x;
+    if (s < n) {
+        B[s].S.g = f.g[s];
+        B[s].S.aim = f.aim[s];
+    }
+}
+
+// flag = 1 when every scene's cg_done (which 0) or gn_done (which 1) is set.
+__global__ void batch_flag_kernel(const SolverEntry* __restrict__ B, int n, int which, int* __restrict__ flag) {
+    if (threadIdx.x != 0) return;
+    int all = 1;
+    for (int s = 0; s < n; s++) all &= (which ? *B[s].gn_done : *B[s].cg_done) != 0 ? 1 : 0;
+    *flag = all;
+}
+
+// Per-scene outputs: x, the energies of the iterations run, the iteration count
+// (-1 when the persistent PCG kernel's grid barrier gave up: the solve is then
+// invalid) and the node table of putFreeInputs (Deform.hpp:140-151).
+__global__ void finish_kernel(const SolverEntry* __restrict__ B, SolverOutPtrs out) {
+    const int s = blockIdx.z;
+    const SolverEntry E = load_entry(B);
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int done = E.gs->iters_done;
+    if (float* xo = out.x[s])
+        for (int64_t i = i0; i < E.n; i += stride) xo[i] = E.x[i];
+    if (double* eo = out.energies[s])
+        for (int64_t i = i0; i <= done; i += stride) eo[i] = E.energies[i];
+    if (int* d = out.iters_done[s])
+        if (i0 == 0) *d = E.gbar->err ? -1 : done;
+    if (NodeXf* table = reinterpret_cast<NodeXf*>(out.table[s])) {
+        const SolverDev& S = E.S;
+        for (int64_t m = i0; m < S.M; m += stride) {
+            NodeXf r;
+            r.g[0] = S.g[3 * m]; r.g[1] = S.g[3 * m + 1]; r.g[2] = S.g[3 * m + 2];
+            const int f = S.free_id[m];
+            if (f >= 0) {
+                const float* xf = E.x + 12 * (int64_t)f;
+#pragma unroll
+                for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) r.R[rr * 3 + c] = xf[3 * c + rr];  // x is column-major
+                r.t[0] = xf[9]; r.t[1] = xf[10]; r.t[2] = xf[11];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; i++) r.R[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+                r.t[0] = r.t[1] = r.t[2] = 0.0f;
+            }
+            r.pad = 0.0f;
+            table[m] = r;
+        }
+    }
 }
 
 }  // namespace
@@ -1394,84 +1586,172 @@ __global__ void cg_begin_kernel(const int* __restrict__ gn_done, int* __restrict
 // ---------------------------------------------------------------- host driver
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    hipLaunchKernelGGL(init_x_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.n, W.x, W.gs, W.gn_done,
-                       (const double*)nullptr);
+static SolverEntry entry_of(const SolverWork& W) {
+    SolverEntry E;
+    E.S = W.dev;
+    E.n = W.n;
+    E.nb_owner = (int)nblk(W.dev.nF, 256 / LANES);
+    E.e_blocks = W.e_blocks;
+    E.acc_blocks = W.acc_blocks;
+    E.n_cand = W.n_cand;
+    E.nt = W.dev.nc_pad / GJ_T;
+    E.cg_max_iters = W.cg_max_iters;
+    E.tol2 = W.cg_tol * W.cg_tol;
+    E.x = W.x; E.hsol = W.hsol; E.r = W.r; E.z = W.z; E.p = W.p; E.q = W.q; E.binv = W.binv; E.rc = W.rc;
+    E.acinv = W.acinv; E.gj_piv = W.gj_piv; E.gj_col = W.gj_col; E.hwarm = W.hwarm; E.ac64 = W.ac64;
+    E.partial = W.partial; E.partial2 = W.partial2; E.rzp = W.rzp; E.e_partial = W.e_partial; E.cand_e = W.cand_e;
+    E.acc_partial = W.acc_partial; E.energies = W.energies;
+    E.cg = W.cg; E.gs = W.gs; E.gn_done = W.gn_done; E.cg_done = W.cg_done;
+    E.gbar = reinterpret_cast<const GridBarrier*>(W.gbar);
+    return E;
+}
+
+void solver_batch_free(SolverBatch& B) {
+    if (B.entries) (void)hipFree(B.entries);
+    if (B.flag) (void)hipFree(B.flag);
+    B.entries = nullptr;
+    B.flag = nullptr;
+    B.w.clear();
+}
+
+int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::string& err) {
+    solver_batch_free(B);
+    if (w.empty() || (int)w.size() > SOLVER_BATCH_MAX) {
+        err = "solver batch: need 1.." + std::to_string(SOLVER_BATCH_MAX) + " solvers";
+        return -1;
+    }
+    B = SolverBatch();
+    B.w = w;
+    B.kt = 8;
+    std::vector<SolverEntry> host;
+    for (SolverWork* W : w) {
+        if (!W) { err = "solver batch: null solver"; return -1; }
+        host.push_back(entry_of(*W));
+        const SolverEntry& E = host.back();
+        if (W->dev.K != 8) B.kt = 0;
+        B.max_nb_owner = std::max(B.max_nb_owner, E.nb_owner);
+        B.max_agg = std::max(B.max_agg, E.S.n_agg);
+        B.max_e_blocks = std::max(B.max_e_blocks, E.e_blocks);
+        B.max_acc_blocks = std::max(B.max_acc_blocks, E.acc_blocks);
+        B.max_n = std::max(B.max_n, std::max(E.n, E.S.M));
+        B.max_groups128 = std::max(B.max_groups128, (int)nblk(E.S.n_groups, 128));
+        B.max_multi = std::max(B.max_multi, E.S.n_multi);
+        B.max_nt = std::max(B.max_nt, E.nt);
+        B.max_cand = std::max(B.max_cand, E.n_cand);
+        B.g.push_back(nullptr);  // the first set_frame always writes the entries' pointers
+        B.aim.push_back(nullptr);
+    }
+    const size_t bytes = sizeof(SolverEntry) * host.size();
+    if (hipMalloc(&B.entries, bytes) != hipSuccess || hipMalloc(&B.flag, sizeof(int)) != hipSuccess ||
+        hipMemcpy(B.entries, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        err = "solver batch: device allocation failed";
+        solver_batch_free(B);
+        return -2;
+    }
+    return 0;
+}
+
+int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* const* aim, hipStream_t st) {
+    const int n = (int)B.w.size();
+    bool changed = false;
+    FramePtrs f;
+    for (int s = 0; s < n; s++) {
+        changed |= g[s] != B.g[s] || aim[s] != B.aim[s];
+        f.g[s] = g[s];
+        f.aim[s] = aim[s];
+        // the persistent PCG kernel takes the scene's SolverDev by value
+        B.w[s]->dev.g = g[s];
+        B.w[s]->dev.aim = aim[s];
+    }
+    if (!changed) return 0;
+    hipLaunchKernelGGL(set_frame_kernel, dim3(1), dim3(SOLVER_BATCH_MAX), 0, st, B.entries, n, f);
+    HIP_RET(hipGetLastError());
+    for (int s = 0; s < n; s++) {
+        B.g[s] = g[s];
+        B.aim[s] = aim[s];
+    }
+    return 0;
+}
+
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, const int** dflag) {
+    if (B.w.size() == 1) {
+        *dflag = which ? B.w[0]->gn_done : B.w[0]->cg_done;
+        return 0;
+    }
+    hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
+                       which, B.flag);
+    HIP_RET(hipGetLastError());
+    *dflag = B.flag;
+    return 0;
+}
+
+int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st) {
+    const unsigned nz = (unsigned)B.w.size();
+    hipLaunchKernelGGL(finish_kernel, dim3(std::max(1u, std::min(64u, nblk(B.max_n, 256))), 1, nz), dim3(256), 0, st,
+                       (const SolverEntry*)B.entries, out);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
+    const SolverEntry* E = B.entries;
+    const unsigned nz = (unsigned)B.w.size();
+    hipLaunchKernelGGL(init_x_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
     if (refresh_coarse) {
         // Galerkin coarse operator at x0 = identity and its inverse; it is only a
         // preconditioner, so callers may keep it across frames.
-        hipLaunchKernelGGL(agg_center_kernel, dim3(S.n_agg), dim3(256), 0, st, S);
-        const int nc = 12 * S.n_agg;
-        const int64_t n2 = (int64_t)S.nc_pad * S.nc_pad;
-        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, reinterpret_cast<float*>(W.ac64), 2 * n2,
-                           (const int*)nullptr);
-        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(nblk(S.n_groups, 128)), dim3(128), 0, st, S, W.x, W.ac64);
-        if (S.n_multi > 0)
-            hipLaunchKernelGGL(coarse_multi_kernel, dim3(S.n_multi), dim3(256), (size_t)24 * S.n_agg, st, S, W.ac64);
-        hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256), dim3(256), 0, st, (const unsigned long long*)W.ac64,
-                           W.acinv, n2);
-        const int np = S.nc_pad;
-        hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(64), 0, st, W.acinv, np, nc);
-        const int nt = np / GJ_T;
-        for (int kb = 0; kb < nt; kb++) {
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv);
-            hipLaunchKernelGGL(gj_panel_kernel, dim3(nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
-            hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(256), 0, st, W.acinv, np, kb, W.gj_piv, W.gj_col);
+        hipLaunchKernelGGL(agg_center_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E);
+        hipLaunchKernelGGL(zero_coarse_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
+        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(B.max_groups128, 1, nz), dim3(128), 0, st, E);
+        if (B.max_multi > 0)
+            hipLaunchKernelGGL(coarse_multi_kernel, dim3(B.max_multi, 1, nz), dim3(256), (size_t)24 * B.max_agg, st, E);
+        hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
+        hipLaunchKernelGGL(pad_identity_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
+        const unsigned nt = (unsigned)B.max_nt;
+        for (int kb = 0; kb < (int)nt; kb++) {
+            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, kb);
+            hipLaunchKernelGGL(gj_panel_kernel, dim3(nt, 1, nz), dim3(256), 0, st, E, kb);
+            hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nz), dim3(256), 0, st, E, kb);
         }
     }
     // E(x0)
-    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, (const float*)nullptr,
-                       W.e_partial, (const int*)nullptr, 0, (const double*)nullptr, (const GnScalars*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, W.cand_e,
-                       (const int*)nullptr, (const double*)nullptr, (const GnScalars*)nullptr);
-    hipLaunchKernelGGL(set_eold_kernel, dim3(1), dim3(1), 0, st, W.gs, W.cand_e);
+    hipLaunchKernelGGL(energy_kernel, dim3(B.max_e_blocks, 1, nz), dim3(256), 0, st, E, 0, 0, 0, 0,
+                       (const float*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, 0, 1, 0, 0, (double*)nullptr);
+    hipLaunchKernelGGL(set_eold_kernel, dim3(1, 1, nz), dim3(1), 0, st, E);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int solver_gn_begin(SolverWork& W, int iter, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    const int nb_owner = nblk(S.nF, 256 / LANES);
-    hipLaunchKernelGGL(cg_begin_kernel, dim3(1), dim3(1), 0, st, W.gn_done, W.cg_done, W.cg, W.cg_max_iters,
-                       W.cg_tol * W.cg_tol);
-    hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial,
-                       W.cg_done);
-    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(64), 0, st, W.partial, nb_owner, W.cg, W.gs, W.cg_done);
-    if (W.warm && iter < W.warm_slots) {
+int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st) {
+    const SolverEntry* E = B.entries;
+    const unsigned nz = (unsigned)B.w.size();
+    hipLaunchKernelGGL(cg_begin_kernel, dim3(1, 1, nz), dim3(1), 0, st, E);
+    hipLaunchKernelGGL(rhs_diag_kernel, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
+    if (B.w[0]->warm && iter < B.w[0]->warm_slots) {
         // q = A h0 (the first-iteration form of the apply), then h = h0, r = b - A h0
-        const float* h0 = W.hwarm + (int64_t)iter * W.n;
-        auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
-        hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, h0, (const double*)W.rzp,
-                           (const double*)W.rzp, 1, W.p, W.q, W.partial, W.cg_done);
-        hipLaunchKernelGGL(warm_init_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, (int64_t)W.n, W.hsol, h0, W.r,
-                           W.q, W.cg_done, W.gn_done);
+        auto apply = B.kt == 8 ? apply_kernel<8> : apply_kernel<0>;
+        hipLaunchKernelGGL(apply, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E, 0, 1, iter, (const float*)nullptr);
+        hipLaunchKernelGGL(warm_init_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E, iter);
     } else {
-        // h = 0 also when the CG is skipped for a negligible gradient (the line search then takes a zero step)
-        hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, W.hsol, (int64_t)W.n, W.gn_done);
+        hipLaunchKernelGGL(zero_step_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
     }
     // z0 = M^-1 r0, rz_0 -> rz[0]
-    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 0, (const double*)nullptr, 0,
-                       (const double*)nullptr, W.hsol, W.r, (const float*)nullptr, (const float*)nullptr, W.binv, W.z,
-                       W.rc, W.partial2, W.cg_done);
-    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z, W.rzp,
-                       W.partial2, W.cg, 0, W.cg_done);
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, 0, 0);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, 0, 0);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int solver_cg_iteration(SolverWork& W, int k, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    const int nb_owner = nblk(S.nF, 256 / LANES);
-    double* rz_cur = W.rzp + (k & 1) * S.n_agg;
-    double* rz_prev = W.rzp + ((k + 1) & 1) * S.n_agg;
-    auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
-    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.z, rz_cur, rz_prev,
-                       k == 0 ? 1 : 0, W.p, W.q, W.partial, W.cg_done);
-    hipLaunchKernelGGL(update_restrict_kernel, dim3(S.n_agg), dim3(256), 0, st, S, 1, W.partial, nb_owner, rz_cur,
-                       W.hsol, W.r, W.p, W.q, W.binv, W.z, W.rc, W.partial2, W.cg_done);
-    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(S.n_agg), dim3(256), 0, st, S, W.acinv, W.rc, W.r, W.z, rz_prev,
-                       W.partial2, W.cg, 1, W.cg_done);
+int solver_cg_iteration(SolverBatch& B, int k, hipStream_t st) {
+    const SolverEntry* E = B.entries;
+    const unsigned nz = (unsigned)B.w.size();
+    auto apply = B.kt == 8 ? apply_kernel<8> : apply_kernel<0>;
+    hipLaunchKernelGGL(apply, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E, k, k == 0 ? 1 : 0, -1,
+                       (const float*)nullptr);
+    hipLaunchKernelGGL(update_restrict_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, 1, k);
+    hipLaunchKernelGGL(coarse_prolong_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, k, 1);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -1509,38 +1789,35 @@ int solver_barrier_error(SolverWork& W, int* err) {
     return 0;
 }
 
-int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    if (W.warm && iter < W.warm_slots)
-        hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.hwarm + (int64_t)iter * W.n,
-                           (const float*)W.hsol, (int64_t)W.n, (const int*)W.gn_done);
+int solver_gn_end(SolverBatch& B, int iter, int stop_rule, hipStream_t st) {
+    const SolverEntry* E = B.entries;
+    const unsigned nz = (unsigned)B.w.size();
+    if (B.w[0]->warm && iter < B.w[0]->warm_slots)
+        hipLaunchKernelGGL(save_warm_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E, iter);
     // the full step first; the 49 halvings only if it was rejected
-    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial, W.gn_done,
-                       0, (const double*)nullptr, (const GnScalars*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, W.cand_e,
-                       W.gn_done, (const double*)nullptr, (const GnScalars*)nullptr);
-    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, W.n_cand - 1), dim3(256), 0, st, S, W.x, W.hsol, W.e_partial,
-                       W.gn_done, 1, (const double*)W.cand_e, (const GnScalars*)W.gs);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 1, W.n_cand - 1,
-                       W.cand_e, W.gn_done, (const double*)W.cand_e, (const GnScalars*)W.gs);
-    hipLaunchKernelGGL(accept_partial_kernel, dim3(W.acc_blocks), dim3(256), 0, st, W.n, W.x, W.hsol, W.cand_e,
-                       W.n_cand, W.gs, W.acc_partial, W.gn_done);
-    hipLaunchKernelGGL(accept_final_kernel, dim3(1), dim3(256), 0, st, W.acc_blocks, W.acc_partial, W.cand_e,
-                       W.n_cand, W.gs, (const CgScalars*)W.cg, energies, iter, stop_rule, W.gn_done);
+    hipLaunchKernelGGL(energy_kernel, dim3(B.max_e_blocks, 1, nz), dim3(256), 0, st, E, 1, 1, 0, 0,
+                       (const float*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, 0, 1, 1, 0, (double*)nullptr);
+    hipLaunchKernelGGL(energy_kernel, dim3(B.max_e_blocks, B.max_cand - 1, nz), dim3(256), 0, st, E, 1, 1, 1, 1,
+                       (const float*)nullptr);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, 1, B.max_cand - 1, 1, 1,
+                       (double*)nullptr);
+    hipLaunchKernelGGL(accept_partial_kernel, dim3(B.max_acc_blocks, 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(accept_final_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, iter, stop_rule);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int solver_apply_test(SolverWork& W, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    const int nb_owner = nblk(S.nF, 256 / LANES);
+int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st) {
+    SolverWork& W = *B.w[0];
+    const SolverEntry* E = B.entries;
+    const unsigned nb_owner = nblk(W.dev.nF, 256 / LANES);
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.x, x, (int64_t)W.n, (const int*)nullptr);
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
                        (const int*)nullptr);
-    hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, S, W.x, W.r, W.binv, W.partial, W.cg_done);
-    auto apply = S.K == 8 ? apply_kernel<8> : apply_kernel<0>;
-    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, S, W.x, v, (const double*)W.rzp, (const double*)W.rzp,
-                       1, W.p, W.q, W.partial, W.cg_done);
+    hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, E);
+    auto apply = W.dev.K == 8 ? apply_kernel<8> : apply_kernel<0>;
+    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, E, 0, 1, -1, v);
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, q_out, (const float*)W.q, (int64_t)W.n,
                        (const int*)nullptr);
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, b_out, (const float*)W.r, (int64_t)W.n,
@@ -1549,12 +1826,10 @@ int solver_apply_test(SolverWork& W, const float* x, const float* v, float* q_ou
     return 0;
 }
 
-int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st) {
-    const SolverDev& S = W.dev;
-    hipLaunchKernelGGL(energy_kernel, dim3(W.e_blocks, 1), dim3(256), 0, st, S, x, (const float*)nullptr, W.e_partial,
-                       (const int*)nullptr, 0, (const double*)nullptr, (const GnScalars*)nullptr);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, W.e_partial, W.e_blocks, 0, 1, out,
-                       (const int*)nullptr, (const double*)nullptr, (const GnScalars*)nullptr);
+int solver_energy(SolverBatch& B, const float* x, double* out, hipStream_t st) {
+    const SolverEntry* E = B.entries;
+    hipLaunchKernelGGL(energy_kernel, dim3(B.max_e_blocks, 1, 1), dim3(256), 0, st, E, 0, 0, 0, 0, x);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1, 1, 1), dim3(256), 0, st, E, 0, 1, 0, 0, out);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -69,6 +69,42 @@ struct GridBarrier {
     int pad;
 };
 
+// Largest Gauss-Newton iteration count of one solve (energies are kept for
+// iterations 0..GN_MAX_ITERS on the device).
+constexpr int GN_MAX_ITERS = 64;
+
+// One scene of a solver launch: everything the solver kernels read or write,
+// in device memory. Every solver kernel takes an array of entries and serves
+// entry blockIdx.z, so a batch of independent scenes runs each phase of the
+// solve in one launch; a single solve is a batch of one. Grid sizes are per
+// entry (blocks past an entry's count return at once), so each scene's blocks
+// do exactly the work of a launch for that scene alone.
+struct SolverEntry {
+    SolverDev S;
+    int n, nb_owner, e_blocks, acc_blocks, n_cand, nt;  // nt = nc_pad / 64 coarse tiles
+    int cg_max_iters;
+    double tol2;
+    float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col, *hwarm;
+    unsigned long long* ac64;
+    double *partial, *partial2, *rzp, *e_partial, *cand_e, *acc_partial;
+    double* energies;  // GN_MAX_ITERS + 1: E(x) before and after every Gauss-Newton iteration
+    CgScalars* cg;
+    GnScalars* gs;
+    int* gn_done;
+    int* cg_done;
+    const GridBarrier* gbar;
+};
+
+constexpr int SOLVER_BATCH_MAX = 64;
+
+// Per-scene outputs of a batched solve (kernel argument, by value).
+struct SolverOutPtrs {
+    float* x[SOLVER_BATCH_MAX];
+    double* energies[SOLVER_BATCH_MAX];
+    int* iters_done[SOLVER_BATCH_MAX];
+    void* table[SOLVER_BATCH_MAX];  // NodeXf rows of the scene's nodes (putFreeInputs), or null
+};
+
 struct SolverWork {
     SolverDev dev;
     int n;  // 12 * nF
@@ -91,8 +127,36 @@ struct SolverWork {
     double cg_tol;
     void* gbar;        // grid barrier of the persistent PCG kernel
     int coop_grid;     // its grid size (0: not yet chosen)
+    double* energies;  // GN_MAX_ITERS + 1
     std::vector<void*> allocs;  // owned device buffers
 };
+
+// A launch group of solvers (one entry per scene, in device memory). The
+// per-frame node positions and aims live in the entries and are rewritten by
+// a small kernel only when a caller passes different pointers.
+struct SolverBatch {
+    std::vector<SolverWork*> w;
+    SolverEntry* entries = nullptr;   // device, w.size()
+    int* flag = nullptr;              // device: AND of the scenes' cg_done / gn_done
+    std::vector<const float*> g, aim; // what the device entries hold
+    int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
+    int max_nb_owner = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
+        max_multi = 0, max_nt = 0, max_cand = 0;
+};
+
+int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::string& err);
+void solver_batch_free(SolverBatch& B);
+// Points the entries at this frame's node positions and aims (one launch when any changed).
+int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* const* aim, hipStream_t st);
+// Device flag (1 int) = AND over the scenes of cg_done (which 0) or gn_done (which 1).
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, const int** dflag);
+int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st);
+// The warm-start branch is taken from the first scene; a batch needs them all alike.
+inline bool solver_batch_warm_uniform(const SolverBatch& B) {
+    for (const SolverWork* W : B.w)
+        if ((W->warm != 0) != (B.w[0]->warm != 0)) return false;
+    return true;
+}
 
 struct SolverDesc {
     int M, K;
@@ -113,17 +177,18 @@ struct SolverDesc {
 int solver_build(const SolverDesc& d, SolverWork& W, std::string& err);
 void solver_free(SolverWork& W);
 
-int solver_frame_setup(SolverWork& W, bool refresh_coarse, hipStream_t st);
-int solver_gn_begin(SolverWork& W, int iter, hipStream_t st);
-int solver_cg_iteration(SolverWork& W, int k, hipStream_t st);
+int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st);
+int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st);
+int solver_cg_iteration(SolverBatch& B, int k, hipStream_t st);
 // The whole PCG solve of one GN iteration as one cooperative launch (after
 // solver_gn_begin); same arithmetic and stopping rule as solver_cg_iteration.
 int solver_cg_persistent(SolverWork& W, hipStream_t st);
 // Reads the persistent kernel's barrier error flag (synchronous).
 int solver_barrier_error(SolverWork& W, int* err);
-int solver_gn_end(SolverWork& W, int iter, int stop_rule, double* energies, hipStream_t st);
-int solver_energy(SolverWork& W, const float* x, double* out, hipStream_t st);
+int solver_gn_end(SolverBatch& B, int iter, int stop_rule, hipStream_t st);
+// Single-scene helpers (B holds one solver).
+int solver_energy(SolverBatch& B, const float* x, double* out, hipStream_t st);
 // Test hook: q = J^T J v and b = -J^T f at x (the CG operator and right-hand side).
-int solver_apply_test(SolverWork& W, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st);
+int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_out, float* b_out, hipStream_t st);
 
 }  // namespace arapgs

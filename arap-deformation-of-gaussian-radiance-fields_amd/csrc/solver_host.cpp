@@ -249,6 +249,7 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.gn_done = alloc<int>(W, 1, rc);
     W.cg_done = alloc<int>(W, 1, rc);
     W.gbar = alloc<int>(W, 4, rc);
+    W.energies = alloc<double>(W, GN_MAX_ITERS + 1, rc);
     W.coop_grid = 0;
     W.cg_max_iters = 100;
     W.cg_tol = 1e-4;

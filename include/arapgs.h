@@ -164,6 +164,31 @@ int arapgs_gn_energy(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, c
 int arapgs_put_free_inputs(arapgs_ctx* ctx, arapgs_solver* s, const float* node_pos, const float* x,
                            arapgs_node_xf* table);
 
+/* ------------------------------------------------------------ scene batches
+ * Independent scenes of uneven size solved together: every phase of the
+ * Gauss-Newton solve (frame set-up, right-hand side, each CG phase, line
+ * search, step acceptance) is ONE launch for all scenes of the batch, scene =
+ * blockIdx.z, each scene's blocks doing exactly the work of a solve of that
+ * scene alone. No counterpart in the reference (it edits one scene); per scene
+ * the result is bit-identical to arapgs_gn_solve + arapgs_put_free_inputs on
+ * the same solver. A batch holds 1..64 solvers created on ctx's device, each
+ * at most once, sharing their state (warm-start steps, coarse operator) with
+ * single solves on them; all need the same warm-start setting. Host
+ * convergence checks default to every 8 CG iterations (on the AND of the
+ * scenes' flags), the coarse refresh to 8 batch solves. */
+typedef struct arapgs_solver_batch arapgs_solver_batch;
+int arapgs_solver_batch_create(arapgs_ctx* ctx, arapgs_solver* const* solvers_h, int n, arapgs_solver_batch** out);
+void arapgs_solver_batch_destroy(arapgs_solver_batch* b);
+int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every);
+int arapgs_solver_batch_set_coarse_refresh(arapgs_solver_batch* b, int frames);
+/* Host arrays of n device pointers, one per scene in batch order: node_pos and
+ * aims f32[M_s*3] (required); x_out f32[12*n_free_s], energies f64[iters+1],
+ * iters_done int[1] and tables (putFreeInputs node table, M_s rows) are
+ * optional (a null array or a null entry skips that output). */
+int arapgs_solver_batch_solve(arapgs_ctx* ctx, arapgs_solver_batch* b, const float* const* node_pos_h,
+                              const float* const* aims_h, int iters, int stop_rule, float* const* x_out_h,
+                              double* const* energies_h, int* const* iters_done_h, arapgs_node_xf* const* tables_h);
+
 /* ------------------------------------------------------------ deformation */
 /* DeformGraph::predict_mesh / predict_samples (Deform.hpp:230-268):
  * out[i] = sum_j w_j (R_j (p_i - g_j) + t_j + g_j); skip (u8[P], optional)
@@ -318,8 +343,19 @@ int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_re
  * views (combined tile id view * ntiles + tile) are sorted together and one
  * blend launch covers every tile of every view, so the deepest tiles of the
  * views overlap. Images are identical bit for bit to n_views separate
- * arapgs_render calls. n_views <= 16; *num_rendered_h = total instances. */
+ * arapgs_render calls. n_views <= 64; *num_rendered_h = total instances. */
 int arapgs_render_views(arapgs_ctx* ctx, const arapgs_render_desc* descs_h, int n_views, int* num_rendered_h);
+
+/* Independent scenes stored back to back, one view each, in one pass (no
+ * counterpart in the reference, which renders one scene): view v draws only
+ * Gaussians [seg_h[v], seg_h[v+1]) (host int64[n_views+1], ascending, from 0
+ * to P). The descs follow arapgs_render_views' rules; a view's surface, radii
+ * and rects are indexed within its segment. One record per Gaussian, keys of
+ * all views sorted together, one blend launch for every tile of every view;
+ * each image is bit-identical to arapgs_render of that segment alone. No
+ * backward. */
+int arapgs_render_segments(arapgs_ctx* ctx, const arapgs_render_desc* descs_h, int n_views, const int64_t* seg_h,
+                           int* num_rendered_h);
 
 /* Tuning of the 16x16 blend: how many Gaussians (2, 4 or 8; default 4) have
  * their alphas evaluated together before being composited in order. Images do
