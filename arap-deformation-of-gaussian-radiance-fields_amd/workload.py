@@ -69,3 +69,61 @@ def drag_delta(sc, drag_frames: int):
 
 def make_solver(ctx, solver_args, **kw):
     return arapgs.Solver(ctx, *solver_args, **kw)
+
+
+def batch_scenes(seed: int, n_scenes: int, n_min: int = 50_000, n_max: int = 2_000_000, W: int = 800,
+                 H: int = 800, node_div: int = 500):
+    """Scene list of the uneven-batch workload: Gaussian counts log-uniform in
+    [n_min, n_max], nodes = ceil(n / node_div) rounded up to a multiple of 64,
+    one capsule scene (seed + 1 + i) with one camera each. Yields scenes one at
+    a time so that only one scene's host arrays live at once."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_scenes)).astype(np.int64)
+    nodes = (np.ceil(counts / node_div).astype(np.int64) + 63) // 64 * 64
+    for i, (c, m) in enumerate(zip(counts, nodes)):
+        yield scenes.capsule_scene(seed + 1 + i, n=int(c), n_nodes=int(m), W=W, H=H, n_views=1)
+
+
+def prepare_batch(ctx, scene_iter, device="cuda:0", log=None):
+    """Independent scenes stored back to back: every per-Gaussian, per-node and
+    per-sample array of prepare() concatenated, node indices offset into the
+    concatenated node list (so one node table serves every scene). Returns
+    dict(state, seg, node_seg, sample_seg, solver_args (per scene), active (per
+    scene), cams (one per scene), bg, opac, K)."""
+    import torch
+    parts = []
+    for sc in scene_iter:
+        wl = prepare(ctx, sc, device)
+        st = wl["state"]
+        keep = {k: st[k] for k in ("pos", "rot", "scale", "sh", "scale0", "ends", "ends_idx", "ends_w", "node_pos",
+                                   "node_src_idx", "node_src_w", "samples", "sample_idx", "sample_w", "sample_feat")}
+        parts.append(dict(state=keep, opac=wl["opac"], solver_args=wl["solver_args"], active=wl["active"],
+                          cam=wl["cams"][0], K=wl["K"]))
+        if log:
+            log(f"batch scene {len(parts) - 1}: {sc.gaussians.n} Gaussians, {sc.n_nodes} nodes, "
+                f"{st['samples'].shape[0]} samples")
+        del wl
+    K = parts[0]["K"]
+    seg, node_seg, sample_seg = [0], [0], [0]
+    for p in parts:
+        seg.append(seg[-1] + p["state"]["pos"].shape[0])
+        node_seg.append(node_seg[-1] + p["state"]["node_pos"].shape[0])
+        sample_seg.append(sample_seg[-1] + p["state"]["samples"].shape[0])
+    state = {}
+    for key in parts[0]["state"]:
+        chunks = []
+        for i, p in enumerate(parts):
+            v = p["state"][key]
+            if key in ("ends_idx", "node_src_idx", "sample_idx"):
+                v = v + node_seg[i]
+            chunks.append(v)
+        state[key] = torch.cat(chunks).contiguous()
+        for p in parts:
+            p["state"][key] = None
+    state.update(gs_static=None, sample_static=None)
+    opac = torch.cat([p["opac"] for p in parts]).contiguous()
+    bg = torch.tensor([0.8, 0.8, 0.8], dtype=torch.float32, device=device)
+    torch.cuda.synchronize(torch.device(device))
+    return dict(state=state, seg=seg, node_seg=node_seg, sample_seg=sample_seg,
+                solver_args=[p["solver_args"] for p in parts], active=[p["active"] for p in parts],
+                cams=[p["cam"] for p in parts], bg=bg, opac=opac, K=K)

@@ -429,6 +429,59 @@ def realign_leg(B, args):
             "step_p50_ms": round(pct(ms, 50), 3), "l1_loss_first_last": [losses[0], losses[-1]]}
 
 
+def batch_leg(args):
+    """The uneven scene batch: args.batch_scenes capsule scenes with Gaussian
+    counts log-uniform in [50k, 2M], nodes = ceil(n / 500) rounded up to a
+    multiple of 64, K=8, 64^3 grids, one 800x800 view each, 10 GN iterations
+    with the stop rule, CG warm start; a seeded random-walk drag (a step of
+    0.01 in a new random direction per scene and frame). One frame = every
+    scene's aims, solve, node table, node / sample / end-point blends,
+    six-point update, sample SH and render. Timed twice: batched (one launch
+    per step over all scenes) and scene after scene with single-scene calls."""
+    import torch
+    load_package()
+    from arapgs_amd import arapgs, batch, workload
+    ctx = arapgs.Context(0, stream=torch.cuda.Stream())
+    n = args.batch_scenes
+    with torch.cuda.stream(ctx.stream):
+        wl = workload.prepare_batch(ctx, workload.batch_scenes(args.seed, n, W=args.width, H=args.height), log=log)
+        sb = batch.SceneBatch(ctx, wl, gn_iters=10)
+    seg, ns, ss = wl["seg"], wl["node_seg"], wl["sample_seg"]
+    counts = [seg[i + 1] - seg[i] for i in range(n)]
+    log(f"batch: {n} scenes, {seg[-1]} Gaussians ({min(counts)}..{max(counts)}), {ns[-1]} nodes, {ss[-1]} samples")
+    frames = args.batch_frames
+    steps = batch.random_walk_steps(args.seed + 2, n, 2 * (args.warmup + frames), step=0.01)
+    res = {}
+    k = 0
+    for form in ("batched", "looped"):
+        run = sb.frame if form == "batched" else sb.frame_looped
+        ms = []
+        for f in range(args.warmup + frames):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            with torch.cuda.stream(ctx.stream):
+                inst = run(steps[k])
+            torch.cuda.synchronize()
+            k += 1
+            if f >= args.warmup:
+                ms.append((time.perf_counter() - a) * 1e3)
+        total = sum(ms) / 1e3
+        res[form] = {"frames_per_s": round(frames / total, 3), "scene_frames_per_s": round(n * frames / total, 2),
+                     "gaussians_per_s": round(seg[-1] * frames / total, 1), "p50_ms": round(pct(ms, 50), 3),
+                     "p99_ms": round(pct(ms, 99), 3), "tile_instances_per_frame": int(inst)}
+        log(f"  batch {form}: {res[form]['frames_per_s']} frames/s ({res[form]['scene_frames_per_s']} scene-frames/s), "
+            f"p50 {res[form]['p50_ms']} ms, p99 {res[form]['p99_ms']} ms")
+    gn = [int(t.item()) for t in sb.iters_done]
+    sb.close()
+    ctx.close()
+    return {"workload": f"{n} uneven capsule scenes", "scenes": n, "gaussians_total": int(seg[-1]),
+            "gaussians_min_max": [int(min(counts)), int(max(counts))], "nodes_total": int(ns[-1]),
+            "samples_total": int(ss[-1]), "views": n, "resolution": [args.width, args.height], "gn_iters": 10,
+            "frames": frames, "last_frame_gn_iters_min_max": [min(gn), max(gn)], **res,
+            "speedup_batched_vs_looped": round(res["batched"]["frames_per_s"] / res["looped"]["frames_per_s"], 3),
+            "check": "batched frames = scene-by-scene frames bit for bit (tests/test_gpu_scene_batch.py)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,6 +507,8 @@ def main():
     ap.add_argument("--adjoint-n", type=int, default=500_000)
     ap.add_argument("--realign-steps", type=int, default=5, help="steps of the realignment leg (0: skip)")
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
+    ap.add_argument("--batch-scenes", type=int, default=32, help="scenes of the uneven-batch leg (0: skip)")
+    ap.add_argument("--batch-frames", type=int, default=8, help="timed frames per form of the uneven-batch leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -554,6 +609,9 @@ def main():
         line["adjoint"] = adjoint_leg(args)
         log(f"  adjoint: {line['adjoint']['steps_per_s']} steps/s, forward p50 {line['adjoint']['forward_p50_ms']} ms, "
             f"backward p50 {line['adjoint']['backward_p50_ms']} ms")
+    if args.batch_scenes > 0:
+        log(f"timing the uneven-batch leg: {args.batch_scenes} scenes")
+        line["batch"] = batch_leg(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line), flush=True)
