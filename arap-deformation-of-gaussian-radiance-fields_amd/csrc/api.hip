@@ -63,6 +63,7 @@ struct arapgs_ctx {
     RasterView* r_views_h = nullptr;  // pinned host staging for it
     int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
     int* r_seg_h = nullptr;           // pinned host staging for it
+    int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (4 ints)
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
@@ -156,6 +157,7 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMalloc(&c->r_seg, sizeof(int) * (RASTER_MAX_VIEWS + 1))) != hipSuccess ||
         (e = hipHostMalloc(&c->r_seg_h, sizeof(int) * (RASTER_MAX_VIEWS + 1), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->flag_h, sizeof(int) * 4, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
         arapgs_destroy(c);
         return -5;
@@ -179,6 +181,7 @@ void arapgs_destroy(arapgs_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
     if (c->r_seg_h) (void)hipHostFree(c->r_seg_h);
+    if (c->flag_h) (void)hipHostFree(c->flag_h);
     delete c;
 }
 
@@ -420,13 +423,14 @@ namespace {
 int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, int iters, int stop_rule) {
     int rc = check_launch(c, solver_frame_setup(B, refresh, c->stream), "gn_solve setup");
     if (rc) return rc;
+    // the flag kernel writes into pinned host memory: no device-to-host copy
     auto host_flag = [&](int which, int* out) -> int {
-        const int* dflag = nullptr;
-        int r = check_launch(c, solver_batch_flag(B, which, c->stream, &dflag), "gn_solve");
+        int r = check_launch(c, solver_batch_flag(B, which, c->stream, c->flag_h), "gn_solve");
         if (r) return r;
-        hipError_t e = hipMemcpyAsync(out, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        return e == hipSuccess ? 0 : hip_fail(c, e, "gn_solve");
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
+        *out = *(volatile int*)c->flag_h;
+        return 0;
     };
     for (int it = 0; it < iters; it++) {
         rc = check_launch(c, solver_gn_begin(B, it, c->stream), "gn_solve rhs");
@@ -642,6 +646,13 @@ int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const 
 
 }  // extern "C"
 
+namespace {
+// The preprocess's counts (instances, depth-bit range) into pinned host memory.
+__global__ void counts_to_host_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+    if (threadIdx.x < 3) dst[threadIdx.x] = src[threadIdx.x];
+}
+}  // namespace
+
 extern "C" {
 
 // seg_h (segment mode, host int64[V+1] or null): view v draws Gaussians
@@ -723,9 +734,12 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
                       "render preprocess");
     if (rc) return rc;
     uint32_t cnt[3] = {0, 0, 0};
-    e = hipMemcpyAsync(cnt, c->r_total, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    uint32_t* cnt_h = reinterpret_cast<uint32_t*>(c->flag_h);
+    hipLaunchKernelGGL(counts_to_host_kernel, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)c->r_total, cnt_h);
+    e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "render");
+    for (int i = 0; i < 3; i++) cnt[i] = ((volatile uint32_t*)cnt_h)[i];
     const uint32_t n = cnt[0];
     const uint32_t dmin = n > 0 ? cnt[1] : 0u;
     int dbits = 1;

@@ -421,39 +421,6 @@ __device__ __forceinline__ double group_reduce16d(double v) {
     return v;
 }
 
-// Cholesky-based inverse of a 12x12 SPD block (row-major), in place.
-__device__ void invert12(float A[144]) {
-    float L[12][12];
-    for (int j = 0; j < 12; j++) {
-        float d = A[j * 12 + j];
-        for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
-        d = d > 1e-30f ? sqrtf(d) : 1e-15f;
-        L[j][j] = d;
-        for (int i = j + 1; i < 12; i++) {
-            float s = A[i * 12 + j];
-            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
-            L[i][j] = s / d;
-        }
-        for (int i = 0; i < j; i++) L[i][j] = 0.0f;
-    }
-    // inv = L^-T L^-1, column by column
-    for (int col = 0; col < 12; col++) {
-        float y[12];
-        for (int i = 0; i < 12; i++) {
-            float s = (i == col) ? 1.0f : 0.0f;
-            for (int k = 0; k < i; k++) s -= L[i][k] * y[k];
-            y[i] = s / L[i][i];
-        }
-        for (int i = 11; i >= 0; i--) {
-            float s = y[i];
-            for (int k = i + 1; k < 12; k++) s -= L[k][i] * y[k];
-            y[i] = s / L[i][i];
-        }
-        for (int i = 0; i < 12; i++) A[i * 12 + col] = y[i];
-    }
-}
-
-// b = -J^T f, diagonal block D_i = sum J_i^T J_i and its inverse; |b|^2 partials.
 // b = -J^T f, diagonal block D_i = sum J_i^T J_i and its inverse; |b|^2 partials.
 // D_i = (the rotation group's 9x9 block, lane 0 only) + the I_3-structured sum
 // of the 3-row groups, C[p][q] = sum a'_p a'_q with a' = (a_0, a_1, a_2, b),
@@ -542,8 +509,9 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
 #pragma unroll
     for (int a = 0; a < 10; a++) C[a] = group_reduce16(C[a]);
     double nb = 0.0;
-    if (fi < S.nF && lane == 0) {
-        float* Dm = sD[grp];
+    const bool own = fi < S.nF;
+    float* Dm = sD[grp];
+    if (own && lane == 0) {
         for (int e = 0; e < 144; e++) Dm[e] = 0.0f;
         int o = 0;
         for (int a = 0; a < 9; a++)
@@ -568,8 +536,31 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
             b[12 * (int64_t)fi + a] = ba;
             nb += (double)ba * ba;
         }
-        invert12(sD[grp]);
-        for (int a = 0; a < 144; a++) binv[144 * (int64_t)fi + a] = sD[grp][a];
+    }
+    __syncthreads();
+    if (own) {
+        // D^-1 by Gauss-Jordan on the 16 lanes of the owner's group: lane c < 12
+        // holds column c; the pivot column is read from lane k by a shuffle.
+        // D is symmetric positive definite (a pivot below 1e-30 is floored).
+        float a[12];
+#pragma unroll
+        for (int r = 0; r < 12; r++) a[r] = lane < 12 ? Dm[r * 12 + lane] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            const float d = __shfl(a[k], k, LANES);
+            const float iv = 1.0f / fmaxf(d, 1e-30f);
+            const float rk = a[k] * iv;
+#pragma unroll
+            for (int r = 0; r < 12; r++) {
+                const float colv = __shfl(a[r], k, LANES);
+                if (lane == k) a[r] = (r == k) ? iv : -colv * iv;
+                else a[r] = (r == k) ? rk : a[r] - colv * rk;
+            }
+        }
+        if (lane < 12) {
+#pragma unroll
+            for (int r = 0; r < 12; r++) binv[144 * (int64_t)fi + r * 12 + lane] = a[r];
+        }
     }
     const double t = block_reduce_d(nb, shd);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -1535,6 +1526,8 @@ __global__ void set_frame_kernel(SolverEntry* __restrict__ B, int n, FramePtrs f
 }
 
 // flag = 1 when every scene's cg_done (which 0) or gn_done (which 1) is set.
+// flag is pinned host memory: the host reads it once the stream reaches this
+// point, without a device-to-host copy.
 __global__ void batch_flag_kernel(const SolverEntry* __restrict__ B, int n, int which, int* __restrict__ flag) {
     if (threadIdx.x != 0) return;
     int all = 1;
@@ -1608,9 +1601,7 @@ static SolverEntry entry_of(const SolverWork& W) {
 
 void solver_batch_free(SolverBatch& B) {
     if (B.entries) (void)hipFree(B.entries);
-    if (B.flag) (void)hipFree(B.flag);
     B.entries = nullptr;
-    B.flag = nullptr;
     B.w.clear();
 }
 
@@ -1642,7 +1633,7 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
         B.aim.push_back(nullptr);
     }
     const size_t bytes = sizeof(SolverEntry) * host.size();
-    if (hipMalloc(&B.entries, bytes) != hipSuccess || hipMalloc(&B.flag, sizeof(int)) != hipSuccess ||
+    if (hipMalloc(&B.entries, bytes) != hipSuccess ||
         hipMemcpy(B.entries, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
         err = "solver batch: device allocation failed";
         solver_batch_free(B);
@@ -1673,15 +1664,10 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
     return 0;
 }
 
-int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, const int** dflag) {
-    if (B.w.size() == 1) {
-        *dflag = which ? B.w[0]->gn_done : B.w[0]->cg_done;
-        return 0;
-    }
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h) {
     hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
-                       which, B.flag);
+                       which, flag_h);
     HIP_RET(hipGetLastError());
-    *dflag = B.flag;
     return 0;
 }
 

@@ -137,7 +137,6 @@ struct SolverWork {
 struct SolverBatch {
     std::vector<SolverWork*> w;
     SolverEntry* entries = nullptr;   // device, w.size()
-    int* flag = nullptr;              // device: AND of the scenes' cg_done / gn_done
     std::vector<const float*> g, aim; // what the device entries hold
     int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
     int max_nb_owner = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
@@ -148,8 +147,9 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
 void solver_batch_free(SolverBatch& B);
 // Points the entries at this frame's node positions and aims (one launch when any changed).
 int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* const* aim, hipStream_t st);
-// Device flag (1 int) = AND over the scenes of cg_done (which 0) or gn_done (which 1).
-int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, const int** dflag);
+// *flag_h (pinned host memory) = AND over the scenes of cg_done (which 0) or
+// gn_done (which 1), written by a kernel queued on st.
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h);
 int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st);
 // The warm-start branch is taken from the first scene; a batch needs them all alike.
 inline bool solver_batch_warm_uniform(const SolverBatch& B) {
