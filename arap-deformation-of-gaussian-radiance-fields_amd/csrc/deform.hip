@@ -166,14 +166,70 @@ __global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* 
     out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
 }
 
+// 48-float rows (SH coefficients / sample features) of a block's 256
+// consecutive items moved between global memory and the owning threads'
+// registers through LDS, in passes of as many rows as `lds` holds. The global
+// side is whole contiguous float4 runs (a wave's access covers 1 KB), so every
+// cache line is fetched once; a lane touches LDS only for its own row (odd row
+// stride: the 64 lanes of a wave hit distinct banks). Every thread of the block
+// must call these (they synchronise); rows [0, n_rows) exist.
+constexpr int ROW48_STRIDE = 49;
+
+__device__ __forceinline__ void rows48_load(const float* __restrict__ src, int n_rows, float* lds, int lds_floats,
+                                            float (&c)[48]) {
+    const int t = threadIdx.x;
+    const int per = lds_floats / ROW48_STRIDE;
+    for (int r0 = 0; r0 < n_rows; r0 += per) {
+        const int nr = min(per, n_rows - r0);
+        const float4* s4 = reinterpret_cast<const float4*>(src + (int64_t)r0 * 48);
+        for (int q = t; q < nr * 12; q += blockDim.x) {
+            const float4 v = s4[q];
+            float* d = lds + (q / 12) * ROW48_STRIDE + (q % 12) * 4;
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+        __syncthreads();
+        if (t >= r0 && t < r0 + nr) {
+            const float* d = lds + (t - r0) * ROW48_STRIDE;
+#pragma unroll
+            for (int k = 0; k < 48; k++) c[k] = d[k];
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void rows48_store(float* __restrict__ dst, int n_rows, float* lds, int lds_floats,
+                                             const float (&c)[48]) {
+    const int t = threadIdx.x;
+    const int per = lds_floats / ROW48_STRIDE;
+    for (int r0 = 0; r0 < n_rows; r0 += per) {
+        const int nr = min(per, n_rows - r0);
+        if (t >= r0 && t < r0 + nr) {
+            float* d = lds + (t - r0) * ROW48_STRIDE;
+#pragma unroll
+            for (int k = 0; k < 48; k++) d[k] = c[k];
+        }
+        __syncthreads();
+        float4* d4 = reinterpret_cast<float4*>(dst + (int64_t)r0 * 48);
+        for (int q = t; q < nr * 12; q += blockDim.x) {
+            const float* d = lds + (q / 12) * ROW48_STRIDE + (q % 12) * 4;
+            d4[q] = make_float4(d[0], d[1], d[2], d[3]);
+        }
+        __syncthreads();
+    }
+}
+
 // The fused per-Gaussian update, one workgroup per 256 Gaussians in two
 // phases. Phase 1 is point-parallel: each thread blends 6 of the block's 1536
 // end points, so consecutive lanes read consecutive neighbour rows (32 B each
 // at K = 8) and the rows are consumed right after they arrive; blended points
 // go to LDS and back to `ends`. Phase 2 is Gaussian-parallel: polar factor,
-// quaternion, scale, centre and the SH rotation of one Gaussian per thread.
+// quaternion, scale, centre and the SH rotation of one Gaussian per thread;
+// the SH rows then move through the same LDS in coalesced passes (loaded after
+// the polar so the 48 coefficients are not live across it: 113 VGPRs, four
+// waves per SIMD; 327 instead of 356 us at 1M Gaussians).
 // Static Gaussians (every end-point neighbour a static node) still blend their
-// end points, as the reference does, but keep their attributes.
+// end points, as the reference does, but keep their attributes (their SH rows
+// are written back unchanged).
 template <int K, bool VEC>
 __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf* __restrict__ table,
                                                         const uint32_t* __restrict__ ends_idx,
@@ -206,13 +262,16 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
     }
     __syncthreads();
     const int64_t g = g0 + threadIdx.x;
-    if (g >= N) return;
-    if (is_static && is_static[g]) return;
+    const int n_here = (int)min((int64_t)256, N - g0);
+    const bool active = g < N && !(is_static && is_static[g]);
     float e[6][3];
 #pragma unroll
     for (int p = 0; p < 6; p++)
 #pragma unroll
         for (int c = 0; c < 3; c++) e[p][c] = s_e[threadIdx.x * 18 + p * 3 + c];
+    __syncthreads();  // s_e is reused for the SH rows
+    float Rsh[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    if (active) {
 
     float center[3];
 #pragma unroll
@@ -253,7 +312,15 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
     quat_normalize(qr);
     float Rr[3][3];
     quat_to_mat(qr[0], qr[1], qr[2], qr[3], Rr);
-    sh_rotate_f4(*T, Rr, sh + g * 48);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Rsh[i][j] = Rr[i][j];
+    }
+    float shc[48];
+    rows48_load(sh + g0 * 48, n_here, s_e, 256 * 18, shc);
+    if (active) sh_rotate_regs(*T, Rsh, shc);
+    rows48_store(sh + g0 * 48, n_here, s_e, 256 * 18, shc);
 }
 
 // Node rotation -> orthogonal polar factor -> unit quaternion (w,x,y,z).
@@ -276,13 +343,15 @@ __global__ void node_quats_kernel(int M, const NodeXf* __restrict__ table, float
 
 // Running slerp from identity towards each neighbour node's quaternion with
 // t = w_j / (w_j + sum of previous weights) (Q_SlerpCUDA, cudakdtree.cu:113-148),
-// then SH rotation of the sample's 48-float feature. (Staging the features
-// through LDS for coalesced rows measured no faster: the kernel is bound by
-// the slerp's transcendentals and the node gathers, not by the row loads.)
+// then SH rotation of the sample's 48-float feature. The features move through
+// LDS (rows48_load / rows48_store): with one row per lane, a wave's float4 load
+// touches 64 rows 192 B apart and the lines are fetched again for each of the
+// row's 12 loads once the L1 has evicted them (bypassing the caches made the
+// kernel 2.6x slower); the coalesced form takes 613 instead of 682 us per
+// 5.64M samples.
 template <int K>
-__device__ __forceinline__ void sample_sh_one(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
-                                              const float* __restrict__ node_q, const ShRotTables* __restrict__ T,
-                                              float* __restrict__ row);
+__device__ __forceinline__ void sample_rotation(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
+                                                const float* __restrict__ node_q, float (&R)[3][3]);
 
 // Five waves per SIMD: the register budget (at most 96) still holds the 48
 // features and the band matrices without spills, and the extra wave hides more
@@ -294,15 +363,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) s
                                                         const uint8_t* __restrict__ is_static,
                                                         float* __restrict__ feat, const ShRotTables* __restrict__ T) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    if (is_static && is_static[s]) return;
-    sample_sh_one<K>(s, w, idx, node_q, T, feat + s * 48);
+    __shared__ float lds[256 * 18];
+    const int64_t s0 = (int64_t)blockIdx.x * blockDim.x;
+    const int n_here = (int)min((int64_t)blockDim.x, S - s0);
+    float c[48];
+    rows48_load(feat + s0 * 48, n_here, lds, 256 * 18, c);
+    if (s < S && !(is_static && is_static[s])) {
+        float R[3][3];
+        sample_rotation<K>(s, w, idx, node_q, R);
+        sh_rotate_regs(*T, R, c);
+    }
+    rows48_store(feat + s0 * 48, n_here, lds, 256 * 18, c);
 }
 
 template <int K>
-__device__ __forceinline__ void sample_sh_one(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
-                                              const float* __restrict__ node_q, const ShRotTables* __restrict__ T,
-                                              float* __restrict__ row) {
+__device__ __forceinline__ void sample_rotation(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
+                                                const float* __restrict__ node_q, float (&R)[3][3]) {
     float acc[4] = {1.0f, 0.0f, 0.0f, 0.0f};
     float last = 0.0f;
 #pragma unroll
@@ -332,9 +408,7 @@ __device__ __forceinline__ void sample_sh_one(int64_t s, const float* __restrict
         last += cw;
     }
     quat_normalize(acc);
-    float R[3][3];
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
-    sh_rotate_f4(*T, R, row);
 }
 
 __global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh,
