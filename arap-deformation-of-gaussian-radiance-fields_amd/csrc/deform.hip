@@ -75,15 +75,6 @@ __device__ __forceinline__ void load_nbrs(const uint32_t* __restrict__ idx, cons
     }
 }
 
-template <int K>
-__device__ __forceinline__ void blend_point_k(const NodeXf* __restrict__ table, const uint32_t* __restrict__ idx,
-                                              const float* __restrict__ w, const float p[3], float o[3]) {
-    uint32_t id[K];
-    float wt[K];
-    load_nbrs<K, false>(idx, w, id, wt);
-    blend_point_regs<K>(table, id, wt, p, o);
-}
-
 __global__ void end_points_kernel(const float* __restrict__ pos, const float* __restrict__ rot,
                                   const float* __restrict__ scale, int64_t N, float pad, float coeff,
                                   float* __restrict__ ends) {
@@ -129,7 +120,10 @@ __global__ void node_table_kernel(int M, const float* __restrict__ node_pos, con
     table[n] = r;
 }
 
-template <int K>
+// VEC: the K indices and weights of a point are read as 16-byte vectors (rows
+// 16-byte aligned): two loads per array instead of K, each covering half of
+// every lane's 32-byte row.
+template <int K, bool VEC>
 __global__ void __launch_bounds__(256) blend_points_kernel(const float* pts, int64_t P,
                                                            const uint32_t* __restrict__ idx,
                                                            const float* __restrict__ w,
@@ -144,7 +138,10 @@ __global__ void __launch_bounds__(256) blend_points_kernel(const float* pts, int
         return;
     }
     float o[3];
-    if (K > 0) blend_point_k<K>(table, idx + i * K, w + i * K, p, o);
+    uint32_t id[K];
+    float wt[K];
+    load_nbrs<K, VEC>(idx + i * K, w + i * K, id, wt);
+    blend_point_regs<K>(table, id, wt, p, o);
     out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
 }
 
@@ -457,9 +454,13 @@ int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const 
                         int M, const uint8_t* skip, float* out, hipStream_t st) {
     (void)M;
     if (P == 0) return 0;
-    if (k == 8)
-        hipLaunchKernelGGL(blend_points_kernel<8>, dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, table, skip,
-                           out);
+    const bool vec = ((uintptr_t)idx % 16 == 0) && ((uintptr_t)w % 16 == 0);
+    if (k == 8 && vec)
+        hipLaunchKernelGGL((blend_points_kernel<8, true>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, table,
+                           skip, out);
+    else if (k == 8)
+        hipLaunchKernelGGL((blend_points_kernel<8, false>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w,
+                           table, skip, out);
     else
         hipLaunchKernelGGL(blend_points_generic_kernel, dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, k,
                            table, skip, out);
