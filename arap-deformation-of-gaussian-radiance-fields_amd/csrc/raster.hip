@@ -33,6 +33,13 @@ struct Geo {
     uint32_t* touched; // P
 };
 
+// The Gaussian falloff exp(power), power <= 0, of the blend: the hardware
+// base-2 exponential of power * log2(e) (a few ulp; the per-pixel test
+// thresholds, 1/255 on alpha and the conservative tile test's 1e-3 margin, are
+// far coarser). The forward kernels and the backward's forward re-run share
+// it, so both see the same contributors.
+__device__ __forceinline__ float blend_exp(float power) { return __expf(power); }
+
 // One Gaussian in one view (record index gv = v*P + g, or g in segment mode);
 // the view's optional per-Gaussian outputs are indexed by gl (g, or in segment
 // mode the Gaussian's index within its view's segment).
@@ -382,7 +389,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
             const float dx = xy.x - fxp, dy = xy.y - fyp;
             const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
             if (power > 0.0f) continue;
-            const float alpha = fminf(0.99f, co.w * expf(power));
+            const float alpha = fminf(0.99f, co.w * blend_exp(power));
             if (alpha < 1.0f / 255.0f) continue;
             const float test_T = T * (1.0f - alpha);
             if (test_T < 0.0001f) { done = true; break; }
@@ -504,7 +511,7 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
                 const float4 co = s_co[k[u]];
                 const float dx = xy.x - fxp, dy = xy.y - fyp;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-                alpha[u] = fminf(0.99f, co.w * expf(power));
+                alpha[u] = fminf(0.99f, co.w * blend_exp(power));
                 use[u] = (j + u < m) && !(power > 0.0f) && !(alpha[u] < 1.0f / 255.0f);
             }
 #pragma unroll
@@ -597,7 +604,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_backward_kernel(RasterParams
             const float dx = xy.x - fxp, dy = xy.y - fyp;
             const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
             if (power > 0.0f) continue;
-            const float alpha = fminf(0.99f, co.w * expf(power));
+            const float alpha = fminf(0.99f, co.w * blend_exp(power));
             if (alpha < 1.0f / 255.0f) continue;
             const float test_T = T * (1.0f - alpha);
             if (test_T < 0.0001f) { done = true; break; }
@@ -639,7 +646,7 @@ __global__ void __launch_bounds__(TILE* TILE) blend_backward_kernel(RasterParams
                 const float dx = xy.x - fxp, dy = xy.y - fyp;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power <= 0.0f) {
-                    const float G = expf(power);
+                    const float G = blend_exp(power);
                     const float alpha = fminf(0.99f, co.w * G);
                     if (alpha >= 1.0f / 255.0f) {
                         act = true;
