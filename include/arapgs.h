@@ -173,9 +173,10 @@ int arapgs_put_free_inputs(arapgs_ctx* ctx, arapgs_solver* s, const float* node_
  * the result is bit-identical to arapgs_gn_solve + arapgs_put_free_inputs on
  * the same solver. A batch holds 1..64 solvers created on ctx's device, each
  * at most once, sharing their state (warm-start steps, coarse operator) with
- * single solves on them; all need the same warm-start setting. Host
- * convergence checks default to every 8 CG iterations (on the AND of the
- * scenes' flags), the coarse refresh to 8 batch solves. */
+ * single solves on them; all need the same warm-start setting. The batch
+ * refers to the solvers' device buffers: destroy it before any of its
+ * solvers. Host convergence checks default to every 8 CG iterations (on the
+ * AND of the scenes' flags), the coarse refresh to 8 batch solves. */
 typedef struct arapgs_solver_batch arapgs_solver_batch;
 int arapgs_solver_batch_create(arapgs_ctx* ctx, arapgs_solver* const* solvers_h, int n, arapgs_solver_batch** out);
 void arapgs_solver_batch_destroy(arapgs_solver_batch* b);
