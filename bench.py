@@ -514,7 +514,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the per-frame path is single-GPU. "
-              "See DESIGN.md for how it would shard.", file=sys.stderr)
+              "Across GPUs the Gaussians and samples would shard by contiguous ranges with a broadcast of the "
+              "solved node table; independent scenes would map one group per GPU.", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
