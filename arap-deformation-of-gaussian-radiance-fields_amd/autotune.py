@@ -20,8 +20,8 @@ import json
 import os
 
 DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1)
-CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(4, 8, 16),
-                  host_checks=(8, 16, 32), warm=(1, 0))
+CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(4, 8, 16, 32),
+                  host_checks=(4, 8, 16, 32), warm=(1, 0))
 ORDER = ("warm", "aggregates", "refresh", "host_checks", "tile", "blend_unroll")
 
 
