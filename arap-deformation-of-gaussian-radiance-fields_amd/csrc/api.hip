@@ -923,17 +923,29 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     uint32_t* offs = (uint32_t*)dmal(4 * (size_t)n);
     uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(std::max(n, gc)));
     uint32_t* total = (uint32_t*)dmal(8);
+    unsigned long long* total64 = (unsigned long long*)dmal(8);
     uint32_t* cell_count = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* cell_ex = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* flags = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* flag_ex = (uint32_t*)dmal(4 * (size_t)gc);
-    if (!cnt || !offs || !scan_ws || !total || !cell_count || !cell_ex || !flags || !flag_ex ||
+    if (!cnt || !offs || !scan_ws || !total || !total64 || !cell_count || !cell_ex || !flags || !flag_ex ||
         hipMalloc(&g->prefix, 4 * (size_t)gc) != hipSuccess) {
         cleanup();
         delete g;
         return fail(c, -2, "grid_build: allocation failed");
     }
     int rc = check_launch(c, launch_box_count(aabbs, n, gg, cnt, c->stream), "grid_build");
+    if (!rc) rc = check_launch(c, launch_sum_counts(cnt, n, total64, c->stream), "grid_build");
+    if (!rc) {
+        // the cell lists use 32-bit signed offsets (as the reference's int prefix sums)
+        unsigned long long pairs64 = 0;
+        hipError_t e = hipMemcpyAsync(&pairs64, total64, 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+        else if (pairs64 > 0x7fffffffull)
+            rc = fail(c, -1, "grid_build: %llu cell-Gaussian pairs exceed 2^31-1 (32-bit list offsets); use a coarser "
+                             "grid", pairs64);
+    }
     if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), "grid_build");
     uint32_t npairs = 0;
     if (!rc) {

@@ -125,6 +125,17 @@ __global__ void box_count_kernel(const float* __restrict__ aabbs, int64_t n, Gri
     cnt[g] = (nx > 0 && ny > 0 && nz > 0) ? (uint32_t)(nx * ny * nz) : 0u;
 }
 
+// 64-bit sum of the per-Gaussian cell counts: the pair lists are addressed
+// with 32-bit signed offsets, so the host rejects totals above 2^31 - 1 before
+// anything is emitted (a 32-bit scan total could wrap unnoticed).
+__global__ void sum_counts_kernel(const uint32_t* __restrict__ cnt, int64_t n, unsigned long long* __restrict__ out) {
+    unsigned long long s = 0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += cnt[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
 __global__ void box_emit_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg,
                                 const uint32_t* __restrict__ offs, uint64_t* __restrict__ keys,
                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_count) {
@@ -294,6 +305,15 @@ int launch_aabbs(const float* pos, const float* rot, const float* scale, const f
 
 int launch_box_count(const float* aabbs, int64_t n, const GridGeom& gg, uint32_t* cnt, hipStream_t st) {
     hipLaunchKernelGGL(box_count_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, cnt);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_sum_counts(const uint32_t* cnt, int64_t n, unsigned long long* out, hipStream_t st) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(out), (int64_t)2);
+    if (n > 0)
+        hipLaunchKernelGGL(sum_counts_kernel, dim3(std::min<unsigned>(nblk(n, 256), 1024u)), dim3(256), 0, st, cnt, n,
+                           out);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -18,6 +18,8 @@ int launch_center_cells(const float* pos, int64_t n, const GridGeom& gg, uint64_
 int launch_aabbs(const float* pos, const float* rot, const float* scale, const float* opacity, int64_t n, float* out,
                  hipStream_t st);
 int launch_box_count(const float* aabbs, int64_t n, const GridGeom& gg, uint32_t* cnt, hipStream_t st);
+// *out (device, 8 bytes) = the 64-bit sum of cnt[0..n).
+int launch_sum_counts(const uint32_t* cnt, int64_t n, unsigned long long* out, hipStream_t st);
 int launch_box_emit(const float* aabbs, int64_t n, const GridGeom& gg, const uint32_t* offs, uint64_t* keys,
                     uint32_t* vals, uint32_t* cell_count, hipStream_t st);
 int launch_cell_prefix(const uint32_t* cell_ex, const uint32_t* cell_count, int64_t gc, int* prefix, uint32_t* flags,

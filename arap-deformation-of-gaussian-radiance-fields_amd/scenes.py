@@ -172,6 +172,36 @@ def capsule_scene(seed=0, n=1_000_000, n_nodes=2048, W=800, H=800, n_views=4) ->
                            direction=u, distance=0.3))
 
 
+def flat_scene(seed=0, n=3_000_000, n_nodes=4096, W=1920, H=1080, n_views=8) -> Scene:
+    """The large high-frequency workload: the capsule union with flat Gaussians
+    (one scale axis exp(N(-9, 0.3)), two exp(N(-4, 0.5))), 4096 nodes, a 128^3
+    grid, 12 GN iterations, 8 views at 1920x1080, and a combined edit of the
+    active capsule: translation by 0.3, twist by 30 degrees about the capsule
+    axis and scaling by 1.3 about its centre."""
+    rng = np.random.default_rng(seed)
+    caps = _capsules(rng)
+    vols = np.array([math.pi * r * r * np.linalg.norm(b - a) + 4.0 / 3.0 * math.pi * r ** 3 for a, b, r in caps])
+    counts = rng.multinomial(n, vols / vols.sum())
+    pts = []
+    for (a, b, r), c in zip(caps, counts):
+        lo, hi = np.minimum(a, b) - r, np.maximum(a, b) + r
+        got = np.zeros((0, 3))
+        while got.shape[0] < c:
+            cand = rng.uniform(lo, hi, size=(max(2 * (c - got.shape[0]), 64), 3))
+            got = np.concatenate([got, cand[_in_capsule(cand, a, b, r)]])
+        pts.append(got[:c])
+    pos = np.concatenate(pts)[rng.permutation(n)]
+    g = random_gaussians(rng, pos, flat=True)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    a1, b1, _ = caps[1]
+    axis = (b1 - a1) / max(np.linalg.norm(b1 - a1), 1e-12)
+    cams = ring_cameras(n_views, 3.0, 15.0, 50.0, W, H)
+    return Scene("flat", g, n_nodes, 8, 128, 12, cams,
+                 edit=dict(kind="bend_twist_scale", capsules=caps, pinned_capsule=0, active_capsule=1, direction=u,
+                           distance=0.3, axis=axis, angle_deg=30.0, scale=1.3))
+
+
 def blocks_for(scene: Scene, node_pos: np.ndarray):
     """Blocks (lists of node ids), types (0 pinned, 1 active) for the scene's edit."""
     e = scene.edit
@@ -220,6 +250,14 @@ def frame_aims(scene: Scene, node_pos: np.ndarray, blocks, types, fraction: floa
         if e["kind"] == "twist":
             center = node_pos[blk].astype(np.float32).mean(axis=0)
             aims[blk] = rotate_about_axis(node_pos[blk], center, e["axis"], math.radians(e["angle_deg"]) * fraction)
+        elif e["kind"] == "bend_twist_scale":
+            # UpdateAimPosition, then UpdateAimPositionTwist, then UpdateAimPositionScale
+            p = node_pos[blk] + np.float32(e["distance"] * fraction) * e["direction"].astype(np.float32)
+            center = p.astype(np.float32).mean(axis=0)
+            p = rotate_about_axis(p, center, e["axis"], math.radians(e["angle_deg"]) * fraction)
+            center = p.astype(np.float32).mean(axis=0)
+            k = np.float32(1.0 + (e["scale"] - 1.0) * fraction)
+            aims[blk] = (center + k * (p - center)).astype(np.float32)
         else:
             aims[blk] = node_pos[blk] + np.float32(e["distance"] * fraction) * e["direction"].astype(np.float32)
     return aims

@@ -1,0 +1,152 @@
+"""The large high-frequency workload at full size: 3M flat Gaussians (one
+scale axis ~e^-9), 4096 nodes, 128^3 grid, 12 Gauss-Newton iterations of a
+combined bend + twist + 1.3x scale edit, 8 views at 1920x1080. The oracle
+cannot run the whole frame in seconds, so the checks are sampled or
+size-independent:
+
+* FPS picks 4096 distinct nodes starting at argmax(x+y+z);
+* KNN indices and weights of 3000 random end points equal the oracle's bit for
+  bit (4096 candidate nodes);
+* the Gauss-Newton energies never increase, and the oracle's FP64 energy
+  function at the returned unknowns equals the reported final energy (1e-5);
+* the six-point update of every Gaussian with the solved node table matches the
+  oracle's blend + six-point on 3000 random Gaussians (end points bit-exact,
+  centres 2e-6, covariances 1e-5 relative, SH 1e-5);
+* the 8 views rendered in one pass equal separate renders bit for bit;
+* the 128^3 cell lists of this scene would hold 2.52e9 cell-Gaussian pairs
+  (counted on the CPU with the oracle's AABBs and the cell rule), beyond the
+  32-bit signed list offsets that the grid (like the reference's int prefix
+  sums) uses: arapgs_grid_build refuses the scene with an error before it
+  emits anything, so the samples and density are not part of this frame.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from arapgs_amd import arapgs
+from arapgs_amd import scenes as S
+from arapgs_amd import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def _cov(rot, scale):
+    R = np.stack([O.quat_to_mat(q / np.linalg.norm(q)).astype(np.float64) for q in rot])
+    s2 = scale.astype(np.float64) ** 2
+    return np.einsum("nij,nj,nkj->nik", R, s2, R)
+
+
+def _prepare_without_grid(ctx, sc):
+    """workload.prepare's graph set-up (cell reorder, FPS, edges, KNN, end
+    points) without the cell lists."""
+    import torch
+    g = sc.gaussians
+    K = sc.knn_k
+    Tt = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    pos, rot, scale, opac, sh = Tt(g.pos), Tt(g.rot), Tt(g.scale), Tt(g.opacity), Tt(g.sh)
+    a7 = ctx.scene_aabb(pos, sc.grid)
+    order = ctx.grid_reorder(pos, a7, sc.grid).long()
+    pos, rot, scale, opac, sh = (t.index_select(0, order).contiguous() for t in (pos, rot, scale, opac, sh))
+    nidx = ctx.fps(pos, sc.n_nodes).long()
+    node_pos = pos.index_select(0, nidx).contiguous()
+    nbr = ctx.graph_edges(node_pos, K)
+    src_idx_full, src_w = ctx.knn(node_pos, node_pos, K)
+    ends = ctx.end_points(pos, rot, scale)
+    eidx_full, ends_w = ctx.knn(node_pos, ends.view(-1, 3), K)
+    torch.cuda.synchronize()
+    npos_h = node_pos.cpu().numpy()
+    blocks, types = S.blocks_for(sc, npos_h)
+    solver_args = (nbr.cpu().numpy(), blocks, types, src_idx_full[:, :K].contiguous().cpu().numpy(),
+                   src_w.cpu().numpy(), npos_h)
+    st = dict(pos=pos, rot=rot, scale=scale, sh=sh, scale0=scale.clone(), ends=ends,
+              ends_idx=eidx_full[:, :K].contiguous(), ends_w=ends_w, node_pos=node_pos)
+    cams = [dict(view=Tt(c.view), proj=Tt(c.proj), campos=Tt(c.campos), tx=c.tanfovx, ty=c.tanfovy, W=c.W, H=c.H)
+            for c in sc.cameras]
+    return dict(state=st, solver_args=solver_args, cams=cams, bg=Tt(np.array([0.8, 0.8, 0.8], np.float32)),
+                opac=opac, K=K, a7=a7)
+
+
+def test_large_flat_frame(ctx):
+    import torch
+    print("large: building 3M flat Gaussians, 4096 nodes", flush=True)
+    sc = S.flat_scene(0)
+    wl = _prepare_without_grid(ctx, sc)
+    st = wl["state"]
+    N, M, K = st["pos"].shape[0], st["node_pos"].shape[0], wl["K"]
+    assert (N, M) == (3_000_000, 4096)
+    pos = H(st["pos"])
+    nodes = H(st["node_pos"])
+    first = int(np.argmax((pos[:, 0] + pos[:, 1]) + pos[:, 2]))  # float32, the reference's x + y + z
+    assert (nodes == pos[first]).all(axis=1).any()
+    assert len(np.unique(nodes, axis=0)) == M
+
+    print("large: KNN of 3000 end points vs the oracle", flush=True)
+    rng = np.random.default_rng(11)
+    ends = H(st["ends"]).reshape(-1, 3)
+    pick = rng.choice(ends.shape[0], 3000, replace=False)
+    ridx, rw = O.knn_weights(nodes, ends[pick], K)
+    assert np.array_equal(H(st["ends_idx"])[pick].astype(np.int64), ridx[:, :K].astype(np.int64))
+    assert np.array_equal(H(st["ends_w"])[pick], rw.astype(np.float32))
+
+    print("large: 12 GN iterations of the combined edit", flush=True)
+    nbr, blocks, types, src_idx, src_w, npos_h = wl["solver_args"]
+    solver = workload.make_solver(ctx, wl["solver_args"])
+    try:
+        aims_h = S.frame_aims(sc, npos_h, blocks, types, 1.0)
+        x, e, it = solver.solve(st["node_pos"], torch.from_numpy(aims_h).cuda(), sc.gn_iters)
+        n_it = int(H(it)[0])
+        e = H(e)[: n_it + 1]
+        assert 1 <= n_it <= sc.gn_iters and e[0] > 0
+        assert np.all(np.diff(e) <= 1e-12 * e[0]), e
+        prob = O.GNProblem(npos_h, nbr, blocks, types, src_idx, src_w, aims_h)
+        f = prob.residual(H(x).astype(np.float64))
+        assert abs(float(f @ f) - e[-1]) <= 1e-5 * e[-1] + 1e-9, (float(f @ f), e[-1])
+        table = solver.put_free_inputs(st["node_pos"], x)
+    finally:
+        solver.close()
+
+    print("large: six-point update of 3M Gaussians vs the oracle on 3000", flush=True)
+    tab = H(table)
+    rot9 = tab[:, 3:12].reshape(M, 3, 3).transpose(0, 2, 1).reshape(M, 9)  # row-major -> column-major
+    t3 = tab[:, 12:15]
+    gsel = rng.choice(N, 3000, replace=False)
+    ends0 = H(st["ends"])[gsel]
+    eidx = H(st["ends_idx"]).reshape(N, 6, K)[gsel].reshape(-1, K)
+    ew = H(st["ends_w"]).reshape(N, 6, K)[gsel].reshape(-1, K)
+    rot0, scale0, sh0, mu0 = H(st["rot"])[gsel], H(st["scale0"])[gsel], H(st["sh"])[gsel], pos[gsel]
+    ref_ends = O.blend(ends0.reshape(-1, 3), eidx, ew, nodes, rot9, t3).reshape(-1, 6, 3)
+    rpos, rrot, rscale, rsh = O.six_point(ref_ends, scale0, np.zeros(len(gsel), np.uint8), mu0, rot0, scale0, sh0)
+    ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"], None, st["pos"], st["rot"],
+                         st["scale"], st["sh"])
+    torch.cuda.synchronize()
+    assert np.array_equal(H(st["ends"])[gsel], ref_ends)
+    np.testing.assert_allclose(H(st["pos"])[gsel], rpos, rtol=0, atol=2e-6)
+    cg, cr = _cov(H(st["rot"])[gsel], H(st["scale"])[gsel]), _cov(rrot, rscale)
+    rel = np.linalg.norm(cg - cr, axis=(1, 2)) / np.linalg.norm(cr, axis=(1, 2))
+    assert rel.max() < 1e-5, rel.max()
+    scale_sh = np.maximum(1.0, np.abs(rsh).max(axis=1, keepdims=True))
+    assert (np.abs(H(st["sh"])[gsel] - rsh) / scale_sh).max() < 1e-5
+
+    print("large: 8 views at 1920x1080 in one pass vs separate renders", flush=True)
+    W, Hh = sc.cameras[0].W, sc.cameras[0].H
+    outs = [torch.zeros((3, Hh, W), dtype=torch.float32, device="cuda") for _ in wl["cams"]]
+    n_all = ctx.render_views(st["pos"], st["sh"], wl["opac"], st["scale"], st["rot"], wl["cams"], W, Hh, wl["bg"],
+                             outs)
+    assert n_all > 0
+    for v in (0, 5):
+        ref = torch.zeros((3, Hh, W), dtype=torch.float32, device="cuda")
+        ctx.render_views(st["pos"], st["sh"], wl["opac"], st["scale"], st["rot"], [wl["cams"][v]], W, Hh, wl["bg"],
+                         [ref])
+        torch.cuda.synchronize()
+        assert torch.equal(outs[v], ref), v
+    img = torch.stack(outs)
+    assert torch.isfinite(img).all() and float(img.min()) >= 0.0
+
+    print("large: 128^3 cell lists refused (pairs beyond 32-bit offsets)", flush=True)
+    aabbs = ctx.gaussian_aabbs(st["pos"], st["rot"], st["scale"], wl["opac"])
+    with pytest.raises(arapgs.ArapgsError, match="exceed 2\\^31-1"):
+        ctx.grid_build(aabbs, wl["a7"], sc.grid, padding=1)
