@@ -100,8 +100,10 @@ def prepare_batch(ctx, scene_iter, device="cuda:0", log=None):
         parts.append(dict(state=keep, opac=wl["opac"], solver_args=wl["solver_args"], active=wl["active"],
                           cam=wl["cams"][0], K=wl["K"]))
         if log:
+            _, blocks, types, _, _, _ = wl["solver_args"]
+            sizes = {int(t): len(b) for b, t in zip(blocks, types)}
             log(f"batch scene {len(parts) - 1}: {sc.gaussians.n} Gaussians, {sc.n_nodes} nodes, "
-                f"{st['samples'].shape[0]} samples")
+                f"{st['samples'].shape[0]} samples, pinned/active nodes {sizes.get(0, 0)}/{sizes.get(1, 0)}")
         del wl
     K = parts[0]["K"]
     seg, node_seg, sample_seg = [0], [0], [0]
