@@ -67,6 +67,7 @@ EXPORTS = {
     "arapgs_last_error": ([P], C.c_char_p),
     "arapgs_set_stream": ([P, P], C.c_int),
     "arapgs_synchronize": ([P], C.c_int),
+    "arapgs_set_grid_chunk": ([P, C.c_int64], C.c_int),
     "arapgs_stream_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_stream_destroy": ([P], C.c_int),
     "arapgs_version": ([], C.c_char_p),
@@ -418,6 +419,10 @@ class Context:
                                                 ptr(out)))
         return out
 
+    def set_grid_chunk(self, pairs):
+        """Cell-Gaussian pairs per chunk of later grid builds (0: default); test hook."""
+        self._check(lib().arapgs_set_grid_chunk(self.h, int(pairs)))
+
     def grid_build(self, aabbs, aabb7, G, padding=1):
         import numpy as np
         a = np.ascontiguousarray(aabb7, np.float32)
@@ -694,7 +699,7 @@ class Grid:
         import torch
         G3 = self.G ** 3
         out = []
-        for p, n, dt, shp in ((self._ptrs[0], G3, np.int32, (G3,)), (self._ptrs[1], self.n_pairs, np.int32,
+        for p, n, dt, shp in ((self._ptrs[0], G3, np.int64, (G3,)), (self._ptrs[1], self.n_pairs, np.int32,
                                                                       (self.n_pairs,)),
                               (self._ptrs[2], self.n_valid, np.int32, (self.n_valid,)),
                               (self._ptrs[3], self.n_valid * 64 * 3, np.float32, (self.n_valid * 64, 3))):
@@ -707,11 +712,36 @@ class Grid:
             out.append(a)
         return out
 
-    def samples_tensor(self):
-        """Device copy of the samples as a torch tensor (S, 3)."""
+    def prefix_valid(self):
+        """Host copies of the 64-bit per-cell prefix and the valid cells only
+        (the lists of a large grid stay on the device)."""
         import numpy as np
         import torch
-        return torch.from_numpy(self.arrays()[3]).cuda()
+        G3 = self.G ** 3
+        prefix = np.empty(G3, np.int64)
+        valid = np.empty(self.n_valid, np.int32)
+        torch.cuda.synchronize()
+        for a, p in ((prefix, self._ptrs[0]), (valid, self._ptrs[2])):
+            if a.size and _hip_memcpy_d2h(a.ctypes.data, p, a.nbytes) != 0:
+                raise ArapgsError("hipMemcpy failed")
+        return prefix, valid
+
+    def list_range(self, beg, end):
+        """Host copy of list entries [beg, end)."""
+        import numpy as np
+        a = np.empty(max(0, end - beg), np.int32)
+        if a.size and _hip_memcpy_d2h(a.ctypes.data, self._ptrs[1] + 4 * beg, a.nbytes) != 0:
+            raise ArapgsError("hipMemcpy failed")
+        return a
+
+    def samples_tensor(self):
+        """Device copy of the samples as a torch tensor (S, 3)."""
+        import torch
+        out = torch.empty((self.n_valid * 64, 3), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        if out.numel() and _hip_memcpy(out.data_ptr(), self._ptrs[3], out.numel() * 4, 3) != 0:
+            raise ArapgsError("hipMemcpy failed")
+        return out
 
     def density_backward(self, pos, rot, scale, opacity, sh, lpf, dfeat=None, dopac=None, samples=None,
                          want_samples=False):
@@ -752,13 +782,22 @@ class Grid:
 _hip = None
 
 
-def _hip_memcpy_d2h(dst, src, nbytes):
+def _hiplib():
     global _hip
     if _hip is None:
         _hip = C.CDLL("libamdhip64.so")
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipMemcpy.restype = C.c_int
-    return _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 2)  # hipMemcpyDeviceToHost
+    return _hip
+
+
+def _hip_memcpy(dst, src, nbytes, kind):
+    """hipMemcpy with an explicit kind (2 device-to-host, 3 device-to-device)."""
+    return _hiplib().hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), kind)
+
+
+def _hip_memcpy_d2h(dst, src, nbytes):
+    return _hip_memcpy(dst, src, nbytes, 2)
 
 
 def frame_buffers(state: dict, k: int, axis_padding=1e-3, end_coeff=2.0) -> FrameBuffers:

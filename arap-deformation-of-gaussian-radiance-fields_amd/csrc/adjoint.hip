@@ -89,7 +89,7 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
 // q22, then q01, q02, q12 each counted for both symmetric entries), d c (48),
 // d o (1).
 __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const int* __restrict__ valid,
-                                                              const int* __restrict__ prefix,
+                                                              const int64_t* __restrict__ prefix,
                                                               const int* __restrict__ list,
                                                               const float* __restrict__ samples,
                                                               const float* __restrict__ pos,
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
     if (v >= nv) return;
     const int t = threadIdx.x;
     const int c = valid[v];
-    const int beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
+    const int64_t beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
     const int64_t si = v * 64 + t;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -120,8 +120,8 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
     for (int k = 0; k < 48; k++) s_df[t][k] = dfeat ? dfeat[48 * si + k] : 0.0f;
     s_df[t][48] = ddens ? ddens[si] : 0.0f;
     __syncthreads();
-    for (int b0 = beg; b0 < end; b0 += 64) {
-        const int li = b0 + t;
+    for (int64_t b0 = beg; b0 < end; b0 += 64) {
+        const int64_t li = b0 + t;
         const bool has = li < end;
         const int g = has ? list[li] : 0;
         float mu[3] = {0.0f, 0.0f, 0.0f}, Q[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, o = 0.0f;
@@ -650,7 +650,7 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st) {
     return 0;
 }
 
-int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
                             float* gdens, int64_t n_gauss, hipStream_t st) {

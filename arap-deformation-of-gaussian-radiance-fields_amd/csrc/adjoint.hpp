@@ -14,7 +14,7 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st);
 // positions: gsamp (S*3, may be null) written, gdens (58 x N component-major:
 // d mu 3, d Q 6, d sh 48, d opacity) accumulated (zeroed by the caller).
 // dfeat / ddens may be null.
-int launch_density_backward(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
                             float* gdens, int64_t n_gauss, hipStream_t st);

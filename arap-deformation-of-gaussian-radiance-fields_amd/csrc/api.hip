@@ -54,6 +54,7 @@ struct arapgs_ctx {
     size_t r_rec_cap = 0;
     void* db_ws = nullptr;          // density backward accumulators
     size_t db_cap = 0;
+    uint64_t grid_chunk_pairs = 0;  // cell-list build chunk (0: GRID_CHUNK_PAIRS)
     uint32_t* r_total = nullptr;
     uint32_t last_rendered = 0;
     int last_tiles = 0;
@@ -218,6 +219,13 @@ int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_o
 int arapgs_stream_destroy(void* stream) {
     if (!stream) return -1;
     return hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : -1;
+}
+
+int arapgs_set_grid_chunk(arapgs_ctx* c, int64_t pairs) {
+    CHECK_CTX(c);
+    if (pairs < 0) return fail(c, -1, "set_grid_chunk: negative chunk");
+    c->grid_chunk_pairs = (uint64_t)pairs;
+    return 0;
 }
 
 int arapgs_synchronize(arapgs_ctx* c) {
@@ -438,8 +446,17 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
         if (cg_mode == 1) {
             rc = check_launch(c, solver_cg_persistent(*B.w[0], c->stream), "gn_solve cg");
             if (rc) return rc;
+            // the grid barrier gives up after a bounded spin: report it instead
+            // of handing an unfinished step to the line search
+            int bar_err = 0;
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg");
+            if ((rc = check_launch(c, solver_barrier_error(*B.w[0], &bar_err), "gn_solve cg"))) return rc;
+            if (bar_err) return fail(c, -6, "gn_solve: the persistent CG kernel's grid barrier timed out");
         } else {
-            const int cap = B.w[0]->cg_max_iters;
+            // every scene's own cap is enforced on the device (cg_done); loop to the largest
+            int cap = 0;
+            for (const SolverWork* w : B.w) cap = std::max(cap, w->cg_max_iters);
             for (int k = 0; k < cap; k++) {
                 rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
                 if (rc) return rc;
@@ -824,7 +841,7 @@ struct arapgs_grid {
     int G = 0;
     int64_t n_pairs = 0, n_valid = 0;
     int64_t n_gauss = 0;  // Gaussians the grid was built for
-    int* prefix = nullptr;
+    int64_t* prefix = nullptr;
     int* list = nullptr;
     int* valid = nullptr;
     float* samples = nullptr;
@@ -903,9 +920,12 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     CHECK_CTX(c);
     if (!aabbs || !aabb7_h || !out || n <= 0 || grid_num <= 0 || grid_num > 1024)
         return fail(c, -1, "grid_build: bad arguments");
+    if (n > 0x7fffffffLL) return fail(c, -1, "grid_build: at most 2^31-1 Gaussians");
     *out = nullptr;
     (void)hipSetDevice(c->device);
-    const int64_t gc = (int64_t)grid_num * grid_num * grid_num;
+    const int G = grid_num;
+    const int64_t G2 = (int64_t)G * G;
+    const int64_t gc = G2 * G;
     GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, padding};
     arapgs_grid* g = new arapgs_grid();
     g->G = grid_num;
@@ -919,62 +939,114 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
         return p;
     };
     auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
-    uint32_t* cnt = (uint32_t*)dmal(4 * (size_t)n);
-    uint32_t* offs = (uint32_t*)dmal(4 * (size_t)n);
-    uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(std::max(n, gc)));
-    uint32_t* total = (uint32_t*)dmal(8);
-    unsigned long long* total64 = (unsigned long long*)dmal(8);
+    auto sync = [&]() -> int {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        return e == hipSuccess ? 0 : hip_fail(c, e, "grid_build");
+    };
+    // 1. pairs per x-slab, and from them the chunks of whole slabs whose pairs
+    //    fit the 32-bit offsets of one emit + sort pass
+    unsigned long long* slab = (unsigned long long*)dmal(8 * (size_t)G);
     uint32_t* cell_count = (uint32_t*)dmal(4 * (size_t)gc);
-    uint32_t* cell_ex = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* flags = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* flag_ex = (uint32_t*)dmal(4 * (size_t)gc);
-    if (!cnt || !offs || !scan_ws || !total || !total64 || !cell_count || !cell_ex || !flags || !flag_ex ||
-        hipMalloc(&g->prefix, 4 * (size_t)gc) != hipSuccess) {
+    uint32_t* total = (uint32_t*)dmal(8);
+    uint32_t* prefix_ws = (uint32_t*)dmal(4 * cell_prefix_workspace_words(gc));
+    uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(std::max<int64_t>(n, gc)));
+    if (!slab || !cell_count || !flags || !flag_ex || !total || !prefix_ws || !scan_ws ||
+        hipMalloc(&g->prefix, 8 * (size_t)gc) != hipSuccess) {
         cleanup();
         delete g;
         return fail(c, -2, "grid_build: allocation failed");
     }
-    int rc = check_launch(c, launch_box_count(aabbs, n, gg, cnt, c->stream), "grid_build");
-    if (!rc) rc = check_launch(c, launch_sum_counts(cnt, n, total64, c->stream), "grid_build");
+    int rc = check_launch(c, launch_slab_counts(aabbs, n, gg, slab, c->stream), "grid_build");
+    std::vector<unsigned long long> slab_h((size_t)G, 0ull);
     if (!rc) {
-        // the cell lists use 32-bit signed offsets (as the reference's int prefix sums)
-        unsigned long long pairs64 = 0;
-        hipError_t e = hipMemcpyAsync(&pairs64, total64, 8, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipError_t e = hipMemcpyAsync(slab_h.data(), slab, 8 * (size_t)G, hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
-        else if (pairs64 > 0x7fffffffull)
-            rc = fail(c, -1, "grid_build: %llu cell-Gaussian pairs exceed 2^31-1 (32-bit list offsets); use a coarser "
-                             "grid", pairs64);
+        else rc = sync();
     }
-    if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), "grid_build");
-    uint32_t npairs = 0;
+    std::vector<std::pair<int, int>> chunks;  // [x0, x1)
+    uint64_t npairs = 0, max_chunk = 0;
+    const uint64_t chunk_pairs = c->grid_chunk_pairs ? c->grid_chunk_pairs : GRID_CHUNK_PAIRS;
     if (!rc) {
-        hipError_t e = hipMemcpyAsync(&npairs, total, 4, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+        uint64_t cur = 0;
+        int x0 = 0;
+        for (int x = 0; x < G; x++) {
+            const uint64_t p = slab_h[x];
+            if (p > GRID_CHUNK_PAIRS_MAX) {
+                rc = fail(c, -1, "grid_build: slab %d holds %llu cell-Gaussian pairs (at most %llu per slab)", x,
+                          (unsigned long long)p, (unsigned long long)GRID_CHUNK_PAIRS_MAX);
+                break;
+            }
+            if (x > x0 && cur + p > chunk_pairs) {
+                chunks.push_back({x0, x});
+                max_chunk = std::max(max_chunk, cur);
+                x0 = x;
+                cur = 0;
+            }
+            cur += p;
+            npairs += p;
+        }
+        if (!rc) {
+            chunks.push_back({x0, G});
+            max_chunk = std::max(max_chunk, cur);
+        }
     }
-    uint64_t* keys = nullptr;
-    uint32_t* vals = nullptr;
+    // 2. per chunk: counts, offsets, pairs keyed by the cell within the chunk,
+    //    a stable radix sort (pairs leave the emitter in Gaussian order, so every
+    //    cell's list comes out ascending) and the values appended to the list
     if (!rc) {
-        const size_t np = std::max<size_t>(npairs, 1);
-        keys = (uint64_t*)dmal(8 * np * 2);
-        uint32_t* vals_alt = (uint32_t*)dmal(4 * np);
+        const size_t np = std::max<uint64_t>(max_chunk, 1);
+        uint32_t* cnt = (uint32_t*)dmal(4 * (size_t)n);
+        uint32_t* offs = (uint32_t*)dmal(4 * (size_t)n);
+        uint32_t* keys = (uint32_t*)dmal(4 * np * 2);
+        uint32_t* vals = (uint32_t*)dmal(4 * np * 2);
         uint32_t* sort_ws = (uint32_t*)dmal(4 * radix_workspace_words((int64_t)np));
-        if (!keys || !vals_alt || !sort_ws || hipMalloc(&g->list, 4 * np) != hipSuccess) rc = fail(c, -2, "grid_build: allocation failed");
-        vals = (uint32_t*)g->list;
-        if (!rc) rc = check_launch(c, launch_box_emit(aabbs, n, gg, offs, keys, vals, cell_count, c->stream), "grid_build");
-        int bits = 1;
-        while ((1ll << bits) < gc) bits++;
-        if (!rc) rc = check_launch(c, launch_radix_sort(keys, vals, keys + np, vals_alt, npairs, bits, sort_ws, c->stream), "grid_build");
-        if (!rc) rc = check_launch(c, launch_exclusive_scan(cell_count, gc, cell_ex, total, scan_ws, c->stream), "grid_build");
-        if (!rc) rc = check_launch(c, launch_cell_prefix(cell_ex, cell_count, gc, g->prefix, flags, c->stream), "grid_build");
-        if (!rc) rc = check_launch(c, launch_exclusive_scan(flags, gc, flag_ex, total + 1, scan_ws, c->stream), "grid_build");
+        if (!cnt || !offs || !keys || !vals || !sort_ws ||
+            hipMalloc(&g->list, 4 * std::max<uint64_t>(npairs, 1)) != hipSuccess)
+            rc = fail(c, -2, "grid_build: allocation failed (%llu cell-Gaussian pairs)", (unsigned long long)npairs);
+        if (!rc) rc = check_launch(c, launch_zero_cells(cell_count, gc, c->stream), "grid_build");
+        uint64_t base = 0;
+        for (size_t k = 0; !rc && k < chunks.size(); k++) {
+            const int x0 = chunks[k].first, x1 = chunks[k].second;
+            rc = check_launch(c, launch_box_count_slabs(aabbs, n, gg, x0, x1, cnt, c->stream), "grid_build");
+            if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), "grid_build");
+            uint32_t cp = 0;
+            if (!rc) {
+                hipError_t e = hipMemcpyAsync(&cp, total, 4, hipMemcpyDeviceToHost, c->stream);
+                if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+                else rc = sync();
+            }
+            uint64_t want = 0;
+            for (int x = x0; x < x1; x++) want += slab_h[x];
+            if (!rc && (uint64_t)cp != want)
+                rc = fail(c, -1, "grid_build: chunk %d..%d counts %u pairs, slabs %llu", x0, x1, cp,
+                          (unsigned long long)want);
+            if (!rc && cp > 0) {
+                int bits = 1;
+                while ((1ll << bits) < (int64_t)(x1 - x0) * G2) bits++;
+                rc = check_launch(c, launch_box_emit_slabs(aabbs, n, gg, x0, x1, offs, keys, vals, cell_count,
+                                                           c->stream), "grid_build");
+                if (!rc)
+                    rc = check_launch(c, launch_radix_sort32(keys, vals, keys + np, vals + np, cp, bits, sort_ws,
+                                                             c->stream), "grid_build");
+                if (!rc) {
+                    hipError_t e = hipMemcpyAsync(g->list + base, vals, 4 * (size_t)cp, hipMemcpyDeviceToDevice,
+                                                  c->stream);
+                    if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+                }
+            }
+            base += cp;
+        }
     }
+    // 3. the 64-bit per-cell prefix, valid cells and their samples
+    if (!rc) rc = check_launch(c, launch_cell_prefix(cell_count, gc, g->prefix, flags, prefix_ws, c->stream), "grid_build");
+    if (!rc) rc = check_launch(c, launch_exclusive_scan(flags, gc, flag_ex, total + 1, scan_ws, c->stream), "grid_build");
     uint32_t nvalid = 0;
     if (!rc) {
         hipError_t e = hipMemcpyAsync(&nvalid, total + 1, 4, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+        else rc = sync();
     }
     if (!rc) {
         if (hipMalloc(&g->valid, 4 * std::max<size_t>(nvalid, 1)) != hipSuccess ||
@@ -982,15 +1054,15 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
             rc = fail(c, -2, "grid_build: allocation failed");
         if (!rc) rc = check_launch(c, launch_compact_valid(flags, flag_ex, gc, g->valid, c->stream), "grid_build");
         if (!rc) rc = check_launch(c, launch_samples(g->valid, nvalid, gg, g->samples, c->stream), "grid_build");
-        hipError_t e = hipStreamSynchronize(c->stream);
-        if (!rc && e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+        const int rs = sync();
+        if (!rc) rc = rs;
     }
     cleanup();
     if (rc) {
         arapgs_grid_destroy(g);
         return rc;
     }
-    g->n_pairs = npairs;
+    g->n_pairs = (int64_t)npairs;
     g->n_valid = nvalid;
     *out = g;
     return 0;
@@ -1004,7 +1076,7 @@ void arapgs_grid_destroy(arapgs_grid* g) {
     delete g;
 }
 
-int arapgs_grid_info(const arapgs_grid* g, int64_t* n_pairs, int64_t* n_valid, const int** prefix, const int** list,
+int arapgs_grid_info(const arapgs_grid* g, int64_t* n_pairs, int64_t* n_valid, const int64_t** prefix, const int** list,
                      const int** valid, const float** samples) {
     if (!g) return -1;
     if (n_pairs) *n_pairs = g->n_pairs;
@@ -1167,7 +1239,7 @@ struct arapgs_chain {
     arapgs_chain_desc d{};
     std::vector<const float*> view, proj, campos;
     std::vector<float> tfx, tfy;
-    const int* g_prefix = nullptr;
+    const int64_t* g_prefix = nullptr;
     const int* g_list = nullptr;
     const int* g_valid = nullptr;
     const float* samples0 = nullptr;

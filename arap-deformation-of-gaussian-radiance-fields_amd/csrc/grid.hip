@@ -125,9 +125,8 @@ __global__ void box_count_kernel(const float* __restrict__ aabbs, int64_t n, Gri
     cnt[g] = (nx > 0 && ny > 0 && nz > 0) ? (uint32_t)(nx * ny * nz) : 0u;
 }
 
-// 64-bit sum of the per-Gaussian cell counts: the pair lists are addressed
-// with 32-bit signed offsets, so the host rejects totals above 2^31 - 1 before
-// anything is emitted (a 32-bit scan total could wrap unnoticed).
+// 64-bit sum of the per-Gaussian cell counts (the total number of
+// cell-Gaussian pairs, which may exceed 2^32 on large grids).
 __global__ void sum_counts_kernel(const uint32_t* __restrict__ cnt, int64_t n, unsigned long long* __restrict__ out) {
     unsigned long long s = 0ull;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -136,31 +135,139 @@ __global__ void sum_counts_kernel(const uint32_t* __restrict__ cnt, int64_t n, u
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
 }
 
-__global__ void box_emit_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg,
-                                const uint32_t* __restrict__ offs, uint64_t* __restrict__ keys,
-                                uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_count) {
+// Pairs per x-slab of the grid (cells x*G^2 .. (x+1)*G^2-1): a Gaussian whose
+// box spans ny*nz cells in y and z adds ny*nz to every slab its box crosses.
+__global__ void slab_count_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg,
+                                  unsigned long long* __restrict__ slab) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     int lo[3], hi[3];
     box_range(aabbs + 6 * g, gg, lo, hi);
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    if (hi[0] < lo[0] || ny <= 0 || nz <= 0) return;
+    for (int x = lo[0]; x <= hi[0]; x++) atomicAdd(&slab[x], (unsigned long long)(ny * nz));
+}
+
+// Cells of each Gaussian's box inside the slabs [x0, x1).
+__global__ void box_count_slabs_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg, int x0, int x1,
+                                       uint32_t* __restrict__ cnt) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lo[3], hi[3];
+    box_range(aabbs + 6 * g, gg, lo, hi);
+    const int nx = min(hi[0], x1 - 1) - max(lo[0], x0) + 1;
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    cnt[g] = (nx > 0 && ny > 0 && nz > 0) ? (uint32_t)(nx * ny * nz) : 0u;
+}
+
+// The pairs of the slabs [x0, x1): key = cell - x0*G^2 (32-bit), value = the
+// Gaussian, written from offs[g] in ascending cell order; every pair counts
+// into its cell.
+__global__ void box_emit_slabs_kernel(const float* __restrict__ aabbs, int64_t n, GridGeom gg, int x0, int x1,
+                                      const uint32_t* __restrict__ offs, uint32_t* __restrict__ keys,
+                                      uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_count) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int lo[3], hi[3];
+    box_range(aabbs + 6 * g, gg, lo, hi);
+    const int xa = max(lo[0], x0), xb = min(hi[0], x1 - 1);
+    const int G2 = gg.G * gg.G;
+    const int c0 = x0 * G2;
     uint32_t o = offs[g];
-    for (int x = lo[0]; x <= hi[0]; x++)
+    for (int x = xa; x <= xb; x++)
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int z = lo[2]; z <= hi[2]; z++) {
-                const uint32_t c = (uint32_t)(x * gg.G * gg.G + y * gg.G + z);
-                keys[o] = c;
+                const int c = x * G2 + y * gg.G + z;
+                keys[o] = (uint32_t)(c - c0);
                 vals[o] = (uint32_t)g;
                 o++;
                 atomicAdd(&cell_count[c], 1u);
             }
 }
 
-__global__ void inclusive_from_exclusive_kernel(const uint32_t* __restrict__ ex, const uint32_t* __restrict__ cnt,
-                                                int64_t n, int* __restrict__ incl, uint32_t* __restrict__ flags) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    incl[i] = (int)(ex[i] + cnt[i]);
-    flags[i] = cnt[i] ? 1u : 0u;
+// 64-bit inclusive scan of the per-cell counts (tiles of 2048 cells): block
+// sums, one-block scan of the sums, then each tile scanned with its offset.
+constexpr int SCAN64_THREADS = 256;
+constexpr int SCAN64_ITEMS = 8;
+constexpr int SCAN64_TILE = SCAN64_THREADS * SCAN64_ITEMS;
+
+__device__ __forceinline__ unsigned long long block_exclusive_scan64(unsigned long long v, unsigned long long* sh,
+                                                                     unsigned long long* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            const unsigned long long c = sh[w];
+            sh[w] = run;
+            run += c;
+        }
+        sh[16] = run;
+    }
+    __syncthreads();
+    const unsigned long long r = sh[wave] + x - v;
+    if (total) *total = sh[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(SCAN64_THREADS) scan64_reduce_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                       unsigned long long* __restrict__ sums) {
+    __shared__ unsigned long long sh[17];
+    const int64_t base = (int64_t)blockIdx.x * SCAN64_TILE + (int64_t)threadIdx.x * SCAN64_ITEMS;
+    unsigned long long s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN64_ITEMS; i++)
+        if (base + i < n) s += in[base + i];
+    unsigned long long tot;
+    block_exclusive_scan64(s, sh, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN64_THREADS) scan64_blocks_kernel(unsigned long long* __restrict__ sums,
+                                                                       int64_t nb) {
+    __shared__ unsigned long long sh[17];
+    unsigned long long carry = 0;
+    for (int64_t base = 0; base < nb; base += SCAN64_THREADS) {
+        const int64_t i = base + threadIdx.x;
+        const unsigned long long v = i < nb ? sums[i] : 0ull;
+        unsigned long long tot;
+        const unsigned long long ex = block_exclusive_scan64(v, sh, &tot);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// prefix[c] = pairs in cells 0..c (inclusive), flags[c] = cell c holds any.
+__global__ void __launch_bounds__(SCAN64_THREADS) scan64_apply_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                                      const unsigned long long* __restrict__ block_off,
+                                                                      int64_t* __restrict__ prefix,
+                                                                      uint32_t* __restrict__ flags) {
+    __shared__ unsigned long long sh[17];
+    const int64_t base = (int64_t)blockIdx.x * SCAN64_TILE + (int64_t)threadIdx.x * SCAN64_ITEMS;
+    uint32_t v[SCAN64_ITEMS];
+    unsigned long long s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN64_ITEMS; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        s += v[i];
+    }
+    unsigned long long run = block_exclusive_scan64(s, sh, nullptr) + block_off[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN64_ITEMS; i++) {
+        run += v[i];
+        if (base + i < n) {
+            prefix[base + i] = (int64_t)run;
+            flags[base + i] = v[i] ? 1u : 0u;
+        }
+    }
 }
 
 __global__ void compact_valid_kernel(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos_ex,
@@ -191,7 +298,7 @@ __global__ void samples_kernel(const int* __restrict__ valid, int64_t nv, GridGe
 // Density sample: a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d),
 // feature = sum a_g sh_g, opacity = sum a_g. One workgroup (64 lanes) per cell.
 __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __restrict__ valid,
-                                                     const int* __restrict__ prefix, const int* __restrict__ list,
+                                                     const int64_t* __restrict__ prefix, const int* __restrict__ list,
                                                      const float* __restrict__ samples, const float* __restrict__ pos,
                                                      const float* __restrict__ rot, const float* __restrict__ scale,
                                                      const float* __restrict__ opacity, const float* __restrict__ sh,
@@ -203,15 +310,15 @@ __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __re
     const int64_t v = blockIdx.x;
     if (v >= nv) return;
     const int c = valid[v];
-    const int beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
+    const int64_t beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
     const int64_t si = v * 64 + threadIdx.x;
     const float x0 = samples[3 * si], x1 = samples[3 * si + 1], x2 = samples[3 * si + 2];
     float acc[48];
 #pragma unroll
     for (int k = 0; k < 48; k++) acc[k] = 0.0f;
     float aop = 0.0f;
-    for (int b0 = beg; b0 < end; b0 += B) {
-        const int li = b0 + threadIdx.x;
+    for (int64_t b0 = beg; b0 < end; b0 += B) {
+        const int64_t li = b0 + threadIdx.x;
         if (li < end) {
             const int g = list[li];
             float q[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
@@ -255,7 +362,7 @@ __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __re
             }
         }
         __syncthreads();
-        const int m = min(B, end - b0);
+        const int m = (int)min((int64_t)B, end - b0);
         for (int j = 0; j < m; j++) {
             const float d0 = x0 - s_q[j][0], d1 = x1 - s_q[j][1], d2 = x2 - s_q[j][2];
             const float qf = s_q[j][3] * d0 * d0 + s_q[j][6] * d1 * d1 + s_q[j][8] * d2 * d2 +
@@ -318,21 +425,47 @@ int launch_sum_counts(const uint32_t* cnt, int64_t n, unsigned long long* out, h
     return 0;
 }
 
-int launch_box_emit(const float* aabbs, int64_t n, const GridGeom& gg, const uint32_t* offs, uint64_t* keys,
-                    uint32_t* vals, uint32_t* cell_count, hipStream_t st) {
-    const int64_t gc = (int64_t)gg.G * gg.G * gg.G;
-    hipLaunchKernelGGL(zero_words_kernel, dim3(std::min<unsigned>(nblk(gc, 256), 4096)), dim3(256), 0, st, cell_count,
-                       gc);
-    hipLaunchKernelGGL(box_emit_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, offs, keys, vals,
-                       cell_count);
+int launch_slab_counts(const float* aabbs, int64_t n, const GridGeom& gg, unsigned long long* slab, hipStream_t st) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<uint32_t*>(slab),
+                       (int64_t)2 * gg.G);
+    if (n > 0) hipLaunchKernelGGL(slab_count_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, slab);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int launch_cell_prefix(const uint32_t* cell_ex, const uint32_t* cell_count, int64_t gc, int* prefix, uint32_t* flags,
+int launch_zero_cells(uint32_t* cell_count, int64_t gc, hipStream_t st) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(std::min<unsigned>(nblk(gc, 256), 4096)), dim3(256), 0, st, cell_count,
+                       gc);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_box_count_slabs(const float* aabbs, int64_t n, const GridGeom& gg, int x0, int x1, uint32_t* cnt,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(box_count_slabs_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, x0, x1, cnt);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_box_emit_slabs(const float* aabbs, int64_t n, const GridGeom& gg, int x0, int x1, const uint32_t* offs,
+                          uint32_t* keys, uint32_t* vals, uint32_t* cell_count, hipStream_t st) {
+    hipLaunchKernelGGL(box_emit_slabs_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, aabbs, n, gg, x0, x1, offs, keys,
+                       vals, cell_count);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+size_t cell_prefix_workspace_words(int64_t gc) { return 2 * (size_t)((gc + SCAN64_TILE - 1) / SCAN64_TILE) + 2; }
+
+int launch_cell_prefix(const uint32_t* cell_count, int64_t gc, int64_t* prefix, uint32_t* flags, uint32_t* ws,
                        hipStream_t st) {
-    hipLaunchKernelGGL(inclusive_from_exclusive_kernel, dim3(nblk(gc, 256)), dim3(256), 0, st, cell_ex, cell_count, gc,
-                       prefix, flags);
+    const int64_t nb = (gc + SCAN64_TILE - 1) / SCAN64_TILE;
+    if (nb == 0) return 0;
+    unsigned long long* sums = reinterpret_cast<unsigned long long*>(ws);
+    hipLaunchKernelGGL(scan64_reduce_kernel, dim3((unsigned)nb), dim3(SCAN64_THREADS), 0, st, cell_count, gc, sums);
+    hipLaunchKernelGGL(scan64_blocks_kernel, dim3(1), dim3(SCAN64_THREADS), 0, st, sums, nb);
+    hipLaunchKernelGGL(scan64_apply_kernel, dim3((unsigned)nb), dim3(SCAN64_THREADS), 0, st, cell_count, gc,
+                       (const unsigned long long*)sums, prefix, flags);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -350,7 +483,7 @@ int launch_samples(const int* valid, int64_t nv, const GridGeom& gg, float* samp
     return 0;
 }
 
-int launch_density(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+int launch_density(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                    const float* pos, const float* rot, const float* scale, const float* opacity, const float* sh,
                    float lpf, float* feat, float* opac, hipStream_t st) {
     if (nv == 0) return 0;

@@ -20,13 +20,28 @@ int launch_aabbs(const float* pos, const float* rot, const float* scale, const f
 int launch_box_count(const float* aabbs, int64_t n, const GridGeom& gg, uint32_t* cnt, hipStream_t st);
 // *out (device, 8 bytes) = the 64-bit sum of cnt[0..n).
 int launch_sum_counts(const uint32_t* cnt, int64_t n, unsigned long long* out, hipStream_t st);
-int launch_box_emit(const float* aabbs, int64_t n, const GridGeom& gg, const uint32_t* offs, uint64_t* keys,
-                    uint32_t* vals, uint32_t* cell_count, hipStream_t st);
-int launch_cell_prefix(const uint32_t* cell_ex, const uint32_t* cell_count, int64_t gc, int* prefix, uint32_t* flags,
+// Pairs per emit + sort pass of the cell-list build (temporary buffers of
+// 16 bytes per pair), and the most one x-slab may hold (32-bit offsets).
+constexpr unsigned long long GRID_CHUNK_PAIRS = 1ull << 28;
+constexpr unsigned long long GRID_CHUNK_PAIRS_MAX = 0x7fffffffull;
+
+// Cell lists are built in chunks of whole x-slabs (cells x*G^2 .. (x+1)*G^2-1)
+// so that each chunk's pairs fit 32-bit sort offsets; the per-cell prefix is
+// 64-bit. slab: 2*G words (G u64 pair counts).
+int launch_slab_counts(const float* aabbs, int64_t n, const GridGeom& gg, unsigned long long* slab, hipStream_t st);
+int launch_zero_cells(uint32_t* cell_count, int64_t gc, hipStream_t st);
+int launch_box_count_slabs(const float* aabbs, int64_t n, const GridGeom& gg, int x0, int x1, uint32_t* cnt,
+                           hipStream_t st);
+int launch_box_emit_slabs(const float* aabbs, int64_t n, const GridGeom& gg, int x0, int x1, const uint32_t* offs,
+                          uint32_t* keys, uint32_t* vals, uint32_t* cell_count, hipStream_t st);
+// prefix[c] (int64) = pairs in cells 0..c, flags[c] = 1 if cell c holds any;
+// ws holds cell_prefix_workspace_words(gc) words.
+size_t cell_prefix_workspace_words(int64_t gc);
+int launch_cell_prefix(const uint32_t* cell_count, int64_t gc, int64_t* prefix, uint32_t* flags, uint32_t* ws,
                        hipStream_t st);
 int launch_compact_valid(const uint32_t* flags, const uint32_t* pos_ex, int64_t gc, int* valid, hipStream_t st);
 int launch_samples(const int* valid, int64_t nv, const GridGeom& gg, float* samples, hipStream_t st);
-int launch_density(int64_t nv, const int* valid, const int* prefix, const int* list, const float* samples,
+int launch_density(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                    const float* pos, const float* rot, const float* scale, const float* opacity, const float* sh,
                    float lpf, float* feat, float* opac, hipStream_t st);
 

@@ -1495,6 +1495,11 @@ __global__ void init_x_kernel(const SolverEntry* __restrict__ B) {
         *gn_done = 0;
         gs->iters_done = 0;
         gs->cg_total = 0;
+        // a barrier failure of an earlier persistent solve must not mark this one
+        GridBarrier* gb = const_cast<GridBarrier*>(E.gbar);
+        gb->count = 0u;
+        gb->gen = 0u;
+        gb->err = 0;
     }
 }
 

@@ -236,18 +236,28 @@ int arapgs_gaussian_aabbs(arapgs_ctx* ctx, const float* pos, const float* rot, c
 
 /* Cell lists + valid cells + 4^3 samples per valid cell (GaussianView.cpp:4021-4133,
  * GetBoxesGsGrid cudakdtree.cu:425-451). The grid owns its device buffers.
- * Synchronous (list and valid-cell counts are data dependent). */
+ * Synchronous (list and valid-cell counts are data dependent). The lists are
+ * built in chunks of whole x-slabs of cells (at most 2^28 cell-Gaussian pairs
+ * per chunk, 16 temporary bytes per pair) and addressed with 64-bit offsets,
+ * so the pair total is not limited to 2^31 (the reference's int prefix sums
+ * are); one slab may hold at most 2^31-1 pairs and n at most 2^31-1. */
 int arapgs_grid_build(arapgs_ctx* ctx, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num,
                       int padding, arapgs_grid** out);
 void arapgs_grid_destroy(arapgs_grid* grid);
-/* Device views of a grid: prefix int[G^3] (inclusive per-cell counts, the
- * reference's grid_gs_prefix_sum), list int[n_pairs] (grided_gs_idx), valid
- * int[n_valid] (valid_grid_idx), samples f32[n_valid*64*3] (sample_positions). */
-int arapgs_grid_info(const arapgs_grid* grid, int64_t* n_pairs, int64_t* n_valid, const int** prefix,
+/* Test hook with no counterpart in the reference: the number of cell-Gaussian
+ * pairs per chunk of later arapgs_grid_build calls on this context (0: the
+ * default 2^28). The lists do not depend on it. */
+int arapgs_set_grid_chunk(arapgs_ctx* ctx, int64_t pairs);
+/* Device views of a grid: prefix int64[G^3] (inclusive per-cell counts, the
+ * reference's grid_gs_prefix_sum widened to 64 bits), list int[n_pairs]
+ * (grided_gs_idx), valid int[n_valid] (valid_grid_idx), samples
+ * f32[n_valid*64*3] (sample_positions). */
+int arapgs_grid_info(const arapgs_grid* grid, int64_t* n_pairs, int64_t* n_valid, const int64_t** prefix,
                      const int** list, const int** valid, const float** samples);
 
 /* Density sample of every grid sample (the forward3d_grid pass,
- * GaussianView.cpp:4159-4186; formula defined by this library, see DESIGN):
+ * GaussianView.cpp:4159-4186; the reference kernel is not available, so the
+ * formula is this library's own):
  * a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d) over the cell's list,
  * feat f32[S*48] = sum a_g sh_g, opac f32[S] = sum a_g. */
 int arapgs_density_sample(arapgs_ctx* ctx, const arapgs_grid* grid, const float* pos, const float* rot,

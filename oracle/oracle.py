@@ -75,6 +75,8 @@ def _declare(L):
     L.orc_gn_jacobian_triplets.restype = C.c_long
     L.orc_gn_solve.argtypes = [C.POINTER(GNArgs), C.c_int, C.c_int, P, P]
     L.orc_gn_solve.restype = C.c_int
+    L.orc_gn_reference_cost.argtypes = [C.c_int]
+    L.orc_gn_reference_cost.restype = None
     L.orc_scene_aabb.argtypes = [P, C.c_long, C.c_int, P]
     L.orc_grid_reorder.argtypes = [P, C.c_long, P, C.c_float, C.c_int, P]
     L.orc_gaussian_aabbs.argtypes = [P, P, P, P, C.c_long, P]
@@ -260,7 +262,10 @@ class GNProblem:
         lib().orc_gn_jacobian_triplets(C.byref(self.args), _p(x), _p(r), _p(c), _p(v))
         return r, c, v
 
-    def solve(self, iters, stop_rule=True):
+    def solve(self, iters, stop_rule=True, reference_cost=False):
+        """reference_cost: every iteration repeats the reference's ordering +
+        two factorisations (Deform.cpp:131-132); same result, for timing."""
+        lib().orc_gn_reference_cost(int(bool(reference_cost)))
         x = np.zeros(self.n)
         e = np.zeros(iters + 1)
         it = lib().orc_gn_solve(C.byref(self.args), iters, int(stop_rule), _p(x), _p(e))

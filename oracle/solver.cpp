@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <iterator>
 #include <set>
 #include <vector>
 
@@ -287,47 +288,172 @@ bool dense_solve(int n, std::vector<double>& A, std::vector<double>& b) {
     return true;
 }
 
-// Sparse direct solve for large systems: reverse Cuthill-McKee ordering of the
-// node graph of J^T J (12x12 blocks), then an envelope (profile) Cholesky in
-// FP64 — the same class of solver as the reference's SimplicialCholesky
-// (sparse LDL^T with a fill-reducing ordering).
-struct Envelope {
-    int n = 0;
-    std::vector<int> first;      // first stored column of each row
-    std::vector<size_t> start;   // offset of row i (entry j at start[i] + j - first[i])
-    std::vector<double> v;
-    double& at(int i, int j) { return v[start[i] + (size_t)(j - first[i])]; }
-};
+// Sparse direct solve for large systems: a block Cholesky factorisation of
+// J^T J over the 12x12 node blocks, with a minimum-degree ordering of the node
+// graph — the class of solver of the reference's SimplicialCholesky (sparse
+// LDL^T after an approximate-minimum-degree ordering, Deform.cpp:103-133). The
+// solution of an SPD system does not depend on the factorisation used.
+struct BlockChol {
+    int nb = 0;
+    std::vector<int> pos_of;  // node -> elimination position
+    std::vector<int> cp, ri;  // strictly-lower block columns by position: rows ri[cp[p]..cp[p+1]) ascending
+    std::vector<double> L;    // 144 doubles (row-major 12x12) per stored block
+    std::vector<double> D;    // 144 doubles per diagonal block (lower Cholesky factor after factor())
 
-std::vector<int> rcm_order(int nb, const std::vector<std::vector<int>>& adj) {
-    std::vector<int> order, deg(nb);
-    std::vector<char> seen(nb, 0);
-    for (int i = 0; i < nb; i++) deg[i] = (int)adj[i].size();
-    for (int s0 = 0; s0 < nb; s0++) {
-        // start each component at a minimum-degree node
-        int best = -1;
-        for (int i = 0; i < nb; i++)
-            if (!seen[i] && (best < 0 || deg[i] < deg[best])) best = i;
-        if (best < 0) break;
-        size_t head = order.size();
-        order.push_back(best);
-        seen[best] = 1;
-        while (head < order.size()) {
-            const int u = order[head++];
-            std::vector<int> nb_list;
-            for (int w : adj[u])
-                if (!seen[w]) { seen[w] = 1; nb_list.push_back(w); }
-            std::sort(nb_list.begin(), nb_list.end(), [&](int a, int b) { return deg[a] < deg[b]; });
-            order.insert(order.end(), nb_list.begin(), nb_list.end());
+    // Exact minimum-degree elimination of the node graph (ties: lowest node
+    // id). The neighbours of a node when it is eliminated are the rows of its
+    // column of L, so the symbolic factorisation falls out of the ordering.
+    void analyse(int nblocks, std::vector<std::vector<int>> adj) {
+        nb = nblocks;
+        pos_of.assign(nb, -1);
+        std::vector<int> order;
+        std::vector<std::vector<int>> reach(nb);
+        std::vector<int> merged;
+        for (int step = 0; step < nb; step++) {
+            int best = -1;
+            size_t bd = (size_t)-1;
+            for (int i = 0; i < nb; i++)
+                if (pos_of[i] < 0 && adj[i].size() < bd) { bd = adj[i].size(); best = i; }
+            pos_of[best] = step;
+            order.push_back(best);
+            reach[best] = adj[best];
+            const std::vector<int>& nv = reach[best];
+            for (int u : nv) {
+                merged.clear();
+                std::set_union(adj[u].begin(), adj[u].end(), nv.begin(), nv.end(), std::back_inserter(merged));
+                adj[u].clear();
+                for (int w : merged)
+                    if (w != u && w != best) adj[u].push_back(w);
+            }
+            adj[best].clear();
+            adj[best].shrink_to_fit();
+        }
+        cp.assign(nb + 1, 0);
+        ri.clear();
+        for (int p = 0; p < nb; p++) {
+            std::vector<int> rows;
+            for (int u : reach[order[p]]) rows.push_back(pos_of[u]);
+            std::sort(rows.begin(), rows.end());
+            ri.insert(ri.end(), rows.begin(), rows.end());
+            cp[p + 1] = (int)ri.size();
+        }
+        L.assign(ri.size() * 144, 0.0);
+        D.assign((size_t)nb * 144, 0.0);
+    }
+
+    double* block(int prow, int pcol) {
+        if (prow == pcol) return &D[(size_t)prow * 144];
+        const int* b = &ri[cp[pcol]];
+        const int* e = &ri[cp[pcol + 1]];
+        const int* it = std::lower_bound(b, e, prow);
+        return &L[(size_t)(it - ri.data()) * 144];
+    }
+
+    // A = sum over merged J rows of row^T row (lower triangle of the permuted matrix).
+    void assemble(const std::vector<std::vector<std::pair<int, double>>>& rows) {
+        std::fill(L.begin(), L.end(), 0.0);
+        std::fill(D.begin(), D.end(), 0.0);
+        for (const auto& r : rows)
+            for (const auto& a : r)
+                for (const auto& b : r) {
+                    const int pa = pos_of[a.first / 12], pb = pos_of[b.first / 12];
+                    if (pa < pb) continue;
+                    block(pa, pb)[(a.first % 12) * 12 + b.first % 12] += a.second * b.second;
+                }
+    }
+
+    // Right-looking block Cholesky: L_pp = chol(A_pp), L_up = A_up L_pp^-T,
+    // A_uw -= L_up L_wp^T for u >= w in the column's rows.
+    bool factor() {
+        for (int p = 0; p < nb; p++) {
+            double* d = &D[(size_t)p * 144];
+            for (int j = 0; j < 12; j++) {
+                double s = d[j * 12 + j];
+                for (int k = 0; k < j; k++) s -= d[j * 12 + k] * d[j * 12 + k];
+                if (!(s > 0.0)) return false;
+                s = std::sqrt(s);
+                d[j * 12 + j] = s;
+                for (int i = j + 1; i < 12; i++) {
+                    double t = d[i * 12 + j];
+                    for (int k = 0; k < j; k++) t -= d[i * 12 + k] * d[j * 12 + k];
+                    d[i * 12 + j] = t / s;
+                }
+                for (int k = j + 1; k < 12; k++) d[j * 12 + k] = 0.0;
+            }
+            const int c0 = cp[p], c1 = cp[p + 1];
+#pragma omp parallel for schedule(static)
+            for (int q = c0; q < c1; q++) {
+                double* l = &L[(size_t)q * 144];
+                for (int i = 0; i < 12; i++)
+                    for (int j = 0; j < 12; j++) {
+                        double t = l[i * 12 + j];
+                        for (int k = 0; k < j; k++) t -= l[i * 12 + k] * d[j * 12 + k];
+                        l[i * 12 + j] = t / d[j * 12 + j];
+                    }
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int qw = c0; qw < c1; qw++) {
+                const int w = ri[qw];
+                const double* lw = &L[(size_t)qw * 144];
+                for (int qu = qw; qu < c1; qu++) {
+                    const int u = ri[qu];
+                    const double* lu = &L[(size_t)qu * 144];
+                    double* t = block(u, w);
+                    for (int i = 0; i < 12; i++)
+                        for (int j = 0; j < 12; j++) {
+                            double s = 0.0;
+                            for (int k = 0; k < 12; k++) s += lu[i * 12 + k] * lw[j * 12 + k];
+                            t[i * 12 + j] -= s;
+                        }
+                }
+            }
+        }
+        return true;
+    }
+
+    // y = A^-1 y in the permuted order (forward then backward substitution).
+    void solve(std::vector<double>& y) const {
+        for (int p = 0; p < nb; p++) {
+            const double* d = &D[(size_t)p * 144];
+            double* yp = &y[(size_t)p * 12];
+            for (int i = 0; i < 12; i++) {
+                double s = yp[i];
+                for (int k = 0; k < i; k++) s -= d[i * 12 + k] * yp[k];
+                yp[i] = s / d[i * 12 + i];
+            }
+            for (int q = cp[p]; q < cp[p + 1]; q++) {
+                const double* l = &L[(size_t)q * 144];
+                double* yu = &y[(size_t)ri[q] * 12];
+                for (int i = 0; i < 12; i++) {
+                    double s = 0.0;
+                    for (int k = 0; k < 12; k++) s += l[i * 12 + k] * yp[k];
+                    yu[i] -= s;
+                }
+            }
+        }
+        for (int p = nb - 1; p >= 0; p--) {
+            double* yp = &y[(size_t)p * 12];
+            for (int q = cp[p]; q < cp[p + 1]; q++) {
+                const double* l = &L[(size_t)q * 144];
+                const double* yu = &y[(size_t)ri[q] * 12];
+                for (int k = 0; k < 12; k++) {
+                    double s = 0.0;
+                    for (int i = 0; i < 12; i++) s += l[i * 12 + k] * yu[i];
+                    yp[k] -= s;
+                }
+            }
+            const double* d = &D[(size_t)p * 144];
+            for (int i = 11; i >= 0; i--) {
+                double s = yp[i];
+                for (int k = i + 1; k < 12; k++) s -= d[k * 12 + i] * yp[k];
+                yp[i] = s / d[i * 12 + i];
+            }
         }
     }
-    std::reverse(order.begin(), order.end());
-    return order;
-}
+};
 
-// Solve (J^T J) h = rhs from merged rows of J (each row: list of (col, val)).
-bool envelope_solve(int n, const std::vector<std::vector<std::pair<int, double>>>& rows, std::vector<double>& rhs) {
-    const int nb = n / 12;
+// Node adjacency of J^T J from merged rows of J.
+std::vector<std::vector<int>> node_graph(int nb, const std::vector<std::vector<std::pair<int, double>>>& rows) {
     std::vector<std::vector<int>> adj(nb);
     for (const auto& r : rows)
         for (size_t a = 0; a < r.size(); a++)
@@ -339,73 +465,38 @@ bool envelope_solve(int n, const std::vector<std::vector<std::pair<int, double>>
         std::sort(a.begin(), a.end());
         a.erase(std::unique(a.begin(), a.end()), a.end());
     }
-    const std::vector<int> ord = rcm_order(nb, adj);
-    std::vector<int> inv(nb);
-    for (int i = 0; i < nb; i++) inv[ord[i]] = i;
-    auto perm = [&](int c) { return inv[c / 12] * 12 + c % 12; };
-    Envelope E;
-    E.n = n;
-    E.first.assign(n, 0);
-    for (int i = 0; i < n; i++) E.first[i] = i;
-    for (const auto& r : rows)
-        for (const auto& a : r)
-            for (const auto& b : r) {
-                const int pa = perm(a.first), pb = perm(b.first);
-                if (pb < pa) E.first[pa] = std::min(E.first[pa], pb);
-            }
-    E.start.assign(n + 1, 0);
-    for (int i = 0; i < n; i++) E.start[i + 1] = E.start[i] + (size_t)(i - E.first[i] + 1);
-    E.v.assign(E.start[n], 0.0);
-    for (const auto& r : rows)
-        for (const auto& a : r)
-            for (const auto& b : r) {
-                const int pa = perm(a.first), pb = perm(b.first);
-                if (pb <= pa) E.at(pa, pb) += a.second * b.second;
-            }
-    // envelope Cholesky, row by row
-    for (int i = 0; i < n; i++) {
-        const int fi = E.first[i];
-        double* Li = &E.v[E.start[i]];
-        for (int j = fi; j <= i; j++) {
-            const int fj = E.first[j];
-            const int k0 = std::max(fi, fj);
-            const double* Lj = &E.v[E.start[j]];
-            double s = Li[j - fi];
-            const double* a = Li + (k0 - fi);
-            const double* b = Lj + (k0 - fj);
-            const int len = j - k0;
-            double acc = 0.0;
-            for (int k = 0; k < len; k++) acc += a[k] * b[k];
-            s -= acc;
-            if (j < i) Li[j - fi] = s / Lj[j - fj];
-            else {
-                if (!(s > 0.0)) return false;
-                Li[j - fi] = std::sqrt(s);
-            }
-        }
+    return adj;
+}
+
+// When set, every Gauss-Newton iteration repeats the reference's work pattern
+// (Deform.cpp:131-132: SimplicialCholesky::compute, i.e. ordering + symbolic +
+// numeric factorisation, then factorize() again); the result is the same.
+int g_reference_cost = 0;
+
+// Solve (J^T J) h = rhs from merged rows of J; the ordering and the symbolic
+// factorisation are computed on the first call for a structure and reused.
+bool sparse_solve(int n, const std::vector<std::vector<std::pair<int, double>>>& rows, std::vector<double>& rhs,
+                  BlockChol& F) {
+    const int nb = n / 12;
+    if (F.nb != nb || g_reference_cost) F.analyse(nb, node_graph(nb, rows));
+    F.assemble(rows);
+    if (!F.factor()) return false;
+    if (g_reference_cost) {
+        F.assemble(rows);
+        if (!F.factor()) return false;
     }
-    std::vector<double> y(n);
-    for (int c = 0; c < n; c++) y[perm(c)] = rhs[c];
-    for (int i = 0; i < n; i++) {
-        const int fi = E.first[i];
-        const double* Li = &E.v[E.start[i]];
-        double s = y[i];
-        for (int k = fi; k < i; k++) s -= Li[k - fi] * y[k];
-        y[i] = s / Li[i - fi];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-        y[i] /= E.v[E.start[i] + (size_t)(i - E.first[i])];
-        const int fi = E.first[i];
-        const double* Li = &E.v[E.start[i]];
-        for (int k = fi; k < i; k++) y[k] -= Li[k - fi] * y[i];
-    }
-    for (int c = 0; c < n; c++) rhs[c] = y[perm(c)];
+    std::vector<double> y((size_t)n);
+    for (int c = 0; c < n; c++) y[(size_t)F.pos_of[c / 12] * 12 + c % 12] = rhs[c];
+    F.solve(y);
+    for (int c = 0; c < n; c++) rhs[c] = y[(size_t)F.pos_of[c / 12] * 12 + c % 12];
     return true;
 }
 
 }  // namespace
 
 extern "C" {
+
+void orc_gn_reference_cost(int on) { g_reference_cost = on; }
 
 struct orc_gn_args {
     int M, K;
@@ -474,9 +565,10 @@ int orc_gn_solve(const orc_gn_args* a, int max_iters, int stop_rule, double* x, 
     const long m = P.rows;
     for (int i = 0; i < n; i++) x[i] = 0.0;
     for (int i = 0; i < n / 12; i++) x[12 * i] = x[12 * i + 4] = x[12 * i + 8] = 1.0;
-    const bool sparse = n > 3072;
+    const bool sparse = n > 1536;
     std::vector<double> f(m), f1(m), xt(n), A(sparse ? 0 : (size_t)n * n), g(n);
     std::vector<Trip> T;
+    BlockChol chol;
     int it = 0;
     for (; it < max_iters; it++) {
         jacobian(P, x, T);
@@ -513,7 +605,7 @@ int orc_gn_solve(const orc_gn_args* a, int max_iters, int stop_rule, double* x, 
         for (long r = 0; r < m; r++) old_e += f[r] * f[r];
         energies[it] = old_e;
         std::vector<double> h(g);
-        if (sparse ? !envelope_solve(n, jrows, h) : !dense_solve(n, A, h)) return -1;
+        if (sparse ? !sparse_solve(n, jrows, h, chol) : !dense_solve(n, A, h)) return -1;
         double normv = 0.0;
         for (int i = 0; i < n; i++) normv += x[i] * x[i];
         normv = std::sqrt(normv);

@@ -205,6 +205,21 @@ def test_gn_matches_dense_restatement():
     assert np.all(np.diff(e) <= 0)
 
 
+def test_sparse_path_matches_dense_restatement():
+    """Above 1536 unknowns the oracle switches to its minimum-degree block
+    Cholesky (with and without the reference's per-iteration re-analysis and
+    second factorisation); iterates must equal a dense numpy GN."""
+    blocks = [np.arange(0, 20), np.arange(20, 40), np.arange(40, 43)]
+    prob, ref = tiny(seed=6, M=160, K=6, blocks=blocks, types=[0, 1, -1])
+    assert prob.n > 1536
+    x_ref, e_ref = gn_numpy(prob, ref, 4)
+    for ref_cost in (False, True):
+        x, e, it = prob.solve(4, reference_cost=ref_cost)
+        assert it == len(e_ref) - 1
+        np.testing.assert_allclose(e, e_ref, rtol=1e-9, atol=1e-14)
+        np.testing.assert_allclose(x, x_ref, atol=1e-8)
+
+
 def test_zero_energy_at_identity_when_aims_are_nodes():
     prob, ref = tiny(seed=5, aims_at_nodes=True)
     x0 = prob.identity_x()

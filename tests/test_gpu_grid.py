@@ -1,5 +1,8 @@
 """GPU parity of the spatial grid (AABB, reorder, cell lists, valid cells,
-samples: all bit-exact) and the density sample (FP32 vs the oracle's FP64)."""
+samples: all bit-exact) and the density sample (FP32 vs the oracle's FP64).
+The cell lists are built in chunks of whole x-slabs; a tiny chunk size makes
+one build run many chunks (one slab per chunk, and slabs holding more pairs
+than the chunk size), which must give the same lists."""
 import numpy as np
 import pytest
 
@@ -18,8 +21,8 @@ def H(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.mark.parametrize("G", [64, 32])
-def test_grid_exact(ctx, G):
+@pytest.mark.parametrize("G,chunk", [(64, 0), (32, 0), (64, 20000), (32, 1)])
+def test_grid_exact(ctx, G, chunk):
     sc = S.bar_scene(1, n=30000, n_nodes=64)
     g = sc.gaussians
     mn, mx, step = O.scene_aabb(g.pos, G)
@@ -31,8 +34,13 @@ def test_grid_exact(ctx, G):
     ab = H(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)))
     ab_ref = O.gaussian_aabbs(g.pos, g.rot, g.scale, g.opacity)
     assert np.array_equal(ab, ab_ref)
-    grid = ctx.grid_build(T(ab), a7, G, padding=1)
+    ctx.set_grid_chunk(chunk)
+    try:
+        grid = ctx.grid_build(T(ab), a7, G, padding=1)
+    finally:
+        ctx.set_grid_chunk(0)
     prefix, lst, valid, samples = grid.arrays()
+    assert prefix.dtype == np.int64 and prefix[-1] == grid.n_pairs == lst.shape[0]
     rp, rl, rv, rs = O.grid_lists(ab_ref, mn, step, G, padding=1)
     assert np.array_equal(prefix, rp)
     assert np.array_equal(lst, rl)
