@@ -55,6 +55,10 @@ class ChainDesc(C.Structure):
                 ("tan_fovx_h", P), ("tan_fovy_h", P), ("axis_padding", C.c_float), ("end_coeff", C.c_float)]
 
 
+class RealignOptions(C.Structure):
+    _fields_ = [("moved", P), ("max_scale", P), ("conv_tol", C.c_float)]
+
+
 class RealignDesc(C.Structure):
     _fields_ = [("lr_pos", C.c_float), ("lr_rot", C.c_float), ("lr_scale", C.c_float), ("lr_opacity", C.c_float),
                 ("lr_sh", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
@@ -85,12 +89,16 @@ EXPORTS = {
     "arapgs_solver_batch_destroy": ([P], None),
     "arapgs_solver_batch_set_host_checks": ([P, C.c_int], C.c_int),
     "arapgs_solver_batch_set_coarse_refresh": ([P, C.c_int], C.c_int),
+    "arapgs_solver_batch_set_async_refresh": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_cg_block": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_async_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_batch_solve": ([P, P, P, P, C.c_int, C.c_int, P, P, P, P], C.c_int),
     "arapgs_blend_points": ([P, P, C.c_int64, P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_six_point_update": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float, P, P, P, P],
                                 C.c_int),
     "arapgs_node_quats": ([P, P, C.c_int, P], C.c_int),
     "arapgs_sample_sh": ([P, C.c_int64, C.c_int, P, P, P, P, P], C.c_int),
+    "arapgs_samples_frames": ([P, C.c_int64, C.c_int, C.c_int, P, P, P, P, P, P, P], C.c_int),
     "arapgs_render": ([P, C.POINTER(RenderDesc), C.POINTER(C.c_int)], C.c_int),
     "arapgs_render_debug": ([P, P, P, C.c_int64, P, C.c_int64], C.c_int),
     "arapgs_scene_aabb": ([P, P, C.c_int64, C.c_int, P], C.c_int),
@@ -98,6 +106,7 @@ EXPORTS = {
     "arapgs_gaussian_aabbs": ([P, P, P, P, P, C.c_int64, P], C.c_int),
     "arapgs_grid_build": ([P, P, C.c_int64, P, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_grid_destroy": ([P], None),
+    "arapgs_grid_rebin": ([P, P, P, C.c_int64, P], C.c_int),
     "arapgs_grid_info": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P), C.POINTER(P),
                           C.POINTER(P)], C.c_int),
     "arapgs_density_sample": ([P, P, P, P, P, P, P, C.c_float, P, P], C.c_int),
@@ -105,6 +114,7 @@ EXPORTS = {
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
+    "arapgs_solver_reset": ([P], C.c_int),
     "arapgs_solver_last_stats": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "arapgs_solver_apply": ([P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_deform_frame": ([P, P, C.POINTER(FrameBuffers), P, C.c_int, C.c_int, P, P], C.c_int),
@@ -115,6 +125,7 @@ EXPORTS = {
     "arapgs_aims_set": ([P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_profile_enable": ([P, C.c_int], C.c_int),
     "arapgs_profile_read": ([P, P, C.c_int], C.c_int),
+    "arapgs_profile_read_kernel": ([P, C.c_int, P, C.c_int], C.c_int),
     "arapgs_sh_rotate": ([P, C.c_int64, P, P], C.c_int),
     "arapgs_polar": ([P, C.c_int64, P, P, P], C.c_int),
     "arapgs_tile_test_host": ([C.c_int64, P, P, P, P], C.c_int),
@@ -125,6 +136,8 @@ EXPORTS = {
     "arapgs_realign_create": ([P, P, C.c_int64, C.POINTER(RealignDesc), C.POINTER(P)], C.c_int),
     "arapgs_realign_destroy": ([P], None),
     "arapgs_realign_step": ([P, P, P, P, P, C.c_float, P, P, P, P, P, P], C.c_int),
+    "arapgs_realign_set_options": ([P, P, P], C.c_int),
+    "arapgs_realign_cells": ([P, C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P)], C.c_int),
     "arapgs_chain_create": ([P, C.POINTER(ChainDesc), C.POINTER(P)], C.c_int),
     "arapgs_chain_destroy": ([P], None),
     "arapgs_chain_forward": ([P, P, P, P, P, P, P, P], C.c_int),
@@ -278,11 +291,20 @@ class Context:
                                                   ptr(ends_w), ptr(ends), ptr(scale0), ptr(is_static), pad, coeff,
                                                   ptr(pos), ptr(rot), ptr(scale), ptr(sh)))
 
-    def node_quats(self, table):
+    def node_quats(self, table, out=None):
         t = self._torch
-        q = t.empty((table.shape[0], 4), dtype=t.float32, device=table.device)
+        q = t.empty((table.shape[0], 4), dtype=t.float32, device=table.device) if out is None else out
         self._check(lib().arapgs_node_quats(self.h, ptr(table), table.shape[0], ptr(q)))
         return q
+
+    def samples_frames(self, tables, node_qs, idx, w, is_static, samples, feat):
+        """len(tables) consecutive frames of the sample blend + sample SH in one
+        pass (arapgs_samples_frames; bit-identical to the per-frame calls)."""
+        nf = len(tables)
+        tp = _ptr_array(tables)
+        qp = _ptr_array(node_qs)
+        self._check(lib().arapgs_samples_frames(self.h, samples.shape[0], idx.shape[1], nf, tp, qp, ptr(idx), ptr(w),
+                                                ptr(is_static), ptr(samples), ptr(feat)))
 
     def sample_sh(self, w, idx, node_q, is_static, feat):
         self._check(lib().arapgs_sample_sh(self.h, feat.shape[0], idx.shape[-1], ptr(w), ptr(idx), ptr(node_q),
@@ -290,8 +312,9 @@ class Context:
 
     def render(self, means, shs, opac, scales, rots, view, proj, campos, tanfovx, tanfovy, W, H, bg, D=3,
                scale_mod=1.0, fast_culling=True, boxmin=None, boxmax=None, tile=16, out=None, want_surface=False,
-               want_radii=False):
-        """Rasterizer::forward (GaussianView.cpp:1099-1128). view/proj/campos/bg are device tensors."""
+               want_radii=False, ortho_scale=None):
+        """Rasterizer::forward (GaussianView.cpp:1099-1128). view/proj/campos/bg are device tensors.
+        ortho_scale: orthographic camera (to_ortho with that ortho_scale; proj must be its matrix)."""
         t = self._torch
         dev = means.device
         P_ = means.shape[0]
@@ -307,7 +330,8 @@ class Context:
             bmin, bmax = self._box[0].ctypes.data_as(P), self._box[1].ctypes.data_as(P)
         d = RenderDesc(P_, D, 16, ptr(bg), W, H, ptr(means), ptr(shs), None, ptr(opac), ptr(scales), scale_mod,
                        ptr(rots), None, ptr(view), ptr(proj), ptr(campos), tanfovx, tanfovy, 0, ptr(out), ptr(surface),
-                       ptr(radii), ptr(rects), bmin, bmax, 0, 1.0, tile)
+                       ptr(radii), ptr(rects), bmin, bmax, int(ortho_scale is not None), float(ortho_scale or 1.0),
+                       tile)
         n = C.c_int(0)
         self._check(lib().arapgs_render(self.h, C.byref(d), C.byref(n)))
         return dict(image=out, num_rendered=n.value, surface=surface, radii=radii, rects=rects)
@@ -473,10 +497,12 @@ class Context:
     def profile_enable(self, on=True):
         self._check(lib().arapgs_profile_enable(self.h, int(on)))
 
-    def profile_read(self, cap=4096):
+    def profile_read(self, cap=4096, which=0):
+        """Per-launch durations (ms) of the six-point (0), sample blend (1) or
+        sample SH (2) launches of deform_frame since profiling was enabled."""
         import numpy as np
         buf = np.zeros(cap, np.float32)
-        n = lib().arapgs_profile_read(self.h, buf.ctypes.data_as(P), cap)
+        n = lib().arapgs_profile_read_kernel(self.h, which, buf.ctypes.data_as(P), cap)
         if n < 0:
             self._check(n)
         return buf[:n]
@@ -585,6 +611,15 @@ class Solver:
     def set_coarse_refresh(self, frames):
         self.ctx._check(lib().arapgs_solver_set_coarse_refresh(self.h, frames))
 
+    def set_cg_block(self, threads):
+        """Workgroup size of the CG operator kernel (64, 128, 256)."""
+        if lib().arapgs_solver_set_cg_block(self.h, int(threads)) != 0:
+            raise ArapgsError(f"set_cg_block({threads}) failed")
+
+    def set_async_refresh(self, on=True):
+        """Coarse rebuilds on the solver's side stream, adopted by the next solve (default on)."""
+        self.ctx._check(lib().arapgs_solver_set_async_refresh(self.h, int(bool(on))))
+
     def apply(self, node_pos, aims, x, v):
         """(J^T J v, -J^T f) at unknowns x (test hook)."""
         t = self.ctx._torch
@@ -604,6 +639,12 @@ class Solver:
     def set_warm_start(self, on=True):
         if lib().arapgs_solver_set_warm_start(self.h, int(bool(on))) != 0:
             raise ArapgsError("set_warm_start failed")
+
+    def reset(self):
+        """Forget the frame-to-frame state (warm-start steps, coarse operator,
+        refresh count): the next solve computes what a new solver's would."""
+        if lib().arapgs_solver_reset(self.h) != 0:
+            raise ArapgsError("solver reset failed")
 
     def set_cg_mode(self, mode):
         """0 = three kernels per CG iteration (default), 1 = persistent cooperative PCG kernel."""
@@ -686,12 +727,28 @@ class Grid:
 
     def __init__(self, ctx, h, G):
         self.ctx, self.h, self.G = ctx, h, G
+        self._info()
+
+    def _info(self):
         n_pairs, n_valid = C.c_int64(0), C.c_int64(0)
         pr, li, va, sa = P(), P(), P(), P()
-        lib().arapgs_grid_info(h, C.byref(n_pairs), C.byref(n_valid), C.byref(pr), C.byref(li), C.byref(va),
+        lib().arapgs_grid_info(self.h, C.byref(n_pairs), C.byref(n_valid), C.byref(pr), C.byref(li), C.byref(va),
                                C.byref(sa))
         self.n_pairs, self.n_valid = n_pairs.value, n_valid.value
         self._ptrs = (pr.value, li.value, va.value, sa.value)
+
+    def rebin(self, pos, rot, scale, opacity):
+        """UpdateContainingRelationship (GaussianView.cpp:3634-3744): cell lists of
+        the Gaussians' current (activated) state on the geometry of their current
+        scene box; valid cells and samples are kept. Returns the geometry (aabb7)."""
+        import numpy as np
+        ctx = self.ctx
+        a7 = ctx.scene_aabb(pos, self.G)
+        aabbs = ctx.gaussian_aabbs(pos, rot, scale, opacity)
+        a = np.ascontiguousarray(a7, np.float32)
+        ctx._check(lib().arapgs_grid_rebin(ctx.h, self.h, ptr(aabbs), aabbs.shape[0], a.ctypes.data_as(P)))
+        self._info()
+        return a7
 
     def arrays(self):
         """Copies of (prefix, list, valid, samples) on the host, via torch."""
@@ -910,6 +967,28 @@ class Realign:
         ctx._check(lib().arapgs_realign_create(ctx.h, grid.h, n, C.byref(d), C.byref(h)))
         self.h = h
 
+    def set_options(self, moved=None, max_scale=None, conv_tol=0.0):
+        """moved (u8 device tensor, N): only these Gaussians move and cells without
+        them are static; max_scale (f32, N x 3): clamp of the activated scales;
+        conv_tol: per-cell convergence (0: off). See arapgs_realign_set_options."""
+        self._opts = (moved, max_scale)
+        o = RealignOptions(ptr(moved), ptr(max_scale), float(conv_tol))
+        self.ctx._check(lib().arapgs_realign_set_options(self.ctx.h, self.h, C.byref(o)))
+
+    def cells(self):
+        """(skip u8[n_valid], cell L1 loss f32[n_valid]) after the last step, on the host."""
+        import numpy as np
+        import torch
+        nv, sk, cl = C.c_int64(0), P(), P()
+        lib().arapgs_realign_cells(self.h, C.byref(nv), C.byref(sk), C.byref(cl))
+        skip = np.empty(nv.value, np.uint8)
+        loss = np.empty(nv.value, np.float32)
+        torch.cuda.synchronize()
+        if nv.value:
+            _hip_memcpy_d2h(skip.ctypes.data, sk.value, skip.nbytes)
+            _hip_memcpy_d2h(loss.ctypes.data, cl.value, loss.nbytes)
+        return skip, loss
+
     def step(self, samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, loss=None):
         self.ctx._check(lib().arapgs_realign_step(self.ctx.h, self.h, ptr(samples), ptr(aim_feat), ptr(aim_dens), lpf,
                                                   ptr(pos), ptr(rot), ptr(log_scale), ptr(logit_opacity), ptr(sh),
@@ -925,3 +1004,21 @@ class Realign:
             self.close()
         except Exception:
             pass
+
+
+def realign_loop(ctx, grid, realign, samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, steps,
+                 rebin_interval=10, loss=None, on_step=None):
+    """The F6 handler of GaussianView (GaussianView.cpp:1695-1733) over the
+    library: the cell lists are rebuilt from the Gaussians' current state
+    before the loop (UpdateContainingRelationshipWithStaticGrids) and every
+    rebin_interval steps after the first (UpdateContainingRelationship,
+    _containing_relation_interval = 10 by default, GaussianView.hpp:427), and
+    every step is one arapgs_realign_step (GPUOptimize). on_step(i, loss) is
+    called after each step when given (it reads the loss: one sync per step)."""
+    import torch
+    for i in range(steps):
+        if i % max(1, rebin_interval) == 0:
+            grid.rebin(pos, rot, torch.exp(log_scale), torch.sigmoid(logit_opacity))
+        realign.step(samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, loss=loss)
+        if on_step is not None:
+            on_step(i, loss)

@@ -8,6 +8,7 @@ its tolerance:
   refresh       solves between rebuilds of the coarse operator
   host_checks   CG iterations between host reads of the convergence flag
   warm          CG warm start from the previous frame's GN steps (1) or from 0
+  cg_block      workgroup size of the CG operator kernel: 256, 128 or 64 threads
 
 `tune` runs a coordinate descent from the defaults: for each parameter in turn
 it times every candidate value (the others held at their current best) with a
@@ -19,10 +20,10 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1)
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(4, 8, 16, 32),
-                  host_checks=(4, 8, 16, 32), warm=(1, 0))
-ORDER = ("warm", "aggregates", "refresh", "host_checks", "tile", "blend_unroll")
+                  host_checks=(4, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64))
+ORDER = ("warm", "aggregates", "refresh", "host_checks", "cg_block", "tile", "blend_unroll")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

@@ -100,13 +100,19 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
                                                               const float* __restrict__ dfeat,
                                                               const float* __restrict__ ddens,
                                                               float* __restrict__ gsamp, float* __restrict__ gdens,
-                                                              int64_t gstride) {
+                                                              int64_t gstride, const uint8_t* __restrict__ skip) {
 #pragma clang fp contract(fast)
     __shared__ float s_x[64][3];
     __shared__ float s_df[64][49];
     __shared__ float s_dx[64][3];
     const int64_t v = blockIdx.x;
     if (v >= nv) return;
+    if (skip && skip[v]) {  // a static or converged cell passes no gradient
+        if (gsamp)
+#pragma unroll
+            for (int k = 0; k < 3; k++) gsamp[3 * (v * 64 + threadIdx.x) + k] = 0.0f;
+        return;
+    }
     const int t = threadIdx.x;
     const int c = valid[v];
     const int64_t beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
@@ -556,7 +562,7 @@ __global__ void __launch_bounds__(256) l1_grad_kernel(int64_t S, const float* __
                                                       const float* __restrict__ dens, const float* __restrict__ aim_feat,
                                                       const float* __restrict__ aim_dens, float w,
                                                       float* __restrict__ dfeat, float* __restrict__ ddens,
-                                                      double* __restrict__ loss) {
+                                                      double* __restrict__ loss, float* __restrict__ cell_loss) {
     __shared__ double s_red[4];
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double acc = 0.0;
@@ -573,9 +579,48 @@ __global__ void __launch_bounds__(256) l1_grad_kernel(int64_t S, const float* __
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    // one wave = the 64 samples of one valid cell
+    if ((threadIdx.x & 63) == 0 && cell_loss && s < S) cell_loss[s >> 6] = (float)acc;
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0 && loss) unsafeAtomicAdd(loss, s_red[0] + s_red[1] + s_red[2] + s_red[3]);
+}
+
+// Cells whose lists hold no moved Gaussian (the reference's current static
+// grids, GaussianView.cpp:3850-3852): one wave per valid cell.
+__global__ void __launch_bounds__(64) static_cells_kernel(int64_t nv, const int* __restrict__ valid,
+                                                          const int64_t* __restrict__ prefix,
+                                                          const int* __restrict__ list,
+                                                          const uint8_t* __restrict__ moved,
+                                                          uint8_t* __restrict__ stat) {
+    const int64_t v = blockIdx.x;
+    if (v >= nv) return;
+    const int c = valid[v];
+    const int64_t beg = c == 0 ? 0 : prefix[c - 1], end = prefix[c];
+    bool any = false;
+    for (int64_t i = beg + threadIdx.x; i < end && !any; i += 64) any = moved[list[i]] != 0;
+    any = __ballot(any) != 0ull;
+    if (threadIdx.x == 0) stat[v] = any ? 0 : 1;
+}
+
+// Per-cell convergence of the realignment: the first step after a re-binning
+// records every cell's loss; a later step marks a cell converged once its loss
+// is at most tol times that record. skip = static or converged (sticky until
+// the next re-binning).
+__global__ void __launch_bounds__(256) converge_kernel(int64_t nv, const float* __restrict__ cell_loss,
+                                                       float* __restrict__ cell_ref,
+                                                       const uint8_t* __restrict__ stat, float tol, int first,
+                                                       uint8_t* __restrict__ skip) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const bool st = stat && stat[v];
+    if (first) {
+        cell_ref[v] = cell_loss[v];
+        skip[v] = st ? 1 : 0;
+        return;
+    }
+    const bool conv = tol > 0.0f && cell_loss[v] <= tol * cell_ref[v];
+    skip[v] = (skip[v] || st || conv) ? 1 : 0;
 }
 
 // One Adam step per Gaussian over its 59 raw parameters (pos 3, rot 4,
@@ -590,9 +635,12 @@ __global__ void __launch_bounds__(256) adam_kernel(int64_t N, const float* __res
                                                    float* __restrict__ log_scale, float* __restrict__ logit_opac,
                                                    float* __restrict__ sh, float* __restrict__ m, float* __restrict__ v,
                                                    float lr_pos, float lr_rot, float lr_scale, float lr_opac,
-                                                   float lr_sh, float b1, float b2, float eps, float bc1, float bc2) {
+                                                   float lr_sh, float b1, float b2, float eps, float bc1, float bc2,
+                                                   const uint8_t* __restrict__ moved,
+                                                   const float* __restrict__ max_scale) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= N) return;
+    if (moved && !moved[g]) return;  // "Only Optimize Deformed Region": the rest stays as it is
     auto step = [&](int k, float grad, float lr, float& p) {
         float& mk = m[k * N + g];
         float& vk = v[k * N + g];
@@ -610,6 +658,9 @@ __global__ void __launch_bounds__(256) adam_kernel(int64_t N, const float* __res
     step(10, dopac[g] * o * (1.0f - o), lr_opac, logit_opac[g]);
 #pragma unroll 4
     for (int c = 0; c < 48; c++) step(11 + c, dsh[48 * g + c], lr_sh, sh[48 * g + c]);
+    if (max_scale)
+#pragma unroll
+        for (int c = 0; c < 3; c++) log_scale[3 * g + c] = fminf(log_scale[3 * g + c], logf(max_scale[3 * g + c]));
 }
 
 }  // namespace
@@ -623,10 +674,27 @@ int launch_activate(int64_t N, const float* log_scale, const float* logit_opac, 
 }
 
 int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float* aim_feat, const float* aim_dens,
-                   float w, float* dfeat, float* ddens, double* loss, hipStream_t st) {
+                   float w, float* dfeat, float* ddens, double* loss, float* cell_loss, hipStream_t st) {
     if (S <= 0) return 0;
     hipLaunchKernelGGL(l1_grad_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, S, feat, dens, aim_feat, aim_dens, w,
-                       dfeat, ddens, loss);
+                       dfeat, ddens, loss, cell_loss);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_static_cells(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const uint8_t* moved,
+                        uint8_t* stat, hipStream_t st) {
+    if (nv <= 0) return 0;
+    hipLaunchKernelGGL(static_cells_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, moved, stat);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_converge(int64_t nv, const float* cell_loss, float* cell_ref, const uint8_t* stat, float tol, int first,
+                    uint8_t* skip, hipStream_t st) {
+    if (nv <= 0) return 0;
+    hipLaunchKernelGGL(converge_kernel, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, cell_loss, cell_ref, stat, tol,
+                       first, skip);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -634,11 +702,11 @@ int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float*
 int launch_adam(int64_t N, const float* dpos, const float* drot, const float* dscale, const float* dopac,
                 const float* dsh, const float* scale, const float* opac, float* pos, float* rot, float* log_scale,
                 float* logit_opac, float* sh, float* m, float* v, const float lr[5], float b1, float b2, float eps,
-                float bc1, float bc2, hipStream_t st) {
+                float bc1, float bc2, const uint8_t* moved, const float* max_scale, hipStream_t st) {
     if (N <= 0) return 0;
     hipLaunchKernelGGL(adam_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, dpos, drot, dscale, dopac, dsh, scale,
                        opac, pos, rot, log_scale, logit_opac, sh, m, v, lr[0], lr[1], lr[2], lr[3], lr[4], b1, b2, eps,
-                       bc1, bc2);
+                       bc1, bc2, moved, max_scale);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -653,10 +721,10 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st) {
 int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
-                            float* gdens, int64_t n_gauss, hipStream_t st) {
+                            float* gdens, int64_t n_gauss, hipStream_t st, const uint8_t* skip) {
     if (nv <= 0) return 0;
     hipLaunchKernelGGL(density_backward_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples,
-                       pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens, n_gauss);
+                       pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens, n_gauss, skip);
     HIP_RET(hipGetLastError());
     return 0;
 }

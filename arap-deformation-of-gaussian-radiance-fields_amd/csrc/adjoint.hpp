@@ -17,7 +17,7 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st);
 int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
                             const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
-                            float* gdens, int64_t n_gauss, hipStream_t st);
+                            float* gdens, int64_t n_gauss, hipStream_t st, const uint8_t* skip = nullptr);
 
 // Gradients of each Gaussian's own attributes from the density accumulators
 // (58 x N): dpos N*3, drot N*4 (w.r.t. the stored rotation), dscale N*3,
@@ -52,11 +52,20 @@ int launch_node_gather(int M, int K, int64_t n_end_entries, const uint2* rng, co
 // parameters per Gaussian (lr: pos, rot, log-scale, logit-opacity, SH).
 int launch_activate(int64_t N, const float* log_scale, const float* logit_opac, float* scale, float* opac,
                     hipStream_t st);
+// cell_loss (nullable, one float per valid cell = 64 samples): that cell's L1 loss
 int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float* aim_feat, const float* aim_dens,
-                   float w, float* dfeat, float* ddens, double* loss, hipStream_t st);
+                   float w, float* dfeat, float* ddens, double* loss, float* cell_loss, hipStream_t st);
+// stat[v] = 1 when valid cell v's list holds no Gaussian with moved[g] != 0
+int launch_static_cells(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const uint8_t* moved,
+                        uint8_t* stat, hipStream_t st);
+// first: cell_ref = cell_loss, skip = stat; else skip |= stat | (tol > 0 && cell_loss <= tol * cell_ref)
+int launch_converge(int64_t nv, const float* cell_loss, float* cell_ref, const uint8_t* stat, float tol, int first,
+                    uint8_t* skip, hipStream_t st);
+// moved (nullable): only Gaussians with moved[g] != 0 are updated; max_scale
+// (nullable, N*3): the activated scales are clamped to it after the update
 int launch_adam(int64_t N, const float* dpos, const float* drot, const float* dscale, const float* dopac,
                 const float* dsh, const float* scale, const float* opac, float* pos, float* rot, float* log_scale,
                 float* logit_opac, float* sh, float* m, float* v, const float lr[5], float b1, float b2, float eps,
-                float bc1, float bc2, hipStream_t st);
+                float bc1, float bc2, const uint8_t* moved, const float* max_scale, hipStream_t st);
 
 }  // namespace arapgs

@@ -69,8 +69,10 @@ struct arapgs_ctx {
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
-    size_t prof_used = 0;
+    // HIP events around the six-point (0), sample blend (1) and sample SH (2)
+    // launches of arapgs_deform_frame while profiling is on
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events[3];
+    size_t prof_used[3] = {0, 0, 0};
 };
 
 struct arapgs_solver {
@@ -172,10 +174,11 @@ void arapgs_destroy(arapgs_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
-    for (auto& pr : c->prof_events) {
-        (void)hipEventDestroy(pr.first);
-        (void)hipEventDestroy(pr.second);
-    }
+    for (auto& list : c->prof_events)
+        for (auto& pr : list) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
                     (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec, c->db_ws,
                     (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views, (void*)c->r_seg})
@@ -407,6 +410,36 @@ int arapgs_solver_set_warm_start(arapgs_solver* s, int on) {
     return 0;
 }
 
+int arapgs_solver_reset(arapgs_solver* s) {
+    if (!s) return -1;
+    (void)hipSetDevice(s->device);
+    // a side-stream rebuild in flight finishes before its buffers are forgotten
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemset(s->w.hwarm, 0, sizeof(float) * (size_t)s->w.warm_slots * s->w.n) != hipSuccess)
+        return -1000;
+    s->w.pending = false;
+    s->w.have_coarse = false;
+    s->solves = 0;
+    return 0;
+}
+
+int arapgs_solver_set_cg_block(arapgs_solver* s, int threads) {
+    if (!s || (threads != 64 && threads != 128 && threads != 256)) return -1;
+    if (threads == s->w.cg_block) return 0;
+    (void)hipSetDevice(s->device);
+    // the batch entries are rebuilt; nothing may still read the old ones
+    if (hipDeviceSynchronize() != hipSuccess) return -1000;
+    s->w.cg_block = threads;
+    std::string err;
+    return solver_batch_init(s->one, {&s->w}, err);
+}
+
+int arapgs_solver_set_async_refresh(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    s->one.async = on != 0;
+    return 0;
+}
+
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames) {
     if (!s || frames < 1) return -1;
     s->coarse_refresh = frames;
@@ -546,6 +579,12 @@ int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every) {
     return 0;
 }
 
+int arapgs_solver_batch_set_async_refresh(arapgs_solver_batch* b, int on) {
+    if (!b || on < 0 || on > 1) return -1;
+    b->b.async = on != 0;
+    return 0;
+}
+
 int arapgs_solver_batch_set_coarse_refresh(arapgs_solver_batch* b, int frames) {
     if (!b || frames < 1) return -1;
     b->coarse_refresh = frames;
@@ -636,6 +675,27 @@ int arapgs_sample_sh(arapgs_ctx* c, int64_t s, int k, const float* w, const uint
     return check_launch(c, launch_sample_sh(s, k, w, idx, node_q, is_static, feat, c->shrot_dev, c->stream), "sample_sh");
 }
 
+int arapgs_samples_frames(arapgs_ctx* c, int64_t s, int k, int nf, const arapgs_node_xf* const* tables_h,
+                          const float* const* node_q_h, const uint32_t* idx, const float* w, const uint8_t* is_static,
+                          float* samples, float* feat) {
+    CHECK_CTX(c);
+    if (!idx || !w || !samples || !feat || s < 0 || nf < 0 || (nf > 0 && (!tables_h || !node_q_h)))
+        return fail(c, -1, "samples_frames: bad arguments");
+    if (k != 8) return fail(c, -1, "samples_frames: k must be 8");
+    if (nf > SAMPLE_FRAMES_MAX) return fail(c, -1, "samples_frames: at most %d frames per call", SAMPLE_FRAMES_MAX);
+    if ((uintptr_t)idx % 16 || (uintptr_t)w % 16 || (uintptr_t)feat % 16)
+        return fail(c, -1, "samples_frames: idx, w and feat must be 16-byte aligned");
+    SampleFrames F{};
+    F.n = nf;
+    for (int f = 0; f < nf; f++) {
+        if (!tables_h[f] || !node_q_h[f]) return fail(c, -1, "samples_frames: frame %d lacks a table or quaternions", f);
+        F.table[f] = (const NodeXf*)tables_h[f];
+        F.q[f] = node_q_h[f];
+    }
+    return check_launch(c, launch_samples_frames(s, k, F, idx, w, is_static, samples, feat, c->shrot_dev, c->stream),
+                        "samples_frames");
+}
+
 int arapgs_sh_rotate(arapgs_ctx* c, int64_t n, const float* r9, float* sh) {
     CHECK_CTX(c);
     if (!r9 || !sh || n < 0) return fail(c, -1, "sh_rotate: bad arguments");
@@ -691,7 +751,9 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
             return fail(c, -1, "render: view %d: bad arguments", v);
         if (e->colors_precomp || e->cov3D_precomp)
             return fail(c, -1, "render: precomputed colours/covariances are not supported");
-        if (e->prefiltered || e->to_ortho) return fail(c, -1, "render: prefiltered / orthographic modes are not supported");
+        if (e->prefiltered) return fail(c, -1, "render: prefiltered mode is not supported");
+        if (e->to_ortho && !(e->ortho_scale > 0.0f))
+            return fail(c, -1, "render: view %d: orthographic mode needs ortho_scale > 0", v);
         if (v > 0 && (e->P != d->P || e->D != d->D || e->width != d->width || e->height != d->height ||
                       (e->tile ? e->tile : 16) != tile || e->means3D != d->means3D || e->shs != d->shs ||
                       e->opacities != d->opacities || e->scales != d->scales || e->rotations != d->rotations ||
@@ -724,8 +786,13 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
         const arapgs_render_desc* e = dv + v;
         hv[v].view = e->viewmatrix; hv[v].proj = e->projmatrix; hv[v].campos = e->cam_pos;
         hv[v].tanfovx = e->tan_fovx; hv[v].tanfovy = e->tan_fovy;
-        hv[v].focal_x = (float)e->width / (2.0f * e->tan_fovx);
-        hv[v].focal_y = (float)e->height / (2.0f * e->tan_fovy);
+        // orthographic (setOrthoCam(ortho_scale, ortho_scale), GaussianView.cpp:1018-1019):
+        // x_ndc = x_view / ortho_scale, so the EWA Jacobian is the constant
+        // pixels-per-unit diag(W, H) / (2 ortho_scale)
+        hv[v].ortho = e->to_ortho ? 1 : 0;
+        hv[v].focal_x = e->to_ortho ? (float)e->width / (2.0f * e->ortho_scale) : (float)e->width / (2.0f * e->tan_fovx);
+        hv[v].focal_y = e->to_ortho ? (float)e->height / (2.0f * e->ortho_scale)
+                                    : (float)e->height / (2.0f * e->tan_fovy);
         hv[v].out = e->out_color; hv[v].surface = e->surface; hv[v].radii = e->radii; hv[v].rects = e->rects;
         hv[v].dout = nullptr;
     }
@@ -839,6 +906,8 @@ int arapgs_render_debug(arapgs_ctx* c, uint64_t* keys_h, uint32_t* vals_h, int64
 
 struct arapgs_grid {
     int G = 0;
+    int padding = 1;
+    int64_t version = 0;  // incremented by every re-binning
     int64_t n_pairs = 0, n_valid = 0;
     int64_t n_gauss = 0;  // Gaussians the grid was built for
     int64_t* prefix = nullptr;
@@ -915,22 +984,22 @@ int arapgs_gaussian_aabbs(arapgs_ctx* c, const float* pos, const float* rot, con
     return check_launch(c, launch_aabbs(pos, rot, scale, opacity, n, aabbs, c->stream), "gaussian_aabbs");
 }
 
-int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num, int padding,
-                      arapgs_grid** out) {
-    CHECK_CTX(c);
-    if (!aabbs || !aabb7_h || !out || n <= 0 || grid_num <= 0 || grid_num > 1024)
-        return fail(c, -1, "grid_build: bad arguments");
-    if (n > 0x7fffffffLL) return fail(c, -1, "grid_build: at most 2^31-1 Gaussians");
-    *out = nullptr;
-    (void)hipSetDevice(c->device);
-    const int G = grid_num;
+}  // extern "C"
+
+namespace {
+// The cell lists of GetBoxesGsGrid (cudakdtree.cu:425-451) for Gaussian boxes
+// `aabbs` on the grid gg: prefix (int64[G^3], inclusive per-cell counts) and
+// flags (u32[G^3], 1 where a cell holds any Gaussian) are caller buffers; the
+// list (int[pairs], every cell's Gaussians ascending) is allocated here and
+// returned in *list_out. The pairs are emitted and sorted in chunks of whole
+// x-slabs (cells x*G^2 .. (x+1)*G^2-1) whose pair counts fit 32-bit offsets.
+int build_cell_lists(arapgs_ctx* c, const char* what, const float* aabbs, int64_t n, const GridGeom& gg,
+                     int64_t* prefix, uint32_t* flags, int** list_out, uint64_t* npairs_out) {
+    *list_out = nullptr;
+    *npairs_out = 0;
+    const int G = gg.G;
     const int64_t G2 = (int64_t)G * G;
     const int64_t gc = G2 * G;
-    GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, padding};
-    arapgs_grid* g = new arapgs_grid();
-    g->G = grid_num;
-    g->n_gauss = n;
-    g->device = c->device;
     std::vector<void*> tmp;
     auto dmal = [&](size_t bytes) -> void* {
         void* p = nullptr;
@@ -941,28 +1010,24 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
     auto sync = [&]() -> int {
         hipError_t e = hipStreamSynchronize(c->stream);
-        return e == hipSuccess ? 0 : hip_fail(c, e, "grid_build");
+        return e == hipSuccess ? 0 : hip_fail(c, e, what);
     };
     // 1. pairs per x-slab, and from them the chunks of whole slabs whose pairs
     //    fit the 32-bit offsets of one emit + sort pass
     unsigned long long* slab = (unsigned long long*)dmal(8 * (size_t)G);
     uint32_t* cell_count = (uint32_t*)dmal(4 * (size_t)gc);
-    uint32_t* flags = (uint32_t*)dmal(4 * (size_t)gc);
-    uint32_t* flag_ex = (uint32_t*)dmal(4 * (size_t)gc);
     uint32_t* total = (uint32_t*)dmal(8);
     uint32_t* prefix_ws = (uint32_t*)dmal(4 * cell_prefix_workspace_words(gc));
-    uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(std::max<int64_t>(n, gc)));
-    if (!slab || !cell_count || !flags || !flag_ex || !total || !prefix_ws || !scan_ws ||
-        hipMalloc(&g->prefix, 8 * (size_t)gc) != hipSuccess) {
+    uint32_t* scan_ws = (uint32_t*)dmal(4 * scan_workspace_words(n));
+    if (!slab || !cell_count || !total || !prefix_ws || !scan_ws) {
         cleanup();
-        delete g;
-        return fail(c, -2, "grid_build: allocation failed");
+        return fail(c, -2, "%s: allocation failed", what);
     }
-    int rc = check_launch(c, launch_slab_counts(aabbs, n, gg, slab, c->stream), "grid_build");
+    int rc = check_launch(c, launch_slab_counts(aabbs, n, gg, slab, c->stream), what);
     std::vector<unsigned long long> slab_h((size_t)G, 0ull);
     if (!rc) {
         hipError_t e = hipMemcpyAsync(slab_h.data(), slab, 8 * (size_t)G, hipMemcpyDeviceToHost, c->stream);
-        if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+        if (e != hipSuccess) rc = hip_fail(c, e, what);
         else rc = sync();
     }
     std::vector<std::pair<int, int>> chunks;  // [x0, x1)
@@ -974,7 +1039,7 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
         for (int x = 0; x < G; x++) {
             const uint64_t p = slab_h[x];
             if (p > GRID_CHUNK_PAIRS_MAX) {
-                rc = fail(c, -1, "grid_build: slab %d holds %llu cell-Gaussian pairs (at most %llu per slab)", x,
+                rc = fail(c, -1, "%s: slab %d holds %llu cell-Gaussian pairs (at most %llu per slab)", what, x,
                           (unsigned long long)p, (unsigned long long)GRID_CHUNK_PAIRS_MAX);
                 break;
             }
@@ -995,6 +1060,7 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     // 2. per chunk: counts, offsets, pairs keyed by the cell within the chunk,
     //    a stable radix sort (pairs leave the emitter in Gaussian order, so every
     //    cell's list comes out ascending) and the values appended to the list
+    int* list = nullptr;
     if (!rc) {
         const size_t np = std::max<uint64_t>(max_chunk, 1);
         uint32_t* cnt = (uint32_t*)dmal(4 * (size_t)n);
@@ -1003,50 +1069,96 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
         uint32_t* vals = (uint32_t*)dmal(4 * np * 2);
         uint32_t* sort_ws = (uint32_t*)dmal(4 * radix_workspace_words((int64_t)np));
         if (!cnt || !offs || !keys || !vals || !sort_ws ||
-            hipMalloc(&g->list, 4 * std::max<uint64_t>(npairs, 1)) != hipSuccess)
-            rc = fail(c, -2, "grid_build: allocation failed (%llu cell-Gaussian pairs)", (unsigned long long)npairs);
-        if (!rc) rc = check_launch(c, launch_zero_cells(cell_count, gc, c->stream), "grid_build");
+            hipMalloc(&list, 4 * std::max<uint64_t>(npairs, 1)) != hipSuccess) {
+            list = nullptr;
+            rc = fail(c, -2, "%s: allocation failed (%llu cell-Gaussian pairs)", what, (unsigned long long)npairs);
+        }
+        if (!rc) rc = check_launch(c, launch_zero_cells(cell_count, gc, c->stream), what);
         uint64_t base = 0;
         for (size_t k = 0; !rc && k < chunks.size(); k++) {
             const int x0 = chunks[k].first, x1 = chunks[k].second;
-            rc = check_launch(c, launch_box_count_slabs(aabbs, n, gg, x0, x1, cnt, c->stream), "grid_build");
-            if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), "grid_build");
+            rc = check_launch(c, launch_box_count_slabs(aabbs, n, gg, x0, x1, cnt, c->stream), what);
+            if (!rc) rc = check_launch(c, launch_exclusive_scan(cnt, n, offs, total, scan_ws, c->stream), what);
             uint32_t cp = 0;
             if (!rc) {
                 hipError_t e = hipMemcpyAsync(&cp, total, 4, hipMemcpyDeviceToHost, c->stream);
-                if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+                if (e != hipSuccess) rc = hip_fail(c, e, what);
                 else rc = sync();
             }
             uint64_t want = 0;
             for (int x = x0; x < x1; x++) want += slab_h[x];
             if (!rc && (uint64_t)cp != want)
-                rc = fail(c, -1, "grid_build: chunk %d..%d counts %u pairs, slabs %llu", x0, x1, cp,
+                rc = fail(c, -1, "%s: chunk %d..%d counts %u pairs, slabs %llu", what, x0, x1, cp,
                           (unsigned long long)want);
             if (!rc && cp > 0) {
                 int bits = 1;
                 while ((1ll << bits) < (int64_t)(x1 - x0) * G2) bits++;
                 rc = check_launch(c, launch_box_emit_slabs(aabbs, n, gg, x0, x1, offs, keys, vals, cell_count,
-                                                           c->stream), "grid_build");
+                                                           c->stream), what);
                 if (!rc)
                     rc = check_launch(c, launch_radix_sort32(keys, vals, keys + np, vals + np, cp, bits, sort_ws,
-                                                             c->stream), "grid_build");
+                                                             c->stream), what);
                 if (!rc) {
-                    hipError_t e = hipMemcpyAsync(g->list + base, vals, 4 * (size_t)cp, hipMemcpyDeviceToDevice,
+                    hipError_t e = hipMemcpyAsync(list + base, vals, 4 * (size_t)cp, hipMemcpyDeviceToDevice,
                                                   c->stream);
-                    if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
+                    if (e != hipSuccess) rc = hip_fail(c, e, what);
                 }
             }
             base += cp;
         }
     }
-    // 3. the 64-bit per-cell prefix, valid cells and their samples
-    if (!rc) rc = check_launch(c, launch_cell_prefix(cell_count, gc, g->prefix, flags, prefix_ws, c->stream), "grid_build");
+    // 3. the 64-bit per-cell prefix and the occupancy flags
+    if (!rc) rc = check_launch(c, launch_cell_prefix(cell_count, gc, prefix, flags, prefix_ws, c->stream), what);
+    if (!rc) rc = sync();
+    cleanup();
+    if (rc) {
+        if (list) (void)hipFree(list);
+        return rc;
+    }
+    *list_out = list;
+    *npairs_out = npairs;
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num, int padding,
+                      arapgs_grid** out) {
+    CHECK_CTX(c);
+    if (!aabbs || !aabb7_h || !out || n <= 0 || grid_num <= 0 || grid_num > 1024)
+        return fail(c, -1, "grid_build: bad arguments");
+    if (n > 0x7fffffffLL) return fail(c, -1, "grid_build: at most 2^31-1 Gaussians");
+    *out = nullptr;
+    (void)hipSetDevice(c->device);
+    const int64_t gc = (int64_t)grid_num * grid_num * grid_num;
+    GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], grid_num, padding};
+    arapgs_grid* g = new arapgs_grid();
+    g->G = grid_num;
+    g->padding = padding;
+    g->n_gauss = n;
+    g->device = c->device;
+    uint32_t *flags = nullptr, *flag_ex = nullptr, *total = nullptr, *scan_ws = nullptr;
+    auto cleanup = [&]() {
+        for (void* p : {(void*)flags, (void*)flag_ex, (void*)total, (void*)scan_ws})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&flags, 4 * (size_t)gc) != hipSuccess || hipMalloc(&flag_ex, 4 * (size_t)gc) != hipSuccess ||
+        hipMalloc(&total, 8) != hipSuccess || hipMalloc(&scan_ws, 4 * scan_workspace_words(gc)) != hipSuccess ||
+        hipMalloc(&g->prefix, 8 * (size_t)gc) != hipSuccess) {
+        cleanup();
+        arapgs_grid_destroy(g);
+        return fail(c, -2, "grid_build: allocation failed");
+    }
+    uint64_t npairs = 0;
+    int rc = build_cell_lists(c, "grid_build", aabbs, n, gg, g->prefix, flags, &g->list, &npairs);
+    // valid cells and their samples
     if (!rc) rc = check_launch(c, launch_exclusive_scan(flags, gc, flag_ex, total + 1, scan_ws, c->stream), "grid_build");
     uint32_t nvalid = 0;
     if (!rc) {
         hipError_t e = hipMemcpyAsync(&nvalid, total + 1, 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(c, e, "grid_build");
-        else rc = sync();
     }
     if (!rc) {
         if (hipMalloc(&g->valid, 4 * std::max<size_t>(nvalid, 1)) != hipSuccess ||
@@ -1054,8 +1166,8 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
             rc = fail(c, -2, "grid_build: allocation failed");
         if (!rc) rc = check_launch(c, launch_compact_valid(flags, flag_ex, gc, g->valid, c->stream), "grid_build");
         if (!rc) rc = check_launch(c, launch_samples(g->valid, nvalid, gg, g->samples, c->stream), "grid_build");
-        const int rs = sync();
-        if (!rc) rc = rs;
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (!rc && e != hipSuccess) rc = hip_fail(c, e, "grid_build");
     }
     cleanup();
     if (rc) {
@@ -1065,6 +1177,37 @@ int arapgs_grid_build(arapgs_ctx* c, const float* aabbs, int64_t n, const float*
     g->n_pairs = (int64_t)npairs;
     g->n_valid = nvalid;
     *out = g;
+    return 0;
+}
+
+int arapgs_grid_rebin(arapgs_ctx* c, arapgs_grid* g, const float* aabbs, int64_t n, const float* aabb7_h) {
+    CHECK_CTX(c);
+    if (!g || !aabbs || !aabb7_h || n != g->n_gauss) return fail(c, -1, "grid_rebin: bad arguments");
+    if (g->device != c->device) return fail(c, -1, "grid_rebin: grid of another device");
+    (void)hipSetDevice(c->device);
+    const int64_t gc = (int64_t)g->G * g->G * g->G;
+    GridGeom gg{{aabb7_h[0], aabb7_h[1], aabb7_h[2]}, aabb7_h[6], g->G, g->padding};
+    uint32_t* flags = nullptr;
+    int64_t* prefix = nullptr;
+    if (hipMalloc(&flags, 4 * (size_t)gc) != hipSuccess || hipMalloc(&prefix, 8 * (size_t)gc) != hipSuccess) {
+        if (flags) (void)hipFree(flags);
+        return fail(c, -2, "grid_rebin: allocation failed");
+    }
+    int* list = nullptr;
+    uint64_t npairs = 0;
+    // new prefix and list; the grid keeps the old ones until both are complete
+    int rc = build_cell_lists(c, "grid_rebin", aabbs, n, gg, prefix, flags, &list, &npairs);
+    (void)hipFree(flags);
+    if (rc) {
+        (void)hipFree(prefix);
+        return rc;
+    }
+    (void)hipFree(g->list);
+    (void)hipFree(g->prefix);
+    g->list = list;
+    g->prefix = prefix;
+    g->n_pairs = (int64_t)npairs;
+    g->version++;
     return 0;
 }
 
@@ -1163,6 +1306,26 @@ int arapgs_static_rows(arapgs_ctx* c, const uint8_t* node_static, const uint32_t
     return check_launch(c, launch_static_rows(node_static, idx, n, len, out, c->stream), "static_rows");
 }
 
+namespace {
+// Start / end events of one profiled launch (null when profiling is off).
+std::pair<hipEvent_t, hipEvent_t>* prof_begin(arapgs_ctx* c, int which) {
+    if (!c->prof) return nullptr;
+    auto& list = c->prof_events[which];
+    size_t& used = c->prof_used[which];
+    if (used == list.size() && list.size() < 4096) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) list.push_back({a, b});
+    }
+    std::pair<hipEvent_t, hipEvent_t>* ev = used < list.size() ? &list[used++] : nullptr;
+    if (ev) (void)hipEventRecord(ev->first, c->stream);
+    return ev;
+}
+
+void prof_end(arapgs_ctx* c, std::pair<hipEvent_t, hipEvent_t>* ev) {
+    if (ev) (void)hipEventRecord(ev->second, c->stream);
+}
+}  // namespace
+
 int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buffers* fb, const float* aims,
                         int gn_iters, int stop_rule, double* energies, int* iters_done) {
     CHECK_CTX(c);
@@ -1174,33 +1337,31 @@ int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buff
     rc = check_launch(c, launch_build_node_table(m, fb->node_pos, s->w.x, s->w.dev.free_id, s->table, c->stream),
                       "deform_frame table");
     // samples first: they use the current node positions (GaussianView.cpp:1500-1504)
-    if (!rc && fb->s > 0)
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    if (!rc && fb->s > 0) {
+        ev = prof_begin(c, 1);
         rc = check_launch(c, launch_blend_points(fb->samples, fb->s, fb->sample_idx, fb->sample_w, k, s->table, m,
                                                  fb->sample_static, fb->samples, c->stream), "deform_frame samples");
-    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-    if (!rc && c->prof) {
-        if (c->prof_used == c->prof_events.size() && c->prof_events.size() < 4096) {
-            hipEvent_t a, b;
-            if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) c->prof_events.push_back({a, b});
-        }
-        if (c->prof_used < c->prof_events.size()) ev = &c->prof_events[c->prof_used++];
-        if (ev) (void)hipEventRecord(ev->first, c->stream);
+        prof_end(c, ev);
     }
+    ev = rc ? nullptr : prof_begin(c, 0);
     if (!rc)
         rc = check_launch(c, launch_six_point(fb->n, k, s->table, m, fb->ends_idx, fb->ends_w, fb->ends, fb->scale0,
                                               fb->gs_static, fb->axis_padding, fb->end_coeff, fb->pos, fb->rot,
                                               fb->scale, fb->sh, c->shrot_dev, c->stream),
                           "deform_frame six-point");
-    if (ev) (void)hipEventRecord(ev->second, c->stream);
+    prof_end(c, ev);
     if (!rc)
         rc = check_launch(c, launch_blend_points(fb->node_pos, m, fb->node_src_idx, fb->node_src_w, k, s->table, m,
                                                  nullptr, s->node_tmp, c->stream), "deform_frame nodes");
     if (!rc) rc = check_launch(c, launch_copy_f32(fb->node_pos, s->node_tmp, 3 * (int64_t)m, c->stream), "deform_frame");
     if (!rc && fb->s > 0 && fb->sample_feat) {
         rc = check_launch(c, launch_node_quats(m, s->table, s->node_q, c->stream), "deform_frame quats");
+        ev = rc ? nullptr : prof_begin(c, 2);
         if (!rc)
             rc = check_launch(c, launch_sample_sh(fb->s, k, fb->sample_w, fb->sample_idx, s->node_q, fb->sample_static,
                                                   fb->sample_feat, c->shrot_dev, c->stream), "deform_frame sample SH");
+        prof_end(c, ev);
     }
     return rc;
 }
@@ -1212,22 +1373,26 @@ extern "C" {
 int arapgs_profile_enable(arapgs_ctx* c, int on) {
     CHECK_CTX(c);
     c->prof = on != 0;
-    c->prof_used = 0;
+    for (size_t& u : c->prof_used) u = 0;
     return 0;
 }
 
-int arapgs_profile_read(arapgs_ctx* c, float* ms_h, int cap) {
+int arapgs_profile_read_kernel(arapgs_ctx* c, int which, float* ms_h, int cap) {
     CHECK_CTX(c);
+    if (which < 0 || which > 2 || (cap > 0 && !ms_h)) return fail(c, -1, "profile_read: bad arguments");
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "profile_read");
     int n = 0;
-    for (size_t i = 0; i < c->prof_used && n < cap; i++) {
+    auto& list = c->prof_events[which];
+    for (size_t i = 0; i < c->prof_used[which] && n < cap; i++) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, c->prof_events[i].first, c->prof_events[i].second) == hipSuccess) ms_h[n++] = ms;
+        if (hipEventElapsedTime(&ms, list[i].first, list[i].second) == hipSuccess) ms_h[n++] = ms;
     }
-    c->prof_used = 0;
+    c->prof_used[which] = 0;
     return n;
 }
+
+int arapgs_profile_read(arapgs_ctx* c, float* ms_h, int cap) { return arapgs_profile_read_kernel(c, 0, ms_h, cap); }
 
 }  // extern "C"
 
@@ -1239,9 +1404,7 @@ struct arapgs_chain {
     arapgs_chain_desc d{};
     std::vector<const float*> view, proj, campos;
     std::vector<float> tfx, tfy;
-    const int64_t* g_prefix = nullptr;
-    const int* g_list = nullptr;
-    const int* g_valid = nullptr;
+    const arapgs_grid* grid = nullptr;  // its lists are read at every call (a re-binning replaces them)
     const float* samples0 = nullptr;
     // deformed state and its inputs
     float *ends0 = nullptr, *ends = nullptr, *pos = nullptr, *rot = nullptr, *scale = nullptr, *sh = nullptr;
@@ -1357,7 +1520,7 @@ int arapgs_chain_create(arapgs_ctx* c, const arapgs_chain_desc* d, arapgs_chain*
     if (d->grid) {
         ch->nv = d->grid->n_valid;
         ch->S = 64 * ch->nv;
-        ch->g_prefix = d->grid->prefix; ch->g_list = d->grid->list; ch->g_valid = d->grid->valid;
+        ch->grid = d->grid;
         ch->samples0 = d->grid->samples;
     }
     ch->n_end_entries = 6 * ch->N * ch->K;
@@ -1422,7 +1585,7 @@ int arapgs_chain_forward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* 
     rc = chain_render(c, ch, images_h);
     if (rc) return rc;
     if (ch->S > 0)
-        rc = check_launch(c, launch_density(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples, ch->pos,
+        rc = check_launch(c, launch_density(ch->nv, ch->grid->valid, ch->grid->prefix, ch->grid->list, ch->samples, ch->pos,
                                             ch->rot, ch->scale, ch->d.opacity, ch->sh, ch->d.lpf, feat, dens,
                                             c->stream),
                           "chain density");
@@ -1476,7 +1639,7 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
     if (ch->S > 0) {
         rc = check_launch(c, launch_zero_f32(ch->gdens, 58 * ch->N, st), "chain backward");
         if (!rc && (dfeat || ddens))
-            rc = check_launch(c, launch_density_backward(ch->nv, ch->g_valid, ch->g_prefix, ch->g_list, ch->samples,
+            rc = check_launch(c, launch_density_backward(ch->nv, ch->grid->valid, ch->grid->prefix, ch->grid->list, ch->samples,
                                                          ch->pos, ch->rot, ch->scale, ch->d.opacity, ch->sh,
                                                          ch->d.lpf, dfeat, ddens, ch->gsamp, ch->gdens, ch->N,
                                                          st),
@@ -1520,6 +1683,15 @@ struct arapgs_realign {
     int64_t t = 0;  // Adam step
     float *scale = nullptr, *opac = nullptr, *feat = nullptr, *dens = nullptr, *dfeat = nullptr, *ddens = nullptr;
     float *gd = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    // options (GaussianView.cpp:1695-1733, 4191-4207, 4452-4478)
+    const uint8_t* moved = nullptr;    // only these Gaussians are updated (null: all)
+    const float* max_scale = nullptr;  // upper bound of the activated scales (null: none)
+    float conv_tol = 0.0f;             // per-cell convergence (0: off)
+    // per valid cell: static (no moved Gaussian), skip (static or converged), loss, reference loss
+    uint8_t *stat = nullptr, *skip = nullptr;
+    float *cell_loss = nullptr, *cell_ref = nullptr;
+    int64_t seen_version = -1;  // grid version the static flags were computed for
+    bool restart = true;        // the next step records the reference losses
     std::vector<void*> allocs;
 };
 
@@ -1550,9 +1722,11 @@ int arapgs_realign_create(arapgs_ctx* c, const arapgs_grid* grid, int64_t n, con
     if (r->d.beta2 == 0.0f) r->d.beta2 = 0.999f;
     if (r->d.eps == 0.0f) r->d.eps = 1e-15f;
     const size_t N = (size_t)n, S = (size_t)std::max<int64_t>(r->S, 1);
+    const size_t NV = (size_t)std::max<int64_t>(grid->n_valid, 1);
     struct { float** p; size_t count; } bufs[] = {
         {&r->scale, 3 * N}, {&r->opac, N}, {&r->feat, 48 * S}, {&r->dens, S}, {&r->dfeat, 48 * S}, {&r->ddens, S},
-        {&r->gd, 58 * N}, {&r->grad, 59 * N}, {&r->m, 59 * N}, {&r->v, 59 * N}};
+        {&r->gd, 58 * N}, {&r->grad, 59 * N}, {&r->m, 59 * N}, {&r->v, 59 * N},
+        {&r->cell_loss, NV}, {&r->cell_ref, NV}, {(float**)&r->stat, (NV + 3) / 4}, {(float**)&r->skip, (NV + 3) / 4}};
     for (auto& b : bufs) {
         hipError_t e = hipMalloc((void**)b.p, sizeof(float) * b.count);
         if (e != hipSuccess) {
@@ -1586,6 +1760,15 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
     const arapgs_grid* g = r->grid;
     const int64_t N = r->N;
     int rc = check_launch(c, launch_activate(N, log_scale, logit_opacity, r->scale, r->opac, st), "realign activate");
+    // a new binning (or new options) resets the static cells and the convergence record
+    const bool cells_new = r->seen_version != g->version;
+    if (!rc && cells_new) {
+        if (r->moved)
+            rc = check_launch(c, launch_static_cells(g->n_valid, g->valid, g->prefix, g->list, r->moved, r->stat, st),
+                              "realign static cells");
+        r->seen_version = g->version;
+        r->restart = true;
+    }
     if (!rc)
         rc = check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot, r->scale,
                                             r->opac, sh, lpf, r->feat, r->dens, st),
@@ -1593,13 +1776,18 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
     if (!rc && loss) rc = check_launch(c, launch_zero_f32((float*)loss, 2, st), "realign loss");  // 0.0 as f64
     if (!rc)
         rc = check_launch(c, launch_l1_grad(r->S, r->feat, r->dens, aim_feat, aim_dens, r->d.w_density, r->dfeat,
-                                            r->ddens, loss, st),
+                                            r->ddens, loss, r->cell_loss, st),
                           "realign loss");
+    if (!rc)
+        rc = check_launch(c, launch_converge(g->n_valid, r->cell_loss, r->cell_ref, r->moved ? r->stat : nullptr,
+                                             r->conv_tol, r->restart ? 1 : 0, r->skip, st),
+                          "realign convergence");
+    r->restart = false;
     if (!rc) rc = check_launch(c, launch_zero_f32(r->gd, 58 * N, st), "realign backward");
     if (!rc)
         rc = check_launch(c, launch_density_backward(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot,
                                                      r->scale, r->opac, sh, lpf, r->dfeat, r->ddens, nullptr, r->gd,
-                                                     N, st),
+                                                     N, st, r->skip),
                           "realign backward");
     float* dpos = r->grad;
     float* drot = dpos + 3 * N;
@@ -1616,8 +1804,26 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
     const float bc2 = 1.0f - (float)std::pow((double)r->d.beta2, (double)r->t);
     return check_launch(c, launch_adam(N, dpos, drot, dscale, dopac, dsh, r->scale, r->opac, pos, rot, log_scale,
                                        logit_opacity, sh, r->m, r->v, lr, r->d.beta1, r->d.beta2, r->d.eps, bc1, bc2,
-                                       st),
+                                       r->moved, r->max_scale, st),
                         "realign adam");
+}
+
+int arapgs_realign_set_options(arapgs_ctx* c, arapgs_realign* r, const arapgs_realign_options* o) {
+    CHECK_CTX(c);
+    if (!r || !o || o->conv_tol < 0.0f) return fail(c, -1, "realign_set_options: bad arguments");
+    r->moved = o->moved;
+    r->max_scale = o->max_scale;
+    r->conv_tol = o->conv_tol;
+    r->seen_version = -1;  // static cells and convergence start over
+    return 0;
+}
+
+int arapgs_realign_cells(const arapgs_realign* r, int64_t* n_valid, const uint8_t** skip, const float** cell_loss) {
+    if (!r) return -1;
+    if (n_valid) *n_valid = r->grid->n_valid;
+    if (skip) *skip = r->skip;
+    if (cell_loss) *cell_loss = r->cell_loss;
+    return 0;
 }
 
 }  // extern "C"

@@ -42,6 +42,17 @@ int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
                      float* rot, float* scale, float* sh, const ShRotTables* Td, hipStream_t st);
 int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st);
+// Node tables and node quaternions of up to SAMPLE_FRAMES_MAX consecutive
+// frames, applied to the samples in order by one launch (K = 8 only; idx and w
+// rows 16-byte aligned).
+constexpr int SAMPLE_FRAMES_MAX = 8;
+struct SampleFrames {
+    int n;
+    const NodeXf* table[SAMPLE_FRAMES_MAX];
+    const float* q[SAMPLE_FRAMES_MAX];
+};
+int launch_samples_frames(int64_t S, int k, const SampleFrames& F, const uint32_t* idx, const float* w,
+                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* Td, hipStream_t st);
 int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
                      const uint8_t* is_static, float* feat, const ShRotTables* Td, hipStream_t st);
 int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables* Td, hipStream_t st);

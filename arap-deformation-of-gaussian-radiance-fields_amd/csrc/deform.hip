@@ -408,6 +408,89 @@ __device__ __forceinline__ void sample_rotation(int64_t s, const float* __restri
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
 }
 
+// sample_rotation with the sample's ids and weights already in registers (the
+// same arithmetic, operation for operation).
+template <int K>
+__device__ __forceinline__ void sample_rotation_regs(const uint32_t (&id)[K], const float (&wt)[K],
+                                                     const float* __restrict__ node_q, float (&R)[3][3]) {
+    float acc[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    float last = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const float cw = wt[j];
+        const float t = cw / (cw + last);
+        const float4 e4 = *reinterpret_cast<const float4*>(node_q + 4 * (int64_t)id[j]);
+        float e[4] = {e4.x, e4.y, e4.z, e4.w};
+        float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
+        if (cs < 0.0f) {
+            e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3];
+            cs = -cs;
+        }
+        float ra, rb;
+        if (cs > 0.99995f) {
+            ra = 1.0f - t;
+            rb = t;
+        } else {
+            const float sn = __builtin_sqrtf(1.0f - cs * cs);
+            const float ang = atan2f(sn, cs);
+            ra = sinf((1.0f - t) * ang) / sn;
+            rb = sinf(t * ang) / sn;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] = ra * acc[c] + rb * e[c];
+        quat_normalize(acc);
+        last += cw;
+    }
+    quat_normalize(acc);
+    quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
+}
+
+// Several consecutive drag frames of the sample update in one pass. During a
+// drag nothing reads the samples (they feed the density-preserving step after
+// the release), so frame f's sample blend (UpdatePositionforSamples) and SH
+// rotation (FastUpdateSamplesSH) can wait for later frames: for f = 0..nf-1 in
+// order, the position is blended with frame f's node table and the feature
+// rotated by frame f's slerped node rotation, each with the per-frame kernels'
+// arithmetic, so the result equals nf launches of blend_points_kernel<8, true>
+// and sample_sh_kernel<8> bit for bit while every sample's 536 bytes (position,
+// ids, weights, feature) move through HBM once instead of nf times.
+template <int K>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+samples_frames_kernel(int64_t S, SampleFrames F, const uint32_t* __restrict__ idx, const float* __restrict__ w,
+                      const uint8_t* __restrict__ is_static, float* __restrict__ pts, float* __restrict__ feat,
+                      const ShRotTables* __restrict__ T) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float lds[256 * 18];
+    const int64_t s0 = (int64_t)blockIdx.x * blockDim.x;
+    const int n_here = (int)min((int64_t)blockDim.x, S - s0);
+    const bool live = s < S && !(is_static && is_static[s]);
+    uint32_t id[K];
+    float wt[K];
+    if (live) {
+        load_nbrs<K, true>(idx + s * K, w + s * K, id, wt);
+        // positions first, all frames, before the features occupy registers
+        float p[3] = {pts[3 * s], pts[3 * s + 1], pts[3 * s + 2]};
+#pragma unroll 1
+        for (int f = 0; f < F.n; f++) {
+            float o[3];
+            blend_point_regs<K>(F.table[f], id, wt, p, o);
+            p[0] = o[0]; p[1] = o[1]; p[2] = o[2];
+        }
+        pts[3 * s] = p[0]; pts[3 * s + 1] = p[1]; pts[3 * s + 2] = p[2];
+    }
+    float c[48];
+    rows48_load(feat + s0 * 48, n_here, lds, 256 * 18, c);
+    if (live) {
+#pragma unroll 1
+        for (int f = 0; f < F.n; f++) {
+            float R[3][3];
+            sample_rotation_regs<K>(id, wt, F.q[f], R);
+            sh_rotate_regs(*T, R, c);
+        }
+    }
+    rows48_store(feat + s0 * 48, n_here, lds, 256 * 18, c);
+}
+
 __global__ void sh_rotate_kernel(int64_t n, const float* __restrict__ R9, float* __restrict__ sh,
                                  const ShRotTables* __restrict__ T) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -515,6 +598,16 @@ int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, cons
         default:
             return -1;
     }
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_samples_frames(int64_t S, int k, const SampleFrames& F, const uint32_t* idx, const float* w,
+                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* T, hipStream_t st) {
+    if (S == 0 || F.n == 0) return 0;
+    if (k != 8 || F.n < 0 || F.n > SAMPLE_FRAMES_MAX || ((uintptr_t)idx % 16) || ((uintptr_t)w % 16)) return -1;
+    hipLaunchKernelGGL(samples_frames_kernel<8>, dim3(nblk(S, 256)), dim3(256), 0, st, S, F, idx, w, is_static, pts,
+                       feat, T);
     HIP_RET(hipGetLastError());
     return 0;
 }

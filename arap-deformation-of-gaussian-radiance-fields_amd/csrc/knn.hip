@@ -307,6 +307,10 @@ __global__ void __launch_bounds__(256) fps_step_kernel(const float* __restrict__
     }
 }
 
+__global__ void clear_count_kernel(uint32_t* __restrict__ count) {
+    if (threadIdx.x == 0) *count = 0u;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -314,7 +318,8 @@ int launch_knn(const float* nodes, int M, const float* queries, int64_t Q, int k
                uint32_t* scratch_list, uint32_t* scratch_count, hipStream_t st) {
     if (Q == 0) return 0;
     const size_t lds = sizeof(float) * 3 * (size_t)M;
-    HIP_RET(hipMemsetAsync(scratch_count, 0, sizeof(uint32_t), st));
+    // a kernel, not a memset, clears the count: callers may capture the launches
+    hipLaunchKernelGGL(clear_count_kernel, dim3(1), dim3(64), 0, st, scratch_count);
     const int64_t blocks = (Q + 255) / 256;
     switch (k) {
 #define ARAPGS_KNN_CASE(KV)                                                                                     \

@@ -87,12 +87,18 @@ __device__ __forceinline__ void project_one(const RasterParams& rp, const Raster
 #pragma unroll
         for (int j = 0; j < 3; j++)
             S[i][j] = R[i][0] * s2[0] * R[j][0] + R[i][1] * s2[1] * R[j][1] + R[i][2] * s2[2] * R[j][2];
-    const float limx = 1.3f * vw.tanfovx, limy = 1.3f * vw.tanfovy;
-    const float txtz = vx / vz, tytz = vy / vz;
-    const float tx = fminf(limx, fmaxf(-limx, txtz)) * vz;
-    const float ty = fminf(limy, fmaxf(-limy, tytz)) * vz;
-    const float j00 = vw.focal_x / vz, j02 = -(vw.focal_x * tx) / (vz * vz);
-    const float j11 = vw.focal_y / vz, j12 = -(vw.focal_y * ty) / (vz * vz);
+    float j00, j02, j11, j12;
+    if (vw.ortho) {
+        j00 = vw.focal_x; j02 = 0.0f;
+        j11 = vw.focal_y; j12 = 0.0f;
+    } else {
+        const float limx = 1.3f * vw.tanfovx, limy = 1.3f * vw.tanfovy;
+        const float txtz = vx / vz, tytz = vy / vz;
+        const float tx = fminf(limx, fmaxf(-limx, txtz)) * vz;
+        const float ty = fminf(limy, fmaxf(-limy, tytz)) * vz;
+        j00 = vw.focal_x / vz; j02 = -(vw.focal_x * tx) / (vz * vz);
+        j11 = vw.focal_y / vz; j12 = -(vw.focal_y * ty) / (vz * vz);
+    }
     float T[2][3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {

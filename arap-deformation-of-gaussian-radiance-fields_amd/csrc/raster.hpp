@@ -12,7 +12,8 @@ struct RasterView {
     const float* view;     // device, 16 column-major
     const float* proj;     // device, 16 column-major
     const float* campos;   // device, 3
-    float tanfovx, tanfovy, focal_x, focal_y;
+    float tanfovx, tanfovy, focal_x, focal_y;  // orthographic: focal_* = pixels per view-space unit
+    int ortho;             // 1: orthographic projection (constant EWA Jacobian)
     float* out;            // device, 3*H*W planar CHW
     int* surface;          // optional, P
     int* radii;            // optional, P

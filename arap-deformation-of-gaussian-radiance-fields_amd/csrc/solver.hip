@@ -604,12 +604,12 @@ __device__ __forceinline__ double block_sum_partials(const double* __restrict__ 
 // it needs itself, in the same order, so all of them take the same decisions.
 
 // apply for owner block ob (16 free nodes, 16 lanes each)
-template <int KT>
+template <int KT, int BS = 256>
 __device__ __forceinline__ void apply_body(const SolverDev& S, const float* __restrict__ x,
                                            const float* __restrict__ z, float beta, bool first, float* __restrict__ p,
                                            float* __restrict__ q, double* __restrict__ partial, int ob, double* shd) {
     const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    const int fi = ob * (256 / LANES) + grp;
+    const int fi = ob * (BS / LANES) + grp;
     float acc[12];
 #pragma unroll
     for (int a = 0; a < 12; a++) acc[a] = 0.0f;
@@ -792,18 +792,20 @@ __device__ __forceinline__ float cg_alpha(const SolverDev& S, const double* pq_p
 // CG iteration k (r.z partials of iterations k and k-1 in the two halves of
 // rzp). The vector is z, or (warm_iter >= 0) the warm-start step of that GN
 // iteration, or v_ext (single-scene test calls).
-template <int KT>
-__global__ void __launch_bounds__(256) apply_kernel(const SolverEntry* __restrict__ B, int k, int first, int warm_iter,
-                                                    const float* __restrict__ v_ext) {
+// BS: workgroup size (the CG block size: BS / 16 owners per workgroup, E.nb_apply
+// workgroups, one p.q partial each).
+template <int KT, int BS = 256>
+__global__ void __launch_bounds__(BS) apply_kernel(const SolverEntry* __restrict__ B, int k, int first, int warm_iter,
+                                                   const float* __restrict__ v_ext) {
     const SolverEntry E = load_entry(B);
-    if ((int)blockIdx.x >= E.nb_owner || *E.cg_done) return;
+    if ((int)blockIdx.x >= E.nb_apply || *E.cg_done) return;
     const SolverDev& S = E.S;
     const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
     const double* rz_prev = E.rzp + ((k + 1) & 1) * S.n_agg;
     const float* z = v_ext ? v_ext : (warm_iter >= 0 ? E.hwarm + (int64_t)warm_iter * E.n : E.z);
     __shared__ double shd[8];
     const float beta = cg_beta(S, rz_cur, rz_prev, first != 0, shd);
-    apply_body<KT>(S, E.x, z, beta, first != 0, E.p, E.q, E.partial, blockIdx.x, shd);
+    apply_body<KT, BS>(S, E.x, z, beta, first != 0, E.p, E.q, E.partial, blockIdx.x, shd);
 }
 
 // One workgroup per aggregate: (mode 1) h += alpha p, r -= alpha q; then
@@ -815,7 +817,7 @@ __global__ void __launch_bounds__(256) update_restrict_kernel(const SolverEntry*
     __shared__ float sred[256 / LANES][12];
     __shared__ double shd[8];
     const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
-    const float alpha = mode == 1 ? cg_alpha(S, E.partial, E.nb_owner, rz_cur, shd) : 0.0f;
+    const float alpha = mode == 1 ? cg_alpha(S, E.partial, E.nb_apply, rz_cur, shd) : 0.0f;
     update_body(S, mode, alpha, E.hsol, E.r, mode == 1 ? E.p : nullptr, mode == 1 ? E.q : nullptr, E.binv, E.z, E.rc,
                 E.partial2, blockIdx.x, sred, shd);
 }
@@ -1517,6 +1519,17 @@ __global__ void cg_begin_kernel(const SolverEntry* __restrict__ B) {
 }
 
 // Entries read this frame's node positions and aims.
+// Node positions of the frame copied for a side-stream coarse rebuild (the
+// caller's buffer changes under it: the node blend writes it in place).
+__global__ void snapshot_g_kernel(const SolverEntry* __restrict__ B, const SolverEntry* __restrict__ Bb) {
+    const SolverEntry E = load_entry(B);
+    const SolverEntry Eb = load_entry(Bb);
+    float* __restrict__ dst = const_cast<float*>(Eb.S.g);
+    const int64_t n = 3 * (int64_t)E.S.M;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = E.S.g[i];
+}
+
 struct FramePtrs {
     const float* g[SOLVER_BATCH_MAX];
     const float* aim[SOLVER_BATCH_MAX];
@@ -1589,6 +1602,7 @@ static SolverEntry entry_of(const SolverWork& W) {
     E.S = W.dev;
     E.n = W.n;
     E.nb_owner = (int)nblk(W.dev.nF, 256 / LANES);
+    E.nb_apply = (int)nblk(W.dev.nF, W.cg_block / LANES);
     E.e_blocks = W.e_blocks;
     E.acc_blocks = W.acc_blocks;
     E.n_cand = W.n_cand;
@@ -1604,9 +1618,26 @@ static SolverEntry entry_of(const SolverWork& W) {
     return E;
 }
 
+// The entry of a side-stream coarse rebuild: node positions from the snapshot,
+// x0 = identity, results into the spare centre / inverse buffers.
+static SolverEntry build_entry_of(const SolverWork& W) {
+    SolverEntry E = entry_of(W);
+    E.S.g = W.g_snap;
+    E.S.agg_center = W.agg_center_alt;
+    E.acinv = W.acinv_alt;
+    E.x = W.x_ident;
+    return E;
+}
+
 void solver_batch_free(SolverBatch& B) {
     if (B.entries) (void)hipFree(B.entries);
+    if (B.entries_build) (void)hipFree(B.entries_build);
+    if (B.fork_ev) (void)hipEventDestroy(B.fork_ev);
+    if (B.side) (void)hipStreamDestroy(B.side);
     B.entries = nullptr;
+    B.entries_build = nullptr;
+    B.fork_ev = nullptr;
+    B.side = nullptr;
     B.w.clear();
 }
 
@@ -1626,6 +1657,11 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
         const SolverEntry& E = host.back();
         if (W->dev.K != 8) B.kt = 0;
         B.max_nb_owner = std::max(B.max_nb_owner, E.nb_owner);
+        B.max_nb_apply = std::max(B.max_nb_apply, E.nb_apply);
+        if (W->cg_block != w[0]->cg_block) {
+            err = "solver batch: every solver of a batch needs the same CG block size";
+            return -1;
+        }
         B.max_agg = std::max(B.max_agg, E.S.n_agg);
         B.max_e_blocks = std::max(B.max_e_blocks, E.e_blocks);
         B.max_acc_blocks = std::max(B.max_acc_blocks, E.acc_blocks);
@@ -1637,9 +1673,15 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
         B.g.push_back(nullptr);  // the first set_frame always writes the entries' pointers
         B.aim.push_back(nullptr);
     }
+    std::vector<SolverEntry> host_build;
+    for (SolverWork* W : w) host_build.push_back(build_entry_of(*W));
     const size_t bytes = sizeof(SolverEntry) * host.size();
     if (hipMalloc(&B.entries, bytes) != hipSuccess ||
-        hipMemcpy(B.entries, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(B.entries, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&B.entries_build, bytes) != hipSuccess ||
+        hipMemcpy(B.entries_build, host_build.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&B.fork_ev, hipEventDisableTiming) != hipSuccess) {
         err = "solver batch: device allocation failed";
         solver_batch_free(B);
         return -2;
@@ -1684,25 +1726,62 @@ int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st
     return 0;
 }
 
+// Galerkin coarse operator at x0 = identity and its inverse for the entries E
+// (it is only a preconditioner, so callers may keep it across frames).
+static int coarse_build(const SolverBatch& B, const SolverEntry* E, hipStream_t st) {
+    const unsigned nz = (unsigned)B.w.size();
+    hipLaunchKernelGGL(agg_center_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(zero_coarse_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(coarse_assemble_kernel, dim3(B.max_groups128, 1, nz), dim3(128), 0, st, E);
+    if (B.max_multi > 0)
+        hipLaunchKernelGGL(coarse_multi_kernel, dim3(B.max_multi, 1, nz), dim3(256), (size_t)24 * B.max_agg, st, E);
+    hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(pad_identity_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
+    const unsigned nt = (unsigned)B.max_nt;
+    for (int kb = 0; kb < (int)nt; kb++) {
+        hipLaunchKernelGGL(gj_pivot_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, kb);
+        hipLaunchKernelGGL(gj_panel_kernel, dim3(nt, 1, nz), dim3(256), 0, st, E, kb);
+        hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nz), dim3(256), 0, st, E, kb);
+    }
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
 int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
     const SolverEntry* E = B.entries;
     const unsigned nz = (unsigned)B.w.size();
+    // adopt side-stream rebuilds that an earlier frame started
+    for (SolverWork* W : B.w) {
+        if (!W->pending) continue;
+        HIP_RET(hipStreamWaitEvent(st, W->built_ev, 0));
+        const size_t nc = (size_t)W->dev.nc_pad;
+        HIP_RET(hipMemcpyAsync(W->acinv, W->acinv_alt, sizeof(float) * nc * nc, hipMemcpyDeviceToDevice, st));
+        HIP_RET(hipMemcpyAsync(W->dev.agg_center, W->agg_center_alt, sizeof(float) * 3 * (size_t)W->dev.n_agg,
+                               hipMemcpyDeviceToDevice, st));
+        W->pending = false;
+    }
     hipLaunchKernelGGL(init_x_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
     if (refresh_coarse) {
-        // Galerkin coarse operator at x0 = identity and its inverse; it is only a
-        // preconditioner, so callers may keep it across frames.
-        hipLaunchKernelGGL(agg_center_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E);
-        hipLaunchKernelGGL(zero_coarse_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
-        hipLaunchKernelGGL(coarse_assemble_kernel, dim3(B.max_groups128, 1, nz), dim3(128), 0, st, E);
-        if (B.max_multi > 0)
-            hipLaunchKernelGGL(coarse_multi_kernel, dim3(B.max_multi, 1, nz), dim3(256), (size_t)24 * B.max_agg, st, E);
-        hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
-        hipLaunchKernelGGL(pad_identity_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
-        const unsigned nt = (unsigned)B.max_nt;
-        for (int kb = 0; kb < (int)nt; kb++) {
-            hipLaunchKernelGGL(gj_pivot_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, kb);
-            hipLaunchKernelGGL(gj_panel_kernel, dim3(nt, 1, nz), dim3(256), 0, st, E, kb);
-            hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nz), dim3(256), 0, st, E, kb);
+        bool all_have = true;
+        for (const SolverWork* W : B.w) all_have = all_have && W->have_coarse;
+        if (!B.async || !all_have) {
+            int rc = coarse_build(B, E, st);
+            if (rc) return rc;
+            for (SolverWork* W : B.w) W->have_coarse = true;
+        } else {
+            // rebuild from this frame's node positions beside the solve; the
+            // next frame adopts it (this one keeps the current operator)
+            hipLaunchKernelGGL(snapshot_g_kernel, dim3(64, 1, nz), dim3(256), 0, st, E,
+                               (const SolverEntry*)B.entries_build);
+            HIP_RET(hipGetLastError());
+            HIP_RET(hipEventRecord(B.fork_ev, st));
+            HIP_RET(hipStreamWaitEvent(B.side, B.fork_ev, 0));
+            int rc = coarse_build(B, B.entries_build, B.side);
+            if (rc) return rc;
+            for (SolverWork* W : B.w) {
+                HIP_RET(hipEventRecord(W->built_ev, B.side));
+                W->pending = true;
+            }
         }
     }
     // E(x0)
@@ -1710,6 +1789,28 @@ int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
                        (const float*)nullptr);
     hipLaunchKernelGGL(reduce_partials_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, 0, 1, 0, 0, (double*)nullptr);
     hipLaunchKernelGGL(set_eold_kernel, dim3(1, 1, nz), dim3(1), 0, st, E);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+// The apply kernel of the batch's K (unrolled for 8) and CG block size.
+static int launch_apply(const SolverBatch& B, unsigned nz, hipStream_t st, int k, int first, int warm_iter,
+                        const float* v_ext) {
+    const SolverEntry* E = B.entries;
+    const dim3 grid(B.max_nb_apply, 1, nz);
+    const int bs = B.w[0]->cg_block;
+#define ARAPGS_APPLY(KTV, BSV) \
+    hipLaunchKernelGGL((apply_kernel<KTV, BSV>), grid, dim3(BSV), 0, st, E, k, first, warm_iter, v_ext)
+    if (B.kt == 8) {
+        if (bs == 64) ARAPGS_APPLY(8, 64);
+        else if (bs == 128) ARAPGS_APPLY(8, 128);
+        else ARAPGS_APPLY(8, 256);
+    } else {
+        if (bs == 64) ARAPGS_APPLY(0, 64);
+        else if (bs == 128) ARAPGS_APPLY(0, 128);
+        else ARAPGS_APPLY(0, 256);
+    }
+#undef ARAPGS_APPLY
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -1722,8 +1823,8 @@ int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st) {
     hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
     if (B.w[0]->warm && iter < B.w[0]->warm_slots) {
         // q = A h0 (the first-iteration form of the apply), then h = h0, r = b - A h0
-        auto apply = B.kt == 8 ? apply_kernel<8> : apply_kernel<0>;
-        hipLaunchKernelGGL(apply, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E, 0, 1, iter, (const float*)nullptr);
+        int rc = launch_apply(B, nz, st, 0, 1, iter, nullptr);
+        if (rc) return rc;
         hipLaunchKernelGGL(warm_init_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E, iter);
     } else {
         hipLaunchKernelGGL(zero_step_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
@@ -1738,9 +1839,8 @@ int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st) {
 int solver_cg_iteration(SolverBatch& B, int k, hipStream_t st) {
     const SolverEntry* E = B.entries;
     const unsigned nz = (unsigned)B.w.size();
-    auto apply = B.kt == 8 ? apply_kernel<8> : apply_kernel<0>;
-    hipLaunchKernelGGL(apply, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E, k, k == 0 ? 1 : 0, -1,
-                       (const float*)nullptr);
+    int rc = launch_apply(B, nz, st, k, k == 0 ? 1 : 0, -1, nullptr);
+    if (rc) return rc;
     hipLaunchKernelGGL(update_restrict_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, 1, k);
     hipLaunchKernelGGL(coarse_prolong_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, k, 1);
     HIP_RET(hipGetLastError());
@@ -1807,8 +1907,8 @@ int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_o
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
                        (const int*)nullptr);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, E);
-    auto apply = W.dev.K == 8 ? apply_kernel<8> : apply_kernel<0>;
-    hipLaunchKernelGGL(apply, dim3(nb_owner), dim3(256), 0, st, E, 0, 1, -1, v);
+    int rc = launch_apply(B, 1, st, 0, 1, -1, v);
+    if (rc) return rc;
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, q_out, (const float*)W.q, (int64_t)W.n,
                        (const int*)nullptr);
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, b_out, (const float*)W.r, (int64_t)W.n,

@@ -82,6 +82,7 @@ constexpr int GN_MAX_ITERS = 64;
 struct SolverEntry {
     SolverDev S;
     int n, nb_owner, e_blocks, acc_blocks, n_cand, nt;  // nt = nc_pad / 64 coarse tiles
+    int nb_apply;      // workgroups of the CG apply (cg_block / 16 owners each)
     int cg_max_iters;
     double tol2;
     float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col, *hwarm;
@@ -125,9 +126,21 @@ struct SolverWork {
     int* cg_done;
     int cg_max_iters;
     double cg_tol;
+    int cg_block = 256;  // workgroup size of the CG apply: 64, 128 or 256 (16 lanes per owner node)
     void* gbar;        // grid barrier of the persistent PCG kernel
     int coop_grid;     // its grid size (0: not yet chosen)
     double* energies;  // GN_MAX_ITERS + 1
+    // Asynchronous coarse refresh: a rebuild of the coarse space (aggregate
+    // centres and the inverse Galerkin matrix) runs on a side stream from a
+    // snapshot of the node positions into the *_alt buffers; the next frame's
+    // setup waits for built_ev and copies them over the current ones.
+    float* acinv_alt = nullptr;
+    float* agg_center_alt = nullptr;
+    float* g_snap = nullptr;   // M*3
+    float* x_ident = nullptr;  // n: identity rotations, zero translations (x0 of the Galerkin operator)
+    hipEvent_t built_ev = nullptr;
+    bool pending = false;      // a side-stream rebuild waits to be adopted
+    bool have_coarse = false;  // a coarse inverse exists (the first one is built in line)
     std::vector<void*> allocs;  // owned device buffers
 };
 
@@ -137,9 +150,13 @@ struct SolverWork {
 struct SolverBatch {
     std::vector<SolverWork*> w;
     SolverEntry* entries = nullptr;   // device, w.size()
+    SolverEntry* entries_build = nullptr;  // device: the entries of a side-stream coarse rebuild
+    hipStream_t side = nullptr;       // side stream of the asynchronous coarse refresh
+    hipEvent_t fork_ev = nullptr;
+    bool async = true;                // coarse refreshes on the side stream (the first one in line)
     std::vector<const float*> g, aim; // what the device entries hold
     int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
-    int max_nb_owner = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
+    int max_nb_owner = 0, max_nb_apply = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
         max_multi = 0, max_nt = 0, max_cand = 0;
 };
 

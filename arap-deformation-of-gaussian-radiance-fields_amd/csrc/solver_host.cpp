@@ -40,6 +40,8 @@ T* alloc(SolverWork& W, size_t n, int& rc) {
 void solver_free(SolverWork& W) {
     for (void* p : W.allocs) (void)hipFree(p);
     W.allocs.clear();
+    if (W.built_ev) (void)hipEventDestroy(W.built_ev);
+    W.built_ev = nullptr;
 }
 
 int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
@@ -228,11 +230,21 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.binv = alloc<float>(W, (size_t)144 * nF, rc);
     W.rc = alloc<float>(W, 12 * (size_t)n_agg, rc);
     W.acinv = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
+    W.acinv_alt = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
+    W.agg_center_alt = alloc<float>(W, 3 * (size_t)n_agg, rc);
+    W.g_snap = alloc<float>(W, 3 * (size_t)d.M, rc);
+    {
+        std::vector<float> ident((size_t)W.n, 0.0f);
+        for (int i = 0; i < nF; i++) ident[12 * (size_t)i] = ident[12 * (size_t)i + 4] = ident[12 * (size_t)i + 8] = 1.0f;
+        W.x_ident = upload(W, ident, rc);
+    }
+    if (!rc && hipEventCreateWithFlags(&W.built_ev, hipEventDisableTiming) != hipSuccess) rc = -2;
     W.ac64 = alloc<unsigned long long>(W, (size_t)S.nc_pad * S.nc_pad, rc);
     W.gj_piv = alloc<float>(W, 64 * 64, rc);
     W.gj_col = alloc<float>(W, (size_t)S.nc_pad * 64, rc);
     const int nb_owner = (nF + 15) / 16;
-    W.partial = alloc<double>(W, std::max(nb_owner, 1), rc);
+    // p.q partials of the CG apply at the smallest CG block size (64: 4 owners per workgroup)
+    W.partial = alloc<double>(W, std::max((nF + 3) / 4, 1), rc);
     W.partial2 = alloc<double>(W, n_agg, rc);
     W.rzp = alloc<double>(W, 2 * (size_t)n_agg, rc);
     W.e_blocks = (int)std::min<int64_t>(512, (S.n_groups + 255) / 256);

@@ -18,7 +18,15 @@ the frame splits into two halves on two streams:
                            (the slot is free again after B[t%2] and C[t%2])
 
 and frame t's deform side runs while frame t+1's solve runs; the samples (they
-read only the node table) run beside the six-point update and the renders. Each side has its
+read only the node table) run beside the six-point update and the renders.
+
+Batched sample frames (sample_batch = B > 1): nothing reads the samples during a
+drag (they feed the density-preserving step after the release), so stream C
+waits for B frames and applies their B node tables to the samples in one pass
+(arapgs_samples_frames: frame after frame in order, bit-identical to B
+per-frame sample updates, with the sample streams read and written once);
+B + 2 table slots keep the tables alive until their batch has run, and run()
+flushes a partial batch before it returns. Each side has its
 own host thread, because each side synchronises with the host (the solve's CG
 convergence checks, the renderer's instance count) and those waits must not
 stall the other side. The two node-table slots bound the lead of the solve side
@@ -35,12 +43,17 @@ from . import arapgs
 
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
-                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True):
+                 stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
+                 aims_fn=None, sample_batch=1):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
         beside the six-point update and the renders (they read only the node
-        table), so they fill the CUs the renders' deep tiles leave idle."""
+        table), so they fill the CUs the renders' deep tiles leave idle.
+        aims_fn(ctx, node_pos, aims): the frame's aims on the solve stream
+        (default: the active nodes translated by delta, UpdateAimPosition).
+        sample_batch: drag frames per sample update pass (1..8; more than one
+        needs split_samples, K = 8 and samples with features)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -52,25 +65,34 @@ class DragPipeline:
         if render_ctx is not None:
             self.stream_b = render_ctx.stream
         self.split = split_samples and state.get("samples") is not None and state["samples"].shape[0] > 0
+        if sample_batch < 1 or sample_batch > 8:
+            raise ValueError("sample_batch must be in 1..8")
+        self.nb = sample_batch if (self.split and state.get("sample_feat") is not None) else 1
+        self.R = 2 if self.nb == 1 else self.nb + 2
+        self.pending = []
         self.stream_c = torch.cuda.Stream(device=dev) if self.split else None
         self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
-        self.ev_c = [torch.cuda.Event() for _ in range(2)]
+        self.ev_c = [torch.cuda.Event() for _ in range(self.R)]
+        self.ev_c_recorded = [False] * self.R
         self.st = state
         self.k = k
         self.active, self.delta = active, delta
+        self.aims_fn = aims_fn
         self.iters, self.stop_rule = gn_iters, stop_rule
         self.cams, self.images, self.bg, self.opac, self.D = cameras, images, bg, opacity, sh_degree
         self.tile = tile
         M = state["node_pos"].shape[0]
         with torch.cuda.stream(self.stream_a):
             self.aims = torch.empty_like(state["node_pos"])
-            self.tables = [arapgs.node_table(M, dev) for _ in range(2)]
-            self.x = [torch.empty(solver.n_unknowns, dtype=torch.float32, device=dev) for _ in range(2)]
-            self.energies = [torch.zeros(gn_iters + 1, dtype=torch.float64, device=dev) for _ in range(2)]
-            self.iters_done = [torch.zeros(1, dtype=torch.int32, device=dev) for _ in range(2)]
-        self.ev_a = [torch.cuda.Event() for _ in range(2)]
-        self.ev_b = [torch.cuda.Event() for _ in range(2)]
-        self.ev_b_recorded = [False, False]
+            self.tables = [arapgs.node_table(M, dev) for _ in range(self.R)]
+            self.quats = [torch.empty((M, 4), dtype=torch.float32, device=dev) for _ in range(self.R)] \
+                if self.nb > 1 else []
+            self.x = [torch.empty(solver.n_unknowns, dtype=torch.float32, device=dev) for _ in range(self.R)]
+            self.energies = [torch.zeros(gn_iters + 1, dtype=torch.float64, device=dev) for _ in range(self.R)]
+            self.iters_done = [torch.zeros(1, dtype=torch.int32, device=dev) for _ in range(self.R)]
+        self.ev_a = [torch.cuda.Event() for _ in range(self.R)]
+        self.ev_b = [torch.cuda.Event() for _ in range(self.R)]
+        self.ev_b_recorded = [False] * self.R
         self.instances = 0
         torch.cuda.synchronize(dev)
 
@@ -81,9 +103,12 @@ class DragPipeline:
             if self.ev_b_recorded[slot]:
                 # the slot is free once both deform streams are done with it
                 self.stream_a.wait_event(self.ev_b[slot])
-                if self.split:
-                    self.stream_a.wait_event(self.ev_c[slot])
-            self.ctx_a.aims_translate(st["node_pos"], self.active, self.delta, self.aims)
+            if self.ev_c_recorded[slot]:
+                self.stream_a.wait_event(self.ev_c[slot])
+            if self.aims_fn is not None:
+                self.aims_fn(self.ctx_a, st["node_pos"], self.aims)
+            else:
+                self.ctx_a.aims_translate(st["node_pos"], self.active, self.delta, self.aims)
             self.solver.solve(st["node_pos"], self.aims, self.iters, self.stop_rule, x_out=self.x[slot],
                               energies=self.energies[slot], iters_done=self.iters_done[slot])
             self.solver.put_free_inputs(st["node_pos"], self.x[slot], table=self.tables[slot])
@@ -100,15 +125,38 @@ class DragPipeline:
                 q = ctx.node_quats(table)
                 ctx.sample_sh(st["sample_w"], st["sample_idx"], q, st.get("sample_static"), st["sample_feat"])
 
+    def _flush_samples(self):
+        """The pending frames' sample updates in one pass on stream C."""
+        if not self.pending:
+            return
+        st, torch = self.st, self.torch
+        with torch.cuda.stream(self.stream_c):
+            # stream A is in order: the last pending frame's solve implies the others
+            self.stream_c.wait_event(self.ev_a[self.pending[-1]])
+            for slot in self.pending:
+                self.ctx_c.node_quats(self.tables[slot], out=self.quats[slot])
+            self.ctx_c.samples_frames([self.tables[k] for k in self.pending], [self.quats[k] for k in self.pending],
+                                      st["sample_idx"], st["sample_w"], st.get("sample_static"), st["samples"],
+                                      st["sample_feat"])
+            for slot in self.pending:
+                self.ev_c[slot].record(self.stream_c)
+                self.ev_c_recorded[slot] = True
+        self.pending = []
+
     def _deform_half(self, slot, render=True):
         st, torch = self.st, self.torch
         ctx = self.ctx_b
         table = self.tables[slot]
-        if self.split:
+        if self.split and self.nb > 1:
+            self.pending.append(slot)
+            if len(self.pending) == self.nb:
+                self._flush_samples()
+        elif self.split:
             with torch.cuda.stream(self.stream_c):
                 self.stream_c.wait_event(self.ev_a[slot])
                 self._samples(self.ctx_c, table)
                 self.ev_c[slot].record(self.stream_c)
+                self.ev_c_recorded[slot] = True
         with torch.cuda.stream(self.stream_b):
             self.stream_b.wait_event(self.ev_a[slot])
             if not self.split:
@@ -139,8 +187,8 @@ class DragPipeline:
                     free.acquire()
                     if err:
                         return
-                    self._solve_half(t % 2)
-                    slots.put(t % 2)
+                    self._solve_half(t % self.R)
+                    slots.put(t % self.R)
             except BaseException as e:  # noqa: BLE001 - re-raised on the caller's thread
                 err.append(e)
                 slots.put(None)
@@ -152,6 +200,8 @@ class DragPipeline:
                     if slot is None:
                         return
                     n = self._deform_half(slot, render)
+                    if t == n_frames - 1:
+                        self._flush_samples()
                     free.release()
                     self.instances = n
                     if on_frame is not None:

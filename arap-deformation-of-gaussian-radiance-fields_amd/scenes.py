@@ -107,6 +107,24 @@ def look_at_camera(eye, target, fov_deg=50.0, W=800, H=800, up=(0.0, 0.0, 1.0), 
                   eye.astype(np.float32), float(tanfovx), float(tanfovy), W, H)
 
 
+def ortho_camera(eye, target, ortho_scale=2.0, W=800, H=800, up=(0.0, 0.0, 1.0), znear=0.01, zfar=100.0,
+                 fov_deg=50.0) -> Camera:
+    """Orthographic camera: GaussianView's setOrthoCam(ortho_scale, ortho_scale)
+    (GaussianView.cpp:1018-1019, sibr::orthographic) in the rasteriser's
+    convention (x right, y down, z forward): x_ndc = x_view / ortho_scale,
+    y_ndc = y_view / ortho_scale, w = 1. tan_fov is passed along unused."""
+    c = look_at_camera(eye, target, fov_deg, W, H, up, znear, zfar)
+    V = c.view.reshape(4, 4).T.astype(np.float64)
+    Pm = np.zeros((4, 4))
+    Pm[0, 0] = 1.0 / ortho_scale
+    Pm[1, 1] = 1.0 / ortho_scale
+    Pm[2, 2] = 1.0 / (zfar - znear)
+    Pm[2, 3] = -znear / (zfar - znear)
+    Pm[3, 3] = 1.0
+    full = Pm @ V
+    return Camera(c.view, full.T.reshape(16).astype(np.float32), c.campos, c.tanfovx, c.tanfovy, W, H)
+
+
 def ring_cameras(n, radius, elevation_deg, fov_deg=50.0, W=800, H=800):
     cams = []
     el = math.radians(elevation_deg)

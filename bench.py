@@ -19,7 +19,7 @@ Three forms are timed from the same starting state (the drag is reset before eac
              sync after each launch);
   fast       stages queued back to back; the CG phases carry a device-side
              convergence flag (kernels queued past convergence exit at once)
-             and the host looks at it only every 16 CG iterations; the views
+             and the host looks at it only every few CG iterations; the views
              are rendered in one pass (arapgs_render_views);
   pipelined  as fast, with frame t's deformation and renders on a second stream
              overlapping frame t+1's solve (pipeline.DragPipeline: two node-table
@@ -27,7 +27,22 @@ Three forms are timed from the same starting state (the drag is reset before eac
              results are bit-identical to the sequential frames.
 `value` is the best form's frames/s (normally pipelined); every form's frames/s
 and p50/p99 frame intervals are reported under `forms`, with `plain_fps` and
-`speedup_vs_plain`. The six-point roofline is timed in the sequential fast form.
+`speedup_vs_plain`. The six-point, sample blend and sample SH kernels are timed
+with HIP events in the fast form (`rooflines`; `roofline` is the longest one).
+
+Further legs, each in its own JSON object:
+  fixed_iteration  the full edit in one frame, 10 GN iterations with the stop
+                   rule off, CG cold-started (plain and fast forms);
+  realign          density-preserving realignment steps after the drag;
+  cpu_baseline     one complete drag frame of the same workload on the oracle
+                   (the CPU restatement of the reference path), measured, plus
+                   the GPU's energies of the timed frames against the oracle's
+                   FP64 Gauss-Newton solve (the timed frames are re-run with
+                   their inputs recorded; the final state must match);
+  adjoint          the differentiation workload (500k Gaussians);
+  batch            32 uneven scenes, batched vs scene after scene, 64 frames;
+  large            3M flat Gaussians, 4096 nodes, 128^3 grid, 8 views at
+                   1920x1080: pipelined drag, kernel HBM rates, release step.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -44,9 +59,18 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "arap-deformation-of-gaussian-radiance-fields_amd")
 HBM_PEAK_GBS = 8000.0
-# 6 ends 72 + idx 192 + w 192 + s0 12 + static flag 1 + q 16 + SH 192 read;
-# mu 12 + q 16 + s 12 + SH 192 + ends 72 written
+# Algorithmic bytes per unit of the three deformation kernels timed with HIP events:
+#   six-point, per Gaussian: 6 ends 72 + idx 192 + w 192 + s0 12 + static flag 1 + q 16 + SH 192 read;
+#                            mu 12 + q 16 + s 12 + SH 192 + ends 72 written
+#   sample blend, per sample: idx 32 + w 32 + position 12 read, position 12 written
+#   sample SH, per sample: idx 32 + w 32 + feature 192 read, feature 192 written
+# (the node tables, M x 64 B and M x 16 B, are read once per launch and left out)
 SIX_POINT_BYTES_PER_GAUSSIAN = 981
+SAMPLE_BLEND_BYTES_PER_SAMPLE = 88
+SAMPLE_SH_BYTES_PER_SAMPLE = 448
+KERNELS = (("six_point", "six_point_kernel<8, true> (fused end-point blend + polar + SH)", 0),
+           ("sample_blend", "blend_points_kernel<8, true> (sample positions)", 1),
+           ("sample_sh", "sample_sh_kernel<8> (slerp of node rotations + SH rotation of the sample features)", 2))
 
 
 def load_package():
@@ -94,7 +118,7 @@ def setup(args):
 
 def reset_state(B):
     """Every form (and every autotuner candidate) starts the drag from the same
-    state, so all of them time the same frames. Python's garbage collector runs
+    state, Gaussians and solver alike, so all of them time the same frames. Python's garbage collector runs
     here, outside the timed frames, and is paused until the next reset."""
     import gc
     import torch
@@ -106,21 +130,24 @@ def reset_state(B):
             if v is not None:
                 v.copy_(B["state0"][k])
     torch.cuda.synchronize()
+    if B.get("solver") is not None:
+        # warm-start steps, coarse operator and refresh count as a new solver's
+        B["solver"].reset()
 
 
-def frame(B, plain):
+def frame(B, plain, delta=None, stop_rule=True):
     import torch
     ctx, st = B["ctx"], B["state"]
     with torch.cuda.stream(ctx.stream):
-        return _frame(B, plain)
+        return _frame(B, plain, B["delta"] if delta is None else delta, stop_rule)
 
 
-def _frame(B, plain):
+def _frame(B, plain, delta, stop_rule):
     ctx, st = B["ctx"], B["state"]
     sync = ctx.synchronize if plain else (lambda: None)
-    ctx.aims_translate(st["node_pos"], B["active"], B["delta"], B["aims"])
+    ctx.aims_translate(st["node_pos"], B["active"], delta, B["aims"])
     sync()
-    _, B["iters_done"] = B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=True)
+    B["energies"], B["iters_done"] = B["solver"].deform_frame(B["fb"], B["aims"], B["iters"], stop_rule=stop_rule)
     sync()
     if not plain:
         # all views in one pass (one sort, one blend launch; bit-identical images)
@@ -137,7 +164,7 @@ def _frame(B, plain):
     return n_inst
 
 
-def timed_pipelined(B, steps, warmup, solve_cus=None):
+def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     """Frame t's deformation and renders overlap frame t+1's solve (two streams,
     two host threads; pipeline.DragPipeline). solve_cus > 0: the solve stream
     runs on CUs [0, solve_cus) and the deform stream on the others."""
@@ -157,9 +184,10 @@ def timed_pipelined(B, steps, warmup, solve_cus=None):
         B["solver"].set_cg_mode(0)
         B["solver"].set_host_checks(B["cfg"]["host_checks"])
         B["solver"].set_warm_start(B["cfg"]["warm"])
+        nb = B["cfg"].get("sample_batch", 1) if sample_batch is None else sample_batch
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
-                            stream_b=streams[1].torch if streams else None)
+                            stream_b=streams[1].torch if streams else None, sample_batch=nb)
         pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
         pipe.run(warmup)
         torch.cuda.synchronize()
@@ -176,7 +204,21 @@ def timed_pipelined(B, steps, warmup, solve_cus=None):
         for s_ in streams:
             s_.close()
     times = [(b - a) * 1e3 for a, b in zip([t0] + stamps[:-1], stamps)]
-    return dict(total_s=total, frame_ms=times, kernel_ms=[], instances=instances)
+    return dict(total_s=total, frame_ms=times, kernel_ms={}, instances=instances)
+
+
+def sample_batch_leg(B, args):
+    """Picks the drag frames per sample update pass of the pipelined form
+    from a few pipelined frames per candidate (results are identical)."""
+    best, best_t, trials = 1, None, {}
+    for nb in (1, 2, 4, 8):
+        r = timed_pipelined(B, args.tune_steps * 2, 2, sample_batch=nb)
+        t = r["total_s"] / (args.tune_steps * 2)
+        trials[nb] = round(t * 1e3, 3)
+        log(f"  sample batch {nb}: {t * 1e3:.3f} ms/frame")
+        if best_t is None or t < best_t:
+            best, best_t = nb, t
+    return best, trials
 
 
 def partition_leg(B, args):
@@ -213,7 +255,7 @@ def timed(B, plain, steps, warmup):
         times.append((time.perf_counter() - a) * 1e3)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    kernel_ms = ctx.profile_read()
+    kernel_ms = {name: ctx.profile_read(which=which) for name, _, which in KERNELS}
     ctx.profile_enable(False)
     gn, cg = B["solver"].last_stats()
     log(f"  last frame: {gn} GN iterations, {cg} CG iterations")
@@ -232,6 +274,7 @@ def apply_config(B, cfg):
             B["solver"] = workload.make_solver(ctx, B["solver_args"], n_aggregates=cfg["aggregates"])
         torch.cuda.synchronize()
     B["solver"].set_coarse_refresh(cfg["refresh"])
+    B["solver"].set_cg_block(cfg.get("cg_block", 256))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     B["cfg"] = dict(cfg)
 
@@ -251,19 +294,42 @@ def autotune_leg(B, args):
     return best, trials
 
 
-def committed_traffic(args):
-    """HBM bytes per six-point launch measured with rocprofv3 PMC counters
+def committed_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
     (FETCH_SIZE, WRITE_SIZE, scaled by a plain-copy calibration) on the default
     workload, read from the committed summary; None for other workloads."""
     default = (args.n, args.nodes, args.seed) == (1_000_000, 2048, 0)
-    path = os.path.join(ROOT, "profiles", "six_point_kernel_capsules-1M_traffic.md")
+    path = os.path.join(ROOT, "profiles", "deform_kernels_capsules-1M_traffic.md")
     if not default or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch six_point_kernel:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return int(line.split(":")[1])
     return None
+
+
+def rooflines(args, B, kernel_ms):
+    """HBM roofline of each timed deformation kernel: algorithmic bytes per
+    launch over its average duration (HIP events on the kernel's stream)."""
+    n = args.n
+    S = int(B["state"]["samples"].shape[0])
+    units = {"six_point": (SIX_POINT_BYTES_PER_GAUSSIAN, n, "Gaussians"),
+             "sample_blend": (SAMPLE_BLEND_BYTES_PER_SAMPLE, S, "samples"),
+             "sample_sh": (SAMPLE_SH_BYTES_PER_SAMPLE, S, "samples")}
+    out = {}
+    for name, label, _ in KERNELS:
+        kms = kernel_ms.get(name, [])
+        per, cnt, what = units[name]
+        avg = float(sum(kms) / len(kms)) if len(kms) else float("nan")
+        b = per * cnt
+        ach = b / (avg * 1e-3) / 1e9 if len(kms) else None
+        out[name] = {"bound": "hbm", "kernel": label, "achieved": round(ach, 1) if ach else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
+                     "traffic": committed_traffic(args, name), "bytes_per_launch": b,
+                     "bytes_per_unit": per, "units_per_launch": cnt, "unit_name": what,
+                     "avg_launch_ms": round(avg, 4), "launches": int(len(kms))}
+    return out
 
 
 def pct(v, q):
@@ -274,48 +340,338 @@ def pct(v, q):
     return s[k]
 
 
-def cpu_baseline(args):
+def record_frames(B, steps, warmup):
+    """Re-runs the timed pipelined drag (same start state, same settings) while
+    keeping every frame's solve inputs (node positions, aims) and outputs
+    (energies, GN iterations), copied on the solve stream, and the state at the
+    start of the first frame. The final state must equal the timed run's bit for
+    bit, so the recorded frames are the timed frames."""
+    import numpy as np
+    import torch
+    from arapgs_amd.pipeline import DragPipeline
+
+    class Recording(DragPipeline):
+        def _solve_half(self, slot):
+            with torch.cuda.stream(self.stream_a):
+                g = self.st["node_pos"].clone()
+            super()._solve_half(slot)
+            with torch.cuda.stream(self.stream_a):
+                self.rec.append((g, self.aims.clone(), self.energies[slot].clone(), self.iters_done[slot].clone()))
+
+    final_timed = {k: B["state"][k].clone() for k in ("pos", "rot", "scale", "sh", "node_pos", "samples",
+                                                      "sample_feat")}
+    reset_state(B)
+    start = {k: (v.cpu().numpy() if v is not None else None) for k, v in B["state"].items()}
+    B["solver"].set_cg_mode(0)
+    B["solver"].set_host_checks(B["cfg"]["host_checks"])
+    B["solver"].set_warm_start(B["cfg"]["warm"])
+    pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
+                     B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1))
+    pipe.rec = []
+    pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
+    pipe.run(warmup + steps)
+    torch.cuda.synchronize()
+    same = all(torch.equal(final_timed[k], B["state"][k]) for k in final_timed)
+    rec = [(g.cpu().numpy(), a.cpu().numpy(), e.cpu().numpy(), int(d.item())) for g, a, e, d in pipe.rec]
+    pipe.close()
+    return dict(frames=rec, timed_from=warmup, same_as_timed=bool(same), start=start)
+
+
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = nproc
+    return model, nproc, share
+
+
+def _set_threads(n):
+    """OpenMP threads of the oracle library for the following calls."""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def cpu_frame(B, rec, O):
+    """One complete drag frame of the reference's CPU path on the oracle
+    (C++/OpenMP, FP64 Gauss-Newton with the reference's per-iteration ordering
+    and two factorisations), from the GPU's state at the start of the timed
+    drag: solve, node table, sample blend, end-point blend + six-point update,
+    node blend, node quaternions + sample SH rotation. The reference renders on
+    the GPU, so the frame has no render. Returns per-stage seconds and the
+    solve's energy / iterations."""
+    import numpy as np
+    st = rec["start"]
+    g, aims = rec["frames"][0][0], rec["frames"][0][1]
+    nbr, blocks, types, src_idx, src_w, _ = B["solver_args"]
+    K = B["K"]
+    t = {}
+    t0 = time.perf_counter()
+    prob = O.GNProblem(g, nbr, blocks, types, src_idx, src_w.astype(np.float64), aims)
+    x, e, it = prob.solve(B["iters"], stop_rule=True, reference_cost=True)
+    t["solve"] = time.perf_counter() - t0
+    # putFreeInputs (Deform.hpp:140-151): free nodes in ascending order, x column-major
+    M = g.shape[0]
+    static = np.zeros(M, bool)
+    for blk, ty in zip(blocks, types):
+        if ty == -1:
+            static[np.asarray(blk)] = True
+    free = np.nonzero(~static)[0]
+    rot9 = np.tile(np.eye(3).reshape(1, 9), (M, 1))
+    t3 = np.zeros((M, 3))
+    xr = x.reshape(-1, 12)
+    rot9[free] = xr[:, :9]
+    t3[free] = xr[:, 9:]
+    t0 = time.perf_counter()
+    O.blend(st["samples"], st["sample_idx"], st["sample_w"], g, rot9, t3)
+    t["sample_blend"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ends = O.blend(st["ends"].reshape(-1, 3), st["ends_idx"].reshape(-1, K), st["ends_w"].reshape(-1, K), g, rot9,
+                   t3).reshape(-1, 6, 3)
+    O.six_point(ends, st["scale0"], None, st["pos"], st["rot"], st["scale"], st["sh"])
+    t["six_point"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.blend(g, st["node_src_idx"], st["node_src_w"], g, rot9, t3)
+    q = O.node_quats(rot9)
+    O.sample_sh(st["sample_w"], st["sample_idx"], q, None, st["sample_feat"])
+    t["nodes_sample_sh"] = time.perf_counter() - t0
+    return dict(seconds=t, total_s=sum(t.values()), energy=float(e[-1]), gn_iters=int(it))
+
+
+def cpu_baseline(args, B, rec):
     """The oracle (CPU restatement of the reference path, C++/OpenMP, FP64
-    solve) timed on a bounded sample of the same workload."""
+    solve) running one complete drag frame of the same workload, timed on this
+    host's cores at min(16, nproc) threads (the reference's hard-coded 16,
+    main.cpp:102-103) and at all the cores of this process; it also checks the
+    GPU's energies of the timed frames against the oracle's Gauss-Newton solve."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    model, nproc, share = _cpu_info()
+    res = {}
+    for label, th in (("ref_threads", min(16, nproc)), ("all_cores", share)):
+        if label == "all_cores" and th == res.get("ref_threads", {}).get("threads"):
+            res[label] = dict(res["ref_threads"], note="same thread count as ref_threads")
+            continue
+        _set_threads(th)
+        log(f"cpu baseline: one complete frame on the oracle, {th} threads ...")
+        f = cpu_frame(B, rec, O)
+        f["threads"] = th
+        res[label] = f
+        log(f"  cpu frame: {f['total_s']:.2f} s ({', '.join(f'{k} {v:.2f}' for k, v in f['seconds'].items())})")
+    _set_threads(min(16, nproc))
+    # energies of the timed frames against the oracle's solve of the same problems
+    frames = rec["frames"][rec["timed_from"]:]
+    if args.energy_check_frames >= 0:
+        frames = frames[: args.energy_check_frames]
+    nbr, blocks, types, src_idx, src_w, _ = B["solver_args"]
+    errs, gn = [], []
+    t0 = time.perf_counter()
+    for g, aims, e, done in frames:
+        prob = O.GNProblem(g, nbr, blocks, types, src_idx, src_w.astype(np.float64), aims)
+        _, er, it = prob.solve(B["iters"], stop_rule=True)
+        errs.append(abs(e[done] - er[-1]) / er[-1])
+        gn.append((done, it))
+    log(f"  energy check: {len(frames)} timed frames in {time.perf_counter() - t0:.1f} s, "
+        f"max rel err {max(errs) if errs else float('nan'):.2e}")
+    first = rec["frames"][0]
+    main = res["ref_threads"]
+    return dict(
+        value=round(1.0 / main["total_s"], 5), unit="frames/s", cores=main["threads"], kind="port",
+        sample=(f"one complete drag frame of the same workload (1M Gaussians, {B['state']['samples'].shape[0]} "
+                f"samples, 2048 nodes, the GPU's first drag frame from the same state) on the oracle (C++/OpenMP, "
+                f"FP64 Gauss-Newton with a minimum-degree block Cholesky, re-analysed and factorised twice per "
+                f"iteration as Deform.cpp:131-132 does; {main['gn_iters']} GN iterations, the GPU took {first[3]}): "
+                f"solve, sample blend, end-point blend + six-point, node blend, sample SH; no render (the reference "
+                f"renders on the GPU); measured, not extrapolated"),
+        cpu_model=model, nproc=nproc, cpu_share=share, frames=res,
+        energy_check={"frames": len(frames), "max_rel_err": max(errs) if errs else None,
+                      "gn_iters_gpu_oracle": gn, "bar": 1e-5,
+                      "recorded_frames_equal_timed": rec["same_as_timed"]},
+        first_frame_energy={"gpu": float(first[2][first[3]]), "oracle": main["energy"]})
+
+
+def fixed_iteration_leg(B, args):
+    """The unit of work with a fixed iteration count: one frame of the whole
+    edit (the active capsule translated by the full 0.3 in one frame), 10
+    Gauss-Newton iterations with the stop rule off and the CG cold-started,
+    then the deformation and the 4 views; every frame starts from the rest
+    state (restored outside the timed frame)."""
+    import torch
+    from arapgs_amd import workload
+    full = workload.drag_delta(B["sc"], 1)
+    res = {}
+    B["solver"].set_warm_start(0)
+    for form in ("plain", "fast"):
+        plain = form == "plain"
+        B["solver"].set_cg_mode(0)
+        B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
+        ms = []
+        for i in range(args.warmup + args.fixed_steps):
+            reset_state(B)
+            a = time.perf_counter()
+            frame(B, plain, delta=full, stop_rule=False)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                ms.append((time.perf_counter() - a) * 1e3)
+        gn, cg = B["solver"].last_stats()
+        e = B["energies"].cpu().numpy()
+        res[form] = {"fps": round(len(ms) / (sum(ms) / 1e3), 3), "p50_ms": round(pct(ms, 50), 3),
+                     "p99_ms": round(pct(ms, 99), 3), "gn_iters": gn, "cg_iters": cg,
+                     "energy_first_last": [float(e[0]), float(e[gn])]}
+        log(f"  fixed-iteration {form}: {res[form]['fps']} frames/s, p50 {res[form]['p50_ms']} ms "
+            f"({gn} GN / {cg} CG iterations)")
+    B["solver"].set_warm_start(B["cfg"]["warm"])
+    reset_state(B)
+    return {"workload": "capsules-1M, full 0.3 edit in one frame", "gn_iters": 10, "stop_rule": False,
+            "cg_warm_start": False, "frames": args.fixed_steps, **res,
+            "speedup_fast_vs_plain": round(res["fast"]["fps"] / res["plain"]["fps"], 3)}
+
+
+def large_leg(args):
+    """The large high-frequency workload: 3M flat Gaussians, 4096 nodes, K=8,
+    128^3 density grid, the combined bend + twist + 1.3x scale edit spread over a
+    30-frame drag (per frame: translate, twist about the active capsule's axis,
+    scale about the active centroid), up to 12 Gauss-Newton iterations with the
+    stop rule, 8 views at 1920x1080. Pipelined drag frames (p50/p99), the
+    deformation kernels' HBM rate from a few sequential frames, and the
+    density-preserving step of the release (density sample over the 128^3 cell
+    lists + realignment steps)."""
+    import math
+    import numpy as np
+    import torch
     load_package()
-    from arapgs_amd import scenes
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sc = scenes.capsule_scene(args.seed, n=args.n, n_nodes=args.nodes)
-    g = sc.gaussians
-    K = sc.knn_k
-    rng = np.random.default_rng(0)
-    sub = np.sort(rng.choice(g.n, size=min(args.cpu_sample, g.n), replace=False))
-    # graph on the full scene (not timed)
-    ni = O.fps(g.pos, args.nodes)
-    nodes = g.pos[ni]
-    nbr = O.graph_edges(nodes, K)
-    nidx, nw = O.knn_weights(nodes, nodes, K)
-    blocks, types = scenes.blocks_for(sc, nodes)
-    aims = scenes.frame_aims(sc, nodes, blocks, types, 1.0 / args.drag_frames)
-    prob = O.GNProblem(nodes, nbr, blocks, types, nidx[:, :K], nw, aims)
-    log(f"cpu baseline: 1 GN iteration at {prob.n} unknowns ...")
-    t = time.perf_counter()
-    prob.solve(1, stop_rule=False)
-    t_gn1 = time.perf_counter() - t
-    # deform of a slice: end-point blend + six-point update (+ SH)
-    pos, rot, scale, sh = g.pos[sub], g.rot[sub], g.scale[sub], g.sh[sub]
-    ends = O.end_points(pos, rot, scale)
-    eidx, ew = O.knn_weights(nodes, ends.reshape(-1, 3), K)
-    rot9 = np.tile(np.eye(3).reshape(1, 9), (nodes.shape[0], 1))
-    t3 = np.zeros((nodes.shape[0], 3)) + 0.001
-    t = time.perf_counter()
-    new_ends = O.blend(ends.reshape(-1, 3), eidx[:, :K], ew, nodes, rot9, t3).reshape(-1, 6, 3)
-    O.six_point(new_ends, scale, None, pos, rot, scale, sh)
-    t_def = time.perf_counter() - t
-    frame_s = sc.gn_iters * t_gn1 + t_def * g.n / len(sub)
-    return dict(value=1.0 / frame_s, unit="frames/s", cores=threads, kind="port",
-                sample=(f"oracle (C++/OpenMP, FP64 envelope-Cholesky GN) on the same scene: 1 of {sc.gn_iters} GN "
-                        f"iterations at {prob.n} unknowns ({t_gn1:.1f}s, x{sc.gn_iters}) + end-point blend and "
-                        f"six-point update of {len(sub)} of {g.n} Gaussians ({t_def:.2f}s, scaled); no render "
-                        f"(the reference renders on the GPU); sample update not included"))
+    from arapgs_amd import arapgs, scenes, workload
+    from arapgs_amd.pipeline import DragPipeline
+    sc = scenes.flat_scene(args.seed, n=args.large_n, n_nodes=args.large_nodes)
+    ctx = arapgs.Context(0, stream=torch.cuda.Stream())
+    t0 = time.perf_counter()
+    with torch.cuda.stream(ctx.stream):
+        W = workload.prepare(ctx, sc, log=log)
+        solver = workload.make_solver(ctx, W["solver_args"], n_aggregates=192)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    st, K = W["state"], W["K"]
+    e = sc.edit
+    frames_per_edit = 30
+    delta = np.float32(e["distance"] / frames_per_edit) * e["direction"].astype(np.float32)
+    radian = math.radians(e["angle_deg"]) / frames_per_edit
+    kscale = 1.0 + (e["scale"] - 1.0) / frames_per_edit
+    active = W["active"]
+    with torch.cuda.stream(ctx.stream):
+        tmp = [torch.empty_like(st["node_pos"]) for _ in range(2)]
+
+    def aims_fn(c, node_pos, aims):
+        # UpdateAimPosition, then UpdateAimPositionTwist, then UpdateAimPositionScale
+        c.aims_translate(node_pos, active, delta, tmp[0])
+        c.aims_twist(tmp[0], active, e["axis"], radian, tmp[1])
+        c.aims_scale(tmp[1], active, kscale, aims)
+
+    images = [torch.empty((3, sc.cameras[0].H, sc.cameras[0].W), dtype=torch.float32, device="cuda:0")
+              for _ in W["cams"]]
+    solver.set_coarse_refresh(8)
+    solver.set_host_checks(4)
+    solver.set_warm_start(1)
+    s0 = workload.clone_state(st)
+    # a few sequential frames with the kernels timed by HIP events
+    fb = arapgs.frame_buffers(st, K)
+    aims = torch.empty_like(st["node_pos"])
+    ctx.profile_enable(True)
+    with torch.cuda.stream(ctx.stream):
+        for _ in range(3):
+            aims_fn(ctx, st["node_pos"], aims)
+            solver.deform_frame(fb, aims, sc.gn_iters, stop_rule=True)
+    torch.cuda.synchronize()
+    kms = {name: ctx.profile_read(which=which) for name, _, which in KERNELS}
+    ctx.profile_enable(False)
+    S = int(st["samples"].shape[0])
+    N = int(st["pos"].shape[0])
+    per = {"six_point": (SIX_POINT_BYTES_PER_GAUSSIAN, N), "sample_blend": (SAMPLE_BLEND_BYTES_PER_SAMPLE, S),
+           "sample_sh": (SAMPLE_SH_BYTES_PER_SAMPLE, S)}
+    kern = {}
+    for name, (b, n) in per.items():
+        v = kms[name]
+        avg = float(sum(v) / len(v)) if len(v) else float("nan")
+        kern[name] = {"avg_ms": round(avg, 3), "gbs": round(b * n / (avg * 1e-3) / 1e9, 1) if len(v) else None}
+    # pipelined drag
+    for k, v in s0.items():
+        if v is not None:
+            st[k].copy_(v)
+    torch.cuda.synchronize()
+    pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
+                        aims_fn=aims_fn)
+    pipe.run(args.warmup)
+    torch.cuda.synchronize()
+    stamps = []
+    t0 = time.perf_counter()
+    pipe.run(args.large_frames, on_frame=lambda t, n: stamps.append(time.perf_counter()))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = [(b - a) * 1e3 for a, b in zip([t0] + stamps[:-1], stamps)]
+    inst = pipe.instances
+    pipe.close()
+    gn, cg = solver.last_stats()
+    frame_bytes = sum(b * n for b, n in per.values())
+    # release: density sample over the 128^3 lists and realignment steps (high_quality)
+    grid = W["grid"]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    with torch.cuda.stream(ctx.stream):
+        ev[0].record(ctx.stream)
+        feat, dens = grid.density(st["pos"], st["rot"], st["scale"], W["opac"], st["sh"], W["lpf"])
+        ev[1].record(ctx.stream)
+    torch.cuda.synchronize()
+    dens_ms = ev[0].elapsed_time(ev[1])
+    with torch.cuda.stream(ctx.stream):
+        pos, rot, shv = st["pos"].clone(), st["rot"].clone(), st["sh"].clone()
+        log_scale = torch.log(st["scale"])
+        o = W["opac"].clamp(1e-6, 1 - 1e-6)
+        logit = torch.log(o / (1 - o))
+        _, aim_dens = grid.density(s0["pos"], s0["rot"], s0["scale"], W["opac"], s0["sh"], W["lpf"])
+        r = arapgs.Realign(ctx, grid, N)
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    rms, losses = [], []
+    for i in range(args.large_realign_steps):
+        with torch.cuda.stream(ctx.stream):
+            ev[0].record(ctx.stream)
+            r.step(st["samples"], st["sample_feat"], aim_dens, W["lpf"], pos, rot, log_scale, logit, shv, loss=loss)
+            ev[1].record(ctx.stream)
+        torch.cuda.synchronize()
+        rms.append(ev[0].elapsed_time(ev[1]))
+        losses.append(float(loss.item()))
+    r.close()
+    n_pairs, n_valid = int(grid.n_pairs), int(grid.n_valid)
+    del feat, dens
+    solver.close()
+    grid.close()
+    ctx.close()
+    fps = args.large_frames / total
+    out = {"workload": "flat-3M (128^3 grid, 8 views at 1920x1080)", "gaussians": N, "nodes": args.large_nodes,
+           "samples": S, "cell_gaussian_pairs": n_pairs, "valid_cells": n_valid, "views": len(images),
+           "resolution": [sc.cameras[0].W, sc.cameras[0].H], "gn_iters_max": sc.gn_iters,
+           "form": "pipelined", "frames": args.large_frames, "fps": round(fps, 3),
+           "p50_ms": round(pct(ms, 50), 3), "p99_ms": round(pct(ms, 99), 3), "last_frame_gn_cg": [gn, cg],
+           "tile_instances_per_frame": int(inst), "setup_s": round(setup_s, 2),
+           "deform_kernels": kern, "deform_bytes_per_frame": frame_bytes,
+           "deform_hbm_gbs_at_frame_rate": round(frame_bytes * fps / 1e9, 1),
+           "release": {"density_sample_ms": round(dens_ms, 2),
+                       "realign_step_p50_ms": round(pct(rms, 50), 2) if rms else None,
+                       "l1_first_last": [losses[0], losses[-1]] if losses else None}}
+    log(f"  large: {out['fps']} frames/s, p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms; density "
+        f"{out['release']['density_sample_ms']} ms, realign {out['release']['realign_step_p50_ms']} ms/step")
+    return out
 
 
 def adjoint_leg(args):
@@ -494,21 +850,30 @@ def main():
     ap.add_argument("--height", type=int, default=800)
     ap.add_argument("--drag-frames", type=int, default=30)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=20_000)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg (and the energy check)")
+    ap.add_argument("--energy-check-frames", type=int, default=-1,
+                    help="timed frames whose energy the CPU leg checks against the oracle (-1: all)")
+    ap.add_argument("--fixed-steps", type=int, default=10,
+                    help="timed frames per form of the fixed-iteration leg (0: skip)")
+    ap.add_argument("--large-frames", type=int, default=20, help="timed frames of the large-scene leg (0: skip)")
+    ap.add_argument("--large-n", type=int, default=3_000_000)
+    ap.add_argument("--large-nodes", type=int, default=4096)
+    ap.add_argument("--large-realign-steps", type=int, default=3)
     ap.add_argument("--forms", default="plain,fast,pipelined")
     ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the differentiation leg (0: skip)")
     ap.add_argument("--aggregates", type=int, default=0, help="coarse aggregates of the CG preconditioner (0: 128)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the autotuner (tile, blend unroll, aggregates, coarse refresh, CG host checks)")
     ap.add_argument("--tune-steps", type=int, default=6, help="timed frames per autotuner candidate")
+    ap.add_argument("--sample-batch", type=int, default=0,
+                    help="drag frames per sample update pass in the pipelined form (0: autotuned, or 4)")
     ap.add_argument("--partition", action="store_true",
                     help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
     ap.add_argument("--realign-steps", type=int, default=5, help="steps of the realignment leg (0: skip)")
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--batch-scenes", type=int, default=32, help="scenes of the uneven-batch leg (0: skip)")
-    ap.add_argument("--batch-frames", type=int, default=8, help="timed frames per form of the uneven-batch leg")
+    ap.add_argument("--batch-frames", type=int, default=64, help="timed frames per form of the uneven-batch leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -542,6 +907,16 @@ def main():
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")
         if form == "pipelined":
+            if args.sample_batch > 0:
+                B["cfg"]["sample_batch"] = args.sample_batch
+            elif args.autotune:
+                B["cfg"]["sample_batch"], sb_trials = sample_batch_leg(B, args)
+                log(f"  sample batch: {B['cfg']['sample_batch']} frames per sample pass")
+                if tuning is not None:
+                    tuning["sample_batch_ms_per_frame"] = sb_trials
+                    tuning["config"] = dict(B["cfg"])
+            else:
+                B["cfg"]["sample_batch"] = 4
             if args.partition and "solve_cus" not in B["cfg"]:
                 B["cfg"]["solve_cus"] = partition_leg(B, args)
                 log(f"  partition: solve stream on {B['cfg']['solve_cus']} CUs")
@@ -557,12 +932,10 @@ def main():
     best = max(res, key=lambda f: args.steps / res[f]["total_s"])
     r = res[best]
     fps = args.steps / r["total_s"]
-    # the six-point kernel is timed (HIP events on its stream) in the sequential forms
+    # the deformation kernels are timed (HIP events on their stream) in the sequential forms
     kr = res.get("fast") or res.get("plain") or r
-    kms = kr["kernel_ms"]
-    avg_k = float(sum(kms) / len(kms)) if len(kms) else float("nan")
-    bytes_per_launch = SIX_POINT_BYTES_PER_GAUSSIAN * args.n
-    achieved = bytes_per_launch / (avg_k * 1e-3) / 1e9 if len(kms) else None
+    roofs = rooflines(args, B, kr["kernel_ms"])
+    dominant = max(roofs, key=lambda k: roofs[k]["avg_launch_ms"] if roofs[k]["launches"] else -1.0)
     line = {
         "metric": "deform+render frames/s @1M Gaussians, adjoint+batched vs plain; p50/p99 ms",
         "value": round(fps, 3),
@@ -577,8 +950,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "capsules-1M", "gaussians": args.n, "nodes": args.nodes, "knn_k": 8,
-                   "grid": 64, "gn_iters": 10, "views": args.views, "resolution": [args.width, args.height],
-                   "sh_degree": 3, "form": best},
+                   "grid": 64, "gn_iters": 10, "stop_rule": True, "views": args.views,
+                   "resolution": [args.width, args.height], "sh_degree": 3, "form": best,
+                   "drag": f"0.3 over {args.drag_frames} frames"},
         "p50_ms": round(pct(r["frame_ms"], 50), 3),
         "p99_ms": round(pct(r["frame_ms"], 99), 3),
     }
@@ -590,31 +964,43 @@ def main():
         line["plain_p50_ms"] = round(pct(rp["frame_ms"], 50), 3)
         line["plain_p99_ms"] = round(pct(rp["frame_ms"], 99), 3)
         line["speedup_vs_plain"] = round(fps / line["plain_fps"], 3)
-    line["roofline"] = {"bound": "hbm", "kernel": "six_point_kernel<8, true> (fused end-point blend + polar + SH)",
-                        "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                        "traffic": committed_traffic(args),
-                        "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_k, 4),
-                        "launches": int(len(kms))}
+    line["roofline"] = dict(roofs[dominant], dominant=True)
+    line["rooflines"] = roofs
     import gc
     gc.enable()
     line["config"]["tuning"] = B["cfg"]
     if tuning:
         line["autotune"] = tuning
+    rec = None
+    if "pipelined" in res and not args.no_cpu:
+        log("re-running the timed pipelined frames with their solve inputs recorded")
+        rec = record_frames(B, args.steps, args.warmup)
+        log(f"  recorded frames equal the timed frames: {rec['same_as_timed']}")
     if args.realign_steps > 0:
         log(f"timing the realignment leg: {args.realign_steps} steps")
         line["realign"] = realign_leg(B, args)
         log(f"  realign: {line['realign']['step_p50_ms']} ms/step, L1 {line['realign']['l1_loss_first_last']}")
+    if args.fixed_steps > 0:
+        log(f"timing the fixed-iteration leg: {args.warmup} warmup + {args.fixed_steps} frames per form")
+        line["fixed_iteration"] = fixed_iteration_leg(B, args)
+    if rec is not None:
+        line["cpu_baseline"] = cpu_baseline(args, B, rec)
+    B["ctx"].close()
+    del B
+    torch.cuda.empty_cache()
     if args.adjoint_steps > 0:
         log(f"timing the differentiation leg: {args.warmup} warmup + {args.adjoint_steps} steps")
         line["adjoint"] = adjoint_leg(args)
         log(f"  adjoint: {line['adjoint']['steps_per_s']} steps/s, forward p50 {line['adjoint']['forward_p50_ms']} ms, "
             f"backward p50 {line['adjoint']['backward_p50_ms']} ms")
+        torch.cuda.empty_cache()
     if args.batch_scenes > 0:
         log(f"timing the uneven-batch leg: {args.batch_scenes} scenes")
         line["batch"] = batch_leg(args)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args)
+        torch.cuda.empty_cache()
+    if args.large_frames > 0:
+        log(f"timing the large-scene leg: {args.warmup} warmup + {args.large_frames} frames")
+        line["large"] = large_leg(args)
     print(json.dumps(line), flush=True)
 
 

@@ -133,6 +133,20 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every);
  * `frames` solves (default 8); in between it is reused. It only affects the
  * CG iteration count, never the solution. */
 int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
+/* on = 1 (default): every rebuild after the first runs on a side stream of the
+ * solver from a snapshot of that solve's node positions, beside the solve, and
+ * the next solve adopts it (waiting for it if needed); the solve that starts
+ * it keeps the previous operator. on = 0: rebuilds run in line on the
+ * context's stream before the solve that needs them. Either way the sequence
+ * of operators is a function of the solve count only, so the results do not
+ * depend on the streams or the host. */
+int arapgs_solver_set_async_refresh(arapgs_solver* s, int on);
+/* The CG block size (no counterpart in the reference; set by the autotuner):
+ * workgroup size of the CG operator kernel, 64, 128 or 256 threads (16 lanes
+ * per free node, so 4, 8 or 16 nodes per workgroup; default 256). It changes
+ * the order of the p.q partial sums only: the solve stops at the same
+ * tolerance. Synchronises the device. Solvers of one batch need the same value. */
+int arapgs_solver_set_cg_block(arapgs_solver* s, int threads);
 
 /* on = 1: the CG of GN iteration k starts from the step that GN iteration k of
  * the previous solve on this solver found (zero before the first), instead of
@@ -141,6 +155,16 @@ int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
  * unchanged. Default 0 (every solve independent of the ones before). Every
  * call clears the stored steps (synchronous). */
 int arapgs_solver_set_warm_start(arapgs_solver* s, int on);
+
+/* Returns the solver's frame-to-frame state to that of a new solver: the
+ * warm-start steps are zero, the next solve builds the coarse operator in line
+ * (also with the asynchronous refresh on; a side-stream rebuild in flight is
+ * waited for and dropped) and the coarse-refresh count restarts. The reference
+ * builds a new Deform every drag frame (GaussianView.cpp:1481); this solver
+ * keeps that state across frames as an optimisation, and a caller resets it
+ * where a new drag starts or where two runs must compute the same frames.
+ * Synchronises the device. */
+int arapgs_solver_reset(arapgs_solver* s);
 
 /* GN iterations and total CG iterations of the solver's last solve
  * (synchronises the device). Diagnostics; no counterpart in the reference. */
@@ -182,6 +206,7 @@ int arapgs_solver_batch_create(arapgs_ctx* ctx, arapgs_solver* const* solvers_h,
 void arapgs_solver_batch_destroy(arapgs_solver_batch* b);
 int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every);
 int arapgs_solver_batch_set_coarse_refresh(arapgs_solver_batch* b, int frames);
+int arapgs_solver_batch_set_async_refresh(arapgs_solver_batch* b, int on);
 /* Host arrays of n device pointers, one per scene in batch order: node_pos and
  * aims f32[M_s*3] (required); x_out f32[12*n_free_s], energies f64[iters+1],
  * iters_done int[1] and tables (putFreeInputs node table, M_s rows) are
@@ -217,6 +242,18 @@ int arapgs_node_quats(arapgs_ctx* ctx, const arapgs_node_xf* table, int m, float
 int arapgs_sample_sh(arapgs_ctx* ctx, int64_t s, int k, const float* w, const uint32_t* idx, const float* node_q,
                      const uint8_t* is_static, float* feat);
 
+/* Batching form with no single counterpart in the reference: nf (<= 8)
+ * consecutive drag frames of the sample update (UpdatePositionforSamples, then
+ * FastUpdateSamplesSH, GaussianView.cpp:1500-1522) in one pass. For f = 0..nf-1
+ * in order the samples f32[s*3] are blended with tables_h[f] (like
+ * arapgs_blend_points) and the features f32[s*48] rotated with node_q_h[f]
+ * (like arapgs_sample_sh): bit-identical to those 2*nf calls. tables_h /
+ * node_q_h are host arrays of nf device pointers; k must be 8 and idx, w, feat
+ * 16-byte aligned. */
+int arapgs_samples_frames(arapgs_ctx* ctx, int64_t s, int k, int nf, const arapgs_node_xf* const* tables_h,
+                          const float* const* node_q_h, const uint32_t* idx, const float* w, const uint8_t* is_static,
+                          float* samples, float* feat);
+
 /* ------------------------------------------------------------ grid + density */
 typedef struct arapgs_grid arapgs_grid;
 
@@ -244,6 +281,13 @@ int arapgs_gaussian_aabbs(arapgs_ctx* ctx, const float* pos, const float* rot, c
 int arapgs_grid_build(arapgs_ctx* ctx, const float* aabbs, int64_t n, const float* aabb7_h, int grid_num,
                       int padding, arapgs_grid** out);
 void arapgs_grid_destroy(arapgs_grid* grid);
+/* UpdateContainingRelationship (GaussianView.cpp:3634-3744): new cell lists
+ * (prefix and list) from the Gaussians' current boxes `aabbs` (n must equal
+ * the grid's Gaussian count) on the geometry aabb7_h (arapgs_scene_aabb of the
+ * current positions); the grid's resolution and padding, its valid cells and
+ * their samples are kept. Synchronous; views from arapgs_grid_info before the
+ * call are invalid after it. */
+int arapgs_grid_rebin(arapgs_ctx* ctx, arapgs_grid* grid, const float* aabbs, int64_t n, const float* aabb7_h);
 /* Test hook with no counterpart in the reference: the number of cell-Gaussian
  * pairs per chunk of later arapgs_grid_build calls on this context (0: the
  * default 2^28). The lists do not depend on it. */
@@ -282,9 +326,12 @@ int arapgs_density_backward(arapgs_ctx* ctx, const arapgs_grid* grid, const floa
  * update3d). The Gaussians' raw parameters — pos, rot (stored quaternion), log
  * scales, logit opacities, SH — are moved by Adam so that the density sample
  * at the (deformed) samples matches the aim features / densities in L1:
- * L = sum |feat - aim_feat| + w_density sum |dens - aim_dens|. The cell lists
- * of the grid stay fixed during the loop. The per-cell convergence flags,
- * ExpandOpRange and adaptive low-pass of the reference are not modelled. */
+ * L = sum |feat - aim_feat| + w_density sum |dens - aim_dens|. The step uses
+ * the grid's current cell lists (arapgs_grid_rebin replaces them: the
+ * reference re-bins every _containing_relation_interval steps,
+ * GaussianView.cpp:1718-1721). The reference's kernels are not vendored: the
+ * loss, the masks and the convergence rule below are this library's
+ * definitions. ExpandOpRange and the adaptive low-pass are not modelled. */
 typedef struct arapgs_realign_desc {
     float lr_pos, lr_rot, lr_scale, lr_opacity, lr_sh;  /* Adam step sizes (raw parameters) */
     float beta1, beta2, eps;                             /* 0 = 0.9, 0.999, 1e-15 */
@@ -299,6 +346,29 @@ void arapgs_realign_destroy(arapgs_realign* r);
 int arapgs_realign_step(arapgs_ctx* ctx, arapgs_realign* r, const float* samples, const float* aim_feat,
                         const float* aim_dens, float lpf, float* pos, float* rot, float* log_scale,
                         float* logit_opacity, float* sh, double* loss);
+/* Options of later steps (defaults: all NULL / 0):
+ *  moved      device u8[n]: only Gaussians with moved[g] != 0 are updated ("Only
+ *             Optimize Deformed Region", the moved_gaussians of CheckMovedGaussians,
+ *             GaussianView.cpp:2092-2109, passed to update3d at :4452-4478), and
+ *             valid cells whose lists hold no moved Gaussian are static (the
+ *             current_static_grids of :3850-3852): they pass no gradient;
+ *  max_scale  device f32[n*3]: after each step the activated scales are clamped
+ *             to it (the max_scale argument of update3d);
+ *  conv_tol   per-cell convergence (grid_is_converged of L1loss3d, :4191-4207):
+ *             the first step after a re-binning or new options records every
+ *             cell's L1 loss, and a cell whose loss later falls to conv_tol times
+ *             its record is converged and passes no gradient until the next
+ *             re-binning (0: off).
+ * The reported loss always covers every cell. */
+typedef struct arapgs_realign_options {
+    const uint8_t* moved;
+    const float* max_scale;
+    float conv_tol;
+} arapgs_realign_options;
+int arapgs_realign_set_options(arapgs_ctx* ctx, arapgs_realign* r, const arapgs_realign_options* opts_h);
+/* Device views of the per-valid-cell state after the last step: skip u8[n_valid]
+ * (static or converged) and the cell's L1 loss f32[n_valid]. */
+int arapgs_realign_cells(const arapgs_realign* r, int64_t* n_valid, const uint8_t** skip, const float** cell_loss);
 
 /* ------------------------------------------------------------ render */
 /* CudaRasterizer::Rasterizer::forward as called at GaussianView.cpp:1099-1128
@@ -313,8 +383,11 @@ int arapgs_realign_step(arapgs_ctx* ctx, arapgs_realign* r, const float* samples
  * int[2*P], optional): when given, tile rectangles use per-axis 3-sigma extents
  * (the "Fast culling" mode, GaussianView.cpp:1062) and the extents are written.
  * boxmin_h / boxmax_h (host float[3], optional): crop box on the means.
- * colors_precomp, cov3D_precomp must be NULL, prefiltered and to_ortho 0.
- * tile: 16 (the 3DGS tile) or 8. Returns the number of (tile, Gaussian)
+ * colors_precomp, cov3D_precomp must be NULL and prefiltered 0. to_ortho = 1
+ * renders through an orthographic camera (GaussianView's setOrthoCam(ortho_scale,
+ * ortho_scale), GaussianView.cpp:1018-1019, 1094-1095): projmatrix is that
+ * camera's world->clip matrix (w = 1) and the EWA Jacobian is the constant
+ * diag(W, H) / (2 ortho_scale); ortho_scale > 0. tile: 16 (the 3DGS tile) or 8. Returns the number of (tile, Gaussian)
  * instances in *num_rendered_h. Synchronises once (instance count). */
 typedef struct arapgs_render_desc {
     int P, D, M;
@@ -349,7 +422,8 @@ int arapgs_render(arapgs_ctx* ctx, const arapgs_render_desc* desc_h, int* num_re
 /* The same forward pass for n_views cameras of one Gaussian set in one go
  * (GaussianView renders its views one Rasterizer::forward call at a time):
  * descs_h[v] differ only in viewmatrix, projmatrix, cam_pos, tan_fovx/y,
- * out_color, surface, radii and rects; every other field must equal view 0's.
+ * to_ortho, ortho_scale, out_color, surface, radii and rects; every other field
+ * must equal view 0's.
  * Each Gaussian's SH is read once for all views, the (tile, depth) keys of all
  * views (combined tile id view * ntiles + tile) are sorted together and one
  * blend launch covers every tile of every view, so the deepest tiles of the
@@ -437,12 +511,15 @@ int arapgs_aims_set(arapgs_ctx* ctx, const float* node_pos, int m, const int* id
 int arapgs_static_rows(arapgs_ctx* ctx, const uint8_t* node_static, const uint32_t* idx, int64_t n, int len,
                        uint8_t* out);
 
-/* Measurement hook: when enabled, arapgs_deform_frame brackets its fused
- * six-point kernel with HIP events on the context stream; arapgs_profile_read
- * synchronises, returns the per-launch durations (ms) and resets. Returns the
- * count written (>= 0) or a negative error. */
+/* Measurement hook (no counterpart in the reference): when enabled,
+ * arapgs_deform_frame brackets its fused six-point kernel (which = 0), the
+ * sample blend (1) and the sample SH rotation (2) with HIP events on the
+ * context stream; arapgs_profile_read_kernel synchronises, returns that
+ * kernel's per-launch durations (ms) and resets its list; arapgs_profile_read
+ * is which = 0. Return the count written (>= 0) or a negative error. */
 int arapgs_profile_enable(arapgs_ctx* ctx, int on);
 int arapgs_profile_read(arapgs_ctx* ctx, float* ms_h, int cap);
+int arapgs_profile_read_kernel(arapgs_ctx* ctx, int which, float* ms_h, int cap);
 
 /* ------------------------------------------------------------ adjoint chain */
 /* The differentiable deform -> density-sample -> render chain from rest

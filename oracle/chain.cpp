@@ -23,29 +23,7 @@
 #include <algorithm>
 #include <vector>
 
-extern "C" {
-struct orc_raster_args {
-    int P, D, W, H, tile;
-    const float* bg;
-    const float* means;
-    const float* shs;
-    const float* opac;
-    const float* scales;
-    float scale_mod;
-    const float* rots;
-    const float* view;
-    const float* proj;
-    const float* campos;
-    float tanfovx, tanfovy;
-    int use_rects;
-    const float* boxmin;
-    const float* boxmax;
-};
-void orc_raster_preprocess(const orc_raster_args* a, float* depth, int* radius, float* xy, float* conic, float* rgb,
-                           int* rect, int* touched);
-long orc_raster_bin(const orc_raster_args* a, const float* depth, const int* rect, const int* touched,
-                    uint64_t* keys, uint32_t* vals, int* ranges);
-}
+#include "raster.hpp"
 
 namespace {
 
@@ -387,7 +365,7 @@ void render_t(const orc_chain_args* a, int v, const Deformed<T>& D, T* img) {
     const float* cp = a->campos + 3 * v;
     const float tfx = a->tanfov[2 * v], tfy = a->tanfov[2 * v + 1];
     orc_raster_args ra{(int)N, a->D, W, H, tile, a->bg, fpos.data(), fsh.data(), a->opac, fscale.data(), 1.0f,
-                       frot.data(), Vm, Pm, cp, tfx, tfy, 0, nullptr, nullptr};
+                       frot.data(), Vm, Pm, cp, tfx, tfy, 0, nullptr, nullptr, 0, 1.0f};
     std::vector<float> depth(N), xyf(2 * N), conf(4 * N), rgbf(3 * N);
     std::vector<int> radius(N), rect(4 * N), touched(N);
     orc_raster_preprocess(&ra, depth.data(), radius.data(), xyf.data(), conf.data(), rgbf.data(), rect.data(),

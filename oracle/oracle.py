@@ -50,7 +50,8 @@ class RasterArgs(C.Structure):
     _fields_ = [("P", C.c_int), ("D", C.c_int), ("W", C.c_int), ("H", C.c_int), ("tile", C.c_int),
                 ("bg", P), ("means", P), ("shs", P), ("opac", P), ("scales", P), ("scale_mod", C.c_float),
                 ("rots", P), ("view", P), ("proj", P), ("campos", P), ("tanfovx", C.c_float),
-                ("tanfovy", C.c_float), ("use_rects", C.c_int), ("boxmin", P), ("boxmax", P)]
+                ("tanfovy", C.c_float), ("use_rects", C.c_int), ("boxmin", P), ("boxmax", P), ("ortho", C.c_int),
+                ("ortho_scale", C.c_float)]
 
 
 def _declare(L):
@@ -324,7 +325,8 @@ def density_sample(valid, prefix, lst, samples, pos, rot, scale, opacity, sh, lp
 
 # --------------------------------------------------------------------------- rasteriser
 def raster(means, shs, opac, scales, rots, view, proj, campos, tanfovx, tanfovy, W, H, bg, D=3, tile=16,
-           scale_mod=1.0, use_rects=True, boxmin=None, boxmax=None, want_surface=False):
+           scale_mod=1.0, use_rects=True, boxmin=None, boxmax=None, want_surface=False, ortho_scale=None):
+    """ortho_scale: an orthographic camera (proj = orthographic(s, s) @ view) when given."""
     """Returns dict with image (3,H,W), keys, vals, ranges and preprocess outputs."""
     means, shs, opac, scales, rots = _f32(means), _f32(shs), _f32(opac), _f32(scales), _f32(rots)
     view, proj, campos, bg = _f32(view).reshape(16), _f32(proj).reshape(16), _f32(campos), _f32(bg)
@@ -332,7 +334,8 @@ def raster(means, shs, opac, scales, rots, view, proj, campos, tanfovx, tanfovy,
     bmax = None if boxmax is None else _f32(boxmax)
     Pn = means.shape[0]
     a = RasterArgs(Pn, D, W, H, tile, _p(bg), _p(means), _p(shs), _p(opac), _p(scales), scale_mod, _p(rots),
-                   _p(view), _p(proj), _p(campos), tanfovx, tanfovy, int(use_rects), _p(bmin), _p(bmax))
+                   _p(view), _p(proj), _p(campos), tanfovx, tanfovy, int(use_rects), _p(bmin), _p(bmax),
+                   int(ortho_scale is not None), float(ortho_scale or 1.0))
     depth = np.zeros(Pn, np.float32); radius = np.zeros(Pn, np.int32); xy = np.zeros((Pn, 2), np.float32)
     conic = np.zeros((Pn, 4), np.float32); rgb = np.zeros((Pn, 3), np.float32)
     rect = np.zeros((Pn, 4), np.int32); touched = np.zeros(Pn, np.int32)

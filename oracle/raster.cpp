@@ -10,6 +10,7 @@
 // evaluated with single float operations in a fixed order (no contraction), so
 // the HIP preprocess can match it bit for bit.
 #include "common.hpp"
+#include "raster.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -27,23 +28,6 @@ const float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.45704579946
 
 extern "C" {
 
-struct orc_raster_args {
-    int P, D, W, H, tile;           // tile edge in pixels
-    const float* bg;                // 3
-    const float* means;             // P*3
-    const float* shs;               // P*48 (interleaved coef*3+ch)
-    const float* opac;              // P
-    const float* scales;            // P*3
-    float scale_mod;
-    const float* rots;              // P*4 (w,x,y,z)
-    const float* view;              // 16, column-major world->camera
-    const float* proj;              // 16, column-major world->clip
-    const float* campos;            // 3
-    float tanfovx, tanfovy;
-    int use_rects;                  // per-axis 3-sigma rectangles (fast culling)
-    const float* boxmin;            // crop box or NULL
-    const float* boxmax;
-};
 
 // Per-Gaussian preprocess. Outputs (P each): depth, radius, xy (2), conic+opacity
 // (4), rgb (3), rect (4: minx,miny,maxx,maxy), tiles touched.
@@ -52,7 +36,9 @@ void orc_raster_preprocess(const orc_raster_args* a, float* depth, int* radius, 
     const int gx = (a->W + a->tile - 1) / a->tile, gy = (a->H + a->tile - 1) / a->tile;
     const float* V = a->view;
     const float* Pm = a->proj;
-    const float fx = (float)a->W / (2.0f * a->tanfovx), fy = (float)a->H / (2.0f * a->tanfovy);
+    // perspective: focal lengths in pixels; orthographic: pixels per view unit
+    const float fx = a->ortho ? (float)a->W / (2.0f * a->ortho_scale) : (float)a->W / (2.0f * a->tanfovx);
+    const float fy = a->ortho ? (float)a->H / (2.0f * a->ortho_scale) : (float)a->H / (2.0f * a->tanfovy);
 #pragma omp parallel for schedule(static)
     for (int g = 0; g < a->P; g++) {
         radius[g] = 0;
@@ -91,13 +77,21 @@ void orc_raster_preprocess(const orc_raster_args* a, float* depth, int* radius, 
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++)
                 S[i][j] = R[i][0] * s2[0] * R[j][0] + R[i][1] * s2[1] * R[j][1] + R[i][2] * s2[2] * R[j][2];
-        // EWA: J W Sigma W^T J^T
-        const float limx = 1.3f * a->tanfovx, limy = 1.3f * a->tanfovy;
-        const float txtz = vx / vz, tytz = vy / vz;
-        const float tx = std::min(limx, std::max(-limx, txtz)) * vz;
-        const float ty = std::min(limy, std::max(-limy, tytz)) * vz;
-        const float j00 = fx / vz, j02 = -(fx * tx) / (vz * vz);
-        const float j11 = fy / vz, j12 = -(fy * ty) / (vz * vz);
+        // EWA: J W Sigma W^T J^T; J is the projection's Jacobian at the mean:
+        // perspective (x/z, y/z with the 1.3 tan(fov) clamp) or orthographic
+        // (constant scale, no dependence on depth)
+        float j00, j02, j11, j12;
+        if (a->ortho) {
+            j00 = fx; j02 = 0.0f;
+            j11 = fy; j12 = 0.0f;
+        } else {
+            const float limx = 1.3f * a->tanfovx, limy = 1.3f * a->tanfovy;
+            const float txtz = vx / vz, tytz = vy / vz;
+            const float tx = std::min(limx, std::max(-limx, txtz)) * vz;
+            const float ty = std::min(limy, std::max(-limy, tytz)) * vz;
+            j00 = fx / vz; j02 = -(fx * tx) / (vz * vz);
+            j11 = fy / vz; j12 = -(fy * ty) / (vz * vz);
+        }
         float T[2][3];
         for (int c = 0; c < 3; c++) {
             // W(r,c) = V[c*4 + r]

@@ -162,3 +162,33 @@ def test_blend_per_pixel_brute_force(small):
                 T = tt
             img[:, py, px] = C + T * bg
     np.testing.assert_allclose(r["image"], img, atol=1e-5)
+
+
+def test_orthographic_projection_closed_form():
+    """The oracle's orthographic mode (setOrthoCam(s, s), GaussianView.cpp:1018-1019):
+    pixel = ((x_view / s + 1) W - 1) / 2 for every Gaussian whatever its depth,
+    and an isotropic Gaussian of std sigma has the 2D covariance
+    (sigma W / (2 s))^2 + 0.3 on both axes (EWA with the constant Jacobian)."""
+    from arapgs_amd import scenes as S
+    rng = np.random.default_rng(4)
+    W, H = 200, 120
+    s = 1.7
+    cam = S.ortho_camera((0.0, -10.0, 0.0), (0.0, 0.0, 0.0), s, W, H)
+    n = 50
+    pos = np.stack([rng.uniform(-1, 1, n), rng.uniform(-6, 6, n), rng.uniform(-0.6, 0.6, n)], 1).astype(np.float32)
+    sig = rng.uniform(0.01, 0.05, n).astype(np.float32)
+    scales = np.repeat(sig[:, None], 3, 1)
+    rot = np.tile(np.array([1.0, 0.0, 0.0, 0.0], np.float32), (n, 1))
+    r = O.raster(pos, np.zeros((n, 48), np.float32), np.full(n, 0.8, np.float32), scales, rot, cam.view, cam.proj,
+                 cam.campos, cam.tanfovx, cam.tanfovy, W, H, np.zeros(3, np.float32), ortho_scale=s)
+    V = cam.view.reshape(4, 4).T.astype(np.float64)
+    pv = (V[:3, :3] @ pos.T.astype(np.float64)).T + V[:3, 3]
+    ix = ((pv[:, 0] / s + 1.0) * W - 1.0) * 0.5
+    iy = ((pv[:, 1] / s + 1.0) * H - 1.0) * 0.5
+    np.testing.assert_allclose(r["xy"][:, 0], ix, atol=1e-3)
+    np.testing.assert_allclose(r["xy"][:, 1], iy, atol=1e-3)
+    vx = (sig.astype(np.float64) * W / (2 * s)) ** 2 + 0.3
+    vy = (sig.astype(np.float64) * H / (2 * s)) ** 2 + 0.3
+    np.testing.assert_allclose(r["conic"][:, 0], 1.0 / vx, rtol=1e-4)
+    np.testing.assert_allclose(r["conic"][:, 2], 1.0 / vy, rtol=1e-4)
+    np.testing.assert_allclose(r["depth"], pv[:, 2], rtol=1e-5)

@@ -135,3 +135,82 @@ def test_chain_backward_reproducible(setup):
     b = _vjp(s, np.random.default_rng(3))[1]
     for x, y in zip(a, b):
         assert np.abs(x - y).max() <= 1e-5 * max(1.0, np.abs(x).max())
+
+
+def _chain_function(chain):
+    """The chain as a torch.autograd.Function: forward = arapgs_chain_forward,
+    backward = arapgs_chain_backward (the vector-Jacobian product), so a torch
+    loss over the images and density outputs back-propagates to the node table
+    and the rest rotations and scales."""
+    import torch
+
+    class ChainFn(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, table, rot0, scale0):
+            images, feat, dens = chain.forward(table.detach(), rot0.detach(), scale0.detach())
+            fctx.save_for_backward(table, rot0, scale0)
+            return torch.stack(images), feat, dens
+
+        @staticmethod
+        def backward(fctx, g_images, g_feat, g_dens):
+            table, rot0, scale0 = fctx.saved_tensors
+            dims = [g_images[v].contiguous() for v in range(g_images.shape[0])]
+            dt, dq, ds = chain.backward(table.detach(), rot0.detach(), scale0.detach(), dimages=dims,
+                                        dfeat=g_feat.contiguous(), ddens=g_dens.contiguous())
+            return dt, dq, ds
+
+    return ChainFn.apply
+
+
+def test_chain_autograd_function(setup):
+    """Through torch.autograd: the gradients of a torch loss equal the C-ABI
+    backward with that loss's cotangents (up to float atomics), agree with the
+    FP64 oracle's directional derivatives, and a few Adam steps on the node
+    translations lower an image-fitting loss."""
+    import torch
+    s = setup
+    ch, d = s["chain"], s["dev"]
+    fn = _chain_function(ch)
+    rng = np.random.default_rng(5)
+    gI = T(rng.normal(size=(ch.V, 3, ch.H, ch.W)).astype(np.float32))
+    gF = T(rng.normal(size=(ch.S, 48)).astype(np.float32))
+    gD = T(rng.normal(size=ch.S).astype(np.float32))
+    table = d["table"].clone().requires_grad_(True)
+    rot0 = d["rot0"].clone().requires_grad_(True)
+    scale0 = d["scale0"].clone().requires_grad_(True)
+    img, feat, dens = fn(table, rot0, scale0)
+    loss = (img * gI).sum() + (feat * gF).sum() + (dens * gD).sum()
+    loss.backward()
+    dt, dq, ds = ch.backward(d["table"], d["rot0"], d["scale0"], dimages=[gI[v] for v in range(ch.V)], dfeat=gF,
+                             ddens=gD)
+    torch.cuda.synchronize()
+    for got, ref in ((table.grad, dt), (rot0.grad, dq), (scale0.grad, ds)):
+        got, ref = H(got), H(ref)
+        assert np.abs(got - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    # one direction of every block against the FP64 forward-mode oracle
+    R, t, q0, s0 = s["params"]
+    v = [rng.normal(size=R.shape), rng.normal(size=t.shape), rng.normal(size=q0.shape), rng.normal(size=s0.shape) * s0]
+    out = s["ora"].jvp(R, t, q0, s0, *v)
+    ref = float(np.sum(H(gI) * out["d_images"]) + np.sum(H(gF) * out["d_feat"]) + np.sum(H(gD) * out["d_dens"]))
+    tg = H(table.grad).astype(np.float64)
+    got = float(np.sum(tg[:, 3:12] * v[0]) + np.sum(tg[:, 12:15] * v[1]) +
+                np.sum(H(rot0.grad).astype(np.float64) * v[2]) + np.sum(H(scale0.grad).astype(np.float64) * v[3]))
+    assert abs(got - ref) <= 1e-3 * abs(ref), (got, ref)
+    # fit the images of a translated table with Adam on the translations
+    target_tab = d["table"].clone()
+    target_tab[:, 12:15] += T(rng.normal(size=(ch.M, 3)).astype(np.float32) * 0.01)
+    with torch.no_grad():
+        target = fn(target_tab, d["rot0"], d["scale0"])[0].clone()
+    trans = d["table"][:, 12:15].clone().requires_grad_(True)
+    opt = torch.optim.Adam([trans], lr=2e-3)
+    losses = []
+    for _ in range(8):
+        opt.zero_grad()
+        tab = torch.cat([d["table"][:, :12], trans, d["table"][:, 15:]], dim=1)
+        img = fn(tab, d["rot0"], d["scale0"])[0]
+        lo = ((img - target) ** 2).sum()
+        lo.backward()
+        opt.step()
+        losses.append(float(lo))
+    print(f"autograd fit: {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert losses[-1] < 0.8 * losses[0], losses

@@ -214,8 +214,13 @@ def _grid_checks(ctx, sc, wl, rng):
         rf, ro = O.density_sample(np.arange(len(pick), dtype=np.int32), sub_prefix, sub_list, sub_samples, pos, rot,
                                   scale, opac, sh, lpf)
         f_g, o_g = H(feat)[rows], H(dens)[rows]
+        # FP32 against FP64: the opacities are sums of positive terms (1e-4); the
+        # features sum thousands of signed terms per sample, and a flat Gaussian's
+        # FP32 inverse covariance carries ~1e-5 relative error that enters the
+        # exponent multiplied by the quadratic form (up to ~10 for terms that
+        # matter), so they are held to 1e-3 of the largest feature value
         assert np.abs(o_g - ro).max() <= 1e-4 * max(1.0, np.abs(ro).max())
-        assert np.abs(f_g - rf).max() <= 1e-4 * max(1.0, np.abs(rf).max())
+        assert np.abs(f_g - rf).max() <= 1e-3 * max(1.0, np.abs(rf).max())
         print("large: one realignment step at 2.52e9 pairs", flush=True)
         noise = torch.from_numpy(rng.normal(size=pos.shape).astype(np.float32)).cuda()
         p_pos = (st["pos"] + 0.002 * noise).contiguous()

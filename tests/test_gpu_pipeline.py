@@ -1,7 +1,11 @@
 """Pipelined drag frames (solve of frame t+1 overlapping the deformation and
 renders of frame t on a second stream) give exactly the sequential frames'
 state and images: GaussianView::onUpdate order (GaussianView.cpp:1481-1522)
-is preserved per kernel, only the streams differ."""
+is preserved per kernel, only the streams differ. With batched sample frames
+(three frames' sample updates per pass, and a final partial batch of two) the
+samples and their features are still bit-identical to per-frame updates.
+arapgs_samples_frames itself equals its per-frame calls bit for bit on random
+node tables, static samples included."""
 import numpy as np
 import pytest
 
@@ -13,7 +17,8 @@ pytestmark = pytest.mark.gpu
 FRAMES = 5
 
 
-def test_pipelined_frames_bit_identical(ctx):
+@pytest.mark.parametrize("sample_batch", [1, 3])
+def test_pipelined_frames_bit_identical(ctx, sample_batch):
     import torch
     sc = scenes.capsule_scene(7, n=40000, n_nodes=128, W=160, H=128, n_views=2)
     sc.grid = 32
@@ -44,7 +49,8 @@ def test_pipelined_frames_bit_identical(ctx):
     solver2.set_host_checks(16)
     imgs_pipe = [torch.empty((3, c["H"], c["W"]), device="cuda") for c in W["cams"]]
     torch.cuda.synchronize()
-    pipe = DragPipeline(solver2, st2, K, W["active"], delta, iters, W["cams"], imgs_pipe, W["bg"], W["opac"])
+    pipe = DragPipeline(solver2, st2, K, W["active"], delta, iters, W["cams"], imgs_pipe, W["bg"], W["opac"],
+                        sample_batch=sample_batch)
     pipe.run(FRAMES)
     torch.cuda.synchronize()
 
@@ -58,3 +64,33 @@ def test_pipelined_frames_bit_identical(ctx):
     pipe.close()
     del pipe, solver2
     ctx_a.close()
+
+
+def test_samples_frames_equal_per_frame_calls(ctx):
+    import torch
+    from helpers import random_node_xf, table_from
+    rng = np.random.default_rng(9)
+    M, S, K = 96, 20000, 8
+    nodes = rng.uniform(-1, 1, size=(M, 3)).astype(np.float32)
+    pts = rng.uniform(-1, 1, size=(S, 3)).astype(np.float32)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    idx_full, w = ctx.knn(T(nodes), T(pts), K)
+    idx = idx_full[:, :K].contiguous()
+    feat0 = T(rng.normal(size=(S, 48)).astype(np.float32))
+    static = T((rng.random(S) < 0.1).astype(np.uint8))
+    tables = []
+    for f in range(5):
+        rot9, t3 = random_node_xf(M, rng, rot_deg=15.0, trans=0.05)
+        tables.append(T(table_from(nodes + 0.01 * f, rot9, t3)))
+    quats = [ctx.node_quats(t) for t in tables]
+    p_ref, f_ref = T(pts), feat0.clone()
+    for t, q in zip(tables, quats):
+        ctx.blend_points(p_ref, idx, w, t, skip=static, out=p_ref)
+        ctx.sample_sh(w, idx, q, static, f_ref)
+    p_bat, f_bat = T(pts), feat0.clone()
+    ctx.samples_frames(tables[:3], quats[:3], idx, w, static, p_bat, f_bat)
+    ctx.samples_frames(tables[3:], quats[3:], idx, w, static, p_bat, f_bat)
+    torch.cuda.synchronize()
+    assert torch.equal(p_bat, p_ref)
+    assert torch.equal(f_bat, f_ref)
+    assert not torch.equal(f_bat, feat0)

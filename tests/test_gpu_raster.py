@@ -105,3 +105,52 @@ def test_blend_unroll_does_not_change_images(ctx, scene):
         imgs.append(H(out["image"]))
     ctx.set_blend_unroll(4)
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[1], imgs[2])
+
+
+@pytest.mark.parametrize("scale", [1.2, 3.0])
+def test_orthographic_render_matches_oracle(ctx, scene, scale):
+    """to_ortho / ortho_scale (GaussianView.cpp:1018-1019, 1094-1095): keys,
+    values, ranges and radii bit-exact and pixels within 1/255 of the oracle's
+    orthographic restatement (the reference's rasteriser is not vendored, so
+    this pins the projection, not the fork)."""
+    g = scene["g"]
+    W, H_ = 480, 320
+    cam = S.ortho_camera((0.4, -5.0, 1.0), (0.0, 0.0, 0.0), scale, W, H_)
+    bg = np.array([0.8, 0.8, 0.8], np.float32)
+    ref = O.raster(g.pos, g.sh, g.opacity, g.scale, g.rot, cam.view, cam.proj, cam.campos, cam.tanfovx,
+                   cam.tanfovy, W, H_, bg, ortho_scale=scale)
+    out = ctx.render(T(g.pos), T(g.sh), T(g.opacity), T(g.scale), T(g.rot), T(cam.view), T(cam.proj),
+                     T(cam.campos), cam.tanfovx, cam.tanfovy, W, H_, T(bg), want_radii=True, ortho_scale=scale)
+    n = out["num_rendered"]
+    assert n == ref["keys"].shape[0] and n > 0
+    keys, vals, ranges = ctx.render_debug(n, ((W + 15) // 16) * ((H_ + 15) // 16))
+    assert np.array_equal(keys, ref["keys"])
+    assert np.array_equal(vals, ref["vals"])
+    assert np.array_equal(ranges.astype(np.int64), ref["ranges"].astype(np.int64))
+    assert np.array_equal(H(out["radii"]), ref["radius"])
+    assert np.abs(H(out["image"]) - ref["image"]).max() <= 1.0 / 255.0
+
+
+def test_orthographic_footprint_does_not_depend_on_depth(ctx):
+    """Under an orthographic camera a Gaussian's screen footprint is its view-plane
+    extent times W / (2 ortho_scale), whatever its depth: two equal isotropic
+    Gaussians at depths 2 and 8 on the optical axis get the same radius and
+    conic, and the conic matches the closed form (s * W / (2 scale))^2 + 0.3."""
+    W = H_ = 256
+    scale = 2.0
+    cam = S.ortho_camera((0.0, -10.0, 0.0), (0.0, 0.0, 0.0), scale, W, H_)
+    pos = np.array([[0.0, -8.0, 0.0], [0.0, -2.0, 0.0]], np.float32)   # depths 2 and 8
+    s = 0.05
+    scales = np.full((2, 3), s, np.float32)
+    rot = np.tile(np.array([1.0, 0.0, 0.0, 0.0], np.float32), (2, 1))
+    sh = np.zeros((2, 48), np.float32)
+    opac = np.array([0.9, 0.9], np.float32)
+    ref = O.raster(pos, sh, opac, scales, rot, cam.view, cam.proj, cam.campos, cam.tanfovx, cam.tanfovy, W, H_,
+                   np.zeros(3, np.float32), ortho_scale=scale)
+    out = ctx.render(T(pos), T(sh), T(opac), T(scales), T(rot), T(cam.view), T(cam.proj), T(cam.campos),
+                     cam.tanfovx, cam.tanfovy, W, H_, T(np.zeros(3, np.float32)), want_radii=True, ortho_scale=scale)
+    r = H(out["radii"])
+    assert r[0] == r[1] == ref["radius"][0] == ref["radius"][1] > 0
+    var = (s * W / (2.0 * scale)) ** 2 + 0.3
+    np.testing.assert_allclose(ref["conic"][:, 0], 1.0 / var, rtol=1e-5)
+    np.testing.assert_allclose(ref["conic"][0, :3], ref["conic"][1, :3], rtol=0, atol=0)

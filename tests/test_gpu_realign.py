@@ -125,3 +125,120 @@ def test_realignment_lowers_the_l1_loss(ctx, dsc):
     print(f"realignment L1 loss: {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
     assert losses[-1] < 0.8 * losses[0], losses
     r.close()
+
+
+def test_rebin_equals_a_fresh_build(ctx, dsc):
+    """arapgs_grid_rebin (UpdateContainingRelationship, GaussianView.cpp:3634-3744)
+    after the Gaussians moved: prefix and lists equal a fresh grid build (and the
+    oracle's lists) on the moved Gaussians' boxes and scene box, bit for bit;
+    the valid cells and samples stay those of the first build."""
+    g = dsc["g"]
+    rng = np.random.default_rng(8)
+    a7 = ctx.scene_aabb(T(g.pos), 12)
+    grid = ctx.grid_build(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)), a7, 12, padding=1)
+    p0, _, v0, s0 = grid.arrays()
+    moved = (g.pos + 0.05 * rng.normal(size=g.pos.shape)).astype(np.float32)
+    a7n = grid.rebin(T(moved), T(g.rot), T(g.scale), T(g.opacity))
+    prefix, lst, valid, samples = grid.arrays()
+    fresh = ctx.grid_build(ctx.gaussian_aabbs(T(moved), T(g.rot), T(g.scale), T(g.opacity)), a7n, 12, padding=1)
+    fp, fl, _, _ = fresh.arrays()
+    assert np.array_equal(prefix, fp) and np.array_equal(lst, fl)
+    mn, mx, step = O.scene_aabb(moved, 12)
+    rp, rl, _, _ = O.grid_lists(O.gaussian_aabbs(moved, g.rot, g.scale, g.opacity), mn, step, 12, padding=1)
+    assert np.array_equal(prefix, rp) and np.array_equal(lst, rl)
+    assert not np.array_equal(prefix, p0)
+    assert np.array_equal(valid, v0) and np.array_equal(samples, s0)
+    fresh.close()
+    grid.close()
+
+
+def test_moved_mask_static_cells_and_scale_clamp(ctx, dsc):
+    """Only Gaussians marked moved change; valid cells whose lists hold no moved
+    Gaussian are static (skip flags = a CPU scan of the lists); the activated
+    scales never exceed max_scale."""
+    import torch
+    g, grid = dsc["g"], dsc["grid"]
+    rng = np.random.default_rng(6)
+    samples = T(dsc["samples"])
+    feat_aim, dens_aim = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    pert = g.take(np.arange(g.pos.shape[0]))
+    pert.pos = (g.pos + 0.01 * rng.normal(size=g.pos.shape)).astype(np.float32)
+    pos, rot, ls, lo, sh = _params(pert)
+    moved = (rng.random(g.pos.shape[0]) < 0.3).astype(np.uint8)
+    max_scale = (pert.scale * 1.001).astype(np.float32)
+    r = arapgs.Realign(ctx, grid, g.pos.shape[0], lr_pos=2e-4, lr_rot=1e-3, lr_scale=5e-3, lr_opacity=1e-2,
+                       lr_sh=1e-3)
+    r.set_options(moved=T(moved), max_scale=T(max_scale))
+    before = [H(x).copy() for x in (pos, rot, ls, lo, sh)]
+    for _ in range(3):
+        r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+    torch.cuda.synchronize()
+    after = [H(x) for x in (pos, rot, ls, lo, sh)]
+    still = moved == 0
+    for b, a in zip(before, after):
+        assert np.array_equal(a[still], b[still])
+    assert any(not np.array_equal(a[~still], b[~still]) for b, a in zip(before, after))
+    assert (np.exp(after[2]) <= max_scale * (1 + 1e-6)).all()
+    skip, _ = r.cells()
+    prefix, lst, valid = dsc["prefix"], dsc["list"], dsc["valid"]
+    want = np.zeros(len(valid), np.uint8)
+    for v, c in enumerate(valid):
+        beg = int(prefix[c - 1]) if c > 0 else 0
+        want[v] = 0 if moved[lst[beg:int(prefix[c])]].any() else 1
+    assert np.array_equal(skip, want) and want.any() and not want.all()
+    r.close()
+
+
+def test_per_cell_convergence(ctx, dsc):
+    """conv_tol: the first step records each cell's loss; at the second step a
+    cell whose loss is at most conv_tol times its record is converged (skip) and
+    stays so. With conv_tol = 10 every cell with a nonzero loss converges; with
+    conv_tol = 1e-9 none does. The per-cell losses add up to the reported loss."""
+    import torch
+    g, grid = dsc["g"], dsc["grid"]
+    rng = np.random.default_rng(12)
+    samples = T(dsc["samples"])
+    feat_aim, dens_aim = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    pert = g.take(np.arange(g.pos.shape[0]))
+    pert.pos = (g.pos + 0.01 * rng.normal(size=g.pos.shape)).astype(np.float32)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for tol, expect_all in ((10.0, True), (1e-9, False)):
+        pos, rot, ls, lo, sh = _params(pert)
+        r = arapgs.Realign(ctx, grid, g.pos.shape[0])
+        r.set_options(conv_tol=tol)
+        r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh, loss=loss)
+        skip1, cl = r.cells()
+        assert not skip1.any()
+        np.testing.assert_allclose(cl.astype(np.float64).sum(), float(loss.item()), rtol=1e-4)
+        r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+        skip2, _ = r.cells()
+        if expect_all:
+            assert skip2[cl > 0].all()
+        else:
+            assert not skip2[cl > 0].any()
+        r.close()
+
+
+def test_realign_loop_with_rebinning(ctx, dsc):
+    """The F6 loop (GaussianView.cpp:1695-1733): re-binning before the loop and
+    every 5 steps, 15 steps; the L1 loss decreases."""
+    import torch
+    g = dsc["g"]
+    a7 = ctx.scene_aabb(T(g.pos), 12)
+    grid = ctx.grid_build(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)), a7, 12, padding=1)
+    samples = grid.samples_tensor()
+    feat_aim, dens_aim = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    rng = np.random.default_rng(13)
+    pert = g.take(np.arange(g.pos.shape[0]))
+    pert.pos = (g.pos + 0.01 * rng.normal(size=g.pos.shape)).astype(np.float32)
+    pos, rot, ls, lo, sh = _params(pert)
+    r = arapgs.Realign(ctx, grid, g.pos.shape[0], lr_pos=2e-4, lr_rot=1e-3, lr_scale=5e-3, lr_opacity=1e-2,
+                       lr_sh=1e-3)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    losses = []
+    arapgs.realign_loop(ctx, grid, r, samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh, 15,
+                        rebin_interval=5, loss=loss, on_step=lambda i, l: losses.append(float(l.item())))
+    print(f"F6 loop L1: {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
+    assert losses[-1] < 0.9 * losses[0]
+    r.close()
+    grid.close()

@@ -275,3 +275,25 @@ def test_coarse_inverse_is_exact_when_every_node_is_an_aggregate(ctx, prob):
     print(f"one GN step: {cg} CG iterations, E {float(H(e)[1]):.6e} vs oracle {e_ref[1]:.6e}", flush=True)
     assert gn == 1 and cg <= 20
     assert abs(float(H(e)[1]) - e_ref[1]) <= 1e-5 * e_ref[1]
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_cg_block_size_keeps_the_solution(ctx, prob, block):
+    """The CG block size (workgroup size of the CG operator kernel, an autotuner
+    knob) changes only the order of the p.q partial sums: the final energy stays
+    within 1e-5 of the oracle's and the GN count within one, with the asynchronous
+    coarse refresh on and off."""
+    aims, ref = _problem(prob, 1.0)
+    x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+    for async_refresh in (True, False):
+        s = _solver(ctx, prob)
+        s.set_cg_block(block)
+        s.set_async_refresh(async_refresh)
+        s.set_coarse_refresh(1)
+        for _ in range(3):  # consecutive solves: in-line first operator, then side-stream rebuilds
+            x, e, it = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+            n = int(H(it)[0])
+            e = H(e)[: n + 1]
+            assert abs(e[-1] - e_ref[-1]) <= 1e-5 * e_ref[-1], (block, async_refresh, e[-1], e_ref[-1])
+            assert abs(n - it_ref) <= 1
+        s.close()
