@@ -77,6 +77,7 @@ EXPORTS = {
     "arapgs_version": ([], C.c_char_p),
     "arapgs_fps": ([P, P, C.c_int64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
     "arapgs_knn": ([P, P, C.c_int, P, C.c_int64, C.c_int, P, P], C.c_int),
+    "arapgs_knn_noalloc": ([P, P, C.c_int, P, C.c_int64, C.c_int, P, P], C.c_int),
     "arapgs_graph_edges": ([P, P, C.c_int, C.c_int, P], C.c_int),
     "arapgs_end_points": ([P, P, P, P, C.c_int64, C.c_float, C.c_float, P], C.c_int),
     "arapgs_solver_create": ([P, C.POINTER(SolverDesc), C.POINTER(P)], C.c_int),
@@ -261,6 +262,13 @@ class Context:
         idx = t.empty((Q, k + 1), dtype=t.int32, device=queries.device)
         w = t.empty((Q, k), dtype=t.float32, device=queries.device) if with_weights else None
         self._check(lib().arapgs_knn(self.h, ptr(nodes), nodes.shape[0], ptr(queries), Q, k, ptr(idx), ptr(w)))
+        return idx, w
+
+    def knn_noalloc(self, nodes, queries, k, idx, w=None):
+        """arapgs_knn_noalloc into caller-owned idx (Q, k+1) int32 and w (Q, k)
+        float32 (or None): the library allocates nothing."""
+        self._check(lib().arapgs_knn_noalloc(self.h, ptr(nodes), nodes.shape[0], ptr(queries), queries.shape[0], k,
+                                             ptr(idx), ptr(w)))
         return idx, w
 
     def graph_edges(self, nodes, k):

@@ -166,6 +166,10 @@ int arapgs_create(int device, arapgs_ctx** out) {
         return -5;
     }
     (void)hipMemset(c->fps_counter, 0, sizeof(unsigned int));
+    // the static KNN scratch's address on this device (arapgs_knn_noalloc);
+    // a failure here is reported by that entry point, which retries
+    uint32_t* knn_static = nullptr;
+    (void)knn_noalloc_scratch(device, &knn_static);
     *out = c;
     return 0;
 }
@@ -281,6 +285,24 @@ int arapgs_knn(arapgs_ctx* c, const float* nodes, int m, const float* queries, i
     if (rc) return rc;
     return check_launch(c, launch_knn(nodes, m, queries, q, k, out_idx, out_w, (uint32_t*)c->knn_list, c->knn_count,
                                       c->stream), "knn");
+}
+
+// Scratch-free variant of arapgs_knn for callers that cannot allocate.
+int arapgs_knn_noalloc(arapgs_ctx* c, const float* nodes, int m, const float* queries, int64_t q, int k,
+                       uint32_t* out_idx, float* out_w) {
+    CHECK_CTX(c);
+    if (!nodes || !queries || !out_idx || q < 0) return fail(c, -1, "knn_noalloc: bad arguments");
+    if (k < 1 || k > KNN_MAX_T) return fail(c, -1, "knn_noalloc: k must be in 1..12");
+    if (m < k + 1) return fail(c, -1, "knn_noalloc: need at least k+1 nodes (m=%d, k=%d)", m, k);
+    if ((size_t)m * 3 * sizeof(float) > 64 * 1024) return fail(c, -1, "knn_noalloc: m=%d nodes exceed the LDS stage", m);
+    if (q > KNN_NOALLOC_MAX_QUERIES)
+        return fail(c, -1, "knn_noalloc: %lld queries exceed the static scratch (at most %lld)", (long long)q,
+                    (long long)KNN_NOALLOC_MAX_QUERIES);
+    (void)hipSetDevice(c->device);
+    uint32_t* s = nullptr;
+    int rc = knn_noalloc_scratch(c->device, &s);
+    if (rc) return check_launch(c, rc, "knn_noalloc (static scratch address)");
+    return check_launch(c, launch_knn(nodes, m, queries, q, k, out_idx, out_w, s + 64, s, c->stream), "knn_noalloc");
 }
 
 }  // extern "C"

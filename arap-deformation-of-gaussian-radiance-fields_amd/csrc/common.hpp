@@ -18,6 +18,13 @@ namespace arapgs {
 // ----------------------------------------------------------------- knn.hip
 int launch_knn(const float* nodes, int M, const float* queries, int64_t Q, int k, uint32_t* out_idx, float* out_w,
                uint32_t* scratch_list, uint32_t* scratch_count, hipStream_t st);
+// Largest query count of arapgs_knn_noalloc (its static scratch: 32 MB per GPU)
+// and the devices it serves.
+constexpr int64_t KNN_NOALLOC_MAX_QUERIES = 1 << 23;
+constexpr int KNN_NOALLOC_MAX_DEVICES = 64;
+// Device address of this GPU's static KNN scratch (hipGetSymbolAddress on the
+// first call per device; `device` must be current). Allocates nothing.
+int knn_noalloc_scratch(int device, uint32_t** out);
 int launch_fps(const float* pos, int64_t N, int m_steps, int* out, float* dist, unsigned long long* partial,
                int n_partial, unsigned int* counter, unsigned long long* best, hipStream_t st);
 

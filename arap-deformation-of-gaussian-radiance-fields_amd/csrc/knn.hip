@@ -16,6 +16,8 @@
 #pragma clang fp contract(off)
 #include "common.hpp"
 
+#include <mutex>
+
 namespace arapgs {
 
 namespace {
@@ -349,6 +351,36 @@ int launch_fps(const float* pos, int64_t N, int m_steps, int* out, float* dist, 
     for (int s = 0; s < m_steps; s++)
         hipLaunchKernelGGL(fps_step_kernel, dim3(n_partial), dim3(256), 0, st, pos, N, dist, s, out, partial, counter);
     HIP_RET(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- no-allocation KNN
+// The scratch of arapgs_knn_noalloc: word 0 is the fallback count, words
+// 64 .. 64 + KNN_NOALLOC_MAX_QUERIES - 1 the fallback list. Every GPU that
+// loads the library holds its own copy until the process ends. In host code
+// `scratch` names a host-side stand-in, not device memory: host code uses only
+// the per-device address that hipGetSymbolAddress returned.
+static __device__ __align__(256) uint32_t scratch[64 + KNN_NOALLOC_MAX_QUERIES];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[KNN_NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+int knn_noalloc_scratch(int device, uint32_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= KNN_NOALLOC_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        HIP_RET(hipGetDevice(&cur));
+        if (cur != device) return -1;  // the caller makes the device current
+        void* p = nullptr;
+        HIP_RET(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return -1;
+        g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[device];
     return 0;
 }
 

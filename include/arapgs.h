@@ -73,6 +73,18 @@ int arapgs_fps(arapgs_ctx* ctx, const float* pos, int64_t n, int m, int* out_idx
 int arapgs_knn(arapgs_ctx* ctx, const float* nodes, int m, const float* queries, int64_t q, int k,
                uint32_t* out_idx, float* out_w);
 
+/* No counterpart in the reference: arapgs_knn for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and bit-identical results; it calls no hipMalloc,
+ * hipFree or allocating routine. Its scratch (the fallback count and list) is
+ * a static device array of 32 MB per GPU, held until the process ends, so q is
+ * at most 2^23 (8,388,608) queries; larger q is rejected with an error. Every
+ * call on one GPU shares that GPU's scratch (the count is cleared by a kernel
+ * at the start of each call): calls that overlap on one GPU, on different
+ * streams or threads, race. */
+int arapgs_knn_noalloc(arapgs_ctx* ctx, const float* nodes, int m, const float* queries, int64_t q, int k,
+                       uint32_t* out_idx, float* out_w);
+
 /* DeformGraph::setupEdges (Deform.hpp:84-95): out_nbr u32[M*k]. */
 int arapgs_graph_edges(arapgs_ctx* ctx, const float* nodes, int m, int k, uint32_t* out_nbr);
 
