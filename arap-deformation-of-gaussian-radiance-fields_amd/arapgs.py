@@ -111,6 +111,8 @@ EXPORTS = {
     "arapgs_grid_info": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P), C.POINTER(P),
                           C.POINTER(P)], C.c_int),
     "arapgs_density_sample": ([P, P, P, P, P, P, P, C.c_float, P, P], C.c_int),
+    "arapgs_grid_set_ada_lpf": ([P, P, P, C.c_float], C.c_int),
+    "arapgs_grid_ada_lpf": ([P, C.POINTER(P)], C.c_int),
     "arapgs_solver_set_host_checks": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
@@ -138,6 +140,7 @@ EXPORTS = {
     "arapgs_realign_destroy": ([P], None),
     "arapgs_realign_step": ([P, P, P, P, P, C.c_float, P, P, P, P, P, P], C.c_int),
     "arapgs_realign_set_options": ([P, P, P], C.c_int),
+    "arapgs_realign_expand": ([P, P], C.c_int),
     "arapgs_realign_cells": ([P, C.POINTER(C.c_int64), C.POINTER(P), C.POINTER(P)], C.c_int),
     "arapgs_chain_create": ([P, C.POINTER(ChainDesc), C.POINTER(P)], C.c_int),
     "arapgs_chain_destroy": ([P], None),
@@ -758,6 +761,28 @@ class Grid:
         self._info()
         return a7
 
+    def set_ada_lpf(self, lpf_param, samples=None):
+        """GetAdaLpfRatio (GaussianView.cpp:4670-4749): per-cell adaptive low-pass
+        filters lpf_param * M M^T from the cells' corner samples (the grid's own
+        samples when None); the density calls on this grid use them until
+        set_ada_lpf(0)."""
+        self.ctx._check(lib().arapgs_grid_set_ada_lpf(self.ctx.h, self.h, ptr(samples), float(lpf_param)))
+
+    def ada_lpf(self):
+        """Host copy of the adaptive filters (n_valid, 6) as (xx, yy, zz, xy, xz, yz),
+        or None while the grid uses lpf I."""
+        import numpy as np
+        import torch
+        p = P()
+        lib().arapgs_grid_ada_lpf(self.h, C.byref(p))
+        if not p.value:
+            return None
+        a = np.empty((self.n_valid, 6), np.float32)
+        torch.cuda.synchronize()
+        if a.size and _hip_memcpy_d2h(a.ctypes.data, p.value, a.nbytes) != 0:
+            raise ArapgsError("hipMemcpy failed")
+        return a
+
     def arrays(self):
         """Copies of (prefix, list, valid, samples) on the host, via torch."""
         import numpy as np
@@ -997,6 +1022,11 @@ class Realign:
             _hip_memcpy_d2h(loss.ctypes.data, cl.value, loss.nbytes)
         return skip, loss
 
+    def expand(self):
+        """ExpandOpRange (GaussianView.cpp:4320-4360): converged cells next to a
+        non-converged one (or that are one) are optimised again."""
+        self.ctx._check(lib().arapgs_realign_expand(self.ctx.h, self.h))
+
     def step(self, samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, loss=None):
         self.ctx._check(lib().arapgs_realign_step(self.ctx.h, self.h, ptr(samples), ptr(aim_feat), ptr(aim_dens), lpf,
                                                   ptr(pos), ptr(rot), ptr(log_scale), ptr(logit_opacity), ptr(sh),
@@ -1015,15 +1045,22 @@ class Realign:
 
 
 def realign_loop(ctx, grid, realign, samples, aim_feat, aim_dens, lpf, pos, rot, log_scale, logit_opacity, sh, steps,
-                 rebin_interval=10, loss=None, on_step=None):
+                 rebin_interval=10, loss=None, on_step=None, high_quality=False, ada_lpf_param=None):
     """The F6 handler of GaussianView (GaussianView.cpp:1695-1733) over the
-    library: the cell lists are rebuilt from the Gaussians' current state
-    before the loop (UpdateContainingRelationshipWithStaticGrids) and every
-    rebin_interval steps after the first (UpdateContainingRelationship,
-    _containing_relation_interval = 10 by default, GaussianView.hpp:427), and
-    every step is one arapgs_realign_step (GPUOptimize). on_step(i, loss) is
+    library: ExpandOpRange first when high_quality (:1699-1701); the adaptive
+    low-pass filters from the current samples when ada_lpf_param is given
+    (GetAdaLpfRatio, :1703-1705; enable_adaptive_lpf, GaussianView.hpp:180);
+    the cell lists rebuilt from the Gaussians' current state before the loop
+    (UpdateContainingRelationshipWithStaticGrids) and every rebin_interval
+    steps after the first (UpdateContainingRelationship,
+    _containing_relation_interval = 10 by default, GaussianView.hpp:427); and
+    every step one arapgs_realign_step (GPUOptimize). on_step(i, loss) is
     called after each step when given (it reads the loss: one sync per step)."""
     import torch
+    if high_quality:
+        realign.expand()
+    if ada_lpf_param is not None:
+        grid.set_ada_lpf(ada_lpf_param, samples=samples)
     for i in range(steps):
         if i % max(1, rebin_interval) == 0:
             grid.rebin(pos, rot, torch.exp(log_scale), torch.sigmoid(logit_opacity))

@@ -42,12 +42,13 @@ __global__ void zero_f32_kernel(float* __restrict__ p, int64_t n) {
         p[i] = 0.0f;
 }
 
-// Inverse covariance of a density Gaussian as the density kernel forms it:
-// Sigma = R(q/|q|) diag(s^2) R^T + lpf I, Q = Sigma^-1 by cofactors. Returns
-// Q (q00, q11, q22, q01, q02, q12), R and Sigma for the backward.
+// Inverse covariance of a density Gaussian as the density kernel forms it in
+// valid cell v: Sigma = R(q/|q|) diag(s^2) R^T + L (L = lpf I or the cell's
+// adaptive filter), Q = Sigma^-1 by cofactors. Returns Q (q00, q11, q22, q01,
+// q02, q12), R and Sigma for the backward.
 __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ rot, const float* __restrict__ scale,
-                                                    int64_t g, float lpf, float Q[6], float R[3][3],
-                                                    float S[3][3]) {
+                                                    int64_t g, const DensityFilter& flt, int64_t v, float Q[6],
+                                                    float R[3][3], float S[3][3]) {
 #pragma clang fp contract(fast)
     float q[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
     quat_normalize(q);
@@ -62,8 +63,9 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
                 const float sk = scale[3 * g + k];
                 a += R[i][k] * sk * sk * R[j][k];
             }
-            S[i][j] = a + (i == j ? lpf : 0.0f);
+            S[i][j] = a;
         }
+    density_filter_add(flt, v, S);
     const float c00 = S[1][1] * S[2][2] - S[1][2] * S[2][1];
     const float c01 = S[1][2] * S[2][0] - S[1][0] * S[2][2];
     const float c02 = S[1][0] * S[2][1] - S[1][1] * S[2][0];
@@ -77,6 +79,27 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
     Q[5] = (S[0][2] * S[1][0] - S[0][0] * S[1][2]) * id;
 }
 
+// dSigma = -Q dQ Q for Q = Sigma^-1, both symmetric and packed (00, 11, 22,
+// 01, 02, 12); the off-diagonal entries of dq6 and of the result count both
+// symmetric entries (dL/dX01 + dL/dX10).
+__device__ __forceinline__ void dinv_to_dsigma(const float Qv[6], const float dq6[6], float ds6[6]) {
+    const float Qm[3][3] = {{Qv[0], Qv[3], Qv[4]}, {Qv[3], Qv[1], Qv[5]}, {Qv[4], Qv[5], Qv[2]}};
+    const float dQ[3][3] = {{dq6[0], 0.5f * dq6[3], 0.5f * dq6[4]},
+                            {0.5f * dq6[3], dq6[1], 0.5f * dq6[5]},
+                            {0.5f * dq6[4], 0.5f * dq6[5], dq6[2]}};
+    float tmp[3][3], dS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) tmp[i][j] = dQ[i][0] * Qm[0][j] + dQ[i][1] * Qm[1][j] + dQ[i][2] * Qm[2][j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) dS[i][j] = -(Qm[i][0] * tmp[0][j] + Qm[i][1] * tmp[1][j] + Qm[i][2] * tmp[2][j]);
+    ds6[0] = dS[0][0]; ds6[1] = dS[1][1]; ds6[2] = dS[2][2];
+    ds6[3] = dS[0][1] + dS[1][0]; ds6[4] = dS[0][2] + dS[2][0]; ds6[5] = dS[1][2] + dS[2][1];
+}
+
 // a_gs = o_g exp(-q/2), q = d^T Q d, d = x_s - mu_g; feat_s = sum_g a_gs c_g,
 // dens_s = sum_g a_gs. With da = dL/dfeat_s . c_g + dL/ddens_s:
 //   dL/dc_g += a dL/dfeat_s,  dL/dq = -a da / 2,  dL/dQ += dL/dq d d^T,
@@ -86,8 +109,9 @@ __device__ __forceinline__ void density_inverse_cov(const float* __restrict__ ro
 // gradients sit in LDS (every lane reads the same sample: broadcast). The
 // sample gradient of each step is a wave sum. gdens is component-major
 // (component k of Gaussian g at k * gstride + g): d mu (3), d Q (q00, q11,
-// q22, then q01, q02, q12 each counted for both symmetric entries), d c (48),
-// d o (1).
+// q22, then q01, q02, q12 each counted for both symmetric entries; with an
+// adaptive filter, whose Q differs per cell, d Sigma = -Q dQ Q in the same
+// packing instead), d c (48), d o (1).
 __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const int* __restrict__ valid,
                                                               const int64_t* __restrict__ prefix,
                                                               const int* __restrict__ list,
@@ -96,7 +120,7 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
                                                               const float* __restrict__ rot,
                                                               const float* __restrict__ scale,
                                                               const float* __restrict__ opacity,
-                                                              const float* __restrict__ sh, float lpf,
+                                                              const float* __restrict__ sh, DensityFilter flt,
                                                               const float* __restrict__ dfeat,
                                                               const float* __restrict__ ddens,
                                                               float* __restrict__ gsamp, float* __restrict__ gdens,
@@ -136,7 +160,7 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
         for (int k = 0; k < 48; k++) cg[k] = 0.0f;
         if (has) {
             float R[3][3], S[3][3];
-            density_inverse_cov(rot, scale, g, lpf, Q, R, S);
+            density_inverse_cov(rot, scale, g, flt, v, Q, R, S);
             mu[0] = pos[3 * g]; mu[1] = pos[3 * g + 1]; mu[2] = pos[3 * g + 2];
             o = opacity[g];
             const float4* s4 = reinterpret_cast<const float4*>(sh + 48 * (int64_t)g);
@@ -179,6 +203,10 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
                 s_dx[s][2] += w2;
             }
         }
+        if (has && flt.ada) {
+            const float gq[6] = {gQ[0], gQ[1], gQ[2], gQ[3], gQ[4], gQ[5]};
+            dinv_to_dsigma(Q, gq, gQ);
+        }
         if (has) {
             // component-major: the wave's atomics of one component land on
             // neighbouring addresses (list entries are sorted by Gaussian)
@@ -198,29 +226,21 @@ __global__ void __launch_bounds__(64) density_backward_kernel(int64_t nv, const 
         for (int k = 0; k < 3; k++) gsamp[3 * si + k] = s_dx[t][k];
 }
 
-// dL/dQ of a density Gaussian (d[0..5] as accumulated by the density backward)
-// back to its rotation q (unnormalised, as stored) and scale s: Sigma = R(q/|q|)
-// diag(s^2) R^T + lpf I, Q = Sigma^-1, dSigma = -Q dQ Q. Adds into dq and ds.
-__device__ __forceinline__ void density_cov_backward(const float* __restrict__ rot, const float* __restrict__ scale,
-                                                     int64_t g, float lpf, const float dq6[6], float dq[4],
-                                                     float ds[3]) {
-    float Qv[6], R[3][3], S[3][3];
-    density_inverse_cov(rot, scale, g, lpf, Qv, R, S);
+// dL/dSigma of a density Gaussian (packed as dinv_to_dsigma writes it) back to
+// its rotation q (unnormalised, as stored) and scale s, Sigma = R(q/|q|)
+// diag(s^2) R^T + L. Adds into dq and ds.
+__device__ __forceinline__ void density_sigma_backward(const float* __restrict__ rot,
+                                                       const float* __restrict__ scale, int64_t g, const float ds6[6],
+                                                       float dq[4], float ds[3]) {
     const float s1[3] = {scale[3 * g], scale[3 * g + 1], scale[3 * g + 2]};
     const float q1[4] = {rot[4 * g], rot[4 * g + 1], rot[4 * g + 2], rot[4 * g + 3]};
-    const float Qm[3][3] = {{Qv[0], Qv[3], Qv[4]}, {Qv[3], Qv[1], Qv[5]}, {Qv[4], Qv[5], Qv[2]}};
-    const float dQ[3][3] = {{dq6[0], 0.5f * dq6[3], 0.5f * dq6[4]},
-                            {0.5f * dq6[3], dq6[1], 0.5f * dq6[5]},
-                            {0.5f * dq6[4], 0.5f * dq6[5], dq6[2]}};
-    float tmp[3][3], dS[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) tmp[i][j] = dQ[i][0] * Qm[0][j] + dQ[i][1] * Qm[1][j] + dQ[i][2] * Qm[2][j];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) dS[i][j] = -(Qm[i][0] * tmp[0][j] + Qm[i][1] * tmp[1][j] + Qm[i][2] * tmp[2][j]);
+    float qn[4] = {q1[0], q1[1], q1[2], q1[3]};
+    quat_normalize(qn);
+    float R[3][3];
+    quat_to_mat(qn[0], qn[1], qn[2], qn[3], R);
+    const float dS[3][3] = {{ds6[0], 0.5f * ds6[3], 0.5f * ds6[4]},
+                            {0.5f * ds6[3], ds6[1], 0.5f * ds6[5]},
+                            {0.5f * ds6[4], 0.5f * ds6[5], ds6[2]}};
     float dR[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)
@@ -240,11 +260,27 @@ __device__ __forceinline__ void density_cov_backward(const float* __restrict__ r
             for (int j = 0; j < 3; j++) dD += dS[i][j] * R[i][k] * R[j][k];
         ds[k] += dD * 2.0f * s1[k];
     }
-    float qn[4] = {q1[0], q1[1], q1[2], q1[3]};
-    quat_normalize(qn);
     float dqn[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     quat_to_mat_backward(qn, dR, dqn);
     quat_normalize_backward(q1, dqn, dq);
+}
+
+// The accumulated covariance gradient d6 (d[3..8] of the density backward) of
+// Gaussian g back to (q, s): dL/dQ with the scalar filter (Sigma = R diag(s^2)
+// R^T + lpf I, dSigma = -Q dQ Q), dL/dSigma itself with an adaptive one.
+__device__ __forceinline__ void density_cov_backward(const float* __restrict__ rot, const float* __restrict__ scale,
+                                                     int64_t g, const DensityFilter& flt, const float d6[6],
+                                                     float dq[4], float ds[3]) {
+    float ds6[6];
+    if (flt.ada) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) ds6[k] = d6[k];
+    } else {
+        float Qv[6], R[3][3], S[3][3];
+        density_inverse_cov(rot, scale, g, flt, 0, Qv, R, S);
+        dinv_to_dsigma(Qv, d6, ds6);
+    }
+    density_sigma_backward(rot, scale, g, ds6, dq, ds);
 }
 
 // The density gradients of every Gaussian's own attributes (the VJP of the
@@ -252,7 +288,7 @@ __device__ __forceinline__ void density_cov_backward(const float* __restrict__ r
 // scale, opacity and SH from the component-major accumulators gd (58 x N).
 __global__ void __launch_bounds__(256) density_attr_grad_kernel(int64_t N, const float* __restrict__ rot,
                                                                 const float* __restrict__ scale,
-                                                                const float* __restrict__ gd, float lpf,
+                                                                const float* __restrict__ gd, DensityFilter flt,
                                                                 float* __restrict__ dpos, float* __restrict__ drot,
                                                                 float* __restrict__ dscale, float* __restrict__ dopac,
                                                                 float* __restrict__ dsh) {
@@ -262,7 +298,7 @@ __global__ void __launch_bounds__(256) density_attr_grad_kernel(int64_t N, const
 #pragma unroll
     for (int k = 0; k < 6; k++) dq6[k] = gd[(3 + k) * N + g];
     float dq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ds[3] = {0.0f, 0.0f, 0.0f};
-    density_cov_backward(rot, scale, g, lpf, dq6, dq, ds);
+    density_cov_backward(rot, scale, g, flt, dq6, dq, ds);
 #pragma unroll
     for (int k = 0; k < 3; k++) dpos[3 * g + k] = gd[k * N + g];
 #pragma unroll
@@ -282,7 +318,7 @@ __global__ void __launch_bounds__(128) six_point_backward_kernel(
     int64_t N, int K, const NodeXf* __restrict__ table, const uint32_t* __restrict__ ends_idx,
     const float* __restrict__ ends_w, const float* __restrict__ ends, const float* __restrict__ rot0,
     const float* __restrict__ scale0, const float* __restrict__ sh0, const float* __restrict__ rot1,
-    const float* __restrict__ scale1, const float* __restrict__ gr, const float* __restrict__ gd, float lpf,
+    const float* __restrict__ scale1, const float* __restrict__ gr, const float* __restrict__ gd, DensityFilter flt,
     float pad, float coeff, const ShRotTables* __restrict__ Tsh, float* __restrict__ gends,
     float* __restrict__ drot0, float* __restrict__ dscale0) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,7 +338,7 @@ __global__ void __launch_bounds__(128) six_point_backward_kernel(
         for (int k = 0; k < 3; k++) dmu[k] += d[k];
 #pragma unroll
         for (int k = 0; k < 48; k++) dc1[k] += d[9 + k];
-        density_cov_backward(rot1, scale1, g, lpf, d + 3, dq1, ds1);
+        density_cov_backward(rot1, scale1, g, flt, d + 3, dq1, ds1);
     }
 
     // recompute the six-point forward from the blended end points
@@ -603,24 +639,58 @@ __global__ void __launch_bounds__(64) static_cells_kernel(int64_t nv, const int*
     if (threadIdx.x == 0) stat[v] = any ? 0 : 1;
 }
 
-// Per-cell convergence of the realignment: the first step after a re-binning
-// records every cell's loss; a later step marks a cell converged once its loss
-// is at most tol times that record. skip = static or converged (sticky until
-// the next re-binning).
+// Per-cell convergence of the realignment: the first step of a run (a new
+// realignment or new options) records every cell's loss and clears the
+// converged flags; a later step marks a cell converged once its loss is at
+// most tol times that record (sticky, also across re-binnings, as the
+// reference's grid_is_converged). skip = static or converged.
 __global__ void __launch_bounds__(256) converge_kernel(int64_t nv, const float* __restrict__ cell_loss,
                                                        float* __restrict__ cell_ref,
                                                        const uint8_t* __restrict__ stat, float tol, int first,
-                                                       uint8_t* __restrict__ skip) {
+                                                       uint8_t* __restrict__ conv, uint8_t* __restrict__ skip) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const bool st = stat && stat[v];
     if (first) {
         cell_ref[v] = cell_loss[v];
+        conv[v] = 0;
         skip[v] = st ? 1 : 0;
         return;
     }
-    const bool conv = tol > 0.0f && cell_loss[v] <= tol * cell_ref[v];
-    skip[v] = (skip[v] || st || conv) ? 1 : 0;
+    const bool c = conv[v] || (tol > 0.0f && cell_loss[v] <= tol * cell_ref[v]);
+    conv[v] = c ? 1 : 0;
+    skip[v] = (st || c) ? 1 : 0;
+}
+
+// ExpandOpRange (GaussianView.cpp:4320-4360): every cell that is, or is a face
+// neighbour (clamped to the grid) of, a non-converged valid cell is marked in
+// map (G^3 bytes, cleared by the caller) ...
+__global__ void __launch_bounds__(256) expand_mark_kernel(int64_t nv, const int* __restrict__ valid,
+                                                          const uint8_t* __restrict__ conv, int G,
+                                                          uint8_t* __restrict__ map) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv || conv[v]) return;
+    const int c = valid[v];
+    const int z = c % G, y = (c / G) % G, x = c / (G * G);
+    const int64_t GG = (int64_t)G * G;
+    map[c] = 1;
+    map[(int64_t)max(x - 1, 0) * GG + (int64_t)y * G + z] = 1;
+    map[(int64_t)min(x + 1, G - 1) * GG + (int64_t)y * G + z] = 1;
+    map[(int64_t)x * GG + (int64_t)max(y - 1, 0) * G + z] = 1;
+    map[(int64_t)x * GG + (int64_t)min(y + 1, G - 1) * G + z] = 1;
+    map[(int64_t)x * GG + (int64_t)y * G + max(z - 1, 0)] = 1;
+    map[(int64_t)x * GG + (int64_t)y * G + min(z + 1, G - 1)] = 1;
+}
+
+// ... and every marked valid cell is no longer converged.
+__global__ void __launch_bounds__(256) expand_apply_kernel(int64_t nv, const int* __restrict__ valid,
+                                                           const uint8_t* __restrict__ map,
+                                                           const uint8_t* __restrict__ stat,
+                                                           uint8_t* __restrict__ conv, uint8_t* __restrict__ skip) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv || !map[valid[v]]) return;
+    conv[v] = 0;
+    skip[v] = (stat && stat[v]) ? 1 : 0;
 }
 
 // One Adam step per Gaussian over its 59 raw parameters (pos 3, rot 4,
@@ -691,10 +761,22 @@ int launch_static_cells(int64_t nv, const int* valid, const int64_t* prefix, con
 }
 
 int launch_converge(int64_t nv, const float* cell_loss, float* cell_ref, const uint8_t* stat, float tol, int first,
-                    uint8_t* skip, hipStream_t st) {
+                    uint8_t* conv, uint8_t* skip, hipStream_t st) {
     if (nv <= 0) return 0;
     hipLaunchKernelGGL(converge_kernel, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, cell_loss, cell_ref, stat, tol,
-                       first, skip);
+                       first, conv, skip);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_expand_op_range(int64_t nv, const int* valid, int G, const uint8_t* stat, uint8_t* conv, uint8_t* skip,
+                           uint8_t* map, hipStream_t st) {
+    if (nv <= 0) return 0;
+    const int64_t gc = (int64_t)G * G * G;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(std::min<int64_t>(nblk((gc + 3) / 4, 256), 4096)), dim3(256), 0, st,
+                       reinterpret_cast<float*>(map), (gc + 3) / 4);
+    hipLaunchKernelGGL(expand_mark_kernel, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, valid, conv, G, map);
+    hipLaunchKernelGGL(expand_apply_kernel, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, valid, map, stat, conv, skip);
     HIP_RET(hipGetLastError());
     return 0;
 }
@@ -720,19 +802,19 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st) {
 
 int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
-                            const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
+                            const float* sh, DensityFilter flt, const float* dfeat, const float* ddens, float* gsamp,
                             float* gdens, int64_t n_gauss, hipStream_t st, const uint8_t* skip) {
     if (nv <= 0) return 0;
     hipLaunchKernelGGL(density_backward_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples,
-                       pos, rot, scale, opacity, sh, lpf, dfeat, ddens, gsamp, gdens, n_gauss, skip);
+                       pos, rot, scale, opacity, sh, flt, dfeat, ddens, gsamp, gdens, n_gauss, skip);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
-int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, float lpf, float* dpos,
+int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, DensityFilter flt, float* dpos,
                              float* drot, float* dscale, float* dopac, float* dsh, hipStream_t st) {
     if (N <= 0) return 0;
-    hipLaunchKernelGGL(density_attr_grad_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, rot, scale, gd, lpf, dpos,
+    hipLaunchKernelGGL(density_attr_grad_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, N, rot, scale, gd, flt, dpos,
                        drot, dscale, dopac, dsh);
     HIP_RET(hipGetLastError());
     return 0;
@@ -740,12 +822,12 @@ int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, co
 
 int launch_six_point_backward(int64_t N, int K, const NodeXf* table, const uint32_t* ends_idx, const float* ends_w,
                               const float* ends, const float* rot0, const float* scale0, const float* sh0,
-                              const float* rot1, const float* scale1, const float* gr, const float* gd, float lpf,
+                              const float* rot1, const float* scale1, const float* gr, const float* gd, DensityFilter flt,
                               float pad, float coeff, const ShRotTables* Tsh, float* gends, float* drot0,
                               float* dscale0, hipStream_t st) {
     if (N <= 0) return 0;
     hipLaunchKernelGGL(six_point_backward_kernel, dim3(nblk(N, 128)), dim3(128), 0, st, N, K, table, ends_idx,
-                       ends_w, ends, rot0, scale0, sh0, rot1, scale1, gr, gd, lpf, pad, coeff, Tsh, gends, drot0,
+                       ends_w, ends, rot0, scale0, sh0, rot1, scale1, gr, gd, flt, pad, coeff, Tsh, gends, drot0,
                        dscale0);
     HIP_RET(hipGetLastError());
     return 0;

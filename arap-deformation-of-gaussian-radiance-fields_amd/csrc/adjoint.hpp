@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "common.hpp"
+#include "grid.hpp"
 
 namespace arapgs {
 
@@ -16,13 +17,13 @@ int launch_zero_f32(float* p, int64_t n, hipStream_t st);
 // dfeat / ddens may be null.
 int launch_density_backward(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                             const float* pos, const float* rot, const float* scale, const float* opacity,
-                            const float* sh, float lpf, const float* dfeat, const float* ddens, float* gsamp,
+                            const float* sh, DensityFilter flt, const float* dfeat, const float* ddens, float* gsamp,
                             float* gdens, int64_t n_gauss, hipStream_t st, const uint8_t* skip = nullptr);
 
 // Gradients of each Gaussian's own attributes from the density accumulators
 // (58 x N): dpos N*3, drot N*4 (w.r.t. the stored rotation), dscale N*3,
 // dopac N, dsh N*48.
-int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, float lpf, float* dpos,
+int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, const float* gd, DensityFilter flt, float* dpos,
                              float* drot, float* dscale, float* dopac, float* dsh, hipStream_t st);
 
 // Per-Gaussian adjoint from (mu', q', s', sh') to the blended end points (gends,
@@ -30,7 +31,7 @@ int launch_density_attr_grad(int64_t N, const float* rot, const float* scale, co
 // gd: density gradients (57 x N component-major) or null.
 int launch_six_point_backward(int64_t N, int K, const NodeXf* table, const uint32_t* ends_idx, const float* ends_w,
                               const float* ends, const float* rot0, const float* scale0, const float* sh0,
-                              const float* rot1, const float* scale1, const float* gr, const float* gd, float lpf,
+                              const float* rot1, const float* scale1, const float* gr, const float* gd, DensityFilter flt,
                               float pad, float coeff, const ShRotTables* Tsh, float* gends, float* drot0,
                               float* dscale0, hipStream_t st);
 
@@ -58,9 +59,14 @@ int launch_l1_grad(int64_t S, const float* feat, const float* dens, const float*
 // stat[v] = 1 when valid cell v's list holds no Gaussian with moved[g] != 0
 int launch_static_cells(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const uint8_t* moved,
                         uint8_t* stat, hipStream_t st);
-// first: cell_ref = cell_loss, skip = stat; else skip |= stat | (tol > 0 && cell_loss <= tol * cell_ref)
+// first: cell_ref = cell_loss, conv = 0, skip = stat; else conv |= (tol > 0 &&
+// cell_loss <= tol * cell_ref), skip = stat | conv
 int launch_converge(int64_t nv, const float* cell_loss, float* cell_ref, const uint8_t* stat, float tol, int first,
-                    uint8_t* skip, hipStream_t st);
+                    uint8_t* conv, uint8_t* skip, hipStream_t st);
+// ExpandOpRange: valid cells that are, or are face neighbours of, a
+// non-converged valid cell lose their converged flag (map: G^3 bytes scratch).
+int launch_expand_op_range(int64_t nv, const int* valid, int G, const uint8_t* stat, uint8_t* conv, uint8_t* skip,
+                           uint8_t* map, hipStream_t st);
 // moved (nullable): only Gaussians with moved[g] != 0 are updated; max_scale
 // (nullable, N*3): the activated scales are clamped to it after the update
 int launch_adam(int64_t N, const float* dpos, const float* drot, const float* dscale, const float* dopac,

@@ -936,8 +936,18 @@ struct arapgs_grid {
     int* list = nullptr;
     int* valid = nullptr;
     float* samples = nullptr;
+    // adaptive low-pass filter per valid cell (6 floats each; null: lpf I)
+    float* ada = nullptr;
+    bool ada_on = false;
     int device = 0;
 };
+
+namespace {
+// The density filter of a grid: its adaptive matrices when set, else lpf I.
+DensityFilter grid_filter(const arapgs_grid* g, float lpf) {
+    return DensityFilter{lpf, (g && g->ada_on) ? g->ada : nullptr};
+}
+}  // namespace
 
 extern "C" {
 
@@ -1236,9 +1246,35 @@ int arapgs_grid_rebin(arapgs_ctx* c, arapgs_grid* g, const float* aabbs, int64_t
 void arapgs_grid_destroy(arapgs_grid* g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
-    for (void* p : {(void*)g->prefix, (void*)g->list, (void*)g->valid, (void*)g->samples})
+    for (void* p : {(void*)g->prefix, (void*)g->list, (void*)g->valid, (void*)g->samples, (void*)g->ada})
         if (p) (void)hipFree(p);
     delete g;
+}
+
+int arapgs_grid_set_ada_lpf(arapgs_ctx* c, arapgs_grid* g, const float* samples, float lpf_param) {
+    CHECK_CTX(c);
+    if (!g || !(lpf_param >= 0.0f)) return fail(c, -1, "grid_set_ada_lpf: bad arguments");
+    if (lpf_param == 0.0f) {
+        g->ada_on = false;
+        return 0;
+    }
+    (void)hipSetDevice(c->device);
+    if (!g->ada && g->n_valid > 0) {
+        // the stream may still run density calls that read the scalar filter
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMalloc((void**)&g->ada, sizeof(float) * 6 * (size_t)g->n_valid);
+        if (e != hipSuccess) return hip_fail(c, e, "grid_set_ada_lpf");
+    }
+    int rc = check_launch(c, launch_ada_lpf(g->n_valid, samples ? samples : g->samples, lpf_param, g->ada, c->stream),
+                          "grid_set_ada_lpf");
+    if (!rc) g->ada_on = true;
+    return rc;
+}
+
+int arapgs_grid_ada_lpf(const arapgs_grid* g, const float** mats) {
+    if (!g || !mats) return -1;
+    *mats = g->ada_on ? g->ada : nullptr;
+    return 0;
 }
 
 int arapgs_grid_info(const arapgs_grid* g, int64_t* n_pairs, int64_t* n_valid, const int64_t** prefix, const int** list,
@@ -1270,13 +1306,13 @@ int arapgs_density_backward(arapgs_ctx* c, const arapgs_grid* g, const float* sa
     if (!rc && (dfeat || dopac))
         rc = check_launch(c, launch_density_backward(g->n_valid, g->valid, g->prefix, g->list,
                                                      samples ? samples : g->samples, pos, rot, scale, opacity, sh,
-                                                     lpf, dfeat, dopac, dsamples, gd, N, c->stream),
+                                                     grid_filter(g, lpf), dfeat, dopac, dsamples, gd, N, c->stream),
                           "density_backward");
     else if (!rc && dsamples)
         rc = check_launch(c, launch_zero_f32(dsamples, 3 * 64 * g->n_valid, c->stream), "density_backward");
     if (!rc)
-        rc = check_launch(c, launch_density_attr_grad(N, rot, scale, gd, lpf, dpos, drot, dscale, dopacity, dsh,
-                                                      c->stream),
+        rc = check_launch(c, launch_density_attr_grad(N, rot, scale, gd, grid_filter(g, lpf), dpos, drot, dscale,
+                                                      dopacity, dsh, c->stream),
                           "density_backward");
     return rc;
 }
@@ -1286,7 +1322,7 @@ int arapgs_density_sample(arapgs_ctx* c, const arapgs_grid* g, const float* pos,
     CHECK_CTX(c);
     if (!g || !pos || !rot || !scale || !opacity || !sh || !feat || !opac) return fail(c, -1, "density_sample: bad arguments");
     return check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, g->samples, pos, rot, scale,
-                                          opacity, sh, lpf, feat, opac, c->stream), "density_sample");
+                                          opacity, sh, grid_filter(g, lpf), feat, opac, c->stream), "density_sample");
 }
 
 }  // extern "C"
@@ -1608,7 +1644,7 @@ int arapgs_chain_forward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* 
     if (rc) return rc;
     if (ch->S > 0)
         rc = check_launch(c, launch_density(ch->nv, ch->grid->valid, ch->grid->prefix, ch->grid->list, ch->samples, ch->pos,
-                                            ch->rot, ch->scale, ch->d.opacity, ch->sh, ch->d.lpf, feat, dens,
+                                            ch->rot, ch->scale, ch->d.opacity, ch->sh, grid_filter(ch->grid, ch->d.lpf), feat, dens,
                                             c->stream),
                           "chain density");
     return rc;
@@ -1663,7 +1699,8 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
         if (!rc && (dfeat || ddens))
             rc = check_launch(c, launch_density_backward(ch->nv, ch->grid->valid, ch->grid->prefix, ch->grid->list, ch->samples,
                                                          ch->pos, ch->rot, ch->scale, ch->d.opacity, ch->sh,
-                                                         ch->d.lpf, dfeat, ddens, ch->gsamp, ch->gdens, ch->N,
+                                                         grid_filter(ch->grid, ch->d.lpf), dfeat, ddens, ch->gsamp,
+                                                         ch->gdens, ch->N,
                                                          st),
                               "chain density backward");
         else if (!rc)
@@ -1673,7 +1710,8 @@ int arapgs_chain_backward(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf*
     }
     rc = check_launch(c, launch_six_point_backward(ch->N, ch->K, (const NodeXf*)table, ch->d.ends_idx, ch->d.ends_w,
                                                    ch->ends, rot0, scale0, ch->d.sh0, ch->rot, ch->scale, ch->gout,
-                                                   gd, ch->d.lpf, ch->d.axis_padding, ch->d.end_coeff, c->shrot_dev,
+                                                   gd, grid_filter(ch->grid, ch->d.lpf), ch->d.axis_padding,
+                                                   ch->d.end_coeff, c->shrot_dev,
                                                    ch->gends, drot0, dscale0, st),
                       "chain six-point backward");
     if (rc) return rc;
@@ -1710,10 +1748,11 @@ struct arapgs_realign {
     const float* max_scale = nullptr;  // upper bound of the activated scales (null: none)
     float conv_tol = 0.0f;             // per-cell convergence (0: off)
     // per valid cell: static (no moved Gaussian), skip (static or converged), loss, reference loss
-    uint8_t *stat = nullptr, *skip = nullptr;
+    uint8_t *stat = nullptr, *skip = nullptr, *conv = nullptr;
     float *cell_loss = nullptr, *cell_ref = nullptr;
+    uint8_t* cell_map = nullptr;  // G^3 bytes for ExpandOpRange (on first use)
     int64_t seen_version = -1;  // grid version the static flags were computed for
-    bool restart = true;        // the next step records the reference losses
+    bool restart = true;        // the next step records the reference losses and clears convergence
     std::vector<void*> allocs;
 };
 
@@ -1748,7 +1787,8 @@ int arapgs_realign_create(arapgs_ctx* c, const arapgs_grid* grid, int64_t n, con
     struct { float** p; size_t count; } bufs[] = {
         {&r->scale, 3 * N}, {&r->opac, N}, {&r->feat, 48 * S}, {&r->dens, S}, {&r->dfeat, 48 * S}, {&r->ddens, S},
         {&r->gd, 58 * N}, {&r->grad, 59 * N}, {&r->m, 59 * N}, {&r->v, 59 * N},
-        {&r->cell_loss, NV}, {&r->cell_ref, NV}, {(float**)&r->stat, (NV + 3) / 4}, {(float**)&r->skip, (NV + 3) / 4}};
+        {&r->cell_loss, NV}, {&r->cell_ref, NV}, {(float**)&r->stat, (NV + 3) / 4}, {(float**)&r->skip, (NV + 3) / 4},
+        {(float**)&r->conv, (NV + 3) / 4}};
     for (auto& b : bufs) {
         hipError_t e = hipMalloc((void**)b.p, sizeof(float) * b.count);
         if (e != hipSuccess) {
@@ -1789,11 +1829,10 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
             rc = check_launch(c, launch_static_cells(g->n_valid, g->valid, g->prefix, g->list, r->moved, r->stat, st),
                               "realign static cells");
         r->seen_version = g->version;
-        r->restart = true;
     }
     if (!rc)
         rc = check_launch(c, launch_density(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot, r->scale,
-                                            r->opac, sh, lpf, r->feat, r->dens, st),
+                                            r->opac, sh, grid_filter(g, lpf), r->feat, r->dens, st),
                           "realign density");
     if (!rc && loss) rc = check_launch(c, launch_zero_f32((float*)loss, 2, st), "realign loss");  // 0.0 as f64
     if (!rc)
@@ -1802,13 +1841,13 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
                           "realign loss");
     if (!rc)
         rc = check_launch(c, launch_converge(g->n_valid, r->cell_loss, r->cell_ref, r->moved ? r->stat : nullptr,
-                                             r->conv_tol, r->restart ? 1 : 0, r->skip, st),
+                                             r->conv_tol, r->restart ? 1 : 0, r->conv, r->skip, st),
                           "realign convergence");
     r->restart = false;
     if (!rc) rc = check_launch(c, launch_zero_f32(r->gd, 58 * N, st), "realign backward");
     if (!rc)
         rc = check_launch(c, launch_density_backward(g->n_valid, g->valid, g->prefix, g->list, samples, pos, rot,
-                                                     r->scale, r->opac, sh, lpf, r->dfeat, r->ddens, nullptr, r->gd,
+                                                     r->scale, r->opac, sh, grid_filter(g, lpf), r->dfeat, r->ddens, nullptr, r->gd,
                                                      N, st, r->skip),
                           "realign backward");
     float* dpos = r->grad;
@@ -1817,7 +1856,8 @@ int arapgs_realign_step(arapgs_ctx* c, arapgs_realign* r, const float* samples, 
     float* dopac = dscale + 3 * N;
     float* dsh = dopac + N;
     if (!rc)
-        rc = check_launch(c, launch_density_attr_grad(N, rot, r->scale, r->gd, lpf, dpos, drot, dscale, dopac, dsh, st),
+        rc = check_launch(c, launch_density_attr_grad(N, rot, r->scale, r->gd, grid_filter(g, lpf), dpos, drot, dscale,
+                                                      dopac, dsh, st),
                           "realign backward");
     if (rc) return rc;
     r->t++;
@@ -1837,7 +1877,25 @@ int arapgs_realign_set_options(arapgs_ctx* c, arapgs_realign* r, const arapgs_re
     r->max_scale = o->max_scale;
     r->conv_tol = o->conv_tol;
     r->seen_version = -1;  // static cells and convergence start over
+    r->restart = true;
     return 0;
+}
+
+int arapgs_realign_expand(arapgs_ctx* c, arapgs_realign* r) {
+    CHECK_CTX(c);
+    if (!r) return fail(c, -1, "realign_expand: bad arguments");
+    if (r->restart) return 0;  // nothing converged yet
+    (void)hipSetDevice(c->device);
+    const arapgs_grid* g = r->grid;
+    if (!r->cell_map) {
+        const int64_t gc = (int64_t)g->G * g->G * g->G;
+        hipError_t e = hipMalloc((void**)&r->cell_map, (size_t)((gc + 3) / 4) * 4);
+        if (e != hipSuccess) return hip_fail(c, e, "realign_expand");
+        r->allocs.push_back(r->cell_map);
+    }
+    return check_launch(c, launch_expand_op_range(g->n_valid, g->valid, g->G, r->moved ? r->stat : nullptr, r->conv,
+                                                  r->skip, r->cell_map, c->stream),
+                        "realign_expand");
 }
 
 int arapgs_realign_cells(const arapgs_realign* r, int64_t* n_valid, const uint8_t** skip, const float** cell_loss) {

@@ -295,14 +295,16 @@ __global__ void samples_kernel(const int* __restrict__ valid, int64_t nv, GridGe
     samples[3 * t + 2] = z0 + sz * interval;
 }
 
-// Density sample: a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d),
-// feature = sum a_g sh_g, opacity = sum a_g. One workgroup (64 lanes) per cell.
+// Density sample: a_g(x) = o_g exp(-1/2 d^T (Sigma_g + L)^-1 d), L = lpf I or
+// the cell's adaptive filter, feature = sum a_g sh_g, opacity = sum a_g. One
+// workgroup (64 lanes) per cell.
 __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __restrict__ valid,
                                                      const int64_t* __restrict__ prefix, const int* __restrict__ list,
                                                      const float* __restrict__ samples, const float* __restrict__ pos,
                                                      const float* __restrict__ rot, const float* __restrict__ scale,
                                                      const float* __restrict__ opacity, const float* __restrict__ sh,
-                                                     float lpf, float* __restrict__ feat, float* __restrict__ opac) {
+                                                     DensityFilter flt, float* __restrict__ feat,
+                                                     float* __restrict__ opac) {
 #pragma clang fp contract(fast)
     constexpr int B = 64;
     __shared__ float s_q[B][10];   // mu(3), inverse covariance (6), opacity
@@ -336,8 +338,9 @@ __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __re
                         const float sk = scale[3 * g + k];
                         a += R[i][k] * sk * sk * R[j][k];
                     }
-                    S[i][j] = a + (i == j ? lpf : 0.0f);
+                    S[i][j] = a;
                 }
+            density_filter_add(flt, v, S);
             const float c00 = S[1][1] * S[2][2] - S[1][2] * S[2][1];
             const float c01 = S[1][2] * S[2][0] - S[1][0] * S[2][2];
             const float c02 = S[1][0] * S[2][1] - S[1][1] * S[2][0];
@@ -378,6 +381,43 @@ __global__ void __launch_bounds__(64) density_kernel(int64_t nv, const int* __re
 #pragma unroll
     for (int k = 0; k < 12; k++) o4[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
     opac[si] = aop;
+}
+
+// GetAdaLpfRatio (GaussianView.cpp:4670-4749), one thread per valid cell. The
+// cell's samples are a 4x4x4 block (sample x*16 + y*4 + z); corner c has
+// offsets q_c = (c&1, (c>>1)&1, (c>>2)&1) and the sample at 48 q_x + 12 q_y +
+// 3 q_z. With p_c = (x_c - x_0) / 3, the reference solves the 24 x 9 least
+// squares min sum_c |M q_c - p_c|^2 by the normal equations; they decouple
+// per row of M, with Q Q^T = 2 I + 2 11^T over the cube's corners, whose
+// inverse is 3/8 on the diagonal and -1/8 off it, so M = (P Q^T)(Q Q^T)^-1
+// in closed form (evaluated in double). out = lpf_param * M M^T.
+__global__ void __launch_bounds__(256) ada_lpf_kernel(int64_t nv, const float* __restrict__ samples,
+                                                      float lpf_param, float* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const float* x = samples + 3 * 64 * v;
+    double PQ[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};  // P Q^T
+    for (int c = 1; c < 8; c++) {
+        const int qx = c & 1, qy = (c >> 1) & 1, qz = (c >> 2) & 1;
+        const float* xc = x + 3 * (48 * qx + 12 * qy + 3 * qz);
+        for (int j = 0; j < 3; j++) {
+            const double pj = (double)((xc[j] - x[j]) / 3.0f);
+            if (qx) PQ[j][0] += pj;
+            if (qy) PQ[j][1] += pj;
+            if (qz) PQ[j][2] += pj;
+        }
+    }
+    double M[3][3];
+    for (int j = 0; j < 3; j++) {
+        const double sum = PQ[j][0] + PQ[j][1] + PQ[j][2];
+        for (int k = 0; k < 3; k++) M[j][k] = 0.5 * PQ[j][k] - 0.125 * sum;
+    }
+    double L[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] = (double)lpf_param * (M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2]);
+    float* o = out + 6 * v;
+    o[0] = (float)L[0][0]; o[1] = (float)L[1][1]; o[2] = (float)L[2][2];
+    o[3] = (float)L[0][1]; o[4] = (float)L[0][2]; o[5] = (float)L[1][2];
 }
 
 __global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
@@ -485,10 +525,17 @@ int launch_samples(const int* valid, int64_t nv, const GridGeom& gg, float* samp
 
 int launch_density(int64_t nv, const int* valid, const int64_t* prefix, const int* list, const float* samples,
                    const float* pos, const float* rot, const float* scale, const float* opacity, const float* sh,
-                   float lpf, float* feat, float* opac, hipStream_t st) {
+                   DensityFilter flt, float* feat, float* opac, hipStream_t st) {
     if (nv == 0) return 0;
     hipLaunchKernelGGL(density_kernel, dim3((unsigned)nv), dim3(64), 0, st, nv, valid, prefix, list, samples, pos, rot,
-                       scale, opacity, sh, lpf, feat, opac);
+                       scale, opacity, sh, flt, feat, opac);
+    HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int launch_ada_lpf(int64_t nv, const float* samples, float lpf_param, float* out, hipStream_t st) {
+    if (nv == 0) return 0;
+    hipLaunchKernelGGL(ada_lpf_kernel, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, samples, lpf_param, out);
     HIP_RET(hipGetLastError());
     return 0;
 }

@@ -311,10 +311,27 @@ int arapgs_set_grid_chunk(arapgs_ctx* ctx, int64_t pairs);
 int arapgs_grid_info(const arapgs_grid* grid, int64_t* n_pairs, int64_t* n_valid, const int64_t** prefix,
                      const int** list, const int** valid, const float** samples);
 
+/* GetAdaLpfRatio (GaussianView.cpp:4670-4749, on by default: GaussianView.hpp:180;
+ * called at F6, :1703-1705): the adaptive low-pass filter of every valid cell.
+ * M is the linear map that best carries the unit cube's corners onto the
+ * cell's 8 corner samples of `samples` (device f32[n_valid*64*3], NULL = the
+ * grid's own samples; taken relative to the cell's first sample and divided by
+ * the 3 sample intervals), in the least-squares sense of the reference's 24 x 9
+ * system, and the cell's filter is lpf_param * M M^T (at rest, lpf_param times
+ * the squared sample interval, times I). While set, arapgs_density_sample,
+ * arapgs_density_backward, the realignment and the chain on this grid add the
+ * cell's filter to each Gaussian's covariance in place of lpf I (their lpf
+ * argument is then unused). lpf_param = 0 switches back to lpf I. */
+int arapgs_grid_set_ada_lpf(arapgs_ctx* ctx, arapgs_grid* grid, const float* samples, float lpf_param);
+/* Device view of the adaptive filters, f32[n_valid*6] (xx, yy, zz, xy, xz, yz
+ * per valid cell), or NULL while the grid uses lpf I. */
+int arapgs_grid_ada_lpf(const arapgs_grid* grid, const float** mats);
+
 /* Density sample of every grid sample (the forward3d_grid pass,
  * GaussianView.cpp:4159-4186; the reference kernel is not available, so the
  * formula is this library's own):
- * a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d) over the cell's list,
+ * a_g(x) = o_g exp(-1/2 d^T (Sigma_g + L)^-1 d) over the cell's list, L = lpf I
+ * or the cell's adaptive filter (arapgs_grid_set_ada_lpf),
  * feat f32[S*48] = sum a_g sh_g, opac f32[S] = sum a_g. */
 int arapgs_density_sample(arapgs_ctx* ctx, const arapgs_grid* grid, const float* pos, const float* rot,
                           const float* scale, const float* opacity, const float* sh, float lpf, float* feat,
@@ -367,10 +384,11 @@ int arapgs_realign_step(arapgs_ctx* ctx, arapgs_realign* r, const float* samples
  *  max_scale  device f32[n*3]: after each step the activated scales are clamped
  *             to it (the max_scale argument of update3d);
  *  conv_tol   per-cell convergence (grid_is_converged of L1loss3d, :4191-4207):
- *             the first step after a re-binning or new options records every
- *             cell's L1 loss, and a cell whose loss later falls to conv_tol times
- *             its record is converged and passes no gradient until the next
- *             re-binning (0: off).
+ *             the first step after creation or new options records every cell's
+ *             L1 loss, and a cell whose loss later falls to conv_tol times its
+ *             record is converged and passes no gradient; the flag is sticky,
+ *             also across re-binnings (the reference clears it only with new
+ *             features, :4154) (0: off).
  * The reported loss always covers every cell. */
 typedef struct arapgs_realign_options {
     const uint8_t* moved;
@@ -378,6 +396,12 @@ typedef struct arapgs_realign_options {
     float conv_tol;
 } arapgs_realign_options;
 int arapgs_realign_set_options(arapgs_ctx* ctx, arapgs_realign* r, const arapgs_realign_options* opts_h);
+/* ExpandOpRange (GaussianView.cpp:4320-4360, run before an F6 pass in high
+ * quality mode, :1699-1701): every converged valid cell that is, or is a face
+ * neighbour (clamped to the grid) of, a non-converged valid cell is no longer
+ * converged, so the next steps optimise a margin around the cells still
+ * moving. Uses a G^3-byte buffer allocated on the first call. */
+int arapgs_realign_expand(arapgs_ctx* ctx, arapgs_realign* r);
 /* Device views of the per-valid-cell state after the last step: skip u8[n_valid]
  * (static or converged) and the cell's L1 loss f32[n_valid]. */
 int arapgs_realign_cells(const arapgs_realign* r, int64_t* n_valid, const uint8_t** skip, const float** cell_loss);

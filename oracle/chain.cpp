@@ -253,6 +253,7 @@ struct orc_chain_args {
     const float* campos;    // V*3
     const float* tanfov;    // V*2
     float pad, coeff;
+    const float* ada;       // nv*6 adaptive filters (xx, yy, zz, xy, xz, yz) or NULL: lpf I
 };
 
 }  // extern "C"
@@ -464,8 +465,9 @@ void render_t(const orc_chain_args* a, int v, const Deformed<T>& D, T* img) {
 
 template <class T>
 void density_t(const orc_chain_args* a, const Deformed<T>& D, T* feat, T* dens) {
+    // Sigma_g + L (L = lpf I, or the cell's adaptive filter when a->ada is set)
     const long N = a->N;
-    std::vector<T> Q(6 * N), mu(3 * N);
+    std::vector<T> Sg(6 * N), mu(3 * N);
     std::vector<char> need(N, 0);
     for (long v = 0; v < a->nv; v++) {
         const int c = a->valid[v];
@@ -484,18 +486,30 @@ void density_t(const orc_chain_args* a, const Deformed<T>& D, T* feat, T* dens) 
             for (int j = 0; j < 3; j++) {
                 T acc = 0.0;
                 for (int k = 0; k < 3; k++) acc += R[i][k] * D.scale[3 * g + k] * D.scale[3 * g + k] * R[j][k];
-                S[i][j] = acc + (i == j ? (double)a->lpf : 0.0);
+                S[i][j] = acc;
             }
-        T I[3][3];
-        inverse3(S, I);
-        Q[6 * g] = I[0][0]; Q[6 * g + 1] = I[1][1]; Q[6 * g + 2] = I[2][2];
-        Q[6 * g + 3] = I[0][1]; Q[6 * g + 4] = I[0][2]; Q[6 * g + 5] = I[1][2];
+        Sg[6 * g] = S[0][0]; Sg[6 * g + 1] = S[1][1]; Sg[6 * g + 2] = S[2][2];
+        Sg[6 * g + 3] = S[0][1]; Sg[6 * g + 4] = S[0][2]; Sg[6 * g + 5] = S[1][2];
         for (int k = 0; k < 3; k++) mu[3 * g + k] = D.pos[3 * g + k];
     }
 #pragma omp parallel for schedule(dynamic, 16)
     for (long v = 0; v < a->nv; v++) {
         const int c = a->valid[v];
         const int beg = c == 0 ? 0 : a->prefix[c - 1], end = a->prefix[c];
+        double L[6] = {(double)a->lpf, (double)a->lpf, (double)a->lpf, 0.0, 0.0, 0.0};
+        if (a->ada)
+            for (int k = 0; k < 6; k++) L[k] = (double)a->ada[6 * v + k];
+        std::vector<T> Qc(6 * (size_t)(end - beg));
+        for (int li = beg; li < end; li++) {
+            const T* sg = &Sg[6 * (size_t)a->list[li]];
+            T S[3][3] = {{sg[0] + L[0], sg[3] + L[3], sg[4] + L[4]},
+                         {sg[3] + L[3], sg[1] + L[1], sg[5] + L[5]},
+                         {sg[4] + L[4], sg[5] + L[5], sg[2] + L[2]}};
+            T I[3][3];
+            inverse3(S, I);
+            T* q = &Qc[6 * (size_t)(li - beg)];
+            q[0] = I[0][0]; q[1] = I[1][1]; q[2] = I[2][2]; q[3] = I[0][1]; q[4] = I[0][2]; q[5] = I[1][2];
+        }
         for (int s = 0; s < 64; s++) {
             const long si = v * 64 + s;
             T acc[48], ao = 0.0;
@@ -504,7 +518,7 @@ void density_t(const orc_chain_args* a, const Deformed<T>& D, T* feat, T* dens) 
                 const int g = a->list[li];
                 const T d0 = D.samples[3 * si] - mu[3 * g], d1 = D.samples[3 * si + 1] - mu[3 * g + 1],
                         d2 = D.samples[3 * si + 2] - mu[3 * g + 2];
-                const T* q = &Q[6 * g];
+                const T* q = &Qc[6 * (size_t)(li - beg)];
                 const T qf = q[0] * d0 * d0 + q[1] * d1 * d1 + q[2] * d2 * d2 +
                              2.0 * (q[3] * d0 * d1 + q[4] * d0 * d2 + q[5] * d1 * d2);
                 const T al = D.opac[g] * exp(-0.5 * qf);
@@ -582,13 +596,15 @@ extern "C" {
 
 // The density sample alone (the formula of arapgs_density_sample) over its own
 // Gaussian attributes and sample positions, with forward-mode derivatives
-// along (dpos, drot, dscale, dopac, dsh, dsamples) (each may be NULL = 0).
+// along (dpos, drot, dscale, dopac, dsh, dsamples) (each may be NULL = 0);
+// ada: per-cell adaptive filters (nv*6) or NULL for lpf I.
 void orc_density_jvp(long nv, const int* valid, const int* prefix, const int* list, long N, const float* samples,
                      const float* pos, const float* rot, const float* scale, const float* opac, const float* sh,
                      float lpf, const double* dpos, const double* drot, const double* dscale, const double* dopac,
                      const double* dsh, const double* dsamples, double* feat, double* dens, double* d_feat,
-                     double* d_dens) {
+                     double* d_dens, const float* ada) {
     orc_chain_args a{};
+    a.ada = ada;
     a.N = N;
     a.nv = nv;
     a.valid = valid;

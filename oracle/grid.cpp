@@ -153,12 +153,13 @@ long orc_grid_lists(const float* aabbs, long N, const float* mn, float step, int
 // Density sample (the un-vendored forward3d_grid, GaussianView.cpp:4159-4186;
 // formula defined here, parity with the reference unpinned): for every sample x
 // of valid cell v and Gaussian g in v's list,
-//   a_g(x) = o_g exp(-1/2 d^T (Sigma_g + lpf I)^-1 d),  d = x - mu_g,
+//   a_g(x) = o_g exp(-1/2 d^T (Sigma_g + L)^-1 d),  d = x - mu_g,
+//   L = lpf I, or ada[6v ..] (xx, yy, zz, xy, xz, yz) when ada is given,
 //   feature(x) = sum_g a_g(x) sh_g (48 floats),  opacity(x) = sum_g a_g(x).
 // Computed in double, stored as float.
 void orc_density_sample(long n_valid, const int* valid, const int* prefix, const int* list, const float* samples,
                         const float* pos, const float* rot, const float* scale, const float* opacity,
-                        const float* sh, float lpf, float* feat, float* opac) {
+                        const float* sh, float lpf, float* feat, float* opac, const float* ada) {
 #pragma omp parallel for schedule(dynamic, 16)
     for (long v = 0; v < n_valid; v++) {
         const int c = valid[v];
@@ -178,8 +179,15 @@ void orc_density_sample(long n_valid, const int* valid, const int* prefix, const
                             const double sk = scale[3 * g + k];
                             a += (double)Rf[i][k] * sk * sk * (double)Rf[j][k];
                         }
-                        S[i][j] = a + (i == j ? (double)lpf : 0.0);
+                        S[i][j] = a + (ada ? 0.0 : (i == j ? (double)lpf : 0.0));
                     }
+                if (ada) {
+                    const float* L = ada + 6 * v;
+                    S[0][0] += L[0]; S[1][1] += L[1]; S[2][2] += L[2];
+                    S[0][1] += L[3]; S[1][0] += L[3];
+                    S[0][2] += L[4]; S[2][0] += L[4];
+                    S[1][2] += L[5]; S[2][1] += L[5];
+                }
                 // inverse of the symmetric 3x3 by cofactors
                 const double c00 = S[1][1] * S[2][2] - S[1][2] * S[2][1];
                 const double c01 = S[1][2] * S[2][0] - S[1][0] * S[2][2];

@@ -83,13 +83,13 @@ def _declare(L):
     L.orc_gaussian_aabbs.argtypes = [P, P, P, P, C.c_long, P]
     L.orc_grid_lists.argtypes = [P, C.c_long, P, C.c_float, C.c_int, C.c_int, P, P, P, P, P]
     L.orc_grid_lists.restype = C.c_long
-    L.orc_density_sample.argtypes = [C.c_long, P, P, P, P, P, P, P, P, P, C.c_float, P, P]
+    L.orc_density_sample.argtypes = [C.c_long, P, P, P, P, P, P, P, P, P, C.c_float, P, P, P]
     L.orc_raster_preprocess.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P, P]
     L.orc_chain_jvp.argtypes = [C.c_void_p] + [P] * 19
     L.orc_chain_jvp.restype = None
     L.orc_chain_forward.argtypes = [C.c_void_p] + [P] * 7
     L.orc_chain_forward.restype = None
-    L.orc_density_jvp.argtypes = [C.c_long, P, P, P, C.c_long, P, P, P, P, P, P, C.c_float] + [P] * 10
+    L.orc_density_jvp.argtypes = [C.c_long, P, P, P, C.c_long, P, P, P, P, P, P, C.c_float] + [P] * 11
     L.orc_density_jvp.restype = None
     L.orc_raster_bin.argtypes = [C.POINTER(RasterArgs), P, P, P, P, P, P]
     L.orc_raster_bin.restype = C.c_long
@@ -313,14 +313,46 @@ def grid_lists(aabbs, mn, step, G, padding=1):
     return prefix, lst, valid, samples
 
 
-def density_sample(valid, prefix, lst, samples, pos, rot, scale, opacity, sh, lpf):
+def density_sample(valid, prefix, lst, samples, pos, rot, scale, opacity, sh, lpf, ada=None):
+    """ada: per-cell adaptive filters (n_valid, 6) as (xx, yy, zz, xy, xz, yz), or None for lpf I."""
     nv = valid.shape[0]
     feat = np.zeros((nv * 64, 48), np.float32)
     opac = np.zeros(nv * 64, np.float32)
+    ada = None if ada is None else _f32(ada)
     lib().orc_density_sample(nv, _p(_i32(valid)), _p(_i32(prefix)), _p(_i32(lst)), _p(_f32(samples)),
                              _p(_f32(pos)), _p(_f32(rot)), _p(_f32(scale)), _p(_f32(opacity)), _p(_f32(sh)),
-                             lpf, _p(feat), _p(opac))
+                             lpf, _p(feat), _p(opac), _p(ada))
     return feat, opac
+
+
+def ada_lpf(samples, lpf_param):
+    """GetAdaLpfRatio (GaussianView.cpp:4670-4749) restated in FP64 with the
+    reference's own construction: per valid cell, the 24 x 9 system A x = b over
+    the 8 cube corners (A(3i+j, 3j+k) = Q(k, i), b(3i+j) = P(j, i), P the corner
+    samples minus the first, divided in float by the 3 intervals), solved by
+    its normal equations; M(i, j) = x(3i + j); returns lpf_param * M M^T
+    packed (n_valid, 6) as (xx, yy, zz, xy, xz, yz)."""
+    x = np.asarray(samples, np.float32).reshape(-1, 64, 3)
+    nv = x.shape[0]
+    out = np.zeros((nv, 6))
+    for v in range(nv):
+        base = x[v, 0]
+        A = np.zeros((24, 9))
+        b = np.zeros(24)
+        row = 0
+        for col in range(8):
+            qv = np.array([col % 2, (col // 2) % 2, (col // 4) % 2], np.float64)
+            idx = 48 * int(qv[0]) + 12 * int(qv[1]) + 3 * int(qv[2])
+            pcol = ((x[v, idx] - base) / np.float32(3.0)).astype(np.float64)
+            for j in range(3):
+                A[row, 3 * j:3 * j + 3] = qv
+                b[row] = pcol[j]
+                row += 1
+        sol = np.linalg.solve(A.T @ A, A.T @ b)
+        M = sol.reshape(3, 3)
+        L = lpf_param * (M @ M.T)
+        out[v] = (L[0, 0], L[1, 1], L[2, 2], L[0, 1], L[0, 2], L[1, 2])
+    return out
 
 
 # --------------------------------------------------------------------------- rasteriser
@@ -359,17 +391,18 @@ class ChainArgs(C.Structure):
                 ("ends_idx", P), ("ends_w", P), ("node_g", P), ("nv", C.c_long), ("valid", P), ("prefix", P),
                 ("list", P), ("samples0", P), ("sample_idx", P), ("sample_w", P), ("lpf", C.c_float),
                 ("V", C.c_int), ("W", C.c_int), ("H", C.c_int), ("D", C.c_int), ("bg", P), ("view", P),
-                ("proj", P), ("campos", P), ("tanfov", P), ("pad", C.c_float), ("coeff", C.c_float)]
+                ("proj", P), ("campos", P), ("tanfov", P), ("pad", C.c_float), ("coeff", C.c_float), ("ada", P)]
 
 
 class Chain:
     """FP64 forward and forward-mode directional derivative of the deform ->
     density-sample -> render chain (oracle/chain.cpp). cams: list of
     (view16, proj16, campos3, tanfovx, tanfovy); grid: (valid, prefix, list,
-    samples0) or None."""
+    samples0) or None; ada: per-cell adaptive filters (n_valid, 6) or None
+    for lpf I."""
 
     def __init__(self, pos0, sh0, opac, ends_idx, ends_w, node_g, cams, W, H, bg, D=3, grid=None,
-                 sample_idx=None, sample_w=None, lpf=0.0, pad=1e-3, coeff=2.0):
+                 sample_idx=None, sample_w=None, lpf=0.0, pad=1e-3, coeff=2.0, ada=None):
         self.N = pos0.shape[0]
         self.M = node_g.shape[0]
         self.K = np.asarray(ends_idx).size // (6 * self.N)
@@ -389,12 +422,13 @@ class Chain:
         else:
             k.update(valid=None, prefix=None, list=None, samples0=None, sample_idx=None, sample_w=None)
         self.nv, self.S = nv, 64 * nv
+        k["ada"] = None if ada is None else _f32(ada)
         self._k = k
         self.args = ChainArgs(self.N, self.K, self.M, _p(k["pos0"]), _p(k["sh0"]), _p(k["opac"]),
                               _p(k["ends_idx"]), _p(k["ends_w"]), _p(k["node_g"]), nv, _p(k["valid"]),
                               _p(k["prefix"]), _p(k["list"]), _p(k["samples0"]), _p(k["sample_idx"]),
                               _p(k["sample_w"]), lpf, self.V, W, H, D, _p(k["bg"]), _p(k["view"]), _p(k["proj"]),
-                              _p(k["campos"]), _p(k["tanfov"]), pad, coeff)
+                              _p(k["campos"]), _p(k["tanfov"]), pad, coeff, _p(k["ada"]))
 
     def jvp(self, R, t, q0, s0, dR=None, dt=None, dq0=None, ds0=None):
         L = lib()
@@ -419,7 +453,7 @@ class Chain:
 
 
 def density_jvp(valid, prefix, lst, samples, pos, rot, scale, opac, sh, lpf, dpos=None, drot=None, dscale=None,
-                dopac=None, dsh=None, dsamples=None):
+                dopac=None, dsh=None, dsamples=None, ada=None):
     """FP64 density sample and its directional derivative along the given
     directions of the Gaussian attributes and sample positions (oracle/chain.cpp)."""
     nv = np.asarray(valid).shape[0]
@@ -432,7 +466,7 @@ def density_jvp(valid, prefix, lst, samples, pos, rot, scale, opac, sh, lpf, dpo
     dirs = [None if d is None else _f64(d) for d in (dpos, drot, dscale, dopac, dsh, dsamples)]
     lib().orc_density_jvp(nv, _p(keep[0]), _p(keep[1]), _p(keep[2]), N, _p(keep[3]), _p(keep[4]), _p(keep[5]),
                           _p(keep[6]), _p(keep[7]), _p(keep[8]), lpf, *[_p(d) for d in dirs], _p(feat), _p(dens),
-                          _p(d_feat), _p(d_dens))
+                          _p(d_feat), _p(d_dens), _p(None if ada is None else _f32(ada)))
     return feat, dens, d_feat, d_dens
 
 

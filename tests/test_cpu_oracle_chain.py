@@ -99,3 +99,38 @@ def test_density_jvp_is_linear_in_opacity_and_sh(sc):
     f1, _, _, _ = O.density_jvp(*args, g.opacity, g.sh + dsh, sc["lpf"])
     assert np.abs(jd).max() == 0.0
     assert np.abs((f1 - f0) - jf).max() <= 1e-5 * np.abs(jf).max()
+
+
+def test_density_jvp_with_adaptive_filter_matches_central_differences(sc):
+    """With per-cell adaptive filters the forward-mode derivative of the density
+    (rotations, scales, positions, samples) equals FP64 central differences.
+    The density takes float32 inputs, so the differences are taken between the
+    float32-rounded points x +- eps d and the derivative along their exact
+    difference."""
+    g = sc["g"]
+    rng = np.random.default_rng(12)
+    nv = len(sc["valid"])
+    ada = np.tile(np.array([sc["lpf"], 1.5 * sc["lpf"], 0.7 * sc["lpf"], 0.2 * sc["lpf"], -0.1 * sc["lpf"],
+                            0.05 * sc["lpf"]], np.float32), (nv, 1))
+    args = (sc["valid"], sc["prefix"], sc["list"])
+    base = dict(samples=sc["samples"], pos=g.pos, rot=g.rot, scale=g.scale)
+    raw = dict(samples=rng.normal(size=sc["samples"].shape) * 1e-2, pos=rng.normal(size=g.pos.shape) * 1e-2,
+               rot=rng.normal(size=g.rot.shape), scale=rng.normal(size=g.scale.shape) * g.scale)
+    eps = 1e-3
+    plus = {k: (base[k] + eps * raw[k]).astype(np.float32) for k in base}
+    minus = {k: (base[k] - eps * raw[k]).astype(np.float32) for k in base}
+    mid = {k: ((plus[k].astype(np.float64) + minus[k]) / 2).astype(np.float32) for k in base}
+    dirs = {k: (plus[k].astype(np.float64) - minus[k]) / (2 * eps) for k in base}
+    gF = rng.normal(size=(64 * nv, 48))
+    gD = rng.normal(size=64 * nv)
+    _, _, jf, jd = O.density_jvp(*args, mid["samples"], mid["pos"], mid["rot"], mid["scale"], g.opacity, g.sh,
+                                 sc["lpf"], ada=ada, dpos=dirs["pos"], drot=dirs["rot"], dscale=dirs["scale"],
+                                 dsamples=dirs["samples"])
+    jv = np.sum(gF * jf) + np.sum(gD * jd)
+    outs = []
+    for pt in (plus, minus):
+        f, dn, _, _ = O.density_jvp(*args, pt["samples"], pt["pos"], pt["rot"], pt["scale"], g.opacity, g.sh,
+                                    sc["lpf"], ada=ada)
+        outs.append(np.sum(gF * f) + np.sum(gD * dn))
+    fd = (outs[0] - outs[1]) / (2 * eps)
+    assert abs(jv - fd) <= 1e-4 * max(abs(fd), 1.0), (jv, fd)

@@ -146,3 +146,97 @@ def test_density_sample_matches_its_definition():
             a[:, j] = opac[g] * np.exp(-0.5 * np.einsum("ni,ij,nj->n", d, np.linalg.inv(Sig), d))
         np.testing.assert_allclose(op[v * 64:(v + 1) * 64], a.sum(1), rtol=1e-5, atol=1e-7)
         np.testing.assert_allclose(feat[v * 64:(v + 1) * 64], a @ sh[gs].astype(np.float64), rtol=1e-4, atol=1e-6)
+
+
+def _lattice(rng, nv, interval):
+    """nv cells of 4x4x4 samples in the grid's order (sample x*16 + y*4 + z)."""
+    ijk = np.array([(x, y, z) for x in range(4) for y in range(4) for z in range(4)], np.float64) * interval
+    base = rng.uniform(-1, 1, size=(nv, 1, 3))
+    return ijk[None], base
+
+
+def test_ada_lpf_recovers_affine_maps():
+    """GetAdaLpfRatio (GaussianView.cpp:4670-4749): the cell's filter is
+    lpf_param * M M^T with M the linear map carrying the unit cube's corners to
+    the corner samples over the 3 sample intervals. For samples that are an
+    exact affine image T u + c of the lattice u, M = interval * T (closed form);
+    at rest (the grid's own samples) the filter is lpf_param * interval^2 * I."""
+    rng = np.random.default_rng(7)
+    interval, lpf_param = 0.05, 0.2
+    u, base = _lattice(rng, 12, interval)
+    T = np.eye(3)[None] + 0.3 * rng.normal(size=(12, 3, 3))
+    x = (np.einsum("vij,vnj->vni", T, np.broadcast_to(u, (12, 64, 3))) + base).astype(np.float32)
+    got = O.ada_lpf(x, lpf_param)
+    for v in range(12):
+        Mv = interval * T[v]
+        L = lpf_param * Mv @ Mv.T
+        want = np.array([L[0, 0], L[1, 1], L[2, 2], L[0, 1], L[0, 2], L[1, 2]])
+        # the corner differences are float32 samples: relative rounding of a few 1e-7 per entry
+        assert np.abs(got[v] - want).max() <= 2e-5 * np.abs(want).max(), (v, got[v], want)
+    pos, rot, scale, opac = scene(5, N=200)
+    G = 8
+    mn, _, step = O.scene_aabb(pos, G)
+    _, _, valid, samples = O.grid_lists(O.gaussian_aabbs(pos, rot, scale, opac), mn, step, G, padding=1)
+    rest = O.ada_lpf(samples, lpf_param)
+    spacing = np.float64(np.float32(step) / np.float32(4.0))
+    want = lpf_param * spacing ** 2
+    assert np.abs(rest[:, :3] - want).max() <= 1e-5 * want
+    assert np.abs(rest[:, 3:]).max() <= 1e-6 * want
+
+
+def test_ada_lpf_is_the_per_row_least_squares_fit():
+    """For corners that no linear map fits, each row m_j of M minimises
+    sum_c (m_j . q_c - p_jc)^2 over the 8 corners: numpy's lstsq on the 8 x 3
+    corner matrix (a different route than the reference's 24 x 9 normal
+    equations) gives the same M."""
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(6, 64, 3)).astype(np.float32)
+    got = O.ada_lpf(x, 1.0)
+    Qc = np.array([[c % 2, (c // 2) % 2, (c // 4) % 2] for c in range(8)], np.float64)
+    for v in range(6):
+        idx = [48 * q[0] + 12 * q[1] + 3 * q[2] for q in Qc.astype(int)]
+        P = ((x[v, idx] - x[v, 0]) / np.float32(3.0)).astype(np.float64)  # 8 x 3
+        M = np.linalg.lstsq(Qc, P, rcond=None)[0].T  # rows m_j
+        L = M @ M.T
+        want = np.array([L[0, 0], L[1, 1], L[2, 2], L[0, 1], L[0, 2], L[1, 2]])
+        assert np.abs(got[v] - want).max() <= 1e-9 * np.abs(want).max()
+
+
+def test_density_with_isotropic_ada_equals_scalar_lpf():
+    """A per-cell filter of lpf I is the scalar filter: identical outputs; a
+    per-cell anisotropic filter changes each cell by its own matrix only
+    (brute force on a few cells)."""
+    pos, rot, scale, opac = scene(6, N=120)
+    sh = np.random.default_rng(6).normal(size=(pos.shape[0], 48)).astype(np.float32)
+    G = 8
+    mn, _, step = O.scene_aabb(pos, G)
+    box = O.gaussian_aabbs(pos, rot, scale, opac)
+    prefix, lst, valid, samples = O.grid_lists(box, mn, step, G, padding=1)
+    lpf = float(np.float32((step / 4) ** 2 * 0.2))
+    iso = np.zeros((len(valid), 6), np.float32)
+    iso[:, :3] = lpf
+    f0, o0 = O.density_sample(valid, prefix, lst, samples, pos, rot, scale, opac, sh, lpf)
+    f1, o1 = O.density_sample(valid, prefix, lst, samples, pos, rot, scale, opac, sh, 123.0, ada=iso)
+    np.testing.assert_array_equal(f0, f1)
+    np.testing.assert_array_equal(o0, o1)
+    rng = np.random.default_rng(9)
+    A = rng.normal(size=(len(valid), 3, 3)) * np.sqrt(lpf)
+    Lm = np.einsum("vij,vkj->vik", A, A) + lpf * np.eye(3)[None]
+    ada = np.stack([Lm[:, 0, 0], Lm[:, 1, 1], Lm[:, 2, 2], Lm[:, 0, 1], Lm[:, 0, 2], Lm[:, 1, 2]], 1)
+    ada = ada.astype(np.float32)
+    feat, op = O.density_sample(valid, prefix, lst, samples, pos, rot, scale, opac, sh, 0.0, ada=ada)
+    for v in range(0, len(valid), max(1, len(valid) // 8)):
+        c = valid[v]
+        s = 0 if c == 0 else prefix[c - 1]
+        gs = lst[s:prefix[c]]
+        x = samples[v * 64:(v + 1) * 64].astype(np.float64)
+        L = ada[v].astype(np.float64)
+        Lv = np.array([[L[0], L[3], L[4]], [L[3], L[1], L[5]], [L[4], L[5], L[2]]])
+        a = np.zeros((64, len(gs)))
+        for j, g in enumerate(gs):
+            R = quat_mat(rot[g].astype(np.float64))
+            Sig = R @ np.diag(scale[g].astype(np.float64) ** 2) @ R.T + Lv
+            d = x - pos[g]
+            a[:, j] = opac[g] * np.exp(-0.5 * np.einsum("ni,ij,nj->n", d, np.linalg.inv(Sig), d))
+        np.testing.assert_allclose(op[v * 64:(v + 1) * 64], a.sum(1), rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(feat[v * 64:(v + 1) * 64], a @ sh[gs].astype(np.float64), rtol=1e-4, atol=1e-6)

@@ -214,3 +214,68 @@ def test_chain_autograd_function(setup):
         losses.append(float(lo))
     print(f"autograd fit: {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
     assert losses[-1] < 0.8 * losses[0], losses
+
+
+def test_chain_with_adaptive_filter(ctx):
+    """The chain over a grid with per-cell adaptive filters
+    (arapgs_grid_set_ada_lpf): forward density = the FP64 chain with the same
+    filters, and the backward satisfies the adjoint identity along 16
+    directions (density cotangents: the only outputs the filter changes)."""
+    import torch
+    sc = chain_scene(seed=2, n=2500, n_nodes=24, K=8, W=64, H=48, n_views=1, G=12, log_scale=-3.4)
+    g = sc["g"]
+    a7 = ctx.scene_aabb(T(g.pos), 12)
+    grid = ctx.grid_build(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)), a7, 12, padding=1)
+    prefix, lst, valid, samples = grid.arrays()
+    x = samples.astype(np.float64)
+    warped = (x + 0.1 * np.sin(4.0 * x[:, [2, 0, 1]])).astype(np.float32)
+    grid.set_ada_lpf(0.3, samples=T(warped))
+    mats = grid.ada_lpf()
+    sidx, sw = O.knn_weights(sc["nodes"], samples, sc["K"])
+    sidx = np.ascontiguousarray(sidx[:, :sc["K"]])
+    sw = sw.astype(np.float32)
+    cams = [dict(view=T(c.view), proj=T(c.proj), campos=T(c.campos), tx=c.tanfovx, ty=c.tanfovy) for c in sc["cams"]]
+    M = sc["nodes"].shape[0]
+    table = np.zeros((M, 16), np.float32)
+    table[:, 0:3] = sc["nodes"]
+    table[:, 3:12] = sc["R"]
+    table[:, 12:15] = sc["t"]
+    dev = dict(pos0=T(g.pos), sh0=T(g.sh), opac=T(g.opacity), ends_idx=T(sc["ends_idx"]),
+               ends_w=T(sc["ends_w"].astype(np.float32)), sidx=T(sidx), sw=T(sw), bg=T(sc["bg"]))
+    chain = arapgs.Chain(ctx, M, dev["pos0"], dev["sh0"], dev["opac"], dev["ends_idx"], dev["ends_w"], cams, sc["W"],
+                         sc["H"], dev["bg"], grid=grid, sample_idx=dev["sidx"], sample_w=dev["sw"], lpf=sc["lpf"])
+    ocams = [(c.view, c.proj, c.campos, c.tanfovx, c.tanfovy) for c in sc["cams"]]
+    ora = O.Chain(g.pos, g.sh, g.opacity, sc["ends_idx"], sc["ends_w"], sc["nodes"], ocams, sc["W"], sc["H"], sc["bg"],
+                  grid=(valid, prefix, lst, samples), sample_idx=sidx, sample_w=sw, lpf=sc["lpf"], ada=mats)
+    params = (table[:, 3:12].astype(np.float64), table[:, 12:15].astype(np.float64), g.rot.astype(np.float64),
+              g.scale.astype(np.float64))
+    _, feat, dens = chain.forward(T(table), T(g.rot), T(g.scale))
+    ref = ora.jvp(*params)
+    assert np.abs(H(feat) - ref["feat"]).max() <= 1e-3 * max(1.0, np.abs(ref["feat"]).max())
+    assert np.abs(H(dens) - ref["dens"]).max() <= 1e-3 * max(1.0, ref["dens"].max())
+    rng = np.random.default_rng(23)
+    S = chain.S
+    gF = rng.normal(size=(S, 48)).astype(np.float32)
+    gD = rng.normal(size=S).astype(np.float32)
+    dt, dq, ds = chain.backward(T(table), T(g.rot), T(g.scale), dimages=None, dfeat=T(gF), ddens=T(gD))
+    torch.cuda.synchronize()
+    dt, dq, ds = (H(a).astype(np.float64) for a in (dt, dq, ds))
+    R, t, q0, s0 = params
+    errs = []
+    for k in range(16):
+        v = [rng.normal(size=R.shape), rng.normal(size=t.shape), rng.normal(size=q0.shape),
+             rng.normal(size=s0.shape) * s0]
+        block = k % 5
+        v = [x if (block == 4 or i == block) else None for i, x in enumerate(v)]
+        out = ora.jvp(R, t, q0, s0, *v)
+        want = float(np.sum(gF * out["d_feat"]) + np.sum(gD * out["d_dens"]))
+        got = 0.0
+        if v[0] is not None: got += float(np.sum(dt[:, 3:12] * v[0]))
+        if v[1] is not None: got += float(np.sum(dt[:, 12:15] * v[1]))
+        if v[2] is not None: got += float(np.sum(dq * v[2]))
+        if v[3] is not None: got += float(np.sum(ds * v[3]))
+        errs.append(abs(got - want) / abs(want))
+    print(f"chain with adaptive filters: adjoint identity max rel err {max(errs):.3e}", flush=True)
+    assert max(errs) < 1e-3, errs
+    chain.close()
+    grid.close()

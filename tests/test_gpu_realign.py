@@ -242,3 +242,111 @@ def test_realign_loop_with_rebinning(ctx, dsc):
     assert losses[-1] < 0.9 * losses[0]
     r.close()
     grid.close()
+
+
+def test_adaptive_filter(ctx, dsc):
+    """arapgs_grid_set_ada_lpf (GetAdaLpfRatio, GaussianView.cpp:4670-4749) from
+    warped samples equals the oracle's restatement of the reference's 24 x 9
+    least squares; with the filters set, the density equals the oracle's with
+    the same per-cell filters (1e-4 of the largest value), its VJP satisfies the
+    adjoint identity against the FP64 forward-mode oracle (1e-3, 16 directions),
+    and switching the filters off gives the scalar-filter density bit for bit."""
+    import torch
+    g = dsc["g"]
+    a7 = ctx.scene_aabb(T(g.pos), 12)
+    grid = ctx.grid_build(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)), a7, 12, padding=1)
+    x = dsc["samples"].astype(np.float64)
+    warped = (x + 0.15 * np.sin(3.0 * x[:, [1, 2, 0]]) + 0.05 * x[:, [2, 0, 1]]).astype(np.float32)
+    grid.set_ada_lpf(0.2, samples=T(warped))
+    mats = grid.ada_lpf()
+    ref = O.ada_lpf(warped, 0.2)
+    assert mats.shape == ref.shape
+    assert np.abs(mats - ref).max() <= 1e-5 * np.abs(ref).max(), np.abs(mats - ref).max()
+    feat, dens = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), 123.0)  # lpf unused
+    rf, rd = O.density_sample(dsc["valid"], dsc["prefix"], dsc["list"], dsc["samples"], g.pos, g.rot, g.scale,
+                              g.opacity, g.sh, 0.0, ada=mats)
+    assert np.abs(H(dens) - rd).max() <= 1e-4 * max(1.0, np.abs(rd).max())
+    assert np.abs(H(feat) - rf).max() <= 1e-4 * max(1.0, np.abs(rf).max())
+    rng = np.random.default_rng(17)
+    S = grid.n_valid * 64
+    gF = rng.normal(size=(S, 48)).astype(np.float32)
+    gD = rng.normal(size=S).astype(np.float32)
+    grads = [H(t).astype(np.float64) for t in grid.density_backward(
+        T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), 123.0, dfeat=T(gF), dopac=T(gD), samples=T(warped),
+        want_samples=True)]
+    errs = []
+    for k in range(16):
+        v = [rng.normal(size=g.pos.shape) * 0.01, rng.normal(size=g.rot.shape), rng.normal(size=g.scale.shape) * g.scale,
+             rng.normal(size=g.opacity.shape), rng.normal(size=g.sh.shape), rng.normal(size=warped.shape) * 0.01]
+        which = k % 7
+        v = [x if (which == 6 or i == which) else None for i, x in enumerate(v)]
+        _, _, jf, jd = O.density_jvp(dsc["valid"], dsc["prefix"], dsc["list"], warped, g.pos, g.rot, g.scale,
+                                     g.opacity, g.sh, 0.0, *v, ada=mats)
+        want = float(np.sum(gF * jf) + np.sum(gD * jd))
+        got = sum(float(np.sum(gr * x)) for gr, x in zip(grads, v) if x is not None)
+        errs.append(abs(got - want) / abs(want))
+    print(f"adaptive-filter density VJP: max rel err {max(errs):.3e}", flush=True)
+    assert max(errs) < 1e-3, errs
+    grid.set_ada_lpf(0.0)
+    assert grid.ada_lpf() is None
+    f0, d0 = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    f1, d1 = dsc["grid"].density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    torch.cuda.synchronize()
+    assert torch.equal(f0, f1) and torch.equal(d0, d1)
+    grid.close()
+
+
+def test_convergence_survives_rebinning_and_expand_op_range(ctx, dsc):
+    """Converged flags stay set across a re-binning (the reference's
+    grid_is_converged is cleared only with new features), and
+    arapgs_realign_expand (ExpandOpRange, GaussianView.cpp:4320-4360) clears
+    exactly the converged valid cells that are, or are face neighbours (clamped
+    to the grid) of, a non-converged valid cell (host brute force)."""
+    import torch
+    g = dsc["g"]
+    G = 12
+    a7 = ctx.scene_aabb(T(g.pos), G)
+    grid = ctx.grid_build(ctx.gaussian_aabbs(T(g.pos), T(g.rot), T(g.scale), T(g.opacity)), a7, G, padding=1)
+    samples = grid.samples_tensor()
+    feat_aim, dens_aim = grid.density(T(g.pos), T(g.rot), T(g.scale), T(g.opacity), T(g.sh), dsc["lpf"])
+    rng = np.random.default_rng(21)
+    pert = g.take(np.arange(g.pos.shape[0]))
+    pert.pos = (g.pos + 0.01 * rng.normal(size=g.pos.shape)).astype(np.float32)
+    # the per-cell loss ratio of the second step to the first, from a dry run
+    pos, rot, ls, lo, sh = _params(pert)
+    r = arapgs.Realign(ctx, grid, g.pos.shape[0])
+    r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+    _, cl1 = r.cells()
+    r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+    _, cl2 = r.cells()
+    r.close()
+    ratio = cl2[cl1 > 0] / cl1[cl1 > 0]
+    tol = float(np.median(ratio))  # about half the cells converge at the second step
+    pos, rot, ls, lo, sh = _params(pert)
+    r = arapgs.Realign(ctx, grid, g.pos.shape[0])
+    r.set_options(conv_tol=tol)
+    r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)  # records
+    r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+    conv, _ = r.cells()
+    assert 0 < conv.sum() < conv.size, (conv.sum(), conv.size)
+    grid.rebin(pos, rot, torch.exp(ls), torch.sigmoid(lo))
+    r.step(samples, feat_aim, dens_aim, dsc["lpf"], pos, rot, ls, lo, sh)
+    after_rebin, _ = r.cells()
+    assert np.all(after_rebin[conv == 1] == 1)
+    conv = after_rebin.copy()
+    r.expand()
+    expanded, _ = r.cells()
+    _, valid = grid.prefix_valid()
+    mark = np.zeros(G ** 3, bool)
+    for v, c in enumerate(valid):
+        if conv[v]:
+            continue
+        x, y, z = c // (G * G), (c // G) % G, c % G
+        for dx, dy, dz in ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)):
+            xx, yy, zz = min(max(x + dx, 0), G - 1), min(max(y + dy, 0), G - 1), min(max(z + dz, 0), G - 1)
+            mark[xx * G * G + yy * G + zz] = True
+    want = np.where(mark[valid], 0, conv).astype(np.uint8)
+    assert np.array_equal(expanded, want)
+    assert (expanded < conv).any()  # some converged cell was reopened
+    r.close()
+    grid.close()
