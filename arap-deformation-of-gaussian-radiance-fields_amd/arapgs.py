@@ -93,6 +93,8 @@ EXPORTS = {
     "arapgs_solver_batch_set_async_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_block": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_async_refresh": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_cg_graphs": ([P, C.c_int], C.c_int),
+    "arapgs_solver_batch_set_cg_graphs": ([P, C.c_int], C.c_int),
     "arapgs_solver_batch_solve": ([P, P, P, P, C.c_int, C.c_int, P, P, P, P], C.c_int),
     "arapgs_blend_points": ([P, P, C.c_int64, P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_six_point_update": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float, P, P, P, P],
@@ -631,6 +633,10 @@ class Solver:
         """Coarse rebuilds on the solver's side stream, adopted by the next solve (default on)."""
         self.ctx._check(lib().arapgs_solver_set_async_refresh(self.h, int(bool(on))))
 
+    def set_cg_graphs(self, on=True):
+        """CG chunks between host checks as captured HIP graphs, checked one chunk behind."""
+        self.ctx._check(lib().arapgs_solver_set_cg_graphs(self.h, int(bool(on))))
+
     def apply(self, node_pos, aims, x, v):
         """(J^T J v, -J^T f) at unknowns x (test hook)."""
         t = self.ctx._torch
@@ -713,6 +719,12 @@ class SolverBatch:
 
     def set_coarse_refresh(self, frames):
         self.ctx._check(lib().arapgs_solver_batch_set_coarse_refresh(self.h, frames))
+
+    def set_cg_graphs(self, on=True):
+        self.ctx._check(lib().arapgs_solver_batch_set_cg_graphs(self.h, int(bool(on))))
+
+    def set_async_refresh(self, on=True):
+        self.ctx._check(lib().arapgs_solver_batch_set_async_refresh(self.h, int(bool(on))))
 
     def solve(self, node_pos, aims, iters, stop_rule=True, x_out=None, energies=None, iters_done=None, tables=None):
         """Lists (one entry per scene) of device tensors; outputs default to new

@@ -9,6 +9,8 @@ its tolerance:
   host_checks   CG iterations between host reads of the convergence flag
   warm          CG warm start from the previous frame's GN steps (1) or from 0
   cg_block      workgroup size of the CG operator kernel: 256, 128 or 64 threads
+  cg_graphs     CG chunks between host checks as captured HIP graphs, checked
+                one chunk behind (1) or launched kernel by kernel (0)
 
 `tune` runs a coordinate descent from the defaults: for each parameter in turn
 it times every candidate value (the others held at their current best) with a
@@ -20,10 +22,12 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256)
-CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(4, 8, 16, 32),
-                  host_checks=(4, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64))
-ORDER = ("warm", "aggregates", "refresh", "host_checks", "cg_block", "tile", "blend_unroll")
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0)
+CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
+                  host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64), cg_graphs=(0, 1))
+# host_checks is revisited after cg_graphs: the graphs' best chunk is shorter
+ORDER = ("warm", "aggregates", "refresh", "host_checks", "cg_graphs", "host_checks", "cg_block", "tile",
+         "blend_unroll")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

@@ -64,7 +64,9 @@ struct arapgs_ctx {
     RasterView* r_views_h = nullptr;  // pinned host staging for it
     int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
     int* r_seg_h = nullptr;           // pinned host staging for it
-    int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (4 ints)
+    int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (8 ints;
+                                      // [4], [5]: the flags of consecutive CG chunk graphs)
+    hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
@@ -160,7 +162,9 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMalloc(&c->r_seg, sizeof(int) * (RASTER_MAX_VIEWS + 1))) != hipSuccess ||
         (e = hipHostMalloc(&c->r_seg_h, sizeof(int) * (RASTER_MAX_VIEWS + 1), hipHostMallocDefault)) != hipSuccess ||
-        (e = hipHostMalloc(&c->flag_h, sizeof(int) * 4, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->flag_h, sizeof(int) * 8, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->chk_ev[0], hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->chk_ev[1], hipEventDisableTiming)) != hipSuccess ||
         (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
         arapgs_destroy(c);
         return -5;
@@ -190,6 +194,8 @@ void arapgs_destroy(arapgs_ctx* c) {
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
     if (c->r_seg_h) (void)hipHostFree(c->r_seg_h);
     if (c->flag_h) (void)hipHostFree(c->flag_h);
+    for (hipEvent_t ev : c->chk_ev)
+        if (ev) (void)hipEventDestroy(ev);
     delete c;
 }
 
@@ -456,6 +462,12 @@ int arapgs_solver_set_cg_block(arapgs_solver* s, int threads) {
     return solver_batch_init(s->one, {&s->w}, err);
 }
 
+int arapgs_solver_set_cg_graphs(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    s->one.graphs = on != 0;
+    return 0;
+}
+
 int arapgs_solver_set_async_refresh(arapgs_solver* s, int on) {
     if (!s || on < 0 || on > 1) return -1;
     s->one.async = on != 0;
@@ -512,13 +524,32 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
             // every scene's own cap is enforced on the device (cg_done); loop to the largest
             int cap = 0;
             for (const SolverWork* w : B.w) cap = std::max(cap, w->cg_max_iters);
-            for (int k = 0; k < cap; k++) {
-                rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
-                if (rc) return rc;
-                if (every > 0 && (k + 1) % every == 0) {
-                    int done = 0;
-                    if ((rc = host_flag(0, &done))) return rc;
-                    if (done) break;
+            if (B.graphs && every > 0 && c->stream) {
+                // chunk j is one graph launch; the host reads chunk j-1's flag
+                // while chunk j is already queued, so the stream never waits for
+                // the host (a chunk queued past convergence exits kernel by kernel)
+                int* const slots[2] = {c->flag_h + 4, c->flag_h + 5};
+                const int nch = (cap + every - 1) / every;
+                for (int j = 0; j < nch; j++) {
+                    rc = check_launch(c, solver_cg_chunk(B, j, every, cap, c->stream, slots), "gn_solve cg graph");
+                    if (rc) return rc;
+                    hipError_t e = hipEventRecord(c->chk_ev[j & 1], c->stream);
+                    if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg graph");
+                    if (j > 0) {
+                        e = hipEventSynchronize(c->chk_ev[(j - 1) & 1]);
+                        if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg graph");
+                        if (*(volatile int*)slots[(j - 1) & 1]) break;
+                    }
+                }
+            } else {
+                for (int k = 0; k < cap; k++) {
+                    rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
+                    if (rc) return rc;
+                    if (every > 0 && (k + 1) % every == 0) {
+                        int done = 0;
+                        if ((rc = host_flag(0, &done))) return rc;
+                        if (done) break;
+                    }
                 }
             }
         }
@@ -604,6 +635,12 @@ int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every) {
 int arapgs_solver_batch_set_async_refresh(arapgs_solver_batch* b, int on) {
     if (!b || on < 0 || on > 1) return -1;
     b->b.async = on != 0;
+    return 0;
+}
+
+int arapgs_solver_batch_set_cg_graphs(arapgs_solver_batch* b, int on) {
+    if (!b || on < 0 || on > 1) return -1;
+    b->b.graphs = on != 0;
     return 0;
 }
 

@@ -373,6 +373,54 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) s
     rows48_store(feat + s0 * 48, n_here, lds, 256 * 18, c);
 }
 
+// Unit quaternion by the hardware reciprocal square root and one Newton step
+// (within an ulp or two of q / |q|; the slerp runs it after every neighbour).
+__device__ __forceinline__ void quat_normalize_rsq(float q[4]) {
+    const float n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (!(n2 > 0.0f)) return;
+    float r = __builtin_amdgcn_rsqf(n2);
+    r = r * (1.5f - 0.5f * n2 * r * r);
+#pragma unroll
+    for (int c = 0; c < 4; c++) q[c] *= r;
+}
+
+// One step of the running slerp of Q_SlerpCUDA (cudakdtree.cu:113-148): acc
+// moves towards e (flipped into acc's hemisphere) by t = cw / (cw + last).
+// The first neighbour (last = 0, t = 1) lands on e itself. Otherwise, with
+// cos(theta) = cs and s = sin(t theta) / sin(theta), the slerp weights are
+// (cos(t theta) - cs s, s): one sincos and one reciprocal instead of two sines
+// and two divisions (angle-addition formula; equal to the reference's FP64
+// weights within a few float ulps).
+__device__ __forceinline__ void slerp_step(float (&acc)[4], float4 e4, float cw, float last, bool first) {
+    float e[4] = {e4.x, e4.y, e4.z, e4.w};
+    float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
+    if (cs < 0.0f) {
+        e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3];
+        cs = -cs;
+    }
+    if (first) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] = e[c];
+    } else {
+        const float t = cw / (cw + last);
+        float ra, rb;
+        if (cs > 0.99995f) {
+            ra = 1.0f - t;
+            rb = t;
+        } else {
+            const float sn = __builtin_sqrtf(1.0f - cs * cs);
+            const float ang = atan2f(sn, cs);
+            float st, ct;
+            sincosf(t * ang, &st, &ct);
+            rb = st * (1.0f / sn);
+            ra = ct - cs * rb;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] = ra * acc[c] + rb * e[c];
+    }
+    quat_normalize_rsq(acc);
+}
+
 template <int K>
 __device__ __forceinline__ void sample_rotation(int64_t s, const float* __restrict__ w, const uint32_t* __restrict__ idx,
                                                 const float* __restrict__ node_q, float (&R)[3][3]) {
@@ -381,30 +429,10 @@ __device__ __forceinline__ void sample_rotation(int64_t s, const float* __restri
 #pragma unroll
     for (int j = 0; j < K; j++) {
         const float cw = w[s * K + j];
-        const float t = cw / (cw + last);
         const float4 e4 = *reinterpret_cast<const float4*>(node_q + 4 * (int64_t)idx[s * K + j]);
-        float e[4] = {e4.x, e4.y, e4.z, e4.w};
-        float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
-        if (cs < 0.0f) {
-            e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3];
-            cs = -cs;
-        }
-        float ra, rb;
-        if (cs > 0.99995f) {
-            ra = 1.0f - t;
-            rb = t;
-        } else {
-            const float sn = __builtin_sqrtf(1.0f - cs * cs);
-            const float ang = atan2f(sn, cs);
-            ra = sinf((1.0f - t) * ang) / sn;
-            rb = sinf(t * ang) / sn;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[c] = ra * acc[c] + rb * e[c];
-        quat_normalize(acc);
+        slerp_step(acc, e4, cw, last, j == 0);
         last += cw;
     }
-    quat_normalize(acc);
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
 }
 
@@ -418,30 +446,10 @@ __device__ __forceinline__ void sample_rotation_regs(const uint32_t (&id)[K], co
 #pragma unroll
     for (int j = 0; j < K; j++) {
         const float cw = wt[j];
-        const float t = cw / (cw + last);
         const float4 e4 = *reinterpret_cast<const float4*>(node_q + 4 * (int64_t)id[j]);
-        float e[4] = {e4.x, e4.y, e4.z, e4.w};
-        float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
-        if (cs < 0.0f) {
-            e[0] = -e[0]; e[1] = -e[1]; e[2] = -e[2]; e[3] = -e[3];
-            cs = -cs;
-        }
-        float ra, rb;
-        if (cs > 0.99995f) {
-            ra = 1.0f - t;
-            rb = t;
-        } else {
-            const float sn = __builtin_sqrtf(1.0f - cs * cs);
-            const float ang = atan2f(sn, cs);
-            ra = sinf((1.0f - t) * ang) / sn;
-            rb = sinf(t * ang) / sn;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[c] = ra * acc[c] + rb * e[c];
-        quat_normalize(acc);
+        slerp_step(acc, e4, cw, last, j == 0);
         last += cw;
     }
-    quat_normalize(acc);
     quat_to_mat(acc[0], acc[1], acc[2], acc[3], R);
 }
 

@@ -1629,7 +1629,14 @@ static SolverEntry build_entry_of(const SolverWork& W) {
     return E;
 }
 
+void solver_batch_drop_graphs(SolverBatch& B) {
+    for (hipGraphExec_t& x : B.cg_graphs)
+        if (x) (void)hipGraphExecDestroy(x);
+    B.cg_graphs.clear();
+}
+
 void solver_batch_free(SolverBatch& B) {
+    solver_batch_drop_graphs(B);
     if (B.entries) (void)hipFree(B.entries);
     if (B.entries_build) (void)hipFree(B.entries_build);
     if (B.fork_ev) (void)hipEventDestroy(B.fork_ev);
@@ -1642,12 +1649,16 @@ void solver_batch_free(SolverBatch& B) {
 }
 
 int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::string& err) {
+    // settings survive a rebuild of the entries
+    const bool async = B.async, graphs = B.graphs;
     solver_batch_free(B);
     if (w.empty() || (int)w.size() > SOLVER_BATCH_MAX) {
         err = "solver batch: need 1.." + std::to_string(SOLVER_BATCH_MAX) + " solvers";
         return -1;
     }
     B = SolverBatch();
+    B.async = async;
+    B.graphs = graphs;
     B.w = w;
     B.kt = 8;
     std::vector<SolverEntry> host;
@@ -1715,6 +1726,36 @@ int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h) {
     hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
                        which, flag_h);
     HIP_RET(hipGetLastError());
+    return 0;
+}
+
+int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, int* const flag_h[2]) {
+    if (B.graph_every != every || B.graph_cap != cap || B.graph_flag[0] != flag_h[0] || B.graph_flag[1] != flag_h[1]) {
+        solver_batch_drop_graphs(B);
+        B.graph_every = every;
+        B.graph_cap = cap;
+        B.graph_flag[0] = flag_h[0];
+        B.graph_flag[1] = flag_h[1];
+    }
+    if ((int)B.cg_graphs.size() <= j) B.cg_graphs.resize(j + 1, nullptr);
+    if (!B.cg_graphs[j]) {
+        HIP_RET(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (int k = j * every; k < std::min(cap, (j + 1) * every) && !rc; k++) rc = solver_cg_iteration(B, k, st);
+        if (!rc) rc = solver_batch_flag(B, 0, st, flag_h[j & 1]);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : -(int)e - 1000;
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_RET(ei);
+        B.cg_graphs[j] = x;
+    }
+    HIP_RET(hipGraphLaunch(B.cg_graphs[j], st));
     return 0;
 }
 

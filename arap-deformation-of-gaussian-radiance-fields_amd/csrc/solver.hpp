@@ -155,6 +155,12 @@ struct SolverBatch {
     hipEvent_t fork_ev = nullptr;
     bool async = true;                // coarse refreshes on the side stream (the first one in line)
     std::vector<const float*> g, aim; // what the device entries hold
+    // CG chunks replayed as captured HIP graphs: chunk j holds CG iterations
+    // [j*every, min(cap, (j+1)*every)) and the flag kernel writing *flag
+    bool graphs = false;
+    int graph_every = 0, graph_cap = 0;
+    const int* graph_flag[2] = {nullptr, nullptr};
+    std::vector<hipGraphExec_t> cg_graphs;
     int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
     int max_nb_owner = 0, max_nb_apply = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
         max_multi = 0, max_nt = 0, max_cand = 0;
@@ -167,6 +173,11 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
 // *flag_h (pinned host memory) = AND over the scenes of cg_done (which 0) or
 // gn_done (which 1), written by a kernel queued on st.
 int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h);
+// Queues CG chunk j (iterations [j*every, min(cap, (j+1)*every)) followed by the
+// cg_done flag kernel writing flag_h[j & 1]) as one graph launch, capturing the
+// chunk on st the first time (st must not be the null stream).
+int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, int* const flag_h[2]);
+void solver_batch_drop_graphs(SolverBatch& B);
 int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st);
 // The warm-start branch is taken from the first scene; a batch needs them all alike.
 inline bool solver_batch_warm_uniform(const SolverBatch& B) {

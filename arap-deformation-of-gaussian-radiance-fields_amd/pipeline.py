@@ -20,13 +20,16 @@ the frame splits into two halves on two streams:
 and frame t's deform side runs while frame t+1's solve runs; the samples (they
 read only the node table) run beside the six-point update and the renders.
 
-Batched sample frames (sample_batch = B > 1): nothing reads the samples during a
-drag (they feed the density-preserving step after the release), so stream C
-waits for B frames and applies their B node tables to the samples in one pass
-(arapgs_samples_frames: frame after frame in order, bit-identical to B
-per-frame sample updates, with the sample streams read and written once);
-B + 2 table slots keep the tables alive until their batch has run, and run()
-flushes a partial batch before it returns. Each side has its
+Rolling sample updates (sample_batch = B > 1): nothing reads the samples during
+a drag (they feed the density-preserving step after the release), so the
+samples are split into B chunks and frame t updates only chunk t mod B, with
+the node tables of every frame since that chunk's last update (at most B) in
+one pass (arapgs_samples_frames: frame after frame in order, bit-identical to
+per-frame sample updates). Every frame then does the same amount of sample
+work as a per-frame update, but reads and writes only 1/B of the sample
+streams; B + 2 table slots keep each table alive until the last chunk has
+applied it, and run() brings every chunk up to date before it returns. Each
+side has its
 own host thread, because each side synchronises with the host (the solve's CG
 convergence checks, the renderer's instance count) and those waits must not
 stall the other side. The two node-table slots bound the lead of the solve side
@@ -52,8 +55,8 @@ class DragPipeline:
         table), so they fill the CUs the renders' deep tiles leave idle.
         aims_fn(ctx, node_pos, aims): the frame's aims on the solve stream
         (default: the active nodes translated by delta, UpdateAimPosition).
-        sample_batch: drag frames per sample update pass (1..8; more than one
-        needs split_samples, K = 8 and samples with features)."""
+        sample_batch: B, the sample chunks of the rolling update (1..8; more
+        than one needs split_samples, K = 8 and samples with features)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -69,7 +72,10 @@ class DragPipeline:
             raise ValueError("sample_batch must be in 1..8")
         self.nb = sample_batch if (self.split and state.get("sample_feat") is not None) else 1
         self.R = 2 if self.nb == 1 else self.nb + 2
-        self.pending = []
+        S = state["samples"].shape[0] if self.split else 0
+        self.chunks = [(c * S // self.nb, (c + 1) * S // self.nb) for c in range(self.nb)]
+        self.applied = [-1] * self.nb  # last frame applied to each chunk
+        self.frame = 0                 # frames whose deform side has been queued
         self.stream_c = torch.cuda.Stream(device=dev) if self.split else None
         self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
         self.ev_c = [torch.cuda.Event() for _ in range(self.R)]
@@ -125,32 +131,44 @@ class DragPipeline:
                 q = ctx.node_quats(table)
                 ctx.sample_sh(st["sample_w"], st["sample_idx"], q, st.get("sample_static"), st["sample_feat"])
 
-    def _flush_samples(self):
-        """The pending frames' sample updates in one pass on stream C."""
-        if not self.pending:
+    def _update_chunk(self, c, upto):
+        """Chunk c of the samples brought up to frame `upto` (all frames since
+        its last update, in order) in one pass on stream C."""
+        st = self.st
+        frames = list(range(self.applied[c] + 1, upto + 1))
+        if not frames:
             return
-        st, torch = self.st, self.torch
-        with torch.cuda.stream(self.stream_c):
-            # stream A is in order: the last pending frame's solve implies the others
-            self.stream_c.wait_event(self.ev_a[self.pending[-1]])
-            for slot in self.pending:
-                self.ctx_c.node_quats(self.tables[slot], out=self.quats[slot])
-            self.ctx_c.samples_frames([self.tables[k] for k in self.pending], [self.quats[k] for k in self.pending],
-                                      st["sample_idx"], st["sample_w"], st.get("sample_static"), st["samples"],
-                                      st["sample_feat"])
-            for slot in self.pending:
-                self.ev_c[slot].record(self.stream_c)
-                self.ev_c_recorded[slot] = True
-        self.pending = []
+        b, e = self.chunks[c]
+        slots = [t % self.R for t in frames]
+        if e > b:
+            static = st.get("sample_static")
+            self.ctx_c.samples_frames([self.tables[k] for k in slots], [self.quats[k] for k in slots],
+                                      st["sample_idx"][b:e], st["sample_w"][b:e],
+                                      None if static is None else static[b:e], st["samples"][b:e],
+                                      st["sample_feat"][b:e])
+        for k in slots:
+            self.ev_c[k].record(self.stream_c)
+            self.ev_c_recorded[k] = True
+        self.applied[c] = upto
 
-    def _deform_half(self, slot, render=True):
+    def _rolling_samples(self, slot, t, last):
+        """Frame t's node quaternions, then chunk t mod B (every chunk when
+        `last`) brought up to frame t."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream_c):
+            self.stream_c.wait_event(self.ev_a[slot])
+            self.ctx_c.node_quats(self.tables[slot], out=self.quats[slot])
+            for c in (range(self.nb) if last else [t % self.nb]):
+                self._update_chunk(c, t)
+
+    def _deform_half(self, slot, render=True, last=False):
         st, torch = self.st, self.torch
         ctx = self.ctx_b
         table = self.tables[slot]
+        t = self.frame
+        self.frame += 1
         if self.split and self.nb > 1:
-            self.pending.append(slot)
-            if len(self.pending) == self.nb:
-                self._flush_samples()
+            self._rolling_samples(slot, t, last)
         elif self.split:
             with torch.cuda.stream(self.stream_c):
                 self.stream_c.wait_event(self.ev_a[slot])
@@ -180,6 +198,7 @@ class DragPipeline:
         slots = queue.Queue()
         free = threading.Semaphore(2)
         err = []
+        base = self.frame  # frames of earlier runs: table slots continue from there
 
         def solve_side():
             try:
@@ -187,8 +206,8 @@ class DragPipeline:
                     free.acquire()
                     if err:
                         return
-                    self._solve_half(t % self.R)
-                    slots.put(t % self.R)
+                    self._solve_half((base + t) % self.R)
+                    slots.put((base + t) % self.R)
             except BaseException as e:  # noqa: BLE001 - re-raised on the caller's thread
                 err.append(e)
                 slots.put(None)
@@ -199,9 +218,7 @@ class DragPipeline:
                     slot = slots.get()
                     if slot is None:
                         return
-                    n = self._deform_half(slot, render)
-                    if t == n_frames - 1:
-                        self._flush_samples()
+                    n = self._deform_half(slot, render, last=t == n_frames - 1)
                     free.release()
                     self.instances = n
                     if on_frame is not None:

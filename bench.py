@@ -184,6 +184,7 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
         B["solver"].set_cg_mode(0)
         B["solver"].set_host_checks(B["cfg"]["host_checks"])
         B["solver"].set_warm_start(B["cfg"]["warm"])
+        B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
         nb = B["cfg"].get("sample_batch", 1) if sample_batch is None else sample_batch
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
@@ -208,12 +209,13 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
 
 
 def sample_batch_leg(B, args):
-    """Picks the drag frames per sample update pass of the pipelined form
+    """Picks the chunk count of the pipelined form's rolling sample update
     from a few pipelined frames per candidate (results are identical)."""
     best, best_t, trials = 1, None, {}
     for nb in (1, 2, 4, 8):
-        r = timed_pipelined(B, args.tune_steps * 2, 2, sample_batch=nb)
-        t = r["total_s"] / (args.tune_steps * 2)
+        n, w = tune_frames(args)
+        r = timed_pipelined(B, n, w, sample_batch=nb)
+        t = r["total_s"] / n
         trials[nb] = round(t * 1e3, 3)
         log(f"  sample batch {nb}: {t * 1e3:.3f} ms/frame")
         if best_t is None or t < best_t:
@@ -226,9 +228,10 @@ def partition_leg(B, args):
     streams on all CUs) from a few pipelined frames per candidate."""
     best, best_t = 0, None
     for cus in (0, 16, 32, 48, 64):
-        r = timed_pipelined(B, args.tune_steps * 2, 2, solve_cus=cus)
+        n, w = tune_frames(args)
+        r = timed_pipelined(B, n, w, solve_cus=cus)
         t = r["total_s"]
-        log(f"  partition: solve on {cus} CUs -> {t / (args.tune_steps * 2) * 1e3:.3f} ms/frame")
+        log(f"  partition: solve on {cus} CUs -> {t / n * 1e3:.3f} ms/frame")
         if best_t is None or t < best_t:
             best, best_t = cus, t
     return best
@@ -240,6 +243,7 @@ def timed(B, plain, steps, warmup):
     B["solver"].set_cg_mode(0)
     B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
     B["solver"].set_warm_start(0 if plain else B["cfg"]["warm"])
+    B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
     for _ in range(warmup):
         frame(B, plain)
     torch.cuda.synchronize()
@@ -275,19 +279,31 @@ def apply_config(B, cfg):
         torch.cuda.synchronize()
     B["solver"].set_coarse_refresh(cfg["refresh"])
     B["solver"].set_cg_block(cfg.get("cg_block", 256))
+    B["solver"].set_cg_graphs(cfg.get("cg_graphs", 0))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     B["cfg"] = dict(cfg)
 
 
+def tune_frames(args):
+    """(timed, warmup) frames per tuning candidate: by default the timed drag
+    itself, so that settings whose effect grows over the drag (the coarse
+    operator's age, the warm start) are judged on the frames that are timed."""
+    if args.tune_steps > 0:
+        return args.tune_steps, 2
+    return args.steps, args.warmup
+
+
 def autotune_leg(B, args):
-    """Coordinate descent over the tunables on the fast form (a few frames per
-    candidate, each from the same starting state)."""
+    """Coordinate descent over the tunables on the form that is reported, the
+    pipelined drag (4 rolling sample chunks while tuning), every candidate
+    timed over the same frames from the same starting state."""
     from arapgs_amd import autotune
 
     def measure(cfg):
         apply_config(B, cfg)
-        r = timed(B, False, args.tune_steps, 2)
-        return r["total_s"] / args.tune_steps
+        n, w = tune_frames(args)
+        r = timed_pipelined(B, n, w, sample_batch=4)
+        return r["total_s"] / n
 
     best, trials = autotune.tune(measure, log=log)
     apply_config(B, best)
@@ -365,6 +381,7 @@ def record_frames(B, steps, warmup):
     B["solver"].set_cg_mode(0)
     B["solver"].set_host_checks(B["cfg"]["host_checks"])
     B["solver"].set_warm_start(B["cfg"]["warm"])
+    B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
     pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
                      B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1))
     pipe.rec = []
@@ -518,6 +535,7 @@ def fixed_iteration_leg(B, args):
         plain = form == "plain"
         B["solver"].set_cg_mode(0)
         B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
+        B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
         ms = []
         for i in range(args.warmup + args.fixed_steps):
             reset_state(B)
@@ -540,7 +558,7 @@ def fixed_iteration_leg(B, args):
             "speedup_fast_vs_plain": round(res["fast"]["fps"] / res["plain"]["fps"], 3)}
 
 
-def large_leg(args):
+def large_leg(args, tuning):
     """The large high-frequency workload: 3M flat Gaussians, 4096 nodes, K=8,
     128^3 density grid, the combined bend + twist + 1.3x scale edit spread over a
     30-frame drag (per frame: translate, twist about the active capsule's axis,
@@ -584,6 +602,7 @@ def large_leg(args):
     solver.set_coarse_refresh(8)
     solver.set_host_checks(4)
     solver.set_warm_start(1)
+    solver.set_cg_graphs(tuning.get("cg_graphs", 0))
     s0 = workload.clone_state(st)
     # a few sequential frames with the kernels timed by HIP events
     fb = arapgs.frame_buffers(st, K)
@@ -864,9 +883,12 @@ def main():
     ap.add_argument("--aggregates", type=int, default=0, help="coarse aggregates of the CG preconditioner (0: 128)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the autotuner (tile, blend unroll, aggregates, coarse refresh, CG host checks)")
-    ap.add_argument("--tune-steps", type=int, default=6, help="timed frames per autotuner candidate")
+    ap.add_argument("--tune-steps", type=int, default=0,
+                    help="timed frames per autotuner candidate (0: the timed drag itself, --warmup + --steps)")
     ap.add_argument("--sample-batch", type=int, default=0,
-                    help="drag frames per sample update pass in the pipelined form (0: autotuned, or 4)")
+                    help="sample chunks of the rolling sample update in the pipelined form (0: autotuned, or 4)")
+    ap.add_argument("--cg-graphs", type=int, default=-1, choices=(-1, 0, 1),
+                    help="CG chunks as captured HIP graphs (-1: autotuned, or off without the autotuner)")
     ap.add_argument("--partition", action="store_true",
                     help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
@@ -898,11 +920,13 @@ def main():
     apply_config(B, B["cfg"])
     tuning = None
     if args.autotune:
-        log(f"autotuning on the fast form ({args.tune_steps} frames per candidate)")
+        log(f"autotuning on the pipelined form ({tune_frames(args)[0]} frames per candidate)")
         best, trials = autotune_leg(B, args)
         tuning = {"config": best, "candidates_timed": len(trials),
                   "ms_per_frame": {json.dumps(c, sort_keys=True): round(t * 1e3, 3) for c, t in trials}}
         log(f"  autotune: {best}")
+    if args.cg_graphs >= 0:
+        apply_config(B, dict(B["cfg"], cg_graphs=args.cg_graphs))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")
@@ -911,7 +935,7 @@ def main():
                 B["cfg"]["sample_batch"] = args.sample_batch
             elif args.autotune:
                 B["cfg"]["sample_batch"], sb_trials = sample_batch_leg(B, args)
-                log(f"  sample batch: {B['cfg']['sample_batch']} frames per sample pass")
+                log(f"  sample batch: {B['cfg']['sample_batch']} rolling sample chunks")
                 if tuning is not None:
                     tuning["sample_batch_ms_per_frame"] = sb_trials
                     tuning["config"] = dict(B["cfg"])
@@ -1000,7 +1024,7 @@ def main():
         torch.cuda.empty_cache()
     if args.large_frames > 0:
         log(f"timing the large-scene leg: {args.warmup} warmup + {args.large_frames} frames")
-        line["large"] = large_leg(args)
+        line["large"] = large_leg(args, line["config"]["tuning"])
     print(json.dumps(line), flush=True)
 
 

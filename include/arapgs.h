@@ -178,6 +178,17 @@ int arapgs_solver_set_warm_start(arapgs_solver* s, int on);
  * Synchronises the device. */
 int arapgs_solver_reset(arapgs_solver* s);
 
+/* on = 1: with host checks every `every` CG iterations (arapgs_solver_set_host_checks
+ * > 0) on a non-null stream, each chunk of `every` CG iterations and its
+ * convergence flag are one captured HIP graph (captured on the first use of each
+ * chunk index and kept), and the host reads chunk j-1's flag while chunk j is
+ * already queued, so the stream does not wait for the host between chunks; a
+ * chunk queued past convergence exits kernel by kernel on the device flag. The
+ * kernels, their arguments and their order are those of the launch loop, so the
+ * iterates are identical bit for bit. No counterpart in the reference (its
+ * solver is a CPU Cholesky). Default 0. */
+int arapgs_solver_set_cg_graphs(arapgs_solver* s, int on);
+
 /* GN iterations and total CG iterations of the solver's last solve
  * (synchronises the device). Diagnostics; no counterpart in the reference. */
 int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h);
@@ -218,6 +229,8 @@ int arapgs_solver_batch_create(arapgs_ctx* ctx, arapgs_solver* const* solvers_h,
 void arapgs_solver_batch_destroy(arapgs_solver_batch* b);
 int arapgs_solver_batch_set_host_checks(arapgs_solver_batch* b, int every);
 int arapgs_solver_batch_set_coarse_refresh(arapgs_solver_batch* b, int frames);
+/* arapgs_solver_set_cg_graphs for a batch's solve. */
+int arapgs_solver_batch_set_cg_graphs(arapgs_solver_batch* b, int on);
 int arapgs_solver_batch_set_async_refresh(arapgs_solver_batch* b, int on);
 /* Host arrays of n device pointers, one per scene in batch order: node_pos and
  * aims f32[M_s*3] (required); x_out f32[12*n_free_s], energies f64[iters+1],

@@ -297,3 +297,34 @@ def test_cg_block_size_keeps_the_solution(ctx, prob, block):
             assert abs(e[-1] - e_ref[-1]) <= 1e-5 * e_ref[-1], (block, async_refresh, e[-1], e_ref[-1])
             assert abs(n - it_ref) <= 1
         s.close()
+
+
+@pytest.mark.parametrize("every", [4, 3])
+def test_cg_graphs_bit_identical(prob, every):
+    """CG chunks replayed as captured HIP graphs (checked one chunk behind)
+    compute the launch loop's iterates bit for bit: three consecutive warm-started
+    solves of different aims, on a context with its own stream (graphs are
+    captured on a non-null stream), chunk sizes 4 and 3 (the CG cap of 200 is not
+    a multiple of 3: the last chunk is shorter)."""
+    import torch
+    c2 = arapgs.Context(0, stream=torch.cuda.Stream())
+    runs = []
+    for graphs in (False, True):
+        with torch.cuda.stream(c2.stream):
+            s = _solver(c2, prob)
+            s.set_host_checks(every)
+            s.set_warm_start(True)
+            s.set_cg_graphs(graphs)
+            out = []
+            for f in range(3):
+                aims, _ = _problem(prob, 0.6 + 0.2 * f)
+                x, e, it = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+                out.append((x.clone(), e.clone(), it.clone()))
+        torch.cuda.synchronize()
+        runs.append([tuple(H(t) for t in o) for o in out])
+        s.close()
+    for a, b in zip(*runs):
+        assert int(a[2][0]) >= 1
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+    c2.close()
