@@ -72,6 +72,8 @@ EXPORTS = {
     "arapgs_set_stream": ([P, P], C.c_int),
     "arapgs_synchronize": ([P], C.c_int),
     "arapgs_set_grid_chunk": ([P, C.c_int64], C.c_int),
+    "arapgs_set_node_cache": ([P, C.c_int], C.c_int),
+    "arapgs_set_blend_split": ([P, C.c_int], C.c_int),
     "arapgs_stream_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_stream_destroy": ([P], C.c_int),
     "arapgs_version": ([], C.c_char_p),
@@ -455,6 +457,14 @@ class Context:
         self._check(lib().arapgs_gaussian_aabbs(self.h, ptr(pos), ptr(rot), ptr(scale), ptr(opacity), pos.shape[0],
                                                 ptr(out)))
         return out
+
+    def set_blend_split(self, entries):
+        """Deep 16x16 tiles blended in segments of `entries` instances (0: off)."""
+        self._check(lib().arapgs_set_blend_split(self.h, int(entries)))
+
+    def set_node_cache(self, on=True):
+        """Six-point end-point blend through the LDS node-row cache (bit-identical)."""
+        self._check(lib().arapgs_set_node_cache(self.h, int(bool(on))))
 
     def set_grid_chunk(self, pairs):
         """Cell-Gaussian pairs per chunk of later grid builds (0: default); test hook."""

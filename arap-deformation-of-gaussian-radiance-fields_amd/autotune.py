@@ -11,6 +11,10 @@ its tolerance:
   cg_block      workgroup size of the CG operator kernel: 256, 128 or 64 threads
   cg_graphs     CG chunks between host checks as captured HIP graphs, checked
                 one chunk behind (1) or launched kernel by kernel (0)
+  node_cache    K = 8 blends through the per-workgroup LDS node-row cache (1) or
+                with every node row gathered from the global table (0)
+  blend_split   16x16 tiles with more instances than this blended in segments by
+                separate workgroups and composited (0: one workgroup per tile)
 
 `tune` runs a coordinate descent from the defaults: for each parameter in turn
 it times every candidate value (the others held at their current best) with a
@@ -22,12 +26,14 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0)
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0,
+                node_cache=1, blend_split=0)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
-                  host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64), cg_graphs=(0, 1))
+                  host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64), cg_graphs=(0, 1),
+                  node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096))
 # host_checks is revisited after cg_graphs: the graphs' best chunk is shorter
 ORDER = ("warm", "aggregates", "refresh", "host_checks", "cg_graphs", "host_checks", "cg_block", "tile",
-         "blend_unroll")
+         "blend_unroll", "blend_split", "node_cache")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

@@ -52,6 +52,8 @@ struct arapgs_ctx {
     size_t r_ranges_cap = 0;
     void* r_rec = nullptr;          // depth sort of the records
     size_t r_rec_cap = 0;
+    void* r_split = nullptr;        // deep-tile split: segment lists and per-segment pixel partials
+    size_t r_split_cap = 0;
     void* db_ws = nullptr;          // density backward accumulators
     size_t db_cap = 0;
     uint64_t grid_chunk_pairs = 0;  // cell-list build chunk (0: GRID_CHUNK_PAIRS)
@@ -69,6 +71,8 @@ struct arapgs_ctx {
     hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
+    int blend_split = 0;              // instances per blend segment of a deep 16x16 tile (0: no split)
+    bool node_cache = true;           // six-point end-point blend through the LDS node-row cache
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
     // HIP events around the six-point (0), sample blend (1) and sample SH (2)
@@ -188,7 +192,7 @@ void arapgs_destroy(arapgs_ctx* c) {
             (void)hipEventDestroy(pr.second);
         }
     for (void* p : {c->knn_list, (void*)c->knn_count, c->fps_dist, (void*)c->fps_partial, (void*)c->fps_counter,
-                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec, c->db_ws,
+                    (void*)c->fps_best, (void*)c->edge_tmp, c->r_geo, c->r_scan, c->r_bin, c->r_sort, c->r_ranges, c->r_rec, c->r_split, c->db_ws,
                     (void*)c->r_total, (void*)c->shrot_dev, (void*)c->r_views, (void*)c->r_seg})
         if (p) (void)hipFree(p);
     if (c->r_views_h) (void)hipHostFree(c->r_views_h);
@@ -232,6 +236,13 @@ int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_o
 int arapgs_stream_destroy(void* stream) {
     if (!stream) return -1;
     return hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : -1;
+}
+
+int arapgs_set_node_cache(arapgs_ctx* c, int on) {
+    CHECK_CTX(c);
+    if (on < 0 || on > 1) return fail(c, -1, "set_node_cache: on must be 0 or 1");
+    c->node_cache = on != 0;
+    return 0;
 }
 
 int arapgs_set_grid_chunk(arapgs_ctx* c, int64_t pairs) {
@@ -701,7 +712,8 @@ int arapgs_blend_points(arapgs_ctx* c, const float* pts, int64_t p, const uint32
     CHECK_CTX(c);
     if (!pts || !idx || !w || !table || !out || p < 0 || k < 1 || k > KNN_MAX_T)
         return fail(c, -1, "blend_points: bad arguments");
-    return check_launch(c, launch_blend_points(pts, p, idx, w, k, (const NodeXf*)table, m, skip, out, c->stream),
+    return check_launch(c, launch_blend_points(pts, p, idx, w, k, (const NodeXf*)table, m, skip, out, c->stream,
+                                               c->node_cache),
                         "blend_points");
 }
 
@@ -717,7 +729,7 @@ int arapgs_six_point_update(arapgs_ctx* c, int64_t n, int k, const arapgs_node_x
     if ((uintptr_t)rot % 16) return fail(c, -1, "six_point_update: rot must be 16-byte aligned");
     if ((uintptr_t)sh % 16) return fail(c, -1, "six_point_update: sh must be 16-byte aligned");
     return check_launch(c, launch_six_point(n, k, (const NodeXf*)table, m, ends_idx, ends_w, ends, scale0, is_static,
-                                            pad, coeff, pos, rot, scale, sh, c->shrot_dev, c->stream),
+                                            pad, coeff, pos, rot, scale, sh, c->shrot_dev, c->stream, c->node_cache),
                         "six_point_update");
 }
 
@@ -793,8 +805,10 @@ extern "C" {
 
 // seg_h (segment mode, host int64[V+1] or null): view v draws Gaussians
 // [seg_h[v], seg_h[v+1]) only.
+// allow_split: deep tiles may be blended in segments (the chain, whose blend
+// backward re-runs the sequential blend, renders without).
 static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int* num_rendered_h,
-                        const int64_t* seg_h = nullptr) {
+                        const int64_t* seg_h = nullptr, bool allow_split = true) {
     CHECK_CTX(c);
     if (!dv || V < 1 || V > RASTER_MAX_VIEWS) return fail(c, -1, "render: need 1..%d views", RASTER_MAX_VIEWS);
     const arapgs_render_desc* d = dv;
@@ -836,6 +850,10 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     rp.use_rects = d->rects != nullptr;
     rp.use_box = (d->boxmin_h && d->boxmax_h) ? 1 : 0;
     rp.unroll = c->blend_unroll;
+    // the split composite does not mark contributors: renders that ask for them blend tile by tile
+    bool any_surface = false;
+    for (int v = 0; v < V; v++) any_surface = any_surface || dv[v].surface != nullptr;
+    rp.split = (allow_split && !any_surface && tile == 16) ? c->blend_split : 0;
     for (int i = 0; i < 3; i++) {
         rp.boxmin[i] = rp.use_box ? d->boxmin_h[i] : 0.0f;
         rp.boxmax[i] = rp.use_box ? d->boxmax_h[i] : 0.0f;
@@ -898,9 +916,12 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     uint32_t* keys_alt = (uint32_t*)(b + nn * 4);
     uint32_t* vals = (uint32_t*)(b + nn * BIN_VALS_AT);
     uint32_t* vals_alt = (uint32_t*)(b + nn * 12);
+    if (rp.split > 0)
+        rc = grow(c, &c->r_split, &c->r_split_cap, raster_split_bytes((int64_t)ntiles * V, n, rp.split));
+    if (rc) return rc;
     rc = check_launch(c, raster_bin_and_blend(rp, c->r_geo, n, c->r_total, dmin, dbits, keys, vals, keys_alt,
                                               vals_alt, (uint32_t*)c->r_sort, (uint2*)c->r_ranges, c->r_rec,
-                                              (uint32_t*)c->r_scan, c->stream),
+                                              (uint32_t*)c->r_scan, rp.split > 0 ? c->r_split : nullptr, c->stream),
                       "render");
     if (rc) return rc;
     c->last_rp = rp;
@@ -909,6 +930,14 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     c->last_rendered = n;
     c->last_tiles = ntiles * V;
     if (num_rendered_h) *num_rendered_h = (int)n;
+    return 0;
+}
+
+int arapgs_set_blend_split(arapgs_ctx* c, int entries) {
+    CHECK_CTX(c);
+    if (entries != 0 && (entries < 256 || entries % 256 != 0))
+        return fail(c, -1, "set_blend_split: 0 or a multiple of 256 (at least 256)");
+    c->blend_split = entries;
     return 0;
 }
 
@@ -1436,19 +1465,21 @@ int arapgs_deform_frame(arapgs_ctx* c, arapgs_solver* s, const arapgs_frame_buff
     if (!rc && fb->s > 0) {
         ev = prof_begin(c, 1);
         rc = check_launch(c, launch_blend_points(fb->samples, fb->s, fb->sample_idx, fb->sample_w, k, s->table, m,
-                                                 fb->sample_static, fb->samples, c->stream), "deform_frame samples");
+                                                 fb->sample_static, fb->samples, c->stream, c->node_cache),
+                          "deform_frame samples");
         prof_end(c, ev);
     }
     ev = rc ? nullptr : prof_begin(c, 0);
     if (!rc)
         rc = check_launch(c, launch_six_point(fb->n, k, s->table, m, fb->ends_idx, fb->ends_w, fb->ends, fb->scale0,
                                               fb->gs_static, fb->axis_padding, fb->end_coeff, fb->pos, fb->rot,
-                                              fb->scale, fb->sh, c->shrot_dev, c->stream),
+                                              fb->scale, fb->sh, c->shrot_dev, c->stream, c->node_cache),
                           "deform_frame six-point");
     prof_end(c, ev);
     if (!rc)
         rc = check_launch(c, launch_blend_points(fb->node_pos, m, fb->node_src_idx, fb->node_src_w, k, s->table, m,
-                                                 nullptr, s->node_tmp, c->stream), "deform_frame nodes");
+                                                 nullptr, s->node_tmp, c->stream, c->node_cache),
+                          "deform_frame nodes");
     if (!rc) rc = check_launch(c, launch_copy_f32(fb->node_pos, s->node_tmp, 3 * (int64_t)m, c->stream), "deform_frame");
     if (!rc && fb->s > 0 && fb->sample_feat) {
         rc = check_launch(c, launch_node_quats(m, s->table, s->node_q, c->stream), "deform_frame quats");
@@ -1548,12 +1579,12 @@ int chain_deform(arapgs_ctx* c, arapgs_chain* ch, const arapgs_node_xf* table, c
     if (e != hipSuccess) return hip_fail(c, e, "chain copies");
     rc = check_launch(c, launch_six_point(ch->N, ch->K, (const NodeXf*)table, ch->M, d.ends_idx, d.ends_w, ch->ends,
                                           scale0, nullptr, d.axis_padding, d.end_coeff, ch->pos, ch->rot, ch->scale,
-                                          ch->sh, c->shrot_dev, st),
+                                          ch->sh, c->shrot_dev, st, c->node_cache),
                       "chain six-point");
     if (rc) return rc;
     if (ch->S > 0)
         rc = check_launch(c, launch_blend_points(ch->samples0, ch->S, d.sample_idx, d.sample_w, ch->K,
-                                                 (const NodeXf*)table, ch->M, nullptr, ch->samples, st),
+                                                 (const NodeXf*)table, ch->M, nullptr, ch->samples, st, c->node_cache),
                           "chain sample blend");
     return rc;
 }
@@ -1572,7 +1603,7 @@ int chain_render(arapgs_ctx* c, arapgs_chain* ch, float* const* outs) {
         r.out_color = outs[v];
         r.ortho_scale = 1.0f; r.tile = 16;
     }
-    return render_views(c, rd.data(), ch->V, nullptr);
+    return render_views(c, rd.data(), ch->V, nullptr, nullptr, false);
 }
 
 }  // namespace

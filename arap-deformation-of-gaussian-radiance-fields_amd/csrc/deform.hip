@@ -56,6 +56,59 @@ __device__ __forceinline__ void blend_point_regs(const NodeXf* __restrict__ tabl
     }
 }
 
+// Node-row cache of a workgroup (CACHE): the block's 256 Gaussians lie close
+// together in the cell order, so their 12,288 end-point neighbour ids name a
+// few dozen nodes. A direct-mapped table of NODE_CACHE_SLOTS 64-byte rows in
+// LDS (slot = id mod slots) is filled by the first thread that needs a node
+// (compare-and-swap on the slot's tag; tags only ever go from empty to one id,
+// so a filled row never changes), and every gather whose slot holds its id
+// reads LDS instead of going through the texture path; ids whose slot another
+// node took read the global table. The rows are the same floats, so the blend
+// is bit-identical.
+constexpr int NODE_CACHE_SLOTS = 256;
+// A cached row takes 4 16-byte units. Padding rows to 5 units (80 bytes) spread
+// neighbouring slots over more bank groups but made both kernels slower (six
+// point 357 vs 303 us, sample blend 222 vs 169 us at 1M Gaussians / 5.6M
+// samples): the extra 4 KB per workgroup costs occupancy, which matters more
+// than the bank conflicts of the unpadded rows.
+constexpr int NODE_CACHE_ROW4 = 4;
+
+// Claims the empty slots of a point's neighbours and fills the claimed rows;
+// the caller synchronises the block before any lane reads the cache.
+template <int K>
+__device__ __forceinline__ void node_cache_claim(const NodeXf* __restrict__ table, const uint32_t (&id)[K], int* s_tag,
+                                                 float4* s_rows) {
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int slot = (int)(id[j] & (NODE_CACHE_SLOTS - 1));
+        if (s_tag[slot] == -1 && atomicCAS(&s_tag[slot], -1, (int)id[j]) == -1) {
+            const float4* src = reinterpret_cast<const float4*>(table + id[j]);
+#pragma unroll
+            for (int q = 0; q < 4; q++) s_rows[NODE_CACHE_ROW4 * slot + q] = src[q];
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void blend_point_cached(const NodeXf* __restrict__ table, const int* s_tag,
+                                                   const float4* s_rows, const uint32_t (&id)[K],
+                                                   const float (&wt)[K], const float p[3], float o[3]) {
+#pragma clang fp contract(off)
+    o[0] = o[1] = o[2] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int slot = (int)(id[j] & (NODE_CACHE_SLOTS - 1));
+        const float4* r = s_tag[slot] == (int)id[j] ? s_rows + NODE_CACHE_ROW4 * slot
+                                                    : reinterpret_cast<const float4*>(table + id[j]);
+        const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+        const float tx = p[0] - a.x, ty = p[1] - a.y, tz = p[2] - a.z;
+        const double wj = (double)wt[j];
+        o[0] = (float)((double)o[0] + wj * ((double)a.w * tx + (double)b.x * ty + (double)b.y * tz + (double)d.x + (double)a.x));
+        o[1] = (float)((double)o[1] + wj * ((double)b.z * tx + (double)b.w * ty + (double)c.x * tz + (double)d.y + (double)a.y));
+        o[2] = (float)((double)o[2] + wj * ((double)c.y * tx + (double)c.z * ty + (double)c.w * tz + (double)d.z + (double)a.z));
+    }
+}
+
 // K neighbour ids and weights of one point; 16-byte vector loads when the
 // rows are 16-byte aligned (K % 4 == 0 and aligned base pointers).
 template <int K, bool VEC>
@@ -123,25 +176,40 @@ __global__ void node_table_kernel(int M, const float* __restrict__ node_pos, con
 // VEC: the K indices and weights of a point are read as 16-byte vectors (rows
 // 16-byte aligned): two loads per array instead of K, each covering half of
 // every lane's 32-byte row.
-template <int K, bool VEC>
+// CACHE: the node rows go through the workgroup's LDS node-row cache (every
+// thread reaches the block barrier; results identical).
+template <int K, bool VEC, bool CACHE>
 __global__ void __launch_bounds__(256) blend_points_kernel(const float* pts, int64_t P,
                                                            const uint32_t* __restrict__ idx,
                                                            const float* __restrict__ w,
                                                            const NodeXf* __restrict__ table,
                                                            const uint8_t* __restrict__ skip,
                                                            float* out) {
+    __shared__ int s_tag[CACHE ? NODE_CACHE_SLOTS : 1];
+    __shared__ float4 s_rows[CACHE ? NODE_CACHE_ROW4 * NODE_CACHE_SLOTS : 1];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P) return;
+    const bool here = i < P;
+    const bool moves = here && !(skip && skip[i]);
+    uint32_t id[K];
+    float wt[K];
+    if (moves) load_nbrs<K, VEC>(idx + i * K, w + i * K, id, wt);
+    if constexpr (CACHE) {
+        for (int t = threadIdx.x; t < NODE_CACHE_SLOTS; t += blockDim.x) s_tag[t] = -1;
+        __syncthreads();
+        if (moves) node_cache_claim<K>(table, id, s_tag, s_rows);
+        __syncthreads();
+    }
+    if (!here) return;
     const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
-    if (skip && skip[i]) {
+    if (!moves) {
         out[3 * i] = p[0]; out[3 * i + 1] = p[1]; out[3 * i + 2] = p[2];
         return;
     }
     float o[3];
-    uint32_t id[K];
-    float wt[K];
-    load_nbrs<K, VEC>(idx + i * K, w + i * K, id, wt);
-    blend_point_regs<K>(table, id, wt, p, o);
+    if constexpr (CACHE)
+        blend_point_cached<K>(table, s_tag, s_rows, id, wt, p, o);
+    else
+        blend_point_regs<K>(table, id, wt, p, o);
     out[3 * i] = o[0]; out[3 * i + 1] = o[1]; out[3 * i + 2] = o[2];
 }
 
@@ -167,28 +235,32 @@ __global__ void __launch_bounds__(256) blend_points_generic_kernel(const float* 
 // consecutive items moved between global memory and the owning threads'
 // registers through LDS, in passes of as many rows as `lds` holds. The global
 // side is whole contiguous float4 runs (a wave's access covers 1 KB), so every
-// cache line is fetched once; a lane touches LDS only for its own row (odd row
-// stride: the 64 lanes of a wave hit distinct banks). Every thread of the block
-// must call these (they synchronise); rows [0, n_rows) exist.
-constexpr int ROW48_STRIDE = 49;
+// cache line is fetched once. LDS is accessed in 16-byte units with a row
+// stride of 13 units (52 floats): 13 is odd, so the 8 lanes that share a bank
+// cycle touch 8 different 16-byte bank groups both when a wave writes 64
+// consecutive units of the global side and when each lane reads its own row
+// (a 49-float stride with 4-byte accesses lost 26-28% of the LDS cycles to
+// bank conflicts). Every thread of the block must call these (they
+// synchronise); rows [0, n_rows) exist.
+constexpr int ROW48_STRIDE4 = 13;
 
 __device__ __forceinline__ void rows48_load(const float* __restrict__ src, int n_rows, float* lds, int lds_floats,
                                             float (&c)[48]) {
     const int t = threadIdx.x;
-    const int per = lds_floats / ROW48_STRIDE;
+    const int per = lds_floats / (4 * ROW48_STRIDE4);
+    float4* l4 = reinterpret_cast<float4*>(lds);
     for (int r0 = 0; r0 < n_rows; r0 += per) {
         const int nr = min(per, n_rows - r0);
         const float4* s4 = reinterpret_cast<const float4*>(src + (int64_t)r0 * 48);
-        for (int q = t; q < nr * 12; q += blockDim.x) {
-            const float4 v = s4[q];
-            float* d = lds + (q / 12) * ROW48_STRIDE + (q % 12) * 4;
-            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
-        }
+        for (int q = t; q < nr * 12; q += blockDim.x) l4[(q / 12) * ROW48_STRIDE4 + q % 12] = s4[q];
         __syncthreads();
         if (t >= r0 && t < r0 + nr) {
-            const float* d = lds + (t - r0) * ROW48_STRIDE;
+            const float4* d = l4 + (t - r0) * ROW48_STRIDE4;
 #pragma unroll
-            for (int k = 0; k < 48; k++) c[k] = d[k];
+            for (int k = 0; k < 12; k++) {
+                const float4 v = d[k];
+                c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w;
+            }
         }
         __syncthreads();
     }
@@ -197,20 +269,18 @@ __device__ __forceinline__ void rows48_load(const float* __restrict__ src, int n
 __device__ __forceinline__ void rows48_store(float* __restrict__ dst, int n_rows, float* lds, int lds_floats,
                                              const float (&c)[48]) {
     const int t = threadIdx.x;
-    const int per = lds_floats / ROW48_STRIDE;
+    const int per = lds_floats / (4 * ROW48_STRIDE4);
+    float4* l4 = reinterpret_cast<float4*>(lds);
     for (int r0 = 0; r0 < n_rows; r0 += per) {
         const int nr = min(per, n_rows - r0);
         if (t >= r0 && t < r0 + nr) {
-            float* d = lds + (t - r0) * ROW48_STRIDE;
+            float4* d = l4 + (t - r0) * ROW48_STRIDE4;
 #pragma unroll
-            for (int k = 0; k < 48; k++) d[k] = c[k];
+            for (int k = 0; k < 12; k++) d[k] = make_float4(c[4 * k], c[4 * k + 1], c[4 * k + 2], c[4 * k + 3]);
         }
         __syncthreads();
         float4* d4 = reinterpret_cast<float4*>(dst + (int64_t)r0 * 48);
-        for (int q = t; q < nr * 12; q += blockDim.x) {
-            const float* d = lds + (q / 12) * ROW48_STRIDE + (q % 12) * 4;
-            d4[q] = make_float4(d[0], d[1], d[2], d[3]);
-        }
+        for (int q = t; q < nr * 12; q += blockDim.x) d4[q] = l4[(q / 12) * ROW48_STRIDE4 + q % 12];
         __syncthreads();
     }
 }
@@ -227,7 +297,7 @@ __device__ __forceinline__ void rows48_store(float* __restrict__ dst, int n_rows
 // Static Gaussians (every end-point neighbour a static node) still blend their
 // end points, as the reference does, but keep their attributes (their SH rows
 // are written back unchanged).
-template <int K, bool VEC>
+template <int K, bool VEC, bool CACHE>
 __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf* __restrict__ table,
                                                         const uint32_t* __restrict__ ends_idx,
                                                         const float* __restrict__ ends_w, float* __restrict__ ends,
@@ -236,20 +306,35 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
                                                         float* __restrict__ pos, float* __restrict__ rot,
                                                         float* __restrict__ scale, float* __restrict__ sh,
                                                         const ShRotTables* __restrict__ T) {
-    __shared__ float s_e[256 * 18];
+    __shared__ __align__(16) float s_e[256 * 18];
+    __shared__ int s_tag[CACHE ? NODE_CACHE_SLOTS : 1];
+    __shared__ float4 s_rows[CACHE ? NODE_CACHE_ROW4 * NODE_CACHE_SLOTS : 1];
     const int64_t g0 = (int64_t)blockIdx.x * 256;
     const int64_t npts = 6 * min((int64_t)256, N - g0);
+    if constexpr (CACHE) {
+        for (int i = threadIdx.x; i < NODE_CACHE_SLOTS; i += 256) s_tag[i] = -1;
+        __syncthreads();
+    }
 #pragma unroll 1
     for (int r = 0; r < 6; r++) {
         const int k = r * 256 + (int)threadIdx.x;
-        if (k < npts) {
-            const int64_t pt = g0 * 6 + k;
-            uint32_t id[K];
-            float wt[K];
-            load_nbrs<K, VEC>(ends_idx + pt * K, ends_w + pt * K, id, wt);
+        const bool here = k < npts;
+        const int64_t pt = g0 * 6 + k;
+        uint32_t id[K];
+        float wt[K];
+        if (here) load_nbrs<K, VEC>(ends_idx + pt * K, ends_w + pt * K, id, wt);
+        if constexpr (CACHE) {
+            // claim empty slots, fill the claimed rows, then every lane reads
+            if (here) node_cache_claim<K>(table, id, s_tag, s_rows);
+            __syncthreads();
+        }
+        if (here) {
             const float in[3] = {ends[pt * 3], ends[pt * 3 + 1], ends[pt * 3 + 2]};
             float o[3];
-            blend_point_regs<K>(table, id, wt, in, o);
+            if constexpr (CACHE)
+                blend_point_cached<K>(table, s_tag, s_rows, id, wt, in, o);
+            else
+                blend_point_regs<K>(table, id, wt, in, o);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 ends[pt * 3 + c] = o[c];
@@ -360,7 +445,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) s
                                                         const uint8_t* __restrict__ is_static,
                                                         float* __restrict__ feat, const ShRotTables* __restrict__ T) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float lds[256 * 18];
+    __shared__ __align__(16) float lds[256 * 18];
     const int64_t s0 = (int64_t)blockIdx.x * blockDim.x;
     const int n_here = (int)min((int64_t)blockDim.x, S - s0);
     float c[48];
@@ -468,7 +553,7 @@ samples_frames_kernel(int64_t S, SampleFrames F, const uint32_t* __restrict__ id
                       const uint8_t* __restrict__ is_static, float* __restrict__ pts, float* __restrict__ feat,
                       const ShRotTables* __restrict__ T) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float lds[256 * 18];
+    __shared__ __align__(16) float lds[256 * 18];
     const int64_t s0 = (int64_t)blockIdx.x * blockDim.x;
     const int n_here = (int)min((int64_t)blockDim.x, S - s0);
     const bool live = s < S && !(is_static && is_static[s]);
@@ -542,16 +627,19 @@ int launch_build_node_table(int M, const float* node_pos, const float* x_free, c
 }
 
 int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const float* w, int k, const NodeXf* table,
-                        int M, const uint8_t* skip, float* out, hipStream_t st) {
+                        int M, const uint8_t* skip, float* out, hipStream_t st, bool node_cache) {
     (void)M;
     if (P == 0) return 0;
     const bool vec = ((uintptr_t)idx % 16 == 0) && ((uintptr_t)w % 16 == 0);
-    if (k == 8 && vec)
-        hipLaunchKernelGGL((blend_points_kernel<8, true>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, table,
-                           skip, out);
-    else if (k == 8)
-        hipLaunchKernelGGL((blend_points_kernel<8, false>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w,
+    if (k == 8 && vec && node_cache)
+        hipLaunchKernelGGL((blend_points_kernel<8, true, true>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w,
                            table, skip, out);
+    else if (k == 8 && vec)
+        hipLaunchKernelGGL((blend_points_kernel<8, true, false>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w,
+                           table, skip, out);
+    else if (k == 8)
+        hipLaunchKernelGGL((blend_points_kernel<8, false, false>), dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx,
+                           w, table, skip, out);
     else
         hipLaunchKernelGGL(blend_points_generic_kernel, dim3(nblk(P, 256)), dim3(256), 0, st, pts, P, idx, w, k,
                            table, skip, out);
@@ -561,18 +649,24 @@ int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const 
 
 int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
-                     float* rot, float* scale, float* sh, const ShRotTables* T, hipStream_t st) {
+                     float* rot, float* scale, float* sh, const ShRotTables* T, hipStream_t st, bool node_cache) {
     (void)M;
     if (N == 0) return 0;
     const bool vec = ((uintptr_t)ends_idx % 16 == 0) && ((uintptr_t)ends_w % 16 == 0);
+    if (node_cache && k == 8 && vec) {
+        hipLaunchKernelGGL((six_point_kernel<8, true, true>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx,
+                           ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);
+        HIP_RET(hipGetLastError());
+        return 0;
+    }
     switch (k) {
 #define ARAPGS_SP_CASE(KV)                                                                                             \
     case KV:                                                                                                           \
         if (vec)                                                                                                       \
-            hipLaunchKernelGGL((six_point_kernel<KV, true>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx, \
-                               ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);                  \
+            hipLaunchKernelGGL((six_point_kernel<KV, true, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,    \
+                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);         \
         else                                                                                                           \
-            hipLaunchKernelGGL((six_point_kernel<KV, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,          \
+            hipLaunchKernelGGL((six_point_kernel<KV, false, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,   \
                                ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);         \
         break;
         ARAPGS_SP_CASE(1) ARAPGS_SP_CASE(2) ARAPGS_SP_CASE(3) ARAPGS_SP_CASE(4) ARAPGS_SP_CASE(5) ARAPGS_SP_CASE(6)

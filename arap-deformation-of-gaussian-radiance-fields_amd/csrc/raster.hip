@@ -427,9 +427,30 @@ __global__ void __launch_bounds__(TILE* TILE) blend_kernel(RasterParams rp, Geo 
 // quadrant is skipped.
 // U (2, 4 or 8) is the number of Gaussians whose alphas are evaluated together
 // before compositing them in order; it changes only the instruction schedule.
-template <int U>
+// Deep-tile split (SPLIT): the block is segment b of the global segment list
+// (tile t holds cnt[t] = max(1, ceil(n_t / split)) segments starting at
+// off[t]); blocks past the last segment exit. A tile of one segment is blended
+// and written exactly as without the split. A segment of a split tile blends
+// its slice of the tile's depth-ordered list from transmittance 1 and writes
+// its pixels' (C, T) to part[b], with T negated when the local test stopped
+// the pixel. split_composite_kernel then combines the segments front to back
+// (C += T C_s, T *= T_s) up to the segment where the pixel's transmittance
+// first falls below 1e-4 (the running product only decreases, so that is the
+// segment where the sequential blend stops it) and blends that one segment
+// again from the true incoming transmittance with the sequential test, so the
+// contributors are those of the tile-by-tile blend; only the association of
+// the products and sums differs.
+struct SplitArgs {
+    const uint32_t* cnt;
+    const uint32_t* off;
+    const uint32_t* total;
+    float4* part;
+    int nt;
+};
+
+template <int U, bool SPLIT>
 __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo geo, const uint32_t* __restrict__ vals,
-                                                         const uint2* __restrict__ ranges) {
+                                                         const uint2* __restrict__ ranges, SplitArgs sa) {
     constexpr int NT = 256;
     __shared__ float2 s_xy[NT];
     __shared__ float4 s_co[NT];
@@ -438,8 +459,29 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
     __shared__ __align__(8) uint8_t s_list[4][NT + 8];
     __shared__ int s_cnt[4][4];  // [quadrant][wave]
     const int ntl = rp.gx * rp.gy;
-    const int view = blockIdx.x / ntl;
-    const int tile = blockIdx.x - view * ntl;
+    int tg = blockIdx.x;          // tile of all views
+    uint2 rg;
+    int64_t pslot = -1;           // this block's partial slot (segment of a split tile)
+    if constexpr (SPLIT) {
+        const uint32_t b = blockIdx.x;
+        if (b >= *sa.total) return;  // uniform per block, before any barrier
+        int lo = 0, hi = sa.nt - 1;  // the last tile whose first segment is <= b
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sa.off[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        tg = lo;
+        const uint32_t sidx = b - sa.off[tg];
+        const uint2 full = ranges[tg];
+        rg.x = full.x + sidx * (uint32_t)rp.split;
+        rg.y = min(full.y, rg.x + (uint32_t)rp.split);
+        if (sa.cnt[tg] > 1) pslot = (int64_t)b;
+    } else {
+        rg = ranges[tg];
+    }
+    const int view = tg / ntl;
+    const int tile = tg - view * ntl;
     float* __restrict__ out = rp.views[view].out;
     int* __restrict__ surface = rp.views[view].surface;
     const uint32_t vbase = rp.seg ? (uint32_t)rp.seg[view] : (uint32_t)view * (uint32_t)rp.P;
@@ -460,7 +502,6 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
         qb[q][2] = (float)min(x0 + 7, rp.W - 1);
         qb[q][3] = (float)min(y0 + 7, rp.H - 1);
     }
-    const uint2 rg = ranges[blockIdx.x];
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     bool done = !inside;
     const int n = (int)(rg.y - rg.x);
@@ -538,12 +579,103 @@ __global__ void __launch_bounds__(256) blend_quad_kernel(RasterParams rp, Geo ge
             }
         }
     }
-    if (inside) {
+    if (SPLIT && pslot >= 0) {
+        sa.part[pslot * NT + threadIdx.x] = make_float4(C0, C1, C2, (done && inside) ? -T : T);
+    } else if (inside) {
         const int64_t HW = (int64_t)rp.W * rp.H, o = (int64_t)py * rp.W + px;
         out[o] = C0 + T * rp.bg[0];
         out[HW + o] = C1 + T * rp.bg[1];
         out[2 * HW + o] = C2 + T * rp.bg[2];
     }
+}
+
+__global__ void split_count_kernel(int64_t nt, const uint2* __restrict__ ranges, int split, uint32_t* __restrict__ cnt) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t n = ranges[t].y - ranges[t].x;
+    cnt[t] = n <= (uint32_t)split ? 1u : (n + (uint32_t)split - 1u) / (uint32_t)split;
+}
+
+// Front-to-back composite of a split tile's segments, per pixel (the pixel
+// mapping of blend_quad_kernel), with the stopping segment blended again.
+__global__ void __launch_bounds__(256) split_composite_kernel(RasterParams rp, Geo geo,
+                                                              const uint32_t* __restrict__ vals,
+                                                              const uint2* __restrict__ ranges, SplitArgs sa) {
+    constexpr int NT = 256;
+    __shared__ float2 s_xy[NT];
+    __shared__ float4 s_co[NT];
+    __shared__ float4 s_rgb[NT];
+    const int tg = blockIdx.x;
+    const uint32_t ns = sa.cnt[tg];
+    if (ns <= 1) return;  // uniform per block
+    const int ntl = rp.gx * rp.gy;
+    const int view = tg / ntl;
+    const int tile = tg - view * ntl;
+    const int tx = tile % rp.gx, ty = tile / rp.gx;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int px = tx * 16 + (wv & 1) * 8 + (lane & 7), py = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
+    const bool inside = px < rp.W && py < rp.H;
+    const float fxp = (float)px, fyp = (float)py;
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    int stop = -1;  // the segment where this pixel's blend stops
+    const float4* p = sa.part + (int64_t)sa.off[tg] * NT + threadIdx.x;
+    if (inside) {
+        for (uint32_t s = 0; s < ns; s++) {
+            const float4 v = p[(int64_t)s * NT];
+            if (v.w < 0.0f || T * v.w < 0.0001f) {
+                stop = (int)s;
+                break;
+            }
+            C0 += T * v.x;
+            C1 += T * v.y;
+            C2 += T * v.z;
+            T *= v.w;
+        }
+    }
+    const uint2 full = ranges[tg];
+    for (uint32_t s = 0; s < ns; s++) {
+        if (!__syncthreads_or(stop == (int)s)) continue;
+        const uint32_t lo = full.x + s * (uint32_t)rp.split, hi = min(full.y, lo + (uint32_t)rp.split);
+        bool done = stop != (int)s;
+        for (uint32_t b0 = lo; b0 < hi; b0 += NT) {
+            if (__syncthreads_count(done) == NT) break;
+            const uint32_t i = b0 + threadIdx.x;
+            if (i < hi) {
+                const uint32_t g = vals[i];
+                s_xy[threadIdx.x] = geo.xy[g];
+                s_co[threadIdx.x] = geo.conic[g];
+                s_rgb[threadIdx.x] = geo.rgb[g];
+            }
+            __syncthreads();
+            const int m = (int)min((uint32_t)NT, hi - b0);
+            for (int j = 0; j < m && !done; j++) {
+                const float2 xy = s_xy[j];
+                const float4 co = s_co[j];
+                const float dx = xy.x - fxp, dy = xy.y - fyp;
+                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                if (power > 0.0f) continue;
+                const float alpha = fminf(0.99f, co.w * blend_exp(power));
+                if (alpha < 1.0f / 255.0f) continue;
+                const float test_T = T * (1.0f - alpha);
+                if (test_T < 0.0001f) {
+                    done = true;
+                    break;
+                }
+                const float4 c = s_rgb[j];
+                C0 += c.x * alpha * T;
+                C1 += c.y * alpha * T;
+                C2 += c.z * alpha * T;
+                T = test_T;
+            }
+            __syncthreads();
+        }
+    }
+    if (!inside) return;
+    float* __restrict__ out = rp.views[view].out;
+    const int64_t HW = (int64_t)rp.W * rp.H, o = (int64_t)py * rp.W + px;
+    out[o] = C0 + T * rp.bg[0];
+    out[HW + o] = C1 + T * rp.bg[1];
+    out[2 * HW + o] = C2 + T * rp.bg[2];
 }
 
 // ----------------------------------------------------------------- backward
@@ -965,6 +1097,35 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
     return launch_exclusive_scan(geo.touched, PV, geo_offsets(geo_ws, PV), d_counts, scan_ws, st);
 }
 
+// split workspace: cnt u32[nt], off u32[nt], total u32[64], scan workspace,
+// then the partials (float4 per pixel of every segment), each 256-byte aligned
+static size_t split_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t raster_split_bytes(int64_t nt, uint32_t n_rendered, int split) {
+    if (split <= 0) return 0;
+    const size_t segs = (size_t)nt + n_rendered / (uint32_t)split + 1;
+    return 2 * split_align(4 * (size_t)nt) + 256 + split_align(4 * scan_workspace_words(nt)) + segs * 256 * 16;
+}
+
+static SplitArgs split_carve(void* ws, int64_t nt, uint32_t n_rendered, int split) {
+    (void)n_rendered; (void)split;
+    char* p = (char*)ws;
+    SplitArgs a;
+    a.cnt = (const uint32_t*)p;
+    p += split_align(4 * (size_t)nt);
+    a.off = (const uint32_t*)p;
+    p += split_align(4 * (size_t)nt);
+    a.total = (const uint32_t*)p;
+    p += 256 + split_align(4 * scan_workspace_words(nt));
+    a.part = (float4*)p;
+    a.nt = (int)nt;
+    return a;
+}
+
+static uint32_t* split_scan_ws(void* ws, int64_t nt) {
+    return (uint32_t*)((char*)ws + 2 * split_align(4 * (size_t)nt) + 256);
+}
+
 size_t raster_record_bytes(int64_t PV) {
     return (size_t)PV * (4 + 4 + 4 + 4 + 4) + sizeof(uint32_t) * radix_workspace_words(PV) + 6 * 256;
 }
@@ -972,7 +1133,7 @@ size_t raster_record_bytes(int64_t PV) {
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint32_t dmin, int dbits, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
-                         hipStream_t st) {
+                         void* split_ws, hipStream_t st) {
     const int64_t PV = raster_records(rp);
     Geo geo = carve_geo(geo_ws, PV);
     const int ntiles = rp.gx * rp.gy;
@@ -1010,13 +1171,32 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
         hipLaunchKernelGGL(ranges_kernel, dim3(std::min<unsigned>(nblk(n_rendered, 256), 4096)), dim3(256), 0, st,
                            keys, d_total, ranges);
     }
-    if (rp.tile == 16) {
+    if (rp.tile == 16 && rp.split > 0 && n_rendered > 0 && split_ws) {
+        SplitArgs sa = split_carve(split_ws, nt, n_rendered, rp.split);
+        hipLaunchKernelGGL(split_count_kernel, dim3(nblk(nt, 256)), dim3(256), 0, st, nt, ranges, rp.split,
+                           const_cast<uint32_t*>(sa.cnt));
+        int rc = launch_exclusive_scan(sa.cnt, nt, const_cast<uint32_t*>(sa.off), const_cast<uint32_t*>(sa.total),
+                                       split_scan_ws(split_ws, nt), st);
+        if (rc) return rc;
+        const unsigned nb = (unsigned)(nt + n_rendered / (uint32_t)rp.split + 1);  // at least the segment count
         if (rp.unroll == 2)
-            hipLaunchKernelGGL(blend_quad_kernel<2>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+            hipLaunchKernelGGL((blend_quad_kernel<2, true>), dim3(nb), dim3(256), 0, st, rp, geo, vals, ranges, sa);
         else if (rp.unroll == 8)
-            hipLaunchKernelGGL(blend_quad_kernel<8>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+            hipLaunchKernelGGL((blend_quad_kernel<8, true>), dim3(nb), dim3(256), 0, st, rp, geo, vals, ranges, sa);
         else
-            hipLaunchKernelGGL(blend_quad_kernel<4>, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges);
+            hipLaunchKernelGGL((blend_quad_kernel<4, true>), dim3(nb), dim3(256), 0, st, rp, geo, vals, ranges, sa);
+        hipLaunchKernelGGL(split_composite_kernel, dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals, ranges, sa);
+    } else if (rp.tile == 16) {
+        const SplitArgs none{nullptr, nullptr, nullptr, nullptr, 0};
+        if (rp.unroll == 2)
+            hipLaunchKernelGGL((blend_quad_kernel<2, false>), dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals,
+                               ranges, none);
+        else if (rp.unroll == 8)
+            hipLaunchKernelGGL((blend_quad_kernel<8, false>), dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals,
+                               ranges, none);
+        else
+            hipLaunchKernelGGL((blend_quad_kernel<4, false>), dim3((unsigned)nt), dim3(256), 0, st, rp, geo, vals,
+                               ranges, none);
     } else if (rp.tile == 8)
         hipLaunchKernelGGL(blend_kernel<8>, dim3((unsigned)nt), dim3(64), 0, st, rp, geo, vals, ranges);
     else

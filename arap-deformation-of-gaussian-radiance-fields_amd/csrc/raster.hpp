@@ -39,6 +39,8 @@ struct RasterParams {
     float scale_mod;
     int use_rects, use_box;
     int unroll;            // Gaussians per compositing step of the 16x16 blend (2, 4, 8)
+    int split;             // 16x16 tiles holding more instances than this are blended in segments of
+                           // this many by separate workgroups and composited (0: one workgroup per tile)
     float boxmin[3], boxmax[3];
     const RasterView* views;  // device, V
     const int* seg;           // device, V+1, or null
@@ -103,10 +105,14 @@ int raster_preprocess(const RasterParams& rp, void* geo_ws, uint32_t* scan_ws, u
 // ntiles + tile). rec_ws holds raster_record_bytes(V*P) bytes, scan_ws is the
 // preprocess's scan workspace (sized for V*P).
 size_t raster_record_bytes(int64_t PV);
+// Workspace of the deep-tile split (rp.split > 0) for nt tiles and n_rendered
+// instances: per-tile segment counts and offsets, their scan and one
+// (colour, transmittance) float4 per pixel of every segment.
+size_t raster_split_bytes(int64_t nt, uint32_t n_rendered, int split);
 int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_rendered, const uint32_t* d_total,
                          uint32_t dmin, int dbits, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
                          uint32_t* vals_alt, uint32_t* sort_ws, uint2* ranges, void* rec_ws, uint32_t* scan_ws,
-                         hipStream_t st);
+                         void* split_ws, hipStream_t st);
 
 // Backward of the blend for every view of the last binning (geo, vals,
 // ranges as left by raster_bin_and_blend): per record gv the gradient of

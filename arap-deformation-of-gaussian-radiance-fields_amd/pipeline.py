@@ -47,7 +47,7 @@ from . import arapgs
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
                  stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
-                 aims_fn=None, sample_batch=1):
+                 aims_fn=None, sample_batch=1, node_cache=True):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
@@ -56,7 +56,9 @@ class DragPipeline:
         aims_fn(ctx, node_pos, aims): the frame's aims on the solve stream
         (default: the active nodes translated by delta, UpdateAimPosition).
         sample_batch: B, the sample chunks of the rolling update (1..8; more
-        than one needs split_samples, K = 8 and samples with features)."""
+        than one needs split_samples, K = 8 and samples with features).
+        node_cache: the deform side's blends through the LDS node-row cache
+        (arapgs_set_node_cache; identical results)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -78,6 +80,9 @@ class DragPipeline:
         self.frame = 0                 # frames whose deform side has been queued
         self.stream_c = torch.cuda.Stream(device=dev) if self.split else None
         self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
+        for c in (self.ctx_b, self.ctx_c):
+            if c is not None:
+                c.set_node_cache(node_cache)
         self.ev_c = [torch.cuda.Event() for _ in range(self.R)]
         self.ev_c_recorded = [False] * self.R
         self.st = state

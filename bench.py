@@ -188,8 +188,10 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
         nb = B["cfg"].get("sample_batch", 1) if sample_batch is None else sample_batch
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
-                            stream_b=streams[1].torch if streams else None, sample_batch=nb)
+                            stream_b=streams[1].torch if streams else None, sample_batch=nb,
+                            node_cache=bool(B["cfg"].get("node_cache", 1)))
         pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
+        pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
         pipe.run(warmup)
         torch.cuda.synchronize()
         stamps = []
@@ -281,6 +283,8 @@ def apply_config(B, cfg):
     B["solver"].set_cg_block(cfg.get("cg_block", 256))
     B["solver"].set_cg_graphs(cfg.get("cg_graphs", 0))
     ctx.set_blend_unroll(cfg["blend_unroll"])
+    ctx.set_blend_split(cfg.get("blend_split", 0))
+    ctx.set_node_cache(cfg.get("node_cache", 1))
     B["cfg"] = dict(cfg)
 
 
@@ -383,9 +387,11 @@ def record_frames(B, steps, warmup):
     B["solver"].set_warm_start(B["cfg"]["warm"])
     B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
     pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
-                     B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1))
+                     B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1),
+                     node_cache=bool(B["cfg"].get("node_cache", 1)))
     pipe.rec = []
     pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
+    pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
     pipe.run(warmup + steps)
     torch.cuda.synchronize()
     same = all(torch.equal(final_timed[k], B["state"][k]) for k in final_timed)
@@ -629,8 +635,12 @@ def large_leg(args, tuning):
         if v is not None:
             st[k].copy_(v)
     torch.cuda.synchronize()
+    # the drag workload's tuned schedule (identical results)
     pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
-                        aims_fn=aims_fn)
+                        aims_fn=aims_fn, sample_batch=tuning.get("sample_batch", 1),
+                        node_cache=bool(tuning.get("node_cache", 1)))
+    pipe.ctx_b.set_blend_unroll(tuning.get("blend_unroll", 4))
+    pipe.ctx_b.set_blend_split(tuning.get("blend_split", 0))
     pipe.run(args.warmup)
     torch.cuda.synchronize()
     stamps = []

@@ -317,6 +317,12 @@ int arapgs_grid_rebin(arapgs_ctx* ctx, arapgs_grid* grid, const float* aabbs, in
  * pairs per chunk of later arapgs_grid_build calls on this context (0: the
  * default 2^28). The lists do not depend on it. */
 int arapgs_set_grid_chunk(arapgs_ctx* ctx, int64_t pairs);
+/* Tuning switch with no counterpart in the reference: the K = 8 blends (the
+ * six-point update's end points, arapgs_blend_points, the frame's sample and
+ * node blends) read node rows through a per-workgroup LDS cache (1, default)
+ * or gather every row from the global table (0). Results are identical bit
+ * for bit. */
+int arapgs_set_node_cache(arapgs_ctx* ctx, int on);
 /* Device views of a grid: prefix int64[G^3] (inclusive per-cell counts, the
  * reference's grid_gs_prefix_sum widened to 64 bits), list int[n_pairs]
  * (grided_gs_idx), valid int[n_valid] (valid_grid_idx), samples
@@ -495,6 +501,18 @@ int arapgs_render_segments(arapgs_ctx* ctx, const arapgs_render_desc* descs_h, i
  * their alphas evaluated together before being composited in order. Images do
  * not depend on it. No counterpart in the reference (set by the autotuner). */
 int arapgs_set_blend_unroll(arapgs_ctx* ctx, int unroll);
+/* Tuning of the 16x16 blend with no counterpart in the reference: a tile whose
+ * list holds more than `entries` instances (a multiple of 256; 0 = never, the
+ * default) is blended in segments of `entries` by separate workgroups, each
+ * from transmittance 1, and the segments' (colour, transmittance) are then
+ * composited front to back per pixel. The deepest tiles otherwise set the
+ * blend's length. Tiles of one segment are unchanged bit for bit; in a split
+ * tile the segment where a pixel's transmittance falls below 1e-4 is blended
+ * again from the true incoming transmittance, so the contributors are those of
+ * the tile-by-tile blend and only the association of the sums and products
+ * differs (float rounding). Renders that ask for `surface` and the adjoint
+ * chain always blend tile by tile. */
+int arapgs_set_blend_split(arapgs_ctx* ctx, int entries);
 
 /* Debug view of the last arapgs_render on this context: sorted keys
  * (u64, tile << 32 | depth bits) and values (Gaussian ids) copied to host

@@ -185,3 +185,48 @@ def test_node_quats_and_sample_sh(ctx, scene):
     fd = T(feat.copy())
     ctx.sample_sh(T(sw32), T(sidx[:, :8].astype(np.int32)), T(rq), T(st), fd)
     np.testing.assert_allclose(H(fd), ref, atol=1e-5)
+
+
+@pytest.mark.parametrize("M", [300, 2048])
+def test_six_point_node_cache_bit_identical(ctx, M):
+    """The six-point update and the K-node blend through the per-workgroup LDS
+    node-row cache equal the global-gather forms bit for bit: 200k Gaussians in grid cell order (the
+    layout the cache is built for) and in a random order (many distinct nodes
+    per workgroup, so many ids find their slot taken and read the global table),
+    with static Gaussians; M = 2048 has many ids per cache slot."""
+    import torch
+    from arapgs_amd import scenes as S
+    rng = np.random.default_rng(31)
+    sc = S.capsule_scene(3, n=200_000, n_nodes=M)
+    g = sc.gaussians
+    nodes = g.pos[rng.choice(g.n, M, replace=False)]
+    for order in ("cells", "random"):
+        if order == "cells":
+            a7 = ctx.scene_aabb(T(g.pos), 64)
+            perm = H(ctx.grid_reorder(T(g.pos), a7, 64)).astype(np.int64)
+        else:
+            perm = rng.permutation(g.n)
+        gg = g.take(perm)
+        ends = H(ctx.end_points(T(gg.pos), T(gg.rot), T(gg.scale)))
+        idx, w = ctx.knn(T(nodes), T(ends.reshape(-1, 3)), 8)
+        idx = idx[:, :8].contiguous()
+        rot9, t3 = random_node_xf(M, rng, rot_deg=15.0)
+        tab = T(table_from(nodes, rot9, t3))
+        st = T((rng.random(g.n) < 0.05).astype(np.uint8))
+        outs = []
+        for cache in (False, True):
+            ctx.set_node_cache(cache)
+            bufs = [T(ends.copy()), T(gg.pos.copy()), T(gg.rot.copy()), T(gg.scale.copy()), T(gg.sh.copy())]
+            ctx.six_point_update(tab, idx, w, bufs[0], T(gg.scale.copy()), st, *bufs[1:])
+            torch.cuda.synchronize()
+            outs.append([H(b) for b in bufs])
+        # the plain K-node blend of the same points (arapgs_blend_points)
+        pts = T(ends.reshape(-1, 3).copy())
+        blends = []
+        for cache in (False, True):
+            ctx.set_node_cache(cache)
+            blends.append(H(ctx.blend_points(pts, idx, w, tab)))
+        ctx.set_node_cache(True)
+        for a, b in zip(*outs):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32)), order
+        assert np.array_equal(blends[0].view(np.int32), blends[1].view(np.int32)), order
