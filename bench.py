@@ -68,8 +68,8 @@ HBM_PEAK_GBS = 8000.0
 SIX_POINT_BYTES_PER_GAUSSIAN = 981
 SAMPLE_BLEND_BYTES_PER_SAMPLE = 88
 SAMPLE_SH_BYTES_PER_SAMPLE = 448
-KERNELS = (("six_point", "six_point_kernel<8, true> (fused end-point blend + polar + SH)", 0),
-           ("sample_blend", "blend_points_kernel<8, true> (sample positions)", 1),
+KERNELS = (("six_point", "six_point_kernel<8, true, true> (fused end-point blend + polar + SH, LDS node cache)", 0),
+           ("sample_blend", "blend_points_kernel<8, true, true> (sample positions, LDS node cache)", 1),
            ("sample_sh", "sample_sh_kernel<8> (slerp of node rotations + SH rotation of the sample features)", 2))
 
 
