@@ -75,6 +75,7 @@ EXPORTS = {
     "arapgs_set_node_cache": ([P, C.c_int], C.c_int),
     "arapgs_set_blend_split": ([P, C.c_int], C.c_int),
     "arapgs_stream_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_stream_create_mask": ([C.c_int, P, C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_stream_destroy": ([P], C.c_int),
     "arapgs_version": ([], C.c_char_p),
     "arapgs_fps": ([P, P, C.c_int64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
@@ -120,6 +121,7 @@ EXPORTS = {
     "arapgs_solver_set_host_checks": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_cg_predict": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_solver_reset": ([P], C.c_int),
     "arapgs_solver_last_stats": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -194,12 +196,20 @@ class CuStream:
     """A HIP stream limited to compute units [cu_begin, cu_begin + cu_count)
     (all of them when cu_count is 0), usable as a torch stream (.torch)."""
 
-    def __init__(self, device=0, cu_begin=0, cu_count=0):
+    def __init__(self, device=0, cu_begin=0, cu_count=0, cus=None):
+        """cus: an explicit list of compute-unit indices (instead of a range)."""
         import torch
         h = P()
-        rc = lib().arapgs_stream_create(device, cu_begin, cu_count, C.byref(h))
+        if cus is not None:
+            words = (max(cus) + 32) // 32 if cus else 1
+            mask = (C.c_uint32 * words)()
+            for cu in cus:
+                mask[cu // 32] |= 1 << (cu % 32)
+            rc = lib().arapgs_stream_create_mask(device, C.cast(mask, P), words, C.byref(h))
+        else:
+            rc = lib().arapgs_stream_create(device, cu_begin, cu_count, C.byref(h))
         if rc != 0:
-            raise ArapgsError(f"arapgs_stream_create({cu_begin}, {cu_count}) failed ({rc})")
+            raise ArapgsError(f"arapgs_stream_create({cu_begin}, {cu_count}, {cus}) failed ({rc})")
         self.h = h
         self.torch = torch.cuda.ExternalStream(h.value, device=torch.device("cuda", device))
 
@@ -635,7 +645,7 @@ class Solver:
         self.ctx._check(lib().arapgs_solver_set_coarse_refresh(self.h, frames))
 
     def set_cg_block(self, threads):
-        """Workgroup size of the CG operator kernel (64, 128, 256)."""
+        """Workgroup size of the CG operator kernel (64, 128, 256, 512; both CG kernels in cg_mode 2)."""
         if lib().arapgs_solver_set_cg_block(self.h, int(threads)) != 0:
             raise ArapgsError(f"set_cg_block({threads}) failed")
 
@@ -673,8 +683,13 @@ class Solver:
         if lib().arapgs_solver_reset(self.h) != 0:
             raise ArapgsError("solver reset failed")
 
+    def set_cg_predict(self, on=True):
+        """First CG host check of each GN iteration at the previous solve's count (same results)."""
+        self.ctx._check(lib().arapgs_solver_set_cg_predict(self.h, int(bool(on))))
+
     def set_cg_mode(self, mode):
-        """0 = three kernels per CG iteration (default), 1 = persistent cooperative PCG kernel."""
+        """0 = three kernels per CG iteration (default), 1 = persistent cooperative PCG kernel,
+        2 = two kernels per CG iteration (coarse restriction by recurrence)."""
         self.ctx._check(lib().arapgs_solver_set_cg_mode(self.h, mode))
 
     def deform_frame(self, fb, aims, iters, stop_rule=True, energies=None, iters_done=None):

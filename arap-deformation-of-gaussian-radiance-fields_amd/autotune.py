@@ -8,13 +8,20 @@ its tolerance:
   refresh       solves between rebuilds of the coarse operator
   host_checks   CG iterations between host reads of the convergence flag
   warm          CG warm start from the previous frame's GN steps (1) or from 0
-  cg_block      workgroup size of the CG operator kernel: 256, 128 or 64 threads
+  cg_block      workgroup size of the CG operator kernel: 256, 128, 64 or 512 threads
   cg_graphs     CG chunks between host checks as captured HIP graphs, checked
                 one chunk behind (1) or launched kernel by kernel (0)
+  cg_mode       CG iterations as three kernels (0) or two, with the coarse
+                restriction carried by a recurrence (2)
+  cg_predict    first CG host check of each GN iteration after as many CG
+                iterations as the previous frame needed there (1) or after
+                host_checks (0)
   node_cache    K = 8 blends through the per-workgroup LDS node-row cache (1) or
                 with every node row gathered from the global table (0)
   blend_split   16x16 tiles with more instances than this blended in segments by
                 separate workgroups and composited (0: one workgroup per tile)
+  deform_skip   pipelined frames: CUs [0, deform_skip) left to the solve by the
+                deform and samples streams (CU-masked streams; 0: all CUs shared)
 
 `tune` runs a coordinate descent from the defaults: for each parameter in turn
 it times every candidate value (the others held at their current best) with a
@@ -27,13 +34,14 @@ import json
 import os
 
 DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0,
-                node_cache=1, blend_split=0)
+                cg_mode=0, cg_predict=0, node_cache=1, blend_split=0, deform_skip=0)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
-                  host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64), cg_graphs=(0, 1),
-                  node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096))
-# host_checks is revisited after cg_graphs: the graphs' best chunk is shorter
-ORDER = ("warm", "aggregates", "refresh", "host_checks", "cg_graphs", "host_checks", "cg_block", "tile",
-         "blend_unroll", "blend_split", "node_cache")
+                  host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),
+                  cg_mode=(0, 2), cg_predict=(0, 1), node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096),
+                  deform_skip=(0, 16, 32, 64))
+# host_checks is revisited after cg_graphs: their best chunk can differ
+ORDER = ("warm", "cg_mode", "cg_predict", "aggregates", "refresh", "host_checks", "cg_graphs", "host_checks",
+         "cg_block", "tile", "blend_unroll", "blend_split", "node_cache", "deform_skip")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

@@ -67,7 +67,9 @@ struct arapgs_ctx {
     int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
     int* r_seg_h = nullptr;           // pinned host staging for it
     int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (8 ints;
-                                      // [4], [5]: the flags of consecutive CG chunk graphs)
+                                      // [3]: the CG iteration count at a host check; [4], [5]: the flags
+                                      // of consecutive CG chunk graphs; [6]: the unread flag of the
+                                      // two-kernel CG's final test)
     hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
@@ -85,7 +87,9 @@ struct arapgs_solver {
     int device = 0;  // the creating context's device; a solver never touches its context again
     SolverWork w;
     int host_check_every = 0;   // 0: device-side convergence only
-    int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG
+    int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG, 2: two launches per iteration
+    bool cg_predict = false;    // first host check of each GN iteration's CG at the previous solve's count
+    int cg_pred[GN_MAX_ITERS] = {};  // those counts (0: none yet)
     int coarse_refresh = 8;     // frames between coarse-operator rebuilds
     int64_t solves = 0;
     SolverBatch one;            // this solver as a launch group of one
@@ -228,6 +232,17 @@ int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_o
         for (int cu = cu_begin; cu < cu_begin + cu_count; cu++) mask[cu / 32] |= 1u << (cu % 32);
         e = hipExtStreamCreateWithCUMask(&st, (uint32_t)mask.size(), mask.data());
     }
+    if (e != hipSuccess) return -1000 - (int)e;
+    *stream_out = st;
+    return 0;
+}
+
+int arapgs_stream_create_mask(int device, const uint32_t* mask_h, int words, void** stream_out) {
+    if (!stream_out || !mask_h || words < 1) return -1;
+    *stream_out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    hipStream_t st = nullptr;
+    const hipError_t e = hipExtStreamCreateWithCUMask(&st, (uint32_t)words, mask_h);
     if (e != hipSuccess) return -1000 - (int)e;
     *stream_out = st;
     return 0;
@@ -413,7 +428,7 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every) {
 }
 
 int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode) {
-    if (!s || mode < 0 || mode > 1) return -1;
+    if (!s || mode < 0 || mode > 2) return -1;
     s->cg_mode = mode;
     return 0;
 }
@@ -459,11 +474,19 @@ int arapgs_solver_reset(arapgs_solver* s) {
     s->w.pending = false;
     s->w.have_coarse = false;
     s->solves = 0;
+    for (int& v : s->cg_pred) v = 0;
+    return 0;
+}
+
+int arapgs_solver_set_cg_predict(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    s->cg_predict = on != 0;
+    for (int& v : s->cg_pred) v = 0;
     return 0;
 }
 
 int arapgs_solver_set_cg_block(arapgs_solver* s, int threads) {
-    if (!s || (threads != 64 && threads != 128 && threads != 256)) return -1;
+    if (!s || (threads != 64 && threads != 128 && threads != 256 && threads != 512)) return -1;
     if (threads == s->w.cg_block) return 0;
     (void)hipSetDevice(s->device);
     // the batch entries are rebuilt; nothing may still read the old ones
@@ -506,12 +529,17 @@ namespace {
 // group of solvers: every phase is one launch for all scenes; the host reads
 // the groups' convergence flags every `every` CG iterations and after every
 // GN iteration (every == 0: device-side flags only, all launches queued).
-int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, int iters, int stop_rule) {
+// pred (when not null): per GN iteration, the CG iteration count of the
+// previous solve; the first host check of that GN iteration's CG comes after
+// that many iterations (then every `every`), and the count found is stored back.
+int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, int iters, int stop_rule,
+           int* pred = nullptr) {
     int rc = check_launch(c, solver_frame_setup(B, refresh, c->stream), "gn_solve setup");
     if (rc) return rc;
     // the flag kernel writes into pinned host memory: no device-to-host copy
-    auto host_flag = [&](int which, int* out) -> int {
-        int r = check_launch(c, solver_batch_flag(B, which, c->stream, c->flag_h), "gn_solve");
+    B.fused = cg_mode == 2;
+    auto host_flag = [&](int which, int* out, int k_done) -> int {
+        int r = check_launch(c, solver_batch_flag(B, which, c->stream, c->flag_h, k_done, c->flag_h + 3), "gn_solve");
         if (r) return r;
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
@@ -553,14 +581,28 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
                     }
                 }
             } else {
+                bool seen_done = false;
+                const bool predict = pred && every > 0 && it < GN_MAX_ITERS;
+                int next_check = predict && pred[it] > 0 ? std::min(pred[it], cap) : every;
                 for (int k = 0; k < cap; k++) {
                     rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
                     if (rc) return rc;
-                    if (every > 0 && (k + 1) % every == 0) {
+                    if (every > 0 && k + 1 == next_check) {
                         int done = 0;
-                        if ((rc = host_flag(0, &done))) return rc;
-                        if (done) break;
+                        if ((rc = host_flag(0, &done, k + 1))) return rc;
+                        if (done) {
+                            seen_done = true;
+                            break;
+                        }
+                        next_check += every;
                     }
+                }
+                if (predict) pred[it] = seen_done ? std::max(1, (int)*(volatile int*)(c->flag_h + 3)) : 0;
+                // the two-kernel form tests convergence at the start of the next
+                // apply: record the test after the last iteration queued
+                if (B.fused && !seen_done) {
+                    rc = check_launch(c, solver_batch_flag(B, 0, c->stream, c->flag_h + 6, cap), "gn_solve cg");
+                    if (rc) return rc;
                 }
             }
         }
@@ -568,7 +610,7 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
         if (rc) return rc;
         if (every > 0) {
             int done = 0;
-            if ((rc = host_flag(1, &done))) return rc;
+            if ((rc = host_flag(1, &done, 0))) return rc;
             if (done) break;
         }
     }
@@ -595,7 +637,9 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     if (rc) return rc;
     const bool refresh = (s->solves % std::max(1, s->coarse_refresh)) == 0;
     s->solves++;
-    if ((rc = run_gn(c, s->one, s->host_check_every, s->cg_mode, refresh, iters, stop_rule))) return rc;
+    if ((rc = run_gn(c, s->one, s->host_check_every, s->cg_mode, refresh, iters, stop_rule,
+                     s->cg_predict ? s->cg_pred : nullptr)))
+        return rc;
     SolverOutPtrs out = {};
     out.x[0] = x_out;
     out.energies[0] = energies;

@@ -845,6 +845,297 @@ __global__ void __launch_bounds__(256) coarse_prolong_kernel(const SolverEntry* 
     }
 }
 
+// ------------------------------------------------ two kernels per iteration
+// The same PCG with the coarse restriction carried by a recurrence, so that
+// the update and the coarse phases become one kernel. With q_k = A p_k and
+// r_{k+1} = r_k - alpha_k q_k, the restricted residual obeys
+//   rc_{k+1} = P^T r_{k+1} = rc_k - alpha_k P^T q_k,
+// and P^T q_k is formed where q_k is: in the apply, whose workgroups own one
+// aggregate each. The coarse product of iteration k then needs only rc_k,
+// P^T q_k (both complete after the apply) and alpha_k, so it runs in the
+// same launch as the node-local update:
+//   apply   [convergence test on |r_k|^2] beta_k; s = A z; p, q; partial p.q
+//           per aggregate; (P^T q)_a                    (workgroup a, aggregate a)
+//   update  alpha; rc_{k+1} = rc_k - alpha P^T q (every workgroup forms the
+//           whole vector for its coarse rows, workgroup a stores its own part);
+//           zc_a = (Ac^-1 rc_{k+1})_a; h += alpha p; r -= alpha q; partial |r|^2;
+//           z = Binv r + P_a zc_a; partial r.z
+// rc ping-pongs between two halves of the rc buffer (iteration k reads half
+// k & 1) and P^T q sits in its third part. The convergence test of |r_k|^2
+// opens apply k (every workgroup sums the partials itself and takes the same
+// decision; workgroup 0 records it) and is repeated by the host-check flag
+// kernel, with the same reduction, after a chunk of iterations.
+
+// Sums of NS arrays of n FP64 partials in an order that does not depend on the
+// workgroup size: the first wave adds lane-strided runs, then a shuffle tree.
+// Every thread gets the results (through sh, NS doubles).
+template <int NS>
+__device__ __forceinline__ void wave0_sums(const double* const (&p)[NS], int n, double* sh, double (&out)[NS]) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            double v = 0.0;
+            for (int i = threadIdx.x; i < n; i += 64) v += p[j][i];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (threadIdx.x == 0) sh[j] = v;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NS; j++) out[j] = sh[j];
+    __syncthreads();
+}
+
+// The convergence test after k CG iterations from |r_k|^2; workgroup 0 (or the
+// flag kernel) records it. Returns whether the CG is done.
+__device__ __forceinline__ bool fused_cg_test(const SolverEntry& E, double rr, int k, bool record) {
+    CgScalars* cs = E.cg;
+    const bool stop = rr <= cs->rr_stop || k >= cs->max_iters;
+    if (record && threadIdx.x == 0) {
+        cs->rr = rr;
+        cs->iters = k;
+        if (stop) *E.cg_done = 1;
+    }
+    return stop;
+}
+
+// BS: the CG block size (BS / 16 member nodes of the aggregate per pass).
+template <int KT, int BS>
+__global__ void __launch_bounds__(BS) cg_apply_agg_kernel(const SolverEntry* __restrict__ B, int k) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    const int a = blockIdx.x;
+    if (a >= S.n_agg || *E.cg_done) return;
+    __shared__ double shd[BS / 64];
+    __shared__ float sred[BS / LANES][12];
+    const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
+    const double* rz_prev = E.rzp + ((k + 1) & 1) * S.n_agg;
+    const bool first = k == 0;
+    float beta = 0.0f;
+    if (!first) {
+        const double* const ps[3] = {E.partial2, rz_cur, rz_prev};
+        double sums[3];
+        wave0_sums<3>(ps, S.n_agg, shd, sums);
+        if (fused_cg_test(E, sums[0], k, a == 0)) return;
+        beta = sums[2] != 0.0 ? (float)(sums[1] / sums[2]) : 0.0f;
+    }
+    const float* __restrict__ x = E.x;
+    const float* __restrict__ z = E.z;
+    float* __restrict__ p = E.p;
+    float* __restrict__ q = E.q;
+    const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
+    const float* ca = S.agg_center + 3 * a;
+    float rcl[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) rcl[c] = 0.0f;
+    double pq = 0.0;
+    for (int m = beg + grp; m < end; m += BS / LANES) {
+        const int fi = S.agg_member[m];
+        float acc[12];
+#pragma unroll
+        for (int c = 0; c < 12; c++) acc[c] = 0.0f;
+        const int ni = owner_items(S, fi);
+        for (int it = lane; it < ni; it += LANES) {
+            owner_item<KT>(
+                S, fi, it, x, z,
+                [&](float (&Ji)[6][9], float (&Jv)[6]) {
+#pragma unroll
+                    for (int r = 0; r < 6; r++)
+#pragma unroll
+                        for (int c = 0; c < 9; c++) acc[c] += Ji[r][c] * Jv[r];
+                },
+                [&](float (&a3)[3], float bb, float (&Jv)[3]) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) acc[3 * c + r] += a3[c] * Jv[r];
+                        acc[9 + r] += bb * Jv[r];
+                    }
+                });
+        }
+        float mine = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 12; c++) {
+            const float t = group_reduce16(acc[c]);
+            if (lane == c) mine = t;
+        }
+        float qn = 0.0f;
+        if (lane < 12) {
+            const int64_t o = 12 * (int64_t)fi + lane;
+            const float pn = first ? z[o] : z[o] + beta * p[o];
+            qn = first ? mine : mine + beta * q[o];
+            p[o] = pn;
+            q[o] = qn;
+            pq += (double)pn * qn;
+        }
+        // (P^T q)_a: the R part (9) passes through; q_t reaches b (identity)
+        // and A(r, c') with weight (g_j - c_a)_c'
+        float qv[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) qv[j] = __shfl(qn, j, LANES);
+        if (lane == 0) {
+            const int node = S.free_nodes[fi];
+            const float dj[3] = {S.g[3 * node] - ca[0], S.g[3 * node + 1] - ca[1], S.g[3 * node + 2] - ca[2]};
+#pragma unroll
+            for (int c = 0; c < 9; c++) rcl[c] += qv[c];
+#pragma unroll
+            for (int r3 = 0; r3 < 3; r3++) {
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) rcl[3 * cc + r3] += dj[cc] * qv[9 + r3];
+                rcl[9 + r3] += qv[9 + r3];
+            }
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 12; c++) sred[grp][c] = rcl[c];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        float t = 0.0f;
+        for (int g2 = 0; g2 < BS / LANES; g2++) t += sred[g2][threadIdx.x];
+        E.rc[2 * (int64_t)S.nc_pad + 12 * a + threadIdx.x] = t;
+    }
+    const double t = block_reduce_d(pq, shd);
+    if (threadIdx.x == 0) E.partial[a] = t;
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) cg_update_coarse_kernel(const SolverEntry* __restrict__ B, int k) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    const int a = blockIdx.x;
+    if (a >= S.n_agg || *E.cg_done) return;
+    __shared__ double shd[BS / 64];
+    __shared__ float zc[12];
+    __shared__ float zred[BS / 64][12];
+    const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
+    double* rz_next = E.rzp + ((k + 1) & 1) * S.n_agg;
+    float alpha;
+    {
+        const double* const ps[2] = {rz_cur, E.partial};
+        double sums[2];
+        wave0_sums<2>(ps, S.n_agg, shd, sums);
+        alpha = sums[1] != 0.0 ? (float)(sums[0] / sums[1]) : 0.0f;
+    }
+    const int nc = 12 * S.n_agg;
+    const int ld = S.nc_pad;
+    const float* rc_cur = E.rc + (k & 1) * (int64_t)ld;
+    float* rc_next = E.rc + ((k + 1) & 1) * (int64_t)ld;
+    const float* ptq = E.rc + 2 * (int64_t)ld;
+    // zc_a = 12 rows of Ac^-1 times rc_{k+1}, formed on the fly
+    {
+        float part[12];
+#pragma unroll
+        for (int row = 0; row < 12; row++) part[row] = 0.0f;
+        const float* A0 = E.acinv + (int64_t)(12 * a) * ld;
+#pragma unroll 4
+        for (int j = threadIdx.x; j < nc; j += BS) {
+            const float rj = rc_cur[j] - alpha * ptq[j];
+#pragma unroll
+            for (int row = 0; row < 12; row++) part[row] += A0[(int64_t)row * ld + j] * rj;
+        }
+        const int wl = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int row = 0; row < 12; row++) {
+            float v = part[row];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (wl == 0) zred[wv][row] = v;
+        }
+        if (threadIdx.x < 12) {
+            const int j = 12 * a + threadIdx.x;
+            rc_next[j] = rc_cur[j] - alpha * ptq[j];
+        }
+        __syncthreads();
+        if (threadIdx.x < 12) {
+            float t = 0.0f;
+            for (int w = 0; w < BS / 64; w++) t += zred[w][threadIdx.x];
+            zc[threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+    float* __restrict__ hsol = E.hsol;
+    float* __restrict__ r = E.r;
+    float* __restrict__ z = E.z;
+    const float* __restrict__ p = E.p;
+    const float* __restrict__ q = E.q;
+    const float* __restrict__ binv = E.binv;
+    const float* ca = S.agg_center + 3 * a;
+    const int grp = threadIdx.x / LANES, c = threadIdx.x % LANES;
+    double rr = 0.0, rz = 0.0;
+    const int beg = S.agg_ptr[a], end = S.agg_ptr[a + 1];
+    for (int m = beg + grp; m < end; m += BS / LANES) {
+        const int fi = S.agg_member[m];
+        float rv = 0.0f;
+        if (c < 12) {
+            const int64_t o = 12 * (int64_t)fi + c;
+            hsol[o] += alpha * p[o];
+            r[o] -= alpha * q[o];
+            rv = r[o];
+            rr += (double)rv * rv;
+        }
+        float rn[12];
+#pragma unroll
+        for (int j = 0; j < 12; j++) rn[j] = __shfl(rv, j, LANES);
+        if (c < 12) {
+            float zz = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 12; j++) zz += binv[144 * (int64_t)fi + c * 12 + j] * rn[j];
+            float add;
+            if (c < 9) {
+                add = zc[c];
+            } else {
+                const int node = S.free_nodes[fi];
+                const int r3 = c - 9;
+                const float dj[3] = {S.g[3 * node] - ca[0], S.g[3 * node + 1] - ca[1], S.g[3 * node + 2] - ca[2]};
+                add = zc[9 + r3] + zc[r3] * dj[0] + zc[3 + r3] * dj[1] + zc[6 + r3] * dj[2];
+            }
+            const float zn = zz + add;
+            z[12 * (int64_t)fi + c] = zn;
+            rz += (double)zn * rv;
+        }
+    }
+    const double trr = block_reduce_d(rr, shd);
+    if (threadIdx.x == 0) E.partial2[a] = trr;
+    const double trz = block_reduce_d(rz, shd);
+    if (threadIdx.x == 0) rz_next[a] = trz;
+}
+
+// flag = 1 when every scene's CG is done after k iterations of the two-kernel
+// form: the convergence test of apply k, run here first so that the host sees
+// it at the end of a chunk (apply k repeats it with the same result).
+// iters (when not null): the largest CG iteration count over the scenes.
+__global__ void fused_flag_kernel(const SolverEntry* __restrict__ B, int n, int k, int* __restrict__ flag,
+                                  int* __restrict__ iters) {
+    __shared__ double shd[4];
+    __shared__ int s_all, s_iters;
+    if (threadIdx.x == 0) {
+        s_all = 1;
+        s_iters = 0;
+    }
+    for (int s = 0; s < n; s++) {
+        const SolverEntry E = load_entry(B + s);  // launched with gridDim.z == 1
+        if (*E.cg_done) {
+            if (threadIdx.x == 0) s_iters = max(s_iters, E.cg->iters);
+            continue;
+        }
+        const double* const ps[1] = {E.partial2};
+        double sums[1];
+        wave0_sums<1>(ps, E.S.n_agg, shd, sums);
+        const bool stop = fused_cg_test(E, sums[0], k, true);
+        if (threadIdx.x == 0) {
+            if (!stop) s_all = 0;
+            s_iters = max(s_iters, k);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *flag = s_all;
+        if (iters) *iters = s_iters;
+    }
+}
+
 // ------------------------------------------------------------ grid barrier
 // Generation barrier for a cooperative launch (all workgroups co-resident).
 // Every thread first fences its own global writes at device scope (a fence
@@ -1546,11 +1837,17 @@ __global__ void set_frame_kernel(SolverEntry* __restrict__ B, int n, FramePtrs f
 // flag = 1 when every scene's cg_done (which 0) or gn_done (which 1) is set.
 // flag is pinned host memory: the host reads it once the stream reaches this
 // point, without a device-to-host copy.
-__global__ void batch_flag_kernel(const SolverEntry* __restrict__ B, int n, int which, int* __restrict__ flag) {
+// iters (when not null, which 0): the largest CG iteration count over the scenes.
+__global__ void batch_flag_kernel(const SolverEntry* __restrict__ B, int n, int which, int* __restrict__ flag,
+                                  int* __restrict__ iters) {
     if (threadIdx.x != 0) return;
-    int all = 1;
-    for (int s = 0; s < n; s++) all &= (which ? *B[s].gn_done : *B[s].cg_done) != 0 ? 1 : 0;
+    int all = 1, most = 0;
+    for (int s = 0; s < n; s++) {
+        all &= (which ? *B[s].gn_done : *B[s].cg_done) != 0 ? 1 : 0;
+        if (iters) most = max(most, B[s].cg->iters);
+    }
     *flag = all;
+    if (iters) *iters = most;
 }
 
 // Per-scene outputs: x, the energies of the iterations run, the iteration count
@@ -1722,18 +2019,25 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
     return 0;
 }
 
-int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h) {
-    hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
-                       which, flag_h);
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done, int* iters_h) {
+    if (which != 0) iters_h = nullptr;
+    if (which == 0 && B.fused)
+        hipLaunchKernelGGL(fused_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
+                           k_done, flag_h, iters_h);
+    else
+        hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
+                           which, flag_h, iters_h);
     HIP_RET(hipGetLastError());
     return 0;
 }
 
 int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, int* const flag_h[2]) {
-    if (B.graph_every != every || B.graph_cap != cap || B.graph_flag[0] != flag_h[0] || B.graph_flag[1] != flag_h[1]) {
+    if (B.graph_every != every || B.graph_cap != cap || B.graph_flag[0] != flag_h[0] || B.graph_flag[1] != flag_h[1] ||
+        B.graph_fused != B.fused) {
         solver_batch_drop_graphs(B);
         B.graph_every = every;
         B.graph_cap = cap;
+        B.graph_fused = B.fused;
         B.graph_flag[0] = flag_h[0];
         B.graph_flag[1] = flag_h[1];
     }
@@ -1741,8 +2045,9 @@ int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, i
     if (!B.cg_graphs[j]) {
         HIP_RET(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = 0;
-        for (int k = j * every; k < std::min(cap, (j + 1) * every) && !rc; k++) rc = solver_cg_iteration(B, k, st);
-        if (!rc) rc = solver_batch_flag(B, 0, st, flag_h[j & 1]);
+        const int k_end = std::min(cap, (j + 1) * every);
+        for (int k = j * every; k < k_end && !rc; k++) rc = solver_cg_iteration(B, k, st);
+        if (!rc) rc = solver_batch_flag(B, 0, st, flag_h[j & 1], k_end);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(st, &g);
         if (rc || e != hipSuccess) {
@@ -1845,11 +2150,13 @@ static int launch_apply(const SolverBatch& B, unsigned nz, hipStream_t st, int k
     if (B.kt == 8) {
         if (bs == 64) ARAPGS_APPLY(8, 64);
         else if (bs == 128) ARAPGS_APPLY(8, 128);
-        else ARAPGS_APPLY(8, 256);
+        else if (bs == 256) ARAPGS_APPLY(8, 256);
+        else ARAPGS_APPLY(8, 512);
     } else {
         if (bs == 64) ARAPGS_APPLY(0, 64);
         else if (bs == 128) ARAPGS_APPLY(0, 128);
-        else ARAPGS_APPLY(0, 256);
+        else if (bs == 256) ARAPGS_APPLY(0, 256);
+        else ARAPGS_APPLY(0, 512);
     }
 #undef ARAPGS_APPLY
     HIP_RET(hipGetLastError());
@@ -1880,6 +2187,23 @@ int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st) {
 int solver_cg_iteration(SolverBatch& B, int k, hipStream_t st) {
     const SolverEntry* E = B.entries;
     const unsigned nz = (unsigned)B.w.size();
+    if (B.fused) {
+        const dim3 grid(B.max_agg, 1, nz);
+        const int bs = B.w[0]->cg_block;
+#define ARAPGS_FUSED(BSV)                                                                                   \
+    do {                                                                                                    \
+        if (B.kt == 8) hipLaunchKernelGGL((cg_apply_agg_kernel<8, BSV>), grid, dim3(BSV), 0, st, E, k);     \
+        else hipLaunchKernelGGL((cg_apply_agg_kernel<0, BSV>), grid, dim3(BSV), 0, st, E, k);               \
+        hipLaunchKernelGGL(cg_update_coarse_kernel<BSV>, grid, dim3(BSV), 0, st, E, k);                     \
+    } while (0)
+        if (bs == 64) ARAPGS_FUSED(64);
+        else if (bs == 128) ARAPGS_FUSED(128);
+        else if (bs == 256) ARAPGS_FUSED(256);
+        else ARAPGS_FUSED(512);
+#undef ARAPGS_FUSED
+        HIP_RET(hipGetLastError());
+        return 0;
+    }
     int rc = launch_apply(B, nz, st, k, k == 0 ? 1 : 0, -1, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL(update_restrict_kernel, dim3(B.max_agg, 1, nz), dim3(256), 0, st, E, 1, k);

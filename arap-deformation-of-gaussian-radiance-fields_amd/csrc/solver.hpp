@@ -158,6 +158,9 @@ struct SolverBatch {
     // CG chunks replayed as captured HIP graphs: chunk j holds CG iterations
     // [j*every, min(cap, (j+1)*every)) and the flag kernel writing *flag
     bool graphs = false;
+    // CG iterations as two kernels (restriction by recurrence) instead of three
+    bool fused = false;
+    bool graph_fused = false;  // the form the cached chunk graphs were captured in
     int graph_every = 0, graph_cap = 0;
     const int* graph_flag[2] = {nullptr, nullptr};
     std::vector<hipGraphExec_t> cg_graphs;
@@ -171,8 +174,11 @@ void solver_batch_free(SolverBatch& B);
 // Points the entries at this frame's node positions and aims (one launch when any changed).
 int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* const* aim, hipStream_t st);
 // *flag_h (pinned host memory) = AND over the scenes of cg_done (which 0) or
-// gn_done (which 1), written by a kernel queued on st.
-int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h);
+// gn_done (which 1), written by a kernel queued on st. In the two-kernel CG
+// form (B.fused) the kernel first runs the convergence test after k_done CG
+// iterations. iters_h (which 0, when not null): the largest CG iteration count
+// over the scenes so far.
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done = 0, int* iters_h = nullptr);
 // Queues CG chunk j (iterations [j*every, min(cap, (j+1)*every)) followed by the
 // cg_done flag kernel writing flag_h[j & 1]) as one graph launch, capturing the
 // chunk on st the first time (st must not be the null stream).
@@ -207,6 +213,7 @@ void solver_free(SolverWork& W);
 
 int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st);
 int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st);
+// CG iteration k: three kernels, or two when B.fused.
 int solver_cg_iteration(SolverBatch& B, int k, hipStream_t st);
 // The whole PCG solve of one GN iteration as one cooperative launch (after
 // solver_gn_begin); same arithmetic and stopping rule as solver_cg_iteration.

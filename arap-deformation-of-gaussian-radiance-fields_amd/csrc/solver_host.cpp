@@ -228,7 +228,8 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.p = alloc<float>(W, W.n, rc);
     W.q = alloc<float>(W, W.n, rc);
     W.binv = alloc<float>(W, (size_t)144 * nF, rc);
-    W.rc = alloc<float>(W, 12 * (size_t)n_agg, rc);
+    // rc_k, rc_{k+1} and P^T q (the two-kernel CG form; the three-kernel form uses the first)
+    W.rc = alloc<float>(W, 3 * (size_t)S.nc_pad, rc);
     W.acinv = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
     W.acinv_alt = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);
     W.agg_center_alt = alloc<float>(W, 3 * (size_t)n_agg, rc);
@@ -243,8 +244,9 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.gj_piv = alloc<float>(W, 64 * 64, rc);
     W.gj_col = alloc<float>(W, (size_t)S.nc_pad * 64, rc);
     const int nb_owner = (nF + 15) / 16;
-    // p.q partials of the CG apply at the smallest CG block size (64: 4 owners per workgroup)
-    W.partial = alloc<double>(W, std::max((nF + 3) / 4, 1), rc);
+    // p.q partials of the CG apply at the smallest CG block size (64: 4 owners per
+    // workgroup) or one per aggregate (two-kernel form)
+    W.partial = alloc<double>(W, std::max(std::max((nF + 3) / 4, n_agg), 1), rc);
     W.partial2 = alloc<double>(W, n_agg, rc);
     W.rzp = alloc<double>(W, 2 * (size_t)n_agg, rc);
     W.e_blocks = (int)std::min<int64_t>(512, (S.n_groups + 255) / 256);

@@ -47,7 +47,7 @@ from . import arapgs
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
                  stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
-                 aims_fn=None, sample_batch=1, node_cache=True):
+                 aims_fn=None, sample_batch=1, node_cache=True, stream_c=None):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
@@ -58,7 +58,8 @@ class DragPipeline:
         sample_batch: B, the sample chunks of the rolling update (1..8; more
         than one needs split_samples, K = 8 and samples with features).
         node_cache: the deform side's blends through the LDS node-row cache
-        (arapgs_set_node_cache; identical results)."""
+        (arapgs_set_node_cache; identical results). stream_c: the samples'
+        stream (a new stream when None)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -78,7 +79,7 @@ class DragPipeline:
         self.chunks = [(c * S // self.nb, (c + 1) * S // self.nb) for c in range(self.nb)]
         self.applied = [-1] * self.nb  # last frame applied to each chunk
         self.frame = 0                 # frames whose deform side has been queued
-        self.stream_c = torch.cuda.Stream(device=dev) if self.split else None
+        self.stream_c = (stream_c if stream_c is not None else torch.cuda.Stream(device=dev)) if self.split else None
         self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
         for c in (self.ctx_b, self.ctx_c):
             if c is not None:

@@ -167,7 +167,9 @@ def _frame(B, plain, delta, stop_rule):
 def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     """Frame t's deformation and renders overlap frame t+1's solve (two streams,
     two host threads; pipeline.DragPipeline). solve_cus > 0: the solve stream
-    runs on CUs [0, solve_cus) and the deform stream on the others."""
+    runs on CUs [0, solve_cus) and the deform stream on the others.
+    cfg "deform_skip" > 0 (with solve_cus 0): the solve keeps every CU and the
+    deform and samples streams leave CUs [0, deform_skip) to it."""
     import torch
     from arapgs_amd import arapgs
     from arapgs_amd.pipeline import DragPipeline
@@ -177,18 +179,25 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     base_stream = ctx.stream
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     streams = []
+    skip = B["cfg"].get("deform_skip", 0) if cus == 0 else 0
+    stream_b = stream_c = None
     if cus > 0:
         streams = [arapgs.CuStream(0, 0, cus), arapgs.CuStream(0, cus, n_cu - cus)]
         ctx.set_stream(streams[0].torch)
+        stream_b = streams[1].torch
+    elif skip > 0:
+        streams = [arapgs.CuStream(0, skip, n_cu - skip), arapgs.CuStream(0, skip, n_cu - skip)]
+        stream_b, stream_c = streams[0].torch, streams[1].torch
     try:
-        B["solver"].set_cg_mode(0)
+        B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
+        B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
         B["solver"].set_host_checks(B["cfg"]["host_checks"])
         B["solver"].set_warm_start(B["cfg"]["warm"])
         B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
         nb = B["cfg"].get("sample_batch", 1) if sample_batch is None else sample_batch
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
-                            stream_b=streams[1].torch if streams else None, sample_batch=nb,
+                            stream_b=stream_b, stream_c=stream_c, sample_batch=nb,
                             node_cache=bool(B["cfg"].get("node_cache", 1)))
         pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
         pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
@@ -229,7 +238,7 @@ def partition_leg(B, args):
     """Picks the solve stream's CU partition for the pipelined form (0 = both
     streams on all CUs) from a few pipelined frames per candidate."""
     best, best_t = 0, None
-    for cus in (0, 16, 32, 48, 64):
+    for cus in (0, 16, 32, 48, 64, 128):
         n, w = tune_frames(args)
         r = timed_pipelined(B, n, w, solve_cus=cus)
         t = r["total_s"]
@@ -242,7 +251,8 @@ def partition_leg(B, args):
 def timed(B, plain, steps, warmup):
     import torch
     reset_state(B)
-    B["solver"].set_cg_mode(0)
+    B["solver"].set_cg_mode(0 if plain else B["cfg"].get("cg_mode", 0))
+    B["solver"].set_cg_predict(0 if plain else B["cfg"].get("cg_predict", 0))
     B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
     B["solver"].set_warm_start(0 if plain else B["cfg"]["warm"])
     B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
@@ -282,6 +292,8 @@ def apply_config(B, cfg):
     B["solver"].set_coarse_refresh(cfg["refresh"])
     B["solver"].set_cg_block(cfg.get("cg_block", 256))
     B["solver"].set_cg_graphs(cfg.get("cg_graphs", 0))
+    B["solver"].set_cg_mode(cfg.get("cg_mode", 0))
+    B["solver"].set_cg_predict(cfg.get("cg_predict", 0))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
@@ -382,7 +394,8 @@ def record_frames(B, steps, warmup):
                                                       "sample_feat")}
     reset_state(B)
     start = {k: (v.cpu().numpy() if v is not None else None) for k, v in B["state"].items()}
-    B["solver"].set_cg_mode(0)
+    B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
+    B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
     B["solver"].set_host_checks(B["cfg"]["host_checks"])
     B["solver"].set_warm_start(B["cfg"]["warm"])
     B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
@@ -539,7 +552,8 @@ def fixed_iteration_leg(B, args):
     B["solver"].set_warm_start(0)
     for form in ("plain", "fast"):
         plain = form == "plain"
-        B["solver"].set_cg_mode(0)
+        B["solver"].set_cg_mode(0 if plain else B["cfg"].get("cg_mode", 0))
+        B["solver"].set_cg_predict(0 if plain else B["cfg"].get("cg_predict", 0))
         B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
         B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
         ms = []
@@ -609,6 +623,8 @@ def large_leg(args, tuning):
     solver.set_host_checks(4)
     solver.set_warm_start(1)
     solver.set_cg_graphs(tuning.get("cg_graphs", 0))
+    solver.set_cg_mode(tuning.get("cg_mode", 0))
+    solver.set_cg_predict(tuning.get("cg_predict", 0))
     s0 = workload.clone_state(st)
     # a few sequential frames with the kernels timed by HIP events
     fb = arapgs.frame_buffers(st, K)
@@ -636,9 +652,13 @@ def large_leg(args, tuning):
             st[k].copy_(v)
     torch.cuda.synchronize()
     # the drag workload's tuned schedule (identical results)
+    skip = tuning.get("deform_skip", 0)
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    masked = [arapgs.CuStream(0, skip, n_cu - skip) for _ in range(2)] if skip > 0 else []
     pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
                         aims_fn=aims_fn, sample_batch=tuning.get("sample_batch", 1),
-                        node_cache=bool(tuning.get("node_cache", 1)))
+                        node_cache=bool(tuning.get("node_cache", 1)),
+                        stream_b=masked[0].torch if masked else None, stream_c=masked[1].torch if masked else None)
     pipe.ctx_b.set_blend_unroll(tuning.get("blend_unroll", 4))
     pipe.ctx_b.set_blend_split(tuning.get("blend_split", 0))
     pipe.run(args.warmup)
@@ -651,6 +671,9 @@ def large_leg(args, tuning):
     ms = [(b - a) * 1e3 for a, b in zip([t0] + stamps[:-1], stamps)]
     inst = pipe.instances
     pipe.close()
+    torch.cuda.synchronize()
+    for s_ in masked:
+        s_.close()
     gn, cg = solver.last_stats()
     frame_bytes = sum(b * n for b, n in per.values())
     # release: density sample over the 128^3 lists and realignment steps (high_quality)
@@ -899,6 +922,14 @@ def main():
                     help="sample chunks of the rolling sample update in the pipelined form (0: autotuned, or 4)")
     ap.add_argument("--cg-graphs", type=int, default=-1, choices=(-1, 0, 1),
                     help="CG chunks as captured HIP graphs (-1: autotuned, or off without the autotuner)")
+    ap.add_argument("--cg-mode", type=int, default=-1, choices=(-1, 0, 2),
+                    help="CG iterations as three kernels (0) or two (2) (-1: autotuned, or 0 without the autotuner)")
+    ap.add_argument("--cg-block", type=int, default=0, choices=(0, 64, 128, 256, 512),
+                    help="CG block size (0: autotuned, or 256 without the autotuner)")
+    ap.add_argument("--host-checks", type=int, default=0,
+                    help="CG iterations between host checks of the fast and pipelined forms (0: autotuned, or 16)")
+    ap.add_argument("--cg-predict", type=int, default=-1, choices=(-1, 0, 1),
+                    help="first CG host check at the previous frame's count (-1: autotuned, or off)")
     ap.add_argument("--partition", action="store_true",
                     help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
@@ -937,6 +968,14 @@ def main():
         log(f"  autotune: {best}")
     if args.cg_graphs >= 0:
         apply_config(B, dict(B["cfg"], cg_graphs=args.cg_graphs))
+    if args.cg_mode >= 0:
+        apply_config(B, dict(B["cfg"], cg_mode=args.cg_mode))
+    if args.cg_block > 0:
+        apply_config(B, dict(B["cfg"], cg_block=args.cg_block))
+    if args.cg_predict >= 0:
+        apply_config(B, dict(B["cfg"], cg_predict=args.cg_predict))
+    if args.host_checks > 0:
+        apply_config(B, dict(B["cfg"], host_checks=args.host_checks))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

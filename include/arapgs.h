@@ -55,6 +55,9 @@ int arapgs_synchronize(arapgs_ctx* ctx);
  * CUs of its own while a bandwidth-bound stage fills the rest. No counterpart
  * in the reference. Destroy with arapgs_stream_destroy. */
 int arapgs_stream_create(int device, int cu_begin, int cu_count, void** stream_out);
+/* The same with an explicit CU mask: bit i of mask_h (words 32-bit words)
+ * enables compute unit i (the order of hipExtStreamCreateWithCUMask). */
+int arapgs_stream_create_mask(int device, const uint32_t* mask_h, int words, void** stream_out);
 int arapgs_stream_destroy(void* stream);
 /* Library version string (static storage). */
 const char* arapgs_version(void);
@@ -154,10 +157,12 @@ int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
  * depend on the streams or the host. */
 int arapgs_solver_set_async_refresh(arapgs_solver* s, int on);
 /* The CG block size (no counterpart in the reference; set by the autotuner):
- * workgroup size of the CG operator kernel, 64, 128 or 256 threads (16 lanes
- * per free node, so 4, 8 or 16 nodes per workgroup; default 256). It changes
- * the order of the p.q partial sums only: the solve stops at the same
- * tolerance. Synchronises the device. Solvers of one batch need the same value. */
+ * workgroup size of the CG operator kernel, 64, 128, 256 or 512 threads (16
+ * lanes per free node, so 4 to 32 nodes per workgroup; default 256); in the
+ * two-kernel CG form (cg_mode 2) the workgroup size of both kernels (one
+ * workgroup per aggregate, BS / 16 member nodes per pass). It changes the
+ * order of the p.q partial sums only: the solve stops at the same tolerance.
+ * Synchronises the device. Solvers of one batch need the same value. */
 int arapgs_solver_set_cg_block(arapgs_solver* s, int threads);
 
 /* on = 1: the CG of GN iteration k starts from the step that GN iteration k of
@@ -197,9 +202,23 @@ int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h)
  * in the reference) is launched: 0 (default) = three kernels per CG iteration
  * (combine with set_host_checks to bound the launches queued past convergence);
  * 1 = one persistent cooperative kernel per solve with grid barriers between
- * the CG phases. Both compute the same iterates bit for bit; on MI355X a grid
- * barrier costs more than a kernel boundary, so mode 0 is the faster one. */
+ * the CG phases (the same iterates as 0 bit for bit; on MI355X a grid barrier
+ * costs more than a kernel boundary); 2 = two kernels per CG iteration: the
+ * coarse restriction is carried by the recurrence rc_{k+1} = rc_k - alpha P^T q_k
+ * (P^T q formed in the operator kernel, one workgroup per aggregate), so the
+ * update and the coarse correction share one launch. Mode 2 is the same
+ * preconditioned CG up to rounding (same tolerance and stopping rule); its
+ * iterates do not depend on the host-check interval or on CG graphs. */
 int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode);
+
+/* on = 1: with host checks on (set_host_checks > 0), the first host check of
+ * GN iteration k's CG comes after as many CG iterations as GN iteration k of
+ * the previous solve needed, then every set_host_checks iterations (no
+ * counterpart in the reference; host scheduling only: the device-side
+ * convergence test decides, so the results are the same bit for bit).
+ * Consecutive drag frames need similar counts, so fewer launches run past
+ * convergence. arapgs_solver_reset forgets the counts. */
+int arapgs_solver_set_cg_predict(arapgs_solver* s, int on);
 
 /* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,
  * Deform.cpp:378-581). energy: device f64[1]. */

@@ -11,7 +11,9 @@ def _cost(cfg):
     return (abs(cfg["aggregates"] - 192) / 64 + abs(cfg["refresh"] - 16) / 4 + abs(cfg["host_checks"] - 8) / 8
             + (0.5 if cfg["tile"] == 8 else 0.0) + abs(cfg["blend_unroll"] - 8) / 4 + (0.3 if cfg["warm"] else 0.0)
             + abs(cfg["cg_block"] - 128) / 128 + (0.2 if cfg["cg_graphs"] == 0 else 0.0)
-            + (0.1 if cfg["node_cache"] == 0 else 0.0) + abs(cfg["blend_split"] - 2048) / 8192 + 1.0)
+            + (0.1 if cfg["node_cache"] == 0 else 0.0) + abs(cfg["blend_split"] - 2048) / 8192
+            + abs(cfg["deform_skip"] - 32) / 64 + (0.15 if cfg["cg_mode"] == 0 else 0.0)
+            + (0.05 if cfg["cg_predict"] == 0 else 0.0) + 1.0)
 
 
 def test_finds_the_separable_optimum():
@@ -23,7 +25,7 @@ def test_finds_the_separable_optimum():
 
     best, trials = A.tune(measure)
     assert best == dict(tile=16, blend_unroll=8, aggregates=192, refresh=16, host_checks=8, warm=0, cg_block=128,
-                        cg_graphs=1, node_cache=1, blend_split=2048)
+                        cg_graphs=1, cg_mode=2, cg_predict=1, node_cache=1, blend_split=2048, deform_skip=32)
     assert len(seen) == len(set(seen)) == len(trials)
     assert min(t for _, t in trials) == _cost(best)
 

@@ -17,11 +17,13 @@ pytestmark = pytest.mark.gpu
 FRAMES = 5
 
 
-@pytest.mark.parametrize("sample_batch", [1, 3, 8])
-def test_pipelined_frames_bit_identical(ctx, sample_batch):
+@pytest.mark.parametrize("sample_batch,masked", [(1, False), (3, False), (8, False), (3, True)])
+def test_pipelined_frames_bit_identical(ctx, sample_batch, masked):
     """Pipelined frames (and the rolling sample chunks of sample_batch > 1) equal
     sequential frames bit for bit; the drag is run as two run() calls (3 + 2
-    frames), so the table slots and chunk records carry over between runs."""
+    frames), so the table slots and chunk records carry over between runs.
+    masked: the deform and samples streams are CU-masked streams that leave the
+    first 32 CUs to the solve (one from a CU range, one from an explicit mask)."""
     import torch
     sc = scenes.capsule_scene(7, n=40000, n_nodes=128, W=160, H=128, n_views=2)
     sc.grid = 32
@@ -52,8 +54,13 @@ def test_pipelined_frames_bit_identical(ctx, sample_batch):
     solver2.set_host_checks(16)
     imgs_pipe = [torch.empty((3, c["H"], c["W"]), device="cuda") for c in W["cams"]]
     torch.cuda.synchronize()
+    masks = []
+    if masked:
+        n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+        masks = [arapgs.CuStream(0, 32, n_cu - 32), arapgs.CuStream(0, cus=list(range(32, n_cu)))]
     pipe = DragPipeline(solver2, st2, K, W["active"], delta, iters, W["cams"], imgs_pipe, W["bg"], W["opac"],
-                        sample_batch=sample_batch)
+                        sample_batch=sample_batch, stream_b=masks[0].torch if masks else None,
+                        stream_c=masks[1].torch if masks else None)
     pipe.run(3)
     pipe.run(FRAMES - 3)
     torch.cuda.synchronize()
@@ -68,6 +75,9 @@ def test_pipelined_frames_bit_identical(ctx, sample_batch):
     pipe.close()
     del pipe, solver2
     ctx_a.close()
+    torch.cuda.synchronize()
+    for m in masks:
+        m.close()
 
 
 def test_samples_frames_equal_per_frame_calls(ctx):

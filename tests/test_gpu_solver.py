@@ -277,7 +277,7 @@ def test_coarse_inverse_is_exact_when_every_node_is_an_aggregate(ctx, prob):
     assert abs(float(H(e)[1]) - e_ref[1]) <= 1e-5 * e_ref[1]
 
 
-@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("block", [64, 128, 256, 512])
 def test_cg_block_size_keeps_the_solution(ctx, prob, block):
     """The CG block size (workgroup size of the CG operator kernel, an autotuner
     knob) changes only the order of the p.q partial sums: the final energy stays
@@ -328,3 +328,99 @@ def test_cg_graphs_bit_identical(prob, every):
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
     c2.close()
+
+
+@pytest.mark.parametrize("n_nodes", [128, 1024])
+def test_two_kernel_cg_matches_oracle(ctx, n_nodes):
+    """CG iterations as two kernels (cg_mode 2: the coarse restriction carried
+    by the recurrence rc_{k+1} = rc_k - alpha P^T q_k, update and coarse phases
+    in one launch): the final energy within 1e-5 of the oracle's and the GN
+    count within one; the iterates do not depend on how the host checks the
+    convergence flag (every iteration, every 3, every 16, never, graph chunks of
+    4) — the device-side test decides, bit for bit the same."""
+    import torch
+    sc = small_scene(n=20000 if n_nodes == 128 else 60000, n_nodes=n_nodes, seed=1)
+    aims, ref = _problem(sc, 0.7)
+    x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+    c2 = arapgs.Context(0, stream=torch.cuda.Stream())
+    outs = []
+    with torch.cuda.stream(c2.stream):
+        s = _solver(c2, sc)
+        s.set_coarse_refresh(1000)
+        s.set_cg_mode(2)
+        for every, graphs in ((1, False), (3, False), (16, False), (0, False), (4, True)):
+            s.set_host_checks(every)
+            s.set_cg_graphs(graphs)
+            x, e, done = s.solve(T(sc["nodes"]), T(aims), 8, stop_rule=True)
+            gn, cg = s.last_stats()
+            outs.append((H(x), H(e), int(H(done)[0]), cg))
+    torch.cuda.synchronize()
+    x, e, done, cg = outs[0]
+    print(f"two-kernel CG, {n_nodes} nodes: GN {done} (oracle {it_ref}), {cg} CG iterations, "
+          f"E {e[done]:.9g} vs {e_ref[-1]:.9g}", flush=True)
+    assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1]
+    assert abs(done - it_ref) <= 1
+    assert np.all(np.diff(e[: done + 1]) <= 1e-9 * e[0])
+    for o in outs[1:]:
+        assert o[2] == done and o[3] == cg
+        assert np.array_equal(o[0], x)
+        assert np.array_equal(o[1][: done + 1], e[: done + 1])
+    s.close()
+    c2.close()
+
+
+@pytest.mark.parametrize("block", [64, 256, 512])
+def test_two_kernel_cg_warm_drag_and_center_mode(ctx, prob, block):
+    """The two-kernel CG form at CG block sizes 64, 256 and 512 (member nodes per
+    pass of the aggregate workgroups) over warm-started drag frames (energies
+    within 1e-5 of the oracle's, GN counts within one) and in centre mode."""
+    s = _solver(ctx, prob)
+    s.set_cg_mode(2)
+    s.set_cg_block(block)
+    s.set_host_checks(4)
+    s.set_warm_start(True)
+    for fraction in (0.30, 0.35, 0.40, 0.45):
+        aims, ref = _problem(prob, fraction)
+        x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+        x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+        e, done = H(e), int(H(done)[0])
+        assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (fraction, e[: done + 1], e_ref)
+        assert abs(done - it_ref) <= 1, (fraction, done, it_ref)
+    s.close()
+    blocks, types = _center_blocks(prob)
+    aims = _translate_aims(prob["nodes"], blocks, types, 0.7)
+    ref = O.GNProblem(prob["nodes"], prob["nbr"], blocks, types, prob["src_idx"], prob["src_w"], aims, center=True)
+    x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+    s = _solver(ctx, prob, blocks=blocks, types=types, center=True)
+    s.set_cg_mode(2)
+    s.set_cg_block(block)
+    x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+    e, done = H(e), int(H(done)[0])
+    assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (e[: done + 1], e_ref)
+    assert abs(done - it_ref) <= 1
+    s.close()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_cg_predict_bit_identical(ctx, prob, mode):
+    """The first CG host check at the previous solve's count (set_cg_predict) is
+    host scheduling only: four warm-started drag frames give the same iterates
+    bit for bit as without it (CG modes 0 and 2, host checks every 2)."""
+    runs = []
+    for predict in (False, True):
+        s = _solver(ctx, prob)
+        s.set_cg_mode(mode)
+        s.set_host_checks(2)
+        s.set_warm_start(True)
+        s.set_cg_predict(predict)
+        out = []
+        for fraction in (0.30, 0.35, 0.40, 0.45):
+            aims, _ = _problem(prob, fraction)
+            x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+            out.append((H(x), H(e), int(H(done)[0]), s.last_stats()))
+        runs.append(out)
+        s.close()
+    for a, b in zip(*runs):
+        assert a[2] == b[2] >= 1 and a[3] == b[3]
+        assert np.array_equal(a[0], b[0])
+        assert np.array_equal(a[1][: a[2] + 1], b[1][: b[2] + 1])
