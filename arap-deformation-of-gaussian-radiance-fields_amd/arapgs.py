@@ -654,7 +654,7 @@ class Solver:
         self.ctx._check(lib().arapgs_solver_set_async_refresh(self.h, int(bool(on))))
 
     def set_cg_graphs(self, on=True):
-        """CG chunks between host checks as captured HIP graphs, checked one chunk behind."""
+        """GN phases and CG chunks as captured HIP graphs (CG flags checked one chunk behind)."""
         self.ctx._check(lib().arapgs_solver_set_cg_graphs(self.h, int(bool(on))))
 
     def apply(self, node_pos, aims, x, v):

@@ -69,7 +69,7 @@ struct arapgs_ctx {
     int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (8 ints;
                                       // [3]: the CG iteration count at a host check; [4], [5]: the flags
                                       // of consecutive CG chunk graphs; [6]: the unread flag of the
-                                      // two-kernel CG's final test)
+                                      // two-kernel CG's final test; [7]: gn_done after a GN-end graph)
     hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
@@ -546,9 +546,14 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
         *out = *(volatile int*)c->flag_h;
         return 0;
     };
+    // graphs: the GN phases and the CG chunks replay as captured graphs (a
+    // context without its own stream cannot capture)
+    const bool graphs = B.graphs && every > 0 && c->stream && cg_mode != 1;
     for (int it = 0; it < iters; it++) {
-        rc = check_launch(c, solver_gn_begin(B, it, c->stream), "gn_solve rhs");
+        rc = check_launch(c, graphs ? solver_gn_begin_graph(B, it, c->stream) : solver_gn_begin(B, it, c->stream),
+                          "gn_solve rhs");
         if (rc) return rc;
+        const bool predict = pred && every > 0 && it < GN_MAX_ITERS;
         if (cg_mode == 1) {
             rc = check_launch(c, solver_cg_persistent(*B.w[0], c->stream), "gn_solve cg");
             if (rc) return rc;
@@ -563,26 +568,40 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
             // every scene's own cap is enforced on the device (cg_done); loop to the largest
             int cap = 0;
             for (const SolverWork* w : B.w) cap = std::max(cap, w->cg_max_iters);
-            if (B.graphs && every > 0 && c->stream) {
+            if (graphs) {
                 // chunk j is one graph launch; the host reads chunk j-1's flag
                 // while chunk j is already queued, so the stream never waits for
-                // the host (a chunk queued past convergence exits kernel by kernel)
+                // the host (a chunk queued past convergence exits kernel by kernel).
+                // The first chunk is the predicted count (or `every`), then `every`.
                 int* const slots[2] = {c->flag_h + 4, c->flag_h + 5};
-                const int nch = (cap + every - 1) / every;
-                for (int j = 0; j < nch; j++) {
-                    rc = check_launch(c, solver_cg_chunk(B, j, every, cap, c->stream, slots), "gn_solve cg graph");
+                bool seen_done = false;
+                int k0 = 0;
+                for (int j = 0; k0 < cap; j++) {
+                    const int L = std::min(cap - k0, j == 0 && predict && pred[it] > 0 ? pred[it] : every);
+                    rc = check_launch(c, solver_cg_chunk(B, k0, L, c->stream, slots[j & 1], c->flag_h + 3),
+                                      "gn_solve cg graph");
                     if (rc) return rc;
+                    k0 += L;
                     hipError_t e = hipEventRecord(c->chk_ev[j & 1], c->stream);
                     if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg graph");
                     if (j > 0) {
                         e = hipEventSynchronize(c->chk_ev[(j - 1) & 1]);
                         if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg graph");
-                        if (*(volatile int*)slots[(j - 1) & 1]) break;
+                        if (*(volatile int*)slots[(j - 1) & 1]) {
+                            seen_done = true;
+                            break;
+                        }
                     }
                 }
+                if (!seen_done) {
+                    // the last chunk's flag (its count is final too)
+                    hipError_t e = hipStreamSynchronize(c->stream);
+                    if (e != hipSuccess) return hip_fail(c, e, "gn_solve cg graph");
+                    seen_done = true;
+                }
+                if (predict) pred[it] = std::max(1, (int)*(volatile int*)(c->flag_h + 3));
             } else {
                 bool seen_done = false;
-                const bool predict = pred && every > 0 && it < GN_MAX_ITERS;
                 int next_check = predict && pred[it] > 0 ? std::min(pred[it], cap) : every;
                 for (int k = 0; k < cap; k++) {
                     rc = check_launch(c, solver_cg_iteration(B, k, c->stream), "gn_solve cg");
@@ -597,14 +616,23 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
                         next_check += every;
                     }
                 }
-                if (predict) pred[it] = seen_done ? std::max(1, (int)*(volatile int*)(c->flag_h + 3)) : 0;
                 // the two-kernel form tests convergence at the start of the next
                 // apply: record the test after the last iteration queued
                 if (B.fused && !seen_done) {
                     rc = check_launch(c, solver_batch_flag(B, 0, c->stream, c->flag_h + 6, cap), "gn_solve cg");
                     if (rc) return rc;
                 }
+                if (predict) pred[it] = seen_done ? std::max(1, (int)*(volatile int*)(c->flag_h + 3)) : 0;
             }
+        }
+        if (graphs) {
+            rc = check_launch(c, solver_gn_end_graph(B, it, stop_rule, c->stream, c->flag_h + 7),
+                              "gn_solve line search");
+            if (rc) return rc;
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "gn_solve line search");
+            if (*(volatile int*)(c->flag_h + 7)) break;
+            continue;
         }
         rc = check_launch(c, solver_gn_end(B, it, stop_rule, c->stream), "gn_solve line search");
         if (rc) return rc;

@@ -469,13 +469,49 @@ __device__ __forceinline__ void quat_normalize_rsq(float q[4]) {
     for (int c = 0; c < 4; c++) q[c] *= r;
 }
 
+// theta = acos(x) for x in [0, 1]: Abramowitz & Stegun 4.4.46 (Hastings),
+// sqrt(1 - x) times a degree-7 polynomial, |error| <= 2e-8 in exact
+// arithmetic (about 2.5e-7 evaluated in FP32).
+__device__ __forceinline__ float acos_unit(float x) {
+    float p = -0.0012624911f;
+    p = __builtin_fmaf(p, x, 0.0066700901f);
+    p = __builtin_fmaf(p, x, -0.0170881256f);
+    p = __builtin_fmaf(p, x, 0.0308918810f);
+    p = __builtin_fmaf(p, x, -0.0501743046f);
+    p = __builtin_fmaf(p, x, 0.0889789874f);
+    p = __builtin_fmaf(p, x, -0.2145988016f);
+    p = __builtin_fmaf(p, x, 1.5707963050f);
+    return __builtin_sqrtf(1.0f - x) * p;
+}
+
+// sin and cos of a in [0, pi/2] by their Taylor series through a^11 / a^12
+// (truncation below 6e-8 and 7e-9 at pi/2; about 1.6e-7 evaluated in FP32).
+__device__ __forceinline__ void sincos_quadrant(float a, float& sn, float& cs) {
+    const float a2 = a * a;
+    float ps = -2.5052108e-8f;               // -1/11!
+    ps = __builtin_fmaf(ps, a2, 2.7557319e-6f);   // 1/9!
+    ps = __builtin_fmaf(ps, a2, -1.9841270e-4f);  // -1/7!
+    ps = __builtin_fmaf(ps, a2, 8.3333333e-3f);   // 1/5!
+    ps = __builtin_fmaf(ps, a2, -1.6666667e-1f);  // -1/3!
+    sn = __builtin_fmaf(a * a2, ps, a);
+    float pc = 2.0876757e-9f;                // 1/12!
+    pc = __builtin_fmaf(pc, a2, -2.7557319e-7f);  // -1/10!
+    pc = __builtin_fmaf(pc, a2, 2.4801587e-5f);   // 1/8!
+    pc = __builtin_fmaf(pc, a2, -1.3888889e-3f);  // -1/6!
+    pc = __builtin_fmaf(pc, a2, 4.1666667e-2f);   // 1/4!
+    pc = __builtin_fmaf(pc, a2, -0.5f);
+    cs = __builtin_fmaf(a2, pc, 1.0f);
+}
+
 // One step of the running slerp of Q_SlerpCUDA (cudakdtree.cu:113-148): acc
 // moves towards e (flipped into acc's hemisphere) by t = cw / (cw + last).
 // The first neighbour (last = 0, t = 1) lands on e itself. Otherwise, with
 // cos(theta) = cs and s = sin(t theta) / sin(theta), the slerp weights are
-// (cos(t theta) - cs s, s): one sincos and one reciprocal instead of two sines
-// and two divisions (angle-addition formula; equal to the reference's FP64
-// weights within a few float ulps).
+// (cos(t theta) - cs s, s): one reciprocal instead of two divisions
+// (angle-addition formula). theta = acos(cs) lies in [0, pi/2] after the flip
+// and t theta too, so theta and sin/cos(t theta) come from the short
+// polynomial forms above instead of the general atan2 and sincos (the weights
+// stay within a few 1e-7 of the reference's FP64 ones).
 __device__ __forceinline__ void slerp_step(float (&acc)[4], float4 e4, float cw, float last, bool first) {
     float e[4] = {e4.x, e4.y, e4.z, e4.w};
     float cs = acc[1] * e[1] + acc[2] * e[2] + acc[3] * e[3] + acc[0] * e[0];
@@ -494,9 +530,9 @@ __device__ __forceinline__ void slerp_step(float (&acc)[4], float4 e4, float cw,
             rb = t;
         } else {
             const float sn = __builtin_sqrtf(1.0f - cs * cs);
-            const float ang = atan2f(sn, cs);
+            const float ang = acos_unit(cs);
             float st, ct;
-            sincosf(t * ang, &st, &ct);
+            sincos_quadrant(t * ang, st, ct);
             rb = st * (1.0f / sn);
             ra = ct - cs * rb;
         }

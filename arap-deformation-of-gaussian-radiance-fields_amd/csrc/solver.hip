@@ -1927,9 +1927,32 @@ static SolverEntry build_entry_of(const SolverWork& W) {
 }
 
 void solver_batch_drop_graphs(SolverBatch& B) {
-    for (hipGraphExec_t& x : B.cg_graphs)
-        if (x) (void)hipGraphExecDestroy(x);
-    B.cg_graphs.clear();
+    for (auto& kv : B.graph_cache)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    B.graph_cache.clear();
+}
+
+// Launches the graph cached under key, capturing body() on st first.
+template <typename Body>
+static int graph_launch(SolverBatch& B, const std::array<int64_t, 6>& key, hipStream_t st, Body&& body) {
+    auto it = B.graph_cache.find(key);
+    if (it == B.graph_cache.end()) {
+        HIP_RET(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : -(int)e - 1000;
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_RET(ei);
+        it = B.graph_cache.emplace(key, x).first;
+    }
+    HIP_RET(hipGraphLaunch(it->second, st));
+    return 0;
 }
 
 void solver_batch_free(SolverBatch& B) {
@@ -2031,37 +2054,30 @@ int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, in
     return 0;
 }
 
-int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, int* const flag_h[2]) {
-    if (B.graph_every != every || B.graph_cap != cap || B.graph_flag[0] != flag_h[0] || B.graph_flag[1] != flag_h[1] ||
-        B.graph_fused != B.fused) {
-        solver_batch_drop_graphs(B);
-        B.graph_every = every;
-        B.graph_cap = cap;
-        B.graph_fused = B.fused;
-        B.graph_flag[0] = flag_h[0];
-        B.graph_flag[1] = flag_h[1];
-    }
-    if ((int)B.cg_graphs.size() <= j) B.cg_graphs.resize(j + 1, nullptr);
-    if (!B.cg_graphs[j]) {
-        HIP_RET(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+int solver_cg_chunk(SolverBatch& B, int k0, int L, hipStream_t st, int* flag_h, int* iters_h) {
+    const std::array<int64_t, 6> key = {0, k0, L, (int64_t)B.fused, (int64_t)reinterpret_cast<uintptr_t>(flag_h),
+                                        (int64_t)reinterpret_cast<uintptr_t>(iters_h)};
+    return graph_launch(B, key, st, [&]() -> int {
         int rc = 0;
-        const int k_end = std::min(cap, (j + 1) * every);
-        for (int k = j * every; k < k_end && !rc; k++) rc = solver_cg_iteration(B, k, st);
-        if (!rc) rc = solver_batch_flag(B, 0, st, flag_h[j & 1], k_end);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(st, &g);
-        if (rc || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc ? rc : -(int)e - 1000;
-        }
-        hipGraphExec_t x = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_RET(ei);
-        B.cg_graphs[j] = x;
-    }
-    HIP_RET(hipGraphLaunch(B.cg_graphs[j], st));
-    return 0;
+        for (int k = k0; k < k0 + L && !rc; k++) rc = solver_cg_iteration(B, k, st);
+        if (!rc) rc = solver_batch_flag(B, 0, st, flag_h, k0 + L, iters_h);
+        return rc;
+    });
+}
+
+int solver_gn_begin_graph(SolverBatch& B, int iter, hipStream_t st) {
+    const std::array<int64_t, 6> key = {1, iter, (int64_t)B.w[0]->warm, (int64_t)B.w[0]->warm_slots, 0, 0};
+    return graph_launch(B, key, st, [&]() -> int { return solver_gn_begin(B, iter, st); });
+}
+
+int solver_gn_end_graph(SolverBatch& B, int iter, int stop_rule, hipStream_t st, int* flag_h) {
+    const std::array<int64_t, 6> key = {2, iter, stop_rule, (int64_t)B.w[0]->warm, (int64_t)B.w[0]->warm_slots,
+                                        (int64_t)reinterpret_cast<uintptr_t>(flag_h)};
+    return graph_launch(B, key, st, [&]() -> int {
+        int rc = solver_gn_end(B, iter, stop_rule, st);
+        if (!rc) rc = solver_batch_flag(B, 1, st, flag_h);
+        return rc;
+    });
 }
 
 int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st) {

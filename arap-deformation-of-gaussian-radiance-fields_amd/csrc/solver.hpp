@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <array>
+#include <map>
 #include <vector>
 
 namespace arapgs {
@@ -155,15 +157,14 @@ struct SolverBatch {
     hipEvent_t fork_ev = nullptr;
     bool async = true;                // coarse refreshes on the side stream (the first one in line)
     std::vector<const float*> g, aim; // what the device entries hold
-    // CG chunks replayed as captured HIP graphs: chunk j holds CG iterations
-    // [j*every, min(cap, (j+1)*every)) and the flag kernel writing *flag
+    // Launch sequences replayed as captured HIP graphs: the CG chunks (CG
+    // iterations [k0, k0 + L) and the flag kernel) and the Gauss-Newton
+    // iteration's begin and end phases, each cached under the arguments it was
+    // captured with (a graph replays its kernels' arguments as captured).
     bool graphs = false;
     // CG iterations as two kernels (restriction by recurrence) instead of three
     bool fused = false;
-    bool graph_fused = false;  // the form the cached chunk graphs were captured in
-    int graph_every = 0, graph_cap = 0;
-    const int* graph_flag[2] = {nullptr, nullptr};
-    std::vector<hipGraphExec_t> cg_graphs;
+    std::map<std::array<int64_t, 6>, hipGraphExec_t> graph_cache;
     int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
     int max_nb_owner = 0, max_nb_apply = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
         max_multi = 0, max_nt = 0, max_cand = 0;
@@ -179,10 +180,14 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
 // iterations. iters_h (which 0, when not null): the largest CG iteration count
 // over the scenes so far.
 int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done = 0, int* iters_h = nullptr);
-// Queues CG chunk j (iterations [j*every, min(cap, (j+1)*every)) followed by the
-// cg_done flag kernel writing flag_h[j & 1]) as one graph launch, capturing the
-// chunk on st the first time (st must not be the null stream).
-int solver_cg_chunk(SolverBatch& B, int j, int every, int cap, hipStream_t st, int* const flag_h[2]);
+// Queues CG iterations [k0, k0 + L) followed by the cg_done flag kernel
+// writing *flag_h (and the CG count into *iters_h) as one graph launch,
+// capturing it on st the first time (st must not be the null stream).
+int solver_cg_chunk(SolverBatch& B, int k0, int L, hipStream_t st, int* flag_h, int* iters_h);
+// solver_gn_begin / solver_gn_end + the gn_done flag kernel writing *flag_h,
+// as graph launches (captured on first use; st must not be the null stream).
+int solver_gn_begin_graph(SolverBatch& B, int iter, hipStream_t st);
+int solver_gn_end_graph(SolverBatch& B, int iter, int stop_rule, hipStream_t st, int* flag_h);
 void solver_batch_drop_graphs(SolverBatch& B);
 int solver_batch_finish(SolverBatch& B, const SolverOutPtrs& out, hipStream_t st);
 // The warm-start branch is taken from the first scene; a batch needs them all alike.

@@ -184,14 +184,18 @@ int arapgs_solver_set_warm_start(arapgs_solver* s, int on);
 int arapgs_solver_reset(arapgs_solver* s);
 
 /* on = 1: with host checks every `every` CG iterations (arapgs_solver_set_host_checks
- * > 0) on a non-null stream, each chunk of `every` CG iterations and its
- * convergence flag are one captured HIP graph (captured on the first use of each
- * chunk index and kept), and the host reads chunk j-1's flag while chunk j is
- * already queued, so the stream does not wait for the host between chunks; a
- * chunk queued past convergence exits kernel by kernel on the device flag. The
- * kernels, their arguments and their order are those of the launch loop, so the
- * iterates are identical bit for bit. No counterpart in the reference (its
- * solver is a CPU Cholesky). Default 0. */
+ * > 0) on a non-null stream, the solve's launch sequences replay as captured HIP
+ * graphs (each captured on first use under the arguments it was captured with,
+ * and kept): every Gauss-Newton iteration's begin phase (right-hand side,
+ * preconditioner, CG start) and end phase (line search, acceptance, gn_done
+ * flag), and the CG in chunks (the predicted count of set_cg_predict, or
+ * `every`, then `every` iterations) with their convergence flag; the host
+ * reads chunk j-1's flag while chunk j is already queued, so the stream does
+ * not wait for the host between chunks, and a chunk queued past convergence
+ * exits kernel by kernel on the device flag. The kernels, their arguments and
+ * their order are those of the launch loop, so the iterates are identical bit
+ * for bit. No counterpart in the reference (its solver is a CPU Cholesky).
+ * Default 0. */
 int arapgs_solver_set_cg_graphs(arapgs_solver* s, int on);
 
 /* GN iterations and total CG iterations of the solver's last solve

@@ -299,13 +299,14 @@ def test_cg_block_size_keeps_the_solution(ctx, prob, block):
         s.close()
 
 
-@pytest.mark.parametrize("every", [4, 3])
-def test_cg_graphs_bit_identical(prob, every):
-    """CG chunks replayed as captured HIP graphs (checked one chunk behind)
-    compute the launch loop's iterates bit for bit: three consecutive warm-started
-    solves of different aims, on a context with its own stream (graphs are
-    captured on a non-null stream), chunk sizes 4 and 3 (the CG cap of 200 is not
-    a multiple of 3: the last chunk is shorter)."""
+@pytest.mark.parametrize("every,predict", [(4, False), (3, False), (2, True)])
+def test_cg_graphs_bit_identical(prob, every, predict):
+    """The GN phases and CG chunks replayed as captured HIP graphs (CG flags
+    checked one chunk behind) compute the launch loop's iterates bit for bit:
+    three consecutive warm-started solves of different aims, on a context with
+    its own stream (graphs are captured on a non-null stream), chunk sizes 4
+    and 3 (the CG cap of 200 is not a multiple of 3: the last chunk is shorter),
+    and predicted first chunks (set_cg_predict) followed by chunks of 2."""
     import torch
     c2 = arapgs.Context(0, stream=torch.cuda.Stream())
     runs = []
@@ -314,6 +315,7 @@ def test_cg_graphs_bit_identical(prob, every):
             s = _solver(c2, prob)
             s.set_host_checks(every)
             s.set_warm_start(True)
+            s.set_cg_predict(predict)
             s.set_cg_graphs(graphs)
             out = []
             for f in range(3):
