@@ -76,6 +76,8 @@ EXPORTS = {
     "arapgs_set_blend_split": ([P, C.c_int], C.c_int),
     "arapgs_stream_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "arapgs_stream_create_mask": ([C.c_int, P, C.c_int, C.POINTER(P)], C.c_int),
+    "arapgs_set_sh_compose": ([P, C.c_int], C.c_int),
+    "arapgs_set_tile_cull": ([P, C.c_int], C.c_int),
     "arapgs_stream_destroy": ([P], C.c_int),
     "arapgs_version": ([], C.c_char_p),
     "arapgs_fps": ([P, P, C.c_int64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
@@ -324,7 +326,8 @@ class Context:
 
     def samples_frames(self, tables, node_qs, idx, w, is_static, samples, feat):
         """len(tables) consecutive frames of the sample blend + sample SH in one
-        pass (arapgs_samples_frames; bit-identical to the per-frame calls)."""
+        pass (arapgs_samples_frames; bit-identical to the per-frame calls, or with
+        set_sh_compose one SH rotation by the product of the frames' rotations)."""
         nf = len(tables)
         tp = _ptr_array(tables)
         qp = _ptr_array(node_qs)
@@ -475,6 +478,14 @@ class Context:
     def set_node_cache(self, on=True):
         """Six-point end-point blend through the LDS node-row cache (bit-identical)."""
         self._check(lib().arapgs_set_node_cache(self.h, int(bool(on))))
+
+    def set_tile_cull(self, on=True):
+        """Renders emit only the tiles that pass the conservative tile test (identical images)."""
+        self._check(lib().arapgs_set_tile_cull(self.h, int(bool(on))))
+
+    def set_sh_compose(self, on=True):
+        """samples_frames: one SH rotation by the product of the frames' rotations (within float rounding)."""
+        self._check(lib().arapgs_set_sh_compose(self.h, int(bool(on))))
 
     def set_grid_chunk(self, pairs):
         """Cell-Gaussian pairs per chunk of later grid builds (0: default); test hook."""

@@ -23,7 +23,13 @@ its tolerance:
   deform_skip   pipelined frames: CUs [0, deform_skip) left to the solve by the
                 deform and samples streams (CU-masked streams; 0: all CUs shared)
   solve_cus     pipelined frames: the solve on CUs [0, solve_cus) and the deform
-                and samples streams on the others (0: all CUs shared)
+                stream on the others (0: all CUs shared)
+  samples_cus   with solve_cus: the samples stream beside the deform (0), on
+                every CU (1) or beside the solve (2)
+  sh_compose    rolling sample chunks: features rotated once by the product of
+                the pending frames' rotations (1) or frame by frame (0)
+  tile_cull     renders emit only the tiles that pass the conservative tile test
+                (1; identical images) or every tile of the rectangle (0)
 
 `tune` runs a coordinate descent from the defaults: for each parameter in turn
 it times every candidate value (the others held at their current best) with a
@@ -37,14 +43,16 @@ import os
 
 DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0,
                 cg_mode=0, cg_predict=0, node_cache=1, blend_split=0, deform_skip=0,
-                solve_cus=0)
+                solve_cus=0, samples_cus=0, sh_compose=0, tile_cull=0)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
                   host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),
                   cg_mode=(0, 2), cg_predict=(0, 1), node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096),
-                  deform_skip=(0, 16, 32, 64), solve_cus=(0, 32, 48, 64))
+                  deform_skip=(0, 16, 32, 64), solve_cus=(0, 32, 48, 64),
+                  samples_cus=(0, 1), sh_compose=(0, 1), tile_cull=(0, 1))
 # host_checks is revisited after cg_graphs: their best chunk can differ
-ORDER = ("warm", "solve_cus", "cg_mode", "cg_predict", "aggregates", "refresh", "host_checks", "cg_graphs",
-         "host_checks", "cg_block", "tile", "blend_unroll", "blend_split", "node_cache", "deform_skip")
+ORDER = ("warm", "solve_cus", "samples_cus", "cg_mode", "cg_predict", "aggregates", "refresh", "host_checks",
+         "cg_graphs", "host_checks", "cg_block", "tile", "tile_cull", "blend_unroll", "blend_split", "node_cache",
+         "sh_compose", "deform_skip")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

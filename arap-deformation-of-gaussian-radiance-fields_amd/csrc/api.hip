@@ -74,7 +74,9 @@ struct arapgs_ctx {
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     int blend_split = 0;              // instances per blend segment of a deep 16x16 tile (0: no split)
-    bool node_cache = true;           // six-point end-point blend through the LDS node-row cache
+    bool node_cache = true;
+    bool sh_compose = false;          // samples_frames: one SH rotation by the product of the frames' rotations
+    bool tile_cull = false;           // renders emit only the tiles that pass the conservative tile test           // six-point end-point blend through the LDS node-row cache
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
     bool prof = false;
     // HIP events around the six-point (0), sample blend (1) and sample SH (2)
@@ -835,8 +837,23 @@ int arapgs_samples_frames(arapgs_ctx* c, int64_t s, int k, int nf, const arapgs_
         F.table[f] = (const NodeXf*)tables_h[f];
         F.q[f] = node_q_h[f];
     }
-    return check_launch(c, launch_samples_frames(s, k, F, idx, w, is_static, samples, feat, c->shrot_dev, c->stream),
+    return check_launch(c, launch_samples_frames(s, k, F, idx, w, is_static, samples, feat, c->shrot_dev, c->stream,
+                                                 c->sh_compose),
                         "samples_frames");
+}
+
+int arapgs_set_tile_cull(arapgs_ctx* c, int on) {
+    CHECK_CTX(c);
+    if (on < 0 || on > 1) return fail(c, -1, "set_tile_cull: on must be 0 or 1");
+    c->tile_cull = on != 0;
+    return 0;
+}
+
+int arapgs_set_sh_compose(arapgs_ctx* c, int on) {
+    CHECK_CTX(c);
+    if (on < 0 || on > 1) return fail(c, -1, "set_sh_compose: on must be 0 or 1");
+    c->sh_compose = on != 0;
+    return 0;
 }
 
 int arapgs_sh_rotate(arapgs_ctx* c, int64_t n, const float* r9, float* sh) {
@@ -926,6 +943,7 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
     bool any_surface = false;
     for (int v = 0; v < V; v++) any_surface = any_surface || dv[v].surface != nullptr;
     rp.split = (allow_split && !any_surface && tile == 16) ? c->blend_split : 0;
+    rp.cull = c->tile_cull ? 1 : 0;
     for (int i = 0; i < 3; i++) {
         rp.boxmin[i] = rp.use_box ? d->boxmin_h[i] : 0.0f;
         rp.boxmax[i] = rp.use_box ? d->boxmax_h[i] : 0.0f;

@@ -61,7 +61,8 @@ struct SampleFrames {
     const float* q[SAMPLE_FRAMES_MAX];
 };
 int launch_samples_frames(int64_t S, int k, const SampleFrames& F, const uint32_t* idx, const float* w,
-                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* Td, hipStream_t st);
+                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* Td, hipStream_t st,
+                          bool compose = false);
 int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, const float* node_q,
                      const uint8_t* is_static, float* feat, const ShRotTables* Td, hipStream_t st);
 int launch_sh_rotate(int64_t n, const float* R9, float* sh, const ShRotTables* Td, hipStream_t st);

@@ -583,11 +583,14 @@ __device__ __forceinline__ void sample_rotation_regs(const uint32_t (&id)[K], co
 // arithmetic, so the result equals nf launches of blend_points_kernel<8, true>
 // and sample_sh_kernel<8> bit for bit while every sample's 536 bytes (position,
 // ids, weights, feature) move through HBM once instead of nf times.
+// compose != 0: the feature is rotated once by R_{nf-1} ... R_1 R_0 instead of
+// nf times (the SH rotation is a representation: D(R_b) D(R_a) = D(R_b R_a));
+// the positions stay bit-identical, the features agree within float rounding.
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 samples_frames_kernel(int64_t S, SampleFrames F, const uint32_t* __restrict__ idx, const float* __restrict__ w,
                       const uint8_t* __restrict__ is_static, float* __restrict__ pts, float* __restrict__ feat,
-                      const ShRotTables* __restrict__ T) {
+                      const ShRotTables* __restrict__ T, int compose) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ __align__(16) float lds[256 * 18];
     const int64_t s0 = (int64_t)blockIdx.x * blockDim.x;
@@ -606,6 +609,31 @@ samples_frames_kernel(int64_t S, SampleFrames F, const uint32_t* __restrict__ id
             p[0] = o[0]; p[1] = o[1]; p[2] = o[2];
         }
         pts[3 * s] = p[0]; pts[3 * s + 1] = p[1]; pts[3 * s + 2] = p[2];
+    }
+    if (compose) {
+        // the product of the frames' rotations before the features occupy registers
+        float Rt[3][3];
+        if (live) {
+            sample_rotation_regs<K>(id, wt, F.q[0], Rt);
+#pragma unroll 1
+            for (int f = 1; f < F.n; f++) {
+                float R[3][3], P[3][3];
+                sample_rotation_regs<K>(id, wt, F.q[f], R);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) P[i][j] = R[i][0] * Rt[0][j] + R[i][1] * Rt[1][j] + R[i][2] * Rt[2][j];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) Rt[i][j] = P[i][j];
+            }
+        }
+        float c[48];
+        rows48_load(feat + s0 * 48, n_here, lds, 256 * 18, c);
+        if (live) sh_rotate_regs(*T, Rt, c);
+        rows48_store(feat + s0 * 48, n_here, lds, 256 * 18, c);
+        return;
     }
     float c[48];
     rows48_load(feat + s0 * 48, n_here, lds, 256 * 18, c);
@@ -741,11 +769,12 @@ int launch_sample_sh(int64_t S, int k, const float* w, const uint32_t* idx, cons
 }
 
 int launch_samples_frames(int64_t S, int k, const SampleFrames& F, const uint32_t* idx, const float* w,
-                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* T, hipStream_t st) {
+                          const uint8_t* is_static, float* pts, float* feat, const ShRotTables* T, hipStream_t st,
+                          bool compose) {
     if (S == 0 || F.n == 0) return 0;
     if (k != 8 || F.n < 0 || F.n > SAMPLE_FRAMES_MAX || ((uintptr_t)idx % 16) || ((uintptr_t)w % 16)) return -1;
     hipLaunchKernelGGL(samples_frames_kernel<8>, dim3(nblk(S, 256)), dim3(256), 0, st, S, F, idx, w, is_static, pts,
-                       feat, T);
+                       feat, T, compose ? 1 : 0);
     HIP_RET(hipGetLastError());
     return 0;
 }

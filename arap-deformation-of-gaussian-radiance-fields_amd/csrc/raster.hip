@@ -40,6 +40,15 @@ struct Geo {
 // it, so both see the same contributors.
 __device__ __forceinline__ float blend_exp(float power) { return __expf(power); }
 
+// The conservative test of record (xy, co) against tile (tx, ty)'s box of pixel
+// centres inside the image.
+__device__ __forceinline__ bool cull_tile_pass(const RasterParams& rp, float2 xy, float4 co, float lthr, int tx,
+                                               int ty) {
+    const int x0 = tx * rp.tile, y0 = ty * rp.tile;
+    return box_may_contribute(xy, co, lthr, (float)x0, (float)y0, (float)min(x0 + rp.tile - 1, rp.W - 1),
+                              (float)min(y0 + rp.tile - 1, rp.H - 1));
+}
+
 // One Gaussian in one view (record index gv = v*P + g, or g in segment mode);
 // the view's optional per-Gaussian outputs are indexed by gl (g, or in segment
 // mode the Gaussian's index within its view's segment).
@@ -162,12 +171,23 @@ __device__ __forceinline__ void project_one(const RasterParams& rp, const Raster
         }
         col[ch] = fmaxf(r + 0.5f, 0.0f);
     }
+    const float2 gxy = make_float2(ix, iy);
+    const float4 gco = make_float4(cc * dinv, -cb * dinv, ca * dinv, rp.opac[g]);
+    uint32_t count = (uint32_t)area;
+    if (rp.cull && area <= RASTER_CULL_MAX_AREA) {
+        // the tiles that pass the conservative test (emit_tiles_cull_kernel
+        // repeats the same test on the same values)
+        const float lthr = cull_log_threshold(gco.w);
+        count = 0u;
+        for (int ty = r0y; ty < r1y; ty++)
+            for (int tx = r0x; tx < r1x; tx++) count += cull_tile_pass(rp, gxy, gco, lthr, tx, ty) ? 1u : 0u;
+    }
     geo.depth[gv] = vz;
-    geo.xy[gv] = make_float2(ix, iy);
-    geo.conic[gv] = make_float4(cc * dinv, -cb * dinv, ca * dinv, rp.opac[g]);
+    geo.xy[gv] = gxy;
+    geo.conic[gv] = gco;
     geo.rgb[gv] = make_float4(col[0], col[1], col[2], 0.0f);
     geo.rect[gv] = make_int4(r0x, r0y, r1x, r1y);
-    geo.touched[gv] = (uint32_t)area;
+    geo.touched[gv] = count;
     if (radii) radii[gl] = rad;
     if (rects_out) { rects_out[2 * gl] = ex; rects_out[2 * gl + 1] = ey; }
 }
@@ -308,6 +328,96 @@ __global__ void __launch_bounds__(256) emit_tiles_kernel(int64_t PV, int P, cons
             const int y = y0 + q / w, x = x0 + q % w;
             keys[j] = b + (uint32_t)(y * gx + x);
             vals[j] = g;
+        }
+    }
+}
+
+// emit_tiles_kernel with emission-time culling: the wave walks the rectangle
+// slots of its 64 records (a wave-local scan of the rectangle areas), tests
+// each slot's tile as the preprocess counted it and writes the passing ones,
+// in rect order, to [offsets[r], offsets[r] + touched) by their rank (the
+// passing slots of the record in earlier 64-slot chunks, carried by the
+// record's lane, plus those below the slot in this chunk). A write past the
+// record's count is never made.
+__global__ void __launch_bounds__(256) emit_tiles_cull_kernel(RasterParams rp, int64_t PV, int ntiles, Geo geo,
+                                                              const uint32_t* __restrict__ rvals,
+                                                              const uint32_t* __restrict__ offsets,
+                                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t gv = 0, first = 0, cnt = 0, area = 0, base = 0;
+    int rx = 0, ry = 0, rw = 1, test = 0;
+    float2 xy = make_float2(0.0f, 0.0f);
+    float4 co = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float lthr = 0.0f;
+    if (r < PV) {
+        gv = rvals[r];
+        first = offsets[r];
+        cnt = geo.touched[gv];
+        if (cnt) {
+            const int4 rc = geo.rect[gv];
+            rx = rc.x;
+            ry = rc.y;
+            rw = rc.z - rc.x;
+            area = (uint32_t)(rw * (rc.w - rc.y));
+            base = (rp.seg ? (uint32_t)segment_of(rp.seg, rp.V, (int)gv) : gv / (uint32_t)rp.P) * (uint32_t)ntiles;
+            test = (rp.cull && (int)area <= RASTER_CULL_MAX_AREA) ? 1 : 0;
+            xy = geo.xy[gv];
+            co = geo.conic[gv];
+            lthr = cull_log_threshold(co.w);
+        }
+    }
+    // wave-inclusive scan of the areas: lane l's slots are [afirst, afirst + area)
+    uint32_t incl = area;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const uint32_t afirst = incl - area;
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t carry = 0;  // passing slots of this lane's record in earlier chunks
+    // uniform trip count: every lane takes part in every shuffle
+    for (uint32_t jb = 0; jb < total; jb += 64) {
+        const uint32_t j = jb + (uint32_t)lane;
+        int lo = 0;  // the last lane whose first slot is <= j (its area is > 0 when j < total)
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const uint32_t f = __shfl(afirst, lo + step);
+            if (f <= j) lo += step;
+        }
+        const uint32_t f0 = __shfl(afirst, lo);
+        const int w = __shfl(rw, lo), x0 = __shfl(rx, lo), y0 = __shfl(ry, lo);
+        const uint32_t b = __shfl(base, lo), g = __shfl(gv, lo), out0 = __shfl(first, lo), n_out = __shfl(cnt, lo);
+        const uint32_t cr = __shfl(carry, lo);
+        const int tst = __shfl(test, lo);
+        const float2 rxy = make_float2(__shfl(xy.x, lo), __shfl(xy.y, lo));
+        const float4 rco = make_float4(__shfl(co.x, lo), __shfl(co.y, lo), __shfl(co.z, lo), __shfl(co.w, lo));
+        const float rl = __shfl(lthr, lo);
+        bool pass = false;
+        int x = 0, y = 0;
+        if (j < total) {
+            const int q = (int)(j - f0);
+            y = y0 + q / w;
+            x = x0 + q % w;
+            pass = !tst || cull_tile_pass(rp, rxy, rco, rl, x, y);
+        }
+        const uint64_t pm = __ballot(pass);
+        const int lstart = (int)(f0 > jb ? f0 - jb : 0u);  // the record's first lane in this chunk (<= lane)
+        const uint64_t below = ((1ull << lane) - 1ull) & ~((1ull << lstart) - 1ull);
+        if (pass) {
+            const uint32_t rank = cr + (uint32_t)__popcll(pm & below);
+            if (rank < n_out) {
+                keys[out0 + rank] = b + (uint32_t)(y * rp.gx + x);
+                vals[out0 + rank] = g;
+            }
+        }
+        // this lane's record: the passing slots of this chunk inside its range
+        const int64_t s0 = (int64_t)afirst - (int64_t)jb, e0 = s0 + (int64_t)area;
+        const int ls = (int)max<int64_t>(0, min<int64_t>(64, s0)), le = (int)max<int64_t>(0, min<int64_t>(64, e0));
+        if (le > ls) {
+            const uint64_t hi = le == 64 ? ~0ull : ((1ull << le) - 1ull);
+            carry += (uint32_t)__popcll(pm & hi & ~((1ull << ls) - 1ull));
         }
     }
 }
@@ -1162,8 +1272,12 @@ int raster_bin_and_blend(const RasterParams& rp, void* geo_ws, uint32_t n_render
         uint32_t* offs = geo_offsets(geo_ws, PV);
         rc = launch_exclusive_scan(rtouched, PV, offs, (uint32_t*)d_total, scan_ws, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(emit_tiles_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.seg, rp.V, rp.gx,
-                           ntiles, geo, rvals, offs, keys, vals);
+        if (rp.cull)
+            hipLaunchKernelGGL(emit_tiles_cull_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, rp, PV, ntiles, geo, rvals,
+                               offs, keys, vals);
+        else
+            hipLaunchKernelGGL(emit_tiles_kernel, dim3(nblk(PV, 256)), dim3(256), 0, st, PV, rp.P, rp.seg, rp.V, rp.gx,
+                               ntiles, geo, rvals, offs, keys, vals);
         int tile_bits = 0;
         while ((int64_t(1) << tile_bits) < nt) tile_bits++;
         rc = launch_radix_sort32(keys, vals, keys_alt, vals_alt, n_rendered, tile_bits, sort_ws, st);

@@ -41,6 +41,8 @@ struct RasterParams {
     int unroll;            // Gaussians per compositing step of the 16x16 blend (2, 4, 8)
     int split;             // 16x16 tiles holding more instances than this are blended in segments of
                            // this many by separate workgroups and composited (0: one workgroup per tile)
+    int cull;              // 1: a record emits only the tiles of its rectangle that pass the conservative
+                           // tile test (rectangles of at most RASTER_CULL_MAX_AREA tiles)
     float boxmin[3], boxmax[3];
     const RasterView* views;  // device, V
     const int* seg;           // device, V+1, or null
@@ -93,6 +95,15 @@ __host__ __device__ inline bool box_may_contribute(float2 xy, float4 co, float l
 __host__ __device__ inline bool tile_may_contribute(float2 xy, float4 co, float x0, float y0, float x1, float y1) {
     return box_may_contribute(xy, co, cull_log_threshold(co.w), x0, y0, x1, y1);
 }
+
+// Emission-time tile culling (RasterParams::cull): a record whose rectangle
+// spans at most this many tiles emits only the tiles whose box (pixel centres
+// inside the image) passes box_may_contribute. Every sub-box the blend tests
+// (the 8x8 quadrants of a 16x16 tile, or the 8x8 tile itself) lies inside its
+// tile's box, so a record dropped from a tile is one the blend would have
+// dropped from every sub-box: the images are unchanged bit for bit, only the
+// instance lists are shorter. Larger rectangles emit every tile.
+constexpr int RASTER_CULL_MAX_AREA = 64;
 
 size_t raster_geo_bytes(int64_t PV);
 // Preprocess of all V views + scan of tiles touched. d_counts (device, 3 words):

@@ -47,7 +47,7 @@ from . import arapgs
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
                  stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
-                 aims_fn=None, sample_batch=1, node_cache=True, stream_c=None):
+                 aims_fn=None, sample_batch=1, node_cache=True, stream_c=None, sh_compose=False):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
@@ -59,7 +59,10 @@ class DragPipeline:
         than one needs split_samples, K = 8 and samples with features).
         node_cache: the deform side's blends through the LDS node-row cache
         (arapgs_set_node_cache; identical results). stream_c: the samples'
-        stream (a new stream when None)."""
+        stream (a new stream when None). sh_compose: a chunk's pending frames
+        rotate its features once, by the product of their rotations
+        (arapgs_set_sh_compose; features within float rounding of per-frame
+        rotations, positions bit-identical)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -84,6 +87,8 @@ class DragPipeline:
         for c in (self.ctx_b, self.ctx_c):
             if c is not None:
                 c.set_node_cache(node_cache)
+        if self.ctx_c is not None:
+            self.ctx_c.set_sh_compose(sh_compose)
         self.ev_c = [torch.cuda.Event() for _ in range(self.R)]
         self.ev_c_recorded = [False] * self.R
         self.st = state

@@ -164,21 +164,25 @@ def _frame(B, plain, delta, stop_rule):
     return n_inst
 
 
-def partition_streams(ctx, solve_cus, deform_skip):
+def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0):
     """CU-masked streams of the pipelined form. solve_cus > 0: ctx (the solve)
-    moves to a stream on CUs [0, solve_cus) and the deform and samples streams
-    run on the other CUs; else deform_skip > 0: the deform and samples streams
-    leave CUs [0, deform_skip) to the solve, which keeps every CU. Returns the
-    streams to close (after the caller restores ctx's stream) and the deform
-    and samples streams (None: the pipeline's own)."""
+    moves to a stream on CUs [0, solve_cus) and the deform stream runs on the
+    other CUs; the samples stream runs beside the deform (samples_cus 0), on
+    every CU (1) or beside the solve (2). Else deform_skip > 0: the deform and
+    samples streams leave CUs [0, deform_skip) to the solve, which keeps every
+    CU. Returns the streams to close (after the caller restores ctx's stream)
+    and the deform and samples streams (None: the pipeline's own)."""
     import torch
     from arapgs_amd import arapgs
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     if solve_cus > 0:
-        streams = [arapgs.CuStream(0, 0, solve_cus)] + [arapgs.CuStream(0, solve_cus, n_cu - solve_cus)
-                                                        for _ in range(2)]
+        streams = [arapgs.CuStream(0, 0, solve_cus), arapgs.CuStream(0, solve_cus, n_cu - solve_cus)]
+        if samples_cus == 0:
+            streams.append(arapgs.CuStream(0, solve_cus, n_cu - solve_cus))
+        elif samples_cus == 2:
+            streams.append(arapgs.CuStream(0, 0, solve_cus))
         ctx.set_stream(streams[0].torch)
-        return streams, streams[1].torch, streams[2].torch
+        return streams, streams[1].torch, streams[2].torch if len(streams) > 2 else None
     if deform_skip > 0:
         streams = [arapgs.CuStream(0, deform_skip, n_cu - deform_skip) for _ in range(2)]
         return streams, streams[0].torch, streams[1].torch
@@ -198,7 +202,8 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     reset_state(B)
     ctx = B["ctx"]
     base_stream = ctx.stream
-    streams, stream_b, stream_c = partition_streams(ctx, cus, B["cfg"].get("deform_skip", 0))
+    streams, stream_b, stream_c = partition_streams(ctx, cus, B["cfg"].get("deform_skip", 0),
+                                                    B["cfg"].get("samples_cus", 0))
     try:
         B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
         B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
@@ -209,9 +214,11 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
                             stream_b=stream_b, stream_c=stream_c, sample_batch=nb,
-                            node_cache=bool(B["cfg"].get("node_cache", 1)))
+                            node_cache=bool(B["cfg"].get("node_cache", 1)),
+                            sh_compose=bool(B["cfg"].get("sh_compose", 0)))
         pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
         pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
+        pipe.ctx_b.set_tile_cull(B["cfg"].get("tile_cull", 0))
         pipe.run(warmup)
         torch.cuda.synchronize()
         stamps = []
@@ -308,6 +315,7 @@ def apply_config(B, cfg):
     ctx.set_blend_unroll(cfg["blend_unroll"])
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
+    ctx.set_tile_cull(cfg.get("tile_cull", 0))
     B["cfg"] = dict(cfg)
 
 
@@ -412,10 +420,11 @@ def record_frames(B, steps, warmup):
     B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
     pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
                      B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1),
-                     node_cache=bool(B["cfg"].get("node_cache", 1)))
+                     node_cache=bool(B["cfg"].get("node_cache", 1)), sh_compose=bool(B["cfg"].get("sh_compose", 0)))
     pipe.rec = []
     pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
     pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
+    pipe.ctx_b.set_tile_cull(B["cfg"].get("tile_cull", 0))
     pipe.run(warmup + steps)
     torch.cuda.synchronize()
     same = all(torch.equal(final_timed[k], B["state"][k]) for k in final_timed)
@@ -664,12 +673,15 @@ def large_leg(args, tuning):
     torch.cuda.synchronize()
     # the drag workload's tuned schedule (identical results)
     base_stream = ctx.stream
-    masked, stream_b, stream_c = partition_streams(ctx, tuning.get("solve_cus", 0), tuning.get("deform_skip", 0))
+    masked, stream_b, stream_c = partition_streams(ctx, tuning.get("solve_cus", 0), tuning.get("deform_skip", 0),
+                                                   tuning.get("samples_cus", 0))
     pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
                         aims_fn=aims_fn, sample_batch=tuning.get("sample_batch", 1),
-                        node_cache=bool(tuning.get("node_cache", 1)), stream_b=stream_b, stream_c=stream_c)
+                        node_cache=bool(tuning.get("node_cache", 1)), stream_b=stream_b, stream_c=stream_c,
+                        sh_compose=bool(tuning.get("sh_compose", 0)))
     pipe.ctx_b.set_blend_unroll(tuning.get("blend_unroll", 4))
     pipe.ctx_b.set_blend_split(tuning.get("blend_split", 0))
+    pipe.ctx_b.set_tile_cull(tuning.get("tile_cull", 0))
     pipe.run(args.warmup)
     torch.cuda.synchronize()
     stamps = []
@@ -940,6 +952,17 @@ def main():
                     help="CG iterations between host checks of the fast and pipelined forms (0: autotuned, or 16)")
     ap.add_argument("--cg-predict", type=int, default=-1, choices=(-1, 0, 1),
                     help="first CG host check at the previous frame's count (-1: autotuned, or off)")
+    ap.add_argument("--solve-cus", type=int, default=-1,
+                    help="pipelined form: the solve on CUs [0, N), the rest for the deform and samples (-1: autotuned, "
+                         "or 0 = all CUs shared)")
+    ap.add_argument("--samples-cus", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="with --solve-cus: the samples stream beside the deform (0), on every CU (1) or beside the "
+                         "solve (2) (-1: autotuned, or 0)")
+    ap.add_argument("--sh-compose", type=int, default=-1, choices=(-1, 0, 1),
+                    help="rolling sample chunks: one SH rotation by the product of the pending frames' rotations "
+                         "(-1: autotuned, or 0)")
+    ap.add_argument("--tile-cull", type=int, default=-1, choices=(-1, 0, 1),
+                    help="renders emit only the tiles passing the conservative tile test (-1: autotuned, or 0)")
     ap.add_argument("--partition", action="store_true",
                     help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
@@ -986,6 +1009,14 @@ def main():
         apply_config(B, dict(B["cfg"], cg_predict=args.cg_predict))
     if args.host_checks > 0:
         apply_config(B, dict(B["cfg"], host_checks=args.host_checks))
+    if args.solve_cus >= 0:
+        apply_config(B, dict(B["cfg"], solve_cus=args.solve_cus))
+    if args.samples_cus >= 0:
+        apply_config(B, dict(B["cfg"], samples_cus=args.samples_cus))
+    if args.sh_compose >= 0:
+        apply_config(B, dict(B["cfg"], sh_compose=args.sh_compose))
+    if args.tile_cull >= 0:
+        apply_config(B, dict(B["cfg"], tile_cull=args.tile_cull))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

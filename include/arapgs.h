@@ -301,6 +301,12 @@ int arapgs_sample_sh(arapgs_ctx* ctx, int64_t s, int k, const float* w, const ui
 int arapgs_samples_frames(arapgs_ctx* ctx, int64_t s, int k, int nf, const arapgs_node_xf* const* tables_h,
                           const float* const* node_q_h, const uint32_t* idx, const float* w, const uint8_t* is_static,
                           float* samples, float* feat);
+/* on = 1: arapgs_samples_frames on this context rotates each feature once, by
+ * the product R_{nf-1} ... R_1 R_0 of the frames' slerped rotations, instead of
+ * nf times (the SH rotation is a group representation, D(R_b) D(R_a) =
+ * D(R_b R_a)): positions stay bit-identical, features agree with the per-frame
+ * calls within float rounding. No counterpart in the reference. Default 0. */
+int arapgs_set_sh_compose(arapgs_ctx* ctx, int on);
 
 /* ------------------------------------------------------------ grid + density */
 typedef struct arapgs_grid arapgs_grid;
@@ -536,6 +542,16 @@ int arapgs_set_blend_unroll(arapgs_ctx* ctx, int unroll);
  * differs (float rounding). Renders that ask for `surface` and the adjoint
  * chain always blend tile by tile. */
 int arapgs_set_blend_split(arapgs_ctx* ctx, int entries);
+
+/* on = 1: the context's renders emit, for a Gaussian whose tile rectangle spans
+ * at most 64 tiles, only the tiles whose box of pixel centres passes the
+ * conservative alpha test the blend applies to every 8x8 quadrant (the
+ * Gaussian cannot reach alpha 1/255 there): the sort, range and blend work on
+ * shorter lists and the images, contributor marks and gradients are identical
+ * bit for bit; the instance keys and values are the subsequence of the full
+ * binning's that passes the test. No counterpart in the reference (its
+ * binning emits the whole rectangle). Default 0. */
+int arapgs_set_tile_cull(arapgs_ctx* ctx, int on);
 
 /* Debug view of the last arapgs_render on this context: sorted keys
  * (u64, tile << 32 | depth bits) and values (Gaussian ids) copied to host

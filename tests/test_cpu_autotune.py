@@ -13,7 +13,9 @@ def _cost(cfg):
             + abs(cfg["cg_block"] - 128) / 128 + (0.2 if cfg["cg_graphs"] == 0 else 0.0)
             + (0.1 if cfg["node_cache"] == 0 else 0.0) + abs(cfg["blend_split"] - 2048) / 8192
             + abs(cfg["deform_skip"] - 32) / 64 + (0.15 if cfg["cg_mode"] == 0 else 0.0)
-            + (0.05 if cfg["cg_predict"] == 0 else 0.0) + abs(cfg["solve_cus"] - 64) / 256 + 1.0)
+            + (0.05 if cfg["cg_predict"] == 0 else 0.0) + abs(cfg["solve_cus"] - 64) / 256
+            + (0.02 if cfg["samples_cus"] != 1 else 0.0) + (0.03 if cfg["sh_compose"] == 0 else 0.0)
+            + (0.04 if cfg["tile_cull"] == 0 else 0.0) + 1.0)
 
 
 def test_finds_the_separable_optimum():
@@ -26,7 +28,8 @@ def test_finds_the_separable_optimum():
     best, trials = A.tune(measure)
     assert best == dict(tile=16, blend_unroll=8, aggregates=192, refresh=16, host_checks=8, warm=0, cg_block=128,
                         cg_graphs=1, cg_mode=2, cg_predict=1, node_cache=1, blend_split=2048, deform_skip=32,
-                        solve_cus=64)
+                        solve_cus=64, samples_cus=1, sh_compose=1,
+                        tile_cull=1)
     assert len(seen) == len(set(seen)) == len(trials)
     assert min(t for _, t in trials) == _cost(best)
 

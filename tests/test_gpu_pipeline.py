@@ -17,13 +17,17 @@ pytestmark = pytest.mark.gpu
 FRAMES = 5
 
 
-@pytest.mark.parametrize("sample_batch,masked", [(1, False), (3, False), (8, False), (3, True)])
-def test_pipelined_frames_bit_identical(ctx, sample_batch, masked):
+@pytest.mark.parametrize("sample_batch,masked,compose", [(1, False, False), (3, False, False), (8, False, False),
+                                                         (3, True, False), (4, False, True)])
+def test_pipelined_frames_bit_identical(ctx, sample_batch, masked, compose):
     """Pipelined frames (and the rolling sample chunks of sample_batch > 1) equal
     sequential frames bit for bit; the drag is run as two run() calls (3 + 2
     frames), so the table slots and chunk records carry over between runs.
     masked: the deform and samples streams are CU-masked streams that leave the
-    first 32 CUs to the solve (one from a CU range, one from an explicit mask)."""
+    first 32 CUs to the solve (one from a CU range, one from an explicit mask).
+    compose: the chunks' pending frames rotate the features once by the product
+    of their rotations; everything but the features stays bit-identical and
+    the features agree within 2e-6 of their scale."""
     import torch
     sc = scenes.capsule_scene(7, n=40000, n_nodes=128, W=160, H=128, n_views=2)
     sc.grid = 32
@@ -60,13 +64,16 @@ def test_pipelined_frames_bit_identical(ctx, sample_batch, masked):
         masks = [arapgs.CuStream(0, 32, n_cu - 32), arapgs.CuStream(0, cus=list(range(32, n_cu)))]
     pipe = DragPipeline(solver2, st2, K, W["active"], delta, iters, W["cams"], imgs_pipe, W["bg"], W["opac"],
                         sample_batch=sample_batch, stream_b=masks[0].torch if masks else None,
-                        stream_c=masks[1].torch if masks else None)
+                        stream_c=masks[1].torch if masks else None, sh_compose=compose)
     pipe.run(3)
     pipe.run(FRAMES - 3)
     torch.cuda.synchronize()
 
     for key in ("pos", "rot", "scale", "sh", "ends", "node_pos", "samples", "sample_feat"):
         a, b = st[key].cpu().numpy(), st2[key].cpu().numpy()
+        if compose and key == "sample_feat":
+            assert np.abs(a - b).max() <= 2e-6 * np.abs(a).max(), key
+            continue
         assert np.array_equal(a, b), key
     for a, b in zip(imgs_seq, imgs_pipe):
         assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
@@ -108,3 +115,20 @@ def test_samples_frames_equal_per_frame_calls(ctx):
     assert torch.equal(p_bat, p_ref)
     assert torch.equal(f_bat, f_ref)
     assert not torch.equal(f_bat, feat0)
+    # one SH rotation by the product of the frames' rotations (set_sh_compose):
+    # positions bit for bit, features within float rounding of the per-frame
+    # rotations (|error| <= 2e-6 of the feature scale after 5 frames), static
+    # samples untouched
+    c2 = arapgs.Context(0)
+    c2.set_sh_compose(True)
+    p_cmp, f_cmp = T(pts), feat0.clone()
+    c2.samples_frames(tables[:3], quats[:3], idx, w, static, p_cmp, f_cmp)
+    c2.samples_frames(tables[3:], quats[3:], idx, w, static, p_cmp, f_cmp)
+    torch.cuda.synchronize()
+    assert torch.equal(p_cmp, p_ref)
+    err = (f_cmp - f_ref).abs().max().item()
+    print(f"composed SH rotation: max |feature error| {err:.3e}", flush=True)
+    assert err <= 2e-6 * f_ref.abs().max().item()
+    st = static.bool()
+    assert torch.equal(f_cmp[st], feat0[st])
+    c2.close()

@@ -208,3 +208,45 @@ def test_blend_split_deep_tiles(ctx, scene, split):
         assert np.array_equal(H(surf["image"]), img0)
     finally:
         ctx.set_blend_split(0)
+
+
+@pytest.mark.parametrize("W,H_,tile,fast,dist", [(800, 800, 16, True, 6.0), (640, 360, 16, False, 3.0),
+                                                 (320, 240, 8, True, 2.0), (1920, 1080, 16, True, 2.5)])
+def test_tile_cull_keeps_images(ctx, scene, W, H_, tile, fast, dist):
+    """Emission-time tile culling (set_tile_cull): the image, the contributor
+    marks and the radii are bit-identical to the full binning; the instance
+    lists are the subsequence of the full binning's (keys, values) that passes
+    the conservative tile test, and shorter."""
+    import torch
+    from arapgs_amd import arapgs
+    g = scene["g"]
+    cam = S.look_at_camera((0.3, -dist, 0.8), (0.0, 0.0, 0.0), 50.0, W, H_)
+    bg = np.array([0.8, 0.8, 0.8], np.float32)
+    gx, gy = (W + tile - 1) // tile, (H_ + tile - 1) // tile
+    outs = []
+    for cull in (False, True):
+        c = arapgs.Context(0)
+        c.set_tile_cull(cull)
+        out = c.render(T(g.pos), T(g.sh), T(g.opacity), T(g.scale), T(g.rot), T(cam.view), T(cam.proj),
+                       T(cam.campos), cam.tanfovx, cam.tanfovy, W, H_, T(bg), tile=tile, fast_culling=fast,
+                       want_surface=True, want_radii=True)
+        torch.cuda.synchronize()
+        n = out["num_rendered"]
+        keys, vals, ranges = c.render_debug(n, gx * gy)
+        # render_debug's keys are (tile << 32 | depth bits)
+        outs.append((H(out["image"]), H(out["surface"]), H(out["radii"]), (keys >> np.uint64(32)).astype(np.int64),
+                     vals.astype(np.int64), n))
+        c.close()
+    full, cul = outs
+    print(f"tile cull {W}x{H_} tile {tile}: {full[5]} -> {cul[5]} instances", flush=True)
+    assert np.array_equal(full[0], cul[0])
+    assert np.array_equal(full[1], cul[1])
+    assert np.array_equal(full[2], cul[2])
+    assert 0 < cul[5] < full[5]
+    # subsequence: every culled (key, value) pair appears in the full lists in the same order
+    # (tiles ascending; within a tile the records in depth order, not value order)
+    assert np.all(np.diff(full[3]) >= 0) and np.all(np.diff(cul[3]) >= 0)
+    tiles = np.unique(cul[3])
+    for t in tiles[:: max(1, len(tiles) // 200)]:
+        it = iter(full[4][full[3] == t].tolist())
+        assert all(x in it for x in cul[4][cul[3] == t].tolist()), t
