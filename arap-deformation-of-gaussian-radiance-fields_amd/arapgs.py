@@ -124,6 +124,8 @@ EXPORTS = {
     "arapgs_solver_set_coarse_refresh": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_predict": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_coarse_lag": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_side_stream": ([P, P], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_solver_reset": ([P], C.c_int),
     "arapgs_solver_last_stats": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -693,6 +695,17 @@ class Solver:
         refresh count): the next solve computes what a new solver's would."""
         if lib().arapgs_solver_reset(self.h) != 0:
             raise ArapgsError("solver reset failed")
+
+    def set_side_stream(self, stream=None):
+        """The stream of the asynchronous coarse rebuilds (a torch stream, or None for the solver's own)."""
+        h = stream.cuda_stream if stream is not None else 0
+        if lib().arapgs_solver_set_side_stream(self.h, P(h) if h else None) != 0:
+            raise ArapgsError("set_side_stream failed")
+
+    def set_coarse_lag(self, solves):
+        """A side-stream coarse rebuild is adopted by the solves-th solve after the one that started it."""
+        if lib().arapgs_solver_set_coarse_lag(self.h, int(solves)) != 0:
+            raise ArapgsError(f"set_coarse_lag({solves}) failed")
 
     def set_cg_predict(self, on=True):
         """First CG host check of each GN iteration at the previous solve's count (same results)."""

@@ -474,6 +474,7 @@ int arapgs_solver_reset(arapgs_solver* s) {
         hipMemset(s->w.hwarm, 0, sizeof(float) * (size_t)s->w.warm_slots * s->w.n) != hipSuccess)
         return -1000;
     s->w.pending = false;
+    s->w.pending_age = 0;
     s->w.have_coarse = false;
     s->solves = 0;
     for (int& v : s->cg_pred) v = 0;
@@ -501,6 +502,21 @@ int arapgs_solver_set_cg_block(arapgs_solver* s, int threads) {
 int arapgs_solver_set_cg_graphs(arapgs_solver* s, int on) {
     if (!s || on < 0 || on > 1) return -1;
     s->one.graphs = on != 0;
+    return 0;
+}
+
+int arapgs_solver_set_side_stream(arapgs_solver* s, void* stream) {
+    if (!s) return -1;
+    (void)hipSetDevice(s->device);
+    // a rebuild in flight on the old side stream finishes before the switch
+    if (hipDeviceSynchronize() != hipSuccess) return -1000;
+    s->one.side_ext = (hipStream_t)stream;
+    return 0;
+}
+
+int arapgs_solver_set_coarse_lag(arapgs_solver* s, int solves) {
+    if (!s || solves < 1 || solves > 8) return -1;
+    s->one.lag = solves;
     return 0;
 }
 

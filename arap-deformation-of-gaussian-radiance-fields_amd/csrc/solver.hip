@@ -1971,6 +1971,8 @@ void solver_batch_free(SolverBatch& B) {
 int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::string& err) {
     // settings survive a rebuild of the entries
     const bool async = B.async, graphs = B.graphs;
+    const int lag = B.lag;
+    hipStream_t side_ext = B.side_ext;
     solver_batch_free(B);
     if (w.empty() || (int)w.size() > SOLVER_BATCH_MAX) {
         err = "solver batch: need 1.." + std::to_string(SOLVER_BATCH_MAX) + " solvers";
@@ -1979,6 +1981,8 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
     B = SolverBatch();
     B.async = async;
     B.graphs = graphs;
+    B.lag = lag;
+    B.side_ext = side_ext;
     B.w = w;
     B.kt = 8;
     std::vector<SolverEntry> host;
@@ -2112,9 +2116,14 @@ static int coarse_build(const SolverBatch& B, const SolverEntry* E, hipStream_t 
 int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
     const SolverEntry* E = B.entries;
     const unsigned nz = (unsigned)B.w.size();
-    // adopt side-stream rebuilds that an earlier frame started
+    // adopt side-stream rebuilds that the lag-th earlier solve started
+    bool any_pending = false;
     for (SolverWork* W : B.w) {
         if (!W->pending) continue;
+        if (++W->pending_age < B.lag) {
+            any_pending = true;
+            continue;
+        }
         HIP_RET(hipStreamWaitEvent(st, W->built_ev, 0));
         const size_t nc = (size_t)W->dev.nc_pad;
         HIP_RET(hipMemcpyAsync(W->acinv, W->acinv_alt, sizeof(float) * nc * nc, hipMemcpyDeviceToDevice, st));
@@ -2123,7 +2132,7 @@ int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
         W->pending = false;
     }
     hipLaunchKernelGGL(init_x_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
-    if (refresh_coarse) {
+    if (refresh_coarse && !any_pending) {
         bool all_have = true;
         for (const SolverWork* W : B.w) all_have = all_have && W->have_coarse;
         if (!B.async || !all_have) {
@@ -2137,12 +2146,14 @@ int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
                                (const SolverEntry*)B.entries_build);
             HIP_RET(hipGetLastError());
             HIP_RET(hipEventRecord(B.fork_ev, st));
-            HIP_RET(hipStreamWaitEvent(B.side, B.fork_ev, 0));
-            int rc = coarse_build(B, B.entries_build, B.side);
+            hipStream_t side = B.side_ext ? B.side_ext : B.side;
+            HIP_RET(hipStreamWaitEvent(side, B.fork_ev, 0));
+            int rc = coarse_build(B, B.entries_build, side);
             if (rc) return rc;
             for (SolverWork* W : B.w) {
-                HIP_RET(hipEventRecord(W->built_ev, B.side));
+                HIP_RET(hipEventRecord(W->built_ev, side));
                 W->pending = true;
+                W->pending_age = 0;
             }
         }
     }

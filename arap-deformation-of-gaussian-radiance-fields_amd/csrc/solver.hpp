@@ -142,6 +142,7 @@ struct SolverWork {
     float* x_ident = nullptr;  // n: identity rotations, zero translations (x0 of the Galerkin operator)
     hipEvent_t built_ev = nullptr;
     bool pending = false;      // a side-stream rebuild waits to be adopted
+    int pending_age = 0;       // solves set up since that rebuild started
     bool have_coarse = false;  // a coarse inverse exists (the first one is built in line)
     std::vector<void*> allocs;  // owned device buffers
 };
@@ -154,8 +155,11 @@ struct SolverBatch {
     SolverEntry* entries = nullptr;   // device, w.size()
     SolverEntry* entries_build = nullptr;  // device: the entries of a side-stream coarse rebuild
     hipStream_t side = nullptr;       // side stream of the asynchronous coarse refresh
+    hipStream_t side_ext = nullptr;   // a caller's stream used instead of it (not owned)
     hipEvent_t fork_ev = nullptr;
     bool async = true;                // coarse refreshes on the side stream (the first one in line)
+    int lag = 1;                      // a side-stream rebuild is adopted by the lag-th solve after the one
+                                      // that started it; no new rebuild starts while one is pending
     std::vector<const float*> g, aim; // what the device entries hold
     // Launch sequences replayed as captured HIP graphs: the CG chunks (CG
     // iterations [k0, k0 + L) and the flag kernel) and the Gauss-Newton

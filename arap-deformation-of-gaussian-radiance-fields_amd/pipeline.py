@@ -47,7 +47,8 @@ from . import arapgs
 class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
                  stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
-                 aims_fn=None, sample_batch=1, node_cache=True, stream_c=None, sh_compose=False):
+                 aims_fn=None, sample_batch=1, node_cache=True, stream_c=None, sh_compose=False,
+                 samples_late=False):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
@@ -62,7 +63,9 @@ class DragPipeline:
         stream (a new stream when None). sh_compose: a chunk's pending frames
         rotate its features once, by the product of their rotations
         (arapgs_set_sh_compose; features within float rounding of per-frame
-        rotations, positions bit-identical)."""
+        rotations, positions bit-identical). samples_late: the samples stream
+        starts a frame's work after that frame's six-point update instead of
+        beside it, so the two HBM-heavy passes do not overlap (same results)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -89,6 +92,8 @@ class DragPipeline:
                 c.set_node_cache(node_cache)
         if self.ctx_c is not None:
             self.ctx_c.set_sh_compose(sh_compose)
+        self.samples_late = samples_late
+        self.ev_six = [torch.cuda.Event() for _ in range(self.R)]
         self.ev_c = [torch.cuda.Event() for _ in range(self.R)]
         self.ev_c_recorded = [False] * self.R
         self.st = state
@@ -168,6 +173,8 @@ class DragPipeline:
         torch = self.torch
         with torch.cuda.stream(self.stream_c):
             self.stream_c.wait_event(self.ev_a[slot])
+            if self.samples_late:
+                self.stream_c.wait_event(self.ev_six[slot])
             self.ctx_c.node_quats(self.tables[slot], out=self.quats[slot])
             for c in (range(self.nb) if last else [t % self.nb]):
                 self._update_chunk(c, t)
@@ -178,22 +185,33 @@ class DragPipeline:
         table = self.tables[slot]
         t = self.frame
         self.frame += 1
-        if self.split and self.nb > 1:
-            self._rolling_samples(slot, t, last)
-        elif self.split:
-            with torch.cuda.stream(self.stream_c):
-                self.stream_c.wait_event(self.ev_a[slot])
-                self._samples(self.ctx_c, table)
-                self.ev_c[slot].record(self.stream_c)
-                self.ev_c_recorded[slot] = True
+
+        def samples():
+            if self.split and self.nb > 1:
+                self._rolling_samples(slot, t, last)
+            elif self.split:
+                with torch.cuda.stream(self.stream_c):
+                    self.stream_c.wait_event(self.ev_a[slot])
+                    if self.samples_late:
+                        self.stream_c.wait_event(self.ev_six[slot])
+                    self._samples(self.ctx_c, table)
+                    self.ev_c[slot].record(self.stream_c)
+                    self.ev_c_recorded[slot] = True
+
+        if not self.samples_late:
+            samples()
         with torch.cuda.stream(self.stream_b):
             self.stream_b.wait_event(self.ev_a[slot])
             if not self.split:
                 self._samples(ctx, table)
             ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"], st.get("gs_static"),
                                  st["pos"], st["rot"], st["scale"], st["sh"])
+            self.ev_six[slot].record(self.stream_b)
             self.ev_b[slot].record(self.stream_b)
             self.ev_b_recorded[slot] = True
+        if self.samples_late:
+            samples()
+        with torch.cuda.stream(self.stream_b):
             n_inst = 0
             if render and self.images:
                 img = self.images[0]

@@ -164,14 +164,16 @@ def _frame(B, plain, delta, stop_rule):
     return n_inst
 
 
-def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0):
+def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0, side=None):
     """CU-masked streams of the pipelined form. solve_cus > 0: ctx (the solve)
     moves to a stream on CUs [0, solve_cus) and the deform stream runs on the
     other CUs; the samples stream runs beside the deform (samples_cus 0), on
     every CU (1) or beside the solve (2). Else deform_skip > 0: the deform and
     samples streams leave CUs [0, deform_skip) to the solve, which keeps every
-    CU. Returns the streams to close (after the caller restores ctx's stream)
-    and the deform and samples streams (None: the pipeline's own)."""
+    CU. side: a solver whose asynchronous coarse rebuilds move to a stream on
+    the deform side's CUs (solve_cus > 0). Returns the streams to close (after
+    the caller restores ctx's stream and the solver's side stream) and the
+    deform and samples streams (None: the pipeline's own)."""
     import torch
     from arapgs_amd import arapgs
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
@@ -182,7 +184,11 @@ def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0):
         elif samples_cus == 2:
             streams.append(arapgs.CuStream(0, 0, solve_cus))
         ctx.set_stream(streams[0].torch)
-        return streams, streams[1].torch, streams[2].torch if len(streams) > 2 else None
+        stream_c = streams[2].torch if len(streams) > 2 else None
+        if side is not None:
+            streams.append(arapgs.CuStream(0, solve_cus, n_cu - solve_cus))
+            side.set_side_stream(streams[-1].torch)
+        return streams, streams[1].torch, stream_c
     if deform_skip > 0:
         streams = [arapgs.CuStream(0, deform_skip, n_cu - deform_skip) for _ in range(2)]
         return streams, streams[0].torch, streams[1].torch
@@ -202,8 +208,9 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     reset_state(B)
     ctx = B["ctx"]
     base_stream = ctx.stream
+    side = B["solver"] if B["cfg"].get("side_cus", 0) else None
     streams, stream_b, stream_c = partition_streams(ctx, cus, B["cfg"].get("deform_skip", 0),
-                                                    B["cfg"].get("samples_cus", 0))
+                                                    B["cfg"].get("samples_cus", 0), side=side)
     try:
         B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
         B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
@@ -215,7 +222,8 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
                             stream_b=stream_b, stream_c=stream_c, sample_batch=nb,
                             node_cache=bool(B["cfg"].get("node_cache", 1)),
-                            sh_compose=bool(B["cfg"].get("sh_compose", 0)))
+                            sh_compose=bool(B["cfg"].get("sh_compose", 0)),
+                            samples_late=bool(B["cfg"].get("samples_late", 0)))
         pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
         pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
         pipe.ctx_b.set_tile_cull(B["cfg"].get("tile_cull", 0))
@@ -231,6 +239,8 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     finally:
         torch.cuda.synchronize()
         ctx.set_stream(base_stream)
+        if side is not None:
+            side.set_side_stream(None)
         for s_ in streams:
             s_.close()
     times = [(b - a) * 1e3 for a, b in zip([t0] + stamps[:-1], stamps)]
@@ -308,6 +318,7 @@ def apply_config(B, cfg):
             B["solver"] = workload.make_solver(ctx, B["solver_args"], n_aggregates=cfg["aggregates"])
         torch.cuda.synchronize()
     B["solver"].set_coarse_refresh(cfg["refresh"])
+    B["solver"].set_coarse_lag(cfg.get("coarse_lag", 1))
     B["solver"].set_cg_block(cfg.get("cg_block", 256))
     B["solver"].set_cg_graphs(cfg.get("cg_graphs", 0))
     B["solver"].set_cg_mode(cfg.get("cg_mode", 0))
@@ -420,7 +431,8 @@ def record_frames(B, steps, warmup):
     B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
     pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
                      B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1),
-                     node_cache=bool(B["cfg"].get("node_cache", 1)), sh_compose=bool(B["cfg"].get("sh_compose", 0)))
+                     node_cache=bool(B["cfg"].get("node_cache", 1)), sh_compose=bool(B["cfg"].get("sh_compose", 0)),
+                            samples_late=bool(B["cfg"].get("samples_late", 0)))
     pipe.rec = []
     pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
     pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
@@ -645,6 +657,7 @@ def large_leg(args, tuning):
     solver.set_cg_graphs(tuning.get("cg_graphs", 0))
     solver.set_cg_mode(tuning.get("cg_mode", 0))
     solver.set_cg_predict(tuning.get("cg_predict", 0))
+    solver.set_coarse_lag(tuning.get("coarse_lag", 1))
     s0 = workload.clone_state(st)
     # a few sequential frames with the kernels timed by HIP events
     fb = arapgs.frame_buffers(st, K)
@@ -674,11 +687,13 @@ def large_leg(args, tuning):
     # the drag workload's tuned schedule (identical results)
     base_stream = ctx.stream
     masked, stream_b, stream_c = partition_streams(ctx, tuning.get("solve_cus", 0), tuning.get("deform_skip", 0),
-                                                   tuning.get("samples_cus", 0))
+                                                   tuning.get("samples_cus", 0),
+                                                   side=solver if tuning.get("side_cus", 0) else None)
     pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
                         aims_fn=aims_fn, sample_batch=tuning.get("sample_batch", 1),
                         node_cache=bool(tuning.get("node_cache", 1)), stream_b=stream_b, stream_c=stream_c,
-                        sh_compose=bool(tuning.get("sh_compose", 0)))
+                        sh_compose=bool(tuning.get("sh_compose", 0)),
+                        samples_late=bool(tuning.get("samples_late", 0)))
     pipe.ctx_b.set_blend_unroll(tuning.get("blend_unroll", 4))
     pipe.ctx_b.set_blend_split(tuning.get("blend_split", 0))
     pipe.ctx_b.set_tile_cull(tuning.get("tile_cull", 0))
@@ -694,6 +709,7 @@ def large_leg(args, tuning):
     pipe.close()
     torch.cuda.synchronize()
     ctx.set_stream(base_stream)
+    solver.set_side_stream(None)
     for s_ in masked:
         s_.close()
     gn, cg = solver.last_stats()
@@ -963,6 +979,15 @@ def main():
                          "(-1: autotuned, or 0)")
     ap.add_argument("--tile-cull", type=int, default=-1, choices=(-1, 0, 1),
                     help="renders emit only the tiles passing the conservative tile test (-1: autotuned, or 0)")
+    ap.add_argument("--refresh", type=int, default=0,
+                    help="solves between coarse-operator rebuilds (0: autotuned, or 8)")
+    ap.add_argument("--side-cus", type=int, default=-1, choices=(-1, 0, 1),
+                    help="with --solve-cus: the asynchronous coarse rebuilds on the deform side's CUs (1) or on "
+                         "every CU (0) (-1: autotuned, or 0)")
+    ap.add_argument("--samples-late", type=int, default=-1, choices=(-1, 0, 1),
+                    help="the samples stream starts after the six-point update (-1: autotuned, or 0)")
+    ap.add_argument("--coarse-lag", type=int, default=0,
+                    help="solves between a side-stream coarse rebuild's start and its adoption (0: autotuned, or 1)")
     ap.add_argument("--partition", action="store_true",
                     help="sweep CU partitions between the solve and deform streams of the pipelined form")
     ap.add_argument("--adjoint-n", type=int, default=500_000)
@@ -1017,6 +1042,14 @@ def main():
         apply_config(B, dict(B["cfg"], sh_compose=args.sh_compose))
     if args.tile_cull >= 0:
         apply_config(B, dict(B["cfg"], tile_cull=args.tile_cull))
+    if args.coarse_lag > 0:
+        apply_config(B, dict(B["cfg"], coarse_lag=args.coarse_lag))
+    if args.refresh > 0:
+        apply_config(B, dict(B["cfg"], refresh=args.refresh))
+    if args.side_cus >= 0:
+        apply_config(B, dict(B["cfg"], side_cus=args.side_cus))
+    if args.samples_late >= 0:
+        apply_config(B, dict(B["cfg"], samples_late=args.samples_late))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

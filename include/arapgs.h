@@ -156,6 +156,17 @@ int arapgs_solver_set_coarse_refresh(arapgs_solver* s, int frames);
  * of operators is a function of the solve count only, so the results do not
  * depend on the streams or the host. */
 int arapgs_solver_set_async_refresh(arapgs_solver* s, int on);
+/* With the asynchronous refresh: a side-stream rebuild is adopted by the
+ * solves-th solve after the one that started it (1..8, default 1), so a rebuild
+ * that takes longer than a frame does not hold up the next solve; no new
+ * rebuild starts while one is waiting. The operator sequence stays a function
+ * of the solve count (deterministic). No counterpart in the reference. */
+int arapgs_solver_set_coarse_lag(arapgs_solver* s, int solves);
+/* The stream the asynchronous coarse rebuilds run on: a caller's stream (e.g.
+ * one restricted to some compute units; it must outlive its use by the solver)
+ * or null for the solver's own. Synchronises the device. No counterpart in
+ * the reference. */
+int arapgs_solver_set_side_stream(arapgs_solver* s, void* stream);
 /* The CG block size (no counterpart in the reference; set by the autotuner):
  * workgroup size of the CG operator kernel, 64, 128, 256 or 512 threads (16
  * lanes per free node, so 4 to 32 nodes per workgroup; default 256); in the

@@ -426,3 +426,34 @@ def test_cg_predict_bit_identical(ctx, prob, mode):
         assert a[2] == b[2] >= 1 and a[3] == b[3]
         assert np.array_equal(a[0], b[0])
         assert np.array_equal(a[1][: a[2] + 1], b[1][: b[2] + 1])
+
+
+@pytest.mark.parametrize("lag,refresh", [(2, 1), (3, 2)])
+def test_coarse_lag_keeps_oracle_energies(ctx, prob, lag, refresh):
+    """A side-stream coarse rebuild adopted `lag` solves after it started
+    (set_coarse_lag), over warm-started drag frames: every frame's energy within
+    1e-5 of the oracle's and the GN count within one; a reset solver repeats
+    the same frames bit for bit (the operator sequence depends on the solve
+    count only)."""
+    runs = []
+    for rep in range(2):
+        s = _solver(ctx, prob)
+        s.set_warm_start(True)
+        s.set_async_refresh(True)
+        s.set_coarse_refresh(refresh)
+        s.set_coarse_lag(lag)
+        s.set_host_checks(4)
+        out = []
+        for fraction in (0.30, 0.33, 0.36, 0.39, 0.42, 0.45):
+            aims, ref = _problem(prob, fraction)
+            x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+            e, done = H(e), int(H(done)[0])
+            if rep == 0:
+                _, e_ref, it_ref = ref.solve(8, stop_rule=True)
+                assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (lag, fraction, e[done], e_ref[-1])
+                assert abs(done - it_ref) <= 1
+            out.append((H(x), e[: done + 1]))
+        runs.append(out)
+        s.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
