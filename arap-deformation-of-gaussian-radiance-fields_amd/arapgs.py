@@ -125,6 +125,7 @@ EXPORTS = {
     "arapgs_solver_set_cg_mode": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_predict": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_coarse_lag": ([P, C.c_int], C.c_int),
+    "arapgs_solver_set_cg_assembled": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_side_stream": ([P, P], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_solver_reset": ([P], C.c_int),
@@ -701,6 +702,12 @@ class Solver:
         h = stream.cuda_stream if stream is not None else 0
         if lib().arapgs_solver_set_side_stream(self.h, P(h) if h else None) != 0:
             raise ArapgsError("set_side_stream failed")
+
+    def set_cg_assembled(self, on=True):
+        """The CG operator applied from per-frame assembled 4x4 pair blocks (plain mode)."""
+        rc = lib().arapgs_solver_set_cg_assembled(self.h, int(bool(on)))
+        if rc != 0:
+            raise ArapgsError(f"set_cg_assembled({on}) failed ({rc})")
 
     def set_coarse_lag(self, solves):
         """A side-stream coarse rebuild is adopted by the solves-th solve after the one that started it."""

@@ -499,6 +499,18 @@ int arapgs_solver_set_cg_block(arapgs_solver* s, int threads) {
     return solver_batch_init(s->one, {&s->w}, err);
 }
 
+int arapgs_solver_set_cg_assembled(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    if (on && s->w.dev.n_pairs == 0) return -2;  // centre mode: not built
+    if ((on != 0) == s->w.cg_asm) return 0;
+    (void)hipSetDevice(s->device);
+    // the batch entries are rebuilt; nothing may still read the old ones
+    if (hipDeviceSynchronize() != hipSuccess) return -1000;
+    s->w.cg_asm = on != 0;
+    std::string err;
+    return solver_batch_init(s->one, {&s->w}, err);
+}
+
 int arapgs_solver_set_cg_graphs(arapgs_solver* s, int on) {
     if (!s || on < 0 || on > 1) return -1;
     s->one.graphs = on != 0;

@@ -58,10 +58,12 @@ __device__ __forceinline__ SolverEntry load_entry(const SolverEntry* __restrict_
     S.caim_ptr = gptr(S.caim_ptr); S.caim_node = gptr(S.caim_node); S.in_ptr = gptr(S.in_ptr);
     S.in_edge = gptr(S.in_edge); S.cm_ptr = gptr(S.cm_ptr); S.cm_con = gptr(S.cm_con); S.cm_sbeg = gptr(S.cm_sbeg);
     S.cm_src = gptr(S.cm_src); S.multi_con = gptr(S.multi_con); S.agg_ptr = gptr(S.agg_ptr);
+    S.pair_ptr = gptr(S.pair_ptr); S.pair_j = gptr(S.pair_j); S.term_ptr = gptr(S.term_ptr); S.terms = gptr(S.terms);
     S.agg_member = gptr(S.agg_member); S.agg_of = gptr(S.agg_of); S.agg_center = gptr(S.agg_center);
     S.g = gptr(S.g); S.aim = gptr(S.aim);
     E.x = gptr(E.x); E.hsol = gptr(E.hsol); E.r = gptr(E.r); E.z = gptr(E.z); E.p = gptr(E.p); E.q = gptr(E.q);
     E.binv = gptr(E.binv); E.rc = gptr(E.rc); E.acinv = gptr(E.acinv); E.gj_piv = gptr(E.gj_piv);
+    E.dmat = gptr(E.dmat); E.pair_m = gptr(E.pair_m);
     E.gj_col = gptr(E.gj_col); E.hwarm = gptr(E.hwarm); E.ac64 = gptr(E.ac64); E.partial = gptr(E.partial);
     E.partial2 = gptr(E.partial2); E.rzp = gptr(E.rzp); E.e_partial = gptr(E.e_partial); E.cand_e = gptr(E.cand_e);
     E.acc_partial = gptr(E.acc_partial); E.energies = gptr(E.energies); E.cg = gptr(E.cg); E.gs = gptr(E.gs);
@@ -439,13 +441,10 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
     const int fi = blockIdx.x * (256 / LANES) + grp;
     double bl[12];
     float C[10];
-    float D9[45];
 #pragma unroll
     for (int a = 0; a < 12; a++) bl[a] = 0.0;
 #pragma unroll
     for (int a = 0; a < 10; a++) C[a] = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 45; a++) D9[a] = 0.0f;
     if (fi < S.nF) {
         const int ni = owner_items(S, fi);
         for (int it = lane; it < ni; it += LANES) {
@@ -481,11 +480,6 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
                     for (int r = 0; r < 6; r++) {
 #pragma unroll
                         for (int a = 0; a < 9; a++) bl[a] -= (double)Ji[r][a] * f[r];
-                        int o = 0;
-#pragma unroll
-                        for (int a = 0; a < 9; a++)
-#pragma unroll
-                            for (int c = a; c < 9; c++) D9[o++] += Ji[r][a] * Ji[r][c];
                     }
                 },
                 [&](float (&a3)[3], float bb, float (&)[3]) {
@@ -513,14 +507,23 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
     float* Dm = sD[grp];
     if (own && lane == 0) {
         for (int e = 0; e < 144; e++) Dm[e] = 0.0f;
+        // the rotation group's 9x9 block J^T J (item 0, the only dense one),
+        // summed over its six rows in order after the walk instead of in
+        // 45 accumulators live across it
+        owner_item<0, false>(
+            S, fi, 0, x, x,
+            [&](float (&Ji)[6][9], float (&)[6]) {
+                for (int a = 0; a < 9; a++)
+                    for (int c = a; c < 9; c++) {
+                        float d = 0.0f;
+#pragma unroll
+                        for (int r = 0; r < 6; r++) d += Ji[r][a] * Ji[r][c];
+                        Dm[a * 12 + c] = d;
+                        Dm[c * 12 + a] = d;
+                    }
+            },
+            [&](float (&)[3], float, float (&)[3]) {});
         int o = 0;
-        for (int a = 0; a < 9; a++)
-            for (int c = a; c < 9; c++) {
-                Dm[a * 12 + c] = D9[o];
-                Dm[c * 12 + a] = D9[o];
-                o++;
-            }
-        o = 0;
         for (int p = 0; p < 4; p++)
             for (int q = p; q < 4; q++) {
                 for (int r = 0; r < 3; r++) {
@@ -545,6 +548,10 @@ __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __rest
         float a[12];
 #pragma unroll
         for (int r = 0; r < 12; r++) a[r] = lane < 12 ? Dm[r * 12 + lane] : 0.0f;
+        if (E.cg_asm && lane < 12) {
+#pragma unroll
+            for (int r = 0; r < 12; r++) E.dmat[144 * (int64_t)fi + r * 12 + lane] = a[r];
+        }
 #pragma unroll
         for (int k = 0; k < 12; k++) {
             const float d = __shfl(a[k], k, LANES);
@@ -787,6 +794,158 @@ __device__ __forceinline__ float cg_alpha(const SolverDev& S, const double* pq_p
     const double pq = block_sum_partials(pq_part, n_pq, shd);
     const double rz = block_sum_partials(rz_cur, S.n_agg, shd);
     return pq != 0.0 ? (float)(rz / pq) : 0.0f;
+}
+
+// ------------------------------------------------------ assembled operator
+// Structured Jacobian row vector of free node `node` in one residual group
+// (rows r = 0..2: a_c at column 3c + r, b at column 9 + r), as u = (a0, a1, a2, b).
+__device__ __forceinline__ void pair_term_u(const SolverDev& S, int4 tm, int node_i, int node_j, float (&ui)[4],
+                                            float (&uj)[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) ui[c] = uj[c] = 0.0f;
+    if (tm.x == 0 || tm.x == 1) {
+        // edge (src, t) with neighbour dst: src gets (sw s d, sw), d = g_dst - g_src; dst gets (0, -sw s)
+        const int src = tm.y, t = tm.z;
+        const int dst = (int)S.nbr[(int64_t)src * S.K + t];
+        const float sc = S.edge_scale[(int64_t)src * S.K + t];
+        const float* gs = S.g + 3 * src;
+        const float* gd = S.g + 3 * dst;
+        const float d[3] = {gd[0] - gs[0], gd[1] - gs[1], gd[2] - gs[2]};
+        float us[4], ud[4];
+#pragma unroll
+        for (int c = 0; c < 3; c++) us[c] = S.sw_reg * sc * d[c];
+        us[3] = S.sw_reg;
+        ud[0] = ud[1] = ud[2] = 0.0f;
+        ud[3] = -S.sw_reg * sc;
+        const bool i_is_src = tm.x == 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            ui[c] = i_is_src ? us[c] : ud[c];
+            uj[c] = i_is_src ? ud[c] : us[c];
+        }
+        return;
+    }
+    // constraint group: every (member, slot) whose source is the node
+    const int cg = tm.y;
+    for (int q = S.con_ptr[cg]; q < S.con_ptr[cg + 1]; q++) {
+        const int m = S.con_mem[q];
+        const float* gm = S.g + 3 * m;
+        for (int sl = 0; sl < S.K; sl++) {
+            const int e = m * S.K + sl;
+            const int nj = (int)S.src_idx[e];
+            if (nj != node_i && nj != node_j) continue;
+            const float wv = S.src_w[e];
+            const float* gj = S.g + 3 * nj;
+            const float dd[3] = {gm[0] - gj[0], gm[1] - gj[1], gm[2] - gj[2]};
+            float* u = nj == node_i ? ui : uj;
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[c] += S.sw_con * wv * dd[c];
+            u[3] += S.sw_con * wv;
+        }
+    }
+}
+
+// M_ij = sum over the pair's terms of u_i u_j^T (row-major 4x4), one thread per
+// free owner (its pairs in order). Once per frame: the node positions change.
+__global__ void pair_coef_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!E.cg_asm || S.n_pairs == 0 || f >= S.nF) return;
+    const int node_i = S.free_nodes[f];
+    for (int p = S.pair_ptr[f]; p < S.pair_ptr[f + 1]; p++) {
+        const int node_j = S.free_nodes[S.pair_j[p]];
+        float m[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) m[c] = 0.0f;
+        for (int t = S.term_ptr[p]; t < S.term_ptr[p + 1]; t++) {
+            float ui[4], uj[4];
+            pair_term_u(S, S.terms[t], node_i, node_j, ui, uj);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) m[4 * a + b] += ui[a] * uj[b];
+        }
+        float4* o = reinterpret_cast<float4*>(E.pair_m + 16 * (int64_t)p);
+#pragma unroll
+        for (int a = 0; a < 4; a++) o[a] = make_float4(m[4 * a], m[4 * a + 1], m[4 * a + 2], m[4 * a + 3]);
+    }
+}
+
+// The CG operator, assembled: s_i = D_i v_i + sum_pairs (M_ij (x) I_3) v_j, with
+// the apply's p, q update and p.q partial. 16 lanes per owner: lane l takes
+// pairs l, l + 16, ... and lane c < 12 row c of D_i v_i.
+template <int BS>
+__device__ __forceinline__ void apply_asm_body(const SolverEntry& E, const float* __restrict__ z, float beta,
+                                               bool first, float* __restrict__ p, float* __restrict__ q,
+                                               double* __restrict__ partial, int ob, double* shd) {
+    const SolverDev& S = E.S;
+    const int grp = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int fi = ob * (BS / LANES) + grp;
+    float acc[12];
+#pragma unroll
+    for (int a = 0; a < 12; a++) acc[a] = 0.0f;
+    float mine = 0.0f;
+    if (fi < S.nF) {
+        for (int pp = S.pair_ptr[fi] + lane; pp < S.pair_ptr[fi + 1]; pp += LANES) {
+            const float4* m4 = reinterpret_cast<const float4*>(E.pair_m + 16 * (int64_t)pp);
+            const float4* v4 = reinterpret_cast<const float4*>(z + 12 * (int64_t)S.pair_j[pp]);
+            const float4 ma = m4[0], mb = m4[1], mc = m4[2], md = m4[3];
+            const float4 va = v4[0], vb = v4[1], vc = v4[2];
+            const float v[12] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w, vc.x, vc.y, vc.z, vc.w};
+            const float M[4][4] = {{ma.x, ma.y, ma.z, ma.w}, {mb.x, mb.y, mb.z, mb.w}, {mc.x, mc.y, mc.z, mc.w},
+                                   {md.x, md.y, md.z, md.w}};
+            // component (c, r): R column c row r at 3c + r, t row r at 9 + r
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    float sacc = 0.0f;
+#pragma unroll
+                    for (int c2 = 0; c2 < 4; c2++) sacc += M[c][c2] * v[(c2 < 3 ? 3 * c2 : 9) + r];
+                    acc[(c < 3 ? 3 * c : 9) + r] += sacc;
+                }
+        }
+        if (lane < 12) {
+            const float4* d4 = reinterpret_cast<const float4*>(E.dmat + 144 * (int64_t)fi + 12 * lane);
+            const float4* v4 = reinterpret_cast<const float4*>(z + 12 * (int64_t)fi);
+#pragma unroll
+            for (int h = 0; h < 3; h++) {
+                const float4 dv = d4[h], vv = v4[h];
+                mine += dv.x * vv.x + dv.y * vv.y + dv.z * vv.z + dv.w * vv.w;
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 12; a++) {
+        const float t = group_reduce16(acc[a]);
+        if (lane == a) mine += t;
+    }
+    double pq = 0.0;
+    if (fi < S.nF && lane < 12) {
+        const int64_t o = 12 * (int64_t)fi + lane;
+        const float pn = first ? z[o] : z[o] + beta * p[o];
+        const float qn = first ? mine : mine + beta * q[o];
+        p[o] = pn;
+        q[o] = qn;
+        pq = (double)pn * qn;
+    }
+    const double t = block_reduce_d(pq, shd);
+    if (threadIdx.x == 0) partial[ob] = t;
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) apply_asm_kernel(const SolverEntry* __restrict__ B, int k, int first,
+                                                       int warm_iter, const float* __restrict__ v_ext) {
+    const SolverEntry E = load_entry(B);
+    if ((int)blockIdx.x >= E.nb_apply || *E.cg_done) return;
+    const SolverDev& S = E.S;
+    const double* rz_cur = E.rzp + (k & 1) * S.n_agg;
+    const double* rz_prev = E.rzp + ((k + 1) & 1) * S.n_agg;
+    const float* z = v_ext ? v_ext : (warm_iter >= 0 ? E.hwarm + (int64_t)warm_iter * E.n : E.z);
+    __shared__ double shd[8];
+    const float beta = cg_beta(S, rz_cur, rz_prev, first != 0, shd);
+    apply_asm_body<BS>(E, z, beta, first != 0, E.p, E.q, E.partial, blockIdx.x, shd);
 }
 
 // CG iteration k (r.z partials of iterations k and k-1 in the two halves of
@@ -1907,6 +2066,7 @@ static SolverEntry entry_of(const SolverWork& W) {
     E.cg_max_iters = W.cg_max_iters;
     E.tol2 = W.cg_tol * W.cg_tol;
     E.x = W.x; E.hsol = W.hsol; E.r = W.r; E.z = W.z; E.p = W.p; E.q = W.q; E.binv = W.binv; E.rc = W.rc;
+    E.dmat = W.dmat; E.pair_m = W.pair_m; E.cg_asm = (W.cg_asm && W.dev.n_pairs > 0) ? 1 : 0;
     E.acinv = W.acinv; E.gj_piv = W.gj_piv; E.gj_col = W.gj_col; E.hwarm = W.hwarm; E.ac64 = W.ac64;
     E.partial = W.partial; E.partial2 = W.partial2; E.rzp = W.rzp; E.e_partial = W.e_partial; E.cand_e = W.cand_e;
     E.acc_partial = W.acc_partial; E.energies = W.energies;
@@ -2132,6 +2292,8 @@ int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
         W->pending = false;
     }
     hipLaunchKernelGGL(init_x_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
+    if (B.w[0]->cg_asm)
+        hipLaunchKernelGGL(pair_coef_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
     if (refresh_coarse && !any_pending) {
         bool all_have = true;
         for (const SolverWork* W : B.w) all_have = all_have && W->have_coarse;
@@ -2172,6 +2334,17 @@ static int launch_apply(const SolverBatch& B, unsigned nz, hipStream_t st, int k
     const SolverEntry* E = B.entries;
     const dim3 grid(B.max_nb_apply, 1, nz);
     const int bs = B.w[0]->cg_block;
+    if (B.w[0]->cg_asm && B.w[0]->dev.n_pairs > 0) {
+#define ARAPGS_APPLY_ASM(BSV) \
+    hipLaunchKernelGGL((apply_asm_kernel<BSV>), grid, dim3(BSV), 0, st, E, k, first, warm_iter, v_ext)
+        if (bs == 64) ARAPGS_APPLY_ASM(64);
+        else if (bs == 128) ARAPGS_APPLY_ASM(128);
+        else if (bs == 256) ARAPGS_APPLY_ASM(256);
+        else ARAPGS_APPLY_ASM(512);
+#undef ARAPGS_APPLY_ASM
+        HIP_RET(hipGetLastError());
+        return 0;
+    }
 #define ARAPGS_APPLY(KTV, BSV) \
     hipLaunchKernelGGL((apply_kernel<KTV, BSV>), grid, dim3(BSV), 0, st, E, k, first, warm_iter, v_ext)
     if (B.kt == 8) {
@@ -2299,6 +2472,7 @@ int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_o
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
                        (const int*)nullptr);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, E);
+    if (W.cg_asm) hipLaunchKernelGGL(pair_coef_kernel, dim3(nblk(W.dev.nF, 256)), dim3(256), 0, st, E);
     int rc = launch_apply(B, 1, st, 0, 1, -1, v);
     if (rc) return rc;
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, q_out, (const float*)W.q, (int64_t)W.n,

@@ -42,6 +42,18 @@ struct SolverDev {
     const int* cm_src;         // source-weight entries (member*K + slot) that reach the free node
     const int* multi_con;      // n_multi groups with more than one member
     int n_multi;
+    // Assembled CG operator (plain constraint mode): J^T J = blockdiag(D) +
+    // sum over free pairs (i, j), i != j, of (M_ij (x) I_3), M_ij a 4x4 matrix
+    // over (R column 0, 1, 2, t). Pair p of owner i: j = pair_j[p], the terms
+    // [term_ptr[p], term_ptr[p+1]) each add u_i u_j^T for one residual group
+    // shared by i and j (int4: type, node, slot, 0; type 0: edge (node, slot)
+    // = (i, t) with nbr = j; type 1: edge (j, t) with nbr = i; type 2:
+    // constraint group `node`). n_pairs = 0 when not built (centre mode).
+    const int* pair_ptr;       // nF+1
+    const int* pair_j;         // n_pairs
+    const int* term_ptr;       // n_pairs+1
+    const int4* terms;
+    int n_pairs;
     const int* agg_ptr;        // n_agg+1 free ranks grouped by aggregate
     const int* agg_member;
     const int* agg_of;         // nF
@@ -88,6 +100,9 @@ struct SolverEntry {
     int cg_max_iters;
     double tol2;
     float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col, *hwarm;
+    float* dmat;       // nF*144: the diagonal blocks D of J^T J (assembled operator)
+    float* pair_m;     // n_pairs*16: M_ij of the assembled operator (per frame)
+    int cg_asm;        // the CG operator is applied assembled
     unsigned long long* ac64;
     double *partial, *partial2, *rzp, *e_partial, *cand_e, *acc_partial;
     double* energies;  // GN_MAX_ITERS + 1: E(x) before and after every Gauss-Newton iteration
@@ -112,6 +127,9 @@ struct SolverWork {
     SolverDev dev;
     int n;  // 12 * nF
     float *x, *hsol, *r, *z, *p, *q, *binv, *rc, *acinv, *gj_piv, *gj_col;
+    float* dmat = nullptr;     // nF*144 diagonal blocks of J^T J (assembled operator)
+    float* pair_m = nullptr;   // n_pairs*16 off-diagonal 4x4 coefficients (per frame)
+    bool cg_asm = false;       // apply the CG operator assembled (needs dev.n_pairs > 0)
     unsigned long long* ac64;  // fixed-point coarse assembly (order independent)
     double *partial, *partial2, *rzp, *e_partial, *cand_e;  // rzp: 2 x n_agg r.z partials
     int e_blocks, n_cand;

@@ -146,6 +146,57 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
         cm_ptr[f + 1] = (int)cm_con.size();
     }
     cm_sbeg.push_back((int)cm_src.size());
+    // Assembled-operator pairs (plain mode only: a multi-member group couples
+    // every pair of its sources). For free owner f: every free j != f sharing a
+    // residual group with f, with that group's terms in a fixed order.
+    std::vector<int> pair_ptr(nF + 1, 0), pair_j, term_ptr{0};
+    std::vector<int4> terms;
+    if (multi_con.empty()) {
+        // constraint groups of each free node (plain mode: one member, K slots)
+        std::vector<std::vector<int>> con_of(nF);
+        for (int c = 0; c < nC; c++) {
+            std::vector<int> seen;
+            for (int q = con_ptr[c]; q < con_ptr[c + 1]; q++)
+                for (int j = 0; j < K; j++) {
+                    const int fj = free_id[(int)d.src_idx[(size_t)con_mem[q] * K + j]];
+                    if (fj >= 0 && std::find(seen.begin(), seen.end(), fj) == seen.end()) seen.push_back(fj);
+                }
+            for (int f : seen) con_of[f].push_back(c);
+        }
+        for (int f = 0; f < nF; f++) {
+            const int i = free_nodes[f];
+            std::vector<std::pair<int, int4>> tl;  // (free j, term)
+            for (int t = 0; t < K; t++) {          // edges (i, t)
+                const int fk = free_id[(int)d.nbr[(size_t)i * K + t]];
+                if (fk >= 0 && fk != f) tl.push_back({fk, make_int4(0, i, t, 0)});
+            }
+            for (int q = in_ptr[f]; q < in_ptr[f + 1]; q++) {  // edges (src, t) into i
+                const int e = in_edge[q], src = e / K, t = e % K;
+                const int fs = free_id[src];
+                if (fs >= 0 && fs != f) tl.push_back({fs, make_int4(1, src, t, 0)});
+            }
+            for (int c : con_of[f]) {              // constraint groups reaching i
+                std::vector<int> js;
+                for (int q = con_ptr[c]; q < con_ptr[c + 1]; q++)
+                    for (int j = 0; j < K; j++) {
+                        const int fj = free_id[(int)d.src_idx[(size_t)con_mem[q] * K + j]];
+                        if (fj >= 0 && fj != f && std::find(js.begin(), js.end(), fj) == js.end()) js.push_back(fj);
+                    }
+                for (int fj : js) tl.push_back({fj, make_int4(2, c, 0, 0)});
+            }
+            std::stable_sort(tl.begin(), tl.end(),
+                             [](const std::pair<int, int4>& a, const std::pair<int, int4>& b) { return a.first < b.first; });
+            for (size_t q = 0; q < tl.size(); q++) {
+                if (q == 0 || tl[q].first != tl[q - 1].first) {
+                    if (q > 0) term_ptr.push_back((int)terms.size());
+                    pair_j.push_back(tl[q].first);
+                }
+                terms.push_back(tl[q].second);
+            }
+            if (!tl.empty()) term_ptr.push_back((int)terms.size());
+            pair_ptr[f + 1] = (int)pair_j.size();
+        }
+    }
     // aggregates: farthest-point seeds over the free nodes' positions, then
     // nearest-seed assignment
     const int n_agg = std::max(1, std::min(d.n_agg > 0 ? d.n_agg : 128, nF));
@@ -213,6 +264,11 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     S.cm_src = upload(W, cm_src, rc);
     S.multi_con = upload(W, multi_con, rc);
     S.n_multi = (int)multi_con.size();
+    S.n_pairs = (int)pair_j.size();
+    S.pair_ptr = upload(W, pair_ptr, rc);
+    S.pair_j = upload(W, pair_j, rc);
+    S.term_ptr = upload(W, term_ptr, rc);
+    S.terms = upload(W, terms, rc);
     S.agg_ptr = upload(W, agg_ptr, rc);
     S.agg_member = upload(W, agg_member, rc);
     S.agg_of = upload(W, agg_of, rc);
@@ -228,6 +284,8 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     W.p = alloc<float>(W, W.n, rc);
     W.q = alloc<float>(W, W.n, rc);
     W.binv = alloc<float>(W, (size_t)144 * nF, rc);
+    W.dmat = alloc<float>(W, (size_t)144 * nF, rc);
+    W.pair_m = alloc<float>(W, (size_t)16 * std::max<size_t>(1, pair_j.size()), rc);
     // rc_k, rc_{k+1} and P^T q (the two-kernel CG form; the three-kernel form uses the first)
     W.rc = alloc<float>(W, 3 * (size_t)S.nc_pad, rc);
     W.acinv = alloc<float>(W, (size_t)S.nc_pad * S.nc_pad, rc);

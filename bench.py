@@ -323,6 +323,7 @@ def apply_config(B, cfg):
     B["solver"].set_cg_graphs(cfg.get("cg_graphs", 0))
     B["solver"].set_cg_mode(cfg.get("cg_mode", 0))
     B["solver"].set_cg_predict(cfg.get("cg_predict", 0))
+    B["solver"].set_cg_assembled(cfg.get("cg_asm", 0))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
@@ -658,6 +659,7 @@ def large_leg(args, tuning):
     solver.set_cg_mode(tuning.get("cg_mode", 0))
     solver.set_cg_predict(tuning.get("cg_predict", 0))
     solver.set_coarse_lag(tuning.get("coarse_lag", 1))
+    solver.set_cg_assembled(tuning.get("cg_asm", 0))
     s0 = workload.clone_state(st)
     # a few sequential frames with the kernels timed by HIP events
     fb = arapgs.frame_buffers(st, K)
@@ -984,6 +986,8 @@ def main():
     ap.add_argument("--side-cus", type=int, default=-1, choices=(-1, 0, 1),
                     help="with --solve-cus: the asynchronous coarse rebuilds on the deform side's CUs (1) or on "
                          "every CU (0) (-1: autotuned, or 0)")
+    ap.add_argument("--cg-asm", type=int, default=-1, choices=(-1, 0, 1),
+                    help="the CG operator applied from assembled pair blocks (-1: autotuned, or 0)")
     ap.add_argument("--samples-late", type=int, default=-1, choices=(-1, 0, 1),
                     help="the samples stream starts after the six-point update (-1: autotuned, or 0)")
     ap.add_argument("--coarse-lag", type=int, default=0,
@@ -1050,6 +1054,8 @@ def main():
         apply_config(B, dict(B["cfg"], side_cus=args.side_cus))
     if args.samples_late >= 0:
         apply_config(B, dict(B["cfg"], samples_late=args.samples_late))
+    if args.cg_asm >= 0:
+        apply_config(B, dict(B["cfg"], cg_asm=args.cg_asm))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

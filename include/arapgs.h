@@ -176,6 +176,17 @@ int arapgs_solver_set_side_stream(arapgs_solver* s, void* stream);
  * Synchronises the device. Solvers of one batch need the same value. */
 int arapgs_solver_set_cg_block(arapgs_solver* s, int threads);
 
+/* on = 1: the CG operator J^T J is applied assembled instead of matrix-free:
+ * per frame, every pair (i, j) of free nodes that share a residual group gets
+ * its 4x4 coefficient matrix M_ij (J^T J = blockdiag(D) + sum M_ij (x) I_3,
+ * the Jacobian blocks being I_3-structured), and the operator kernel reads
+ * M_ij and v_j instead of re-deriving every group's Jacobian from the graph.
+ * Same operator up to rounding (the solve stops at the same tolerance). Plain
+ * constraint mode only (-2 in centre mode, whose multi-member groups couple
+ * every pair of their sources). Synchronises the device. No counterpart in
+ * the reference (its J^T J is assembled for a sparse Cholesky). Default 0. */
+int arapgs_solver_set_cg_assembled(arapgs_solver* s, int on);
+
 /* on = 1: the CG of GN iteration k starts from the step that GN iteration k of
  * the previous solve on this solver found (zero before the first), instead of
  * from zero. Consecutive drag frames solve nearly the same systems, so the CG

@@ -457,3 +457,51 @@ def test_coarse_lag_keeps_oracle_energies(ctx, prob, lag, refresh):
         s.close()
     for a, b in zip(*runs):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_assembled_operator_matches_oracle_and_matrix_free(ctx, prob):
+    """The assembled CG operator (set_cg_assembled: per-frame 4x4 pair blocks
+    plus the diagonal blocks) equals the oracle's dense J^T J v to 1e-5 and the
+    matrix-free walk to 1e-6; centre mode refuses it."""
+    aims, ref = _problem(prob, 0.5)
+    rng = np.random.default_rng(11)
+    x = (ref.identity_x() + 0.05 * rng.normal(size=ref.n)).astype(np.float32)
+    v = rng.normal(size=ref.n).astype(np.float32)
+    s = _solver(ctx, prob)
+    q_mf, b_mf = s.apply(T(prob["nodes"]), T(aims), T(x), T(v))
+    q_mf, b_mf = H(q_mf).copy(), H(b_mf).copy()
+    s.set_cg_assembled(True)
+    q_as, b_as = s.apply(T(prob["nodes"]), T(aims), T(x), T(v))
+    q_as, b_as = H(q_as), H(b_as)
+    J = ref.jacobian(x.astype(np.float64))
+    q_ref = J.T @ (J @ v.astype(np.float64))
+    rel_o = np.linalg.norm(q_as - q_ref) / np.linalg.norm(q_ref)
+    rel_m = np.linalg.norm(q_as - q_mf) / np.linalg.norm(q_mf)
+    print(f"assembled operator: vs oracle {rel_o:.3e}, vs matrix-free {rel_m:.3e}", flush=True)
+    assert rel_o <= 1e-5 and rel_m <= 1e-6
+    assert np.array_equal(b_as, b_mf)
+    s.close()
+    blocks, types = _center_blocks(prob)
+    sc = _solver(ctx, prob, blocks=blocks, types=types, center=True)
+    with pytest.raises(arapgs.ArapgsError):
+        sc.set_cg_assembled(True)
+    sc.close()
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_assembled_operator_solves_match_oracle(ctx, prob, mode):
+    """Warm-started drag frames with the assembled CG operator (CG modes 0 and
+    2): energies within 1e-5 of the oracle's, GN counts within one."""
+    s = _solver(ctx, prob)
+    s.set_cg_assembled(True)
+    s.set_cg_mode(mode)
+    s.set_host_checks(4)
+    s.set_warm_start(True)
+    for fraction in (0.30, 0.35, 0.40, 0.45):
+        aims, ref = _problem(prob, fraction)
+        x_ref, e_ref, it_ref = ref.solve(8, stop_rule=True)
+        x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+        e, done = H(e), int(H(done)[0])
+        assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (fraction, e[: done + 1], e_ref)
+        assert abs(done - it_ref) <= 1, (fraction, done, it_ref)
+    s.close()
