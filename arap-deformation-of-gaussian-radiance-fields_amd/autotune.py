@@ -48,9 +48,10 @@ from __future__ import annotations
 import json
 import os
 
-DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=16, warm=1, cg_block=256, cg_graphs=0,
-                cg_mode=0, cg_predict=0, node_cache=1, blend_split=0, deform_skip=0,
-                solve_cus=0, samples_cus=0, sh_compose=0, tile_cull=0,
+# the defaults are the best configuration measured on the capsules-1M drag
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=3, warm=1, cg_block=256, cg_graphs=0,
+                cg_mode=0, cg_predict=1, node_cache=1, blend_split=0, deform_skip=0,
+                solve_cus=32, samples_cus=0, sh_compose=1, tile_cull=0,
                 coarse_lag=1, side_cus=0, samples_late=0, cg_asm=0)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
                   host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),

@@ -347,10 +347,10 @@ def autotune_leg(B, args):
     from arapgs_amd import autotune
 
     def measure(cfg):
-        # the better of two runs: frame times vary by a few percent run to run
+        # the best of three runs: frame times vary by a few percent run to run
         apply_config(B, cfg)
         n, w = tune_frames(args)
-        return min(timed_pipelined(B, n, w, sample_batch=4)["total_s"] / n for _ in range(2))
+        return min(timed_pipelined(B, n, w, sample_batch=4)["total_s"] / n for _ in range(3))
 
     best, trials = autotune.tune(measure, log=log)
     apply_config(B, best)
@@ -999,6 +999,8 @@ def main():
     ap.add_argument("--adjoint-nodes", type=int, default=1024)
     ap.add_argument("--batch-scenes", type=int, default=32, help="scenes of the uneven-batch leg (0: skip)")
     ap.add_argument("--batch-frames", type=int, default=64, help="timed frames per form of the uneven-batch leg")
+    ap.add_argument("--copy-rate", type=int, default=0, metavar="MIB",
+                    help="only time 20 plain device copies of MIB MiB (read + write GB/s) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 plain device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -1018,6 +1020,24 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
+    if args.copy_rate > 0:
+        # the practical HBM rate of a plain stream (read + write) for comparison with the roofline kernels
+        n = args.copy_rate * (1 << 20) // 4
+        a = torch.rand(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(20):
+            b.copy_(a)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / 20
+        print(json.dumps({"copy_bytes_moved": 8 * n, "copy_ms": round(ms, 4),
+                          "copy_gbs": round(8 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
         return
     B = setup(args)
     apply_config(B, B["cfg"])
