@@ -104,6 +104,8 @@ EXPORTS = {
     "arapgs_blend_points": ([P, P, C.c_int64, P, P, C.c_int, P, C.c_int, P, P], C.c_int),
     "arapgs_six_point_update": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float, P, P, P, P],
                                 C.c_int),
+    "arapgs_six_point_update_from": ([P, C.c_int64, C.c_int, P, C.c_int, P, P, P, P, P, C.c_float, C.c_float,
+                                      P, P, P, P, P, P, P, P], C.c_int),
     "arapgs_node_quats": ([P, P, C.c_int, P], C.c_int),
     "arapgs_sample_sh": ([P, C.c_int64, C.c_int, P, P, P, P, P], C.c_int),
     "arapgs_samples_frames": ([P, C.c_int64, C.c_int, C.c_int, P, P, P, P, P, P, P], C.c_int),
@@ -315,8 +317,16 @@ class Context:
         return out
 
     def six_point_update(self, table, ends_idx, ends_w, ends, scale0, is_static, pos, rot, scale, sh, pad=1e-3,
-                         coeff=2.0):
+                         coeff=2.0, src=None):
+        """src: (pos, rot, scale, sh) to read instead of the written set
+        (arapgs_six_point_update_from)."""
         k = ends_idx.shape[-1]
+        if src is not None:
+            self._check(lib().arapgs_six_point_update_from(
+                self.h, pos.shape[0], k, ptr(table), table.shape[0], ptr(ends_idx), ptr(ends_w), ptr(ends),
+                ptr(scale0), ptr(is_static), pad, coeff, ptr(src[0]), ptr(src[1]), ptr(src[2]), ptr(src[3]),
+                ptr(pos), ptr(rot), ptr(scale), ptr(sh)))
+            return
         self._check(lib().arapgs_six_point_update(self.h, pos.shape[0], k, ptr(table), table.shape[0], ptr(ends_idx),
                                                   ptr(ends_w), ptr(ends), ptr(scale0), ptr(is_static), pad, coeff,
                                                   ptr(pos), ptr(rot), ptr(scale), ptr(sh)))

@@ -13,6 +13,8 @@ its tolerance:
                 or beside it (0)
   cg_asm        the CG operator applied from per-frame assembled 4x4 pair blocks
                 (1) or matrix-free (0)
+  split_render  renders on their own stream from double-buffered attributes, so
+                a frame's six-point update overlaps the previous frame's render (1)
   host_checks   CG iterations between host reads of the convergence flag
   warm          CG warm start from the previous frame's GN steps (1) or from 0
   cg_block      workgroup size of the CG operator kernel: 256, 128, 64 or 512 threads
@@ -32,7 +34,8 @@ its tolerance:
   solve_cus     pipelined frames: the solve on CUs [0, solve_cus) and the deform
                 stream on the others (0: all CUs shared)
   samples_cus   with solve_cus: the samples stream beside the deform (0), on
-                every CU (1) or beside the solve (2)
+                every CU (1), beside the solve (2), or the samples queued on the
+                deform stream itself (3)
   sh_compose    rolling sample chunks: features rotated once by the product of
                 the pending frames' rotations (1) or frame by frame (0)
   tile_cull     renders emit only the tiles that pass the conservative tile test
@@ -52,17 +55,19 @@ import os
 DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=3, warm=1, cg_block=256, cg_graphs=0,
                 cg_mode=0, cg_predict=1, node_cache=1, blend_split=0, deform_skip=0,
                 solve_cus=32, samples_cus=0, sh_compose=1, tile_cull=0,
-                coarse_lag=1, side_cus=0, samples_late=0, cg_asm=0)
+                coarse_lag=1, side_cus=0, samples_late=0, cg_asm=0,
+                split_render=0)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
                   host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),
                   cg_mode=(0, 2), cg_predict=(0, 1), node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096),
                   deform_skip=(0, 16, 32, 64), solve_cus=(0, 32, 48, 64),
-                  samples_cus=(0, 1), sh_compose=(0, 1), tile_cull=(0, 1),
-                  coarse_lag=(1, 2, 3), side_cus=(0, 1), samples_late=(0, 1), cg_asm=(0, 1))
+                  samples_cus=(0, 1, 3), sh_compose=(0, 1), tile_cull=(0, 1),
+                  coarse_lag=(1, 2, 3), side_cus=(0, 1), samples_late=(0, 1), cg_asm=(0, 1),
+                  split_render=(0, 1))
 # host_checks is revisited after cg_graphs: their best chunk can differ
-ORDER = ("warm", "solve_cus", "samples_cus", "side_cus", "samples_late", "cg_mode", "cg_asm", "cg_predict",
-         "aggregates", "refresh", "coarse_lag", "host_checks", "cg_graphs", "host_checks", "cg_block", "tile",
-         "tile_cull", "blend_unroll", "blend_split", "node_cache", "sh_compose", "deform_skip")
+ORDER = ("warm", "split_render", "solve_cus", "samples_cus", "side_cus", "samples_late", "cg_mode", "cg_asm",
+         "cg_predict", "aggregates", "refresh", "coarse_lag", "host_checks", "cg_graphs", "host_checks", "cg_block",
+         "tile", "tile_cull", "blend_unroll", "blend_split", "node_cache", "sh_compose", "deform_skip")
 
 
 def tune(measure, key=None, cache_path=None, candidates=None, order=ORDER, log=None, rounds=1):

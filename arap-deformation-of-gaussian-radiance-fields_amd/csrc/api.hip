@@ -835,6 +835,32 @@ int arapgs_six_point_update(arapgs_ctx* c, int64_t n, int k, const arapgs_node_x
                         "six_point_update");
 }
 
+int arapgs_six_point_update_from(arapgs_ctx* c, int64_t n, int k, const arapgs_node_xf* table, int m,
+                                 const uint32_t* ends_idx, const float* ends_w, float* ends, const float* scale0,
+                                 const uint8_t* is_static, float pad, float coeff, const float* pos_in,
+                                 const float* rot_in, const float* scale_in, const float* sh_in, float* pos,
+                                 float* rot, float* scale, float* sh) {
+    CHECK_CTX(c);
+    if (!table || !ends_idx || !ends_w || !ends || !scale0 || !pos || !rot || !scale || !sh || !pos_in || !rot_in ||
+        !scale_in || !sh_in || n < 0 || k < 1 || k > KNN_MAX_T)
+        return fail(c, -1, "six_point_update_from: bad arguments");
+    if (m < 1) return fail(c, -1, "six_point_update_from: need m >= 1 nodes");
+    if ((uintptr_t)rot % 16 || (uintptr_t)rot_in % 16 || (uintptr_t)sh % 16 || (uintptr_t)sh_in % 16)
+        return fail(c, -1, "six_point_update_from: rot and sh must be 16-byte aligned");
+    // the two sets either coincide or do not overlap
+    auto overlap = [](const float* a, const float* b, int64_t cnt) {
+        return a != b && a < b + cnt && b < a + cnt;
+    };
+    if (overlap(pos_in, pos, 3 * n) || overlap(rot_in, rot, 4 * n) || overlap(scale_in, scale, 3 * n) ||
+        overlap(sh_in, sh, 48 * n))
+        return fail(c, -1, "six_point_update_from: input and output sets overlap");
+    const SixPointIn in{pos_in, rot_in, scale_in, sh_in};
+    return check_launch(c, launch_six_point(n, k, (const NodeXf*)table, m, ends_idx, ends_w, ends, scale0, is_static,
+                                            pad, coeff, pos, rot, scale, sh, c->shrot_dev, c->stream, c->node_cache,
+                                            &in),
+                        "six_point_update_from");
+}
+
 int arapgs_node_quats(arapgs_ctx* c, const arapgs_node_xf* table, int m, float* q) {
     CHECK_CTX(c);
     if (!table || !q || m < 0) return fail(c, -1, "node_quats: bad arguments");

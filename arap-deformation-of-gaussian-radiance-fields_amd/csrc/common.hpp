@@ -46,10 +46,18 @@ int launch_build_node_table(int M, const float* node_pos, const float* x_free, c
 int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const float* w, int k, const NodeXf* table,
                         int M, const uint8_t* skip, float* out, hipStream_t st,
                         bool node_cache = true);
+// The attributes a six-point update starts from when they are not the ones it
+// writes (double-buffered Gaussians: read one set, write the other).
+struct SixPointIn {
+    const float* pos;
+    const float* rot;
+    const float* scale;
+    const float* sh;
+};
 int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
                      float* rot, float* scale, float* sh, const ShRotTables* Td, hipStream_t st,
-                     bool node_cache = true);
+                     bool node_cache = true, const SixPointIn* in = nullptr);
 int launch_node_quats(int M, const NodeXf* table, float* q, hipStream_t st);
 // Node tables and node quaternions of up to SAMPLE_FRAMES_MAX consecutive
 // frames, applied to the samples in order by one launch (K = 8 only; idx and w

@@ -303,9 +303,8 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
                                                         const float* __restrict__ ends_w, float* __restrict__ ends,
                                                         const float* __restrict__ scale0,
                                                         const uint8_t* __restrict__ is_static, float pad, float coeff,
-                                                        float* __restrict__ pos, float* __restrict__ rot,
-                                                        float* __restrict__ scale, float* __restrict__ sh,
-                                                        const ShRotTables* __restrict__ T) {
+                                                        float* pos, float* rot, float* scale, float* sh,
+                                                        const ShRotTables* __restrict__ T, SixPointIn in) {
     __shared__ __align__(16) float s_e[256 * 18];
     __shared__ int s_tag[CACHE ? NODE_CACHE_SLOTS : 1];
     __shared__ float4 s_rows[CACHE ? NODE_CACHE_ROW4 * NODE_CACHE_SLOTS : 1];
@@ -378,7 +377,7 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
     float q[4];
     mat_to_quat(Rf, q);
     quat_normalize(q);
-    const float4 qo = *reinterpret_cast<const float4*>(rot + 4 * g);
+    const float4 qo = *reinterpret_cast<const float4*>(in.rot + 4 * g);
     *reinterpret_cast<float4*>(rot + 4 * g) = make_float4(q[0], q[1], q[2], q[3]);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -399,8 +398,17 @@ __global__ void __launch_bounds__(256) six_point_kernel(int64_t N, const NodeXf*
 #pragma unroll
         for (int j = 0; j < 3; j++) Rsh[i][j] = Rr[i][j];
     }
+    if (!active && g < N && in.pos != pos) {
+        // a static Gaussian keeps its attributes: carried into the written set
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            pos[3 * g + i] = in.pos[3 * g + i];
+            scale[3 * g + i] = in.scale[3 * g + i];
+        }
+        *reinterpret_cast<float4*>(rot + 4 * g) = *reinterpret_cast<const float4*>(in.rot + 4 * g);
+    }
     float shc[48];
-    rows48_load(sh + g0 * 48, n_here, s_e, 256 * 18, shc);
+    rows48_load(in.sh + g0 * 48, n_here, s_e, 256 * 18, shc);
     if (active) sh_rotate_regs(*T, Rsh, shc);
     rows48_store(sh + g0 * 48, n_here, s_e, 256 * 18, shc);
 }
@@ -713,13 +721,15 @@ int launch_blend_points(const float* pts, int64_t P, const uint32_t* idx, const 
 
 int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_t* ends_idx, const float* ends_w,
                      float* ends, const float* scale0, const uint8_t* is_static, float pad, float coeff, float* pos,
-                     float* rot, float* scale, float* sh, const ShRotTables* T, hipStream_t st, bool node_cache) {
+                     float* rot, float* scale, float* sh, const ShRotTables* T, hipStream_t st, bool node_cache,
+                     const SixPointIn* in_p) {
     (void)M;
     if (N == 0) return 0;
     const bool vec = ((uintptr_t)ends_idx % 16 == 0) && ((uintptr_t)ends_w % 16 == 0);
+    const SixPointIn in = in_p ? *in_p : SixPointIn{pos, rot, scale, sh};
     if (node_cache && k == 8 && vec) {
         hipLaunchKernelGGL((six_point_kernel<8, true, true>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table, ends_idx,
-                           ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);
+                           ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T, in);
         HIP_RET(hipGetLastError());
         return 0;
     }
@@ -728,10 +738,10 @@ int launch_six_point(int64_t N, int k, const NodeXf* table, int M, const uint32_
     case KV:                                                                                                           \
         if (vec)                                                                                                       \
             hipLaunchKernelGGL((six_point_kernel<KV, true, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,    \
-                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);         \
+                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T, in);     \
         else                                                                                                           \
             hipLaunchKernelGGL((six_point_kernel<KV, false, false>), dim3(nblk(N, 256)), dim3(256), 0, st, N, table,   \
-                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T);         \
+                               ends_idx, ends_w, ends, scale0, is_static, pad, coeff, pos, rot, scale, sh, T, in);     \
         break;
         ARAPGS_SP_CASE(1) ARAPGS_SP_CASE(2) ARAPGS_SP_CASE(3) ARAPGS_SP_CASE(4) ARAPGS_SP_CASE(5) ARAPGS_SP_CASE(6)
         ARAPGS_SP_CASE(7) ARAPGS_SP_CASE(8) ARAPGS_SP_CASE(9) ARAPGS_SP_CASE(10) ARAPGS_SP_CASE(11) ARAPGS_SP_CASE(12)

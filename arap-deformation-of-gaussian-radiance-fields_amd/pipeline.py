@@ -48,7 +48,7 @@ class DragPipeline:
     def __init__(self, solver, state: dict, k: int, active, delta, gn_iters: int, cameras, images, bg, opacity,
                  stop_rule=True, render_ctx=None, sh_degree=3, tile=16, stream_b=None, split_samples=True,
                  aims_fn=None, sample_batch=1, node_cache=True, stream_c=None, sh_compose=False,
-                 samples_late=False):
+                 samples_late=False, split_render=False, stream_r=None, samples_on_deform=False):
         """The solve side runs on solver.ctx's stream, the deform side on
         stream_b (a new stream when None), e.g. two arapgs.CuStream partitions.
         split_samples: the samples' blend and SH rotation run on a third stream
@@ -65,7 +65,14 @@ class DragPipeline:
         (arapgs_set_sh_compose; features within float rounding of per-frame
         rotations, positions bit-identical). samples_late: the samples stream
         starts a frame's work after that frame's six-point update instead of
-        beside it, so the two HBM-heavy passes do not overlap (same results)."""
+        beside it, so the two HBM-heavy passes do not overlap (same results).
+        split_render: the renders run on a fourth stream (stream_r, or a new
+        one) from double-buffered Gaussian attributes, so frame t+1's six-point
+        update (reading frame t's set, writing the other) overlaps frame t's
+        render; run() leaves the final attributes in `state` (same results).
+        ctx_render is the context that renders (set its render options).
+        samples_on_deform: the samples' work is queued on the deform stream
+        instead of a stream of its own (fewer streams than hardware queues)."""
         import torch
         self.torch = torch
         self.solver = solver
@@ -76,6 +83,19 @@ class DragPipeline:
         self.ctx_b = render_ctx if render_ctx is not None else arapgs.Context(dev.index or 0, stream=self.stream_b)
         if render_ctx is not None:
             self.stream_b = render_ctx.stream
+        self.split_render = split_render
+        if split_render:
+            self.stream_r = stream_r if stream_r is not None else torch.cuda.Stream(device=dev)
+            self.ctx_r = arapgs.Context(dev.index or 0, stream=self.stream_r)
+            self.ctx_r.set_node_cache(node_cache)
+            with torch.cuda.stream(self.stream_b):
+                self.alt = {k: state[k].clone() for k in ("pos", "rot", "scale", "sh")}
+            self.ev_r = [torch.cuda.Event(), torch.cuda.Event()]
+            self.ev_r_recorded = [False, False]
+            self.cur = 0  # the set holding the latest attributes: 0 = state, 1 = alt
+        else:
+            self.stream_r, self.ctx_r = self.stream_b, None
+        self.ctx_render = self.ctx_r if split_render else self.ctx_b
         self.split = split_samples and state.get("samples") is not None and state["samples"].shape[0] > 0
         if sample_batch < 1 or sample_batch > 8:
             raise ValueError("sample_batch must be in 1..8")
@@ -85,6 +105,8 @@ class DragPipeline:
         self.chunks = [(c * S // self.nb, (c + 1) * S // self.nb) for c in range(self.nb)]
         self.applied = [-1] * self.nb  # last frame applied to each chunk
         self.frame = 0                 # frames whose deform side has been queued
+        if samples_on_deform:
+            stream_c = self.stream_b
         self.stream_c = (stream_c if stream_c is not None else torch.cuda.Stream(device=dev)) if self.split else None
         self.ctx_c = arapgs.Context(dev.index or 0, stream=self.stream_c) if self.split else None
         for c in (self.ctx_b, self.ctx_c):
@@ -200,23 +222,45 @@ class DragPipeline:
 
         if not self.samples_late:
             samples()
+        keys = ("pos", "rot", "scale", "sh")
+        if self.split_render:
+            sets = [{k: st[k] for k in keys}, self.alt]
+            src, d = sets[self.cur], 1 - self.cur
+            dst = sets[d]
+        else:
+            dst = {k: st[k] for k in keys}
         with torch.cuda.stream(self.stream_b):
             self.stream_b.wait_event(self.ev_a[slot])
             if not self.split:
                 self._samples(ctx, table)
-            ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"], st.get("gs_static"),
-                                 st["pos"], st["rot"], st["scale"], st["sh"])
+            if self.split_render:
+                # the set written here was last read by the render of two frames ago
+                if self.ev_r_recorded[d]:
+                    self.stream_b.wait_event(self.ev_r[d])
+                ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"],
+                                     st.get("gs_static"), dst["pos"], dst["rot"], dst["scale"], dst["sh"],
+                                     src=(src["pos"], src["rot"], src["scale"], src["sh"]))
+            else:
+                ctx.six_point_update(table, st["ends_idx"], st["ends_w"], st["ends"], st["scale0"],
+                                     st.get("gs_static"), st["pos"], st["rot"], st["scale"], st["sh"])
             self.ev_six[slot].record(self.stream_b)
             self.ev_b[slot].record(self.stream_b)
             self.ev_b_recorded[slot] = True
         if self.samples_late:
             samples()
-        with torch.cuda.stream(self.stream_b):
+        with torch.cuda.stream(self.stream_r):
+            if self.split_render:
+                self.stream_r.wait_event(self.ev_six[slot])
             n_inst = 0
             if render and self.images:
                 img = self.images[0]
-                n_inst = ctx.render_views(st["pos"], st["sh"], self.opac, st["scale"], st["rot"], self.cams,
-                                          img.shape[2], img.shape[1], self.bg, self.images, D=self.D, tile=self.tile)
+                n_inst = self.ctx_render.render_views(dst["pos"], dst["sh"], self.opac, dst["scale"], dst["rot"],
+                                                      self.cams, img.shape[2], img.shape[1], self.bg, self.images,
+                                                      D=self.D, tile=self.tile)
+            if self.split_render:
+                self.ev_r[d].record(self.stream_r)
+                self.ev_r_recorded[d] = True
+                self.cur = d
             return n_inst
 
     # ------------------------------------------------------------ driver
@@ -266,10 +310,19 @@ class DragPipeline:
         if err:
             raise err[0]
         self.torch.cuda.synchronize(self.st["pos"].device)
+        if self.split_render and self.cur == 1:
+            # the final attributes back into `state`
+            for k, v in self.alt.items():
+                self.st[k].copy_(v)
+            self.torch.cuda.synchronize(self.st["pos"].device)
+            self.cur = 0
 
     def close(self):
         if self.ctx_c is not None:
             self.ctx_c.close()
             self.ctx_c = None
+        if self.ctx_r is not None:
+            self.ctx_r.close()
+            self.ctx_r = None
         if self.ctx_b is not None:
             self.ctx_b.close()

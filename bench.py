@@ -188,11 +188,12 @@ def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0, side=None):
         if side is not None:
             streams.append(arapgs.CuStream(0, solve_cus, n_cu - solve_cus))
             side.set_side_stream(streams[-1].torch)
-        return streams, streams[1].torch, stream_c
+        streams.append(arapgs.CuStream(0, solve_cus, n_cu - solve_cus))  # the renders (split_render)
+        return streams, streams[1].torch, stream_c, streams[-1].torch
     if deform_skip > 0:
-        streams = [arapgs.CuStream(0, deform_skip, n_cu - deform_skip) for _ in range(2)]
-        return streams, streams[0].torch, streams[1].torch
-    return [], None, None
+        streams = [arapgs.CuStream(0, deform_skip, n_cu - deform_skip) for _ in range(3)]
+        return streams, streams[0].torch, streams[1].torch, streams[2].torch
+    return [], None, None, None
 
 
 def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
@@ -209,7 +210,7 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     ctx = B["ctx"]
     base_stream = ctx.stream
     side = B["solver"] if B["cfg"].get("side_cus", 0) else None
-    streams, stream_b, stream_c = partition_streams(ctx, cus, B["cfg"].get("deform_skip", 0),
+    streams, stream_b, stream_c, stream_r = partition_streams(ctx, cus, B["cfg"].get("deform_skip", 0),
                                                     B["cfg"].get("samples_cus", 0), side=side)
     try:
         B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
@@ -220,13 +221,15 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
         nb = B["cfg"].get("sample_batch", 1) if sample_batch is None else sample_batch
         pipe = DragPipeline(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"],
                             B["images"], B["bg"], B["opac"], tile=B["cfg"]["tile"],
-                            stream_b=stream_b, stream_c=stream_c, sample_batch=nb,
+                            stream_b=stream_b, stream_c=stream_c, stream_r=stream_r, sample_batch=nb,
+                            split_render=bool(B["cfg"].get("split_render", 0)),
+                            samples_on_deform=B["cfg"].get("samples_cus", 0) == 3,
                             node_cache=bool(B["cfg"].get("node_cache", 1)),
                             sh_compose=bool(B["cfg"].get("sh_compose", 0)),
                             samples_late=bool(B["cfg"].get("samples_late", 0)))
-        pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
-        pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
-        pipe.ctx_b.set_tile_cull(B["cfg"].get("tile_cull", 0))
+        pipe.ctx_render.set_blend_unroll(B["cfg"]["blend_unroll"])
+        pipe.ctx_render.set_blend_split(B["cfg"].get("blend_split", 0))
+        pipe.ctx_render.set_tile_cull(B["cfg"].get("tile_cull", 0))
         pipe.run(warmup)
         torch.cuda.synchronize()
         stamps = []
@@ -433,11 +436,13 @@ def record_frames(B, steps, warmup):
     pipe = Recording(B["solver"], B["state"], B["K"], B["active"], B["delta"], B["iters"], B["cams"], B["images"],
                      B["bg"], B["opac"], tile=B["cfg"]["tile"], sample_batch=B["cfg"].get("sample_batch", 1),
                      node_cache=bool(B["cfg"].get("node_cache", 1)), sh_compose=bool(B["cfg"].get("sh_compose", 0)),
-                            samples_late=bool(B["cfg"].get("samples_late", 0)))
+                     split_render=bool(B["cfg"].get("split_render", 0)),
+                     samples_on_deform=B["cfg"].get("samples_cus", 0) == 3,
+                     samples_late=bool(B["cfg"].get("samples_late", 0)))
     pipe.rec = []
-    pipe.ctx_b.set_blend_unroll(B["cfg"]["blend_unroll"])
-    pipe.ctx_b.set_blend_split(B["cfg"].get("blend_split", 0))
-    pipe.ctx_b.set_tile_cull(B["cfg"].get("tile_cull", 0))
+    pipe.ctx_render.set_blend_unroll(B["cfg"]["blend_unroll"])
+    pipe.ctx_render.set_blend_split(B["cfg"].get("blend_split", 0))
+    pipe.ctx_render.set_tile_cull(B["cfg"].get("tile_cull", 0))
     pipe.run(warmup + steps)
     torch.cuda.synchronize()
     same = all(torch.equal(final_timed[k], B["state"][k]) for k in final_timed)
@@ -688,17 +693,20 @@ def large_leg(args, tuning):
     torch.cuda.synchronize()
     # the drag workload's tuned schedule (identical results)
     base_stream = ctx.stream
-    masked, stream_b, stream_c = partition_streams(ctx, tuning.get("solve_cus", 0), tuning.get("deform_skip", 0),
+    masked, stream_b, stream_c, stream_r = partition_streams(ctx, tuning.get("solve_cus", 0),
+                                                             tuning.get("deform_skip", 0),
                                                    tuning.get("samples_cus", 0),
                                                    side=solver if tuning.get("side_cus", 0) else None)
     pipe = DragPipeline(solver, st, K, active, delta, sc.gn_iters, W["cams"], images, W["bg"], W["opac"],
                         aims_fn=aims_fn, sample_batch=tuning.get("sample_batch", 1),
                         node_cache=bool(tuning.get("node_cache", 1)), stream_b=stream_b, stream_c=stream_c,
+                        stream_r=stream_r, split_render=bool(tuning.get("split_render", 0)),
+                        samples_on_deform=tuning.get("samples_cus", 0) == 3,
                         sh_compose=bool(tuning.get("sh_compose", 0)),
                         samples_late=bool(tuning.get("samples_late", 0)))
-    pipe.ctx_b.set_blend_unroll(tuning.get("blend_unroll", 4))
-    pipe.ctx_b.set_blend_split(tuning.get("blend_split", 0))
-    pipe.ctx_b.set_tile_cull(tuning.get("tile_cull", 0))
+    pipe.ctx_render.set_blend_unroll(tuning.get("blend_unroll", 4))
+    pipe.ctx_render.set_blend_split(tuning.get("blend_split", 0))
+    pipe.ctx_render.set_tile_cull(tuning.get("tile_cull", 0))
     pipe.run(args.warmup)
     torch.cuda.synchronize()
     stamps = []
@@ -973,9 +981,9 @@ def main():
     ap.add_argument("--solve-cus", type=int, default=-1,
                     help="pipelined form: the solve on CUs [0, N), the rest for the deform and samples (-1: autotuned, "
                          "or 0 = all CUs shared)")
-    ap.add_argument("--samples-cus", type=int, default=-1, choices=(-1, 0, 1, 2),
-                    help="with --solve-cus: the samples stream beside the deform (0), on every CU (1) or beside the "
-                         "solve (2) (-1: autotuned, or 0)")
+    ap.add_argument("--samples-cus", type=int, default=-1, choices=(-1, 0, 1, 2, 3),
+                    help="with --solve-cus: the samples stream beside the deform (0), on every CU (1), beside the "
+                         "solve (2), or the samples queued on the deform stream itself (3) (-1: autotuned, or 0)")
     ap.add_argument("--sh-compose", type=int, default=-1, choices=(-1, 0, 1),
                     help="rolling sample chunks: one SH rotation by the product of the pending frames' rotations "
                          "(-1: autotuned, or 0)")
@@ -986,6 +994,8 @@ def main():
     ap.add_argument("--side-cus", type=int, default=-1, choices=(-1, 0, 1),
                     help="with --solve-cus: the asynchronous coarse rebuilds on the deform side's CUs (1) or on "
                          "every CU (0) (-1: autotuned, or 0)")
+    ap.add_argument("--split-render", type=int, default=-1, choices=(-1, 0, 1),
+                    help="renders on their own stream from double-buffered attributes (-1: autotuned, or 0)")
     ap.add_argument("--cg-asm", type=int, default=-1, choices=(-1, 0, 1),
                     help="the CG operator applied from assembled pair blocks (-1: autotuned, or 0)")
     ap.add_argument("--samples-late", type=int, default=-1, choices=(-1, 0, 1),
@@ -1076,6 +1086,8 @@ def main():
         apply_config(B, dict(B["cfg"], samples_late=args.samples_late))
     if args.cg_asm >= 0:
         apply_config(B, dict(B["cfg"], cg_asm=args.cg_asm))
+    if args.split_render >= 0:
+        apply_config(B, dict(B["cfg"], split_render=args.split_render))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

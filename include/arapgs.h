@@ -302,6 +302,18 @@ int arapgs_six_point_update(arapgs_ctx* ctx, int64_t n, int k, const arapgs_node
                             const uint8_t* is_static, float axis_padding, float end_coeff, float* pos, float* rot,
                             float* scale, float* sh);
 
+/* The same update reading the attributes from one set (pos_in, rot_in,
+ * scale_in, sh_in) and writing them to another (pos, rot, scale, sh; static
+ * Gaussians' attributes are copied over), so that a render may still read the
+ * previous frame's set meanwhile (double-buffered Gaussians); ends are blended
+ * in place. Results bit-identical to the in-place update. No counterpart in the
+ * reference (its update is in place). */
+int arapgs_six_point_update_from(arapgs_ctx* ctx, int64_t n, int k, const arapgs_node_xf* table, int m,
+                                 const uint32_t* ends_idx, const float* ends_w, float* ends, const float* scale0,
+                                 const uint8_t* is_static, float axis_padding, float end_coeff, const float* pos_in,
+                                 const float* rot_in, const float* scale_in, const float* sh_in, float* pos,
+                                 float* rot, float* scale, float* sh);
+
 /* FastUpdateSamplesSH (GaussianView.cpp:3169-3186): node quaternions
  * q f32[M*4] (w,x,y,z) from the table's rotations (orthogonal polar factor). */
 int arapgs_node_quats(arapgs_ctx* ctx, const arapgs_node_xf* table, int m, float* q);

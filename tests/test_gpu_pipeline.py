@@ -17,11 +17,11 @@ pytestmark = pytest.mark.gpu
 FRAMES = 5
 
 
-@pytest.mark.parametrize("sample_batch,masked,compose,late", [(1, False, False, False), (3, False, False, False),
-                                                              (8, False, False, False), (3, True, False, False),
-                                                              (4, False, True, False), (1, False, False, True),
-                                                              (4, False, False, True)])
-def test_pipelined_frames_bit_identical(ctx, sample_batch, masked, compose, late):
+@pytest.mark.parametrize("sample_batch,masked,compose,late,split", [
+    (1, False, False, False, False), (3, False, False, False, False), (8, False, False, False, False),
+    (3, True, False, False, False), (4, False, True, False, False), (1, False, False, True, False),
+    (4, False, False, True, False), (1, False, False, False, True), (4, True, False, False, True)])
+def test_pipelined_frames_bit_identical(ctx, sample_batch, masked, compose, late, split):
     """Pipelined frames (and the rolling sample chunks of sample_batch > 1) equal
     sequential frames bit for bit; the drag is run as two run() calls (3 + 2
     frames), so the table slots and chunk records carry over between runs.
@@ -30,7 +30,9 @@ def test_pipelined_frames_bit_identical(ctx, sample_batch, masked, compose, late
     compose: the chunks' pending frames rotate the features once by the product
     of their rotations; everything but the features stays bit-identical and
     the features agree within 2e-6 of their scale. late: the samples stream
-    starts each frame after the six-point update (bit-identical)."""
+    starts each frame after the six-point update (bit-identical). split: the
+    renders run on their own stream from double-buffered attributes (the
+    5-frame drag in runs of 3 and 2 ends on each of the two sets once)."""
     import torch
     sc = scenes.capsule_scene(7, n=40000, n_nodes=128, W=160, H=128, n_views=2)
     sc.grid = 32
@@ -64,10 +66,12 @@ def test_pipelined_frames_bit_identical(ctx, sample_batch, masked, compose, late
     masks = []
     if masked:
         n_cu = torch.cuda.get_device_properties(0).multi_processor_count
-        masks = [arapgs.CuStream(0, 32, n_cu - 32), arapgs.CuStream(0, cus=list(range(32, n_cu)))]
+        masks = [arapgs.CuStream(0, 32, n_cu - 32), arapgs.CuStream(0, cus=list(range(32, n_cu))),
+                 arapgs.CuStream(0, 32, n_cu - 32)]
     pipe = DragPipeline(solver2, st2, K, W["active"], delta, iters, W["cams"], imgs_pipe, W["bg"], W["opac"],
                         sample_batch=sample_batch, stream_b=masks[0].torch if masks else None,
-                        stream_c=masks[1].torch if masks else None, sh_compose=compose, samples_late=late)
+                        stream_c=masks[1].torch if masks else None, sh_compose=compose, samples_late=late,
+                        split_render=split, stream_r=masks[2].torch if (masks and split) else None)
     pipe.run(3)
     pipe.run(FRAMES - 3)
     torch.cuda.synchronize()
