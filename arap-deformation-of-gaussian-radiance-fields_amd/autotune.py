@@ -54,9 +54,9 @@ import os
 # the defaults are the best configuration measured on the capsules-1M drag
 DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=3, warm=1, cg_block=256, cg_graphs=0,
                 cg_mode=0, cg_predict=1, node_cache=1, blend_split=0, deform_skip=0,
-                solve_cus=32, samples_cus=0, sh_compose=1, tile_cull=0,
-                coarse_lag=1, side_cus=0, samples_late=0, cg_asm=0,
-                split_render=0)
+                solve_cus=32, samples_cus=3, sh_compose=1, tile_cull=0,
+                coarse_lag=1, side_cus=1, samples_late=1, cg_asm=0,
+                split_render=1)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
                   host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),
                   cg_mode=(0, 2), cg_predict=(0, 1), node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096),

@@ -429,6 +429,17 @@ int arapgs_solver_set_host_checks(arapgs_solver* s, int every) {
     return 0;
 }
 
+int arapgs_solver_set_cg_tol(arapgs_solver* s, float tol) {
+    if (!s || !(tol > 0.0f) || !(tol < 1.0f)) return -1;
+    if ((double)tol == s->w.cg_tol) return 0;
+    (void)hipSetDevice(s->device);
+    // the batch entries carry the tolerance; nothing may still read the old ones
+    if (hipDeviceSynchronize() != hipSuccess) return -1000;
+    s->w.cg_tol = tol;
+    std::string err;
+    return solver_batch_init(s->one, {&s->w}, err);
+}
+
 int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode) {
     if (!s || mode < 0 || mode > 2) return -1;
     s->cg_mode = mode;

@@ -327,6 +327,7 @@ def apply_config(B, cfg):
     B["solver"].set_cg_mode(cfg.get("cg_mode", 0))
     B["solver"].set_cg_predict(cfg.get("cg_predict", 0))
     B["solver"].set_cg_assembled(cfg.get("cg_asm", 0))
+    B["solver"].set_cg_tol(cfg.get("cg_tol", 1e-4))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
@@ -994,6 +995,8 @@ def main():
     ap.add_argument("--side-cus", type=int, default=-1, choices=(-1, 0, 1),
                     help="with --solve-cus: the asynchronous coarse rebuilds on the deform side's CUs (1) or on "
                          "every CU (0) (-1: autotuned, or 0)")
+    ap.add_argument("--cg-tol", type=float, default=0.0,
+                    help="CG relative residual tolerance of the fast and pipelined forms (0: 1e-4)")
     ap.add_argument("--split-render", type=int, default=-1, choices=(-1, 0, 1),
                     help="renders on their own stream from double-buffered attributes (-1: autotuned, or 0)")
     ap.add_argument("--cg-asm", type=int, default=-1, choices=(-1, 0, 1),
@@ -1088,6 +1091,8 @@ def main():
         apply_config(B, dict(B["cfg"], cg_asm=args.cg_asm))
     if args.split_render >= 0:
         apply_config(B, dict(B["cfg"], split_render=args.split_render))
+    if args.cg_tol > 0:
+        apply_config(B, dict(B["cfg"], cg_tol=args.cg_tol))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

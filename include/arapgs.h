@@ -237,6 +237,12 @@ int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h)
  * iterates do not depend on the host-check interval or on CG graphs. */
 int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode);
 
+/* The CG's relative residual tolerance after creation (desc.cg_tol; 0 < tol
+ * < 1). The reference solves each Gauss-Newton step exactly (sparse
+ * Cholesky); a tolerance is an inexact-Newton choice whose accuracy is judged
+ * by the final energies against an exact FP64 solve. Synchronises the device. */
+int arapgs_solver_set_cg_tol(arapgs_solver* s, float tol);
+
 /* on = 1: with host checks on (set_host_checks > 0), the first host check of
  * GN iteration k's CG comes after as many CG iterations as GN iteration k of
  * the previous solve needed, then every set_host_checks iterations (no

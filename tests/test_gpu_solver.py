@@ -505,3 +505,31 @@ def test_assembled_operator_solves_match_oracle(ctx, prob, mode):
         assert abs(e[done] - e_ref[-1]) <= 1e-5 * e_ref[-1], (fraction, e[: done + 1], e_ref)
         assert abs(done - it_ref) <= 1, (fraction, done, it_ref)
     s.close()
+
+
+def test_cg_tol_setter_matches_creation_tolerance(ctx, prob):
+    """set_cg_tol after creation solves exactly like a solver created with that
+    tolerance (same bits), the drag energies stay within 1e-5 of the oracle's,
+    and tolerances outside (0, 1) are refused."""
+    def make(tol):
+        return arapgs.Solver(ctx, prob["nbr"], prob["blocks"], prob["types"], prob["src_idx"],
+                             prob["src_w"].astype(np.float32), prob["nodes"], cg_max_iters=200, cg_tol=tol)
+
+    a, b = make(1e-4), make(1e-5)
+    b.set_cg_tol(1e-4)
+    for s in (a, b):
+        s.set_warm_start(True)
+        s.set_host_checks(4)
+    for fraction in (0.30, 0.35, 0.40):
+        aims, ref = _problem(prob, fraction)
+        xa, ea, da = a.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+        xb, eb, db = b.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+        da, db = int(H(da)[0]), int(H(db)[0])
+        assert da == db and np.array_equal(H(xa), H(xb)) and np.array_equal(H(ea), H(eb))
+        _, e_ref, _ = ref.solve(8, stop_rule=True)
+        assert abs(H(ea)[da] - e_ref[-1]) <= 1e-5 * e_ref[-1], (fraction, H(ea)[: da + 1], e_ref)
+    for bad in (0.0, -1e-3, 1.0, float("nan")):
+        with pytest.raises(arapgs.ArapgsError):
+            b.set_cg_tol(bad)
+    a.close()
+    b.close()
