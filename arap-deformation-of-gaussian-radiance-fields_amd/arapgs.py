@@ -129,6 +129,7 @@ EXPORTS = {
     "arapgs_solver_set_coarse_lag": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_assembled": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_tol": ([P, C.c_float], C.c_int),
+    "arapgs_solver_set_host_spin": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_side_stream": ([P, P], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_solver_reset": ([P], C.c_int),
@@ -713,6 +714,11 @@ class Solver:
         h = stream.cuda_stream if stream is not None else 0
         if lib().arapgs_solver_set_side_stream(self.h, P(h) if h else None) != 0:
             raise ArapgsError("set_side_stream failed")
+
+    def set_host_spin(self, on=True):
+        """Host checks poll the pinned flag instead of waiting for the stream."""
+        if lib().arapgs_solver_set_host_spin(self.h, int(bool(on))) != 0:
+            raise ArapgsError("set_host_spin failed")
 
     def set_cg_tol(self, tol):
         """The CG's relative residual tolerance (inexact Newton; default 1e-4)."""

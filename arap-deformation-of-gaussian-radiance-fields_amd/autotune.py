@@ -23,8 +23,11 @@ its tolerance:
   cg_mode       CG iterations as three kernels (0) or two, with the coarse
                 restriction carried by a recurrence (2)
   cg_predict    first CG host check of each GN iteration after as many CG
-                iterations as the previous frame needed there (1) or after
-                host_checks (0)
+                iterations as the previous frame needed there, and the GN stop
+                flag read only from the previous frame's GN count on (1), or
+                after host_checks and after every GN iteration (0)
+  host_spin     host checks poll the flag's pinned slot (1) or wait for the
+                solve stream (0)
   node_cache    K = 8 blends through the per-workgroup LDS node-row cache (1) or
                 with every node row gathered from the global table (0)
   blend_split   16x16 tiles with more instances than this blended in segments by
@@ -56,17 +59,17 @@ DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=
                 cg_mode=0, cg_predict=1, node_cache=1, blend_split=0, deform_skip=0,
                 solve_cus=32, samples_cus=3, sh_compose=1, tile_cull=0,
                 coarse_lag=1, side_cus=1, samples_late=1, cg_asm=0,
-                split_render=1)
+                split_render=1, host_spin=1)
 CANDIDATES = dict(tile=(16, 8), blend_unroll=(2, 4, 8), aggregates=(64, 128, 192, 256), refresh=(2, 4, 8, 16, 32),
                   host_checks=(2, 3, 4, 6, 8, 16, 32), warm=(1, 0), cg_block=(256, 128, 64, 512), cg_graphs=(0, 1),
                   cg_mode=(0, 2), cg_predict=(0, 1), node_cache=(1, 0), blend_split=(0, 1024, 2048, 4096),
                   deform_skip=(0, 16, 32, 64), solve_cus=(0, 32, 48, 64),
                   samples_cus=(0, 1, 3), sh_compose=(0, 1), tile_cull=(0, 1),
                   coarse_lag=(1, 2, 3), side_cus=(0, 1), samples_late=(0, 1), cg_asm=(0, 1),
-                  split_render=(0, 1))
+                  split_render=(0, 1), host_spin=(0, 1))
 # host_checks is revisited after cg_graphs: their best chunk can differ
 ORDER = ("warm", "split_render", "solve_cus", "samples_cus", "side_cus", "samples_late", "cg_mode", "cg_asm",
-         "cg_predict", "aggregates", "refresh", "coarse_lag", "host_checks", "cg_graphs", "host_checks", "cg_block",
+         "cg_predict", "host_spin", "aggregates", "refresh", "coarse_lag", "host_checks", "cg_graphs", "host_checks", "cg_block",
          "tile", "tile_cull", "blend_unroll", "blend_split", "node_cache", "sh_compose", "deform_skip")
 
 

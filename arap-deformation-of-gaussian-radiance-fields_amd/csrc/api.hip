@@ -1,6 +1,7 @@
 // C-ABI of the arapgs library (include/arapgs.h): context, workspaces, argument
 // checks and dispatch to the gfx950 kernels.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -66,10 +67,11 @@ struct arapgs_ctx {
     RasterView* r_views_h = nullptr;  // pinned host staging for it
     int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
     int* r_seg_h = nullptr;           // pinned host staging for it
-    int* flag_h = nullptr;            // pinned: convergence flags and render counts written by kernels (8 ints;
-                                      // [3]: the CG iteration count at a host check; [4], [5]: the flags
-                                      // of consecutive CG chunk graphs; [6]: the unread flag of the
-                                      // two-kernel CG's final test; [7]: gn_done after a GN-end graph)
+    int* flag_h = nullptr;            // pinned, coherent: convergence flags and render counts written by kernels
+                                      // (16 ints; [3]: the CG iteration count at a host check; [4], [5]: the
+                                      // flags of consecutive CG chunk graphs; [6]: the unread flag of the
+                                      // two-kernel CG's final test; [7]: gn_done after a GN-end graph; [8],
+                                      // [9]: every GN loop stopped and the GN count, at a host check)
     hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
@@ -92,6 +94,8 @@ struct arapgs_solver {
     int cg_mode = 0;            // 0: one launch per CG phase, 1: persistent cooperative PCG, 2: two launches per iteration
     bool cg_predict = false;    // first host check of each GN iteration's CG at the previous solve's count
     int cg_pred[GN_MAX_ITERS] = {};  // those counts (0: none yet)
+    int gn_pred = 0;            // the previous solve's GN count (0: none yet): the stop flag is read only after it
+    bool host_spin = false;     // host checks poll the pinned flag instead of waiting for the stream
     int coarse_refresh = 8;     // frames between coarse-operator rebuilds
     int64_t solves = 0;
     SolverBatch one;            // this solver as a launch group of one
@@ -172,7 +176,7 @@ int arapgs_create(int device, arapgs_ctx** out) {
         (e = hipHostMalloc(&c->r_views_h, sizeof(RasterView) * RASTER_MAX_VIEWS, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMalloc(&c->r_seg, sizeof(int) * (RASTER_MAX_VIEWS + 1))) != hipSuccess ||
         (e = hipHostMalloc(&c->r_seg_h, sizeof(int) * (RASTER_MAX_VIEWS + 1), hipHostMallocDefault)) != hipSuccess ||
-        (e = hipHostMalloc(&c->flag_h, sizeof(int) * 8, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->flag_h, sizeof(int) * 16, hipHostMallocCoherent)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->chk_ev[0], hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->chk_ev[1], hipEventDisableTiming)) != hipSuccess ||
         (e = hipMemcpy(c->shrot_dev, &c->shrot, sizeof(ShRotTables), hipMemcpyHostToDevice)) != hipSuccess) {
@@ -489,6 +493,7 @@ int arapgs_solver_reset(arapgs_solver* s) {
     s->w.have_coarse = false;
     s->solves = 0;
     for (int& v : s->cg_pred) v = 0;
+    s->gn_pred = 0;
     return 0;
 }
 
@@ -496,6 +501,13 @@ int arapgs_solver_set_cg_predict(arapgs_solver* s, int on) {
     if (!s || on < 0 || on > 1) return -1;
     s->cg_predict = on != 0;
     for (int& v : s->cg_pred) v = 0;
+    s->gn_pred = 0;
+    return 0;
+}
+
+int arapgs_solver_set_host_spin(arapgs_solver* s, int on) {
+    if (!s || on < 0 || on > 1) return -1;
+    s->host_spin = on != 0;
     return 0;
 }
 
@@ -573,20 +585,46 @@ namespace {
 // pred (when not null): per GN iteration, the CG iteration count of the
 // previous solve; the first host check of that GN iteration's CG comes after
 // that many iterations (then every `every`), and the count found is stored back.
+// gn_pred (when not null, with host checks and the three-kernel CG): the GN
+// count of the previous solve; the stop flag of GN iterations before that
+// count is not waited for: the next iteration is queued at once (its kernels
+// do nothing once the loop has stopped) and the flag is read with that
+// iteration's first CG check. The count found is stored back. The iterates do
+// not depend on it.
+// spin: host checks poll the flag's pinned slot (published with a
+// system-scope release by the flag kernel) instead of waiting for the stream;
+// a slot still unset after HOST_SPIN_US falls back to the stream wait, which
+// also reports a failed launch.
+constexpr int HOST_SPIN_US = 50000;
+
 int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, int iters, int stop_rule,
-           int* pred = nullptr) {
+           int* pred = nullptr, bool spin = false, int* gn_pred = nullptr) {
     int rc = check_launch(c, solver_frame_setup(B, refresh, c->stream), "gn_solve setup");
     if (rc) return rc;
     // the flag kernel writes into pinned host memory: no device-to-host copy
     B.fused = cg_mode == 2;
-    auto host_flag = [&](int which, int* out, int k_done) -> int {
-        int r = check_launch(c, solver_batch_flag(B, which, c->stream, c->flag_h, k_done, c->flag_h + 3), "gn_solve");
+    int* const gn_slot = c->flag_h + 8;
+    auto host_flag = [&](int which, int* out, int k_done, int* gn = nullptr) -> int {
+        __atomic_store_n(c->flag_h, -1, __ATOMIC_SEQ_CST);
+        int r = check_launch(c, solver_batch_flag(B, which, c->stream, c->flag_h, k_done, c->flag_h + 3, gn),
+                             "gn_solve");
         if (r) return r;
-        hipError_t e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
-        *out = *(volatile int*)c->flag_h;
+        bool seen = false;
+        if (spin) {
+            const auto t0 = std::chrono::steady_clock::now();
+            while (!(seen = __atomic_load_n(c->flag_h, __ATOMIC_ACQUIRE) != -1) &&
+                   std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(HOST_SPIN_US)) {
+            }
+        }
+        if (!seen) {
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "gn_solve");
+        }
+        *out = __atomic_load_n(c->flag_h, __ATOMIC_ACQUIRE);
         return 0;
     };
+    const bool gn_spec = gn_pred && every > 0 && cg_mode == 0 && *gn_pred > 0;
+    bool gn_unread = false;  // an earlier GN iteration's stop flag has not been read
     // graphs: the GN phases and the CG chunks replay as captured graphs (a
     // context without its own stream cannot capture)
     const bool graphs = B.graphs && every > 0 && c->stream && cg_mode != 1;
@@ -649,7 +687,15 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
                     if (rc) return rc;
                     if (every > 0 && k + 1 == next_check) {
                         int done = 0;
-                        if ((rc = host_flag(0, &done, k + 1))) return rc;
+                        if ((rc = host_flag(0, &done, k + 1, gn_unread ? gn_slot : nullptr))) return rc;
+                        if (gn_unread) {
+                            gn_unread = false;
+                            if (gn_slot[0]) {
+                                // the loop stopped before this iteration: its kernels did nothing
+                                *gn_pred = std::max(1, gn_slot[1]);
+                                return 0;
+                            }
+                        }
                         if (done) {
                             seen_done = true;
                             break;
@@ -678,8 +724,14 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
         rc = check_launch(c, solver_gn_end(B, it, stop_rule, c->stream), "gn_solve line search");
         if (rc) return rc;
         if (every > 0) {
+            if (gn_spec && it + 1 < *gn_pred && it + 1 < iters) {
+                gn_unread = true;
+                continue;
+            }
             int done = 0;
-            if ((rc = host_flag(1, &done, 0))) return rc;
+            if ((rc = host_flag(1, &done, 0, gn_pred ? gn_slot : nullptr))) return rc;
+            gn_unread = false;
+            if (gn_pred) *gn_pred = std::max(1, gn_slot[1]);
             if (done) break;
         }
     }
@@ -707,7 +759,7 @@ int arapgs_gn_solve(arapgs_ctx* c, arapgs_solver* s, const float* node_pos, cons
     const bool refresh = (s->solves % std::max(1, s->coarse_refresh)) == 0;
     s->solves++;
     if ((rc = run_gn(c, s->one, s->host_check_every, s->cg_mode, refresh, iters, stop_rule,
-                     s->cg_predict ? s->cg_pred : nullptr)))
+                     s->cg_predict ? s->cg_pred : nullptr, s->host_spin, s->cg_predict ? &s->gn_pred : nullptr)))
         return rc;
     SolverOutPtrs out = {};
     out.x[0] = x_out;

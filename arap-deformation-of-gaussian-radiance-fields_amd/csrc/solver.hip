@@ -1290,8 +1290,8 @@ __global__ void fused_flag_kernel(const SolverEntry* __restrict__ B, int n, int 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        *flag = s_all;
         if (iters) *iters = s_iters;
+        __hip_atomic_store(flag, s_all, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1997,16 +1997,26 @@ __global__ void set_frame_kernel(SolverEntry* __restrict__ B, int n, FramePtrs f
 // flag is pinned host memory: the host reads it once the stream reaches this
 // point, without a device-to-host copy.
 // iters (when not null, which 0): the largest CG iteration count over the scenes.
+// gn (when not null): gn[0] = 1 when every scene's gn_done is set, gn[1] =
+// the largest Gauss-Newton iteration count so far. The flag itself is stored
+// last, with release semantics at system scope, so a host that sees it (in
+// pinned memory, without waiting for the stream) also sees iters and gn.
 __global__ void batch_flag_kernel(const SolverEntry* __restrict__ B, int n, int which, int* __restrict__ flag,
-                                  int* __restrict__ iters) {
+                                  int* __restrict__ iters, int* __restrict__ gn) {
     if (threadIdx.x != 0) return;
-    int all = 1, most = 0;
+    int all = 1, most = 0, gn_all = 1, gn_most = 0;
     for (int s = 0; s < n; s++) {
         all &= (which ? *B[s].gn_done : *B[s].cg_done) != 0 ? 1 : 0;
         if (iters) most = max(most, B[s].cg->iters);
+        gn_all &= *B[s].gn_done != 0 ? 1 : 0;
+        gn_most = max(gn_most, B[s].gs->iters_done);
     }
-    *flag = all;
     if (iters) *iters = most;
+    if (gn) {
+        gn[0] = gn_all;
+        gn[1] = gn_most;
+    }
+    __hip_atomic_store(flag, all, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Per-scene outputs: x, the energies of the iterations run, the iteration count
@@ -2206,14 +2216,16 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
     return 0;
 }
 
-int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done, int* iters_h) {
+int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done, int* iters_h, int* gn_h) {
     if (which != 0) iters_h = nullptr;
-    if (which == 0 && B.fused)
+    if (which == 0 && B.fused && !gn_h)
         hipLaunchKernelGGL(fused_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
                            k_done, flag_h, iters_h);
+    else if (which == 0 && B.fused)
+        return -1;  // the two-kernel form's flag reports no Gauss-Newton state
     else
         hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
-                           which, flag_h, iters_h);
+                           which, flag_h, iters_h, gn_h);
     HIP_RET(hipGetLastError());
     return 0;
 }

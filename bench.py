@@ -168,7 +168,8 @@ def partition_streams(ctx, solve_cus, deform_skip, samples_cus=0, side=None):
     """CU-masked streams of the pipelined form. solve_cus > 0: ctx (the solve)
     moves to a stream on CUs [0, solve_cus) and the deform stream runs on the
     other CUs; the samples stream runs beside the deform (samples_cus 0), on
-    every CU (1) or beside the solve (2). Else deform_skip > 0: the deform and
+    every CU (1) or beside the solve (2); samples_cus 3 queues the samples on
+    the deform stream itself. Else deform_skip > 0: the deform and
     samples streams leave CUs [0, deform_skip) to the solve, which keeps every
     CU. side: a solver whose asynchronous coarse rebuilds move to a stream on
     the deform side's CUs (solve_cus > 0). Returns the streams to close (after
@@ -215,6 +216,7 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
     try:
         B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
         B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
+        B["solver"].set_host_spin(B["cfg"].get("host_spin", 0))
         B["solver"].set_host_checks(B["cfg"]["host_checks"])
         B["solver"].set_warm_start(B["cfg"]["warm"])
         B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
@@ -284,6 +286,7 @@ def timed(B, plain, steps, warmup):
     reset_state(B)
     B["solver"].set_cg_mode(0 if plain else B["cfg"].get("cg_mode", 0))
     B["solver"].set_cg_predict(0 if plain else B["cfg"].get("cg_predict", 0))
+    B["solver"].set_host_spin(0 if plain else B["cfg"].get("host_spin", 0))
     B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
     B["solver"].set_warm_start(0 if plain else B["cfg"]["warm"])
     B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
@@ -328,6 +331,7 @@ def apply_config(B, cfg):
     B["solver"].set_cg_predict(cfg.get("cg_predict", 0))
     B["solver"].set_cg_assembled(cfg.get("cg_asm", 0))
     B["solver"].set_cg_tol(cfg.get("cg_tol", 1e-4))
+    B["solver"].set_host_spin(cfg.get("host_spin", 0))
     ctx.set_blend_unroll(cfg["blend_unroll"])
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
@@ -431,6 +435,7 @@ def record_frames(B, steps, warmup):
     start = {k: (v.cpu().numpy() if v is not None else None) for k, v in B["state"].items()}
     B["solver"].set_cg_mode(B["cfg"].get("cg_mode", 0))
     B["solver"].set_cg_predict(B["cfg"].get("cg_predict", 0))
+    B["solver"].set_host_spin(B["cfg"].get("host_spin", 0))
     B["solver"].set_host_checks(B["cfg"]["host_checks"])
     B["solver"].set_warm_start(B["cfg"]["warm"])
     B["solver"].set_cg_graphs(B["cfg"].get("cg_graphs", 0))
@@ -593,6 +598,7 @@ def fixed_iteration_leg(B, args):
         plain = form == "plain"
         B["solver"].set_cg_mode(0 if plain else B["cfg"].get("cg_mode", 0))
         B["solver"].set_cg_predict(0 if plain else B["cfg"].get("cg_predict", 0))
+        B["solver"].set_host_spin(0 if plain else B["cfg"].get("host_spin", 0))
         B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
         B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
         ms = []
@@ -664,6 +670,7 @@ def large_leg(args, tuning):
     solver.set_cg_graphs(tuning.get("cg_graphs", 0))
     solver.set_cg_mode(tuning.get("cg_mode", 0))
     solver.set_cg_predict(tuning.get("cg_predict", 0))
+    solver.set_host_spin(tuning.get("host_spin", 0))
     solver.set_coarse_lag(tuning.get("coarse_lag", 1))
     solver.set_cg_assembled(tuning.get("cg_asm", 0))
     s0 = workload.clone_state(st)
@@ -995,6 +1002,8 @@ def main():
     ap.add_argument("--side-cus", type=int, default=-1, choices=(-1, 0, 1),
                     help="with --solve-cus: the asynchronous coarse rebuilds on the deform side's CUs (1) or on "
                          "every CU (0) (-1: autotuned, or 0)")
+    ap.add_argument("--host-spin", type=int, default=-1, choices=(-1, 0, 1),
+                    help="host checks poll the pinned flag (1) or wait for the stream (0) (-1: autotuned, or 0)")
     ap.add_argument("--cg-tol", type=float, default=0.0,
                     help="CG relative residual tolerance of the fast and pipelined forms (0: 1e-4)")
     ap.add_argument("--split-render", type=int, default=-1, choices=(-1, 0, 1),
@@ -1093,6 +1102,8 @@ def main():
         apply_config(B, dict(B["cfg"], split_render=args.split_render))
     if args.cg_tol > 0:
         apply_config(B, dict(B["cfg"], cg_tol=args.cg_tol))
+    if args.host_spin >= 0:
+        apply_config(B, dict(B["cfg"], host_spin=args.host_spin))
     res = {}
     for form in args.forms.split(","):
         log(f"timing form={form}: {args.warmup} warmup + {args.steps} steps")

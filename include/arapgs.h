@@ -237,6 +237,13 @@ int arapgs_solver_last_stats(arapgs_solver* s, int* gn_iters_h, int* cg_iters_h)
  * iterates do not depend on the host-check interval or on CG graphs. */
 int arapgs_solver_set_cg_mode(arapgs_solver* s, int mode);
 
+/* on = 1: the host checks of arapgs_gn_solve poll the flag's pinned slot,
+ * which the flag kernel publishes with a system-scope release, instead of
+ * waiting for the stream (a slot still unset after 50 ms falls back to the
+ * stream wait). Iterates are unchanged; only the host's wake-up is faster.
+ * No counterpart in the reference (its solve runs on the CPU). */
+int arapgs_solver_set_host_spin(arapgs_solver* s, int on);
+
 /* The CG's relative residual tolerance after creation (desc.cg_tol; 0 < tol
  * < 1). The reference solves each Gauss-Newton step exactly (sparse
  * Cholesky); a tolerance is an inexact-Newton choice whose accuracy is judged
@@ -249,7 +256,11 @@ int arapgs_solver_set_cg_tol(arapgs_solver* s, float tol);
  * counterpart in the reference; host scheduling only: the device-side
  * convergence test decides, so the results are the same bit for bit).
  * Consecutive drag frames need similar counts, so fewer launches run past
- * convergence. arapgs_solver_reset forgets the counts. */
+ * convergence. With the three-kernel CG (cg_mode 0) the Gauss-Newton stop
+ * flag is predicted too: before the previous solve's GN count, the next GN
+ * iteration is queued without waiting for the flag (its kernels do nothing
+ * once the loop has stopped) and the flag is read at that iteration's first
+ * CG check. arapgs_solver_reset forgets the counts. */
 int arapgs_solver_set_cg_predict(arapgs_solver* s, int on);
 
 /* Energy |f(x)|^2 (FP64 reduction) of a given unknown vector (Deform::CalcEnergyFunc,

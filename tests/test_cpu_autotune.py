@@ -17,7 +17,8 @@ def _cost(cfg):
             + (0.02 if cfg["samples_cus"] != 1 else 0.0) + (0.03 if cfg["sh_compose"] == 0 else 0.0)
             + (0.04 if cfg["tile_cull"] == 0 else 0.0) + abs(cfg["coarse_lag"] - 2) / 10
             + (0.01 if cfg["side_cus"] == 0 else 0.0) + (0.01 if cfg["samples_late"] == 0 else 0.0)
-            + (0.01 if cfg["cg_asm"] == 0 else 0.0) + (0.01 if cfg["split_render"] == 0 else 0.0) + 1.0)
+            + (0.01 if cfg["cg_asm"] == 0 else 0.0) + (0.01 if cfg["split_render"] == 0 else 0.0)
+            + (0.01 if cfg["host_spin"] == 0 else 0.0) + 1.0)
 
 
 def test_finds_the_separable_optimum():
@@ -33,7 +34,7 @@ def test_finds_the_separable_optimum():
                         solve_cus=64, samples_cus=1, sh_compose=1,
                         tile_cull=1, coarse_lag=2, side_cus=1,
                         samples_late=1, cg_asm=1,
-                        split_render=1)
+                        split_render=1, host_spin=1)
     assert len(seen) == len(set(seen)) == len(trials)
     assert min(t for _, t in trials) == _cost(best)
 

@@ -405,24 +405,38 @@ def test_two_kernel_cg_warm_drag_and_center_mode(ctx, prob, block):
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_cg_predict_bit_identical(ctx, prob, mode):
-    """The first CG host check at the previous solve's count (set_cg_predict) is
-    host scheduling only: four warm-started drag frames give the same iterates
-    bit for bit as without it (CG modes 0 and 2, host checks every 2)."""
+    """The first CG host check at the previous solve's count (set_cg_predict),
+    with the three-kernel CG also the Gauss-Newton stop flag read only from the
+    previous solve's GN count on, and host checks that poll the pinned flag
+    (set_host_spin) are host scheduling only: warm-started drag frames give the
+    same iterates bit for bit as without them (CG modes 0 and 2, host checks
+    every 2). The frames include a fixed-count solve (stop rule off) followed
+    by stop-rule solves, so the predicted GN count is both too high and too
+    low at least once."""
     runs = []
-    for predict in (False, True):
+    frames = ((0.30, True, 8), (0.35, False, 5), (0.40, True, 8), (0.45, True, 8), (0.70, True, 8),
+              (0.72, True, 8))
+    for predict, spin in ((False, False), (True, False), (True, True)):
         s = _solver(ctx, prob)
         s.set_cg_mode(mode)
         s.set_host_checks(2)
         s.set_warm_start(True)
         s.set_cg_predict(predict)
+        s.set_host_spin(spin)
         out = []
-        for fraction in (0.30, 0.35, 0.40, 0.45):
+        for fraction, stop, iters in frames:
             aims, _ = _problem(prob, fraction)
-            x, e, done = s.solve(T(prob["nodes"]), T(aims), 8, stop_rule=True)
+            x, e, done = s.solve(T(prob["nodes"]), T(aims), iters, stop_rule=stop)
             out.append((H(x), H(e), int(H(done)[0]), s.last_stats()))
         runs.append(out)
         s.close()
-    for a, b in zip(*runs):
+    print("GN counts:", [o[2] for o in runs[0]], flush=True)
+    for other in runs[2:]:
+        for a, b in zip(runs[0], other):
+            assert a[2] == b[2] >= 1 and a[3] == b[3]
+            assert np.array_equal(a[0], b[0])
+            assert np.array_equal(a[1][: a[2] + 1], b[1][: b[2] + 1])
+    for a, b in zip(runs[0], runs[1]):
         assert a[2] == b[2] >= 1 and a[3] == b[3]
         assert np.array_equal(a[0], b[0])
         assert np.array_equal(a[1][: a[2] + 1], b[1][: b[2] + 1])
