@@ -1709,27 +1709,32 @@ __global__ void __launch_bounds__(128) coarse_assemble_kernel(const SolverEntry*
 
 // Blocked in-place Gauss-Jordan inverse of the SPD coarse matrix (no pivoting;
 // n a multiple of 64, padding rows/cols hold the identity). Per 64-wide pivot
-// block K: (1) invert A_KK in LDS, (2) scale the row panel by A_KK^-1 and save
-// the old column panel, (3) rank-64 update of every other tile and the new
-// column panel -A_iK A_KK^-1.
+// block K with P = A_KK^-1: A_Kj <- P A_Kj, A_ij <- A_ij - A_iK (P A_Kj),
+// A_iK <- -A_iK P, A_KK <- P. One launch per K: every tile reads the old
+// column and row panels from copies made by the previous launch (so tiles are
+// updated in place without races) and writes the copies of the next panels;
+// the tile (K+1, K+1), dispatched first, inverts its new value into the next
+// pivot buffer after its update.
 constexpr int GJ_T = 64;
 
 // Inverse of the 64x64 pivot tile by in-place Gauss-Jordan, four waves: lane j
 // of wave w keeps rows 16w..16w+15 of column j in registers. Step k: the wave
-// holding row k publishes it in LDS (one barrier), every wave reads its rows of
-// the pivot column from lane k with v_readlane (the loops are unrolled, so
-// every register index is static). Per entry the same operations in the same
-// order as the textbook in-place form: row k scaled by 1/d,
-// P_ij -= P_ik (P_kj / d), column k -> -P_ik / d.
+// holding row k publishes it in LDS (one barrier; the row buffer alternates, so
+// step k+2 overwrites a buffer only after every wave has passed step k+1's
+// barrier), every wave reads its rows of the pivot column from lane k with
+// v_readlane (the loops are unrolled, so every register index is static). Per
+// entry the same operations in the same order as the textbook in-place form:
+// row k scaled by 1/d, P_ij -= P_ik (P_kj / d), column k -> -P_ik / d.
 constexpr int GJ_R = GJ_T / 4;  // rows per wave
 
 template <int K>
-__device__ __forceinline__ void gj_step4(float (&c)[GJ_R], int lane, int wave, float* prow) {
-    if (wave == K / GJ_R) prow[lane] = c[K % GJ_R];
+__device__ __forceinline__ void gj_step4(float (&c)[GJ_R], int lane, int wave, float (*prow)[GJ_T]) {
+    float* pr = prow[K & 1];
+    if (wave == K / GJ_R) pr[lane] = c[K % GJ_R];
     __syncthreads();
-    const float d = prow[K];
+    const float d = pr[K];
     const float iv = (fabsf(d) > 1e-30f) ? 1.0f / d : 0.0f;
-    const float rk = prow[lane] * iv;
+    const float rk = pr[lane] * iv;
 #pragma unroll
     for (int r = 0; r < GJ_R; r++) {
         const float colv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[r]), K));
@@ -1737,32 +1742,35 @@ __device__ __forceinline__ void gj_step4(float (&c)[GJ_R], int lane, int wave, f
         if (lane == K) c[r] = row_k ? iv : colv * -iv;
         else c[r] = row_k ? rk : c[r] - colv * rk;
     }
-    __syncthreads();
 }
 
 template <int K>
-__device__ __forceinline__ void gj_steps4(float (&c)[GJ_R], int lane, int wave, float* prow) {
+__device__ __forceinline__ void gj_steps4(float (&c)[GJ_R], int lane, int wave, float (*prow)[GJ_T]) {
     if constexpr (K < GJ_T) {
         gj_step4<K>(c, lane, wave, prow);
         gj_steps4<K + 1>(c, lane, wave, prow);
     }
 }
 
-__global__ void __launch_bounds__(256) gj_pivot_kernel(const SolverEntry* __restrict__ B, int kb) {
-    const SolverEntry E = load_entry(B);
-    if (kb >= E.nt) return;
-    float* __restrict__ A = E.acinv;
-    const int n = E.S.nc_pad;
-    float* __restrict__ piv = E.gj_piv;
-    __shared__ float prow[GJ_T];
-    const int k0 = kb * GJ_T;
+// The inverse of the 64x64 tile T (LDS) into dst (64x64 row-major); 256 threads.
+__device__ __forceinline__ void gj_invert_tile(const float (*T)[GJ_T + 1], float* __restrict__ dst,
+                                               float (*prow)[GJ_T]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float c[GJ_R];
 #pragma unroll
-    for (int r = 0; r < GJ_R; r++) c[r] = A[(int64_t)(k0 + wave * GJ_R + r) * n + k0 + lane];
+    for (int r = 0; r < GJ_R; r++) c[r] = T[wave * GJ_R + r][lane];
     gj_steps4<0>(c, lane, wave, prow);
 #pragma unroll
-    for (int r = 0; r < GJ_R; r++) piv[(wave * GJ_R + r) * GJ_T + lane] = c[r];
+    for (int r = 0; r < GJ_R; r++) dst[(wave * GJ_R + r) * GJ_T + lane] = c[r];
+}
+
+// 64x64 tile (row stride ld) into LDS.
+__device__ __forceinline__ void load_tile(float (*T)[GJ_T + 1], const float* __restrict__ src, int64_t ld) {
+    for (int e = threadIdx.x; e < GJ_T * GJ_T / 4; e += blockDim.x) {
+        const int r = e / (GJ_T / 4), c4 = e % (GJ_T / 4);
+        const float4 v = *reinterpret_cast<const float4*>(src + (int64_t)r * ld + 4 * c4);
+        T[r][4 * c4] = v.x; T[r][4 * c4 + 1] = v.y; T[r][4 * c4 + 2] = v.z; T[r][4 * c4 + 3] = v.w;
+    }
 }
 
 // 64x64 = (64x64) (64x64) products from LDS tiles: thread t accumulates the 4x4
@@ -1787,71 +1795,134 @@ __device__ __forceinline__ void tile_mm(const float (*X)[GJ_T + 1], const float 
     }
 }
 
-// blockIdx.x = tile column jb != kb: row panel A_Kj <- piv * A_Kj; also copies
-// the old column panel tile A_{jb,K} into colbuf.
-__global__ void __launch_bounds__(256) gj_panel_kernel(const SolverEntry* __restrict__ B, int kb) {
-    const SolverEntry E = load_entry(B);
-    const int jb = blockIdx.x;
-    if (kb >= E.nt || jb >= E.nt || jb == kb) return;
-    float* __restrict__ A = E.acinv;
-    const int n = E.S.nc_pad;
-    const float* __restrict__ piv = E.gj_piv;
-    float* __restrict__ colbuf = E.gj_col;
-    __shared__ float Pv[GJ_T][GJ_T + 1];
-    __shared__ float R[GJ_T][GJ_T + 1];
-    const int k0 = kb * GJ_T, j0 = jb * GJ_T;
-    for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
-        const int r = e / GJ_T, c = e % GJ_T;
-        Pv[r][c] = piv[e];
-        R[r][c] = A[(int64_t)(k0 + r) * n + j0 + c];
-        colbuf[(int64_t)(j0 + r) * GJ_T + c] = A[(int64_t)(j0 + r) * n + k0 + c];
-    }
-    __syncthreads();
-    float acc[4][4];
-    tile_mm(Pv, R, acc);
-    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) A[(int64_t)(k0 + 4 * tr + i) * n + j0 + 4 * tc + j] = acc[i][j];
+// Panel copies of the Gauss-Jordan sweep: gj_col holds four arrays of nt tiles
+// (64x64, contiguous): column panels [0], [1] and row panels [2], [3], the
+// pair K & 1 read by the launch of pivot block K. gj_piv holds two pivot
+// inverses (K & 1).
+__device__ __forceinline__ float* gj_panel(const SolverEntry& E, int which, int tile) {
+    return E.gj_col + ((int64_t)which * E.S.nc_pad + (int64_t)tile * GJ_T) * GJ_T;
 }
 
-// blockIdx = (ib, jb): A_ij -= C_iK * A_Kj (new row panel) for ib != kb;
-// for jb == kb the tile becomes -C_iK * piv; tile (kb, kb) becomes piv.
-__global__ void __launch_bounds__(256) gj_update_kernel(const SolverEntry* __restrict__ B, int kb) {
+// Before pivot block 0: the panels of block 0 and the inverse of A_00.
+// blockIdx.x = tile b: column-panel tile (b, 0) and row-panel tile (0, b).
+__global__ void __launch_bounds__(256) gj_start_kernel(const SolverEntry* __restrict__ B) {
     const SolverEntry E = load_entry(B);
-    const int ib = blockIdx.y, jb = blockIdx.x;
-    if (kb >= E.nt || ib >= E.nt || jb >= E.nt) return;
-    float* __restrict__ A = E.acinv;
+    const int b = blockIdx.x;
+    if (b >= E.nt) return;
+    const float* __restrict__ A = E.acinv;
     const int n = E.S.nc_pad;
-    const float* __restrict__ piv = E.gj_piv;
-    const float* __restrict__ colbuf = E.gj_col;
-    const int k0 = kb * GJ_T, i0 = ib * GJ_T, j0 = jb * GJ_T;
-    if (ib == kb) {
-        if (jb == kb)
-            for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x)
-                A[(int64_t)(k0 + e / GJ_T) * n + k0 + e % GJ_T] = piv[e];
-        return;
-    }
-    __shared__ float Cs[GJ_T][GJ_T + 1];
-    __shared__ float Bs[GJ_T][GJ_T + 1];
+    float* __restrict__ col = gj_panel(E, 0, b);
+    float* __restrict__ row = gj_panel(E, 2, b);
     for (int e = threadIdx.x; e < GJ_T * GJ_T; e += blockDim.x) {
         const int r = e / GJ_T, c = e % GJ_T;
-        Cs[r][c] = colbuf[(int64_t)(i0 + r) * GJ_T + c];
-        Bs[r][c] = (jb == kb) ? piv[e] : A[(int64_t)(k0 + r) * n + j0 + c];
+        col[e] = A[(int64_t)(b * GJ_T + r) * n + c];
+        row[e] = A[(int64_t)r * n + b * GJ_T + c];
     }
-    __syncthreads();
-    float acc[4][4];
-    tile_mm(Cs, Bs, acc);
+    if (b == 0) {
+        __shared__ float T[GJ_T][GJ_T + 1];
+        __shared__ float prow[2][GJ_T];
+        load_tile(T, A, n);
+        __syncthreads();
+        gj_invert_tile(T, E.gj_piv, prow);
+    }
+}
+
+// Pivot block k: one workgroup per tile (i, j), linear order with the tile
+// (k+1, k+1) first. Products in the order of the three-phase form
+// (P A_Kj first, then A_iK times that), so the inverse does not depend on how
+// the phases are launched.
+__global__ void __launch_bounds__(256) gj_step_kernel(const SolverEntry* __restrict__ B, int k) {
+    const SolverEntry E = load_entry(B);
+    const int nt = E.nt;
+    if (k >= nt || (int)blockIdx.x >= nt * nt) return;
+    const int kn = k + 1;
+    int lin = blockIdx.x;
+    if (kn < nt) {
+        const int dn = kn * nt + kn;
+        lin = lin == 0 ? dn : (lin == dn ? 0 : lin);
+    }
+    const int i = lin / nt, j = lin % nt;
+    float* __restrict__ A = E.acinv;
+    const int n = E.S.nc_pad;
+    const float* __restrict__ piv = E.gj_piv + (k & 1) * GJ_T * GJ_T;
+    __shared__ float Ps[GJ_T][GJ_T + 1];
+    __shared__ float Rs[GJ_T][GJ_T + 1];
+    __shared__ float Cs[GJ_T][GJ_T + 1];
+    __shared__ float prow[2][GJ_T];
     const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    float out[4][4];
+    if (i == k && j == k) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int64_t o = (int64_t)(i0 + 4 * tr + i) * n + j0 + 4 * tc + j;
-            if (jb == kb) A[o] = -acc[i][j];
-            else A[o] -= acc[i][j];
-        }
+            for (int b = 0; b < 4; b++) out[a][b] = piv[(4 * tr + a) * GJ_T + 4 * tc + b];
+    } else if (i == k) {
+        load_tile(Ps, piv, GJ_T);
+        load_tile(Rs, gj_panel(E, 2 + (k & 1), j), GJ_T);
+        __syncthreads();
+        tile_mm(Ps, Rs, out);
+    } else if (j == k) {
+        load_tile(Cs, gj_panel(E, k & 1, i), GJ_T);
+        load_tile(Ps, piv, GJ_T);
+        __syncthreads();
+        tile_mm(Cs, Ps, out);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) out[a][b] = -out[a][b];
+    } else {
+        load_tile(Ps, piv, GJ_T);
+        load_tile(Rs, gj_panel(E, 2 + (k & 1), j), GJ_T);
+        load_tile(Cs, gj_panel(E, k & 1, i), GJ_T);
+        float old[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) old[a][b] = A[(int64_t)(i * GJ_T + 4 * tr + a) * n + j * GJ_T + 4 * tc + b];
+        __syncthreads();
+        float t[4][4];
+        tile_mm(Ps, Rs, t);  // the new row-panel tile P A_Kj
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) Rs[4 * tr + a][4 * tc + b] = t[a][b];
+        __syncthreads();
+        tile_mm(Cs, Rs, out);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) out[a][b] = old[a][b] - out[a][b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) A[(int64_t)(i * GJ_T + 4 * tr + a) * n + j * GJ_T + 4 * tc + b] = out[a][b];
+    if (kn >= nt || (i != kn && j != kn)) return;
+    // panels of pivot block k+1
+    if (j == kn) {
+        float* __restrict__ col = gj_panel(E, kn & 1, i);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) col[(4 * tr + a) * GJ_T + 4 * tc + b] = out[a][b];
+    }
+    if (i == kn) {
+        float* __restrict__ row = gj_panel(E, 2 + (kn & 1), j);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) row[(4 * tr + a) * GJ_T + 4 * tc + b] = out[a][b];
+    }
+    if (i == kn && j == kn) {
+        __syncthreads();  // Cs is free again
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) Cs[4 * tr + a][4 * tc + b] = out[a][b];
+        __syncthreads();
+        gj_invert_tile(Cs, E.gj_piv + (kn & 1) * GJ_T * GJ_T, prow);
+    }
 }
 
 __global__ void pad_identity_kernel(const SolverEntry* __restrict__ B) {
@@ -2276,11 +2347,9 @@ static int coarse_build(const SolverBatch& B, const SolverEntry* E, hipStream_t 
     hipLaunchKernelGGL(coarse_to_float_kernel, dim3(256, 1, nz), dim3(256), 0, st, E);
     hipLaunchKernelGGL(pad_identity_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
     const unsigned nt = (unsigned)B.max_nt;
-    for (int kb = 0; kb < (int)nt; kb++) {
-        hipLaunchKernelGGL(gj_pivot_kernel, dim3(1, 1, nz), dim3(256), 0, st, E, kb);
-        hipLaunchKernelGGL(gj_panel_kernel, dim3(nt, 1, nz), dim3(256), 0, st, E, kb);
-        hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nz), dim3(256), 0, st, E, kb);
-    }
+    hipLaunchKernelGGL(gj_start_kernel, dim3(nt, 1, nz), dim3(256), 0, st, E);
+    for (int kb = 0; kb < (int)nt; kb++)
+        hipLaunchKernelGGL(gj_step_kernel, dim3(nt * nt, 1, nz), dim3(256), 0, st, E, kb);
     HIP_RET(hipGetLastError());
     return 0;
 }

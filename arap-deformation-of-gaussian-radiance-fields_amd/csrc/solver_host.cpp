@@ -299,8 +299,8 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     }
     if (!rc && hipEventCreateWithFlags(&W.built_ev, hipEventDisableTiming) != hipSuccess) rc = -2;
     W.ac64 = alloc<unsigned long long>(W, (size_t)S.nc_pad * S.nc_pad, rc);
-    W.gj_piv = alloc<float>(W, 64 * 64, rc);
-    W.gj_col = alloc<float>(W, (size_t)S.nc_pad * 64, rc);
+    W.gj_piv = alloc<float>(W, 2 * 64 * 64, rc);                  // two pivot inverses
+    W.gj_col = alloc<float>(W, 4 * (size_t)S.nc_pad * 64, rc);    // column and row panels, two each
     const int nb_owner = (nF + 15) / 16;
     // p.q partials of the CG apply at the smallest CG block size (64: 4 owners per
     // workgroup) or one per aggregate (two-kernel form)

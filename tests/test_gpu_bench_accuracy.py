@@ -59,6 +59,7 @@ def _tuned_solver(ctx, args):
     s.set_cg_mode(TUNED["cg_mode"])
     s.set_cg_predict(TUNED["cg_predict"])
     s.set_cg_block(TUNED["cg_block"])
+    s.set_host_spin(TUNED["host_spin"])
     return s
 
 
