@@ -61,6 +61,10 @@ __device__ __forceinline__ SolverEntry load_entry(const SolverEntry* __restrict_
     S.pair_ptr = gptr(S.pair_ptr); S.pair_j = gptr(S.pair_j); S.term_ptr = gptr(S.term_ptr); S.terms = gptr(S.terms);
     S.agg_member = gptr(S.agg_member); S.agg_of = gptr(S.agg_of); S.agg_center = gptr(S.agg_center);
     S.g = gptr(S.g); S.aim = gptr(S.aim);
+    S.wt_out_nb = gptr(S.wt_out_nb); S.wt_in_nb = gptr(S.wt_in_nb); S.wt_cm_rng = gptr(S.wt_cm_rng);
+    S.wt_con_nb = gptr(S.wt_con_nb); S.wt_out = gptr(S.wt_out); S.wt_in = gptr(S.wt_in); S.wt_cm = gptr(S.wt_cm);
+    S.wt_con = gptr(S.wt_con); S.wt_out_ik = gptr(S.wt_out_ik); S.wt_in_is = gptr(S.wt_in_is);
+    S.wt_cm_i = gptr(S.wt_cm_i);
     E.x = gptr(E.x); E.hsol = gptr(E.hsol); E.r = gptr(E.r); E.z = gptr(E.z); E.p = gptr(E.p); E.q = gptr(E.q);
     E.binv = gptr(E.binv); E.rc = gptr(E.rc); E.acinv = gptr(E.acinv); E.gj_piv = gptr(E.gj_piv);
     E.dmat = gptr(E.dmat); E.pair_m = gptr(E.pair_m);
@@ -109,9 +113,11 @@ __device__ __forceinline__ int group_residual_rows(const SolverDev& S, const flo
     g -= S.nF;
     if (g < (int64_t)S.nF * S.K) {
         const int fi = (int)(g / S.K), t = (int)(g % S.K);
-        const int i = S.free_nodes[fi];
-        const int k = (int)S.nbr[(int64_t)i * S.K + t];
-        const int fk = S.free_id[k];
+        // (i, k) and k's free id from the static walk tables (the index
+        // chain free_nodes -> nbr -> free_id resolved once)
+        const int2 ik = S.wt_out_ik[g];
+        const int i = ik.x, k = ik.y;
+        const int fk = S.wt_out_nb[g];
         const double s = S.edge_scale[(int64_t)i * S.K + t];
         const float* gi = S.g + 3 * i;
         const float* gk = S.g + 3 * k;
@@ -160,7 +166,7 @@ __device__ __forceinline__ int group_residual_rows(const SolverDev& S, const flo
         for (int j = 0; j < S.K; j++) {
             const int nj = (int)S.src_idx[(int64_t)c * S.K + j];
             const double w = S.src_w[(int64_t)c * S.K + j];
-            const int fj = S.free_id[nj];
+            const int fj = S.wt_con_nb[(int64_t)q * S.K + j];  // free_id[nj]
             if (fj < 0) {
                 n0 += w * v0;
                 n1 += w * v1;
@@ -273,7 +279,6 @@ __global__ void reduce_partials_kernel(const SolverEntry* __restrict__ B, int c0
 template <int KT = 0, bool WITH_JV = true, typename FnRot, typename Fn3>
 __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item, const float* __restrict__ x,
                                            const float* __restrict__ v, FnRot&& fr, Fn3&& f3) {
-    const int i = S.free_nodes[fi];
     if (item == 0) {
         // rot(i): d(c_a . c_b)/d c_a = c_b, d|c_j|^2 / d c_j = 2 c_j
         float Ji[6][9];
@@ -311,14 +316,15 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
     float a[3] = {0.0f, 0.0f, 0.0f}, b = 0.0f;
     float Jv[3] = {0.0f, 0.0f, 0.0f};
     item -= 1;
+    // the walk tables hold the frame's edge vectors, scales, group blocks and
+    // source slots, computed with the same operations as from the node
+    // positions directly (walk_table_kernel)
     if (item < S.K) {
-        const int t = item;
-        const int k = (int)S.nbr[(int64_t)i * S.K + t];
-        const int fk = S.free_id[k];
-        const float s = S.edge_scale[(int64_t)i * S.K + t];
-        const float* gi = S.g + 3 * i;
-        const float* gk = S.g + 3 * k;
-        const float d[3] = {gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2]};
+        const int64_t o = (int64_t)fi * S.K + item;
+        const float4 e = S.wt_out[o];
+        const int fk = S.wt_out_nb[o];
+        const float s = e.w;
+        const float d[3] = {e.x, e.y, e.z};
 #pragma unroll
         for (int c = 0; c < 3; c++) a[c] = S.sw_reg * s * d[c];
         b = S.sw_reg;
@@ -335,11 +341,11 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         return;
     }
     item -= S.K;
-    const int n_in = S.in_ptr[fi + 1] - S.in_ptr[fi];
+    const int ib = S.in_ptr[fi];
+    const int n_in = S.in_ptr[fi + 1] - ib;
     if (item < n_in) {
-        const int e = S.in_edge[S.in_ptr[fi] + item];  // src*K + t
-        const int src = e / S.K;
-        const int fs = S.free_id[src];
+        const int q = ib + item;
+        const int fs = S.wt_in_nb[q];
         if (fs < 0) {
             // static(src, t): rows -sw_reg t_i
             b = -S.sw_reg;
@@ -348,12 +354,11 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
                 for (int r = 0; r < 3; r++) Jv[r] = -S.sw_reg * v[12 * (int64_t)fi + 9 + r];
             }
         } else {
-            const float s = S.edge_scale[e];
+            const float4 e = S.wt_in[q];
+            const float s = e.w;
             b = -S.sw_reg * s;
             if (WITH_JV) {
-                const float* gs = S.g + 3 * src;
-                const float* gi = S.g + 3 * i;
-                const float d[3] = {gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2]};
+                const float d[3] = {e.x, e.y, e.z};
                 const int64_t so = 12 * (int64_t)fs;
 #pragma unroll
                 for (int r = 0; r < 3; r++)
@@ -366,34 +371,25 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
     }
     item -= n_in;
     const int ent = S.cm_ptr[fi] + item;
-    const int cid = S.cm_con[ent];
     {
         // J of the group wrt node i: sum over the (member, slot) entries that reach i
-        const float* gi = S.g + 3 * i;
-        for (int q = S.cm_sbeg[ent]; q < S.cm_sbeg[ent + 1]; q++) {
-            const int e = S.cm_src[q];
-            const int c = e / S.K;
-            const float wi = S.src_w[e];
-            const float dd[3] = {S.g[3 * c] - gi[0], S.g[3 * c + 1] - gi[1], S.g[3 * c + 2] - gi[2]};
-#pragma unroll
-            for (int cc = 0; cc < 3; cc++) a[cc] += S.sw_con * wi * dd[cc];
-            b += S.sw_con * wi;
-        }
+        const float4 ab = S.wt_cm[ent];
+        a[0] = ab.x; a[1] = ab.y; a[2] = ab.z;
+        b = ab.w;
     }
     if (WITH_JV) {
         const int KK = KT > 0 ? KT : S.K;
-        for (int m = S.con_ptr[cid]; m < S.con_ptr[cid + 1]; m++) {
-            const int c = S.con_mem[m];
-            const float vv[3] = {S.g[3 * c], S.g[3 * c + 1], S.g[3 * c + 2]};
+        const int2 rng = S.wt_cm_rng[ent];
+        for (int m = rng.x; m < rng.y; m++) {
 #pragma unroll
             for (int j = 0; j < (KT > 0 ? KT : KNN_MAX_T); j++) {
                 if (KT == 0 && j >= KK) break;
-                const int nj = (int)S.src_idx[(int64_t)c * KK + j];
-                const int fj = S.free_id[nj];
+                const int64_t o = (int64_t)m * KK + j;
+                const int fj = S.wt_con_nb[o];
                 if (fj < 0) continue;
-                const float w = S.src_w[(int64_t)c * KK + j];
-                const float* gj = S.g + 3 * nj;
-                const float dd[3] = {vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2]};
+                const float4 e = S.wt_con[o];
+                const float w = e.w;
+                const float dd[3] = {e.x, e.y, e.z};
                 const int64_t jo = 12 * (int64_t)fj;
 #pragma unroll
                 for (int r = 0; r < 3; r++) Jv[r] += S.sw_con * w * (v[jo + r] * dd[0] + v[jo + 3 + r] * dd[1] + v[jo + 6 + r] * dd[2] + v[jo + 9 + r]);
@@ -401,6 +397,65 @@ __device__ __forceinline__ void owner_item(const SolverDev& S, int fi, int item,
         }
     }
     f3(a, b, Jv);
+}
+
+// The frame's walk tables (node positions fixed for the solve), one record per
+// thread: out-edges, in-edges, memberships, then constraint members (K source
+// slots each). Every value is formed with the operations the owner walk used
+// on the node positions.
+__device__ __forceinline__ int64_t walk_records(const SolverDev& S) {
+    return (int64_t)S.nF * S.K + S.n_in + S.n_cm + S.n_mem;
+}
+
+__global__ void __launch_bounds__(256) walk_table_kernel(const SolverEntry* __restrict__ B) {
+    const SolverEntry E = load_entry(B);
+    const SolverDev& S = E.S;
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = S.K;
+    const float* __restrict__ g = S.g;
+    if (r < (int64_t)S.nF * K) {
+        const int2 ik = S.wt_out_ik[r];
+        const float* gi = g + 3 * ik.x;
+        const float* gk = g + 3 * ik.y;
+        S.wt_out[r] = make_float4(gk[0] - gi[0], gk[1] - gi[1], gk[2] - gi[2],
+                                  S.edge_scale[(int64_t)ik.x * K + r % K]);
+        return;
+    }
+    r -= (int64_t)S.nF * K;
+    if (r < S.n_in) {
+        const int2 is = S.wt_in_is[r];
+        const float* gi = g + 3 * is.x;
+        const float* gs = g + 3 * is.y;
+        S.wt_in[r] = make_float4(gi[0] - gs[0], gi[1] - gs[1], gi[2] - gs[2], S.edge_scale[S.in_edge[r]]);
+        return;
+    }
+    r -= S.n_in;
+    if (r < S.n_cm) {
+        const int ent = (int)r;
+        const float* gi = g + 3 * S.wt_cm_i[ent];
+        float a[3] = {0.0f, 0.0f, 0.0f}, b = 0.0f;
+        for (int q = S.cm_sbeg[ent]; q < S.cm_sbeg[ent + 1]; q++) {
+            const int e = S.cm_src[q];
+            const int c = e / K;
+            const float wi = S.src_w[e];
+            const float dd[3] = {g[3 * c] - gi[0], g[3 * c + 1] - gi[1], g[3 * c + 2] - gi[2]};
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) a[cc] += S.sw_con * wi * dd[cc];
+            b += S.sw_con * wi;
+        }
+        S.wt_cm[ent] = make_float4(a[0], a[1], a[2], b);
+        return;
+    }
+    r -= S.n_cm;
+    if (r < S.n_mem) {
+        const int c = S.con_mem[r];
+        const float vv[3] = {g[3 * c], g[3 * c + 1], g[3 * c + 2]};
+        for (int j = 0; j < K; j++) {
+            const int nj = (int)S.src_idx[(int64_t)c * K + j];
+            const float* gj = g + 3 * nj;
+            S.wt_con[r * K + j] = make_float4(vv[0] - gj[0], vv[1] - gj[1], vv[2] - gj[2], S.src_w[(int64_t)c * K + j]);
+        }
+    }
 }
 
 __device__ __forceinline__ int owner_items(const SolverDev& S, int fi) {
@@ -2233,6 +2288,8 @@ int solver_batch_init(SolverBatch& B, const std::vector<SolverWork*>& w, std::st
         const SolverEntry& E = host.back();
         if (W->dev.K != 8) B.kt = 0;
         B.max_nb_owner = std::max(B.max_nb_owner, E.nb_owner);
+        B.max_wt_blocks = std::max(B.max_wt_blocks,
+                                   (int)nblk((int64_t)E.S.nF * E.S.K + E.S.n_in + E.S.n_cm + E.S.n_mem, 256));
         B.max_nb_apply = std::max(B.max_nb_apply, E.nb_apply);
         if (W->cg_block != w[0]->cg_block) {
             err = "solver batch: every solver of a batch needs the same CG block size";
@@ -2373,6 +2430,7 @@ int solver_frame_setup(SolverBatch& B, bool refresh_coarse, hipStream_t st) {
         W->pending = false;
     }
     hipLaunchKernelGGL(init_x_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
+    hipLaunchKernelGGL(walk_table_kernel, dim3(B.max_wt_blocks, 1, nz), dim3(256), 0, st, E);
     if (B.w[0]->cg_asm)
         hipLaunchKernelGGL(pair_coef_kernel, dim3(nblk(B.max_n, 256), 1, nz), dim3(256), 0, st, E);
     if (refresh_coarse && !any_pending) {
@@ -2552,6 +2610,9 @@ int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_o
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.x, x, (int64_t)W.n, (const int*)nullptr);
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
                        (const int*)nullptr);
+    hipLaunchKernelGGL(walk_table_kernel,
+                       dim3(nblk((int64_t)W.dev.nF * W.dev.K + W.dev.n_in + W.dev.n_cm + W.dev.n_mem, 256)), dim3(256),
+                       0, st, E);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(nb_owner), dim3(256), 0, st, E);
     if (W.cg_asm) hipLaunchKernelGGL(pair_coef_kernel, dim3(nblk(W.dev.nF, 256)), dim3(256), 0, st, E);
     int rc = launch_apply(B, 1, st, 0, 1, -1, v);

@@ -42,6 +42,22 @@ struct SolverDev {
     const int* cm_src;         // source-weight entries (member*K + slot) that reach the free node
     const int* multi_con;      // n_multi groups with more than one member
     int n_multi;
+    // Walk tables of the CG operator: the index chains of the owner walk
+    // resolved once (static: neighbour free ids, member ranges) and the
+    // frame's geometry per record (rebuilt by walk_table_kernel every solve).
+    int n_mem;                 // constraint members (con_mem entries)
+    int n_in, n_cm;            // in-edges of free nodes, memberships
+    const int2* wt_out_ik;     // nF*K  (node i, neighbour k) of out-edge (free rank, t)
+    const int2* wt_in_is;      // in-edges: (owner node i, source node)
+    const int* wt_cm_i;        // memberships: owner node i
+    const int* wt_out_nb;      // nF*K  free id of out-edge t's neighbour (-1: static)
+    const int* wt_in_nb;       // in-edges: free id of the source (-1: static)
+    const int2* wt_cm_rng;     // memberships: the group's member range [con_ptr[c], con_ptr[c+1])
+    const int* wt_con_nb;      // n_mem*K  free id of each member's source slot (-1: static)
+    float4* wt_out;            // nF*K  (g_k - g_i, edge scale) of out-edge (i, t)
+    float4* wt_in;             // in-edges: (g_i - g_src, edge scale)
+    float4* wt_cm;             // memberships: the group's I3 block (a, b) wrt the node
+    float4* wt_con;            // n_mem*K  (g_member - g_src, weight) of each source slot
     // Assembled CG operator (plain constraint mode): J^T J = blockdiag(D) +
     // sum over free pairs (i, j), i != j, of (M_ij (x) I_3), M_ij a 4x4 matrix
     // over (R column 0, 1, 2, t). Pair p of owner i: j = pair_j[p], the terms
@@ -188,6 +204,7 @@ struct SolverBatch {
     bool fused = false;
     std::map<std::array<int64_t, 6>, hipGraphExec_t> graph_cache;
     int kt = 0;                       // 8 when every scene has K == 8 (unrolled walks), else 0
+    int max_wt_blocks = 0;  // walk_table_kernel workgroups (largest entry)
     int max_nb_owner = 0, max_nb_apply = 0, max_agg = 0, max_e_blocks = 0, max_acc_blocks = 0, max_n = 0, max_groups128 = 0,
         max_multi = 0, max_nt = 0, max_cand = 0;
 };

@@ -264,6 +264,40 @@ int solver_build(const SolverDesc& d, SolverWork& W, std::string& err) {
     S.cm_src = upload(W, cm_src, rc);
     S.multi_con = upload(W, multi_con, rc);
     S.n_multi = (int)multi_con.size();
+    {
+        // walk tables: static parts here, per-frame parts in walk_table_kernel
+        std::vector<int> out_nb((size_t)nF * K), in_nb(in_edge.size()), con_nb(con_mem.size() * (size_t)K);
+        std::vector<int2> cm_rng(cm_con.size());
+        for (int f = 0; f < nF; f++)
+            for (int t = 0; t < K; t++) out_nb[(size_t)f * K + t] = free_id[d.nbr[(size_t)free_nodes[f] * K + t]];
+        for (size_t q = 0; q < in_edge.size(); q++) in_nb[q] = free_id[in_edge[q] / K];
+        for (size_t e = 0; e < cm_con.size(); e++) cm_rng[e] = make_int2(con_ptr[cm_con[e]], con_ptr[cm_con[e] + 1]);
+        for (size_t m = 0; m < con_mem.size(); m++)
+            for (int j = 0; j < K; j++)
+                con_nb[m * K + j] = free_id[d.src_idx[(size_t)con_mem[m] * K + j]];
+        std::vector<int2> out_ik((size_t)nF * K), in_is(in_edge.size());
+        std::vector<int> cm_i(cm_con.size());
+        for (int f = 0; f < nF; f++) {
+            const int i = free_nodes[f];
+            for (int t = 0; t < K; t++) out_ik[(size_t)f * K + t] = make_int2(i, (int)d.nbr[(size_t)i * K + t]);
+            for (int q = in_ptr[f]; q < in_ptr[f + 1]; q++) in_is[q] = make_int2(i, in_edge[q] / K);
+            for (int e = cm_ptr[f]; e < cm_ptr[f + 1]; e++) cm_i[e] = i;
+        }
+        S.n_in = (int)in_edge.size();
+        S.n_cm = (int)cm_con.size();
+        S.wt_out_ik = upload(W, out_ik, rc);
+        S.wt_in_is = upload(W, in_is, rc);
+        S.wt_cm_i = upload(W, cm_i, rc);
+        S.n_mem = (int)con_mem.size();
+        S.wt_out_nb = upload(W, out_nb, rc);
+        S.wt_in_nb = upload(W, in_nb, rc);
+        S.wt_cm_rng = upload(W, cm_rng, rc);
+        S.wt_con_nb = upload(W, con_nb, rc);
+        S.wt_out = alloc<float4>(W, std::max<size_t>(1, out_nb.size()), rc);
+        S.wt_in = alloc<float4>(W, std::max<size_t>(1, in_nb.size()), rc);
+        S.wt_cm = alloc<float4>(W, std::max<size_t>(1, cm_rng.size()), rc);
+        S.wt_con = alloc<float4>(W, std::max<size_t>(1, con_nb.size()), rc);
+    }
     S.n_pairs = (int)pair_j.size();
     S.pair_ptr = upload(W, pair_ptr, rc);
     S.pair_j = upload(W, pair_j, rc);
