@@ -484,7 +484,7 @@ __device__ __forceinline__ double group_reduce16d(double v) {
 // placed at (3p + r, 3q + r) (index 3 -> 9 + r): 10 sums per lane to reduce.
 __global__ void __launch_bounds__(256) rhs_diag_kernel(const SolverEntry* __restrict__ B) {
     const SolverEntry E = load_entry(B);
-    if ((int)blockIdx.x >= E.nb_owner || *E.cg_done) return;
+    if ((int)blockIdx.x >= E.nb_owner || *E.gn_done) return;
     const SolverDev& S = E.S;
     const float* __restrict__ x = E.x;
     float* __restrict__ b = E.r;
@@ -1453,12 +1453,17 @@ __global__ void grid_barrier_reset_kernel(GridBarrier* gb) {
 // all-zero right-hand side ends the solve.
 constexpr double CG_TOL_FLOOR = 1e-6;
 
+// Also opens the Gauss-Newton iteration's CG: cg_done starts as gn_done (a
+// stopped loop runs no CG), with the entry's cap and tolerance.
 __global__ void cg_init_kernel(const SolverEntry* __restrict__ B) {
     const SolverEntry E = load_entry(B);
-    int* __restrict__ skip = E.cg_done;
-    if (*skip) return;
     if (threadIdx.x != 0) return;
+    int* __restrict__ skip = E.cg_done;
     CgScalars* __restrict__ cs = E.cg;
+    *skip = *E.gn_done;
+    cs->max_iters = E.cg_max_iters;
+    cs->tol2 = E.tol2;
+    if (*skip) return;
     GnScalars* __restrict__ gs = E.gs;
     const double bb = sum_partials(E.partial, E.nb_owner);
     if (gs->iters_done == 0) gs->bb0 = bb;
@@ -2087,13 +2092,6 @@ __global__ void set_eold_kernel(const SolverEntry* __restrict__ B) {
     E.energies[0] = E.cand_e[0];
 }
 
-__global__ void cg_begin_kernel(const SolverEntry* __restrict__ B) {
-    const SolverEntry E = load_entry(B);
-    *E.cg_done = *E.gn_done;
-    E.cg->max_iters = E.cg_max_iters;
-    E.cg->tol2 = E.tol2;
-}
-
 // Entries read this frame's node positions and aims.
 // Node positions of the frame copied for a side-stream coarse rebuild (the
 // caller's buffer changes under it: the node blend writes it in place).
@@ -2505,7 +2503,6 @@ static int launch_apply(const SolverBatch& B, unsigned nz, hipStream_t st, int k
 int solver_gn_begin(SolverBatch& B, int iter, hipStream_t st) {
     const SolverEntry* E = B.entries;
     const unsigned nz = (unsigned)B.w.size();
-    hipLaunchKernelGGL(cg_begin_kernel, dim3(1, 1, nz), dim3(1), 0, st, E);
     hipLaunchKernelGGL(rhs_diag_kernel, dim3(B.max_nb_owner, 1, nz), dim3(256), 0, st, E);
     hipLaunchKernelGGL(cg_init_kernel, dim3(1, 1, nz), dim3(64), 0, st, E);
     if (B.w[0]->warm && iter < B.w[0]->warm_slots) {
@@ -2609,6 +2606,8 @@ int solver_apply_test(SolverBatch& B, const float* x, const float* v, float* q_o
     const unsigned nb_owner = nblk(W.dev.nF, 256 / LANES);
     hipLaunchKernelGGL(copy_kernel, dim3(nblk(W.n, 256)), dim3(256), 0, st, W.x, x, (int64_t)W.n, (const int*)nullptr);
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.cg_done), (int64_t)1,
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<float*>(W.gn_done), (int64_t)1,
                        (const int*)nullptr);
     hipLaunchKernelGGL(walk_table_kernel,
                        dim3(nblk((int64_t)W.dev.nF * W.dev.K + W.dev.n_in + W.dev.n_cm + W.dev.n_mem, 256)), dim3(256),
