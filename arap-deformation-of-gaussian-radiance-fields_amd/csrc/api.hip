@@ -585,7 +585,7 @@ namespace {
 // pred (when not null): per GN iteration, the CG iteration count of the
 // previous solve; the first host check of that GN iteration's CG comes after
 // that many iterations (then every `every`), and the count found is stored back.
-// gn_pred (when not null, with host checks and the three-kernel CG): the GN
+// gn_pred (when not null, with host checks and a CG of separate launches): the GN
 // count of the previous solve; the stop flag of GN iterations before that
 // count is not waited for: the next iteration is queued at once (its kernels
 // do nothing once the loop has stopped) and the flag is read with that
@@ -623,7 +623,7 @@ int run_gn(arapgs_ctx* c, SolverBatch& B, int every, int cg_mode, bool refresh, 
         *out = __atomic_load_n(c->flag_h, __ATOMIC_ACQUIRE);
         return 0;
     };
-    const bool gn_spec = gn_pred && every > 0 && cg_mode == 0 && *gn_pred > 0;
+    const bool gn_spec = gn_pred && every > 0 && cg_mode != 1 && *gn_pred > 0;
     bool gn_unread = false;  // an earlier GN iteration's stop flag has not been read
     // graphs: the GN phases and the CG chunks replay as captured graphs (a
     // context without its own stream cannot capture)

@@ -1321,7 +1321,7 @@ __global__ void __launch_bounds__(BS) cg_update_coarse_kernel(const SolverEntry*
 // it at the end of a chunk (apply k repeats it with the same result).
 // iters (when not null): the largest CG iteration count over the scenes.
 __global__ void fused_flag_kernel(const SolverEntry* __restrict__ B, int n, int k, int* __restrict__ flag,
-                                  int* __restrict__ iters) {
+                                  int* __restrict__ iters, int* __restrict__ gn) {
     __shared__ double shd[4];
     __shared__ int s_all, s_iters;
     if (threadIdx.x == 0) {
@@ -1346,6 +1346,15 @@ __global__ void fused_flag_kernel(const SolverEntry* __restrict__ B, int n, int 
     __syncthreads();
     if (threadIdx.x == 0) {
         if (iters) *iters = s_iters;
+        if (gn) {
+            int gn_all = 1, gn_most = 0;
+            for (int s = 0; s < n; s++) {
+                gn_all &= *B[s].gn_done != 0 ? 1 : 0;
+                gn_most = max(gn_most, B[s].gs->iters_done);
+            }
+            gn[0] = gn_all;
+            gn[1] = gn_most;
+        }
         __hip_atomic_store(flag, s_all, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
@@ -2344,11 +2353,9 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
 
 int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done, int* iters_h, int* gn_h) {
     if (which != 0) iters_h = nullptr;
-    if (which == 0 && B.fused && !gn_h)
+    if (which == 0 && B.fused)
         hipLaunchKernelGGL(fused_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
-                           k_done, flag_h, iters_h);
-    else if (which == 0 && B.fused)
-        return -1;  // the two-kernel form's flag reports no Gauss-Newton state
+                           k_done, flag_h, iters_h, gn_h);
     else
         hipLaunchKernelGGL(batch_flag_kernel, dim3(1), dim3(64), 0, st, (const SolverEntry*)B.entries, (int)B.w.size(),
                            which, flag_h, iters_h, gn_h);

@@ -219,8 +219,8 @@ int solver_batch_set_frame(SolverBatch& B, const float* const* g, const float* c
 // iterations. iters_h (which 0, when not null): the largest CG iteration count
 // over the scenes so far.
 // flag_h = 1 when every scene's CG (which 0) or Gauss-Newton loop (which 1) is
-// done; iters_h: the largest CG count (which 0); gn_h (not with the two-kernel
-// form): gn_h[0] = every Gauss-Newton loop done, gn_h[1] = largest GN count.
+// done; iters_h: the largest CG count (which 0); gn_h (when not null):
+// gn_h[0] = every Gauss-Newton loop done, gn_h[1] = the largest GN count.
 int solver_batch_flag(SolverBatch& B, int which, hipStream_t st, int* flag_h, int k_done = 0, int* iters_h = nullptr,
                       int* gn_h = nullptr);
 // Queues CG iterations [k0, k0 + L) followed by the cg_done flag kernel

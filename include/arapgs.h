@@ -256,8 +256,8 @@ int arapgs_solver_set_cg_tol(arapgs_solver* s, float tol);
  * counterpart in the reference; host scheduling only: the device-side
  * convergence test decides, so the results are the same bit for bit).
  * Consecutive drag frames need similar counts, so fewer launches run past
- * convergence. With the three-kernel CG (cg_mode 0) the Gauss-Newton stop
- * flag is predicted too: before the previous solve's GN count, the next GN
+ * convergence. With the CG as separate launches (cg_mode 0 or 2) the
+ * Gauss-Newton stop flag is predicted too: before the previous solve's GN count, the next GN
  * iteration is queued without waiting for the flag (its kernels do nothing
  * once the loop has stopped) and the flag is read at that iteration's first
  * CG check. arapgs_solver_reset forgets the counts. */

@@ -547,3 +547,31 @@ def test_cg_tol_setter_matches_creation_tolerance(ctx, prob):
             b.set_cg_tol(bad)
     a.close()
     b.close()
+
+
+def test_host_scheduling_keeps_fixed_count_iterates(ctx, prob):
+    """Fixed-count solves (stop rule off, cold CG, a reset solver before each,
+    as the fixed-iteration bench leg runs them): host checks after every CG
+    iteration or every 3, with or without count prediction and polled flags,
+    give the same iterates bit for bit."""
+    settings = ((1, False, False), (3, False, False), (3, True, False), (3, False, True), (3, True, True))
+    runs = []
+    for every, predict, spin in settings:
+        s = _solver(ctx, prob)
+        s.set_warm_start(False)
+        s.set_host_checks(every)
+        s.set_cg_predict(predict)
+        s.set_host_spin(spin)
+        out = []
+        for fraction in (0.5, 0.6):
+            s.reset()
+            aims, _ = _problem(prob, fraction)
+            x, e, done = s.solve(T(prob["nodes"]), T(aims), 10, stop_rule=False)
+            out.append((H(x), H(e), int(H(done)[0]), s.last_stats()))
+        runs.append(out)
+        s.close()
+    for cfg, other in zip(settings[1:], runs[1:]):
+        for a, b in zip(runs[0], other):
+            assert a[2] == b[2] and a[3] == b[3], (cfg, a[3], b[3])
+            assert np.array_equal(a[0], b[0]), cfg
+            assert np.array_equal(a[1][: a[2] + 1], b[1][: b[2] + 1]), cfg
