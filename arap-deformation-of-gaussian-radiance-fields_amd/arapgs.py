@@ -130,6 +130,7 @@ EXPORTS = {
     "arapgs_solver_set_cg_assembled": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_cg_tol": ([P, C.c_float], C.c_int),
     "arapgs_solver_set_host_spin": ([P, C.c_int], C.c_int),
+    "arapgs_set_host_spin": ([P, C.c_int], C.c_int),
     "arapgs_solver_set_side_stream": ([P, P], C.c_int),
     "arapgs_solver_set_warm_start": ([P, C.c_int], C.c_int),
     "arapgs_solver_reset": ([P], C.c_int),
@@ -493,6 +494,10 @@ class Context:
     def set_node_cache(self, on=True):
         """Six-point end-point blend through the LDS node-row cache (bit-identical)."""
         self._check(lib().arapgs_set_node_cache(self.h, int(bool(on))))
+
+    def set_host_spin(self, on=True):
+        """The render's instance-count read polls its pinned slot (images unchanged)."""
+        self._check(lib().arapgs_set_host_spin(self.h, int(bool(on))))
 
     def set_tile_cull(self, on=True):
         """Renders emit only the tiles that pass the conservative tile test (identical images)."""

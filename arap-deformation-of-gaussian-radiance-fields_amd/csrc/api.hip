@@ -68,15 +68,17 @@ struct arapgs_ctx {
     int* r_seg = nullptr;             // device: Gaussian segments of a segment-mode render (RASTER_MAX_VIEWS + 1)
     int* r_seg_h = nullptr;           // pinned host staging for it
     int* flag_h = nullptr;            // pinned, coherent: convergence flags and render counts written by kernels
-                                      // (16 ints; [3]: the CG iteration count at a host check; [4], [5]: the
-                                      // flags of consecutive CG chunk graphs; [6]: the unread flag of the
-                                      // two-kernel CG's final test; [7]: gn_done after a GN-end graph; [8],
-                                      // [9]: every GN loop stopped and the GN count, at a host check)
+                                      // (16 ints; [0]: a host check's flag; [3]: the CG iteration count at a
+                                      // host check; [4], [5]: the flags of consecutive CG chunk graphs; [6]:
+                                      // the unread flag of the two-kernel CG's final test; [7]: gn_done after
+                                      // a GN-end graph; [8], [9]: every GN loop stopped and the GN count, at a
+                                      // host check; [10..12]: a render's instance counts, [13]: their ready word)
     hipEvent_t chk_ev[2] = {nullptr, nullptr};  // recorded after each CG chunk graph
     RasterParams last_rp{};           // parameters of the last render (its binning stays in r_geo / r_bin)
     int blend_unroll = 4;             // Gaussians per compositing step of the 16x16 blend
     int blend_split = 0;              // instances per blend segment of a deep 16x16 tile (0: no split)
     bool node_cache = true;
+    bool host_spin = false;           // the render's instance-count read polls its pinned slot
     bool sh_compose = false;          // samples_frames: one SH rotation by the product of the frames' rotations
     bool tile_cull = false;           // renders emit only the tiles that pass the conservative tile test           // six-point end-point blend through the LDS node-row cache
     // optional HIP-event timing of the fused six-point kernel inside deform_frame
@@ -959,6 +961,13 @@ int arapgs_samples_frames(arapgs_ctx* c, int64_t s, int k, int nf, const arapgs_
                         "samples_frames");
 }
 
+int arapgs_set_host_spin(arapgs_ctx* c, int on) {
+    CHECK_CTX(c);
+    if (on < 0 || on > 1) return fail(c, -1, "set_host_spin: on must be 0 or 1");
+    c->host_spin = on != 0;
+    return 0;
+}
+
 int arapgs_set_tile_cull(arapgs_ctx* c, int on) {
     CHECK_CTX(c);
     if (on < 0 || on > 1) return fail(c, -1, "set_tile_cull: on must be 0 or 1");
@@ -1002,8 +1011,12 @@ int arapgs_tile_test_host(int64_t n, const float* xy, const float* conic, const 
 
 namespace {
 // The preprocess's counts (instances, depth-bit range) into pinned host memory.
+// dst[0..2] = the counts, then dst[3] = 1 stored with release semantics at
+// system scope (a host polling dst[3] then sees the counts).
 __global__ void counts_to_host_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
     if (threadIdx.x < 3) dst[threadIdx.x] = src[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(dst + 3, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 }  // namespace
 
@@ -1102,10 +1115,19 @@ static int render_views(arapgs_ctx* c, const arapgs_render_desc* dv, int V, int*
                       "render preprocess");
     if (rc) return rc;
     uint32_t cnt[3] = {0, 0, 0};
-    uint32_t* cnt_h = reinterpret_cast<uint32_t*>(c->flag_h);
+    uint32_t* cnt_h = reinterpret_cast<uint32_t*>(c->flag_h + 10);
+    __atomic_store_n(cnt_h + 3, 0u, __ATOMIC_SEQ_CST);
     hipLaunchKernelGGL(counts_to_host_kernel, dim3(1), dim3(64), 0, c->stream, (const uint32_t*)c->r_total, cnt_h);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    bool seen = false;
+    if (e == hipSuccess && c->host_spin) {
+        // poll the ready word (bounded; then the stream wait, which reports a failed launch)
+        const auto t0 = std::chrono::steady_clock::now();
+        while (!(seen = __atomic_load_n(cnt_h + 3, __ATOMIC_ACQUIRE) != 0u) &&
+               std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(50)) {
+        }
+    }
+    if (e == hipSuccess && !seen) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "render");
     for (int i = 0; i < 3; i++) cnt[i] = ((volatile uint32_t*)cnt_h)[i];
     const uint32_t n = cnt[0];

@@ -232,6 +232,8 @@ def timed_pipelined(B, steps, warmup, solve_cus=None, sample_batch=None):
         pipe.ctx_render.set_blend_unroll(B["cfg"]["blend_unroll"])
         pipe.ctx_render.set_blend_split(B["cfg"].get("blend_split", 0))
         pipe.ctx_render.set_tile_cull(B["cfg"].get("tile_cull", 0))
+        for c_ in {id(pipe.ctx_b): pipe.ctx_b, id(pipe.ctx_render): pipe.ctx_render}.values():
+            c_.set_host_spin(B["cfg"].get("host_spin", 0))
         pipe.run(warmup)
         torch.cuda.synchronize()
         stamps = []
@@ -287,6 +289,7 @@ def timed(B, plain, steps, warmup):
     B["solver"].set_cg_mode(0 if plain else B["cfg"].get("cg_mode", 0))
     B["solver"].set_cg_predict(0 if plain else B["cfg"].get("cg_predict", 0))
     B["solver"].set_host_spin(0 if plain else B["cfg"].get("host_spin", 0))
+    B["ctx"].set_host_spin(0 if plain else B["cfg"].get("host_spin", 0))
     B["solver"].set_host_checks(1 if plain else B["cfg"]["host_checks"])
     B["solver"].set_warm_start(0 if plain else B["cfg"]["warm"])
     B["solver"].set_cg_graphs(0 if plain else B["cfg"].get("cg_graphs", 0))
@@ -336,6 +339,7 @@ def apply_config(B, cfg):
     ctx.set_blend_split(cfg.get("blend_split", 0))
     ctx.set_node_cache(cfg.get("node_cache", 1))
     ctx.set_tile_cull(cfg.get("tile_cull", 0))
+    ctx.set_host_spin(cfg.get("host_spin", 0))
     B["cfg"] = dict(cfg)
 
 

@@ -594,6 +594,12 @@ int arapgs_set_blend_unroll(arapgs_ctx* ctx, int unroll);
  * chain always blend tile by tile. */
 int arapgs_set_blend_split(arapgs_ctx* ctx, int entries);
 
+/* on = 1: a render's read of its instance count (the one host read of a
+ * render) polls the pinned slot, which the count kernel publishes with a
+ * system-scope release, instead of waiting for the stream (bounded at 50 ms,
+ * then the stream wait). Images unchanged. No counterpart in the reference. */
+int arapgs_set_host_spin(arapgs_ctx* ctx, int on);
+
 /* on = 1: the context's renders emit, for a Gaussian whose tile rectangle spans
  * at most 64 tiles, only the tiles whose box of pixel centres passes the
  * conservative alpha test the blend applies to every 8x8 quadrant (the

@@ -250,3 +250,31 @@ def test_tile_cull_keeps_images(ctx, scene, W, H_, tile, fast, dist):
     for t in tiles[:: max(1, len(tiles) // 200)]:
         it = iter(full[4][full[3] == t].tolist())
         assert all(x in it for x in cul[4][cul[3] == t].tolist()), t
+
+
+def test_polled_count_read_keeps_images(ctx, scene):
+    """A render whose instance-count read polls the pinned slot (set_host_spin)
+    draws the same image and instance count as one that waits for the stream,
+    over consecutive renders on one context (the ready word is re-armed per
+    render)."""
+    import torch
+    from arapgs_amd import arapgs
+    g = scene["g"]
+    bg = np.array([0.8, 0.8, 0.8], np.float32)
+    runs = []
+    for spin in (False, True):
+        c = arapgs.Context(0)
+        c.set_host_spin(spin)
+        out = []
+        for dist in (2.5, 3.5, 2.5):
+            cam = S.look_at_camera((0.3, -dist, 0.8), (0.0, 0.0, 0.0), 50.0, 200, 160)
+            r = c.render(T(g.pos), T(g.sh), T(g.opacity), T(g.scale), T(g.rot), T(cam.view), T(cam.proj),
+                         T(cam.campos), cam.tanfovx, cam.tanfovy, 200, 160, T(bg))
+            torch.cuda.synchronize()
+            out.append((H(r["image"]), r["num_rendered"]))
+        runs.append(out)
+        c.close()
+    for a, b in zip(*runs):
+        assert a[1] == b[1] > 0
+        assert np.array_equal(a[0], b[0])
+    assert runs[0][0][1] != runs[0][1][1]
