@@ -55,7 +55,7 @@ import json
 import os
 
 # the defaults are the best configuration measured on the capsules-1M drag
-DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=8, host_checks=3, warm=1, cg_block=256, cg_graphs=0,
+DEFAULTS = dict(tile=16, blend_unroll=4, aggregates=128, refresh=4, host_checks=3, warm=1, cg_block=256, cg_graphs=0,
                 cg_mode=2, cg_predict=1, node_cache=1, blend_split=0, deform_skip=0,
                 solve_cus=32, samples_cus=3, sh_compose=1, tile_cull=0,
                 coarse_lag=1, side_cus=1, samples_late=1, cg_asm=0,
